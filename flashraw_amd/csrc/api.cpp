@@ -1,0 +1,560 @@
+// C ABI of the develop path (include/flashraw_hip.h): context, validation of the reference's
+// develop descriptors, upload, band planning, launch and device-resident images.
+// Reference: crates/flashraw-gpu/src/context.rs (context, pool, limits), run.rs (run_banded:
+// validation 439-646, bands 650-662, timings 227-239) and image.rs (GpuImage).
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "flashraw_hip.h"
+#include "frh_device.h"
+#include "frh_launch.h"
+#include "frh_internal.h"
+
+using namespace frh;
+
+// ------------------------------------------------------------------------------------------
+// Errors: a status per call plus a thread-local message (crates/flashraw-capi/src/lib.rs:1-16)
+
+static thread_local std::string g_last_error;
+
+int32_t frh::fail(int32_t code, const std::string &msg) {
+    g_last_error = msg;
+    return code;
+}
+
+int32_t frh::hip_fail(hipError_t e, const char *what) {
+    int32_t code = e == hipErrorOutOfMemory ? FRH_ERROR_OUT_OF_MEMORY : FRH_ERROR_GPU_UNAVAILABLE;
+    return fail(code, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+extern "C" const char *frh_last_error_message(void) { return g_last_error.c_str(); }
+
+// ------------------------------------------------------------------------------------------
+// Tables (crates/flashraw-gpu/src/kernels.rs:348-371; process.rs:1903-1934)
+
+static uint8_t sat_u8(float x) {
+    if (!(x >= 1.0f)) return 0;
+    if (x >= 255.0f) return 255;
+    return (uint8_t)x;
+}
+
+// The sRGB 8-bit table: x = i / 4096 in f32, the curve with f32 powf, (s * 255 + 0.5) clamped
+// and truncated.
+void frh::build_srgb8(uint8_t out[4097]) {
+    const float e = 1.0f / 2.4f;
+    for (int i = 0; i <= 4096; i++) {
+        float x = (float)i / 4096.0f;
+        volatile float p = powf(x, e);  // the libm call, as the reference's f32::powf
+        float s = (x <= 0.0031308f) ? 12.92f * x : 1.055f * p - 0.055f;
+        float v = s * 255.0f + 0.5f;
+        v = v < 0.0f ? 0.0f : (v > 255.0f ? 255.0f : v);
+        out[i] = sat_u8(v);
+    }
+}
+
+// The sRGB 16-bit curve at 16385 points (f64, stored as f32) as (value, step) pairs.
+static void build_srgb16_pairs(float *pairs) {
+    std::vector<float> lut(16385);
+    for (int i = 0; i <= 16384; i++) {
+        double x = (double)i / 16384.0;
+        double s = (x <= 0.0031308) ? 12.92 * x : 1.055 * pow(x, 1.0 / 2.4) - 0.055;
+        lut[i] = (float)(s * 65535.0);
+    }
+    for (int i = 0; i < 16384; i++) {
+        pairs[2 * i] = lut[i];
+        pairs[2 * i + 1] = lut[i + 1] - lut[i];
+    }
+}
+
+extern "C" int32_t frh_srgb8_table(uint8_t *out, size_t len) {
+    if (!out || len < 4097) return fail(FRH_ERROR_INVALID_ARGUMENT, "sRGB table needs 4097 bytes");
+    build_srgb8(out);
+    return FRH_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// Context (context.rs:116-150, 775-842)
+
+extern "C" int32_t frh_available(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n > 0 ? 1 : 0;
+}
+
+extern "C" int32_t frh_context_create(int32_t device, frh_context **out) {
+    if (!out) return fail(FRH_ERROR_INVALID_ARGUMENT, "out is NULL");
+    *out = nullptr;
+    try {
+        int n = 0;
+        hipError_t e = hipGetDeviceCount(&n);
+        if (e != hipSuccess || n == 0) return fail(FRH_ERROR_GPU_UNAVAILABLE, "no HIP device");
+        if (device < 0 || device >= n) return fail(FRH_ERROR_INVALID_ARGUMENT, "device index out of range");
+        std::unique_ptr<frh_context> c(new frh_context());
+        c->device = device;
+        if ((e = hipSetDevice(device)) != hipSuccess) return hip_fail(e, "hipSetDevice");
+        hipDeviceProp_t prop;
+        if ((e = hipGetDeviceProperties(&prop, device)) != hipSuccess) return hip_fail(e, "hipGetDeviceProperties");
+        c->num_cus = prop.multiProcessorCount;
+        c->name = std::string(prop.name) + " (" + prop.gcnArchName + ")";
+        c->total_mem = prop.totalGlobalMem;
+        if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess)
+            return hip_fail(e, "hipStreamCreate");
+        uint8_t lut[4100] = {0};
+        build_srgb8(lut);
+        std::vector<float> pairs(2 * 16384);
+        build_srgb16_pairs(pairs.data());
+        if ((e = hipMalloc(&c->d_lut8, 4100)) != hipSuccess) return hip_fail(e, "hipMalloc");
+        if ((e = hipMalloc(&c->d_srgb16, pairs.size() * sizeof(float))) != hipSuccess) return hip_fail(e, "hipMalloc");
+        if ((e = hipMemcpy(c->d_lut8, lut, 4100, hipMemcpyHostToDevice)) != hipSuccess) return hip_fail(e, "upload");
+        if ((e = hipMemcpy(c->d_srgb16, pairs.data(), pairs.size() * sizeof(float), hipMemcpyHostToDevice)) !=
+            hipSuccess)
+            return hip_fail(e, "upload");
+        *out = c.release();
+        return FRH_OK;
+    } catch (const std::bad_alloc &) {
+        return fail(FRH_ERROR_OUT_OF_MEMORY, "host allocation failed");
+    } catch (...) {
+        return fail(FRH_ERROR_INTERNAL, "unexpected exception");
+    }
+}
+
+frh_context::~frh_context() {
+    hipSetDevice(device);
+    for (auto &b : pool) hipFree(b.ptr);
+    if (d_lut8) hipFree(d_lut8);
+    if (d_srgb16) hipFree(d_srgb16);
+    if (stream) hipStreamDestroy(stream);
+}
+
+extern "C" void frh_context_destroy(frh_context *ctx) { delete ctx; }
+
+extern "C" int32_t frh_context_info(const frh_context *ctx, char *buf, size_t len) {
+    if (!ctx || !buf || len == 0) return fail(FRH_ERROR_INVALID_ARGUMENT, "NULL argument");
+    snprintf(buf, len, "%s, %d CUs, %.1f GB", ctx->name.c_str(), ctx->num_cus, ctx->total_mem / 1e9);
+    return FRH_OK;
+}
+
+extern "C" int32_t frh_set_limits(frh_context *ctx, uint64_t max_binding_bytes) {
+    if (!ctx) return fail(FRH_ERROR_INVALID_ARGUMENT, "NULL context");
+    std::lock_guard<std::mutex> g(ctx->mu);
+    ctx->max_binding = max_binding_bytes;
+    return FRH_OK;
+}
+
+// A caching device allocator bounded like the reference's pool (POOL_MAX_BYTES, context.rs:153).
+void *frh_context::alloc(uint64_t bytes, hipError_t *err) {
+    {
+        std::lock_guard<std::mutex> g(mu);
+        size_t best = pool.size();
+        for (size_t i = 0; i < pool.size(); i++)
+            if (pool[i].size >= bytes && pool[i].size <= bytes * 2 &&
+                (best == pool.size() || pool[i].size < pool[best].size))
+                best = i;
+        if (best != pool.size()) {
+            void *p = pool[best].ptr;
+            pooled -= pool[best].size;
+            pool.erase(pool.begin() + (long)best);
+            *err = hipSuccess;
+            return p;
+        }
+    }
+    void *p = nullptr;
+    hipSetDevice(device);
+    *err = hipMalloc(&p, bytes ? bytes : 1);
+    return *err == hipSuccess ? p : nullptr;
+}
+
+void frh_context::release(void *p, uint64_t bytes) {
+    if (!p) return;
+    std::lock_guard<std::mutex> g(mu);
+    if (pooled + bytes <= POOL_MAX_BYTES) {
+        pool.push_back({p, bytes});
+        pooled += bytes;
+        return;
+    }
+    hipSetDevice(device);
+    hipFree(p);
+}
+
+// ------------------------------------------------------------------------------------------
+// Images (image.rs:52-306)
+
+extern "C" uint32_t frh_image_width(const frh_image *i) { return i ? i->w : 0; }
+extern "C" uint32_t frh_image_height(const frh_image *i) { return i ? i->h : 0; }
+extern "C" uint32_t frh_image_channels(const frh_image *i) { return i ? i->ch : 0; }
+extern "C" int32_t frh_image_format(const frh_image *i) { return i ? i->fmt : -1; }
+extern "C" const void *frh_image_device_ptr(const frh_image *i) { return i ? i->dptr : nullptr; }
+extern "C" uint64_t frh_image_bytes(const frh_image *i) { return i ? i->bytes : 0; }
+
+extern "C" int32_t frh_image_download(const frh_image *img, void *host, uint64_t bytes) {
+    if (!img || !host) return fail(FRH_ERROR_INVALID_ARGUMENT, "NULL argument");
+    if (bytes < img->bytes) return fail(FRH_ERROR_INVALID_ARGUMENT, "host buffer too small");
+    hipSetDevice(img->ctx->device);
+    hipError_t e = hipMemcpy(host, img->dptr, img->bytes, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(e, "download");
+    return FRH_OK;
+}
+
+extern "C" void frh_image_free(frh_image *img) {
+    if (!img) return;
+    img->ctx->release(img->dptr, img->bytes);
+    delete img;
+}
+
+// ------------------------------------------------------------------------------------------
+// Develop (run.rs:432-1072)
+
+static uint64_t gcd64(uint64_t a, uint64_t b) { return b == 0 ? a : gcd64(b, a % b); }
+
+// Bytes per sample and samples per pixel of an output encoding.
+static void out_format(int32_t mode, uint32_t *ch, uint32_t *bps, int32_t *fmt) {
+    switch (mode) {
+    case OUT_RGBA8: *ch = 4; *bps = 1; *fmt = FRH_FORMAT_RGBA8; break;
+    case OUT_LINEAR_F32: *ch = 3; *bps = 4; *fmt = FRH_FORMAT_F32; break;
+    default: *ch = 3; *bps = 2; *fmt = FRH_FORMAT_U16; break;
+    }
+}
+
+static bool valid_mode(int32_t m) { return m >= OUT_RGBA8 && m <= OUT_LINEAR_F16; }
+static bool display_mode(int32_t m) { return m == OUT_RGBA8 || m == OUT_SRGB16; }
+
+// Everything a develop launch needs, validated (run.rs:439-646).
+struct Plan {
+    DevArgs a;
+    bool ahd;
+    uint32_t out_w[2], out_h[2], ch[2];
+    uint64_t bytes[2];
+    int32_t fmt[2];
+    uint64_t in_row_bytes;
+};
+
+static int32_t plan_develop(const frh_context *ctx, const frh_frame *f, const frh_frame_info *info,
+                            const frh_develop_spec *spec, Plan &p) {
+    if (!f || !info || !spec) return fail(FRH_ERROR_INVALID_ARGUMENT, "NULL descriptor");
+    if (f->size < sizeof(frh_frame) || info->size < sizeof(frh_frame_info) || spec->size < sizeof(frh_develop_spec))
+        return fail(FRH_ERROR_INVALID_ARGUMENT, "descriptor size field too small");
+    memset(&p, 0, sizeof(p));
+    DevArgs &a = p.a;
+    const uint32_t w = f->window.width, h = f->window.height, cpp = f->cpp ? f->cpp : 1;
+    if (w == 0 || h == 0 || (uint64_t)f->window.x + w > f->raster_width ||
+        (uint64_t)f->window.y + h > f->raster_height)
+        return fail(FRH_ERROR_INVALID_ARGUMENT, "window outside the raster");
+    if (f->input != FRH_INPUT_U16)
+        return fail(FRH_ERROR_NOT_AVAILABLE, "only u16 input is supported by this develop path");
+    if (!f->data) return fail(FRH_ERROR_INVALID_ARGUMENT, "frame data is NULL");
+    if (f->data_len < (uint64_t)f->raster_width * f->raster_height * cpp)
+        return fail(FRH_ERROR_INVALID_ARGUMENT, "u16 raster shorter than its dimensions");
+    if (f->linearization) return fail(FRH_ERROR_NOT_AVAILABLE, "linearisation tables are not supported yet");
+    if (cpp != 1) return fail(FRH_ERROR_NOT_AVAILABLE, "only single-component mosaics are supported");
+    if (info->cfa_width != 2 || info->cfa_height != 2)
+        return fail(FRH_ERROR_NOT_AVAILABLE, "only Bayer mosaics are supported");
+    {
+        int cnt[256] = {0};
+        for (int i = 0; i < 4; i++) cnt[info->cfa[i]]++;
+        if (!(cnt[0] == 1 && cnt[1] == 2 && cnt[2] == 1))
+            return fail(FRH_ERROR_NOT_AVAILABLE, "only Bayer mosaics are supported");
+    }
+    if (spec->half_size) return fail(FRH_ERROR_NOT_AVAILABLE, "half-size develop is not supported yet");
+    if (spec->curve || spec->tone) return fail(FRH_ERROR_NOT_AVAILABLE, "output curves and tone are not supported");
+    if (!valid_mode(spec->output)) return fail(FRH_ERROR_INVALID_ARGUMENT, "unknown output");
+    if (spec->output2 != OUT_NONE && !valid_mode(spec->output2))
+        return fail(FRH_ERROR_INVALID_ARGUMENT, "unknown second output");
+    for (int i = 0; i < 4; i++)
+        if (!std::isfinite(spec->wb[i])) return fail(FRH_ERROR_INVALID_ARGUMENT, "non-finite white balance");
+    for (int i = 0; i < 9; i++)
+        if (!std::isfinite(spec->cam_to_rgb[i])) return fail(FRH_ERROR_INVALID_ARGUMENT, "non-finite matrix");
+    if (spec->has_levels && !(std::isfinite(spec->levels[0]) && std::isfinite(spec->levels[1])))
+        return fail(FRH_ERROR_INVALID_ARGUMENT, "non-finite levels");
+
+    a.src = static_cast<const uint16_t *>(f->data) + (size_t)f->window.y * f->raster_width + f->window.x;
+    a.stride = f->raster_width;
+    a.width = w;
+    a.height = h;
+    a.ox = f->origin_x;
+    a.oy = f->origin_y;
+    // Calib::new (process.rs:256-286); the kernels take at most 64 pattern positions.
+    const uint64_t rows = info->black_rows ? info->black_rows : 1, cols = info->black_cols ? info->black_cols : 1;
+    const uint64_t samples = info->black_samples ? info->black_samples : 1;
+    uint64_t pw = 2 / gcd64(2, cols) * cols, ph = 2 / gcd64(2, rows) * rows;
+    pw = pw < 64 ? pw : 64;
+    ph = ph < 64 ? ph : 64;
+    if (pw * ph > 64) return fail(FRH_ERROR_NOT_AVAILABLE, "black-level pattern larger than 64 positions");
+    if ((info->black_values_len && !info->black_values) || (info->white_values_len && !info->white_values))
+        return fail(FRH_ERROR_INVALID_ARGUMENT, "NULL calibration array");
+    a.cal_pw = (uint32_t)pw;
+    a.cal_ph = (uint32_t)ph;
+    a.cal_pow2 = ((pw & (pw - 1)) == 0 && (ph & (ph - 1)) == 0) ? 1u : 0u;
+    for (uint64_t y = 0; y < ph; y++) {
+        for (uint64_t x = 0; x < pw; x++) {
+            uint64_t idx = ((y % rows) * cols + x % cols) * samples;
+            float bl = idx < info->black_values_len ? info->black_values[idx] : 0.0f;
+            uint32_t colour = info->cfa[(y % 2) * 2 + x % 2];
+            uint32_t nw = info->white_values_len;
+            uint32_t wi = nw > 1 ? (colour < nw - 1 ? colour : nw - 1) : 0;
+            uint32_t wv = wi < nw ? info->white_values[wi] : (nw ? info->white_values[0] : 0);
+            float white = (float)wv;
+            a.black[y * pw + x] = bl;
+            a.scale[y * pw + x] = white > bl ? 1.0f / (white - bl) : 1.0f;
+        }
+    }
+    if (info->black_row_deltas_len) {
+        if (!info->black_row_deltas) return fail(FRH_ERROR_INVALID_ARGUMENT, "NULL row deltas");
+        a.n_rowd = info->black_row_deltas_len;
+    }
+    if (info->black_col_deltas_len) {
+        if (!info->black_col_deltas) return fail(FRH_ERROR_INVALID_ARGUMENT, "NULL column deltas");
+        a.n_cold = info->black_col_deltas_len;
+    }
+    for (int i = 0; i < 4; i++) {
+        uint32_t colour = info->cfa[i];  // pattern position (i & 1, i >> 1) of sensor parity
+        a.chan[i] = colour < 3 ? colour : 3;
+    }
+    // crop.clamp_to(w, h) (crates/flashraw-core/src/types.rs:68-70)
+    uint64_t cx = spec->crop.x, cy = spec->crop.y;
+    uint64_t cw = spec->crop.width, chh = spec->crop.height;
+    if (cx >= w || cy >= h || cw == 0 || chh == 0) return fail(FRH_ERROR_INVALID_ARGUMENT, "develop region too small");
+    cw = cw < w - cx ? cw : w - cx;
+    chh = chh < h - cy ? chh : h - cy;
+    a.crop_x = (uint32_t)cx;
+    a.crop_y = (uint32_t)cy;
+    a.dev_w = (uint32_t)cw;
+    a.dev_h = (uint32_t)chh;
+    a.band_y0 = 0;
+    a.band_y1 = a.dev_h;
+    memcpy(a.wb, spec->wb, sizeof(a.wb));
+    memcpy(a.m, spec->cam_to_rgb, sizeof(a.m));
+    a.has_levels = spec->has_levels ? 1 : 0;
+    a.lscale = spec->levels[0];
+    a.loffset = spec->levels[1];
+    a.orientation = (spec->orientation >= 1 && spec->orientation <= 8) ? spec->orientation : 1;
+    const bool swap = a.orientation >= 5;
+    a.dst_w = swap ? a.dev_h : a.dev_w;
+    a.out_mode = spec->output;
+    a.out_mode2 = spec->output2;
+    a.lut8 = ctx->d_lut8;
+    a.srgb16 = reinterpret_cast<const float2 *>(ctx->d_srgb16);
+    p.ahd = false;
+    if (spec->edge) {
+        const frh_edge_spec *e = spec->edge;
+        if (e->size < sizeof(frh_edge_spec)) return fail(FRH_ERROR_INVALID_ARGUMENT, "edge spec size too small");
+        if (!e->ahd)
+            return fail(FRH_ERROR_NOT_AVAILABLE, "the edge-aware filter demosaic is not supported yet (AHD is)");
+        // run.rs:588-598: AHD needs period 2 and beta 0
+        if (e->period != 2 || e->beta != 0.0f) return fail(FRH_ERROR_INVALID_ARGUMENT, "inconsistent AHD spec");
+        if (e->halo < 5 || e->halo > 64) return fail(FRH_ERROR_INVALID_ARGUMENT, "AHD halo must be in [5, 64]");
+        if (e->canon) {
+            if (e->canon_len != 4) return fail(FRH_ERROR_INVALID_ARGUMENT, "inconsistent edge-aware filter tables");
+            // The pattern the spec describes must be the frame's: colour of frame (x, y) is
+            // canon[((y + cy) % 2) * 2 + (x + cx) % 2].
+            for (uint32_t y = 0; y < 2; y++)
+                for (uint32_t x = 0; x < 2; x++) {
+                    uint32_t want = a.chan[((y + a.oy) & 1) * 2 + ((x + a.ox) & 1)];
+                    if (e->canon[((y + e->cy) % 2) * 2 + (x + e->cx) % 2] != want)
+                        return fail(FRH_ERROR_INVALID_ARGUMENT, "AHD pattern does not match the frame's CFA");
+                }
+        }
+        a.ahd_halo = e->halo;
+        p.ahd = true;
+    }
+    int32_t modes[2] = {spec->output, spec->output2};
+    for (int k = 0; k < 2; k++) {
+        if (k == 1 && modes[1] == OUT_NONE) break;
+        uint32_t bps;
+        out_format(modes[k], &p.ch[k], &bps, &p.fmt[k]);
+        bool oriented = display_mode(modes[k]);
+        p.out_w[k] = oriented && swap ? a.dev_h : a.dev_w;
+        p.out_h[k] = oriented && swap ? a.dev_w : a.dev_h;
+        p.bytes[k] = (uint64_t)p.out_w[k] * p.out_h[k] * p.ch[k] * bps;
+    }
+    p.in_row_bytes = (uint64_t)f->raster_width * 2;
+    return FRH_OK;
+}
+
+// Developed rows per launch under the context's binding cap (BandGeometry::plan, run.rs:357-389):
+// every band's input rows (+ halo) and output rows must fit; at least one 16-row tile.
+static uint32_t band_rows(const frh_context *ctx, const Plan &p) {
+    if (!ctx->max_binding) return p.a.dev_h;
+    uint64_t halo = p.ahd ? p.a.ahd_halo : 1;
+    uint64_t out_row = 0;
+    for (int k = 0; k < 2; k++) out_row += p.bytes[k] / (p.out_h[k] ? p.out_h[k] : 1);
+    uint64_t rows_in = ctx->max_binding / (p.in_row_bytes ? p.in_row_bytes : 1);
+    uint64_t n_in = rows_in > 2 * halo ? rows_in - 2 * halo : 0;
+    uint64_t n_out = ctx->max_binding / (out_row ? out_row : 1);
+    uint64_t n = n_in < n_out ? n_in : n_out;
+    if (n >= 16) n -= n % 16;
+    if (n == 0) n = 1;
+    return n < p.a.dev_h ? (uint32_t)n : p.a.dev_h;
+}
+
+static int32_t launch_bands(frh_context *ctx, Plan &p, hipStream_t stream, uint32_t *nbands) {
+    if (p.a.orientation >= 5 && ctx->max_binding)
+        return fail(FRH_ERROR_LIMIT_EXCEEDED, "banded runs of rotated outputs are not supported");
+    uint32_t n = band_rows(ctx, p);
+    uint32_t bands = 0;
+    for (uint32_t y0 = 0; y0 < p.a.dev_h; y0 += n) {
+        DevArgs a = p.a;
+        a.band_y0 = y0;
+        a.band_y1 = y0 + n < p.a.dev_h ? y0 + n : p.a.dev_h;
+        hipError_t e = launch_develop(a, p.ahd, stream, ctx->num_cus);
+        if (e != hipSuccess) return hip_fail(e, "kernel launch");
+        bands++;
+    }
+    if (nbands) *nbands = bands;
+    return FRH_OK;
+}
+
+extern "C" int32_t frh_develop_output_size(const frh_frame *f, const frh_develop_spec *spec, int32_t output,
+                                           uint32_t *width, uint32_t *height, uint32_t *channels, uint64_t *bytes) {
+    if (!f || !spec) return fail(FRH_ERROR_INVALID_ARGUMENT, "NULL descriptor");
+    if (!valid_mode(output)) return fail(FRH_ERROR_INVALID_ARGUMENT, "unknown output");
+    uint64_t w = f->window.width, h = f->window.height;
+    uint64_t cx = spec->crop.x, cy = spec->crop.y, cw = spec->crop.width, ch = spec->crop.height;
+    if (cx >= w || cy >= h || cw == 0 || ch == 0) return fail(FRH_ERROR_INVALID_ARGUMENT, "develop region too small");
+    cw = cw < w - cx ? cw : w - cx;
+    ch = ch < h - cy ? ch : h - cy;
+    bool swap = display_mode(output) && spec->orientation >= 5 && spec->orientation <= 8;
+    uint32_t c, bps;
+    int32_t fmt;
+    out_format(output, &c, &bps, &fmt);
+    if (width) *width = (uint32_t)(swap ? ch : cw);
+    if (height) *height = (uint32_t)(swap ? cw : ch);
+    if (channels) *channels = c;
+    if (bytes) *bytes = cw * ch * c * bps;
+    return FRH_OK;
+}
+
+extern "C" int32_t frh_develop_into(frh_context *ctx, const frh_frame *frame, const frh_frame_info *info,
+                                    const frh_develop_spec *spec, void *dst, uint64_t dst_bytes, void *dst2,
+                                    uint64_t dst2_bytes, void *stream) {
+    try {
+        if (!ctx) return fail(FRH_ERROR_INVALID_ARGUMENT, "NULL context");
+        Plan p;
+        int32_t rc = plan_develop(ctx, frame, info, spec, p);
+        if (rc) return rc;
+        if (!frame->data_on_device) return fail(FRH_ERROR_INVALID_ARGUMENT, "frh_develop_into needs device input");
+        if (!dst || dst_bytes < p.bytes[0]) return fail(FRH_ERROR_INVALID_ARGUMENT, "output buffer too small");
+        if (spec->output2 != OUT_NONE && (!dst2 || dst2_bytes < p.bytes[1]))
+            return fail(FRH_ERROR_INVALID_ARGUMENT, "second output buffer too small");
+        if ((p.a.n_rowd || p.a.n_cold))
+            return fail(FRH_ERROR_NOT_AVAILABLE, "black-level deltas need frh_run_develop (host tables)");
+        p.a.dst = dst;
+        p.a.dst2 = dst2;
+        hipSetDevice(ctx->device);
+        return launch_bands(ctx, p, stream ? (hipStream_t)stream : ctx->stream, nullptr);
+    } catch (...) {
+        return fail(FRH_ERROR_INTERNAL, "unexpected exception");
+    }
+}
+
+static double ms_since(std::chrono::steady_clock::time_point t0) {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+extern "C" int32_t frh_run_develop(frh_context *ctx, const frh_frame *frame, const frh_frame_info *info,
+                                   const frh_develop_spec *spec, frh_image **out, frh_image **out2,
+                                   frh_timings *timings) {
+    try {
+        if (!ctx || !out) return fail(FRH_ERROR_INVALID_ARGUMENT, "NULL argument");
+        *out = nullptr;
+        if (out2) *out2 = nullptr;
+        Plan p;
+        int32_t rc = plan_develop(ctx, frame, info, spec, p);
+        if (rc) return rc;
+        if (spec->output2 != OUT_NONE && !out2) return fail(FRH_ERROR_INVALID_ARGUMENT, "second output needs out2");
+        hipSetDevice(ctx->device);
+        hipError_t e;
+        auto t0 = std::chrono::steady_clock::now();
+        // Upload: the window's rows of the raster (run.rs:915-923), unless resident.
+        void *d_in = nullptr;
+        uint64_t in_bytes = 0;
+        std::vector<void *> scratch;
+        auto cleanup = [&]() {
+            if (d_in) ctx->release(d_in, in_bytes);
+            for (void *s : scratch) hipFree(s);
+        };
+        if (!frame->data_on_device) {
+            const uint64_t row = (uint64_t)frame->raster_width;
+            in_bytes = row * frame->window.height * 2;
+            d_in = ctx->alloc(in_bytes, &e);
+            if (!d_in) return hip_fail(e, "input allocation");
+            const uint16_t *src = static_cast<const uint16_t *>(frame->data) + (size_t)frame->window.y * row;
+            e = hipMemcpyAsync(d_in, src, in_bytes, hipMemcpyHostToDevice, ctx->stream);
+            if (e != hipSuccess) {
+                cleanup();
+                return hip_fail(e, "upload");
+            }
+            p.a.src = static_cast<const uint16_t *>(d_in) + frame->window.x;
+        }
+        if (p.a.n_rowd) {
+            void *d = nullptr;
+            if ((e = hipMalloc(&d, p.a.n_rowd * 4)) != hipSuccess) { cleanup(); return hip_fail(e, "hipMalloc"); }
+            scratch.push_back(d);
+            hipMemcpyAsync(d, info->black_row_deltas, p.a.n_rowd * 4, hipMemcpyHostToDevice, ctx->stream);
+            p.a.rowd = static_cast<const float *>(d);
+        }
+        if (p.a.n_cold) {
+            void *d = nullptr;
+            if ((e = hipMalloc(&d, p.a.n_cold * 4)) != hipSuccess) { cleanup(); return hip_fail(e, "hipMalloc"); }
+            scratch.push_back(d);
+            hipMemcpyAsync(d, info->black_col_deltas, p.a.n_cold * 4, hipMemcpyHostToDevice, ctx->stream);
+            p.a.cold = static_cast<const float *>(d);
+        }
+        hipStreamSynchronize(ctx->stream);
+        double upload_ms = ms_since(t0);
+        auto t1 = std::chrono::steady_clock::now();
+        frh_image *imgs[2] = {nullptr, nullptr};
+        for (int k = 0; k < 2; k++) {
+            if (k == 1 && spec->output2 == OUT_NONE) break;
+            void *d = ctx->alloc(p.bytes[k], &e);
+            if (!d) {
+                if (imgs[0]) frh_image_free(imgs[0]);
+                cleanup();
+                return hip_fail(e, "output allocation");
+            }
+            imgs[k] = new frh_image{ctx, d, p.bytes[k], p.out_w[k], p.out_h[k], p.ch[k], p.fmt[k]};
+        }
+        p.a.dst = imgs[0]->dptr;
+        p.a.dst2 = imgs[1] ? imgs[1]->dptr : nullptr;
+        hipEvent_t ev0, ev1;
+        hipEventCreate(&ev0);
+        hipEventCreate(&ev1);
+        hipEventRecord(ev0, ctx->stream);
+        uint32_t bands = 0;
+        rc = launch_bands(ctx, p, ctx->stream, &bands);
+        hipEventRecord(ev1, ctx->stream);
+        e = hipStreamSynchronize(ctx->stream);
+        float kms = 0.0f;
+        hipEventElapsedTime(&kms, ev0, ev1);
+        hipEventDestroy(ev0);
+        hipEventDestroy(ev1);
+        cleanup();
+        if (rc == FRH_OK && e != hipSuccess) rc = hip_fail(e, "develop");
+        if (rc != FRH_OK) {
+            frh_image_free(imgs[0]);
+            frh_image_free(imgs[1]);
+            return rc;
+        }
+        if (timings) {
+            timings->upload_ms = upload_ms;
+            timings->compute_ms = ms_since(t1);
+            timings->kernel_ms = kms;
+            timings->has_kernel_ms = 1;
+            timings->bands = bands;
+        }
+        *out = imgs[0];
+        if (out2) *out2 = imgs[1];
+        return FRH_OK;
+    } catch (const std::bad_alloc &) {
+        return fail(FRH_ERROR_OUT_OF_MEMORY, "host allocation failed");
+    } catch (...) {
+        return fail(FRH_ERROR_INTERNAL, "unexpected exception");
+    }
+}

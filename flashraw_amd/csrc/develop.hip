@@ -1,0 +1,448 @@
+// Fused develop kernels for gfx950: normalise -> demosaic (bilinear or AHD) -> white balance ->
+// clip -> camera->sRGB matrix -> levels -> encode -> store, one pass per frame, no full-size
+// intermediate. CPU statement of the same arithmetic: crates/flashraw/src/process.rs
+// (row_cols 967-1129, finish_rgb 654-707, encoders 1593-1991) and crates/flashraw/src/ahd.rs.
+//
+// Both kernels are persistent: a block loads the sRGB table and the calibration pattern into LDS
+// once, then walks tiles `blockIdx.x, blockIdx.x + gridDim.x, ...` of the developed image.
+#include "frh_device.h"
+#include "frh_launch.h"
+
+namespace frh {
+
+// ------------------------------------------------------------------------------------------
+// Shared helpers
+
+struct Lds {
+    float black[64];
+    float scale[64];
+    uint32_t lut[1025];  // 4097 sRGB codes, bytes packed four per word
+};
+
+__device__ __forceinline__ void load_lds(const DevArgs &a, Lds &s) {
+    for (int i = threadIdx.x; i < 64; i += blockDim.x) {
+        s.black[i] = a.black[i];
+        s.scale[i] = a.scale[i];
+    }
+    const uint32_t *l = reinterpret_cast<const uint32_t *>(a.lut8);
+    for (int i = threadIdx.x; i < 1025; i += blockDim.x) s.lut[i] = l[i];
+}
+
+__device__ __forceinline__ uint32_t lut8(const Lds &s, uint32_t i) {
+    return reinterpret_cast<const uint8_t *>(s.lut)[i];
+}
+
+// Channel (0 R, 1 G, 2 B) of frame sample (x, y) of a Bayer mosaic.
+__device__ __forceinline__ uint32_t chan_at(const DevArgs &a, int x, int y) {
+    return a.chan[(((uint32_t)y + a.oy) & 1u) * 2u + (((uint32_t)x + a.ox) & 1u)];
+}
+
+// Calib::norm (process.rs:288-298) of frame sample (x, y), in bounds.
+__device__ __forceinline__ float norm_at(const DevArgs &a, const Lds &s, uint32_t x, uint32_t y) {
+    uint32_t X = x + a.ox, Y = y + a.oy;
+    uint32_t i = a.cal_pow2 ? (Y & (a.cal_ph - 1u)) * a.cal_pw + (X & (a.cal_pw - 1u))
+                            : (Y % a.cal_ph) * a.cal_pw + X % a.cal_pw;
+    float b = s.black[i];
+    if (a.n_rowd) b += Y < a.n_rowd ? a.rowd[Y] : 0.0f;
+    if (a.n_cold) b += X < a.n_cold ? a.cold[X] : 0.0f;
+    float v = (float)a.src[(size_t)y * a.stride + x];
+    return (v - b) * s.scale[i];
+}
+
+// finish_rgb with Highlights::Clip (process.rs:654-707). `balanced` pixels come from the AHD
+// mosaic, which is already white balanced (finish_run's unit balance, process.rs:1209-1213).
+__device__ __forceinline__ void finish(const DevArgs &a, float r, float g, float b, bool balanced, float &o0,
+                                       float &o1, float &o2) {
+    if (!balanced) {
+        r = r * a.wb[0];
+        g = g * a.wb[1];
+        b = b * a.wb[2];
+    }
+    r = rclamp01(r);
+    g = rclamp01(g);
+    b = rclamp01(b);
+    o0 = a.m[0] * r + a.m[1] * g + a.m[2] * b;
+    o1 = a.m[3] * r + a.m[4] * g + a.m[5] * b;
+    o2 = a.m[6] * r + a.m[7] * g + a.m[8] * b;
+    if (a.has_levels) {
+        o0 = o0 * a.lscale - a.loffset;
+        o1 = o1 * a.lscale - a.loffset;
+        o2 = o2 * a.lscale - a.loffset;
+    }
+}
+
+// f32_to_f16 (process.rs:1978-1991).
+__device__ __forceinline__ uint32_t f16_bits(float x) {
+    uint32_t b = __float_as_uint(x);
+    uint32_t sign = (b >> 16) & 0x8000u;
+    uint32_t a = b & 0x7fffffffu;
+    uint32_t normal = (a + 0xc8000fffu + ((a >> 13) & 1u)) >> 13;
+    uint32_t sub = __float_as_uint(__uint_as_float(a) + 0.5f) - 0x3f000000u;
+    uint32_t h = a < 0x38800000u ? sub : normal;
+    h = a >= 0x477ff000u ? (a > 0x7f800000u ? 0x7e00u : 0x7bffu) : h;
+    return sign | h;
+}
+
+// srgb16_paired (process.rs:1960-1967).
+__device__ __forceinline__ uint32_t srgb16(const DevArgs &a, float v) {
+    float x = rclamp01(v) * 16384.0f;
+    uint32_t i = trunc_u32(x);
+    i = i < 16383u ? i : 16383u;
+    float t = x - (float)i;
+    float2 p = a.srgb16[i];
+    return trunc_u32(p.x + p.y * t + 0.5f);
+}
+
+// linear16 (process.rs:1971-1973).
+__device__ __forceinline__ uint32_t lin16(float v) { return trunc_u32(rclamp01(v) * 65535.0f + 0.5f); }
+
+// Destination of developed pixel (dx, dy) under EXIF orientation o (the inverse of
+// encode_developed's source mapping, process.rs:2035-2044).
+__device__ __forceinline__ void orient(int o, uint32_t dx, uint32_t dy, uint32_t w, uint32_t h, uint32_t &px,
+                                       uint32_t &py) {
+    switch (o) {
+    case 2: px = w - 1 - dx; py = dy; break;
+    case 3: px = w - 1 - dx; py = h - 1 - dy; break;
+    case 4: px = dx; py = h - 1 - dy; break;
+    case 5: px = dy; py = dx; break;
+    case 6: px = h - 1 - dy; py = dx; break;
+    case 7: px = h - 1 - dy; py = w - 1 - dx; break;
+    case 8: px = dy; py = w - 1 - dx; break;
+    default: px = dx; py = dy; break;
+    }
+}
+
+// Encode and store linear output pixel (r, g, b) of developed pixel (dx, dy).
+__device__ __forceinline__ void store_px(const DevArgs &a, const Lds &s, int mode, void *dst, uint32_t dx,
+                                         uint32_t dy, float r, float g, float b) {
+    if (mode == OUT_RGBA8) {
+        uint32_t px, py;
+        orient(a.orientation, dx, dy, a.dev_w, a.dev_h, px, py);
+        uint32_t v = lut8(s, lut8_index(r)) | (lut8(s, lut8_index(g)) << 8) | (lut8(s, lut8_index(b)) << 16) |
+                     0xff000000u;
+        reinterpret_cast<uint32_t *>(dst)[(size_t)py * a.dst_w + px] = v;
+    } else if (mode == OUT_LINEAR_F32) {
+        float *p = reinterpret_cast<float *>(dst) + ((size_t)dy * a.dev_w + dx) * 3;
+        p[0] = r;
+        p[1] = g;
+        p[2] = b;
+    } else if (mode == OUT_SRGB16) {
+        uint32_t px, py;
+        orient(a.orientation, dx, dy, a.dev_w, a.dev_h, px, py);
+        uint16_t *p = reinterpret_cast<uint16_t *>(dst) + ((size_t)py * a.dst_w + px) * 3;
+        p[0] = (uint16_t)srgb16(a, r);
+        p[1] = (uint16_t)srgb16(a, g);
+        p[2] = (uint16_t)srgb16(a, b);
+    } else if (mode == OUT_LINEAR16) {
+        uint16_t *p = reinterpret_cast<uint16_t *>(dst) + ((size_t)dy * a.dev_w + dx) * 3;
+        p[0] = (uint16_t)lin16(r);
+        p[1] = (uint16_t)lin16(g);
+        p[2] = (uint16_t)lin16(b);
+    } else if (mode == OUT_LINEAR_F16) {
+        uint16_t *p = reinterpret_cast<uint16_t *>(dst) + ((size_t)dy * a.dev_w + dx) * 3;
+        p[0] = (uint16_t)f16_bits(r);
+        p[1] = (uint16_t)f16_bits(g);
+        p[2] = (uint16_t)f16_bits(b);
+    }
+}
+
+__device__ __forceinline__ void store_both(const DevArgs &a, const Lds &s, uint32_t dx, uint32_t dy, float r,
+                                           float g, float b) {
+    store_px(a, s, a.out_mode, a.dst, dx, dy, r, g, b);
+    if (a.out_mode2 != OUT_NONE) store_px(a, s, a.out_mode2, a.dst2, dx, dy, r, g, b);
+}
+
+// Kernel::row_cols for one pixel of a Bayer mosaic, read straight from the frame
+// (process.rs:1076-1126): the interior formulas, or at the frame border the mean of the
+// in-frame same-colour 3x3 samples in row-major order.
+__device__ void bilinear_global(const DevArgs &a, const Lds &s, int x, int y, float rgb[3]) {
+    const int w = (int)a.width, h = (int)a.height;
+    uint32_t c0 = chan_at(a, x, y);
+    if (x > 0 && y > 0 && x + 1 < w && y + 1 < h) {
+        float own = norm_at(a, s, x, y);
+        float l = norm_at(a, s, x - 1, y), r = norm_at(a, s, x + 1, y);
+        float u = norm_at(a, s, x, y - 1), d = norm_at(a, s, x, y + 1);
+        if (c0 == 1) {
+            uint32_t ch = chan_at(a, x + 1, y);
+            float hv = (l + r) * 0.5f, vv = (u + d) * 0.5f;
+            rgb[1] = own;
+            rgb[0] = ch == 0 ? hv : vv;
+            rgb[2] = ch == 0 ? vv : hv;
+        } else {
+            float orth = (l + r + u + d) * 0.25f;
+            float diag = (norm_at(a, s, x - 1, y - 1) + norm_at(a, s, x + 1, y - 1) + norm_at(a, s, x - 1, y + 1) +
+                          norm_at(a, s, x + 1, y + 1)) *
+                         0.25f;
+            rgb[0] = c0 == 0 ? own : diag;
+            rgb[1] = orth;
+            rgb[2] = c0 == 0 ? diag : own;
+        }
+        return;
+    }
+    for (uint32_t c = 0; c < 3; c++) {
+        float sum = 0.0f, n = 0.0f;
+        if (c == c0) {
+            sum = norm_at(a, s, x, y);
+            n = 1.0f;
+        } else {
+            for (int dy = -1; dy <= 1; dy++) {
+                for (int dx = -1; dx <= 1; dx++) {
+                    int xx = x + dx, yy = y + dy;
+                    if (chan_at(a, xx, yy) != c) continue;
+                    if (xx < 0 || xx >= w || yy < 0 || yy >= h) continue;
+                    sum += norm_at(a, s, xx, yy);
+                    n += 1.0f;
+                }
+            }
+        }
+        rgb[c] = n > 0.0f ? sum / n : 0.0f;
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Bilinear develop (the reference's `develop` kernel, develop.wgsl:185-260; CPU
+// process.rs:1044-1129). A tile of BW x BH developed pixels; the normalised frame samples it
+// reads (one-sample apron) are staged in LDS; each thread develops 2x2 quads.
+
+constexpr int BW = 128, BH = 16;
+constexpr int BLW = BW + 2, BLH = BH + 2;
+
+__global__ __launch_bounds__(256) void k_develop_bilinear(DevArgs a, uint32_t tiles_x, uint32_t ntiles) {
+    __shared__ Lds s;
+    __shared__ float t[BLH][BLW + 2];
+    load_lds(a, s);
+    const int w = (int)a.width, h = (int)a.height;
+    for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const uint32_t tx = tile % tiles_x, ty = tile / tiles_x;
+        const int dx0 = (int)(tx * BW), dy0 = (int)(a.band_y0 + ty * BH);
+        const int x0 = (int)a.crop_x + dx0, y0 = (int)a.crop_y + dy0;
+        __syncthreads();  // LDS tables ready / previous tile consumed
+        for (int i = threadIdx.x; i < BLH * BLW; i += blockDim.x) {
+            int r = i / BLW, c = i - r * BLW;
+            int x = x0 - 1 + c, y = y0 - 1 + r;
+            t[r][c] = (x >= 0 && x < w && y >= 0 && y < h) ? norm_at(a, s, x, y) : 0.0f;
+        }
+        __syncthreads();
+        for (int q = threadIdx.x; q < (BW / 2) * (BH / 2); q += blockDim.x) {
+            const int qx = q % (BW / 2), qy = q / (BW / 2);
+#pragma unroll
+            for (int p = 0; p < 4; p++) {
+                const int ddx = dx0 + qx * 2 + (p & 1), ddy = dy0 + qy * 2 + (p >> 1);
+                if (ddx >= (int)a.dev_w || ddy >= (int)a.band_y1) continue;
+                const int x = (int)a.crop_x + ddx, y = (int)a.crop_y + ddy;
+                float rgb[3];
+                if (x > 0 && y > 0 && x + 1 < w && y + 1 < h) {
+                    const int lx = x - x0 + 1, ly = y - y0 + 1;
+                    const uint32_t c0 = chan_at(a, x, y);
+                    float own = t[ly][lx], l = t[ly][lx - 1], r = t[ly][lx + 1], u = t[ly - 1][lx],
+                          d = t[ly + 1][lx];
+                    if (c0 == 1) {
+                        const uint32_t ch = chan_at(a, x + 1, y);
+                        float hv = (l + r) * 0.5f, vv = (u + d) * 0.5f;
+                        rgb[1] = own;
+                        rgb[0] = ch == 0 ? hv : vv;
+                        rgb[2] = ch == 0 ? vv : hv;
+                    } else {
+                        float orth = (l + r + u + d) * 0.25f;
+                        float diag =
+                            (t[ly - 1][lx - 1] + t[ly - 1][lx + 1] + t[ly + 1][lx - 1] + t[ly + 1][lx + 1]) * 0.25f;
+                        rgb[0] = c0 == 0 ? own : diag;
+                        rgb[1] = orth;
+                        rgb[2] = c0 == 0 ? diag : own;
+                    }
+                } else {
+                    bilinear_global(a, s, x, y, rgb);
+                }
+                float o0, o1, o2;
+                finish(a, rgb[0], rgb[1], rgb[2], false, o0, o1, o2);
+                store_both(a, s, (uint32_t)ddx, (uint32_t)ddy, o0, o1, o2);
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// AHD develop (ahd.rs:191-393; GPU statement edge.wgsl:386-548), fused into one pass: a tile of
+// AW x AH developed pixels stages the balanced mosaic (apron 5), both green candidates (apron 3),
+// both candidates' L, a, b (apron 2) and both homogeneity counts (apron 1) in LDS, then blends,
+// finishes and encodes. Pixels nearer the frame border than `ahd_halo` are developed bilinearly
+// (strip_to, process.rs:1183-1224).
+
+constexpr int AW = 32, AH = 16;
+constexpr int MW = AW + 10, MH = AH + 10;
+constexpr int GW = AW + 6, GH = AH + 6;
+constexpr int LW = AW + 4, LH = AH + 4;
+constexpr int HW = AW + 2, HH = AH + 2;
+
+struct AhdLds {
+    float m[MH][MW];
+    float g[2][GH][GW];
+    float lab[6][LH][LW];
+    float hom[2][HH][HW];
+};
+
+// Candidate RGB of candidate d at M index (mi, mj) / G index (mi - 2, mj - 2)
+// (ahd.rs:249-291; the measured colour kept, the others from colour differences).
+__device__ __forceinline__ void cand_rgb(const AhdLds &L, int d, int mi, int mj, uint32_t c, bool hr, float &red,
+                                         float &gg, float &blue) {
+    const int gi = mi - 2, gj = mj - 2;
+#define DF(di, dj) (L.m[mi + (di)][mj + (dj)] - L.g[d][gi + (di)][gj + (dj)])
+    const float g = L.g[d][gi][gj];
+    const float cm = L.m[mi][mj];
+    float diag = g + 0.25f * (DF(-1, -1) + DF(-1, 1) + DF(1, -1) + DF(1, 1));
+    float row = g + 0.5f * (DF(0, -1) + DF(0, 1));
+    float col = g + 0.5f * (DF(-1, 0) + DF(1, 0));
+#undef DF
+    float on_r = hr ? row : col, on_b = hr ? col : row;
+    red = c == 0 ? cm : (c == 1 ? on_r : diag);
+    blue = c == 2 ? cm : (c == 1 ? on_b : diag);
+    gg = g;
+}
+
+__global__ __launch_bounds__(256) void k_develop_ahd(DevArgs a, uint32_t tiles_x, uint32_t ntiles) {
+    __shared__ Lds s;
+    __shared__ AhdLds L;
+    load_lds(a, s);
+    const int w = (int)a.width, h = (int)a.height;
+    const int halo = (int)a.ahd_halo;
+    for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const uint32_t tx = tile % tiles_x, ty = tile / tiles_x;
+        const int dx0 = (int)(tx * AW), dy0 = (int)(a.band_y0 + ty * AH);
+        const int x0 = (int)a.crop_x + dx0, y0 = (int)a.crop_y + dy0;
+        // Does the tile hold any AHD pixel (frame x in [halo, w - halo), y in [halo, h - halo))?
+        const bool any = x0 + AW > halo && x0 < w - halo && y0 + AH > halo && y0 < h - halo;
+        __syncthreads();
+        if (any) {
+            // 1. balanced mosaic, clipped at 1 (ahd.rs:143-151)
+            for (int i = threadIdx.x; i < MH * MW; i += blockDim.x) {
+                int r = i / MW, c = i - r * MW;
+                int x = x0 - 5 + c, y = y0 - 5 + r;
+                float v = 0.0f;
+                if (x >= 0 && x < w && y >= 0 && y < h) v = fminf(norm_at(a, s, x, y) * a.wb[chan_at(a, x, y)], 1.0f);
+                L.m[r][c] = v;
+            }
+            __syncthreads();
+            // 2. green along rows and columns (ahd.rs:225-244)
+            for (int i = threadIdx.x; i < GH * GW; i += blockDim.x) {
+                int gi = i / GW, gj = i - gi * GW;
+                int mi = gi + 2, mj = gj + 2;
+                int x = x0 - 3 + gj, y = y0 - 3 + gi;
+                float c = L.m[mi][mj];
+                float gh = c, gv = c;
+                if (chan_at(a, x, y) != 1) {
+                    float l = L.m[mi][mj - 1], r = L.m[mi][mj + 1], l2 = L.m[mi][mj - 2], r2 = L.m[mi][mj + 2];
+                    float u = L.m[mi - 1][mj], d = L.m[mi + 1][mj], u2 = L.m[mi - 2][mj], d2 = L.m[mi + 2][mj];
+                    gh = fminf(fmaxf((l + r) * 0.5f + (2.0f * c - l2 - r2) * 0.25f, fminf(l, r)), fmaxf(l, r));
+                    gv = fminf(fmaxf((u + d) * 0.5f + (2.0f * c - u2 - d2) * 0.25f, fminf(u, d)), fmaxf(u, d));
+                }
+                L.g[0][gi][gj] = gh;
+                L.g[1][gi][gj] = gv;
+            }
+            __syncthreads();
+            // 3. candidates' perceptual coordinates (ahd.rs:249-298)
+            for (int i = threadIdx.x; i < LH * LW; i += blockDim.x) {
+                int li = i / LW, lj = i - li * LW;
+                int x = x0 - 2 + lj, y = y0 - 2 + li;
+                uint32_t c = chan_at(a, x, y);
+                bool hr = c == 1 && chan_at(a, x + 1, y) == 0;
+#pragma unroll
+                for (int d = 0; d < 2; d++) {
+                    float red, g, blue;
+                    cand_rgb(L, d, li + 3, lj + 3, c, hr, red, g, blue);
+                    float sr = sqrtf(fmaxf(red, 0.0f)), sg = sqrtf(fmaxf(g, 0.0f)), sb = sqrtf(fmaxf(blue, 0.0f));
+                    L.lab[d * 3 + 0][li][lj] = (sr + 2.0f * sg + sb) * 0.25f;
+                    L.lab[d * 3 + 1][li][lj] = sr - sg;
+                    L.lab[d * 3 + 2][li][lj] = sb - sg;
+                }
+            }
+            __syncthreads();
+            // 4. thresholds and homogeneity counts (ahd.rs:300-351)
+            for (int i = threadIdx.x; i < HH * HW; i += blockDim.x) {
+                int hi = i / HW, hj = i - hi * HW;
+                int li = hi + 1, lj = hj + 1;
+#define STEP_L(d, di, dj) fabsf(L.lab[(d) * 3][li + (di)][lj + (dj)] - L.lab[(d) * 3][li][lj])
+#define STEP_C(d, di, dj)                                                                           \
+    ((L.lab[(d) * 3 + 1][li + (di)][lj + (dj)] - L.lab[(d) * 3 + 1][li][lj]) *                      \
+         (L.lab[(d) * 3 + 1][li + (di)][lj + (dj)] - L.lab[(d) * 3 + 1][li][lj]) +                  \
+     (L.lab[(d) * 3 + 2][li + (di)][lj + (dj)] - L.lab[(d) * 3 + 2][li][lj]) *                      \
+         (L.lab[(d) * 3 + 2][li + (di)][lj + (dj)] - L.lab[(d) * 3 + 2][li][lj]))
+                float el = fminf(fmaxf(STEP_L(0, 0, -1), STEP_L(0, 0, 1)), fmaxf(STEP_L(1, -1, 0), STEP_L(1, 1, 0)));
+                float ec = fminf(fmaxf(STEP_C(0, 0, -1), STEP_C(0, 0, 1)), fmaxf(STEP_C(1, -1, 0), STEP_C(1, 1, 0)));
+#pragma unroll
+                for (int d = 0; d < 2; d++) {
+                    float cnt = 0.0f;
+#pragma unroll
+                    for (int di = -1; di <= 1; di++) {
+#pragma unroll
+                        for (int dj = -1; dj <= 1; dj++) {
+                            if (di == 0 && dj == 0) continue;
+                            if (STEP_L(d, di, dj) <= el && STEP_C(d, di, dj) <= ec) cnt += 1.0f;
+                        }
+                    }
+                    L.hom[d][hi][hj] = cnt;
+                }
+#undef STEP_L
+#undef STEP_C
+            }
+            __syncthreads();
+        }
+        // 5. blend (ahd.rs:353-392), finish and store; border pixels bilinear
+        for (int i = threadIdx.x; i < AH * AW; i += blockDim.x) {
+            int oi = i / AW, oj = i - oi * AW;
+            int ddx = dx0 + oj, ddy = dy0 + oi;
+            if (ddx >= (int)a.dev_w || ddy >= (int)a.band_y1) continue;
+            int x = x0 + oj, y = y0 + oi;
+            float o0, o1, o2;
+            if (x >= halo && x < w - halo && y >= halo && y < h - halo) {
+                float sh = 0.0f, sv = 0.0f;
+#pragma unroll
+                for (int di = 0; di < 3; di++)
+#pragma unroll
+                    for (int dj = 0; dj < 3; dj++) {
+                        sh += L.hom[0][oi + di][oj + dj];
+                        sv += L.hom[1][oi + di][oj + dj];
+                    }
+                float wgt = sh + sv > 0.0f ? sh / (sh + sv) : 0.5f;
+                uint32_t c = chan_at(a, x, y);
+                bool hr = c == 1 && chan_at(a, x + 1, y) == 0;
+                float r0, g0, b0, r1, g1, b1;
+                cand_rgb(L, 0, oi + 5, oj + 5, c, hr, r0, g0, b0);
+                cand_rgb(L, 1, oi + 5, oj + 5, c, hr, r1, g1, b1);
+                float iw = 1.0f - wgt;
+                float r = wgt * r0 + iw * r1, g = wgt * g0 + iw * g1, b = wgt * b0 + iw * b1;
+                finish(a, r, g, b, true, o0, o1, o2);
+            } else {
+                float rgb[3];
+                bilinear_global(a, s, x, y, rgb);
+                finish(a, rgb[0], rgb[1], rgb[2], false, o0, o1, o2);
+            }
+            store_both(a, s, (uint32_t)ddx, (uint32_t)ddy, o0, o1, o2);
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Launchers
+
+static uint32_t persistent_grid(uint32_t ntiles, int num_cus, int blocks_per_cu) {
+    uint32_t g = (uint32_t)num_cus * (uint32_t)blocks_per_cu;
+    return ntiles < g ? ntiles : g;
+}
+
+hipError_t launch_develop(const DevArgs &a, bool ahd, hipStream_t stream, int num_cus) {
+    const uint32_t rows = a.band_y1 - a.band_y0;
+    if (rows == 0 || a.dev_w == 0) return hipSuccess;
+    if (ahd) {
+        uint32_t tx = (a.dev_w + AW - 1) / AW, ty = (rows + AH - 1) / AH;
+        uint32_t n = tx * ty;
+        hipLaunchKernelGGL(k_develop_ahd, dim3(persistent_grid(n, num_cus, 4)), dim3(256), 0, stream, a, tx, n);
+    } else {
+        uint32_t tx = (a.dev_w + BW - 1) / BW, ty = (rows + BH - 1) / BH;
+        uint32_t n = tx * ty;
+        hipLaunchKernelGGL(k_develop_bilinear, dim3(persistent_grid(n, num_cus, 8)), dim3(256), 0, stream, a, tx,
+                           n);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace frh

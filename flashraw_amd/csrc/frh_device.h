@@ -1,0 +1,76 @@
+// Device-side parameter block and scalar helpers shared by the develop kernels (gfx950).
+//
+// Arithmetic rules (they make the kernels bit-identical to the reference's CPU develop,
+// crates/flashraw/src/process.rs and ahd.rs): every file is compiled with -ffp-contract=off, so
+// no multiply-add is fused; f32 division and sqrt are HIP's correctly rounded defaults; sums keep
+// the reference's left-to-right order.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace frh {
+
+// Output encodings (frh_develop_output).
+enum OutMode : int32_t {
+    OUT_RGBA8 = 0,
+    OUT_LINEAR_F32 = 1,
+    OUT_SRGB16 = 2,
+    OUT_LINEAR16 = 3,
+    OUT_LINEAR_F16 = 4,
+    OUT_NONE = -1
+};
+
+// Everything one develop launch reads, passed by value as the kernel argument.
+struct DevArgs {
+    // input: `src` points at the window's sample (0, 0); `stride` samples per raster row
+    const uint16_t *src;
+    uint32_t stride;
+    uint32_t width, height;  // frame (window) size
+    uint32_t ox, oy;         // sensor position of frame sample (0, 0)
+    // calibration pattern (Calib, process.rs:246-299): cal_pw x cal_ph <= 64 entries
+    uint32_t cal_pw, cal_ph;
+    uint32_t cal_pow2;       // 1 when cal_pw and cal_ph are powers of two (mask instead of modulo)
+    float black[64];
+    float scale[64];
+    const float *rowd;
+    uint32_t n_rowd;
+    const float *cold;
+    uint32_t n_cold;
+    // Bayer channel of sensor parity ((Y & 1) * 2 + (X & 1))
+    uint32_t chan[4];
+    // develop region and band
+    uint32_t crop_x, crop_y, dev_w, dev_h;
+    uint32_t band_y0, band_y1;  // developed rows [band_y0, band_y1) of this launch
+    float wb[4];
+    float m[9];
+    int32_t has_levels;
+    float lscale, loffset;
+    int32_t orientation;  // EXIF 1..8 for the display encodings
+    uint32_t dst_w;       // destination width of the display encodings (oriented)
+    int32_t out_mode;
+    void *dst;
+    int32_t out_mode2;
+    void *dst2;
+    const uint8_t *lut8;     // 4097 entries (+ padding to 4100)
+    const float2 *srgb16;    // 16384 (value, step) pairs
+    uint32_t ahd_halo;       // 6: pixels nearer the border than this are bilinear (ahd.rs:33)
+};
+
+// Rust `x as u32` for x below 2^32: truncation, with NaN and everything below 1 giving 0.
+__device__ __forceinline__ uint32_t trunc_u32(float x) { return x >= 1.0f ? (uint32_t)x : 0u; }
+
+// The sRGB8 table index of linear v: `min(trunc(clamp(v, 0, 1) * 4096), 4096)`
+// (process.rs:1712-1728). Scaling by 4096 is exact, so clamping after the scale is the same.
+__device__ __forceinline__ uint32_t lut8_index(float v) {
+    float x = v * 4096.0f;
+    return x >= 1.0f ? (uint32_t)fminf(x, 4096.0f) : 0u;
+}
+
+// f32::clamp(x, 0, 1): `if x < 0 {0}; if x > 1 {1}; x` (NaN and signed zero pass through).
+__device__ __forceinline__ float rclamp01(float x) {
+    x = x < 0.0f ? 0.0f : x;
+    return x > 1.0f ? 1.0f : x;
+}
+
+}  // namespace frh

@@ -1,0 +1,57 @@
+// Host-side internals shared by the C ABI translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "flashraw_hip.h"
+
+namespace frh {
+
+// Records `msg` as the calling thread's last error and returns `code`.
+int32_t fail(int32_t code, const std::string &msg);
+// A HIP error as a status (out of memory, or the device failed).
+int32_t hip_fail(hipError_t e, const char *what);
+// The 4097-entry sRGB 8-bit table (process.rs:1903-1915).
+void build_srgb8(uint8_t out[4097]);
+
+// Bytes of freed device buffers a context keeps for reuse (the reference's POOL_MAX_BYTES,
+// crates/flashraw-gpu/src/context.rs:153).
+constexpr uint64_t POOL_MAX_BYTES = 1ull << 30;
+
+}  // namespace frh
+
+struct frh_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int num_cus = 0;
+    std::string name;
+    uint64_t total_mem = 0;
+    uint8_t *d_lut8 = nullptr;
+    float *d_srgb16 = nullptr;
+    uint64_t max_binding = 0;
+    std::mutex mu;
+    struct Block {
+        void *ptr;
+        uint64_t size;
+    };
+    std::vector<Block> pool;
+    uint64_t pooled = 0;
+
+    void *alloc(uint64_t bytes, hipError_t *err);
+    void release(void *p, uint64_t bytes);
+    ~frh_context();
+};
+
+struct frh_image {
+    frh_context *ctx;
+    void *dptr;
+    uint64_t bytes;
+    uint32_t w, h, ch;
+    int32_t fmt;
+};

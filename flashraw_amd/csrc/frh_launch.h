@@ -1,0 +1,13 @@
+// Host-side launchers of the device kernels (defined next to the kernels they launch).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "frh_device.h"
+
+namespace frh {
+
+// Develop rows [a.band_y0, a.band_y1) of the developed image on `stream` (bilinear, or AHD).
+hipError_t launch_develop(const DevArgs &a, bool ahd, hipStream_t stream, int num_cus);
+
+}  // namespace frh

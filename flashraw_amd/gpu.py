@@ -1,0 +1,598 @@
+"""Host-side mirror of the reference's `flashraw-gpu` develop interface over the C ABI.
+
+Names, argument meaning and error behaviour follow crates/flashraw-gpu/src/run.rs
+(`GpuFrame`, `GpuOp`, `DevelopSpec`, `EdgeSpec`, `DevelopOutput`, `GpuTimings`, `GpuRun`),
+crates/flashraw-gpu/src/context.rs (`context()`, `available()`, `override_limits`) and
+crates/flashraw-gpu/src/image.rs (`GpuImage`). Every call goes to libflashraw_hip.so; there is no
+CPU fallback, and the module fails loudly when the library is missing.
+"""
+from __future__ import annotations
+
+import ctypes
+import enum
+import os
+import threading
+from dataclasses import dataclass, field
+from typing import Optional, Sequence
+
+import numpy as np
+
+_LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libflashraw_hip.so")
+
+
+# ---- errors (crates/flashraw-core/src/error.rs kinds; numbering of flashraw.h:44-114) ----
+
+class FlashRawError(Exception):
+    kind = "Other"
+
+    def __init__(self, code: int, message: str):
+        super().__init__(f"{self.kind}: {message}")
+        self.code = code
+        self.message = message
+
+
+class GpuUnavailable(FlashRawError):
+    kind = "GpuUnavailable"
+
+
+class OutOfMemory(FlashRawError):
+    kind = "OutOfMemory"
+
+
+class LimitExceeded(FlashRawError):
+    kind = "LimitExceeded"
+
+
+class NotAvailable(FlashRawError):
+    kind = "NotAvailable"
+
+
+class InvalidArgument(FlashRawError):
+    kind = "InvalidArgument"
+
+
+class Internal(FlashRawError):
+    kind = "Internal"
+
+
+class Cancelled(FlashRawError):
+    kind = "Cancelled"
+
+
+_ERRORS = {6: GpuUnavailable, 7: OutOfMemory, 8: LimitExceeded, 9: NotAvailable, 10: InvalidArgument,
+           12: Internal, 14: Cancelled}
+
+
+# ---- C structures (include/flashraw_hip.h) ----
+
+class _Rect(ctypes.Structure):
+    _fields_ = [("x", ctypes.c_uint32), ("y", ctypes.c_uint32), ("width", ctypes.c_uint32),
+                ("height", ctypes.c_uint32)]
+
+
+class _Frame(ctypes.Structure):
+    _fields_ = [
+        ("size", ctypes.c_uint32), ("input", ctypes.c_int32), ("data", ctypes.c_void_p),
+        ("data_len", ctypes.c_uint64), ("data_on_device", ctypes.c_int32),
+        ("raster_width", ctypes.c_uint32), ("raster_height", ctypes.c_uint32), ("cpp", ctypes.c_uint32),
+        ("window", _Rect), ("origin_x", ctypes.c_uint32), ("origin_y", ctypes.c_uint32),
+        ("linearization", ctypes.c_void_p), ("linearization_len", ctypes.c_uint32),
+    ]
+
+
+class _FrameInfo(ctypes.Structure):
+    _fields_ = [
+        ("size", ctypes.c_uint32), ("cfa_width", ctypes.c_uint32), ("cfa_height", ctypes.c_uint32),
+        ("cfa", ctypes.c_uint8 * 64), ("black_rows", ctypes.c_uint32), ("black_cols", ctypes.c_uint32),
+        ("black_samples", ctypes.c_uint32), ("black_values", ctypes.c_void_p),
+        ("black_values_len", ctypes.c_uint32), ("black_row_deltas", ctypes.c_void_p),
+        ("black_row_deltas_len", ctypes.c_uint32), ("black_col_deltas", ctypes.c_void_p),
+        ("black_col_deltas_len", ctypes.c_uint32), ("white_values", ctypes.c_void_p),
+        ("white_values_len", ctypes.c_uint32),
+    ]
+
+
+class _EdgeSpec(ctypes.Structure):
+    _fields_ = [
+        ("size", ctypes.c_uint32), ("period", ctypes.c_uint32), ("cx", ctypes.c_uint32), ("cy", ctypes.c_uint32),
+        ("beta", ctypes.c_float), ("halo", ctypes.c_uint32), ("canon", ctypes.c_void_p),
+        ("canon_len", ctypes.c_uint32), ("g0", ctypes.c_void_p), ("g0_len", ctypes.c_uint32),
+        ("dir", ctypes.c_void_p), ("dir_len", ctypes.c_uint32), ("ahd", ctypes.c_int32),
+    ]
+
+
+class _DevelopSpec(ctypes.Structure):
+    _fields_ = [
+        ("size", ctypes.c_uint32), ("crop", _Rect), ("half_size", ctypes.c_int32), ("wb", ctypes.c_float * 4),
+        ("cam_to_rgb", ctypes.c_float * 9), ("orientation", ctypes.c_int32), ("output", ctypes.c_int32),
+        ("output2", ctypes.c_int32), ("has_levels", ctypes.c_int32), ("levels", ctypes.c_float * 2),
+        ("edge", ctypes.POINTER(_EdgeSpec)), ("curve", ctypes.c_void_p), ("tone", ctypes.c_void_p),
+    ]
+
+
+class _Timings(ctypes.Structure):
+    _fields_ = [("upload_ms", ctypes.c_double), ("compute_ms", ctypes.c_double), ("kernel_ms", ctypes.c_double),
+                ("has_kernel_ms", ctypes.c_int32), ("bands", ctypes.c_uint32)]
+
+
+_lib = None
+_lib_lock = threading.Lock()
+
+
+def lib():
+    """The loaded C ABI library (raises when it has not been built)."""
+    global _lib
+    with _lib_lock:
+        if _lib is None:
+            if not os.path.exists(_LIB_PATH):
+                raise ImportError(f"libflashraw_hip.so not built ({_LIB_PATH}); run __graft_entry__.build()")
+            L = ctypes.CDLL(_LIB_PATH)
+            vp, i32, u32, u64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_uint32, ctypes.c_uint64
+            L.frh_available.restype = i32
+            L.frh_context_create.argtypes = [i32, ctypes.POINTER(vp)]
+            L.frh_context_create.restype = i32
+            L.frh_context_destroy.argtypes = [vp]
+            L.frh_context_info.argtypes = [vp, ctypes.c_char_p, ctypes.c_size_t]
+            L.frh_context_info.restype = i32
+            L.frh_set_limits.argtypes = [vp, u64]
+            L.frh_set_limits.restype = i32
+            L.frh_last_error_message.restype = ctypes.c_char_p
+            L.frh_run_develop.argtypes = [vp, ctypes.POINTER(_Frame), ctypes.POINTER(_FrameInfo),
+                                          ctypes.POINTER(_DevelopSpec), ctypes.POINTER(vp), ctypes.POINTER(vp),
+                                          ctypes.POINTER(_Timings)]
+            L.frh_run_develop.restype = i32
+            L.frh_develop_into.argtypes = [vp, ctypes.POINTER(_Frame), ctypes.POINTER(_FrameInfo),
+                                           ctypes.POINTER(_DevelopSpec), vp, u64, vp, u64, vp]
+            L.frh_develop_into.restype = i32
+            L.frh_develop_output_size.argtypes = [ctypes.POINTER(_Frame), ctypes.POINTER(_DevelopSpec), i32,
+                                                  ctypes.POINTER(u32), ctypes.POINTER(u32), ctypes.POINTER(u32),
+                                                  ctypes.POINTER(u64)]
+            L.frh_develop_output_size.restype = i32
+            for name, rt in [("frh_image_width", u32), ("frh_image_height", u32), ("frh_image_channels", u32),
+                             ("frh_image_format", i32), ("frh_image_device_ptr", vp), ("frh_image_bytes", u64)]:
+                getattr(L, name).argtypes = [vp]
+                getattr(L, name).restype = rt
+            L.frh_image_download.argtypes = [vp, vp, u64]
+            L.frh_image_download.restype = i32
+            L.frh_image_free.argtypes = [vp]
+            L.frh_srgb8_table.argtypes = [vp, ctypes.c_size_t]
+            L.frh_srgb8_table.restype = i32
+            _lib = L
+        return _lib
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        msg = lib().frh_last_error_message().decode(errors="replace")
+        raise _ERRORS.get(rc, FlashRawError)(rc, msg)
+
+
+# ---- core types (crates/flashraw-core/src/types.rs) ----
+
+class CfaColor(enum.IntEnum):
+    Red = 0
+    Green = 1
+    Blue = 2
+    Cyan = 3
+    Magenta = 4
+    Yellow = 5
+    White = 6
+    Emerald = 7
+
+
+@dataclass(frozen=True)
+class CfaPattern:
+    """CfaPattern (types.rs:152-252): colours row-major, at most 8x8."""
+    width: int
+    height: int
+    colors: tuple
+
+    def color_at(self, x: int, y: int) -> int:
+        return self.colors[(y % self.height) * self.width + (x % self.width)]
+
+    def is_bayer(self) -> bool:
+        if (self.width, self.height) != (2, 2):
+            return False
+        return sorted(self.colors) == [0, 1, 1, 2]
+
+    @staticmethod
+    def bayer(name: str) -> "CfaPattern":
+        m = {"R": 0, "G": 1, "B": 2}
+        return CfaPattern(2, 2, tuple(m[c] for c in name))
+
+
+CfaPattern.RGGB = CfaPattern.bayer("RGGB")
+CfaPattern.BGGR = CfaPattern.bayer("BGGR")
+CfaPattern.GRBG = CfaPattern.bayer("GRBG")
+CfaPattern.GBRG = CfaPattern.bayer("GBRG")
+
+
+@dataclass
+class BlackLevel:
+    """BlackLevel (types.rs:357-436)."""
+    rows: int = 1
+    cols: int = 1
+    samples: int = 1
+    values: Sequence[float] = (0.0,)
+    row_deltas: Sequence[float] = ()
+    col_deltas: Sequence[float] = ()
+
+    @staticmethod
+    def uniform(v: float) -> "BlackLevel":
+        return BlackLevel(1, 1, 1, (float(v),))
+
+    @staticmethod
+    def pattern2x2(v: Sequence[float]) -> "BlackLevel":
+        return BlackLevel(2, 2, 1, tuple(float(x) for x in v))
+
+
+@dataclass
+class WhiteLevel:
+    """WhiteLevel (types.rs:438-467)."""
+    values: Sequence[int] = (65535,)
+
+    @staticmethod
+    def uniform(v: int) -> "WhiteLevel":
+        return WhiteLevel((int(v),))
+
+
+@dataclass
+class FrameInfo:
+    """The FrameInfo fields the develop path reads (crates/flashraw-core/src/decoder.rs:25-74)."""
+    width: int
+    height: int
+    cfa: Optional[CfaPattern] = None
+    black_level: BlackLevel = field(default_factory=BlackLevel)
+    white_level: WhiteLevel = field(default_factory=WhiteLevel)
+
+    def _c(self):
+        s = _FrameInfo()
+        s.size = ctypes.sizeof(_FrameInfo)
+        keep = []
+        if self.cfa is not None:
+            s.cfa_width, s.cfa_height = self.cfa.width, self.cfa.height
+            for i, c in enumerate(self.cfa.colors):
+                s.cfa[i] = int(c)
+        b = self.black_level
+        s.black_rows, s.black_cols, s.black_samples = b.rows, b.cols, b.samples
+        for name, vals in (("black_values", b.values), ("black_row_deltas", b.row_deltas),
+                           ("black_col_deltas", b.col_deltas)):
+            if len(vals):
+                a = np.ascontiguousarray(vals, dtype=np.float32)
+                keep.append(a)
+                setattr(s, name, a.ctypes.data)
+                setattr(s, name + "_len", a.size)
+        wv = np.ascontiguousarray(self.white_level.values, dtype=np.uint32)
+        keep.append(wv)
+        s.white_values, s.white_values_len = wv.ctypes.data, wv.size
+        return s, keep
+
+
+@dataclass
+class Rect:
+    x: int
+    y: int
+    width: int
+    height: int
+
+    @staticmethod
+    def full(w: int, h: int) -> "Rect":
+        return Rect(0, 0, w, h)
+
+
+# ---- run.rs types ----
+
+class DevelopOutput(enum.IntEnum):
+    """DevelopOutput (run.rs:96-111); 16-bit encodings hold 3 samples per pixel here."""
+    Rgba8 = 0
+    LinearRgbF32 = 1
+    Rgb16 = 2
+    LinearRgb16 = 3
+    LinearRgbF16 = 4
+
+
+class Orientation(enum.IntEnum):
+    """Orientation (types.rs:616-640), EXIF numbering."""
+    Normal = 1
+    FlipH = 2
+    Rotate180 = 3
+    FlipV = 4
+    Transpose = 5
+    Rotate90 = 6
+    Transverse = 7
+    Rotate270 = 8
+
+    def swaps_dimensions(self) -> bool:
+        return self.value >= 5
+
+
+@dataclass
+class EdgeSpec:
+    """EdgeSpec (run.rs:141-160). The AHD form uses the pattern and the halo only."""
+    period: int = 2
+    cx: int = 0
+    cy: int = 0
+    beta: float = 0.0
+    halo: int = 6
+    canon: Sequence[int] = ()
+    g0: Sequence[float] = ()
+    dir: Sequence[float] = ()
+    ahd: bool = False
+
+
+def ahd_spec(info: FrameInfo, origin=(0, 0)) -> Optional[EdgeSpec]:
+    """flashraw::gpu::ahd_spec (crates/flashraw/src/gpu.rs:133-138): the AHD spec of a Bayer frame
+    whose sample (0, 0) sits at sensor `origin` (HALO 6, ahd.rs:33), or None for other patterns."""
+    if info.cfa is None or not info.cfa.is_bayer():
+        return None
+    canon = tuple(info.cfa.colors)
+    return EdgeSpec(period=2, cx=origin[0] % 2, cy=origin[1] % 2, beta=0.0, halo=6, canon=canon, ahd=True)
+
+
+@dataclass
+class DevelopSpec:
+    """DevelopSpec (run.rs:115-135). `output2` asks the same pass for a second encoding."""
+    crop: Rect
+    wb: Sequence[float] = (1.0, 1.0, 1.0, 1.0)
+    cam_to_rgb: Sequence[Sequence[float]] = ((1.0, 0.0, 0.0), (0.0, 1.0, 0.0), (0.0, 0.0, 1.0))
+    orientation: Orientation = Orientation.Normal
+    output: DevelopOutput = DevelopOutput.Rgba8
+    levels: Optional[Sequence[float]] = None
+    half_size: bool = False
+    curve: object = None
+    tone: object = None
+    edge: Optional[EdgeSpec] = None
+    output2: Optional[DevelopOutput] = None
+
+    def _c(self):
+        s = _DevelopSpec()
+        s.size = ctypes.sizeof(_DevelopSpec)
+        s.crop = _Rect(self.crop.x, self.crop.y, self.crop.width, self.crop.height)
+        s.half_size = int(self.half_size)
+        for i in range(4):
+            s.wb[i] = float(self.wb[i]) if i < len(self.wb) else 1.0
+        m = np.asarray(self.cam_to_rgb, dtype=np.float32).reshape(-1)
+        for i in range(9):
+            s.cam_to_rgb[i] = float(m[i])
+        s.orientation = int(self.orientation)
+        s.output = int(self.output)
+        s.output2 = -1 if self.output2 is None else int(self.output2)
+        if self.levels is not None:
+            s.has_levels = 1
+            s.levels[0], s.levels[1] = float(self.levels[0]), float(self.levels[1])
+        keep = []
+        if self.curve is not None or self.tone is not None:
+            s.curve = 1 if self.curve is not None else None
+            s.tone = 1 if self.tone is not None else None
+        if self.edge is not None:
+            e = _EdgeSpec()
+            e.size = ctypes.sizeof(_EdgeSpec)
+            e.period, e.cx, e.cy, e.beta, e.halo = self.edge.period, self.edge.cx, self.edge.cy, \
+                self.edge.beta, self.edge.halo
+            e.ahd = int(self.edge.ahd)
+            if len(self.edge.canon):
+                c = np.ascontiguousarray(self.edge.canon, dtype=np.uint8)
+                keep.append(c)
+                e.canon, e.canon_len = c.ctypes.data, c.size
+            keep.append(e)
+            s.edge = ctypes.pointer(e)
+        return s, keep
+
+
+class GpuOp:
+    """GpuOp (run.rs:219-225); only Develop is accepted by this path."""
+
+    @dataclass
+    class Develop:
+        spec: DevelopSpec
+
+
+def _device_ptr(t):
+    """Device pointer of a torch tensor (or an int address)."""
+    if isinstance(t, int):
+        return t
+    return t.data_ptr()
+
+
+@dataclass
+class GpuFrame:
+    """GpuFrame (run.rs:27-93). `data` is a numpy u16 array (host) or a torch tensor / int
+    device address (`on_device=True`, resident input)."""
+    data: object
+    raster_width: int
+    raster_height: int
+    cpp: int = 1
+    window: Optional[Rect] = None
+    origin: tuple = (0, 0)
+    on_device: bool = False
+    linearization: object = None
+
+    @staticmethod
+    def from_u16(data, width: int, height: int, cpp: int = 1) -> "GpuFrame":
+        on_dev = not isinstance(data, np.ndarray)
+        return GpuFrame(data, width, height, cpp, Rect.full(width, height), (0, 0), on_dev)
+
+    def with_window(self, r: Rect) -> "GpuFrame":
+        w = self.window or Rect.full(self.raster_width, self.raster_height)
+        self.origin = (self.origin[0] + r.x - w.x, self.origin[1] + r.y - w.y)
+        self.window = r
+        return self
+
+    def with_origin(self, origin) -> "GpuFrame":
+        self.origin = tuple(origin)
+        return self
+
+    def _c(self):
+        f = _Frame()
+        f.size = ctypes.sizeof(_Frame)
+        f.input = 0
+        keep = []
+        n = self.raster_width * self.raster_height * self.cpp
+        if self.on_device:
+            f.data = _device_ptr(self.data)
+            f.data_on_device = 1
+            f.data_len = n
+        else:
+            a = np.ascontiguousarray(self.data, dtype=np.uint16)
+            keep.append(a)
+            f.data = a.ctypes.data
+            f.data_len = a.size
+        f.raster_width, f.raster_height, f.cpp = self.raster_width, self.raster_height, self.cpp
+        w = self.window or Rect.full(self.raster_width, self.raster_height)
+        f.window = _Rect(w.x, w.y, w.width, w.height)
+        f.origin_x, f.origin_y = self.origin
+        if self.linearization is not None:
+            f.linearization = 1
+        return f, keep
+
+
+@dataclass
+class GpuTimings:
+    """GpuTimings (run.rs:229-239)."""
+    upload_ms: float
+    compute_ms: float
+    kernel_ms: Optional[float]
+    bands: int
+
+
+_FMT_DTYPE = {0: np.uint8, 1: np.float32, 2: np.uint16}
+
+
+class GpuImage:
+    """A device-resident develop result (image.rs:52-306); freed with the object."""
+
+    def __init__(self, handle: int):
+        self._h = handle
+
+    def width(self) -> int:
+        return lib().frh_image_width(self._h)
+
+    def height(self) -> int:
+        return lib().frh_image_height(self._h)
+
+    def channels(self) -> int:
+        return lib().frh_image_channels(self._h)
+
+    def device_ptr(self) -> int:
+        return lib().frh_image_device_ptr(self._h)
+
+    def download(self) -> np.ndarray:
+        """GpuImage::download (image.rs:171-214): (height, width, channels) host array."""
+        L = lib()
+        dt = _FMT_DTYPE[L.frh_image_format(self._h)]
+        out = np.empty((self.height(), self.width(), self.channels()), dtype=dt)
+        check(L.frh_image_download(self._h, out.ctypes.data, out.nbytes))
+        return out
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            try:
+                lib().frh_image_free(self._h)
+            except Exception:
+                pass
+            self._h = None
+
+
+@dataclass
+class GpuRun:
+    """GpuRun (run.rs:243-246); `image2` holds DevelopSpec.output2 when asked for."""
+    image: GpuImage
+    timings: GpuTimings
+    image2: Optional[GpuImage] = None
+
+
+class GpuContext:
+    """GpuContext (context.rs:116-150): one HIP device, its stream, tables and buffer pool."""
+
+    def __init__(self, device: int = 0):
+        h = ctypes.c_void_p()
+        check(lib().frh_context_create(device, ctypes.byref(h)))
+        self._h = h.value
+
+    @property
+    def handle(self) -> int:
+        return self._h
+
+    def info(self) -> str:
+        buf = ctypes.create_string_buffer(256)
+        check(lib().frh_context_info(self._h, buf, 256))
+        return buf.value.decode()
+
+    def override_limits(self, max_binding_bytes: int) -> None:
+        """context.rs:704: cap the bytes one band binds (forces banded runs); 0 resets."""
+        check(lib().frh_set_limits(self._h, max_binding_bytes))
+
+    def run(self, frame: GpuFrame, info: FrameInfo, op) -> GpuRun:
+        """GpuContext::run (run.rs:426) with GpuOp::Develop; blocking (run_blocking, run.rs:1076)."""
+        if not isinstance(op, GpuOp.Develop):
+            raise NotAvailable(9, "only GpuOp.Develop is supported by this path")
+        f, k1 = frame._c()
+        i, k2 = info._c()
+        s, k3 = op.spec._c()
+        out, out2 = ctypes.c_void_p(), ctypes.c_void_p()
+        t = _Timings()
+        check(lib().frh_run_develop(self._h, ctypes.byref(f), ctypes.byref(i), ctypes.byref(s), ctypes.byref(out),
+                                    ctypes.byref(out2), ctypes.byref(t)))
+        del k1, k2, k3
+        timings = GpuTimings(t.upload_ms, t.compute_ms, t.kernel_ms if t.has_kernel_ms else None, t.bands)
+        return GpuRun(GpuImage(out.value), timings, GpuImage(out2.value) if out2.value else None)
+
+    run_blocking = run
+
+    def develop_into(self, frame: GpuFrame, info: FrameInfo, spec: DevelopSpec, dst, dst2=None, stream=None):
+        """Resident develop into caller-owned device buffers (torch tensors or addresses) on
+        `stream` (a torch stream, a raw hipStream_t address, or None for the context stream);
+        returns without synchronising."""
+        f, k1 = frame._c()
+        i, k2 = info._c()
+        s, k3 = spec._c()
+        sp = None
+        if stream is not None:
+            sp = stream if isinstance(stream, int) else stream.cuda_stream
+        n1 = dst.numel() * dst.element_size() if hasattr(dst, "numel") else (1 << 62)
+        n2 = 0 if dst2 is None else (dst2.numel() * dst2.element_size() if hasattr(dst2, "numel") else (1 << 62))
+        check(lib().frh_develop_into(self._h, ctypes.byref(f), ctypes.byref(i), ctypes.byref(s),
+                                     _device_ptr(dst), n1, None if dst2 is None else _device_ptr(dst2), n2, sp))
+        del k1, k2, k3
+
+    def output_size(self, frame: GpuFrame, spec: DevelopSpec, output: DevelopOutput):
+        """(width, height, channels, bytes) of `output` for this frame and spec."""
+        f, k1 = frame._c()
+        s, k3 = spec._c()
+        w, h, c, b = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint64()
+        check(lib().frh_develop_output_size(ctypes.byref(f), ctypes.byref(s), int(output), ctypes.byref(w),
+                                            ctypes.byref(h), ctypes.byref(c), ctypes.byref(b)))
+        return w.value, h.value, c.value, b.value
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            try:
+                lib().frh_context_destroy(self._h)
+            except Exception:
+                pass
+            self._h = None
+
+
+_ctx = None
+_ctx_lock = threading.Lock()
+
+
+def available() -> bool:
+    """flashraw_gpu::available (context.rs:775-842)."""
+    return bool(lib().frh_available())
+
+
+def context() -> GpuContext:
+    """flashraw_gpu::context(): the process-wide context on device 0."""
+    global _ctx
+    with _ctx_lock:
+        if _ctx is None:
+            _ctx = GpuContext(0)
+        return _ctx
+
+
+def srgb8_table() -> np.ndarray:
+    """The 4097-entry sRGB table every develop encodes with."""
+    t = np.zeros(4097, dtype=np.uint8)
+    check(lib().frh_srgb8_table(t.ctypes.data, t.size))
+    return t

@@ -1,0 +1,64 @@
+"""Seeded synthetic Bayer mosaics (flashraw_amd/lib/libfrsynth.so, host code).
+
+The workloads: a 24 MP RGGB 14-bit frame (6048x4024, black 512, white 16383, gain 0.5 e-/DN,
+read sigma 3 DN; 200 edges, 50 discs), its 45 MP BGGR variant with dense stripe patches
+(8256x5504, 500 stripe patches), and the denoise frame with read sigma 12 DN.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libfrsynth.so")
+_lib = None
+
+BAYER = {"RGGB": (0, 1, 1, 2), "BGGR": (2, 1, 1, 0), "GRBG": (1, 0, 2, 1), "GBRG": (1, 2, 0, 1)}
+
+
+def _L():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_PATH):
+            raise ImportError(f"libfrsynth.so not built ({_PATH}); run __graft_entry__.build()")
+        L = ctypes.CDLL(_PATH)
+        L.frs_scene.restype = ctypes.c_int
+        L.frs_scene.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_float,
+                                ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_uint32, ctypes.c_uint32,
+                                ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int]
+        _lib = L
+    return _lib
+
+
+def mosaic(width, height, cfa="RGGB", black=512.0, white=16383.0, gain=0.5, read_sigma=3.0, edges=200, discs=50,
+           stripes=0, seed=0, threads=0) -> np.ndarray:
+    """A (height, width) uint16 Bayer mosaic of the seeded scene."""
+    out = np.empty((height, width), dtype=np.uint16)
+    c = np.asarray(BAYER[cfa] if isinstance(cfa, str) else cfa, dtype=np.uint8)
+    rc = _L().frs_scene(out.ctypes.data, width, height, c.ctypes.data, black, white, gain, read_sigma, edges, discs,
+                        stripes, seed, threads)
+    if rc != 0:
+        raise ValueError("invalid scene parameters")
+    return out
+
+
+def config0(seed=0, **kw):
+    """6048x4024 RGGB, 14-bit, black 512, white 16383, gain 0.5 e-/DN, read sigma 3 DN."""
+    return mosaic(6048, 4024, "RGGB", 512.0, 16383.0, 0.5, 3.0, 200, 50, 0, seed, **kw)
+
+
+def config1(seed=1, **kw):
+    """8256x5504 BGGR with 500 stripe patches (periods 2-6 px)."""
+    return mosaic(8256, 5504, "BGGR", 512.0, 16383.0, 0.5, 3.0, 200, 50, 500, seed, **kw)
+
+
+def config2(seed=2, **kw):
+    """The 6048x4024 RGGB frame with read sigma 12 DN (denoise workload)."""
+    return mosaic(6048, 4024, "RGGB", 512.0, 16383.0, 0.5, 12.0, 200, 50, 0, seed, **kw)
+
+
+# Develop parameters of the workloads: WB (2.1, 1, 1.6) and a row-normalised camera->sRGB
+# matrix with a diagonal near 1.6 and off-diagonals in [-0.5, -0.1].
+WB = (2.1, 1.0, 1.6, 1.0)
+MATRIX = ((1.62, -0.38, -0.24), (-0.21, 1.58, -0.37), (-0.12, -0.45, 1.57))

@@ -1,0 +1,186 @@
+/*
+ * flashraw_hip.h: C ABI of the MI355X (gfx950) develop path.
+ *
+ * Drop-in boundary for the reference's GPU develop entry points (crates/flashraw-gpu). Every
+ * declaration cites the reference interface it replaces. Conventions follow the reference's C
+ * API (crates/flashraw-capi/src/lib.rs:1-16): `frh_` prefix, int32 enums, structs that start with
+ * their own `size`, a status code per call and a thread-local last error message. No call
+ * panics or throws across the boundary.
+ */
+#ifndef FLASHRAW_HIP_H
+#define FLASHRAW_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes: the numbering of the reference's `flashraw_status`
+ * (crates/flashraw-capi/include/flashraw.h:44-114) for the kinds this path can return. */
+enum frh_status {
+    FRH_OK = 0,
+    FRH_ERROR_GPU_UNAVAILABLE = 6,
+    FRH_ERROR_OUT_OF_MEMORY = 7,
+    FRH_ERROR_LIMIT_EXCEEDED = 8,
+    FRH_ERROR_NOT_AVAILABLE = 9,
+    FRH_ERROR_INVALID_ARGUMENT = 10,
+    FRH_ERROR_INTERNAL = 12,
+    FRH_ERROR_CANCELLED = 14
+};
+
+typedef struct frh_context frh_context;
+typedef struct frh_image frh_image;
+
+/* flashraw_core::Rect */
+typedef struct frh_rect {
+    uint32_t x, y, width, height;
+} frh_rect;
+
+/* GpuInput (crates/flashraw-gpu/src/run.rs:11-24). Packed payloads are not accepted yet. */
+enum frh_input_kind { FRH_INPUT_U16 = 0, FRH_INPUT_F32 = 1 };
+
+/* GpuFrame (run.rs:27-40). `data` holds raster_width * cpp samples per row; when
+ * `data_on_device` is 1 it is a device pointer and nothing is uploaded (resident input). */
+typedef struct frh_frame {
+    uint32_t size;
+    int32_t input;
+    const void *data;
+    uint64_t data_len; /* samples */
+    int32_t data_on_device;
+    uint32_t raster_width, raster_height, cpp;
+    frh_rect window;
+    uint32_t origin_x, origin_y;
+    const uint16_t *linearization; /* must be NULL (not supported yet) */
+    uint32_t linearization_len;
+} frh_frame;
+
+/* The FrameInfo fields the develop path reads (crates/flashraw-core/src/decoder.rs:25-74;
+ * crates/flashraw-gpu/src/kernels.rs:275-345): CFA, black level (BlackLevel, types.rs:357-436)
+ * and white level (WhiteLevel, types.rs:438-467). CFA colours are CfaColor codes
+ * (R 0, G 1, B 2, types.rs:84-102), row-major, at most 8x8. */
+typedef struct frh_frame_info {
+    uint32_t size;
+    uint32_t cfa_width, cfa_height; /* 0: no CFA */
+    uint8_t cfa[64];
+    uint32_t black_rows, black_cols, black_samples;
+    const float *black_values;
+    uint32_t black_values_len;
+    const float *black_row_deltas;
+    uint32_t black_row_deltas_len;
+    const float *black_col_deltas;
+    uint32_t black_col_deltas_len;
+    const uint32_t *white_values;
+    uint32_t white_values_len;
+} frh_frame_info;
+
+/* DevelopOutput (run.rs:96-111). 16-bit outputs are stored as 3 samples per pixel. */
+enum frh_develop_output {
+    FRH_OUTPUT_RGBA8 = 0,
+    FRH_OUTPUT_LINEAR_RGB_F32 = 1,
+    FRH_OUTPUT_RGB16 = 2,
+    FRH_OUTPUT_LINEAR_RGB16 = 3,
+    FRH_OUTPUT_LINEAR_RGB_F16 = 4,
+    FRH_OUTPUT_NONE = -1
+};
+
+/* EdgeSpec (run.rs:141-160). Only the AHD form (`ahd` = 1, period 2, beta 0) is accepted; the
+ * filter tables of the edge-aware form are not used by it and may be NULL. */
+typedef struct frh_edge_spec {
+    uint32_t size;
+    uint32_t period, cx, cy;
+    float beta;
+    uint32_t halo;
+    const uint8_t *canon;
+    uint32_t canon_len;
+    const float *g0;
+    uint32_t g0_len;
+    const float *dir;
+    uint32_t dir_len;
+    int32_t ahd;
+} frh_edge_spec;
+
+/* DevelopSpec (run.rs:115-135). `output2` (FRH_OUTPUT_NONE for none) asks the same pass to
+ * store a second encoding, e.g. linear 16-bit and RGBA8 together. `curve` and `tone` must be
+ * NULL (sRGB curve, no tone). */
+typedef struct frh_develop_spec {
+    uint32_t size;
+    frh_rect crop;
+    int32_t half_size;
+    float wb[4];
+    float cam_to_rgb[9];
+    int32_t orientation; /* EXIF 1..8 */
+    int32_t output;
+    int32_t output2;
+    int32_t has_levels;
+    float levels[2]; /* scale, offset */
+    const frh_edge_spec *edge;
+    const void *curve;
+    const void *tone;
+} frh_develop_spec;
+
+/* GpuTimings (run.rs:229-239). */
+typedef struct frh_timings {
+    double upload_ms;
+    double compute_ms;
+    double kernel_ms;
+    int32_t has_kernel_ms;
+    uint32_t bands;
+} frh_timings;
+
+/* ---- context (crates/flashraw-gpu/src/context.rs:775-842) ---- */
+
+/* flashraw_gpu::available() */
+int32_t frh_available(void);
+/* flashraw_gpu::init() / context(): a context on HIP device `device`. */
+int32_t frh_context_create(int32_t device, frh_context **out);
+void frh_context_destroy(frh_context *ctx);
+/* GpuContext::info() / adapter_info(): a description of the device into `buf`. */
+int32_t frh_context_info(const frh_context *ctx, char *buf, size_t len);
+/* GpuContext::override_limits (context.rs:704): caps the bytes one band may bind, forcing
+ * banded runs (0 restores the default, which never bands on this device). */
+int32_t frh_set_limits(frh_context *ctx, uint64_t max_binding_bytes);
+/* The message of the calling thread's last error (empty when none). */
+const char *frh_last_error_message(void);
+
+/* ---- develop (run.rs:426 GpuContext::run with GpuOp::Develop; run_blocking run.rs:1076) ---- */
+
+/* Upload `frame` (unless resident), develop it and return the device-resident image. */
+int32_t frh_run_develop(frh_context *ctx, const frh_frame *frame, const frh_frame_info *info,
+                        const frh_develop_spec *spec, frh_image **out, frh_image **out2, frh_timings *timings);
+
+/* The same develop into caller-owned device buffers on `stream` (a hipStream_t; NULL: the
+ * context's stream), without synchronising: the resident form of GpuContext::run. `frame->data`
+ * must be a device pointer. */
+int32_t frh_develop_into(frh_context *ctx, const frh_frame *frame, const frh_frame_info *info,
+                         const frh_develop_spec *spec, void *dst, uint64_t dst_bytes, void *dst2,
+                         uint64_t dst2_bytes, void *stream);
+
+/* Output geometry of a develop: width, height (after orientation), channels and bytes. */
+int32_t frh_develop_output_size(const frh_frame *frame, const frh_develop_spec *spec, int32_t output,
+                                uint32_t *width, uint32_t *height, uint32_t *channels, uint64_t *bytes);
+
+/* ---- images (crates/flashraw-gpu/src/image.rs:52-306) ---- */
+
+enum frh_pixel_format { FRH_FORMAT_RGBA8 = 0, FRH_FORMAT_F32 = 1, FRH_FORMAT_U16 = 2 };
+
+uint32_t frh_image_width(const frh_image *img);
+uint32_t frh_image_height(const frh_image *img);
+uint32_t frh_image_channels(const frh_image *img);
+int32_t frh_image_format(const frh_image *img);
+const void *frh_image_device_ptr(const frh_image *img);
+uint64_t frh_image_bytes(const frh_image *img);
+/* GpuImage::download (image.rs:171-214): copy the pixels to host memory. */
+int32_t frh_image_download(const frh_image *img, void *host, uint64_t bytes);
+void frh_image_free(frh_image *img);
+
+/* ---- tables shared with every develop (kernels.rs:348-371) ---- */
+
+/* The 4097-entry sRGB 8-bit table the kernels encode with (process.rs:1903-1915). */
+int32_t frh_srgb8_table(uint8_t *out, size_t len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,88 @@
+/*
+ * CPU oracle for the fused RAW develop path and the CFA-plane DCT block denoiser.
+ *
+ * TEST INFRASTRUCTURE ONLY. This library restates, operation for operation, the reference's
+ * CPU develop path (crates/flashraw/src/process.rs, crates/flashraw/src/ahd.rs) so that the
+ * HIP kernels can be checked against it, and it doubles as the CPU baseline that bench.py
+ * times. Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.
+ * The product library (flashraw_amd/lib/libflashraw_hip.so) never links or calls it.
+ */
+#ifndef FLASHRAW_ORACLE_H
+#define FLASHRAW_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* A single-component CFA mosaic with its calibration (the subset of the reference's RawImage +
+ * FrameInfo the develop path reads: crates/flashraw-core/src/decoder.rs:25-74). */
+typedef struct fro_frame {
+    const uint16_t *u16;      /* width*height samples, row-major */
+    uint32_t width, height;
+    uint32_t origin_x, origin_y; /* sensor position of sample (0,0): anchors CFA and black */
+    uint32_t cfa_w, cfa_h;    /* 1..8 */
+    uint8_t cfa[64];          /* CfaColor codes (0 R, 1 G, 2 B, others undeveloped), [y*cfa_w+x] */
+    uint32_t black_rows, black_cols, black_samples;
+    const float *black_values;
+    uint32_t n_black_values;
+    const float *row_deltas;
+    uint32_t n_row_deltas;
+    const float *col_deltas;
+    uint32_t n_col_deltas;
+    const uint32_t *white_values;
+    uint32_t n_white;
+} fro_frame;
+
+enum fro_demosaic { FRO_BILINEAR = 0, FRO_AHD = 1 };
+
+/* Encodings (crates/flashraw/src/process.rs:770-781), plus 8-bit RGB without alpha. */
+enum fro_encoding {
+    FRO_RGBA8 = 0,
+    FRO_SRGB16 = 1,
+    FRO_LINEAR16 = 2,
+    FRO_LINEAR_F16 = 3,
+    FRO_LINEAR_F32 = 4,
+    FRO_RGB8 = 5
+};
+
+typedef struct fro_params {
+    int32_t demosaic;
+    uint32_t crop_x, crop_y, crop_w, crop_h;
+    float wb[4];
+    float m[9];              /* camera -> linear sRGB, row-major */
+    int32_t has_levels;
+    float levels_scale, levels_offset;
+    int32_t orientation;     /* EXIF 1..8, applied to RGBA8 / SRGB16 / RGB8 only */
+    int32_t encoding;
+} fro_params;
+
+/* Fused develop of `f` with `p` into `out` (out_w*out_h*channels samples of the encoding's
+ * type). Returns 0, or -1 for an invalid request. `threads` <= 0 uses all OpenMP threads. */
+int fro_develop(const fro_frame *f, const fro_params *p, void *out, uint64_t out_bytes, int threads,
+                uint32_t *out_w, uint32_t *out_h);
+
+/* The 4097-entry sRGB 8-bit table (process.rs:1903-1915). */
+void fro_srgb8_table(uint8_t out[4097]);
+/* The 16385-entry sRGB 16-bit table (process.rs:1922-1934). */
+void fro_srgb16_table(float out[16385]);
+/* IEEE half bits of x (process.rs:1978-1991). */
+uint16_t fro_f32_to_f16(float x);
+
+/* ---- CFA-plane DCT block denoiser (build-defined workload, no reference counterpart) ----
+ * The mosaic is split into its four 2x2 phase planes (W/2 x H/2 each, black subtracted, in DN);
+ * 8x8 blocks at stride `step` (positions include the last one) are transformed with the
+ * orthonormal DCT-II, AC coefficients with |c| < thr * sigma are zeroed, where
+ * sigma^2 = sig_a * max(DC / 8, 0) + sig_b, the blocks are transformed back and overlap-added
+ * with uniform weights. `out_f32` (may be NULL) receives the denoised planes re-interleaved as a
+ * mosaic in DN above black; `out_u16` (may be NULL) receives the mosaic with black added back,
+ * rounded and clamped to [0, 65535]. `precision` 64 computes in double, 32 in float. */
+int fro_dct_denoise(const uint16_t *mosaic, uint32_t width, uint32_t height, const float black[4],
+                    float sig_a, float sig_b, float thr, uint32_t step, int precision, float *out_f32,
+                    uint16_t *out_u16, int threads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

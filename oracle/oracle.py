@@ -1,0 +1,194 @@
+"""ctypes bindings of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: the oracle restates the reference's CPU develop path
+(crates/flashraw/src/process.rs, crates/flashraw/src/ahd.rs) and is the checker the HIP path is
+compared against, and the CPU baseline bench.py times. Only tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg may import this module; the product package never does.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+# CfaColor codes (crates/flashraw-core/src/types.rs:84-102)
+R, G, B = 0, 1, 2
+PATTERNS = {
+    "RGGB": [[R, G], [G, B]],
+    "BGGR": [[B, G], [G, R]],
+    "GRBG": [[G, R], [B, G]],
+    "GBRG": [[G, B], [R, G]],
+}
+
+DEMOSAIC = {"bilinear": 0, "ahd": 1}
+ENCODING = {"rgba8": 0, "srgb16": 1, "linear16": 2, "linear_f16": 3, "linear_f32": 4, "rgb8": 5}
+_DTYPE = {0: np.uint8, 1: np.uint16, 2: np.uint16, 3: np.uint16, 4: np.float32, 5: np.uint8}
+
+
+class FroFrame(ctypes.Structure):
+    _fields_ = [
+        ("u16", ctypes.c_void_p),
+        ("width", ctypes.c_uint32),
+        ("height", ctypes.c_uint32),
+        ("origin_x", ctypes.c_uint32),
+        ("origin_y", ctypes.c_uint32),
+        ("cfa_w", ctypes.c_uint32),
+        ("cfa_h", ctypes.c_uint32),
+        ("cfa", ctypes.c_uint8 * 64),
+        ("black_rows", ctypes.c_uint32),
+        ("black_cols", ctypes.c_uint32),
+        ("black_samples", ctypes.c_uint32),
+        ("black_values", ctypes.c_void_p),
+        ("n_black_values", ctypes.c_uint32),
+        ("row_deltas", ctypes.c_void_p),
+        ("n_row_deltas", ctypes.c_uint32),
+        ("col_deltas", ctypes.c_void_p),
+        ("n_col_deltas", ctypes.c_uint32),
+        ("white_values", ctypes.c_void_p),
+        ("n_white", ctypes.c_uint32),
+    ]
+
+
+class FroParams(ctypes.Structure):
+    _fields_ = [
+        ("demosaic", ctypes.c_int32),
+        ("crop_x", ctypes.c_uint32),
+        ("crop_y", ctypes.c_uint32),
+        ("crop_w", ctypes.c_uint32),
+        ("crop_h", ctypes.c_uint32),
+        ("wb", ctypes.c_float * 4),
+        ("m", ctypes.c_float * 9),
+        ("has_levels", ctypes.c_int32),
+        ("levels_scale", ctypes.c_float),
+        ("levels_offset", ctypes.c_float),
+        ("orientation", ctypes.c_int32),
+        ("encoding", ctypes.c_int32),
+    ]
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"oracle library missing: {path} (run __graft_entry__.build())")
+        L = ctypes.CDLL(path)
+        L.fro_develop.restype = ctypes.c_int
+        L.fro_develop.argtypes = [ctypes.POINTER(FroFrame), ctypes.POINTER(FroParams), ctypes.c_void_p,
+                                  ctypes.c_uint64, ctypes.c_int, ctypes.POINTER(ctypes.c_uint32),
+                                  ctypes.POINTER(ctypes.c_uint32)]
+        L.fro_srgb8_table.argtypes = [ctypes.c_void_p]
+        L.fro_srgb16_table.argtypes = [ctypes.c_void_p]
+        L.fro_f32_to_f16.restype = ctypes.c_uint16
+        L.fro_f32_to_f16.argtypes = [ctypes.c_float]
+        L.fro_dct_denoise.restype = ctypes.c_int
+        L.fro_dct_denoise.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p,
+                                      ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_uint32,
+                                      ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
+        _LIB = L
+    return _LIB
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+def make_frame(mosaic, cfa, origin=(0, 0), black=(1, 1, 1, [0.0]), white=(16383,),
+               row_deltas=None, col_deltas=None):
+    """Build the frame descriptor. `black` is (rows, cols, samples, values); the arrays that
+    the descriptor points to are returned alongside it and must stay alive while it is used."""
+    mosaic = np.ascontiguousarray(mosaic, dtype=np.uint16)
+    if isinstance(cfa, str):
+        cfa = PATTERNS[cfa]
+    cfa = np.asarray(cfa, dtype=np.uint8)
+    f = FroFrame()
+    f.u16 = mosaic.ctypes.data
+    f.height, f.width = mosaic.shape
+    f.origin_x, f.origin_y = origin
+    f.cfa_h, f.cfa_w = cfa.shape
+    for i, v in enumerate(cfa.reshape(-1)):
+        f.cfa[i] = int(v)
+    rows, cols, samples, values = black
+    bv = np.ascontiguousarray(values, dtype=np.float32)
+    f.black_rows, f.black_cols, f.black_samples = rows, cols, samples
+    f.black_values, f.n_black_values = _ptr(bv), bv.size
+    keep = [mosaic, bv]
+    if row_deltas is not None:
+        rd = np.ascontiguousarray(row_deltas, dtype=np.float32)
+        f.row_deltas, f.n_row_deltas = _ptr(rd), rd.size
+        keep.append(rd)
+    if col_deltas is not None:
+        cd = np.ascontiguousarray(col_deltas, dtype=np.float32)
+        f.col_deltas, f.n_col_deltas = _ptr(cd), cd.size
+        keep.append(cd)
+    wv = np.ascontiguousarray(white, dtype=np.uint32)
+    f.white_values, f.n_white = _ptr(wv), wv.size
+    keep.append(wv)
+    return f, keep
+
+
+def develop(mosaic, cfa, *, origin=(0, 0), black=(1, 1, 1, [0.0]), white=(16383,), wb=(1, 1, 1, 1),
+            matrix=((1, 0, 0), (0, 1, 0), (0, 0, 1)), demosaic="bilinear", encoding="rgba8",
+            orientation=1, crop=None, levels=None, row_deltas=None, col_deltas=None, threads=0):
+    """Fused develop of a single-component CFA mosaic (process.rs:1475-1591). Returns an
+    (out_h, out_w, channels) array of the encoding's sample type."""
+    f, keep = make_frame(mosaic, cfa, origin, black, white, row_deltas, col_deltas)
+    h, w = np.asarray(mosaic).shape
+    p = FroParams()
+    p.demosaic = DEMOSAIC[demosaic]
+    cx, cy, cw, ch = crop if crop is not None else (0, 0, w, h)
+    p.crop_x, p.crop_y, p.crop_w, p.crop_h = cx, cy, cw, ch
+    for i in range(4):
+        p.wb[i] = float(wb[i] if i < len(wb) else 1.0)
+    mm = np.asarray(matrix, dtype=np.float32).reshape(-1)
+    for i in range(9):
+        p.m[i] = float(mm[i])
+    if levels is not None:
+        p.has_levels, p.levels_scale, p.levels_offset = 1, levels[0], levels[1]
+    p.orientation = orientation
+    p.encoding = ENCODING[encoding]
+    chans = 4 if p.encoding == 0 else 3
+    cw = min(cw, w - min(cx, w))
+    ch = min(ch, h - min(cy, h))
+    out = np.zeros(max(cw * ch * chans, 1), dtype=_DTYPE[p.encoding])
+    ow, oh = ctypes.c_uint32(), ctypes.c_uint32()
+    rc = lib().fro_develop(ctypes.byref(f), ctypes.byref(p), out.ctypes.data, out.nbytes, threads,
+                           ctypes.byref(ow), ctypes.byref(oh))
+    del keep
+    if rc != 0:
+        raise ValueError("oracle develop rejected the request")
+    return out[: oh.value * ow.value * chans].reshape(oh.value, ow.value, chans)
+
+
+def srgb8_table():
+    t = np.zeros(4097, dtype=np.uint8)
+    lib().fro_srgb8_table(t.ctypes.data)
+    return t
+
+
+def srgb16_table():
+    t = np.zeros(16385, dtype=np.float32)
+    lib().fro_srgb16_table(t.ctypes.data)
+    return t
+
+
+def f32_to_f16(x):
+    return lib().fro_f32_to_f16(float(x))
+
+
+def dct_denoise(mosaic, black, sig_a, sig_b, thr=2.7, step=4, precision=64, threads=0):
+    """CFA-plane DCT block denoise; returns (f32 mosaic in DN above black, u16 mosaic)."""
+    mosaic = np.ascontiguousarray(mosaic, dtype=np.uint16)
+    h, w = mosaic.shape
+    bl = np.ascontiguousarray(black, dtype=np.float32)
+    of = np.zeros((h, w), dtype=np.float32)
+    ou = np.zeros((h, w), dtype=np.uint16)
+    rc = lib().fro_dct_denoise(mosaic.ctypes.data, w, h, bl.ctypes.data, sig_a, sig_b, thr, step, precision,
+                               of.ctypes.data, ou.ctypes.data, threads)
+    if rc != 0:
+        raise ValueError("oracle denoise rejected the request")
+    return of, ou

@@ -1,0 +1,194 @@
+"""GPU develop parity: the HIP kernels through the C ABI against the CPU oracle.
+
+The bar is bit-exactness for every encoding (the kernels perform the oracle's IEEE f32
+operations in the same order), across the four Bayer phases, sensor origins, crops, windows,
+black patterns with row/column deltas, levels, all eight orientations, dual outputs and banded
+runs, plus the full-size workloads.
+"""
+import numpy as np
+import pytest
+
+import flashraw_amd as F
+from flashraw_amd import synth
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+ENC = {
+    F.DevelopOutput.Rgba8: "rgba8",
+    F.DevelopOutput.LinearRgbF32: "linear_f32",
+    F.DevelopOutput.Rgb16: "srgb16",
+    F.DevelopOutput.LinearRgb16: "linear16",
+    F.DevelopOutput.LinearRgbF16: "linear_f16",
+}
+MATRIX = ((1.5, -0.3, -0.2), (-0.2, 1.4, -0.2), (0.05, -0.4, 1.35))
+
+
+def _info(w, h, cfa, black=(1, 1, 1, [512.0]), white=(16383,), row_deltas=(), col_deltas=()):
+    rows, cols, samples, values = black
+    return F.FrameInfo(w, h, F.CfaPattern.bayer(cfa),
+                       F.BlackLevel(rows, cols, samples, tuple(values), tuple(row_deltas), tuple(col_deltas)),
+                       F.WhiteLevel(tuple(white)))
+
+
+def _compare(ctx, mos, cfa, *, demosaic="bilinear", output=F.DevelopOutput.Rgba8, origin=(0, 0), window=None,
+             crop=None, black=(1, 1, 1, [512.0]), white=(16383,), wb=(2.1, 1.0, 1.6, 1.0), matrix=MATRIX,
+             orientation=1, levels=None, row_deltas=None, col_deltas=None, output2=None, what=""):
+    H, W = mos.shape
+    win = window or F.Rect(0, 0, W, H)
+    sub = mos[win.y:win.y + win.height, win.x:win.x + win.width]
+    sub_origin = (origin[0] + win.x, origin[1] + win.y)
+    info = _info(win.width, win.height, cfa, black, white, row_deltas or (), col_deltas or ())
+    crop_r = F.Rect(*crop) if crop else F.Rect.full(win.width, win.height)
+    edge = F.ahd_spec(info, sub_origin) if demosaic == "ahd" else None
+    spec = F.DevelopSpec(crop_r, wb=wb, cam_to_rgb=matrix, orientation=F.Orientation(orientation), output=output,
+                         levels=levels, edge=edge, output2=output2)
+    frame = F.GpuFrame.from_u16(mos, W, H).with_origin(origin).with_window(win)
+    run = ctx.run(frame, info, F.GpuOp.Develop(spec))
+    outs = [(output, run.image)] + ([(output2, run.image2)] if output2 is not None else [])
+    for out, img in outs:
+        got = img.download()
+        want = O.develop(sub, cfa, origin=sub_origin, black=black, white=white, wb=wb, matrix=matrix,
+                         demosaic=demosaic, encoding=ENC[out], orientation=orientation,
+                         crop=crop, levels=levels, row_deltas=row_deltas, col_deltas=col_deltas)
+        assert got.shape == want.shape, (what, out, got.shape, want.shape)
+        if out == F.DevelopOutput.LinearRgbF32:
+            bad = np.count_nonzero(got != want)
+        else:
+            bad = np.count_nonzero(got.astype(np.int64) != want.astype(np.int64))
+        if bad:
+            d = np.abs(got.astype(np.float64) - want.astype(np.float64))
+            idx = np.unravel_index(np.argmax(d), d.shape)
+            raise AssertionError(f"{what} {out.name}: {bad} samples differ, max {d.max()} at {idx}")
+    return run
+
+
+@pytest.mark.parametrize("cfa", ["RGGB", "BGGR", "GRBG", "GBRG"])
+@pytest.mark.parametrize("origin", [(0, 0), (1, 0), (0, 1), (3, 5)])
+@pytest.mark.parametrize("demosaic", ["bilinear", "ahd"])
+def test_phases_and_origins(ctx, cfa, origin, demosaic):
+    mos = synth.mosaic(93, 61, cfa, seed=11, edges=30, discs=10, stripes=8)
+    for out in ENC:
+        _compare(ctx, mos, cfa, demosaic=demosaic, output=out, origin=origin, what=f"{cfa} {origin} {demosaic}")
+
+
+@pytest.mark.parametrize("demosaic", ["bilinear", "ahd"])
+@pytest.mark.parametrize("orientation", range(1, 9))
+def test_orientations(ctx, demosaic, orientation):
+    mos = synth.mosaic(70, 45, "GRBG", seed=3, edges=20, discs=5)
+    for out in (F.DevelopOutput.Rgba8, F.DevelopOutput.Rgb16, F.DevelopOutput.LinearRgb16):
+        _compare(ctx, mos, "GRBG", demosaic=demosaic, output=out, orientation=orientation,
+                 what=f"orientation {orientation}")
+
+
+@pytest.mark.parametrize("demosaic", ["bilinear", "ahd"])
+def test_crop_window_and_calibration(ctx, demosaic):
+    """The calibrated_info frame of the reference's GPU tests (tests/equivalence.rs:166-177):
+    2x2 black pattern, row and column deltas, per-colour white levels."""
+    W, H = 180, 130
+    mos = synth.mosaic(W, H, "BGGR", seed=5, edges=40, discs=10, stripes=10)
+    black = (2, 2, 1, [510.0, 512.0, 509.0, 515.0])
+    white = (16000, 16383, 15800)
+    win = F.Rect(7, 5, 150, 111)
+    rowd = [(y % 5) * 0.5 for y in range(H + 8)]
+    cold = [(x % 3) * -0.25 for x in range(W + 8)]
+    for crop in [None, (0, 0, 150, 111), (13, 9, 101, 77), (3, 2, 140, 100)]:
+        for out in (F.DevelopOutput.Rgba8, F.DevelopOutput.LinearRgbF32):
+            _compare(ctx, mos, "BGGR", demosaic=demosaic, output=out, origin=(1, 0), window=win, crop=crop,
+                     black=black, white=white, row_deltas=rowd, col_deltas=cold, what=f"crop {crop}")
+
+
+@pytest.mark.parametrize("demosaic", ["bilinear", "ahd"])
+def test_levels_and_dual_output(ctx, demosaic):
+    mos = synth.mosaic(120, 80, "RGGB", seed=9, stripes=20)
+    _compare(ctx, mos, "RGGB", demosaic=demosaic, output=F.DevelopOutput.LinearRgb16, levels=(1.37, 0.02),
+             output2=F.DevelopOutput.Rgba8, what="levels + dual")
+
+
+@pytest.mark.parametrize("demosaic", ["bilinear", "ahd"])
+@pytest.mark.parametrize("size", [(1, 1), (2, 2), (3, 5), (7, 4), (12, 12), (13, 13), (14, 17), (40, 3)])
+def test_tiny_frames(ctx, demosaic, size):
+    """Frames with no or few AHD interior pixels develop bilinearly at the border."""
+    w, h = size
+    mos = synth.mosaic(w, h, "GBRG", seed=w * 31 + h, edges=2, discs=1)
+    _compare(ctx, mos, "GBRG", demosaic=demosaic, output=F.DevelopOutput.LinearRgbF32, what=f"{size}")
+    _compare(ctx, mos, "GBRG", demosaic=demosaic, output=F.DevelopOutput.Rgba8, what=f"{size}")
+
+
+@pytest.mark.parametrize("demosaic", ["bilinear", "ahd"])
+def test_banded_equals_whole(ctx, demosaic):
+    """Forced bands (override_limits, context.rs:704) give the same bytes as one launch."""
+    mos = synth.mosaic(300, 257, "RGGB", seed=4, stripes=30)
+    info = _info(300, 257, "RGGB")
+    spec = F.DevelopSpec(F.Rect.full(300, 257), wb=synth.WB, cam_to_rgb=MATRIX,
+                         edge=F.ahd_spec(info) if demosaic == "ahd" else None)
+    frame = F.GpuFrame.from_u16(mos, 300, 257)
+    whole = ctx.run(frame, info, F.GpuOp.Develop(spec))
+    assert whole.timings.bands == 1
+    ctx.override_limits(40 * 600)
+    try:
+        banded = ctx.run(frame, info, F.GpuOp.Develop(spec))
+    finally:
+        ctx.override_limits(0)
+    assert banded.timings.bands > 1
+    assert np.array_equal(whole.image.download(), banded.image.download())
+
+
+def test_ahd_differs_from_bilinear(ctx):
+    mos = synth.mosaic(160, 120, "RGGB", seed=12, stripes=20)
+    info = _info(160, 120, "RGGB")
+    out = {}
+    for name, edge in (("bilinear", None), ("ahd", F.ahd_spec(info))):
+        spec = F.DevelopSpec(F.Rect.full(160, 120), output=F.DevelopOutput.LinearRgbF32, edge=edge)
+        out[name] = ctx.run(F.GpuFrame.from_u16(mos, 160, 120), info, F.GpuOp.Develop(spec)).image.download()
+    assert np.abs(out["ahd"] - out["bilinear"]).max() > 1e-2
+
+
+def test_resident_develop_into(ctx):
+    import torch
+    w, h = 256, 130
+    mos = synth.mosaic(w, h, "RGGB", seed=21)
+    info = _info(w, h, "RGGB")
+    spec = F.DevelopSpec(F.Rect.full(w, h), wb=synth.WB, cam_to_rgb=MATRIX, edge=F.ahd_spec(info))
+    d_in = torch.from_numpy(mos.view(np.int16)).to("cuda")
+    dst = torch.empty((h, w, 4), dtype=torch.uint8, device="cuda")
+    ctx.develop_into(F.GpuFrame.from_u16(d_in, w, h), info, spec, dst, stream=torch.cuda.current_stream())
+    torch.cuda.synchronize()
+    want = O.develop(mos, "RGGB", black=(1, 1, 1, [512.0]), wb=synth.WB, matrix=MATRIX, demosaic="ahd")
+    assert np.array_equal(dst.cpu().numpy(), want)
+
+
+def test_errors(ctx):
+    mos = synth.mosaic(64, 64, "RGGB", seed=1)
+    info = _info(64, 64, "RGGB")
+    frame = F.GpuFrame.from_u16(mos, 64, 64)
+    with pytest.raises(F.InvalidArgument):
+        ctx.run(F.GpuFrame.from_u16(mos, 64, 64).with_window(F.Rect(10, 10, 64, 64)), info,
+                F.GpuOp.Develop(F.DevelopSpec(F.Rect.full(64, 64))))
+    with pytest.raises(F.NotAvailable):
+        ctx.run(frame, info, F.GpuOp.Develop(F.DevelopSpec(F.Rect.full(64, 64), half_size=True)))
+    xt = F.FrameInfo(64, 64, F.CfaPattern(2, 2, (0, 1, 1, 1)))
+    with pytest.raises(F.NotAvailable):
+        ctx.run(frame, xt, F.GpuOp.Develop(F.DevelopSpec(F.Rect.full(64, 64))))
+    with pytest.raises(F.InvalidArgument):
+        ctx.run(frame, info, F.GpuOp.Develop(F.DevelopSpec(F.Rect(64, 0, 10, 10))))
+    bad = F.EdgeSpec(period=2, beta=0.5, halo=6, ahd=True)
+    with pytest.raises(F.InvalidArgument):
+        ctx.run(frame, info, F.GpuOp.Develop(F.DevelopSpec(F.Rect.full(64, 64), edge=bad)))
+
+
+@pytest.mark.parametrize("demosaic", ["bilinear", "ahd"])
+def test_config0_full_frame(ctx, demosaic):
+    """The 24 MP workload (6048x4024 RGGB 14-bit) against the oracle, every byte."""
+    mos = synth.config0()
+    run = _compare(ctx, mos, "RGGB", demosaic=demosaic, output=F.DevelopOutput.Rgba8, wb=synth.WB,
+                   matrix=synth.MATRIX, what="config0")
+    print(f"config0 {demosaic}: kernel {run.timings.kernel_ms:.3f} ms")
+
+
+def test_config1_full_frame_dual_output(ctx):
+    """The 45 MP BGGR workload with dense stripes: linear 16-bit and RGBA8 from one AHD pass."""
+    mos = synth.config1()
+    run = _compare(ctx, mos, "BGGR", demosaic="ahd", output=F.DevelopOutput.LinearRgb16,
+                   output2=F.DevelopOutput.Rgba8, wb=synth.WB, matrix=synth.MATRIX, what="config1")
+    print(f"config1 AHD dual: kernel {run.timings.kernel_ms:.3f} ms")
