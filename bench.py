@@ -83,10 +83,11 @@ def main():
     d_in = torch.from_numpy(mos.view(np.int16)).to("cuda")
     dst = torch.empty((H, W, 4), dtype=torch.uint8, device="cuda")
     frame = F.GpuFrame.from_u16(d_in, W, H)
-    stream = torch.cuda.current_stream()
+    plan = ctx.prepare(frame, info, spec)
+    stream = torch.cuda.Stream()  # the launch stream; the HIP events below are recorded on it
 
     for _ in range(args.warmup):
-        ctx.develop_into(frame, info, spec, dst, stream=stream)
+        plan.launch(d_in, dst, stream=stream)
     torch.cuda.synchronize()
     # Timed region: K back-to-back steps; per-launch HIP events on the launch stream.
     evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
@@ -94,7 +95,7 @@ def main():
     t0 = time.perf_counter()
     for s, e in evs:
         s.record(stream)
-        ctx.develop_into(frame, info, spec, dst, stream=stream)
+        plan.launch(d_in, dst, stream=stream)
         e.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0

@@ -7,6 +7,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <new>
@@ -316,6 +317,7 @@ static int32_t plan_develop(const frh_context *ctx, const frh_frame *f, const fr
     for (int i = 0; i < 4; i++) {
         uint32_t colour = info->cfa[i];  // pattern position (i & 1, i >> 1) of sensor parity
         a.chan[i] = colour < 3 ? colour : 3;
+        a.chan_bits |= a.chan[i] << (2 * i);
     }
     // crop.clamp_to(w, h) (crates/flashraw-core/src/types.rs:68-70)
     uint64_t cx = spec->crop.x, cy = spec->crop.y;
@@ -342,6 +344,12 @@ static int32_t plan_develop(const frh_context *ctx, const frh_frame *f, const fr
     a.lut8 = ctx->d_lut8;
     a.srgb16 = reinterpret_cast<const float2 *>(ctx->d_srgb16);
     p.ahd = false;
+    {
+        const char *impl = getenv("FLASHRAW_BILINEAR_IMPL");
+        a.bil_impl = (impl && strcmp(impl, "tiled") == 0) ? 1u : 0u;
+        const char *rows = getenv("FLASHRAW_BILINEAR_ROWS");
+        a.bil_rows = rows ? (uint32_t)atoi(rows) : ctx->bil_rows;
+    }
     if (spec->edge) {
         const frh_edge_spec *e = spec->edge;
         if (e->size < sizeof(frh_edge_spec)) return fail(FRH_ERROR_INVALID_ARGUMENT, "edge spec size too small");
@@ -362,6 +370,12 @@ static int32_t plan_develop(const frh_context *ctx, const frh_frame *f, const fr
                 }
         }
         a.ahd_halo = e->halo;
+        // Implementation knobs (environment, like the reference's FLASHRAW_* variables): the LDS-tiled
+        // AHD kernel instead of the row walker, and the walker's rows per strip.
+        const char *impl = getenv("FLASHRAW_AHD_IMPL");
+        a.ahd_impl = (impl && strcmp(impl, "tiled") == 0) ? 1u : 0u;
+        const char *rows = getenv("FLASHRAW_AHD_ROWS");
+        a.ahd_rows = rows ? (uint32_t)atoi(rows) : ctx->ahd_rows;
         p.ahd = true;
     }
     int32_t modes[2] = {spec->output, spec->output2};
@@ -447,12 +461,80 @@ extern "C" int32_t frh_develop_into(frh_context *ctx, const frh_frame *frame, co
             return fail(FRH_ERROR_NOT_AVAILABLE, "black-level deltas need frh_run_develop (host tables)");
         p.a.dst = dst;
         p.a.dst2 = dst2;
-        hipSetDevice(ctx->device);
-        return launch_bands(ctx, p, stream ? (hipStream_t)stream : ctx->stream, nullptr);
+        (void)hipSetDevice(ctx->device);
+        return launch_bands(ctx, p, (hipStream_t)stream, nullptr);
     } catch (...) {
         return fail(FRH_ERROR_INTERNAL, "unexpected exception");
     }
 }
+
+// ------------------------------------------------------------------------------------------
+// Prepared plans
+
+struct frh_plan {
+    frh_context *ctx;
+    Plan p;
+    size_t src_off;  // samples from the raster start to the window origin
+    void *d_rowd = nullptr, *d_cold = nullptr;
+    ~frh_plan() {
+        if (d_rowd) (void)hipFree(d_rowd);
+        if (d_cold) (void)hipFree(d_cold);
+    }
+};
+
+extern "C" void *frh_context_stream(const frh_context *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+extern "C" int32_t frh_plan_create(frh_context *ctx, const frh_frame *frame, const frh_frame_info *info,
+                                   const frh_develop_spec *spec, frh_plan **out) {
+    try {
+        if (!ctx || !out) return fail(FRH_ERROR_INVALID_ARGUMENT, "NULL argument");
+        *out = nullptr;
+        std::unique_ptr<frh_plan> pl(new frh_plan());
+        pl->ctx = ctx;
+        int32_t rc = plan_develop(ctx, frame, info, spec, pl->p);
+        if (rc) return rc;
+        pl->src_off = (size_t)frame->window.y * frame->raster_width + frame->window.x;
+        (void)hipSetDevice(ctx->device);
+        hipError_t e;
+        if (pl->p.a.n_rowd) {
+            if ((e = hipMalloc(&pl->d_rowd, pl->p.a.n_rowd * 4)) != hipSuccess) return hip_fail(e, "hipMalloc");
+            if ((e = hipMemcpy(pl->d_rowd, info->black_row_deltas, pl->p.a.n_rowd * 4, hipMemcpyHostToDevice)) !=
+                hipSuccess)
+                return hip_fail(e, "upload");
+            pl->p.a.rowd = static_cast<const float *>(pl->d_rowd);
+        }
+        if (pl->p.a.n_cold) {
+            if ((e = hipMalloc(&pl->d_cold, pl->p.a.n_cold * 4)) != hipSuccess) return hip_fail(e, "hipMalloc");
+            if ((e = hipMemcpy(pl->d_cold, info->black_col_deltas, pl->p.a.n_cold * 4, hipMemcpyHostToDevice)) !=
+                hipSuccess)
+                return hip_fail(e, "upload");
+            pl->p.a.cold = static_cast<const float *>(pl->d_cold);
+        }
+        *out = pl.release();
+        return FRH_OK;
+    } catch (const std::bad_alloc &) {
+        return fail(FRH_ERROR_OUT_OF_MEMORY, "host allocation failed");
+    } catch (...) {
+        return fail(FRH_ERROR_INTERNAL, "unexpected exception");
+    }
+}
+
+extern "C" int32_t frh_plan_launch(const frh_plan *plan, const void *src, void *dst, void *dst2, void *stream) {
+    if (!plan || !src || !dst) return fail(FRH_ERROR_INVALID_ARGUMENT, "NULL argument");
+    if (plan->p.a.out_mode2 != OUT_NONE && !dst2) return fail(FRH_ERROR_INVALID_ARGUMENT, "second output is NULL");
+    Plan p = plan->p;
+    p.a.src = static_cast<const uint16_t *>(src) + plan->src_off;
+    p.a.dst = dst;
+    p.a.dst2 = dst2;
+    return launch_bands(plan->ctx, p, (hipStream_t)stream, nullptr);
+}
+
+extern "C" uint64_t frh_plan_output_bytes(const frh_plan *plan, int32_t which) {
+    if (!plan || which < 0 || which > 1) return 0;
+    return plan->p.bytes[which];
+}
+
+extern "C" void frh_plan_destroy(frh_plan *plan) { delete plan; }
 
 static double ms_since(std::chrono::steady_clock::time_point t0) {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

@@ -5,199 +5,10 @@
 //
 // Both kernels are persistent: a block loads the sRGB table and the calibration pattern into LDS
 // once, then walks tiles `blockIdx.x, blockIdx.x + gridDim.x, ...` of the developed image.
-#include "frh_device.h"
+#include "develop_common.h"
 #include "frh_launch.h"
 
 namespace frh {
-
-// ------------------------------------------------------------------------------------------
-// Shared helpers
-
-struct Lds {
-    float black[64];
-    float scale[64];
-    uint32_t lut[1025];  // 4097 sRGB codes, bytes packed four per word
-};
-
-__device__ __forceinline__ void load_lds(const DevArgs &a, Lds &s) {
-    for (int i = threadIdx.x; i < 64; i += blockDim.x) {
-        s.black[i] = a.black[i];
-        s.scale[i] = a.scale[i];
-    }
-    const uint32_t *l = reinterpret_cast<const uint32_t *>(a.lut8);
-    for (int i = threadIdx.x; i < 1025; i += blockDim.x) s.lut[i] = l[i];
-}
-
-__device__ __forceinline__ uint32_t lut8(const Lds &s, uint32_t i) {
-    return reinterpret_cast<const uint8_t *>(s.lut)[i];
-}
-
-// Channel (0 R, 1 G, 2 B) of frame sample (x, y) of a Bayer mosaic.
-__device__ __forceinline__ uint32_t chan_at(const DevArgs &a, int x, int y) {
-    return a.chan[(((uint32_t)y + a.oy) & 1u) * 2u + (((uint32_t)x + a.ox) & 1u)];
-}
-
-// Calib::norm (process.rs:288-298) of frame sample (x, y), in bounds.
-__device__ __forceinline__ float norm_at(const DevArgs &a, const Lds &s, uint32_t x, uint32_t y) {
-    uint32_t X = x + a.ox, Y = y + a.oy;
-    uint32_t i = a.cal_pow2 ? (Y & (a.cal_ph - 1u)) * a.cal_pw + (X & (a.cal_pw - 1u))
-                            : (Y % a.cal_ph) * a.cal_pw + X % a.cal_pw;
-    float b = s.black[i];
-    if (a.n_rowd) b += Y < a.n_rowd ? a.rowd[Y] : 0.0f;
-    if (a.n_cold) b += X < a.n_cold ? a.cold[X] : 0.0f;
-    float v = (float)a.src[(size_t)y * a.stride + x];
-    return (v - b) * s.scale[i];
-}
-
-// finish_rgb with Highlights::Clip (process.rs:654-707). `balanced` pixels come from the AHD
-// mosaic, which is already white balanced (finish_run's unit balance, process.rs:1209-1213).
-__device__ __forceinline__ void finish(const DevArgs &a, float r, float g, float b, bool balanced, float &o0,
-                                       float &o1, float &o2) {
-    if (!balanced) {
-        r = r * a.wb[0];
-        g = g * a.wb[1];
-        b = b * a.wb[2];
-    }
-    r = rclamp01(r);
-    g = rclamp01(g);
-    b = rclamp01(b);
-    o0 = a.m[0] * r + a.m[1] * g + a.m[2] * b;
-    o1 = a.m[3] * r + a.m[4] * g + a.m[5] * b;
-    o2 = a.m[6] * r + a.m[7] * g + a.m[8] * b;
-    if (a.has_levels) {
-        o0 = o0 * a.lscale - a.loffset;
-        o1 = o1 * a.lscale - a.loffset;
-        o2 = o2 * a.lscale - a.loffset;
-    }
-}
-
-// f32_to_f16 (process.rs:1978-1991).
-__device__ __forceinline__ uint32_t f16_bits(float x) {
-    uint32_t b = __float_as_uint(x);
-    uint32_t sign = (b >> 16) & 0x8000u;
-    uint32_t a = b & 0x7fffffffu;
-    uint32_t normal = (a + 0xc8000fffu + ((a >> 13) & 1u)) >> 13;
-    uint32_t sub = __float_as_uint(__uint_as_float(a) + 0.5f) - 0x3f000000u;
-    uint32_t h = a < 0x38800000u ? sub : normal;
-    h = a >= 0x477ff000u ? (a > 0x7f800000u ? 0x7e00u : 0x7bffu) : h;
-    return sign | h;
-}
-
-// srgb16_paired (process.rs:1960-1967).
-__device__ __forceinline__ uint32_t srgb16(const DevArgs &a, float v) {
-    float x = rclamp01(v) * 16384.0f;
-    uint32_t i = trunc_u32(x);
-    i = i < 16383u ? i : 16383u;
-    float t = x - (float)i;
-    float2 p = a.srgb16[i];
-    return trunc_u32(p.x + p.y * t + 0.5f);
-}
-
-// linear16 (process.rs:1971-1973).
-__device__ __forceinline__ uint32_t lin16(float v) { return trunc_u32(rclamp01(v) * 65535.0f + 0.5f); }
-
-// Destination of developed pixel (dx, dy) under EXIF orientation o (the inverse of
-// encode_developed's source mapping, process.rs:2035-2044).
-__device__ __forceinline__ void orient(int o, uint32_t dx, uint32_t dy, uint32_t w, uint32_t h, uint32_t &px,
-                                       uint32_t &py) {
-    switch (o) {
-    case 2: px = w - 1 - dx; py = dy; break;
-    case 3: px = w - 1 - dx; py = h - 1 - dy; break;
-    case 4: px = dx; py = h - 1 - dy; break;
-    case 5: px = dy; py = dx; break;
-    case 6: px = h - 1 - dy; py = dx; break;
-    case 7: px = h - 1 - dy; py = w - 1 - dx; break;
-    case 8: px = dy; py = w - 1 - dx; break;
-    default: px = dx; py = dy; break;
-    }
-}
-
-// Encode and store linear output pixel (r, g, b) of developed pixel (dx, dy).
-__device__ __forceinline__ void store_px(const DevArgs &a, const Lds &s, int mode, void *dst, uint32_t dx,
-                                         uint32_t dy, float r, float g, float b) {
-    if (mode == OUT_RGBA8) {
-        uint32_t px, py;
-        orient(a.orientation, dx, dy, a.dev_w, a.dev_h, px, py);
-        uint32_t v = lut8(s, lut8_index(r)) | (lut8(s, lut8_index(g)) << 8) | (lut8(s, lut8_index(b)) << 16) |
-                     0xff000000u;
-        reinterpret_cast<uint32_t *>(dst)[(size_t)py * a.dst_w + px] = v;
-    } else if (mode == OUT_LINEAR_F32) {
-        float *p = reinterpret_cast<float *>(dst) + ((size_t)dy * a.dev_w + dx) * 3;
-        p[0] = r;
-        p[1] = g;
-        p[2] = b;
-    } else if (mode == OUT_SRGB16) {
-        uint32_t px, py;
-        orient(a.orientation, dx, dy, a.dev_w, a.dev_h, px, py);
-        uint16_t *p = reinterpret_cast<uint16_t *>(dst) + ((size_t)py * a.dst_w + px) * 3;
-        p[0] = (uint16_t)srgb16(a, r);
-        p[1] = (uint16_t)srgb16(a, g);
-        p[2] = (uint16_t)srgb16(a, b);
-    } else if (mode == OUT_LINEAR16) {
-        uint16_t *p = reinterpret_cast<uint16_t *>(dst) + ((size_t)dy * a.dev_w + dx) * 3;
-        p[0] = (uint16_t)lin16(r);
-        p[1] = (uint16_t)lin16(g);
-        p[2] = (uint16_t)lin16(b);
-    } else if (mode == OUT_LINEAR_F16) {
-        uint16_t *p = reinterpret_cast<uint16_t *>(dst) + ((size_t)dy * a.dev_w + dx) * 3;
-        p[0] = (uint16_t)f16_bits(r);
-        p[1] = (uint16_t)f16_bits(g);
-        p[2] = (uint16_t)f16_bits(b);
-    }
-}
-
-__device__ __forceinline__ void store_both(const DevArgs &a, const Lds &s, uint32_t dx, uint32_t dy, float r,
-                                           float g, float b) {
-    store_px(a, s, a.out_mode, a.dst, dx, dy, r, g, b);
-    if (a.out_mode2 != OUT_NONE) store_px(a, s, a.out_mode2, a.dst2, dx, dy, r, g, b);
-}
-
-// Kernel::row_cols for one pixel of a Bayer mosaic, read straight from the frame
-// (process.rs:1076-1126): the interior formulas, or at the frame border the mean of the
-// in-frame same-colour 3x3 samples in row-major order.
-__device__ void bilinear_global(const DevArgs &a, const Lds &s, int x, int y, float rgb[3]) {
-    const int w = (int)a.width, h = (int)a.height;
-    uint32_t c0 = chan_at(a, x, y);
-    if (x > 0 && y > 0 && x + 1 < w && y + 1 < h) {
-        float own = norm_at(a, s, x, y);
-        float l = norm_at(a, s, x - 1, y), r = norm_at(a, s, x + 1, y);
-        float u = norm_at(a, s, x, y - 1), d = norm_at(a, s, x, y + 1);
-        if (c0 == 1) {
-            uint32_t ch = chan_at(a, x + 1, y);
-            float hv = (l + r) * 0.5f, vv = (u + d) * 0.5f;
-            rgb[1] = own;
-            rgb[0] = ch == 0 ? hv : vv;
-            rgb[2] = ch == 0 ? vv : hv;
-        } else {
-            float orth = (l + r + u + d) * 0.25f;
-            float diag = (norm_at(a, s, x - 1, y - 1) + norm_at(a, s, x + 1, y - 1) + norm_at(a, s, x - 1, y + 1) +
-                          norm_at(a, s, x + 1, y + 1)) *
-                         0.25f;
-            rgb[0] = c0 == 0 ? own : diag;
-            rgb[1] = orth;
-            rgb[2] = c0 == 0 ? diag : own;
-        }
-        return;
-    }
-    for (uint32_t c = 0; c < 3; c++) {
-        float sum = 0.0f, n = 0.0f;
-        if (c == c0) {
-            sum = norm_at(a, s, x, y);
-            n = 1.0f;
-        } else {
-            for (int dy = -1; dy <= 1; dy++) {
-                for (int dx = -1; dx <= 1; dx++) {
-                    int xx = x + dx, yy = y + dy;
-                    if (chan_at(a, xx, yy) != c) continue;
-                    if (xx < 0 || xx >= w || yy < 0 || yy >= h) continue;
-                    sum += norm_at(a, s, xx, yy);
-                    n += 1.0f;
-                }
-            }
-        }
-        rgb[c] = n > 0.0f ? sum / n : 0.0f;
-    }
-}
 
 // ------------------------------------------------------------------------------------------
 // Bilinear develop (the reference's `develop` kernel, develop.wgsl:185-260; CPU
@@ -318,7 +129,7 @@ __global__ __launch_bounds__(256) void k_develop_ahd(DevArgs a, uint32_t tiles_x
                 int r = i / MW, c = i - r * MW;
                 int x = x0 - 5 + c, y = y0 - 5 + r;
                 float v = 0.0f;
-                if (x >= 0 && x < w && y >= 0 && y < h) v = fminf(norm_at(a, s, x, y) * a.wb[chan_at(a, x, y)], 1.0f);
+                if (x >= 0 && x < w && y >= 0 && y < h) v = fminf(norm_at(a, s, x, y) * wb_of(a, chan_at(a, x, y)), 1.0f);
                 L.m[r][c] = v;
             }
             __syncthreads();
@@ -422,6 +233,172 @@ __global__ __launch_bounds__(256) void k_develop_ahd(DevArgs a, uint32_t tiles_x
 }
 
 // ------------------------------------------------------------------------------------------
+// Bilinear develop, register row walker: lane j owns frame columns xs + 2j and xs + 2j + 1, the
+// wave walks down a strip keeping the three normalised rows the 3x3 neighbourhood needs in
+// registers, horizontal neighbours come from the adjacent lanes (DPP). With the Bayer phase a
+// template parameter and rows unrolled by two, every pixel evaluates only its own formulas
+// (process.rs:1076-1096). Lanes 1..62 hold results (124 columns per strip); pixels on the frame
+// border are developed by extra blocks with the in-frame neighbour means (process.rs:1109-1126).
+
+constexpr int BWALK_COLS = 124;
+constexpr int BWALK_WAVES = 4;
+
+struct BWalk {
+    float n[3][2];  // normalised rows R-2, R-1, R; [k] even / odd column
+    float nl[3];    // odd column of lane - 1
+    float nr[3];    // even column of lane + 1
+};
+
+// Two normalised samples (x, y), (x + 1, y) of a lane, 0 outside the frame.
+__device__ __forceinline__ void load_pair(const DevArgs &a, const Lds &s, int x, int y, float &v0, float &v1) {
+    v0 = v1 = 0.0f;
+    if (y < 0 || y >= (int)a.height) return;
+    if (x >= 0 && x < (int)a.width) v0 = norm_at(a, s, (uint32_t)x, (uint32_t)y);
+    if (x + 1 >= 0 && x + 1 < (int)a.width) v1 = norm_at(a, s, (uint32_t)x + 1, (uint32_t)y);
+}
+
+template <int RP, int PY, bool FAST>
+__device__ __forceinline__ void bwalk_row(const DevArgs &a, const Lds &s, BWalk &W, int R, int xs, int lane,
+                                          int out_row0, int out_row1, int out_x0, int out_x1) {
+    const int x0 = xs + 2 * lane;
+#pragma unroll
+    for (int r = 0; r < 2; r++) {
+        W.n[r][0] = W.n[r + 1][0];
+        W.n[r][1] = W.n[r + 1][1];
+        W.nl[r] = W.nl[r + 1];
+        W.nr[r] = W.nr[r + 1];
+    }
+    load_pair(a, s, x0, R, W.n[2][0], W.n[2][1]);
+    W.nl[2] = from_left(W.n[2][1]);
+    W.nr[2] = from_right(W.n[2][0]);
+    const int ro = R - 1;
+    if (ro < out_row0 || ro >= out_row1 || lane < 1 || lane > 62) return;
+    constexpr int PO = PY ^ 1;
+    float o[2][3];
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        const float own = W.n[1][k];
+        const float l = k == 0 ? W.nl[1] : W.n[1][0], r = k == 0 ? W.n[1][1] : W.nr[1];
+        const float u = W.n[0][k], d = W.n[2][k];
+        float rgb[3];
+        const int col = site<RP>(PO, k);
+        if (col == 1) {
+            const int ch = site<RP>(PO, k ^ 1);  // the colour of this row's other sites
+            const float hv = (l + r) * 0.5f, vv = (u + d) * 0.5f;
+            rgb[0] = ch == 0 ? hv : vv;
+            rgb[1] = own;
+            rgb[2] = ch == 0 ? vv : hv;
+        } else {
+            const float ul = k == 0 ? W.nl[0] : W.n[0][0], ur = k == 0 ? W.n[0][1] : W.nr[0];
+            const float dl = k == 0 ? W.nl[2] : W.n[2][0], dr = k == 0 ? W.n[2][1] : W.nr[2];
+            const float orth = (l + r + u + d) * 0.25f;
+            const float diag = (ul + ur + dl + dr) * 0.25f;
+            rgb[0] = col == 0 ? own : diag;
+            rgb[1] = orth;
+            rgb[2] = col == 0 ? diag : own;
+        }
+        finish(a, rgb[0], rgb[1], rgb[2], false, o[k][0], o[k][1], o[k][2]);
+    }
+    const int dx = x0 - (int)a.crop_x, dy = ro - (int)a.crop_y;
+    const bool in0 = x0 >= out_x0 && x0 < out_x1, in1 = x0 + 1 >= out_x0 && x0 + 1 < out_x1;
+    if (FAST) {
+        const uint32_t w0 = rgba8_word(s, o[0][0], o[0][1], o[0][2]);
+        const uint32_t w1 = rgba8_word(s, o[1][0], o[1][1], o[1][2]);
+        if (in0 && in1) {
+            store_rgba8_pair(a, (uint32_t)dx, (uint32_t)dy, w0, w1, 2);
+        } else if (in0) {
+            store_rgba8_pair(a, (uint32_t)dx, (uint32_t)dy, w0, 0, 1);
+        } else if (in1) {
+            store_rgba8_pair(a, (uint32_t)dx + 1, (uint32_t)dy, w1, 0, 1);
+        }
+    } else {
+        if (in0) store_both(a, s, (uint32_t)dx, (uint32_t)dy, o[0][0], o[0][1], o[0][2]);
+        if (in1) store_both(a, s, (uint32_t)dx + 1, (uint32_t)dy, o[1][0], o[1][1], o[1][2]);
+    }
+}
+
+struct BWalkPlan {
+    int x_first, x_end, y_first, y_end;
+    int xs0, ys0;
+    int strips, chunks, rows;
+    Border border;
+};
+
+template <int RP, bool FAST>
+__global__ __launch_bounds__(BWALK_WAVES * 64) void k_develop_bilinear_walk(DevArgs a, BWalkPlan P,
+                                                                            uint32_t walk_blocks) {
+    __shared__ Lds s;
+    load_lds(a, s);
+    __syncthreads();
+    if (blockIdx.x >= walk_blocks) {
+        const uint32_t i = (blockIdx.x - walk_blocks) * blockDim.x + threadIdx.x;
+        if (i < P.border.count) border_pixel(a, s, P.border, i);
+        return;
+    }
+    const int lane = threadIdx.x & 63;
+    const int item = (int)(blockIdx.x * BWALK_WAVES + (threadIdx.x >> 6));
+    if (item >= P.strips * P.chunks) return;
+    const int sx = item % P.strips, sy = item / P.strips;
+    const int xs = P.xs0 + BWALK_COLS * sx - 2;  // lane 1 holds output column xs + 2
+    const int row0 = P.ys0 + P.rows * sy;
+    const int out_row0 = row0 > P.y_first ? row0 : P.y_first;
+    const int out_row1 = row0 + P.rows < P.y_end ? row0 + P.rows : P.y_end;
+    const int out_x0 = max(xs + 2, P.x_first), out_x1 = min(xs + 2 + BWALK_COLS, P.x_end);
+    BWalk W;
+    {
+        float *f = reinterpret_cast<float *>(&W);
+        for (int i = 0; i < (int)(sizeof(BWalk) / 4); i++) f[i] = 0.0f;
+    }
+    for (int R = row0 - 2; R < out_row1 + 1; R += 2) {
+        bwalk_row<RP, 0, FAST>(a, s, W, R, xs, lane, out_row0, out_row1, out_x0, out_x1);
+        bwalk_row<RP, 1, FAST>(a, s, W, R + 1, xs, lane, out_row0, out_row1, out_x0, out_x1);
+    }
+}
+
+template <int RP>
+static void launch_bil_rp(const DevArgs &a, const BWalkPlan &P, uint32_t walk_blocks, uint32_t grid,
+                          hipStream_t stream) {
+    if (fast_rgba8(a))
+        hipLaunchKernelGGL((k_develop_bilinear_walk<RP, true>), dim3(grid), dim3(BWALK_WAVES * 64), 0, stream, a,
+                           P, walk_blocks);
+    else
+        hipLaunchKernelGGL((k_develop_bilinear_walk<RP, false>), dim3(grid), dim3(BWALK_WAVES * 64), 0, stream, a,
+                           P, walk_blocks);
+}
+
+static hipError_t launch_develop_bilinear_walk(const DevArgs &a, hipStream_t stream, int rows_per_strip) {
+    const int rp = red_position(a);
+    if (rp < 0) return hipErrorInvalidValue;
+    BWalkPlan P;
+    P.border = make_border(a, 1);
+    P.rows = rows_per_strip < 2 ? 2 : (rows_per_strip & ~1);
+    if (P.border.ix1 > P.border.ix0) {
+        P.x_first = P.border.ix0 + (int)a.crop_x;
+        P.x_end = P.border.ix1 + (int)a.crop_x;
+        P.y_first = P.border.iy0 + (int)a.crop_y;
+        P.y_end = P.border.iy1 + (int)a.crop_y;
+        P.xs0 = P.x_first & ~1;
+        P.ys0 = P.y_first & ~1;
+        P.strips = (P.x_end - P.xs0 + BWALK_COLS - 1) / BWALK_COLS;
+        P.chunks = (P.y_end - P.ys0 + P.rows - 1) / P.rows;
+    } else {
+        P.x_first = P.x_end = P.y_first = P.y_end = P.xs0 = P.ys0 = 0;
+        P.strips = P.chunks = 0;
+    }
+    const uint32_t walk_blocks = (uint32_t)((P.strips * P.chunks + BWALK_WAVES - 1) / BWALK_WAVES);
+    const uint32_t border_blocks = (P.border.count + BWALK_WAVES * 64 - 1) / (BWALK_WAVES * 64);
+    const uint32_t grid = walk_blocks + border_blocks;
+    if (grid == 0) return hipSuccess;
+    switch (rp) {
+    case 0: launch_bil_rp<0>(a, P, walk_blocks, grid, stream); break;
+    case 1: launch_bil_rp<1>(a, P, walk_blocks, grid, stream); break;
+    case 2: launch_bil_rp<2>(a, P, walk_blocks, grid, stream); break;
+    default: launch_bil_rp<3>(a, P, walk_blocks, grid, stream); break;
+    }
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------------------
 // Launchers
 
 static uint32_t persistent_grid(uint32_t ntiles, int num_cus, int blocks_per_cu) {
@@ -432,6 +409,8 @@ static uint32_t persistent_grid(uint32_t ntiles, int num_cus, int blocks_per_cu)
 hipError_t launch_develop(const DevArgs &a, bool ahd, hipStream_t stream, int num_cus) {
     const uint32_t rows = a.band_y1 - a.band_y0;
     if (rows == 0 || a.dev_w == 0) return hipSuccess;
+    if (ahd && a.ahd_impl == 0) return launch_develop_ahd_walk(a, stream, a.ahd_rows ? (int)a.ahd_rows : 64);
+    if (!ahd && a.bil_impl == 0) return launch_develop_bilinear_walk(a, stream, a.bil_rows ? (int)a.bil_rows : 64);
     if (ahd) {
         uint32_t tx = (a.dev_w + AW - 1) / AW, ty = (rows + AH - 1) / AH;
         uint32_t n = tx * ty;

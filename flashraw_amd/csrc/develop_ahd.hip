@@ -1,0 +1,394 @@
+// AHD develop, register-resident row walker (gfx950).
+//
+// The method of crates/flashraw/src/ahd.rs:191-393 (two green candidates, colour differences,
+// square-root Lab, homogeneity counts, proportional blend) fused with the finish and the encode,
+// with the same f32 operations in the same order. Instead of staging planes in LDS, one wave
+// walks down a strip of the frame: lane j owns frame columns x_s + 2j and x_s + 2j + 1 (an even
+// and an odd column), every stage keeps the rows it still needs in registers, and horizontal
+// neighbours come from the adjacent lanes through DPP wave shifts. With the Bayer phase a
+// template parameter and the row loop unrolled by two, the colour of every sample a lane touches
+// is a compile-time constant, so each pixel evaluates only its own formulas.
+//
+// Per iteration the wave loads mosaic row R and advances each stage one row:
+//   green candidates of row R-2, candidate RGB + Lab of row R-3, pair steps between rows R-4 and
+//   R-3 (each step computed once and read from both ends, ahd.rs:300-351), homogeneity counts of
+//   row R-4, and the blend, finish and encode of row R-5.
+// Lanes 3..60 hold complete results (116 output columns per strip); the halo lanes only feed
+// their neighbours. Pixels nearer the frame border than the AHD halo are developed bilinearly by
+// extra blocks of the same launch (strip_to, process.rs:1183-1224).
+#include "develop_common.h"
+#include "frh_launch.h"
+
+namespace frh {
+
+constexpr int WALK_COLS = 116;  // output columns per strip (lanes 3..60)
+constexpr int WALK_WAVES = 4;   // waves per block
+
+// Lightness step and squared chroma distance between two pixels of one candidate
+// (ahd.rs:172-180): |L_q - L_p| and (a_q - a_p)^2 + (b_q - b_p)^2 (the same from either end).
+struct Step {
+    float l, c;
+};
+__device__ __forceinline__ Step step(float pl, float pa, float pb, float ql, float qa, float qb) {
+    Step s;
+    s.l = fabsf(ql - pl);
+    float da = qa - pa, db = qb - pb;
+    s.c = da * da + db * db;
+    return s;
+}
+__device__ __forceinline__ Step step_left(Step s) { return Step{from_left(s.l), from_left(s.c)}; }
+__device__ __forceinline__ Step step_right(Step s) { return Step{from_right(s.l), from_right(s.c)}; }
+
+// Everything a lane carries from one row to the next. Index k: 0 the even column, 1 the odd one.
+struct Walk {
+    float m[6][2];       // balanced mosaic, rows R-5 .. R
+    float g[2][2][2];    // [cand][row R-3, R-2][k] green candidates (0 H, 1 V)
+    float df[2][3][2];   // [cand][row R-4, R-3, R-2][k] mosaic minus green
+    float dfl[2][3];     // [cand][row] df of the odd column of lane - 1
+    float dfr[2][3];     // [cand][row] df of the even column of lane + 1
+    float lab[2][3][2];  // [cand][L, a, b][k] of row R-4
+    Step sd[2][2], sdl[2][2], sdr[2][2];  // [cand][k] steps from row R-5 down / down-left / down-right
+    Step sr[2][2];                        // [cand][k] steps of row R-4 to the right
+    uint32_t hom[2][2];  // [row R-6, R-5][k] homogeneity counts, H + (V << 16)
+    float u[3][2][2][2]; // [row R-5, R-4, R-3][cand][k][2] the two interpolated channels
+};
+
+// Normalised, white-balanced, clipped mosaic sample of channel c (ahd.rs:143-151):
+// min(norm * wb[c], 1); 0 outside the frame.
+__device__ __forceinline__ float mosaic_at(const DevArgs &a, const Lds &s, int x, int y, int c) {
+    if (x < 0 || y < 0 || x >= (int)a.width || y >= (int)a.height) return 0.0f;
+    return fminf(norm_at(a, s, (uint32_t)x, (uint32_t)y) * a.wb[c], 1.0f);
+}
+
+// One walker iteration: load mosaic row R (parity PY) and advance every stage.
+template <int RP, int PY, bool FAST>
+__device__ __forceinline__ void walk_row(const DevArgs &a, const Lds &s, Walk &W, int R, int xs, int lane,
+                                         int out_row0, int out_row1, int out_x0, int out_x1) {
+    const int x0 = xs + 2 * lane;
+    // ---- load row R ----
+#pragma unroll
+    for (int i = 0; i < 5; i++) {
+        W.m[i][0] = W.m[i + 1][0];
+        W.m[i][1] = W.m[i + 1][1];
+    }
+    W.m[5][0] = mosaic_at(a, s, x0, R, site<RP>(PY, 0));
+    W.m[5][1] = mosaic_at(a, s, x0 + 1, R, site<RP>(PY, 1));
+
+    // ---- green candidates of row R-2 (ahd.rs:225-244); m index 3 ----
+    constexpr int PG = PY;
+#pragma unroll
+    for (int d = 0; d < 2; d++) {
+        W.g[d][0][0] = W.g[d][1][0];
+        W.g[d][0][1] = W.g[d][1][1];
+#pragma unroll
+        for (int r = 0; r < 2; r++) {
+            W.df[d][r][0] = W.df[d][r + 1][0];
+            W.df[d][r][1] = W.df[d][r + 1][1];
+            W.dfl[d][r] = W.dfl[d][r + 1];
+            W.dfr[d][r] = W.dfr[d][r + 1];
+        }
+    }
+    {
+        const float me = W.m[3][0], mo = W.m[3][1];
+        const float pe = from_left(me), po = from_left(mo);   // columns x0 - 2, x0 - 1
+        const float ne = from_right(me), no = from_right(mo); // columns x0 + 2, x0 + 3
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            const float c = k == 0 ? me : mo;
+            float gh = c, gv = c;
+            if (site<RP>(PG, k) != 1) {
+                const float l = k == 0 ? po : me, r = k == 0 ? mo : ne;
+                const float l2 = k == 0 ? pe : po, r2 = k == 0 ? ne : no;
+                const float u = W.m[2][k], dn = W.m[4][k], u2 = W.m[1][k], d2 = W.m[5][k];
+                gh = fminf(fmaxf((l + r) * 0.5f + (2.0f * c - l2 - r2) * 0.25f, fminf(l, r)), fmaxf(l, r));
+                gv = fminf(fmaxf((u + dn) * 0.5f + (2.0f * c - u2 - d2) * 0.25f, fminf(u, dn)), fmaxf(u, dn));
+            }
+            W.g[0][1][k] = gh;
+            W.g[1][1][k] = gv;
+            W.df[0][2][k] = c - gh;
+            W.df[1][2][k] = c - gv;
+        }
+#pragma unroll
+        for (int d = 0; d < 2; d++) {
+            W.dfl[d][2] = from_left(W.df[d][2][1]);
+            W.dfr[d][2] = from_right(W.df[d][2][0]);
+        }
+    }
+
+    // ---- candidate RGB and Lab of row R-3 (ahd.rs:249-298); m index 2 ----
+    constexpr int PL = PY ^ 1;
+    float lq[2][3][2];
+#pragma unroll
+    for (int r = 0; r < 2; r++)
+#pragma unroll
+        for (int d = 0; d < 2; d++)
+#pragma unroll
+            for (int k = 0; k < 2; k++) {
+                W.u[r][d][k][0] = W.u[r + 1][d][k][0];
+                W.u[r][d][k][1] = W.u[r + 1][d][k][1];
+            }
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        const float cm = W.m[2][k];
+        const float sm = sqrtf(fmaxf(cm, 0.0f));  // the measured channel, the same in both candidates
+        const int col = site<RP>(PL, k);
+#pragma unroll
+        for (int d = 0; d < 2; d++) {
+            const float g = W.g[d][0][k];
+            // colour differences around (R-3, column k): rows 0 (R-4), 1 (R-3), 2 (R-2)
+            auto dleft = [&](int r) { return k == 0 ? W.dfl[d][r] : W.df[d][r][0]; };
+            auto dright = [&](int r) { return k == 1 ? W.dfr[d][r] : W.df[d][r][1]; };
+            float red, gg, blue, u0, u1;
+            if (col == 1) {
+                const float row = g + 0.5f * (dleft(1) + dright(1));
+                const float clm = g + 0.5f * (W.df[d][0][k] + W.df[d][2][k]);
+                const bool hr = site<RP>(PL, k ^ 1) == 0;  // red neighbours along the row
+                red = hr ? row : clm;
+                blue = hr ? clm : row;
+                gg = cm;
+                u0 = red;
+                u1 = blue;
+            } else {
+                const float diag = g + 0.25f * (dleft(0) + dright(0) + dleft(2) + dright(2));
+                red = col == 0 ? cm : diag;
+                blue = col == 0 ? diag : cm;
+                gg = g;
+                u0 = g;
+                u1 = diag;
+            }
+            float sr, sg, sb;
+            if (col == 0) {
+                sr = sm;
+                sg = sqrtf(fmaxf(gg, 0.0f));
+                sb = sqrtf(fmaxf(blue, 0.0f));
+            } else if (col == 1) {
+                sr = sqrtf(fmaxf(red, 0.0f));
+                sg = sm;
+                sb = sqrtf(fmaxf(blue, 0.0f));
+            } else {
+                sr = sqrtf(fmaxf(red, 0.0f));
+                sg = sqrtf(fmaxf(gg, 0.0f));
+                sb = sm;
+            }
+            lq[d][0][k] = (sr + 2.0f * sg + sb) * 0.25f;
+            lq[d][1][k] = sr - sg;
+            lq[d][2][k] = sb - sg;
+            W.u[2][d][k][0] = u0;
+            W.u[2][d][k][1] = u1;
+        }
+    }
+
+    // ---- pair steps between rows R-4 (p) and R-3 (q); homogeneity of row R-4 (ahd.rs:300-351) ----
+    Step nd[2][2], ndl[2][2], ndr[2][2], nr[2][2];
+#pragma unroll
+    for (int d = 0; d < 2; d++) {
+        // q neighbours: even column of lane + 1, odd column of lane - 1
+        float qn[3], qp[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            qn[c] = from_right(lq[d][c][0]);
+            qp[c] = from_left(lq[d][c][1]);
+        }
+        nd[d][0] = step(W.lab[d][0][0], W.lab[d][1][0], W.lab[d][2][0], lq[d][0][0], lq[d][1][0], lq[d][2][0]);
+        nd[d][1] = step(W.lab[d][0][1], W.lab[d][1][1], W.lab[d][2][1], lq[d][0][1], lq[d][1][1], lq[d][2][1]);
+        ndr[d][0] = step(W.lab[d][0][0], W.lab[d][1][0], W.lab[d][2][0], lq[d][0][1], lq[d][1][1], lq[d][2][1]);
+        ndr[d][1] = step(W.lab[d][0][1], W.lab[d][1][1], W.lab[d][2][1], qn[0], qn[1], qn[2]);
+        ndl[d][0] = step(W.lab[d][0][0], W.lab[d][1][0], W.lab[d][2][0], qp[0], qp[1], qp[2]);
+        ndl[d][1] = step(W.lab[d][0][1], W.lab[d][1][1], W.lab[d][2][1], lq[d][0][0], lq[d][1][0], lq[d][2][0]);
+        nr[d][0] = step(lq[d][0][0], lq[d][1][0], lq[d][2][0], lq[d][0][1], lq[d][1][1], lq[d][2][1]);
+        nr[d][1] = step(lq[d][0][1], lq[d][1][1], lq[d][2][1], qn[0], qn[1], qn[2]);
+    }
+    // the eight neighbour steps of each pixel of row p, per candidate
+    Step nb[2][2][8];
+#pragma unroll
+    for (int d = 0; d < 2; d++) {
+        const Step ul_e = step_left(W.sdr[d][1]);  // down-right from (p-1, x0-1)
+        const Step ur_o = step_right(W.sdl[d][0]); // down-left from (p-1, x0+2)
+        const Step l_e = step_left(W.sr[d][1]);    // right step of (p, x0-1)
+        // even column
+        nb[d][0][0] = ul_e;
+        nb[d][0][1] = W.sd[d][0];
+        nb[d][0][2] = W.sdl[d][1];
+        nb[d][0][3] = l_e;
+        nb[d][0][4] = W.sr[d][0];
+        nb[d][0][5] = ndl[d][0];
+        nb[d][0][6] = nd[d][0];
+        nb[d][0][7] = ndr[d][0];
+        // odd column
+        nb[d][1][0] = W.sdr[d][0];
+        nb[d][1][1] = W.sd[d][1];
+        nb[d][1][2] = ur_o;
+        nb[d][1][3] = W.sr[d][0];
+        nb[d][1][4] = W.sr[d][1];
+        nb[d][1][5] = ndl[d][1];
+        nb[d][1][6] = nd[d][1];
+        nb[d][1][7] = ndr[d][1];
+    }
+    uint32_t hp[2];
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        // thresholds: the smaller of the largest steps along the row in H (left, right) and along
+        // the column in V (up, down)
+        const Step hl = nb[0][k][3], hrr = nb[0][k][4], vu = nb[1][k][1], vd = nb[1][k][6];
+        const float el = fminf(fmaxf(hl.l, hrr.l), fmaxf(vu.l, vd.l));
+        const float ec = fminf(fmaxf(hl.c, hrr.c), fmaxf(vu.c, vd.c));
+        uint32_t ch = 0, cv = 0;
+#pragma unroll
+        for (int n = 0; n < 8; n++) {
+            ch += (nb[0][k][n].l <= el && nb[0][k][n].c <= ec) ? 1u : 0u;
+            cv += (nb[1][k][n].l <= el && nb[1][k][n].c <= ec) ? 1u : 0u;
+        }
+        hp[k] = ch | (cv << 16);
+    }
+#pragma unroll
+    for (int d = 0; d < 2; d++)
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            W.sd[d][k] = nd[d][k];
+            W.sdl[d][k] = ndl[d][k];
+            W.sdr[d][k] = ndr[d][k];
+            W.sr[d][k] = nr[d][k];
+#pragma unroll
+            for (int c = 0; c < 3; c++) W.lab[d][c][k] = lq[d][c][k];
+        }
+
+    // ---- blend, finish and store row R-5 (ahd.rs:353-392) ----
+    const uint32_t v0 = W.hom[0][0] + W.hom[1][0] + hp[0];
+    const uint32_t v1 = W.hom[0][1] + W.hom[1][1] + hp[1];
+    W.hom[0][0] = W.hom[1][0];
+    W.hom[0][1] = W.hom[1][1];
+    W.hom[1][0] = hp[0];
+    W.hom[1][1] = hp[1];
+    // 3x3 sums (the DPP reads happen with every lane active: a lane switched off by a branch
+    // would hand its neighbour a 0)
+    const uint32_t sums[2] = {from_left_u(v1) + v0 + v1, v0 + v1 + from_right_u(v0)};
+    const int ro = R - 5;
+    if (ro >= out_row0 && ro < out_row1 && lane >= 3 && lane <= 60) {
+        constexpr int PO = PY ^ 1;
+        float o[2][3];
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            const float sh = (float)(sums[k] & 0xffffu), sv = (float)(sums[k] >> 16);
+            const float w = sh + sv > 0.0f ? sh / (sh + sv) : 0.5f;
+            const float cm = W.m[0][k];
+            const int col = site<RP>(PO, k);
+            float rgb[2][3];
+#pragma unroll
+            for (int d = 0; d < 2; d++) {
+                const float u0 = W.u[0][d][k][0], u1 = W.u[0][d][k][1];
+                rgb[d][0] = col == 0 ? cm : (col == 1 ? u0 : u1);
+                rgb[d][1] = col == 1 ? cm : u0;
+                rgb[d][2] = col == 2 ? cm : u1;
+            }
+            const float iw = 1.0f - w;
+            const float r = w * rgb[0][0] + iw * rgb[1][0];
+            const float g = w * rgb[0][1] + iw * rgb[1][1];
+            const float b = w * rgb[0][2] + iw * rgb[1][2];
+            finish(a, r, g, b, true, o[k][0], o[k][1], o[k][2]);
+        }
+        const int dx = x0 - (int)a.crop_x, dy = ro - (int)a.crop_y;
+        const bool in0 = x0 >= out_x0 && x0 < out_x1, in1 = x0 + 1 >= out_x0 && x0 + 1 < out_x1;
+        if (FAST) {
+            const uint32_t w0 = rgba8_word(s, o[0][0], o[0][1], o[0][2]);
+            const uint32_t w1 = rgba8_word(s, o[1][0], o[1][1], o[1][2]);
+            if (in0 && in1) {
+                store_rgba8_pair(a, (uint32_t)dx, (uint32_t)dy, w0, w1, 2);
+            } else if (in0) {
+                store_rgba8_pair(a, (uint32_t)dx, (uint32_t)dy, w0, 0, 1);
+            } else if (in1) {
+                store_rgba8_pair(a, (uint32_t)dx + 1, (uint32_t)dy, w1, 0, 1);
+            }
+        } else {
+            if (in0) store_both(a, s, (uint32_t)dx, (uint32_t)dy, o[0][0], o[0][1], o[0][2]);
+            if (in1) store_both(a, s, (uint32_t)dx + 1, (uint32_t)dy, o[1][0], o[1][1], o[1][2]);
+        }
+    }
+}
+
+// Work split of one AHD develop: strips of WALK_COLS columns x `rows` rows over the AHD
+// interior of the developed band, plus the bilinear border pixels.
+struct WalkPlan {
+    int x_first, x_end, y_first, y_end;  // interior, frame coordinates
+    int xs0, ys0;                        // even-aligned origins
+    int strips, chunks, rows;
+    Border border;
+};
+
+template <int RP, bool FAST>
+__global__ __launch_bounds__(WALK_WAVES * 64) void k_develop_ahd_walk(DevArgs a, WalkPlan P, uint32_t walk_blocks) {
+    __shared__ Lds s;
+    load_lds(a, s);
+    __syncthreads();
+    if (blockIdx.x >= walk_blocks) {
+        const uint32_t i = (blockIdx.x - walk_blocks) * blockDim.x + threadIdx.x;
+        if (i < P.border.count) border_pixel(a, s, P.border, i);
+        return;
+    }
+    const int lane = threadIdx.x & 63;
+    const int item = (int)(blockIdx.x * WALK_WAVES + (threadIdx.x >> 6));
+    if (item >= P.strips * P.chunks) return;
+    const int sx = item % P.strips, sy = item / P.strips;
+    const int xs = P.xs0 + WALK_COLS * sx - 6;  // lane 3 holds output column xs + 6
+    const int row0 = P.ys0 + P.rows * sy;
+    const int out_row0 = row0 > P.y_first ? row0 : P.y_first;
+    const int out_row1 = row0 + P.rows < P.y_end ? row0 + P.rows : P.y_end;
+    const int out_x0 = max(xs + 6, P.x_first), out_x1 = min(xs + 6 + WALK_COLS, P.x_end);
+    Walk W;
+    {
+        float *f = reinterpret_cast<float *>(&W);
+        for (int i = 0; i < (int)(sizeof(Walk) / 4); i++) f[i] = 0.0f;
+        W.hom[0][0] = W.hom[0][1] = W.hom[1][0] = W.hom[1][1] = 0u;
+    }
+    // mosaic row R runs from row0 - 6 (even) until the last output row out_row1 - 1 = R - 5
+    const int r_end = out_row1 + 5;
+    for (int R = row0 - 6; R < r_end; R += 2) {
+        walk_row<RP, 0, FAST>(a, s, W, R, xs, lane, out_row0, out_row1, out_x0, out_x1);
+        walk_row<RP, 1, FAST>(a, s, W, R + 1, xs, lane, out_row0, out_row1, out_x0, out_x1);
+    }
+}
+
+template <int RP>
+static void launch_ahd_rp(const DevArgs &a, const WalkPlan &P, uint32_t walk_blocks, uint32_t grid,
+                          hipStream_t stream) {
+    if (fast_rgba8(a))
+        hipLaunchKernelGGL((k_develop_ahd_walk<RP, true>), dim3(grid), dim3(WALK_WAVES * 64), 0, stream, a, P,
+                           walk_blocks);
+    else
+        hipLaunchKernelGGL((k_develop_ahd_walk<RP, false>), dim3(grid), dim3(WALK_WAVES * 64), 0, stream, a, P,
+                           walk_blocks);
+}
+
+hipError_t launch_develop_ahd_walk(const DevArgs &a, hipStream_t stream, int rows_per_strip) {
+    const int rp = red_position(a);
+    if (rp < 0) return hipErrorInvalidValue;
+    const int h = (int)a.ahd_halo;
+    WalkPlan P;
+    P.border = make_border(a, h);
+    P.rows = rows_per_strip < 2 ? 2 : (rows_per_strip & ~1);
+    if (P.border.ix1 > P.border.ix0) {
+        P.x_first = P.border.ix0 + (int)a.crop_x;
+        P.x_end = P.border.ix1 + (int)a.crop_x;
+        P.y_first = P.border.iy0 + (int)a.crop_y;
+        P.y_end = P.border.iy1 + (int)a.crop_y;
+        P.xs0 = P.x_first & ~1;
+        P.ys0 = P.y_first & ~1;
+        P.strips = (P.x_end - P.xs0 + WALK_COLS - 1) / WALK_COLS;
+        P.chunks = (P.y_end - P.ys0 + P.rows - 1) / P.rows;
+    } else {
+        P.x_first = P.x_end = P.y_first = P.y_end = P.xs0 = P.ys0 = 0;
+        P.strips = P.chunks = 0;
+    }
+    const uint32_t walk_blocks = (uint32_t)((P.strips * P.chunks + WALK_WAVES - 1) / WALK_WAVES);
+    const uint32_t border_blocks = (P.border.count + WALK_WAVES * 64 - 1) / (WALK_WAVES * 64);
+    const uint32_t grid = walk_blocks + border_blocks;
+    if (grid == 0) return hipSuccess;
+    switch (rp) {
+    case 0: launch_ahd_rp<0>(a, P, walk_blocks, grid, stream); break;
+    case 1: launch_ahd_rp<1>(a, P, walk_blocks, grid, stream); break;
+    case 2: launch_ahd_rp<2>(a, P, walk_blocks, grid, stream); break;
+    default: launch_ahd_rp<3>(a, P, walk_blocks, grid, stream); break;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace frh

@@ -1,0 +1,318 @@
+// Device helpers shared by the develop kernels: calibration, the finish (white balance, clip,
+// matrix, levels), the encoders, orientation, and the bilinear pixel read straight from the
+// frame. CPU statement: crates/flashraw/src/process.rs (Calib 246-299, finish_rgb 654-707,
+// encoders 1593-1991, encode_developed 2018-2051, row_cols 967-1129).
+#pragma once
+
+#include "frh_device.h"
+
+namespace frh {
+
+struct Lds {
+    float black[64];
+    float scale[64];
+    uint32_t lut[1025];  // 4097 sRGB codes, bytes packed four per word
+};
+
+__device__ __forceinline__ void load_lds(const DevArgs &a, Lds &s) {
+    for (int i = threadIdx.x; i < 64; i += blockDim.x) {
+        s.black[i] = a.black[i];
+        s.scale[i] = a.scale[i];
+    }
+    const uint32_t *l = reinterpret_cast<const uint32_t *>(a.lut8);
+    for (int i = threadIdx.x; i < 1025; i += blockDim.x) s.lut[i] = l[i];
+}
+
+__device__ __forceinline__ uint32_t lut8(const Lds &s, uint32_t i) {
+    return reinterpret_cast<const uint8_t *>(s.lut)[i];
+}
+
+// Channel (0 R, 1 G, 2 B) of frame sample (x, y) of a Bayer mosaic.
+__device__ __forceinline__ uint32_t chan_at(const DevArgs &a, int x, int y) {
+    const uint32_t i = (((uint32_t)y + a.oy) & 1u) * 2u + (((uint32_t)x + a.ox) & 1u);
+    return (a.chan_bits >> (2u * i)) & 3u;
+}
+
+// White-balance multiplier of channel c (selects on uniform values, no indexed argument load).
+__device__ __forceinline__ float wb_of(const DevArgs &a, uint32_t c) {
+    return c == 0 ? a.wb[0] : (c == 1 ? a.wb[1] : a.wb[2]);
+}
+
+// Calib::norm (process.rs:288-298) of frame sample (x, y), in bounds.
+__device__ __forceinline__ float norm_at(const DevArgs &a, const Lds &s, uint32_t x, uint32_t y) {
+    uint32_t X = x + a.ox, Y = y + a.oy;
+    uint32_t i = a.cal_pow2 ? (Y & (a.cal_ph - 1u)) * a.cal_pw + (X & (a.cal_pw - 1u))
+                            : (Y % a.cal_ph) * a.cal_pw + X % a.cal_pw;
+    float b = s.black[i];
+    if (a.n_rowd) b += Y < a.n_rowd ? a.rowd[Y] : 0.0f;
+    if (a.n_cold) b += X < a.n_cold ? a.cold[X] : 0.0f;
+    float v = (float)a.src[(size_t)y * a.stride + x];
+    return (v - b) * s.scale[i];
+}
+
+// finish_rgb with Highlights::Clip (process.rs:654-707). `balanced` pixels come from the AHD
+// mosaic, which is already white balanced (finish_run's unit balance, process.rs:1209-1213).
+__device__ __forceinline__ void finish(const DevArgs &a, float r, float g, float b, bool balanced, float &o0,
+                                       float &o1, float &o2) {
+    if (!balanced) {
+        r = r * a.wb[0];
+        g = g * a.wb[1];
+        b = b * a.wb[2];
+    }
+    r = rclamp01(r);
+    g = rclamp01(g);
+    b = rclamp01(b);
+    o0 = a.m[0] * r + a.m[1] * g + a.m[2] * b;
+    o1 = a.m[3] * r + a.m[4] * g + a.m[5] * b;
+    o2 = a.m[6] * r + a.m[7] * g + a.m[8] * b;
+    if (a.has_levels) {
+        o0 = o0 * a.lscale - a.loffset;
+        o1 = o1 * a.lscale - a.loffset;
+        o2 = o2 * a.lscale - a.loffset;
+    }
+}
+
+// f32_to_f16 (process.rs:1978-1991).
+__device__ __forceinline__ uint32_t f16_bits(float x) {
+    uint32_t b = __float_as_uint(x);
+    uint32_t sign = (b >> 16) & 0x8000u;
+    uint32_t a = b & 0x7fffffffu;
+    uint32_t normal = (a + 0xc8000fffu + ((a >> 13) & 1u)) >> 13;
+    uint32_t sub = __float_as_uint(__uint_as_float(a) + 0.5f) - 0x3f000000u;
+    uint32_t h = a < 0x38800000u ? sub : normal;
+    h = a >= 0x477ff000u ? (a > 0x7f800000u ? 0x7e00u : 0x7bffu) : h;
+    return sign | h;
+}
+
+// srgb16_paired (process.rs:1960-1967).
+__device__ __forceinline__ uint32_t srgb16(const DevArgs &a, float v) {
+    float x = rclamp01(v) * 16384.0f;
+    uint32_t i = trunc_u32(x);
+    i = i < 16383u ? i : 16383u;
+    float t = x - (float)i;
+    float2 p = a.srgb16[i];
+    return trunc_u32(p.x + p.y * t + 0.5f);
+}
+
+// linear16 (process.rs:1971-1973).
+__device__ __forceinline__ uint32_t lin16(float v) { return trunc_u32(rclamp01(v) * 65535.0f + 0.5f); }
+
+// Destination of developed pixel (dx, dy) under EXIF orientation o (the inverse of
+// encode_developed's source mapping, process.rs:2035-2044).
+__device__ __forceinline__ void orient(int o, uint32_t dx, uint32_t dy, uint32_t w, uint32_t h, uint32_t &px,
+                                       uint32_t &py) {
+    switch (o) {
+    case 2: px = w - 1 - dx; py = dy; break;
+    case 3: px = w - 1 - dx; py = h - 1 - dy; break;
+    case 4: px = dx; py = h - 1 - dy; break;
+    case 5: px = dy; py = dx; break;
+    case 6: px = h - 1 - dy; py = dx; break;
+    case 7: px = h - 1 - dy; py = w - 1 - dx; break;
+    case 8: px = dy; py = w - 1 - dx; break;
+    default: px = dx; py = dy; break;
+    }
+}
+
+// Encode and store linear output pixel (r, g, b) of developed pixel (dx, dy) (any encoding and
+// orientation).
+__device__ __forceinline__ void store_px(const DevArgs &a, const Lds &s, int mode, void *dst, uint32_t dx,
+                                         uint32_t dy, float r, float g, float b) {
+    if (mode == OUT_RGBA8) {
+        uint32_t px, py;
+        orient(a.orientation, dx, dy, a.dev_w, a.dev_h, px, py);
+        uint32_t v = lut8(s, lut8_index(r)) | (lut8(s, lut8_index(g)) << 8) | (lut8(s, lut8_index(b)) << 16) |
+                     0xff000000u;
+        reinterpret_cast<uint32_t *>(dst)[(size_t)py * a.dst_w + px] = v;
+    } else if (mode == OUT_LINEAR_F32) {
+        float *p = reinterpret_cast<float *>(dst) + ((size_t)dy * a.dev_w + dx) * 3;
+        p[0] = r;
+        p[1] = g;
+        p[2] = b;
+    } else if (mode == OUT_SRGB16) {
+        uint32_t px, py;
+        orient(a.orientation, dx, dy, a.dev_w, a.dev_h, px, py);
+        uint16_t *p = reinterpret_cast<uint16_t *>(dst) + ((size_t)py * a.dst_w + px) * 3;
+        p[0] = (uint16_t)srgb16(a, r);
+        p[1] = (uint16_t)srgb16(a, g);
+        p[2] = (uint16_t)srgb16(a, b);
+    } else if (mode == OUT_LINEAR16) {
+        uint16_t *p = reinterpret_cast<uint16_t *>(dst) + ((size_t)dy * a.dev_w + dx) * 3;
+        p[0] = (uint16_t)lin16(r);
+        p[1] = (uint16_t)lin16(g);
+        p[2] = (uint16_t)lin16(b);
+    } else if (mode == OUT_LINEAR_F16) {
+        uint16_t *p = reinterpret_cast<uint16_t *>(dst) + ((size_t)dy * a.dev_w + dx) * 3;
+        p[0] = (uint16_t)f16_bits(r);
+        p[1] = (uint16_t)f16_bits(g);
+        p[2] = (uint16_t)f16_bits(b);
+    }
+}
+
+__device__ __forceinline__ void store_both(const DevArgs &a, const Lds &s, uint32_t dx, uint32_t dy, float r,
+                                           float g, float b) {
+    store_px(a, s, a.out_mode, a.dst, dx, dy, r, g, b);
+    if (a.out_mode2 != OUT_NONE) store_px(a, s, a.out_mode2, a.dst2, dx, dy, r, g, b);
+}
+
+// The RGBA8 word of linear output (r, g, b) (process.rs:1712-1728; alpha 255, R in the low byte).
+__device__ __forceinline__ uint32_t rgba8_word(const Lds &s, float r, float g, float b) {
+    return lut8(s, lut8_index(r)) | (lut8(s, lut8_index(g)) << 8) | (lut8(s, lut8_index(b)) << 16) | 0xff000000u;
+}
+
+// True when a launch stores RGBA8 only, unrotated: the kernels' inline fast path.
+__host__ __device__ __forceinline__ bool fast_rgba8(const DevArgs &a) {
+    return a.out_mode == OUT_RGBA8 && a.out_mode2 == OUT_NONE && a.orientation == 1;
+}
+
+// Store the RGBA8 words of developed pixels (dx, dy) and (dx + 1, dy) on the fast path; `n` is 1
+// or 2 pixels, 8-byte stores when aligned.
+__device__ __forceinline__ void store_rgba8_pair(const DevArgs &a, uint32_t dx, uint32_t dy, uint32_t w0,
+                                                 uint32_t w1, int n) {
+    const size_t i = (size_t)dy * a.dst_w + dx;
+    uint32_t *p = reinterpret_cast<uint32_t *>(a.dst) + i;
+    if (n == 2 && (i & 1) == 0) {
+        *reinterpret_cast<uint2 *>(p) = make_uint2(w0, w1);
+    } else {
+        p[0] = w0;
+        if (n == 2) p[1] = w1;
+    }
+}
+
+// Kernel::row_cols for one pixel of a Bayer mosaic, read straight from the frame
+// (process.rs:1076-1126): the interior formulas, or at the frame border the mean of the
+// in-frame same-colour 3x3 samples in row-major order.
+__device__ __forceinline__ void bilinear_global(const DevArgs &a, const Lds &s, int x, int y, float rgb[3]) {
+    const int w = (int)a.width, h = (int)a.height;
+    uint32_t c0 = chan_at(a, x, y);
+    if (x > 0 && y > 0 && x + 1 < w && y + 1 < h) {
+        float own = norm_at(a, s, x, y);
+        float l = norm_at(a, s, x - 1, y), r = norm_at(a, s, x + 1, y);
+        float u = norm_at(a, s, x, y - 1), d = norm_at(a, s, x, y + 1);
+        if (c0 == 1) {
+            uint32_t ch = chan_at(a, x + 1, y);
+            float hv = (l + r) * 0.5f, vv = (u + d) * 0.5f;
+            rgb[1] = own;
+            rgb[0] = ch == 0 ? hv : vv;
+            rgb[2] = ch == 0 ? vv : hv;
+        } else {
+            float orth = (l + r + u + d) * 0.25f;
+            float diag = (norm_at(a, s, x - 1, y - 1) + norm_at(a, s, x + 1, y - 1) + norm_at(a, s, x - 1, y + 1) +
+                          norm_at(a, s, x + 1, y + 1)) *
+                         0.25f;
+            rgb[0] = c0 == 0 ? own : diag;
+            rgb[1] = orth;
+            rgb[2] = c0 == 0 ? diag : own;
+        }
+        return;
+    }
+    for (uint32_t c = 0; c < 3; c++) {
+        float sum = 0.0f, n = 0.0f;
+        if (c == c0) {
+            sum = norm_at(a, s, x, y);
+            n = 1.0f;
+        } else {
+            for (int dy = -1; dy <= 1; dy++) {
+                for (int dx = -1; dx <= 1; dx++) {
+                    int xx = x + dx, yy = y + dy;
+                    if (chan_at(a, xx, yy) != c) continue;
+                    if (xx < 0 || xx >= w || yy < 0 || yy >= h) continue;
+                    sum += norm_at(a, s, xx, yy);
+                    n += 1.0f;
+                }
+            }
+        }
+        rgb[c] = n > 0.0f ? sum / n : 0.0f;
+    }
+}
+
+// Pixels of a band outside a demosaic's interior [halo, w - halo) x [halo, h - halo), developed
+// by the bilinear formulas (strip_to, process.rs:1183-1224), enumerated row-major: the rows
+// above the interior, the side columns of the interior rows, the rows below.
+struct Border {
+    int ix0, ix1, iy0, iy1;  // interior in developed coordinates (iy0 = iy1 = band end: no interior)
+    uint32_t count;
+};
+
+__host__ inline Border make_border(const DevArgs &a, int halo) {
+    Border b;
+    const int cx0 = (int)a.crop_x, cy0 = (int)a.crop_y;
+    const int x_first = halo > cx0 ? halo : cx0;
+    const int x_end = (int)a.width - halo < cx0 + (int)a.dev_w ? (int)a.width - halo : cx0 + (int)a.dev_w;
+    const int y_first = halo > cy0 + (int)a.band_y0 ? halo : cy0 + (int)a.band_y0;
+    const int y_end = (int)a.height - halo < cy0 + (int)a.band_y1 ? (int)a.height - halo : cy0 + (int)a.band_y1;
+    const uint64_t band_px = (uint64_t)a.dev_w * (a.band_y1 - a.band_y0);
+    if (x_end > x_first && y_end > y_first) {
+        b.ix0 = x_first - cx0;
+        b.ix1 = x_end - cx0;
+        b.iy0 = y_first - cy0;
+        b.iy1 = y_end - cy0;
+        b.count = (uint32_t)(band_px - (uint64_t)(b.ix1 - b.ix0) * (b.iy1 - b.iy0));
+    } else {
+        b.ix0 = b.ix1 = 0;
+        b.iy0 = b.iy1 = (int)a.band_y1;
+        b.count = (uint32_t)band_px;
+    }
+    return b;
+}
+
+__device__ __forceinline__ void border_pixel(const DevArgs &a, const Lds &s, const Border &B, uint32_t i) {
+    const int dw = (int)a.dev_w;
+    const int b0 = (int)a.band_y0;
+    const int top = B.iy0 - b0;
+    const int mid = B.iy1 - B.iy0;
+    const int side = B.ix0 + (dw - B.ix1);
+    const uint32_t ntop = (uint32_t)top * dw, nmid = (uint32_t)mid * side;
+    int dx, dy;
+    if (i < ntop) {
+        dy = b0 + (int)(i / dw);
+        dx = (int)(i % dw);
+    } else if (i < ntop + nmid) {
+        const uint32_t j = i - ntop;
+        dy = B.iy0 + (int)(j / side);
+        const int c = (int)(j % side);
+        dx = c < B.ix0 ? c : B.ix1 + (c - B.ix0);
+    } else {
+        const uint32_t j = i - ntop - nmid;
+        dy = B.iy1 + (int)(j / dw);
+        dx = (int)(j % dw);
+    }
+    float rgb[3];
+    bilinear_global(a, s, (int)a.crop_x + dx, (int)a.crop_y + dy, rgb);
+    float o0, o1, o2;
+    finish(a, rgb[0], rgb[1], rgb[2], false, o0, o1, o2);
+    store_both(a, s, (uint32_t)dx, (uint32_t)dy, o0, o1, o2);
+}
+
+// Red's index in the frame-parity colour table fc[(y & 1) * 2 + (x & 1)] (the walkers' template
+// parameter), or -1.
+__host__ inline int red_position(const DevArgs &a) {
+    for (int i = 0; i < 4; i++) {
+        const uint32_t y = (uint32_t)(i >> 1), x = (uint32_t)(i & 1);
+        if (a.chan[((y + a.oy) & 1u) * 2u + ((x + a.ox) & 1u)] == 0) return i;
+    }
+    return -1;
+}
+
+// Value of the lane to the left (lane - 1) / right (lane + 1) of this one (DPP wave shifts; the
+// edge lanes read 0 and are halo lanes). Every lane must be active when these run.
+__device__ __forceinline__ float from_left(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float from_right(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, false));
+}
+__device__ __forceinline__ uint32_t from_left_u(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x138, 0xf, 0xf, false);
+}
+__device__ __forceinline__ uint32_t from_right_u(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x130, 0xf, 0xf, false);
+}
+
+// Colour (0 R, 1 G, 2 B) of a sample with frame row parity py and column parity px, when red
+// sits at index RP of the frame-parity table and blue opposite it.
+template <int RP>
+__device__ __forceinline__ constexpr int site(int py, int px) {
+    return (py * 2 + px) == RP ? 0 : ((py * 2 + px) == 3 - RP ? 2 : 1);
+}
+
+}  // namespace frh

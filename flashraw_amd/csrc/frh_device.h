@@ -39,6 +39,7 @@ struct DevArgs {
     uint32_t n_cold;
     // Bayer channel of sensor parity ((Y & 1) * 2 + (X & 1))
     uint32_t chan[4];
+    uint32_t chan_bits;      // chan[i] in bits 2i..2i+1 (register-only lookups in the kernels)
     // develop region and band
     uint32_t crop_x, crop_y, dev_w, dev_h;
     uint32_t band_y0, band_y1;  // developed rows [band_y0, band_y1) of this launch
@@ -55,6 +56,10 @@ struct DevArgs {
     const uint8_t *lut8;     // 4097 entries (+ padding to 4100)
     const float2 *srgb16;    // 16384 (value, step) pairs
     uint32_t ahd_halo;       // 6: pixels nearer the border than this are bilinear (ahd.rs:33)
+    uint32_t ahd_impl;       // 0: register row walker, 1: LDS-tiled (both give the same bytes)
+    uint32_t ahd_rows;       // rows per walker strip (even)
+    uint32_t bil_impl;       // bilinear: 0 register row walker, 1 LDS-tiled (the same bytes)
+    uint32_t bil_rows;       // rows per bilinear walker strip (even)
 };
 
 // Rust `x as u32` for x below 2^32: truncation, with NaN and everything below 1 giving 0.

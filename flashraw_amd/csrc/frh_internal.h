@@ -35,6 +35,8 @@ struct frh_context {
     uint8_t *d_lut8 = nullptr;
     float *d_srgb16 = nullptr;
     uint64_t max_binding = 0;
+    uint32_t ahd_rows = 64;  // rows per AHD walker strip
+    uint32_t bil_rows = 64;  // rows per bilinear walker strip
     std::mutex mu;
     struct Block {
         void *ptr;

@@ -124,6 +124,12 @@ def lib():
     global _lib
     with _lib_lock:
         if _lib is None:
+            # One HIP runtime per process: torch ships its own libamdhip64 (soname libamdhip64.so.7),
+            # so it is loaded first and this library binds to that copy.
+            try:
+                import torch  # noqa: F401
+            except ImportError:
+                pass
             if not os.path.exists(_LIB_PATH):
                 raise ImportError(f"libflashraw_hip.so not built ({_LIB_PATH}); run __graft_entry__.build()")
             L = ctypes.CDLL(_LIB_PATH)
@@ -157,6 +163,16 @@ def lib():
             L.frh_image_free.argtypes = [vp]
             L.frh_srgb8_table.argtypes = [vp, ctypes.c_size_t]
             L.frh_srgb8_table.restype = i32
+            L.frh_plan_create.argtypes = [vp, ctypes.POINTER(_Frame), ctypes.POINTER(_FrameInfo),
+                                          ctypes.POINTER(_DevelopSpec), ctypes.POINTER(vp)]
+            L.frh_plan_create.restype = i32
+            L.frh_plan_launch.argtypes = [vp, vp, vp, vp, vp]
+            L.frh_plan_launch.restype = i32
+            L.frh_plan_output_bytes.argtypes = [vp, i32]
+            L.frh_plan_output_bytes.restype = u64
+            L.frh_plan_destroy.argtypes = [vp]
+            L.frh_context_stream.argtypes = [vp]
+            L.frh_context_stream.restype = vp
             _lib = L
         return _lib
 
@@ -539,6 +555,16 @@ class GpuContext:
 
     run_blocking = run
 
+    def stream_handle(self, stream=None) -> int:
+        """hipStream_t of `stream` (a torch stream or an address); None is the context's stream."""
+        if stream is None:
+            return lib().frh_context_stream(self._h) or 0
+        return stream if isinstance(stream, int) else stream.cuda_stream
+
+    def prepare(self, frame: GpuFrame, info: FrameInfo, spec: DevelopSpec) -> "Plan":
+        """A validated develop to launch repeatedly on resident buffers (frh_plan_create)."""
+        return Plan(self, frame, info, spec)
+
     def develop_into(self, frame: GpuFrame, info: FrameInfo, spec: DevelopSpec, dst, dst2=None, stream=None):
         """Resident develop into caller-owned device buffers (torch tensors or addresses) on
         `stream` (a torch stream, a raw hipStream_t address, or None for the context stream);
@@ -546,9 +572,7 @@ class GpuContext:
         f, k1 = frame._c()
         i, k2 = info._c()
         s, k3 = spec._c()
-        sp = None
-        if stream is not None:
-            sp = stream if isinstance(stream, int) else stream.cuda_stream
+        sp = self.stream_handle(stream)
         n1 = dst.numel() * dst.element_size() if hasattr(dst, "numel") else (1 << 62)
         n2 = 0 if dst2 is None else (dst2.numel() * dst2.element_size() if hasattr(dst2, "numel") else (1 << 62))
         check(lib().frh_develop_into(self._h, ctypes.byref(f), ctypes.byref(i), ctypes.byref(s),
@@ -568,6 +592,37 @@ class GpuContext:
         if getattr(self, "_h", None):
             try:
                 lib().frh_context_destroy(self._h)
+            except Exception:
+                pass
+            self._h = None
+
+
+class Plan:
+    """A prepared develop (frh_plan): one ctypes call per launch."""
+
+    def __init__(self, ctx: GpuContext, frame: GpuFrame, info: FrameInfo, spec: DevelopSpec):
+        f, k1 = frame._c()
+        i, k2 = info._c()
+        s, k3 = spec._c()
+        h = ctypes.c_void_p()
+        check(lib().frh_plan_create(ctx.handle, ctypes.byref(f), ctypes.byref(i), ctypes.byref(s), ctypes.byref(h)))
+        del k1, k2, k3
+        self._h = h.value
+        self.ctx = ctx
+        self.output_bytes = (lib().frh_plan_output_bytes(self._h, 0), lib().frh_plan_output_bytes(self._h, 1))
+        self._launch = lib().frh_plan_launch
+
+    def launch(self, src, dst, dst2=None, stream=None) -> None:
+        """Develop device input `src` into `dst` (and `dst2`) on `stream`, asynchronously."""
+        rc = self._launch(self._h, _device_ptr(src), _device_ptr(dst), None if dst2 is None else _device_ptr(dst2),
+                          self.ctx.stream_handle(stream))
+        if rc:
+            check(rc)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            try:
+                lib().frh_plan_destroy(self._h)
             except Exception:
                 pass
             self._h = None

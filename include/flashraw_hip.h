@@ -150,12 +150,27 @@ const char *frh_last_error_message(void);
 int32_t frh_run_develop(frh_context *ctx, const frh_frame *frame, const frh_frame_info *info,
                         const frh_develop_spec *spec, frh_image **out, frh_image **out2, frh_timings *timings);
 
-/* The same develop into caller-owned device buffers on `stream` (a hipStream_t; NULL: the
- * context's stream), without synchronising: the resident form of GpuContext::run. `frame->data`
+/* The same develop into caller-owned device buffers on `stream` (a hipStream_t; NULL is the
+ * null stream), without synchronising: the resident form of GpuContext::run. `frame->data`
  * must be a device pointer. */
 int32_t frh_develop_into(frh_context *ctx, const frh_frame *frame, const frh_frame_info *info,
                          const frh_develop_spec *spec, void *dst, uint64_t dst_bytes, void *dst2,
                          uint64_t dst2_bytes, void *stream);
+
+/* A validated develop (the descriptor work of run.rs:439-760 done once) that is launched many
+ * times on resident buffers: repeated frames of one shape, batching and serving. The frame's
+ * `data` is only used for its geometry; each launch names its own device input. */
+typedef struct frh_plan frh_plan;
+int32_t frh_plan_create(frh_context *ctx, const frh_frame *frame, const frh_frame_info *info,
+                        const frh_develop_spec *spec, frh_plan **out);
+/* Develop device input `src` (the raster the frame describes) into `dst` (and `dst2` for
+ * output2) on `stream` (NULL: the null stream), without synchronising. */
+int32_t frh_plan_launch(const frh_plan *plan, const void *src, void *dst, void *dst2, void *stream);
+/* Bytes of the plan's first and second outputs. */
+uint64_t frh_plan_output_bytes(const frh_plan *plan, int32_t which);
+void frh_plan_destroy(frh_plan *plan);
+/* The context's own stream (a hipStream_t). */
+void *frh_context_stream(const frh_context *ctx);
 
 /* Output geometry of a develop: width, height (after orientation), channels and bytes. */
 int32_t frh_develop_output_size(const frh_frame *frame, const frh_develop_spec *spec, int32_t output,

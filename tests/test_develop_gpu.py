@@ -192,3 +192,25 @@ def test_config1_full_frame_dual_output(ctx):
     run = _compare(ctx, mos, "BGGR", demosaic="ahd", output=F.DevelopOutput.LinearRgb16,
                    output2=F.DevelopOutput.Rgba8, wb=synth.WB, matrix=synth.MATRIX, what="config1")
     print(f"config1 AHD dual: kernel {run.timings.kernel_ms:.3f} ms")
+
+
+@pytest.mark.parametrize("cfa", ["RGGB", "BGGR", "GRBG", "GBRG"])
+@pytest.mark.parametrize("demosaic", ["bilinear", "ahd"])
+def test_walker_equals_tiled(ctx, monkeypatch, cfa, demosaic):
+    """The register row walkers and the LDS-tiled kernels are independent GPU statements of the
+    same CPU code; they must produce the same bytes (crops, origins, odd sizes, strip heights)."""
+    knob = "AHD" if demosaic == "ahd" else "BILINEAR"
+    mos = synth.mosaic(517, 263, cfa, seed=17, stripes=60)
+    info = _info(517, 263, cfa)
+    for origin, crop, rows in [((0, 0), None, "64"), ((1, 1), (5, 3, 500, 250), "2"), ((3, 2), (64, 40, 300, 101), "18")]:
+        frame = F.GpuFrame.from_u16(mos, 517, 263).with_origin(origin)
+        c = F.Rect(*crop) if crop else F.Rect.full(517, 263)
+        for out in (F.DevelopOutput.LinearRgbF32, F.DevelopOutput.Rgba8):
+            spec = F.DevelopSpec(c, wb=synth.WB, cam_to_rgb=MATRIX, output=out,
+                                 edge=F.ahd_spec(info, origin) if demosaic == "ahd" else None)
+            monkeypatch.setenv(f"FLASHRAW_{knob}_ROWS", rows)
+            monkeypatch.delenv(f"FLASHRAW_{knob}_IMPL", raising=False)
+            walk = ctx.run(frame, info, F.GpuOp.Develop(spec)).image.download()
+            monkeypatch.setenv(f"FLASHRAW_{knob}_IMPL", "tiled")
+            tiled = ctx.run(frame, info, F.GpuOp.Develop(spec)).image.download()
+            assert np.array_equal(walk, tiled), (origin, crop, rows, out)
