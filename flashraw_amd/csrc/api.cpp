@@ -319,6 +319,26 @@ static int32_t plan_develop(const frh_context *ctx, const frh_frame *f, const fr
         a.chan[i] = colour < 3 ? colour : 3;
         a.chan_bits |= a.chan[i] << (2 * i);
     }
+    // The walkers' scalar calibration (pattern of at most 2x2, no deltas) and the bound that lets
+    // the RGBA8 fast path clamp with v_med3_f32.
+    a.simple_cal = (pw <= 2 && ph <= 2 && !a.n_rowd && !a.n_cold) ? 1u : 0u;
+    for (uint32_t i = 0; i < 4; i++) {
+        const uint32_t X = (i & 1) + a.ox, Y = (i >> 1) + a.oy;
+        const uint32_t k = (Y % a.cal_ph) * a.cal_pw + X % a.cal_pw;
+        a.qb[i] = a.black[k];
+        a.qs[i] = a.scale[k];
+    }
+    {
+        bool ok = true;
+        auto fine = [](float v) { return std::isfinite(v) && std::fabs(v) <= 1e6f; };
+        for (int i = 0; i < 4; i++) ok = ok && fine(spec->wb[i]);
+        for (int i = 0; i < 9; i++) ok = ok && fine(spec->cam_to_rgb[i]);
+        if (spec->has_levels) ok = ok && fine(spec->levels[0]) && fine(spec->levels[1]);
+        for (uint64_t i = 0; i < pw * ph; i++) ok = ok && fine(a.black[i]);
+        for (uint32_t i = 0; i < a.n_rowd; i++) ok = ok && fine(info->black_row_deltas[i]);
+        for (uint32_t i = 0; i < a.n_cold; i++) ok = ok && fine(info->black_col_deltas[i]);
+        a.bounded = ok ? 1u : 0u;
+    }
     // crop.clamp_to(w, h) (crates/flashraw-core/src/types.rs:68-70)
     uint64_t cx = spec->crop.x, cy = spec->crop.y;
     uint64_t cw = spec->crop.width, chh = spec->crop.height;

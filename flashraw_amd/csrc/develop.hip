@@ -242,24 +242,19 @@ __global__ __launch_bounds__(256) void k_develop_ahd(DevArgs a, uint32_t tiles_x
 
 constexpr int BWALK_COLS = 124;
 constexpr int BWALK_WAVES = 4;
+constexpr int BPF = 4;  // rows fetched ahead (each slot is loaded and consumed in place, never moved)
 
 struct BWalk {
-    float n[3][2];  // normalised rows R-2, R-1, R; [k] even / odd column
-    float nl[3];    // odd column of lane - 1
-    float nr[3];    // even column of lane + 1
+    float n[3][2];           // normalised rows R-2, R-1, R; [k] even / odd column
+    float nl[3];             // odd column of lane - 1
+    float nr[3];             // even column of lane + 1
+    uint32_t pf[BPF];        // raw pairs fetched ahead, slot (row - start) % BPF
 };
 
-// Two normalised samples (x, y), (x + 1, y) of a lane, 0 outside the frame.
-__device__ __forceinline__ void load_pair(const DevArgs &a, const Lds &s, int x, int y, float &v0, float &v1) {
-    v0 = v1 = 0.0f;
-    if (y < 0 || y >= (int)a.height) return;
-    if (x >= 0 && x < (int)a.width) v0 = norm_at(a, s, (uint32_t)x, (uint32_t)y);
-    if (x + 1 >= 0 && x + 1 < (int)a.width) v1 = norm_at(a, s, (uint32_t)x + 1, (uint32_t)y);
-}
-
-template <int RP, int PY, bool FAST>
+template <int RP, int PY, int SLOT, bool FAST, bool HOT>
 __device__ __forceinline__ void bwalk_row(const DevArgs &a, const Lds &s, BWalk &W, int R, int xs, int lane,
-                                          int out_row0, int out_row1, int out_x0, int out_x1) {
+                                          int out_row0, int out_row1, int out_x0, int out_x1, bool in0, bool in1,
+                                          __amdgpu_buffer_rsrc_t rs, __amdgpu_buffer_rsrc_t rd) {
     const int x0 = xs + 2 * lane;
 #pragma unroll
     for (int r = 0; r < 2; r++) {
@@ -268,7 +263,9 @@ __device__ __forceinline__ void bwalk_row(const DevArgs &a, const Lds &s, BWalk 
         W.nl[r] = W.nl[r + 1];
         W.nr[r] = W.nr[r + 1];
     }
-    load_pair(a, s, x0, R, W.n[2][0], W.n[2][1]);
+    const uint32_t raw = W.pf[SLOT];
+    W.pf[SLOT] = fetch_pair<HOT>(a, rs, x0, R + BPF);
+    norm_pair<PY, HOT>(a, s, raw, x0, R, R >= 0 && R < (int)a.height, in0, in1, W.n[2][0], W.n[2][1]);
     W.nl[2] = from_left(W.n[2][1]);
     W.nr[2] = from_right(W.n[2][0]);
     const int ro = R - 1;
@@ -297,23 +294,53 @@ __device__ __forceinline__ void bwalk_row(const DevArgs &a, const Lds &s, BWalk 
             rgb[1] = orth;
             rgb[2] = col == 0 ? diag : own;
         }
-        finish(a, rgb[0], rgb[1], rgb[2], false, o[k][0], o[k][1], o[k][2]);
+        if (FAST) {
+            o[k][0] = __uint_as_float(finish_rgba8_fast(a, s, rgb[0], rgb[1], rgb[2], false));
+        } else {
+            finish(a, rgb[0], rgb[1], rgb[2], false, o[k][0], o[k][1], o[k][2]);
+        }
     }
     const int dx = x0 - (int)a.crop_x, dy = ro - (int)a.crop_y;
-    const bool in0 = x0 >= out_x0 && x0 < out_x1, in1 = x0 + 1 >= out_x0 && x0 + 1 < out_x1;
+    const bool st0 = x0 >= out_x0 && x0 < out_x1, st1 = x0 + 1 >= out_x0 && x0 + 1 < out_x1;
     if (FAST) {
-        const uint32_t w0 = rgba8_word(s, o[0][0], o[0][1], o[0][2]);
-        const uint32_t w1 = rgba8_word(s, o[1][0], o[1][1], o[1][2]);
-        if (in0 && in1) {
-            store_rgba8_pair(a, (uint32_t)dx, (uint32_t)dy, w0, w1, 2);
-        } else if (in0) {
-            store_rgba8_pair(a, (uint32_t)dx, (uint32_t)dy, w0, 0, 1);
-        } else if (in1) {
-            store_rgba8_pair(a, (uint32_t)dx + 1, (uint32_t)dy, w1, 0, 1);
+        const uint32_t w0 = __float_as_uint(o[0][0]), w1 = __float_as_uint(o[1][0]);
+        if (st0 && st1) {
+            store_rgba8_pair(a, rd, (uint32_t)dx, (uint32_t)dy, w0, w1, 2);
+        } else if (st0) {
+            store_rgba8_pair(a, rd, (uint32_t)dx, (uint32_t)dy, w0, 0, 1);
+        } else if (st1) {
+            store_rgba8_pair(a, rd, (uint32_t)dx + 1, (uint32_t)dy, w1, 0, 1);
         }
     } else {
-        if (in0) store_both(a, s, (uint32_t)dx, (uint32_t)dy, o[0][0], o[0][1], o[0][2]);
-        if (in1) store_both(a, s, (uint32_t)dx + 1, (uint32_t)dy, o[1][0], o[1][1], o[1][2]);
+        if (st0) store_both(a, s, (uint32_t)dx, (uint32_t)dy, o[0][0], o[0][1], o[0][2]);
+        if (st1) store_both(a, s, (uint32_t)dx + 1, (uint32_t)dy, o[1][0], o[1][1], o[1][2]);
+    }
+}
+
+// One wave's strip: rows from row0 - 2 (even) until the last output row out_row1 - 1 = R - 1,
+// four rows per trip so every prefetch slot is static.
+template <int RP, bool FAST, bool HOT>
+__device__ __forceinline__ void bwalk_strip(const DevArgs &a, const Lds &s, int xs, int lane, int row0, int out_row0,
+                                            int out_row1, int out_x0, int out_x1) {
+    BWalk W;
+    {
+        float *f = reinterpret_cast<float *>(&W);
+        for (int i = 0; i < (int)(sizeof(BWalk) / 4) - BPF; i++) f[i] = 0.0f;
+    }
+    const int x0 = xs + 2 * lane;
+    const bool in0 = x0 >= 0 && x0 < (int)a.width, in1 = x0 + 1 >= 0 && x0 + 1 < (int)a.width;
+    const __amdgpu_buffer_rsrc_t rs = make_rsrc(a.src, src_bytes(a));
+    const __amdgpu_buffer_rsrc_t rd = make_rsrc(a.dst, a.dst_w * a.dev_h * 4u);
+#pragma unroll
+    for (int i = 0; i < BPF; i++) W.pf[i] = fetch_pair<HOT>(a, rs, x0, row0 - 2 + i);
+    for (int R = row0 - 2; R < out_row1 + 1; R += BPF) {
+        bwalk_row<RP, 0, 0, FAST, HOT>(a, s, W, R, xs, lane, out_row0, out_row1, out_x0, out_x1, in0, in1, rs, rd);
+        bwalk_row<RP, 1, 1, FAST, HOT>(a, s, W, R + 1, xs, lane, out_row0, out_row1, out_x0, out_x1, in0, in1, rs,
+                                       rd);
+        bwalk_row<RP, 0, 2, FAST, HOT>(a, s, W, R + 2, xs, lane, out_row0, out_row1, out_x0, out_x1, in0, in1, rs,
+                                       rd);
+        bwalk_row<RP, 1, 3, FAST, HOT>(a, s, W, R + 3, xs, lane, out_row0, out_row1, out_x0, out_x1, in0, in1, rs,
+                                       rd);
     }
 }
 
@@ -336,7 +363,9 @@ __global__ __launch_bounds__(BWALK_WAVES * 64) void k_develop_bilinear_walk(DevA
         return;
     }
     const int lane = threadIdx.x & 63;
-    const int item = (int)(blockIdx.x * BWALK_WAVES + (threadIdx.x >> 6));
+    // the wave index is the same for every lane; readfirstlane lets the compiler keep all the strip
+    // bookkeeping in scalar registers
+    const int item = (int)uniform(blockIdx.x * BWALK_WAVES + (threadIdx.x >> 6));
     if (item >= P.strips * P.chunks) return;
     const int sx = item % P.strips, sy = item / P.strips;
     const int xs = P.xs0 + BWALK_COLS * sx - 2;  // lane 1 holds output column xs + 2
@@ -344,15 +373,12 @@ __global__ __launch_bounds__(BWALK_WAVES * 64) void k_develop_bilinear_walk(DevA
     const int out_row0 = row0 > P.y_first ? row0 : P.y_first;
     const int out_row1 = row0 + P.rows < P.y_end ? row0 + P.rows : P.y_end;
     const int out_x0 = max(xs + 2, P.x_first), out_x1 = min(xs + 2 + BWALK_COLS, P.x_end);
-    BWalk W;
-    {
-        float *f = reinterpret_cast<float *>(&W);
-        for (int i = 0; i < (int)(sizeof(BWalk) / 4); i++) f[i] = 0.0f;
-    }
-    for (int R = row0 - 2; R < out_row1 + 1; R += 2) {
-        bwalk_row<RP, 0, FAST>(a, s, W, R, xs, lane, out_row0, out_row1, out_x0, out_x1);
-        bwalk_row<RP, 1, FAST>(a, s, W, R + 1, xs, lane, out_row0, out_row1, out_x0, out_x1);
-    }
+    const bool hot = a.simple_cal && xs >= 0 && xs + 128 <= (int)a.width && ((uintptr_t)a.src & 3u) == 0 &&
+                     (a.stride & 1u) == 0;
+    if (hot)
+        bwalk_strip<RP, FAST, true>(a, s, xs, lane, row0, out_row0, out_row1, out_x0, out_x1);
+    else
+        bwalk_strip<RP, FAST, false>(a, s, xs, lane, row0, out_row0, out_row1, out_x0, out_x1);
 }
 
 template <int RP>

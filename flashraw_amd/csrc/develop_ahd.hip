@@ -51,19 +51,15 @@ struct Walk {
     Step sr[2][2];                        // [cand][k] steps of row R-4 to the right
     uint32_t hom[2][2];  // [row R-6, R-5][k] homogeneity counts, H + (V << 16)
     float u[3][2][2][2]; // [row R-5, R-4, R-3][cand][k][2] the two interpolated channels
+    uint32_t pf[2];       // raw pairs fetched two rows ahead, slot = row parity (consumed in place)
 };
 
-// Normalised, white-balanced, clipped mosaic sample of channel c (ahd.rs:143-151):
-// min(norm * wb[c], 1); 0 outside the frame.
-__device__ __forceinline__ float mosaic_at(const DevArgs &a, const Lds &s, int x, int y, int c) {
-    if (x < 0 || y < 0 || x >= (int)a.width || y >= (int)a.height) return 0.0f;
-    return fminf(norm_at(a, s, (uint32_t)x, (uint32_t)y) * a.wb[c], 1.0f);
-}
 
 // One walker iteration: load mosaic row R (parity PY) and advance every stage.
-template <int RP, int PY, bool FAST>
+template <int RP, int PY, bool FAST, bool HOT>
 __device__ __forceinline__ void walk_row(const DevArgs &a, const Lds &s, Walk &W, int R, int xs, int lane,
-                                         int out_row0, int out_row1, int out_x0, int out_x1) {
+                                         int out_row0, int out_row1, int out_x0, int out_x1, bool in0, bool in1,
+                                         __amdgpu_buffer_rsrc_t rs, __amdgpu_buffer_rsrc_t rd) {
     const int x0 = xs + 2 * lane;
     // ---- load row R ----
 #pragma unroll
@@ -71,8 +67,15 @@ __device__ __forceinline__ void walk_row(const DevArgs &a, const Lds &s, Walk &W
         W.m[i][0] = W.m[i + 1][0];
         W.m[i][1] = W.m[i + 1][1];
     }
-    W.m[5][0] = mosaic_at(a, s, x0, R, site<RP>(PY, 0));
-    W.m[5][1] = mosaic_at(a, s, x0 + 1, R, site<RP>(PY, 1));
+    // balanced mosaic, clipped at 1 (ahd.rs:143-151): min(norm * wb[c], 1); 0 outside the frame
+    {
+        const uint32_t raw = W.pf[PY];
+        W.pf[PY] = fetch_pair<HOT>(a, rs, x0, R + 2);
+        float n0, n1;
+        norm_pair<PY, HOT>(a, s, raw, x0, R, R >= 0 && R < (int)a.height, in0, in1, n0, n1);
+        W.m[5][0] = fminf(n0 * a.wb[site<RP>(PY, 0)], 1.0f);
+        W.m[5][1] = fminf(n1 * a.wb[site<RP>(PY, 1)], 1.0f);
+    }
 
     // ---- green candidates of row R-2 (ahd.rs:225-244); m index 3 ----
     constexpr int PG = PY;
@@ -284,24 +287,51 @@ __device__ __forceinline__ void walk_row(const DevArgs &a, const Lds &s, Walk &W
             const float r = w * rgb[0][0] + iw * rgb[1][0];
             const float g = w * rgb[0][1] + iw * rgb[1][1];
             const float b = w * rgb[0][2] + iw * rgb[1][2];
-            finish(a, r, g, b, true, o[k][0], o[k][1], o[k][2]);
+            if (FAST) {
+                o[k][0] = __uint_as_float(finish_rgba8_fast(a, s, r, g, b, true));
+            } else {
+                finish(a, r, g, b, true, o[k][0], o[k][1], o[k][2]);
+            }
         }
         const int dx = x0 - (int)a.crop_x, dy = ro - (int)a.crop_y;
-        const bool in0 = x0 >= out_x0 && x0 < out_x1, in1 = x0 + 1 >= out_x0 && x0 + 1 < out_x1;
+        const bool st0 = x0 >= out_x0 && x0 < out_x1, st1 = x0 + 1 >= out_x0 && x0 + 1 < out_x1;
         if (FAST) {
-            const uint32_t w0 = rgba8_word(s, o[0][0], o[0][1], o[0][2]);
-            const uint32_t w1 = rgba8_word(s, o[1][0], o[1][1], o[1][2]);
-            if (in0 && in1) {
-                store_rgba8_pair(a, (uint32_t)dx, (uint32_t)dy, w0, w1, 2);
-            } else if (in0) {
-                store_rgba8_pair(a, (uint32_t)dx, (uint32_t)dy, w0, 0, 1);
-            } else if (in1) {
-                store_rgba8_pair(a, (uint32_t)dx + 1, (uint32_t)dy, w1, 0, 1);
+            const uint32_t w0 = __float_as_uint(o[0][0]), w1 = __float_as_uint(o[1][0]);
+            if (st0 && st1) {
+                store_rgba8_pair(a, rd, (uint32_t)dx, (uint32_t)dy, w0, w1, 2);
+            } else if (st0) {
+                store_rgba8_pair(a, rd, (uint32_t)dx, (uint32_t)dy, w0, 0, 1);
+            } else if (st1) {
+                store_rgba8_pair(a, rd, (uint32_t)dx + 1, (uint32_t)dy, w1, 0, 1);
             }
         } else {
-            if (in0) store_both(a, s, (uint32_t)dx, (uint32_t)dy, o[0][0], o[0][1], o[0][2]);
-            if (in1) store_both(a, s, (uint32_t)dx + 1, (uint32_t)dy, o[1][0], o[1][1], o[1][2]);
+            if (st0) store_both(a, s, (uint32_t)dx, (uint32_t)dy, o[0][0], o[0][1], o[0][2]);
+            if (st1) store_both(a, s, (uint32_t)dx + 1, (uint32_t)dy, o[1][0], o[1][1], o[1][2]);
         }
+    }
+}
+
+// One wave's strip: mosaic row R runs from row0 - 6 (even) until the last output row
+// out_row1 - 1 = R - 5, two rows per trip (prefetch slot = row parity).
+template <int RP, bool FAST, bool HOT>
+__device__ __forceinline__ void walk_strip(const DevArgs &a, const Lds &s, int xs, int lane, int row0, int out_row0,
+                                           int out_row1, int out_x0, int out_x1) {
+    Walk W;
+    {
+        float *f = reinterpret_cast<float *>(&W);
+        for (int i = 0; i < (int)(sizeof(Walk) / 4) - 2; i++) f[i] = 0.0f;
+        W.hom[0][0] = W.hom[0][1] = W.hom[1][0] = W.hom[1][1] = 0u;
+    }
+    const int x0 = xs + 2 * lane;
+    const bool in0 = x0 >= 0 && x0 < (int)a.width, in1 = x0 + 1 >= 0 && x0 + 1 < (int)a.width;
+    const __amdgpu_buffer_rsrc_t rs = make_rsrc(a.src, src_bytes(a));
+    const __amdgpu_buffer_rsrc_t rd = make_rsrc(a.dst, a.dst_w * a.dev_h * 4u);
+#pragma unroll
+    for (int i = 0; i < 2; i++) W.pf[i] = fetch_pair<HOT>(a, rs, x0, row0 - 6 + i);
+    const int r_end = out_row1 + 5;
+    for (int R = row0 - 6; R < r_end; R += 2) {
+        walk_row<RP, 0, FAST, HOT>(a, s, W, R, xs, lane, out_row0, out_row1, out_x0, out_x1, in0, in1, rs, rd);
+        walk_row<RP, 1, FAST, HOT>(a, s, W, R + 1, xs, lane, out_row0, out_row1, out_x0, out_x1, in0, in1, rs, rd);
     }
 }
 
@@ -325,7 +355,9 @@ __global__ __launch_bounds__(WALK_WAVES * 64) void k_develop_ahd_walk(DevArgs a,
         return;
     }
     const int lane = threadIdx.x & 63;
-    const int item = (int)(blockIdx.x * WALK_WAVES + (threadIdx.x >> 6));
+    // the wave index is the same for every lane; readfirstlane lets the compiler keep all the strip
+    // bookkeeping in scalar registers
+    const int item = (int)uniform(blockIdx.x * WALK_WAVES + (threadIdx.x >> 6));
     if (item >= P.strips * P.chunks) return;
     const int sx = item % P.strips, sy = item / P.strips;
     const int xs = P.xs0 + WALK_COLS * sx - 6;  // lane 3 holds output column xs + 6
@@ -333,18 +365,12 @@ __global__ __launch_bounds__(WALK_WAVES * 64) void k_develop_ahd_walk(DevArgs a,
     const int out_row0 = row0 > P.y_first ? row0 : P.y_first;
     const int out_row1 = row0 + P.rows < P.y_end ? row0 + P.rows : P.y_end;
     const int out_x0 = max(xs + 6, P.x_first), out_x1 = min(xs + 6 + WALK_COLS, P.x_end);
-    Walk W;
-    {
-        float *f = reinterpret_cast<float *>(&W);
-        for (int i = 0; i < (int)(sizeof(Walk) / 4); i++) f[i] = 0.0f;
-        W.hom[0][0] = W.hom[0][1] = W.hom[1][0] = W.hom[1][1] = 0u;
-    }
-    // mosaic row R runs from row0 - 6 (even) until the last output row out_row1 - 1 = R - 5
-    const int r_end = out_row1 + 5;
-    for (int R = row0 - 6; R < r_end; R += 2) {
-        walk_row<RP, 0, FAST>(a, s, W, R, xs, lane, out_row0, out_row1, out_x0, out_x1);
-        walk_row<RP, 1, FAST>(a, s, W, R + 1, xs, lane, out_row0, out_row1, out_x0, out_x1);
-    }
+    const bool hot = a.simple_cal && xs >= 0 && xs + 128 <= (int)a.width && ((uintptr_t)a.src & 3u) == 0 &&
+                     (a.stride & 1u) == 0;
+    if (hot)
+        walk_strip<RP, FAST, true>(a, s, xs, lane, row0, out_row0, out_row1, out_x0, out_x1);
+    else
+        walk_strip<RP, FAST, false>(a, s, xs, lane, row0, out_row0, out_row1, out_x0, out_x1);
 }
 
 template <int RP>

@@ -39,15 +39,9 @@ __device__ __forceinline__ float wb_of(const DevArgs &a, uint32_t c) {
 }
 
 // Calib::norm (process.rs:288-298) of frame sample (x, y), in bounds.
+__device__ __forceinline__ float norm_raw(const DevArgs &a, const Lds &s, float v, uint32_t x, uint32_t y);
 __device__ __forceinline__ float norm_at(const DevArgs &a, const Lds &s, uint32_t x, uint32_t y) {
-    uint32_t X = x + a.ox, Y = y + a.oy;
-    uint32_t i = a.cal_pow2 ? (Y & (a.cal_ph - 1u)) * a.cal_pw + (X & (a.cal_pw - 1u))
-                            : (Y % a.cal_ph) * a.cal_pw + X % a.cal_pw;
-    float b = s.black[i];
-    if (a.n_rowd) b += Y < a.n_rowd ? a.rowd[Y] : 0.0f;
-    if (a.n_cold) b += X < a.n_cold ? a.cold[X] : 0.0f;
-    float v = (float)a.src[(size_t)y * a.stride + x];
-    return (v - b) * s.scale[i];
+    return norm_raw(a, s, (float)a.src[(size_t)y * a.stride + x], x, y);
 }
 
 // finish_rgb with Highlights::Clip (process.rs:654-707). `balanced` pixels come from the AHD
@@ -159,22 +153,113 @@ __device__ __forceinline__ uint32_t rgba8_word(const Lds &s, float r, float g, f
     return lut8(s, lut8_index(r)) | (lut8(s, lut8_index(g)) << 8) | (lut8(s, lut8_index(b)) << 16) | 0xff000000u;
 }
 
-// True when a launch stores RGBA8 only, unrotated: the kernels' inline fast path.
+// True when a launch stores RGBA8 only, unrotated, without levels, with bounded parameters: the
+// kernels' inline fast path.
 __host__ __device__ __forceinline__ bool fast_rgba8(const DevArgs &a) {
-    return a.out_mode == OUT_RGBA8 && a.out_mode2 == OUT_NONE && a.orientation == 1;
+    return a.out_mode == OUT_RGBA8 && a.out_mode2 == OUT_NONE && a.orientation == 1 && a.bounded && !a.has_levels;
+}
+
+__device__ __forceinline__ float med01(float x) { return __builtin_amdgcn_fmed3f(x, 0.0f, 1.0f); }
+
+// finish + RGBA8 encode of the fast path (finite values, no levels): the same operations as
+// finish() and rgba8_word() with the clamps as v_med3_f32 (or the clamp output modifier).
+__device__ __forceinline__ uint32_t finish_rgba8_fast(const DevArgs &a, const Lds &s, float r, float g, float b,
+                                                      bool balanced) {
+    if (!balanced) {
+        r = r * a.wb[0];
+        g = g * a.wb[1];
+        b = b * a.wb[2];
+    }
+    r = med01(r);
+    g = med01(g);
+    b = med01(b);
+    const float o0 = a.m[0] * r + a.m[1] * g + a.m[2] * b;
+    const float o1 = a.m[3] * r + a.m[4] * g + a.m[5] * b;
+    const float o2 = a.m[6] * r + a.m[7] * g + a.m[8] * b;
+    const uint32_t i0 = (uint32_t)(med01(o0) * 4096.0f), i1 = (uint32_t)(med01(o1) * 4096.0f),
+                   i2 = (uint32_t)(med01(o2) * 4096.0f);
+    return lut8(s, i0) | (lut8(s, i1) << 8) | (lut8(s, i2) << 16) | 0xff000000u;
+}
+
+// Raw samples of frame columns x0, x0 + 1 of row y as one word (x0 in the low half). Addresses
+// are clamped into the frame, so the load never faults and needs no branch; the caller masks
+// samples outside the frame. HOT: the pair is one aligned 32-bit word inside the frame.
+// A raw buffer resource over `bytes` bytes at `base`: loads and stores then take a scalar
+// (wave-uniform) byte offset plus a 32-bit per-lane offset, with no 64-bit address arithmetic.
+__device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *base, uint32_t bytes) {
+    const uint64_t p = (uint64_t)(uintptr_t)base;
+    const uint64_t q = ((uint64_t)uniform((uint32_t)(p >> 32)) << 32) | uniform((uint32_t)p);
+    return __builtin_amdgcn_make_buffer_rsrc((void *)(uintptr_t)q, (short)0, (int)uniform(bytes), 0x00020000);
+}
+
+// Bytes from `a.src` to the end of the frame's last sample.
+__device__ __forceinline__ uint32_t src_bytes(const DevArgs &a) {
+    return ((a.height - 1u) * a.stride + a.width) * 2u;
+}
+
+template <bool HOT>
+__device__ __forceinline__ uint32_t fetch_pair(const DevArgs &a, __amdgpu_buffer_rsrc_t rs, int x0, int y) {
+    const int yc = min(max(y, 0), (int)a.height - 1);
+    const int soff = (int)uniform((uint32_t)yc * a.stride * 2u);
+    if (HOT) return (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rs, x0 * 2, soff, 0);
+    const int c0 = min(max(x0, 0), (int)a.width - 1), c1 = min(max(x0 + 1, 0), (int)a.width - 1);
+    const uint32_t v0 = __builtin_amdgcn_raw_buffer_load_b16(rs, c0 * 2, soff, 0);
+    const uint32_t v1 = __builtin_amdgcn_raw_buffer_load_b16(rs, c1 * 2, soff, 0);
+    return v0 | (v1 << 16);
+}
+
+// Calib::norm (process.rs:288-298) of raw sample v at frame (x, y), in the frame.
+__device__ __forceinline__ float norm_raw(const DevArgs &a, const Lds &s, float v, uint32_t x, uint32_t y) {
+    uint32_t X = x + a.ox, Y = y + a.oy;
+    uint32_t i = a.cal_pow2 ? (Y & (a.cal_ph - 1u)) * a.cal_pw + (X & (a.cal_pw - 1u))
+                            : (Y % a.cal_ph) * a.cal_pw + X % a.cal_pw;
+    float b = s.black[i];
+    if (a.n_rowd) b += Y < a.n_rowd ? a.rowd[Y] : 0.0f;
+    if (a.n_cold) b += X < a.n_cold ? a.cold[X] : 0.0f;
+    return (v - b) * s.scale[i];
+}
+
+// Normalised samples of a fetched pair at row y (parity PY); 0 outside the frame (`in0`, `in1`:
+// the lane's columns are inside, `row_in`: the row is). HOT: every column of the wave is inside
+// the frame and the calibration is the scalar one.
+template <int PY, bool HOT>
+__device__ __forceinline__ void norm_pair(const DevArgs &a, const Lds &s, uint32_t raw, int x0, int y, bool row_in,
+                                          bool in0, bool in1, float &n0, float &n1) {
+    const float v0 = (float)(raw & 0xffffu), v1 = (float)(raw >> 16);
+    if (HOT) {
+        n0 = row_in ? (v0 - a.qb[PY * 2 + 0]) * a.qs[PY * 2 + 0] : 0.0f;
+        n1 = row_in ? (v1 - a.qb[PY * 2 + 1]) * a.qs[PY * 2 + 1] : 0.0f;
+        return;
+    }
+    if (a.simple_cal) {
+        n0 = (v0 - a.qb[PY * 2 + 0]) * a.qs[PY * 2 + 0];
+        n1 = (v1 - a.qb[PY * 2 + 1]) * a.qs[PY * 2 + 1];
+    } else {
+        const uint32_t yc = (uint32_t)min(max(y, 0), (int)a.height - 1);
+        n0 = norm_raw(a, s, v0, (uint32_t)min(max(x0, 0), (int)a.width - 1), yc);
+        n1 = norm_raw(a, s, v1, (uint32_t)min(max(x0 + 1, 0), (int)a.width - 1), yc);
+    }
+    n0 = row_in && in0 ? n0 : 0.0f;
+    n1 = row_in && in1 ? n1 : 0.0f;
 }
 
 // Store the RGBA8 words of developed pixels (dx, dy) and (dx + 1, dy) on the fast path; `n` is 1
 // or 2 pixels, 8-byte stores when aligned.
-__device__ __forceinline__ void store_rgba8_pair(const DevArgs &a, uint32_t dx, uint32_t dy, uint32_t w0,
-                                                 uint32_t w1, int n) {
-    const size_t i = (size_t)dy * a.dst_w + dx;
-    uint32_t *p = reinterpret_cast<uint32_t *>(a.dst) + i;
-    if (n == 2 && (i & 1) == 0) {
-        *reinterpret_cast<uint2 *>(p) = make_uint2(w0, w1);
+__device__ __forceinline__ void store_rgba8_pair(const DevArgs &a, __amdgpu_buffer_rsrc_t rd, uint32_t dx,
+                                                 uint32_t dy, uint32_t w0, uint32_t w1, int n) {
+    const int soff = (int)uniform(dy * a.dst_w * 4u);
+    const int voff = (int)(dx * 4u);
+    if (n == 2 && ((dy * a.dst_w + dx) & 1u) == 0) {
+        typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+        u32x2 v;
+        v.x = w0;
+        v.y = w1;
+        __builtin_amdgcn_raw_buffer_store_b64(v, rd, voff, soff, 0);
     } else {
-        p[0] = w0;
-        if (n == 2) p[1] = w1;
+        __builtin_amdgcn_raw_buffer_store_b32(w0, rd, voff, soff, 0);
+        if (n == 2) __builtin_amdgcn_raw_buffer_store_b32(w1, rd, voff + 4, soff, 0);
     }
 }
 

@@ -40,6 +40,14 @@ struct DevArgs {
     // Bayer channel of sensor parity ((Y & 1) * 2 + (X & 1))
     uint32_t chan[4];
     uint32_t chan_bits;      // chan[i] in bits 2i..2i+1 (register-only lookups in the kernels)
+    // The calibration at frame parity (y & 1) * 2 + (x & 1) when the pattern is at most 2x2 and
+    // there are no row/column deltas (`simple_cal`): the walkers read these as scalars.
+    uint32_t simple_cal;
+    float qb[4], qs[4];
+    // All parameters are bounded (|value| <= 1e6), so no infinity or NaN can arise and the RGBA8
+    // fast path may clamp with v_med3_f32 (identical to f32::clamp on finite values; the sign of a
+    // zero cannot change an 8-bit code).
+    uint32_t bounded;
     // develop region and band
     uint32_t crop_x, crop_y, dev_w, dev_h;
     uint32_t band_y0, band_y1;  // developed rows [band_y0, band_y1) of this launch
