@@ -85,6 +85,8 @@ def main():
     frame = F.GpuFrame.from_u16(d_in, W, H)
     plan = ctx.prepare(frame, info, spec)
     stream = torch.cuda.Stream()  # the launch stream; the HIP events below are recorded on it
+    # one-time setup outside the timed region: the autotuner picks the strip height
+    rows = plan.autotune(d_in, dst, stream=stream) if os.environ.get("FLASHRAW_NO_AUTOTUNE") is None else None
 
     for _ in range(args.warmup):
         plan.launch(d_in, dst, stream=stream)
@@ -142,7 +144,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"6048x4024 RGGB 14-bit u16 -> RGBA8, {args.demosaic} demosaic, resident in HBM",
                    "frame_mp": round(mp, 3), "demosaic": args.demosaic, "output": "RGBA8",
-                   "max_abs_diff_vs_oracle_lsb": max_diff},
+                   "max_abs_diff_vs_oracle_lsb": max_diff, "autotuned_strip_rows": rows},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": _traffic(tag),
                      "kernel_ms_avg": round(k_avg, 5), "kernel_ms_p50": round(statistics.median(kms), 5),

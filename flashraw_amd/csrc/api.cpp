@@ -75,6 +75,16 @@ static void build_srgb16_pairs(float *pairs) {
     }
 }
 
+// The AHD blend weights w = a / (a + b) for homogeneity sums a, b in 0..72 (0.5 when both are 0),
+// f32 division as ahd.rs:389 performs it.
+static void build_blend_table(float *out) {
+    for (int a = 0; a <= 72; a++)
+        for (int b = 0; b <= 72; b++) {
+            const float fa = (float)a, fb = (float)b;
+            out[a * 73 + b] = fa + fb > 0.0f ? fa / (fa + fb) : 0.5f;
+        }
+}
+
 extern "C" int32_t frh_srgb8_table(uint8_t *out, size_t len) {
     if (!out || len < 4097) return fail(FRH_ERROR_INVALID_ARGUMENT, "sRGB table needs 4097 bytes");
     build_srgb8(out);
@@ -118,6 +128,12 @@ extern "C" int32_t frh_context_create(int32_t device, frh_context **out) {
         if ((e = hipMemcpy(c->d_srgb16, pairs.data(), pairs.size() * sizeof(float), hipMemcpyHostToDevice)) !=
             hipSuccess)
             return hip_fail(e, "upload");
+        std::vector<float> wtab(73 * 73);
+        build_blend_table(wtab.data());
+        if ((e = hipMalloc(&c->d_wtab, wtab.size() * sizeof(float))) != hipSuccess) return hip_fail(e, "hipMalloc");
+        if ((e = hipMemcpy(c->d_wtab, wtab.data(), wtab.size() * sizeof(float), hipMemcpyHostToDevice)) !=
+            hipSuccess)
+            return hip_fail(e, "upload");
         *out = c.release();
         return FRH_OK;
     } catch (const std::bad_alloc &) {
@@ -128,11 +144,12 @@ extern "C" int32_t frh_context_create(int32_t device, frh_context **out) {
 }
 
 frh_context::~frh_context() {
-    hipSetDevice(device);
-    for (auto &b : pool) hipFree(b.ptr);
-    if (d_lut8) hipFree(d_lut8);
-    if (d_srgb16) hipFree(d_srgb16);
-    if (stream) hipStreamDestroy(stream);
+    (void)hipSetDevice(device);
+    for (auto &b : pool) (void)hipFree(b.ptr);
+    if (d_lut8) (void)hipFree(d_lut8);
+    if (d_srgb16) (void)hipFree(d_srgb16);
+    if (d_wtab) (void)hipFree(d_wtab);
+    if (stream) (void)hipStreamDestroy(stream);
 }
 
 extern "C" void frh_context_destroy(frh_context *ctx) { delete ctx; }
@@ -168,7 +185,7 @@ void *frh_context::alloc(uint64_t bytes, hipError_t *err) {
         }
     }
     void *p = nullptr;
-    hipSetDevice(device);
+    (void)hipSetDevice(device);
     *err = hipMalloc(&p, bytes ? bytes : 1);
     return *err == hipSuccess ? p : nullptr;
 }
@@ -181,8 +198,8 @@ void frh_context::release(void *p, uint64_t bytes) {
         pooled += bytes;
         return;
     }
-    hipSetDevice(device);
-    hipFree(p);
+    (void)hipSetDevice(device);
+    (void)hipFree(p);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -198,7 +215,7 @@ extern "C" uint64_t frh_image_bytes(const frh_image *i) { return i ? i->bytes : 
 extern "C" int32_t frh_image_download(const frh_image *img, void *host, uint64_t bytes) {
     if (!img || !host) return fail(FRH_ERROR_INVALID_ARGUMENT, "NULL argument");
     if (bytes < img->bytes) return fail(FRH_ERROR_INVALID_ARGUMENT, "host buffer too small");
-    hipSetDevice(img->ctx->device);
+    (void)hipSetDevice(img->ctx->device);
     hipError_t e = hipMemcpy(host, img->dptr, img->bytes, hipMemcpyDeviceToHost);
     if (e != hipSuccess) return hip_fail(e, "download");
     return FRH_OK;
@@ -226,6 +243,17 @@ static void out_format(int32_t mode, uint32_t *ch, uint32_t *bps, int32_t *fmt) 
 
 static bool valid_mode(int32_t m) { return m >= OUT_RGBA8 && m <= OUT_LINEAR_F16; }
 static bool display_mode(int32_t m) { return m == OUT_RGBA8 || m == OUT_SRGB16; }
+
+static std::tuple<int, uint32_t, uint32_t, int> tune_key(bool ahd, const DevArgs &a) {
+    return std::make_tuple(ahd ? 1 : 0, a.dev_w, a.dev_h, (a.out_mode == OUT_RGBA8 && a.out_mode2 == OUT_NONE) ? 1 : 0);
+}
+
+// The autotuned strip height for this geometry, or `fallback`.
+static uint32_t tuned_rows(const frh_context *ctx, bool ahd, const DevArgs &a, uint32_t fallback) {
+    std::lock_guard<std::mutex> g(const_cast<frh_context *>(ctx)->mu);
+    auto it = ctx->tuned.find(tune_key(ahd, a));
+    return it == ctx->tuned.end() ? fallback : it->second;
+}
 
 // Everything a develop launch needs, validated (run.rs:439-646).
 struct Plan {
@@ -362,13 +390,14 @@ static int32_t plan_develop(const frh_context *ctx, const frh_frame *f, const fr
     a.out_mode = spec->output;
     a.out_mode2 = spec->output2;
     a.lut8 = ctx->d_lut8;
+    a.wtab = ctx->d_wtab;
     a.srgb16 = reinterpret_cast<const float2 *>(ctx->d_srgb16);
     p.ahd = false;
     {
         const char *impl = getenv("FLASHRAW_BILINEAR_IMPL");
         a.bil_impl = (impl && strcmp(impl, "tiled") == 0) ? 1u : 0u;
         const char *rows = getenv("FLASHRAW_BILINEAR_ROWS");
-        a.bil_rows = rows ? (uint32_t)atoi(rows) : ctx->bil_rows;
+        a.bil_rows = rows ? (uint32_t)atoi(rows) : tuned_rows(ctx, false, a, ctx->bil_rows);
     }
     if (spec->edge) {
         const frh_edge_spec *e = spec->edge;
@@ -395,7 +424,7 @@ static int32_t plan_develop(const frh_context *ctx, const frh_frame *f, const fr
         const char *impl = getenv("FLASHRAW_AHD_IMPL");
         a.ahd_impl = (impl && strcmp(impl, "tiled") == 0) ? 1u : 0u;
         const char *rows = getenv("FLASHRAW_AHD_ROWS");
-        a.ahd_rows = rows ? (uint32_t)atoi(rows) : ctx->ahd_rows;
+        a.ahd_rows = rows ? (uint32_t)atoi(rows) : tuned_rows(ctx, true, a, ctx->ahd_rows);
         p.ahd = true;
     }
     int32_t modes[2] = {spec->output, spec->output2};
@@ -549,6 +578,69 @@ extern "C" int32_t frh_plan_launch(const frh_plan *plan, const void *src, void *
     return launch_bands(plan->ctx, p, (hipStream_t)stream, nullptr);
 }
 
+extern "C" int32_t frh_plan_autotune(frh_plan *plan, const void *src, void *dst, void *dst2, void *stream,
+                                     int32_t iters, uint32_t *best) {
+    try {
+        if (!plan || !src || !dst) return fail(FRH_ERROR_INVALID_ARGUMENT, "NULL argument");
+        if (plan->p.a.out_mode2 != OUT_NONE && !dst2) return fail(FRH_ERROR_INVALID_ARGUMENT, "second output is NULL");
+        if (iters < 1) iters = 1;
+        frh_context *ctx = plan->ctx;
+        hipStream_t st = (hipStream_t)stream;
+        const bool ahd = plan->p.ahd;
+        std::vector<uint32_t> cands;
+        if (ahd)
+            for (uint32_t r = 16; r <= 128; r += 4) cands.push_back(r);
+        else
+            for (uint32_t r = 8; r <= 64; r += 4) cands.push_back(r);
+        hipEvent_t e0, e1;
+        if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+            return fail(FRH_ERROR_GPU_UNAVAILABLE, "hipEventCreate failed");
+        float best_ms = 1e30f;
+        uint32_t best_rows = ahd ? plan->p.a.ahd_rows : plan->p.a.bil_rows;
+        int32_t rc = FRH_OK;
+        for (uint32_t r : cands) {
+            Plan p = plan->p;
+            if (ahd)
+                p.a.ahd_rows = r;
+            else
+                p.a.bil_rows = r;
+            p.a.src = static_cast<const uint16_t *>(src) + plan->src_off;
+            p.a.dst = dst;
+            p.a.dst2 = dst2;
+            if ((rc = launch_bands(ctx, p, st, nullptr)) != FRH_OK) break;  // warm
+            (void)hipEventRecord(e0, st);
+            for (int i = 0; i < iters && rc == FRH_OK; i++) rc = launch_bands(ctx, p, st, nullptr);
+            (void)hipEventRecord(e1, st);
+            if (rc != FRH_OK) break;
+            if (hipEventSynchronize(e1) != hipSuccess) {
+                rc = fail(FRH_ERROR_GPU_UNAVAILABLE, "autotune launch failed");
+                break;
+            }
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            if (ms < best_ms) {
+                best_ms = ms;
+                best_rows = r;
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (rc != FRH_OK) return rc;
+        if (ahd)
+            plan->p.a.ahd_rows = best_rows;
+        else
+            plan->p.a.bil_rows = best_rows;
+        {
+            std::lock_guard<std::mutex> g(ctx->mu);
+            ctx->tuned[tune_key(ahd, plan->p.a)] = best_rows;
+        }
+        if (best) *best = best_rows;
+        return FRH_OK;
+    } catch (...) {
+        return fail(FRH_ERROR_INTERNAL, "unexpected exception");
+    }
+}
+
 extern "C" uint64_t frh_plan_output_bytes(const frh_plan *plan, int32_t which) {
     if (!plan || which < 0 || which > 1) return 0;
     return plan->p.bytes[which];
@@ -571,7 +663,7 @@ extern "C" int32_t frh_run_develop(frh_context *ctx, const frh_frame *frame, con
         int32_t rc = plan_develop(ctx, frame, info, spec, p);
         if (rc) return rc;
         if (spec->output2 != OUT_NONE && !out2) return fail(FRH_ERROR_INVALID_ARGUMENT, "second output needs out2");
-        hipSetDevice(ctx->device);
+        (void)hipSetDevice(ctx->device);
         hipError_t e;
         auto t0 = std::chrono::steady_clock::now();
         // Upload: the window's rows of the raster (run.rs:915-923), unless resident.
@@ -580,7 +672,7 @@ extern "C" int32_t frh_run_develop(frh_context *ctx, const frh_frame *frame, con
         std::vector<void *> scratch;
         auto cleanup = [&]() {
             if (d_in) ctx->release(d_in, in_bytes);
-            for (void *s : scratch) hipFree(s);
+            for (void *s : scratch) (void)hipFree(s);
         };
         if (!frame->data_on_device) {
             const uint64_t row = (uint64_t)frame->raster_width;
@@ -599,17 +691,22 @@ extern "C" int32_t frh_run_develop(frh_context *ctx, const frh_frame *frame, con
             void *d = nullptr;
             if ((e = hipMalloc(&d, p.a.n_rowd * 4)) != hipSuccess) { cleanup(); return hip_fail(e, "hipMalloc"); }
             scratch.push_back(d);
-            hipMemcpyAsync(d, info->black_row_deltas, p.a.n_rowd * 4, hipMemcpyHostToDevice, ctx->stream);
+            if ((e = hipMemcpyAsync(d, info->black_row_deltas, p.a.n_rowd * 4, hipMemcpyHostToDevice, ctx->stream)) !=
+                hipSuccess) { cleanup(); return hip_fail(e, "upload"); }
             p.a.rowd = static_cast<const float *>(d);
         }
         if (p.a.n_cold) {
             void *d = nullptr;
             if ((e = hipMalloc(&d, p.a.n_cold * 4)) != hipSuccess) { cleanup(); return hip_fail(e, "hipMalloc"); }
             scratch.push_back(d);
-            hipMemcpyAsync(d, info->black_col_deltas, p.a.n_cold * 4, hipMemcpyHostToDevice, ctx->stream);
+            if ((e = hipMemcpyAsync(d, info->black_col_deltas, p.a.n_cold * 4, hipMemcpyHostToDevice, ctx->stream)) !=
+                hipSuccess) { cleanup(); return hip_fail(e, "upload"); }
             p.a.cold = static_cast<const float *>(d);
         }
-        hipStreamSynchronize(ctx->stream);
+        if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) {
+            cleanup();
+            return hip_fail(e, "upload");
+        }
         double upload_ms = ms_since(t0);
         auto t1 = std::chrono::steady_clock::now();
         frh_image *imgs[2] = {nullptr, nullptr};
@@ -625,18 +722,17 @@ extern "C" int32_t frh_run_develop(frh_context *ctx, const frh_frame *frame, con
         }
         p.a.dst = imgs[0]->dptr;
         p.a.dst2 = imgs[1] ? imgs[1]->dptr : nullptr;
-        hipEvent_t ev0, ev1;
-        hipEventCreate(&ev0);
-        hipEventCreate(&ev1);
-        hipEventRecord(ev0, ctx->stream);
+        hipEvent_t ev0 = nullptr, ev1 = nullptr;
+        const bool timed = hipEventCreate(&ev0) == hipSuccess && hipEventCreate(&ev1) == hipSuccess;
+        if (timed) (void)hipEventRecord(ev0, ctx->stream);
         uint32_t bands = 0;
         rc = launch_bands(ctx, p, ctx->stream, &bands);
-        hipEventRecord(ev1, ctx->stream);
+        if (timed) (void)hipEventRecord(ev1, ctx->stream);
         e = hipStreamSynchronize(ctx->stream);
         float kms = 0.0f;
-        hipEventElapsedTime(&kms, ev0, ev1);
-        hipEventDestroy(ev0);
-        hipEventDestroy(ev1);
+        if (timed && hipEventElapsedTime(&kms, ev0, ev1) != hipSuccess) kms = 0.0f;
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
         cleanup();
         if (rc == FRH_OK && e != hipSuccess) rc = hip_fail(e, "develop");
         if (rc != FRH_OK) {
@@ -648,7 +744,7 @@ extern "C" int32_t frh_run_develop(frh_context *ctx, const frh_frame *frame, con
             timings->upload_ms = upload_ms;
             timings->compute_ms = ms_since(t1);
             timings->kernel_ms = kms;
-            timings->has_kernel_ms = 1;
+            timings->has_kernel_ms = timed ? 1 : 0;
             timings->bands = bands;
         }
         *out = imgs[0];

@@ -40,59 +40,52 @@ __device__ __forceinline__ Step step_left(Step s) { return Step{from_left(s.l), 
 __device__ __forceinline__ Step step_right(Step s) { return Step{from_right(s.l), from_right(s.c)}; }
 
 // Everything a lane carries from one row to the next. Index k: 0 the even column, 1 the odd one.
+// The walker runs in trips of six rows; inside a trip every buffer is a ring indexed with the trip
+// position T (a compile-time constant), so no register is ever copied from one row to the next.
 struct Walk {
-    float m[6][2];       // balanced mosaic, rows R-5 .. R
-    float g[2][2][2];    // [cand][row R-3, R-2][k] green candidates (0 H, 1 V)
-    float df[2][3][2];   // [cand][row R-4, R-3, R-2][k] mosaic minus green
-    float dfl[2][3];     // [cand][row] df of the odd column of lane - 1
-    float dfr[2][3];     // [cand][row] df of the even column of lane + 1
-    float lab[2][3][2];  // [cand][L, a, b][k] of row R-4
-    Step sd[2][2], sdl[2][2], sdr[2][2];  // [cand][k] steps from row R-5 down / down-left / down-right
-    Step sr[2][2];                        // [cand][k] steps of row R-4 to the right
-    uint32_t hom[2][2];  // [row R-6, R-5][k] homogeneity counts, H + (V << 16)
-    float u[3][2][2][2]; // [row R-5, R-4, R-3][cand][k][2] the two interpolated channels
-    uint32_t pf[2];       // raw pairs fetched two rows ahead, slot = row parity (consumed in place)
+    float m[6][2];          // balanced mosaic, row R - k in slot (T - k) mod 6
+    float g[2][2][2];       // [slot][cand][k] green candidates (0 H, 1 V): R-2 in T mod 2, R-3 in the other
+    float df[3][2][2];      // [slot][cand][k] mosaic minus green: R-2 in T mod 3, R-3 in T+2, R-4 in T+1
+    float dfl[3][2];        // [slot][cand] df of the odd column of lane - 1
+    float dfr[3][2];        // [slot][cand] df of the even column of lane + 1
+    float lab[2][2][3][2];  // [slot][cand][L, a, b][k] of a row: R-3 in T mod 2, R-4 in the other
+    Step sd[2][2][2], sdl[2][2][2], sdr[2][2][2];  // [slot][cand][k] steps down / down-left / down-right
+    Step sr[2][2][2];                              // [slot][cand][k] steps to the right
+    uint32_t hom[2][2];     // [slot][k] homogeneity counts H + (V << 16): R-6 in T mod 2, R-5 in the other
+    float u[3][2][2][2];    // [slot][cand][k][2] interpolated channels: R-3 in T mod 3, R-4 T+2, R-5 T+1
+    uint32_t pf[2];         // raw pairs fetched two rows ahead, slot = row parity (consumed in place)
 };
 
-
-// One walker iteration: load mosaic row R (parity PY) and advance every stage.
-template <int RP, int PY, bool FAST, bool HOT>
-__device__ __forceinline__ void walk_row(const DevArgs &a, const Lds &s, Walk &W, int R, int xs, int lane,
-                                         int out_row0, int out_row1, int out_x0, int out_x1, bool in0, bool in1,
-                                         __amdgpu_buffer_rsrc_t rs, __amdgpu_buffer_rsrc_t rd) {
+// One walker row at trip position T: load mosaic row R (parity T & 1) and advance every stage.
+template <int RP, int T, bool FAST, bool HOT>
+__device__ __forceinline__ void walk_row(const DevArgs &a, const Lds &s, const float *wt, Walk &W, int R, int xs,
+                                         int lane, int out_row0, int out_row1, int out_x0, int out_x1, bool in0,
+                                         bool in1, __amdgpu_buffer_rsrc_t rs, __amdgpu_buffer_rsrc_t rd) {
+    constexpr int PY = T & 1;
+    // ring slots
+    constexpr int M0 = T % 6, M1 = (T + 5) % 6, M2 = (T + 4) % 6, M3 = (T + 3) % 6, M4 = (T + 2) % 6,
+                  M5 = (T + 1) % 6;                        // rows R, R-1, ..., R-5
+    constexpr int G2 = T % 2, G3 = (T + 1) % 2;            // green rows R-2, R-3
+    constexpr int D2 = T % 3, D3 = (T + 2) % 3, D4 = (T + 1) % 3;  // df rows R-2, R-3, R-4
+    constexpr int LQ = T % 2, LP = (T + 1) % 2;            // lab / steps: new row (R-3), previous row
+    constexpr int H6 = T % 2, H5 = (T + 1) % 2;            // hom rows R-6, R-5 (R-4 goes to H6)
+    constexpr int U3 = T % 3, U5 = (T + 1) % 3;            // interpolated rows R-3 (new), R-5
     const int x0 = xs + 2 * lane;
-    // ---- load row R ----
-#pragma unroll
-    for (int i = 0; i < 5; i++) {
-        W.m[i][0] = W.m[i + 1][0];
-        W.m[i][1] = W.m[i + 1][1];
-    }
-    // balanced mosaic, clipped at 1 (ahd.rs:143-151): min(norm * wb[c], 1); 0 outside the frame
+
+    // ---- balanced mosaic of row R, clipped at 1 (ahd.rs:143-151): min(norm * wb[c], 1) ----
     {
         const uint32_t raw = W.pf[PY];
         W.pf[PY] = fetch_pair<HOT>(a, rs, x0, R + 2);
         float n0, n1;
         norm_pair<PY, HOT>(a, s, raw, x0, R, R >= 0 && R < (int)a.height, in0, in1, n0, n1);
-        W.m[5][0] = fminf(n0 * a.wb[site<RP>(PY, 0)], 1.0f);
-        W.m[5][1] = fminf(n1 * a.wb[site<RP>(PY, 1)], 1.0f);
+        W.m[M0][0] = fminf(n0 * a.wb[site<RP>(PY, 0)], 1.0f);
+        W.m[M0][1] = fminf(n1 * a.wb[site<RP>(PY, 1)], 1.0f);
     }
 
-    // ---- green candidates of row R-2 (ahd.rs:225-244); m index 3 ----
+    // ---- green candidates of row R-2 (ahd.rs:225-244) ----
     constexpr int PG = PY;
-#pragma unroll
-    for (int d = 0; d < 2; d++) {
-        W.g[d][0][0] = W.g[d][1][0];
-        W.g[d][0][1] = W.g[d][1][1];
-#pragma unroll
-        for (int r = 0; r < 2; r++) {
-            W.df[d][r][0] = W.df[d][r + 1][0];
-            W.df[d][r][1] = W.df[d][r + 1][1];
-            W.dfl[d][r] = W.dfl[d][r + 1];
-            W.dfr[d][r] = W.dfr[d][r + 1];
-        }
-    }
     {
-        const float me = W.m[3][0], mo = W.m[3][1];
+        const float me = W.m[M2][0], mo = W.m[M2][1];
         const float pe = from_left(me), po = from_left(mo);   // columns x0 - 2, x0 - 1
         const float ne = from_right(me), no = from_right(mo); // columns x0 + 2, x0 + 3
 #pragma unroll
@@ -102,49 +95,39 @@ __device__ __forceinline__ void walk_row(const DevArgs &a, const Lds &s, Walk &W
             if (site<RP>(PG, k) != 1) {
                 const float l = k == 0 ? po : me, r = k == 0 ? mo : ne;
                 const float l2 = k == 0 ? pe : po, r2 = k == 0 ? ne : no;
-                const float u = W.m[2][k], dn = W.m[4][k], u2 = W.m[1][k], d2 = W.m[5][k];
+                const float u = W.m[M3][k], dn = W.m[M1][k], u2 = W.m[M4][k], d2 = W.m[M0][k];
                 gh = fminf(fmaxf((l + r) * 0.5f + (2.0f * c - l2 - r2) * 0.25f, fminf(l, r)), fmaxf(l, r));
                 gv = fminf(fmaxf((u + dn) * 0.5f + (2.0f * c - u2 - d2) * 0.25f, fminf(u, dn)), fmaxf(u, dn));
             }
-            W.g[0][1][k] = gh;
-            W.g[1][1][k] = gv;
-            W.df[0][2][k] = c - gh;
-            W.df[1][2][k] = c - gv;
+            W.g[G2][0][k] = gh;
+            W.g[G2][1][k] = gv;
+            W.df[D2][0][k] = c - gh;
+            W.df[D2][1][k] = c - gv;
         }
 #pragma unroll
         for (int d = 0; d < 2; d++) {
-            W.dfl[d][2] = from_left(W.df[d][2][1]);
-            W.dfr[d][2] = from_right(W.df[d][2][0]);
+            W.dfl[D2][d] = from_left(W.df[D2][d][1]);
+            W.dfr[D2][d] = from_right(W.df[D2][d][0]);
         }
     }
 
-    // ---- candidate RGB and Lab of row R-3 (ahd.rs:249-298); m index 2 ----
+    // ---- candidate RGB and Lab of row R-3 (ahd.rs:249-298) ----
     constexpr int PL = PY ^ 1;
-    float lq[2][3][2];
-#pragma unroll
-    for (int r = 0; r < 2; r++)
-#pragma unroll
-        for (int d = 0; d < 2; d++)
-#pragma unroll
-            for (int k = 0; k < 2; k++) {
-                W.u[r][d][k][0] = W.u[r + 1][d][k][0];
-                W.u[r][d][k][1] = W.u[r + 1][d][k][1];
-            }
 #pragma unroll
     for (int k = 0; k < 2; k++) {
-        const float cm = W.m[2][k];
+        const float cm = W.m[M3][k];
         const float sm = sqrtf(fmaxf(cm, 0.0f));  // the measured channel, the same in both candidates
         const int col = site<RP>(PL, k);
 #pragma unroll
         for (int d = 0; d < 2; d++) {
-            const float g = W.g[d][0][k];
-            // colour differences around (R-3, column k): rows 0 (R-4), 1 (R-3), 2 (R-2)
-            auto dleft = [&](int r) { return k == 0 ? W.dfl[d][r] : W.df[d][r][0]; };
-            auto dright = [&](int r) { return k == 1 ? W.dfr[d][r] : W.df[d][r][1]; };
+            const float g = W.g[G3][d][k];
+            // colour differences around (R-3, column k) in rows R-4, R-3, R-2
+            auto dleft = [&](int slot) { return k == 0 ? W.dfl[slot][d] : W.df[slot][d][0]; };
+            auto dright = [&](int slot) { return k == 1 ? W.dfr[slot][d] : W.df[slot][d][1]; };
             float red, gg, blue, u0, u1;
             if (col == 1) {
-                const float row = g + 0.5f * (dleft(1) + dright(1));
-                const float clm = g + 0.5f * (W.df[d][0][k] + W.df[d][2][k]);
+                const float row = g + 0.5f * (dleft(D3) + dright(D3));
+                const float clm = g + 0.5f * (W.df[D4][d][k] + W.df[D2][d][k]);
                 const bool hr = site<RP>(PL, k ^ 1) == 0;  // red neighbours along the row
                 red = hr ? row : clm;
                 blue = hr ? clm : row;
@@ -152,7 +135,7 @@ __device__ __forceinline__ void walk_row(const DevArgs &a, const Lds &s, Walk &W
                 u0 = red;
                 u1 = blue;
             } else {
-                const float diag = g + 0.25f * (dleft(0) + dright(0) + dleft(2) + dright(2));
+                const float diag = g + 0.25f * (dleft(D4) + dright(D4) + dleft(D2) + dright(D2));
                 red = col == 0 ? cm : diag;
                 blue = col == 0 ? diag : cm;
                 gg = g;
@@ -173,59 +156,59 @@ __device__ __forceinline__ void walk_row(const DevArgs &a, const Lds &s, Walk &W
                 sg = sqrtf(fmaxf(gg, 0.0f));
                 sb = sm;
             }
-            lq[d][0][k] = (sr + 2.0f * sg + sb) * 0.25f;
-            lq[d][1][k] = sr - sg;
-            lq[d][2][k] = sb - sg;
-            W.u[2][d][k][0] = u0;
-            W.u[2][d][k][1] = u1;
+            W.lab[LQ][d][0][k] = (sr + 2.0f * sg + sb) * 0.25f;
+            W.lab[LQ][d][1][k] = sr - sg;
+            W.lab[LQ][d][2][k] = sb - sg;
+            W.u[U3][d][k][0] = u0;
+            W.u[U3][d][k][1] = u1;
         }
     }
 
     // ---- pair steps between rows R-4 (p) and R-3 (q); homogeneity of row R-4 (ahd.rs:300-351) ----
-    Step nd[2][2], ndl[2][2], ndr[2][2], nr[2][2];
 #pragma unroll
     for (int d = 0; d < 2; d++) {
+        const float(&p)[3][2] = W.lab[LP][d];
+        const float(&q)[3][2] = W.lab[LQ][d];
         // q neighbours: even column of lane + 1, odd column of lane - 1
         float qn[3], qp[3];
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            qn[c] = from_right(lq[d][c][0]);
-            qp[c] = from_left(lq[d][c][1]);
+            qn[c] = from_right(q[c][0]);
+            qp[c] = from_left(q[c][1]);
         }
-        nd[d][0] = step(W.lab[d][0][0], W.lab[d][1][0], W.lab[d][2][0], lq[d][0][0], lq[d][1][0], lq[d][2][0]);
-        nd[d][1] = step(W.lab[d][0][1], W.lab[d][1][1], W.lab[d][2][1], lq[d][0][1], lq[d][1][1], lq[d][2][1]);
-        ndr[d][0] = step(W.lab[d][0][0], W.lab[d][1][0], W.lab[d][2][0], lq[d][0][1], lq[d][1][1], lq[d][2][1]);
-        ndr[d][1] = step(W.lab[d][0][1], W.lab[d][1][1], W.lab[d][2][1], qn[0], qn[1], qn[2]);
-        ndl[d][0] = step(W.lab[d][0][0], W.lab[d][1][0], W.lab[d][2][0], qp[0], qp[1], qp[2]);
-        ndl[d][1] = step(W.lab[d][0][1], W.lab[d][1][1], W.lab[d][2][1], lq[d][0][0], lq[d][1][0], lq[d][2][0]);
-        nr[d][0] = step(lq[d][0][0], lq[d][1][0], lq[d][2][0], lq[d][0][1], lq[d][1][1], lq[d][2][1]);
-        nr[d][1] = step(lq[d][0][1], lq[d][1][1], lq[d][2][1], qn[0], qn[1], qn[2]);
+        W.sd[LQ][d][0] = step(p[0][0], p[1][0], p[2][0], q[0][0], q[1][0], q[2][0]);
+        W.sd[LQ][d][1] = step(p[0][1], p[1][1], p[2][1], q[0][1], q[1][1], q[2][1]);
+        W.sdr[LQ][d][0] = step(p[0][0], p[1][0], p[2][0], q[0][1], q[1][1], q[2][1]);
+        W.sdr[LQ][d][1] = step(p[0][1], p[1][1], p[2][1], qn[0], qn[1], qn[2]);
+        W.sdl[LQ][d][0] = step(p[0][0], p[1][0], p[2][0], qp[0], qp[1], qp[2]);
+        W.sdl[LQ][d][1] = step(p[0][1], p[1][1], p[2][1], q[0][0], q[1][0], q[2][0]);
+        W.sr[LQ][d][0] = step(q[0][0], q[1][0], q[2][0], q[0][1], q[1][1], q[2][1]);
+        W.sr[LQ][d][1] = step(q[0][1], q[1][1], q[2][1], qn[0], qn[1], qn[2]);
     }
-    // the eight neighbour steps of each pixel of row p, per candidate
+    // the eight neighbour steps of each pixel of row p, per candidate: up-left, up, up-right, left,
+    // right (steps of the previous rows, LP), down-left, down, down-right (this row's, LQ)
     Step nb[2][2][8];
 #pragma unroll
     for (int d = 0; d < 2; d++) {
-        const Step ul_e = step_left(W.sdr[d][1]);  // down-right from (p-1, x0-1)
-        const Step ur_o = step_right(W.sdl[d][0]); // down-left from (p-1, x0+2)
-        const Step l_e = step_left(W.sr[d][1]);    // right step of (p, x0-1)
-        // even column
+        const Step ul_e = step_left(W.sdr[LP][d][1]);  // down-right from (p-1, x0-1)
+        const Step ur_o = step_right(W.sdl[LP][d][0]); // down-left from (p-1, x0+2)
+        const Step l_e = step_left(W.sr[LP][d][1]);    // right step of (p, x0-1)
         nb[d][0][0] = ul_e;
-        nb[d][0][1] = W.sd[d][0];
-        nb[d][0][2] = W.sdl[d][1];
+        nb[d][0][1] = W.sd[LP][d][0];
+        nb[d][0][2] = W.sdl[LP][d][1];
         nb[d][0][3] = l_e;
-        nb[d][0][4] = W.sr[d][0];
-        nb[d][0][5] = ndl[d][0];
-        nb[d][0][6] = nd[d][0];
-        nb[d][0][7] = ndr[d][0];
-        // odd column
-        nb[d][1][0] = W.sdr[d][0];
-        nb[d][1][1] = W.sd[d][1];
+        nb[d][0][4] = W.sr[LP][d][0];
+        nb[d][0][5] = W.sdl[LQ][d][0];
+        nb[d][0][6] = W.sd[LQ][d][0];
+        nb[d][0][7] = W.sdr[LQ][d][0];
+        nb[d][1][0] = W.sdr[LP][d][0];
+        nb[d][1][1] = W.sd[LP][d][1];
         nb[d][1][2] = ur_o;
-        nb[d][1][3] = W.sr[d][0];
-        nb[d][1][4] = W.sr[d][1];
-        nb[d][1][5] = ndl[d][1];
-        nb[d][1][6] = nd[d][1];
-        nb[d][1][7] = ndr[d][1];
+        nb[d][1][3] = W.sr[LP][d][0];
+        nb[d][1][4] = W.sr[LP][d][1];
+        nb[d][1][5] = W.sdl[LQ][d][1];
+        nb[d][1][6] = W.sd[LQ][d][1];
+        nb[d][1][7] = W.sdr[LQ][d][1];
     }
     uint32_t hp[2];
 #pragma unroll
@@ -243,25 +226,12 @@ __device__ __forceinline__ void walk_row(const DevArgs &a, const Lds &s, Walk &W
         }
         hp[k] = ch | (cv << 16);
     }
-#pragma unroll
-    for (int d = 0; d < 2; d++)
-#pragma unroll
-        for (int k = 0; k < 2; k++) {
-            W.sd[d][k] = nd[d][k];
-            W.sdl[d][k] = ndl[d][k];
-            W.sdr[d][k] = ndr[d][k];
-            W.sr[d][k] = nr[d][k];
-#pragma unroll
-            for (int c = 0; c < 3; c++) W.lab[d][c][k] = lq[d][c][k];
-        }
 
     // ---- blend, finish and store row R-5 (ahd.rs:353-392) ----
-    const uint32_t v0 = W.hom[0][0] + W.hom[1][0] + hp[0];
-    const uint32_t v1 = W.hom[0][1] + W.hom[1][1] + hp[1];
-    W.hom[0][0] = W.hom[1][0];
-    W.hom[0][1] = W.hom[1][1];
-    W.hom[1][0] = hp[0];
-    W.hom[1][1] = hp[1];
+    const uint32_t v0 = W.hom[H6][0] + W.hom[H5][0] + hp[0];
+    const uint32_t v1 = W.hom[H6][1] + W.hom[H5][1] + hp[1];
+    W.hom[H6][0] = hp[0];
+    W.hom[H6][1] = hp[1];
     // 3x3 sums (the DPP reads happen with every lane active: a lane switched off by a branch
     // would hand its neighbour a 0)
     const uint32_t sums[2] = {from_left_u(v1) + v0 + v1, v0 + v1 + from_right_u(v0)};
@@ -271,14 +241,14 @@ __device__ __forceinline__ void walk_row(const DevArgs &a, const Lds &s, Walk &W
         float o[2][3];
 #pragma unroll
         for (int k = 0; k < 2; k++) {
-            const float sh = (float)(sums[k] & 0xffffu), sv = (float)(sums[k] >> 16);
-            const float w = sh + sv > 0.0f ? sh / (sh + sv) : 0.5f;
-            const float cm = W.m[0][k];
+            // w = sh / (sh + sv), 0.5 when both are 0 (ahd.rs:388-389), read from the exact table
+            const float w = wt[(sums[k] & 0xffffu) * 73u + (sums[k] >> 16)];
+            const float cm = W.m[M5][k];
             const int col = site<RP>(PO, k);
             float rgb[2][3];
 #pragma unroll
             for (int d = 0; d < 2; d++) {
-                const float u0 = W.u[0][d][k][0], u1 = W.u[0][d][k][1];
+                const float u0 = W.u[U5][d][k][0], u1 = W.u[U5][d][k][1];
                 rgb[d][0] = col == 0 ? cm : (col == 1 ? u0 : u1);
                 rgb[d][1] = col == 1 ? cm : u0;
                 rgb[d][2] = col == 2 ? cm : u1;
@@ -312,10 +282,10 @@ __device__ __forceinline__ void walk_row(const DevArgs &a, const Lds &s, Walk &W
 }
 
 // One wave's strip: mosaic row R runs from row0 - 6 (even) until the last output row
-// out_row1 - 1 = R - 5, two rows per trip (prefetch slot = row parity).
+// out_row1 - 1 = R - 5, in trips of six rows (prefetch slot = row parity).
 template <int RP, bool FAST, bool HOT>
-__device__ __forceinline__ void walk_strip(const DevArgs &a, const Lds &s, int xs, int lane, int row0, int out_row0,
-                                           int out_row1, int out_x0, int out_x1) {
+__device__ __forceinline__ void walk_strip(const DevArgs &a, const Lds &s, const float *wt, int xs, int lane,
+                                           int row0, int out_row0, int out_row1, int out_x0, int out_x1) {
     Walk W;
     {
         float *f = reinterpret_cast<float *>(&W);
@@ -329,9 +299,18 @@ __device__ __forceinline__ void walk_strip(const DevArgs &a, const Lds &s, int x
 #pragma unroll
     for (int i = 0; i < 2; i++) W.pf[i] = fetch_pair<HOT>(a, rs, x0, row0 - 6 + i);
     const int r_end = out_row1 + 5;
-    for (int R = row0 - 6; R < r_end; R += 2) {
-        walk_row<RP, 0, FAST, HOT>(a, s, W, R, xs, lane, out_row0, out_row1, out_x0, out_x1, in0, in1, rs, rd);
-        walk_row<RP, 1, FAST, HOT>(a, s, W, R + 1, xs, lane, out_row0, out_row1, out_x0, out_x1, in0, in1, rs, rd);
+    // the last trip stops at r_end (a wave-uniform test between rows)
+    for (int R = row0 - 6; R < r_end; R += 6) {
+#define WALK_ROW(t)                                                                                            \
+    if (t > 0 && R + t >= r_end) break;                                                                       \
+    walk_row<RP, t, FAST, HOT>(a, s, wt, W, R + t, xs, lane, out_row0, out_row1, out_x0, out_x1, in0, in1, rs, rd);
+        WALK_ROW(0)
+        WALK_ROW(1)
+        WALK_ROW(2)
+        WALK_ROW(3)
+        WALK_ROW(4)
+        WALK_ROW(5)
+#undef WALK_ROW
     }
 }
 
@@ -347,13 +326,16 @@ struct WalkPlan {
 template <int RP, bool FAST>
 __global__ __launch_bounds__(WALK_WAVES * 64) void k_develop_ahd_walk(DevArgs a, WalkPlan P, uint32_t walk_blocks) {
     __shared__ Lds s;
+    __shared__ float wt[73 * 73];
     load_lds(a, s);
-    __syncthreads();
     if (blockIdx.x >= walk_blocks) {
+        __syncthreads();
         const uint32_t i = (blockIdx.x - walk_blocks) * blockDim.x + threadIdx.x;
         if (i < P.border.count) border_pixel(a, s, P.border, i);
         return;
     }
+    for (int i = threadIdx.x; i < 73 * 73; i += blockDim.x) wt[i] = a.wtab[i];
+    __syncthreads();
     const int lane = threadIdx.x & 63;
     // the wave index is the same for every lane; readfirstlane lets the compiler keep all the strip
     // bookkeeping in scalar registers
@@ -368,9 +350,9 @@ __global__ __launch_bounds__(WALK_WAVES * 64) void k_develop_ahd_walk(DevArgs a,
     const bool hot = a.simple_cal && xs >= 0 && xs + 128 <= (int)a.width && ((uintptr_t)a.src & 3u) == 0 &&
                      (a.stride & 1u) == 0;
     if (hot)
-        walk_strip<RP, FAST, true>(a, s, xs, lane, row0, out_row0, out_row1, out_x0, out_x1);
+        walk_strip<RP, FAST, true>(a, s, wt, xs, lane, row0, out_row0, out_row1, out_x0, out_x1);
     else
-        walk_strip<RP, FAST, false>(a, s, xs, lane, row0, out_row0, out_row1, out_x0, out_x1);
+        walk_strip<RP, FAST, false>(a, s, wt, xs, lane, row0, out_row0, out_row1, out_x0, out_x1);
 }
 
 template <int RP>

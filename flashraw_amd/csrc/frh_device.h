@@ -63,6 +63,7 @@ struct DevArgs {
     void *dst2;
     const uint8_t *lut8;     // 4097 entries (+ padding to 4100)
     const float2 *srgb16;    // 16384 (value, step) pairs
+    const float *wtab;       // 73 x 73 AHD blend weights a / (a + b)
     uint32_t ahd_halo;       // 6: pixels nearer the border than this are bilinear (ahd.rs:33)
     uint32_t ahd_impl;       // 0: register row walker, 1: LDS-tiled (both give the same bytes)
     uint32_t ahd_rows;       // rows per walker strip (even)

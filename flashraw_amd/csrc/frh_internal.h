@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
+#include <tuple>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -34,9 +36,12 @@ struct frh_context {
     uint64_t total_mem = 0;
     uint8_t *d_lut8 = nullptr;
     float *d_srgb16 = nullptr;
+    float *d_wtab = nullptr;  // 73 x 73 AHD blend weights
     uint64_t max_binding = 0;
     uint32_t ahd_rows = 64;  // rows per AHD walker strip
-    uint32_t bil_rows = 64;  // rows per bilinear walker strip
+    uint32_t bil_rows = 32;  // rows per bilinear walker strip
+    // strip heights found by frh_plan_autotune, by (AHD, developed width, rows, fast path)
+    std::map<std::tuple<int, uint32_t, uint32_t, int>, uint32_t> tuned;
     std::mutex mu;
     struct Block {
         void *ptr;

@@ -168,6 +168,8 @@ def lib():
             L.frh_plan_create.restype = i32
             L.frh_plan_launch.argtypes = [vp, vp, vp, vp, vp]
             L.frh_plan_launch.restype = i32
+            L.frh_plan_autotune.argtypes = [vp, vp, vp, vp, vp, i32, ctypes.POINTER(u32)]
+            L.frh_plan_autotune.restype = i32
             L.frh_plan_output_bytes.argtypes = [vp, i32]
             L.frh_plan_output_bytes.restype = u64
             L.frh_plan_destroy.argtypes = [vp]
@@ -618,6 +620,15 @@ class Plan:
                           self.ctx.stream_handle(stream))
         if rc:
             check(rc)
+
+    def autotune(self, src, dst, dst2=None, stream=None, iters: int = 3) -> int:
+        """Time the candidate strip heights on these resident buffers and keep the fastest
+        (frh_plan_autotune); returns it."""
+        best = ctypes.c_uint32()
+        check(lib().frh_plan_autotune(self._h, _device_ptr(src), _device_ptr(dst),
+                                      None if dst2 is None else _device_ptr(dst2), self.ctx.stream_handle(stream),
+                                      iters, ctypes.byref(best)))
+        return best.value
 
     def __del__(self):
         if getattr(self, "_h", None):

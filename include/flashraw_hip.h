@@ -166,6 +166,13 @@ int32_t frh_plan_create(frh_context *ctx, const frh_frame *frame, const frh_fram
 /* Develop device input `src` (the raster the frame describes) into `dst` (and `dst2` for
  * output2) on `stream` (NULL: the null stream), without synchronising. */
 int32_t frh_plan_launch(const frh_plan *plan, const void *src, void *dst, void *dst2, void *stream);
+/* Autotune the plan's strip height (frame rows one wave develops: the tile shape of the row-walker
+ * kernels). Each candidate is timed with HIP events on `stream` over `iters` launches on the given
+ * resident buffers; the fastest is kept in the plan and remembered by the context for develops of
+ * the same geometry. *best (optional) receives it. No reference counterpart: the reference fixes
+ * 16x16 workgroups (crates/flashraw-gpu/shaders/develop.wgsl:185). */
+int32_t frh_plan_autotune(frh_plan *plan, const void *src, void *dst, void *dst2, void *stream, int32_t iters,
+                          uint32_t *best);
 /* Bytes of the plan's first and second outputs. */
 uint64_t frh_plan_output_bytes(const frh_plan *plan, int32_t which);
 void frh_plan_destroy(frh_plan *plan);

@@ -214,3 +214,28 @@ def test_walker_equals_tiled(ctx, monkeypatch, cfa, demosaic):
             monkeypatch.setenv(f"FLASHRAW_{knob}_IMPL", "tiled")
             tiled = ctx.run(frame, info, F.GpuOp.Develop(spec)).image.download()
             assert np.array_equal(walk, tiled), (origin, crop, rows, out)
+
+
+@pytest.mark.parametrize("demosaic", ["bilinear", "ahd"])
+def test_autotuned_plan(ctx, demosaic):
+    """The strip height the autotuner keeps changes timing only: the plan and a later develop of
+    the same geometry (which reuses the tuned height) give the oracle's bytes."""
+    import torch
+    w, h = 1030, 777
+    mos = synth.mosaic(w, h, "GRBG", seed=23, stripes=40)
+    info = _info(w, h, "GRBG")
+    spec = F.DevelopSpec(F.Rect.full(w, h), wb=synth.WB, cam_to_rgb=MATRIX,
+                         edge=F.ahd_spec(info) if demosaic == "ahd" else None)
+    d_in = torch.from_numpy(mos.view(np.int16)).to("cuda")
+    dst = torch.empty((h, w, 4), dtype=torch.uint8, device="cuda")
+    plan = ctx.prepare(F.GpuFrame.from_u16(d_in, w, h), info, spec)
+    stream = torch.cuda.Stream()
+    rows = plan.autotune(d_in, dst, stream=stream, iters=2)
+    assert (16 <= rows <= 128) if demosaic == "ahd" else (8 <= rows <= 64)
+    dst.zero_()
+    plan.launch(d_in, dst, stream=stream)
+    stream.synchronize()
+    want = O.develop(mos, "GRBG", black=(1, 1, 1, [512.0]), wb=synth.WB, matrix=MATRIX, demosaic=demosaic)
+    assert np.array_equal(dst.cpu().numpy(), want)
+    again = ctx.run(F.GpuFrame.from_u16(mos, w, h), info, F.GpuOp.Develop(spec)).image.download()
+    assert np.array_equal(again, want)
