@@ -5,7 +5,7 @@ synthetic scene with Poisson-Gaussian noise), AHD demosaic, WB (2.1, 1, 1.6), a 
 camera->sRGB matrix, RGBA8 output. A step is one develop of that frame with its input already
 resident in HBM and the output written to HBM. Prints one JSON line.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--demosaic ahd|bilinear]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--demosaic ahd|bilinear] [--strip-rows R]
   python bench.py --mode copy   # a plain device copy of 1 GiB (PMC byte-counter calibration)
 """
 import argparse
@@ -30,6 +30,9 @@ def _parse():
     p.add_argument("--demosaic", default="ahd", choices=["ahd", "bilinear"])
     p.add_argument("--mode", default="develop", choices=["develop", "copy"])
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    p.add_argument("--strip-rows", type=int, default=0,
+                   help="use this strip height instead of autotuning (profiled runs, where the autotuner's "
+                        "own launches would mix into the per-kernel statistics)")
     return p.parse_args()
 
 
@@ -73,9 +76,11 @@ def main():
     import flashraw_amd as F
     from flashraw_amd import synth
 
+    if args.strip_rows:
+        os.environ["FLASHRAW_AHD_ROWS" if args.demosaic == "ahd" else "FLASHRAW_BILINEAR_ROWS"] = str(args.strip_rows)
     ctx = F.context()
     W, H = 6048, 4024
-    mos = synth.config0()
+    mos = synth.frame_24mp()
     info = F.FrameInfo(W, H, F.CfaPattern.RGGB, F.BlackLevel.uniform(512.0), F.WhiteLevel.uniform(16383))
     edge = F.ahd_spec(info) if args.demosaic == "ahd" else None
     spec = F.DevelopSpec(F.Rect.full(W, H), wb=synth.WB, cam_to_rgb=synth.MATRIX, output=F.DevelopOutput.Rgba8,
@@ -86,7 +91,7 @@ def main():
     plan = ctx.prepare(frame, info, spec)
     stream = torch.cuda.Stream()  # the launch stream; the HIP events below are recorded on it
     # one-time setup outside the timed region: the autotuner picks the strip height
-    rows = plan.autotune(d_in, dst, stream=stream) if os.environ.get("FLASHRAW_NO_AUTOTUNE") is None else None
+    rows = args.strip_rows or plan.autotune(d_in, dst, stream=stream)
 
     for _ in range(args.warmup):
         plan.launch(d_in, dst, stream=stream)
@@ -128,7 +133,7 @@ def main():
         cpu = {"value": round(mp / cpu_s, 2), "unit": "MP/s", "cores": threads, "kind": "port",
                "sample": f"the same 6048x4024 frame, median of 5 runs after 1 warm-up, "
                          f"{args.demosaic} RGBA8, C restatement of the reference CPU path"}
-    tag = f"develop_{args.demosaic}_config0"
+    tag = f"develop_{args.demosaic}_6048x4024_rgba8"
     line = {
         "metric": "MP/s RAW->sRGB develop (24 MP AHD, % HBM roofline)",
         "value": round(value, 1),
@@ -144,7 +149,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"6048x4024 RGGB 14-bit u16 -> RGBA8, {args.demosaic} demosaic, resident in HBM",
                    "frame_mp": round(mp, 3), "demosaic": args.demosaic, "output": "RGBA8",
-                   "max_abs_diff_vs_oracle_lsb": max_diff, "autotuned_strip_rows": rows},
+                   "max_abs_diff_vs_oracle_lsb": max_diff, "strip_rows": rows, "strip_rows_autotuned": not args.strip_rows},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": _traffic(tag),
                      "kernel_ms_avg": round(k_avg, 5), "kernel_ms_p50": round(statistics.median(kms), 5),

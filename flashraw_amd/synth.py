@@ -43,17 +43,17 @@ def mosaic(width, height, cfa="RGGB", black=512.0, white=16383.0, gain=0.5, read
     return out
 
 
-def config0(seed=0, **kw):
+def frame_24mp(seed=0, **kw):
     """6048x4024 RGGB, 14-bit, black 512, white 16383, gain 0.5 e-/DN, read sigma 3 DN."""
     return mosaic(6048, 4024, "RGGB", 512.0, 16383.0, 0.5, 3.0, 200, 50, 0, seed, **kw)
 
 
-def config1(seed=1, **kw):
+def frame_45mp_stripes(seed=1, **kw):
     """8256x5504 BGGR with 500 stripe patches (periods 2-6 px)."""
     return mosaic(8256, 5504, "BGGR", 512.0, 16383.0, 0.5, 3.0, 200, 50, 500, seed, **kw)
 
 
-def config2(seed=2, **kw):
+def frame_24mp_noisy(seed=2, **kw):
     """The 6048x4024 RGGB frame with read sigma 12 DN (denoise workload)."""
     return mosaic(6048, 4024, "RGGB", 512.0, 16383.0, 0.5, 12.0, 200, 50, 0, seed, **kw)
 

@@ -178,20 +178,20 @@ def test_errors(ctx):
 
 
 @pytest.mark.parametrize("demosaic", ["bilinear", "ahd"])
-def test_config0_full_frame(ctx, demosaic):
+def test_24mp_full_frame(ctx, demosaic):
     """The 24 MP workload (6048x4024 RGGB 14-bit) against the oracle, every byte."""
-    mos = synth.config0()
+    mos = synth.frame_24mp()
     run = _compare(ctx, mos, "RGGB", demosaic=demosaic, output=F.DevelopOutput.Rgba8, wb=synth.WB,
-                   matrix=synth.MATRIX, what="config0")
-    print(f"config0 {demosaic}: kernel {run.timings.kernel_ms:.3f} ms")
+                   matrix=synth.MATRIX, what="24mp")
+    print(f"24 MP {demosaic}: kernel {run.timings.kernel_ms:.3f} ms")
 
 
-def test_config1_full_frame_dual_output(ctx):
+def test_45mp_stripes_dual_output(ctx):
     """The 45 MP BGGR workload with dense stripes: linear 16-bit and RGBA8 from one AHD pass."""
-    mos = synth.config1()
+    mos = synth.frame_45mp_stripes()
     run = _compare(ctx, mos, "BGGR", demosaic="ahd", output=F.DevelopOutput.LinearRgb16,
-                   output2=F.DevelopOutput.Rgba8, wb=synth.WB, matrix=synth.MATRIX, what="config1")
-    print(f"config1 AHD dual: kernel {run.timings.kernel_ms:.3f} ms")
+                   output2=F.DevelopOutput.Rgba8, wb=synth.WB, matrix=synth.MATRIX, what="45mp")
+    print(f"45 MP AHD dual: kernel {run.timings.kernel_ms:.3f} ms")
 
 
 @pytest.mark.parametrize("cfa", ["RGGB", "BGGR", "GRBG", "GBRG"])
