@@ -889,13 +889,17 @@ int fro_dct_denoise(const uint16_t *mosaic, uint32_t width, uint32_t height, con
         float bl = black[plane];
         memset(acc, 0, pw * ph * sizeof(double));
         memset(cnt, 0, pw * ph * sizeof(float));
-        /* Block rows are processed in `groups` interleaved sets whose members lie at least 8 rows
-         * apart (the last position may follow its predecessor by less than `step`), so blocks
-         * running concurrently never overlap and every sum has a fixed order. */
+        /* Block rows on the regular grid (multiples of `step`) are processed in `groups`
+         * interleaved sets whose members lie more than 8 rows apart, so blocks running
+         * concurrently never overlap and every sum has a fixed order. The last position, which
+         * may follow its predecessor by less than `step`, runs in a pass of its own. */
         size_t groups = 8 / step + 1;
-        for (size_t par = 0; par < groups; par++) {
+        size_t ngrid = (ph - 8) / step + 1;
+        for (size_t par = 0; par <= groups; par++) {
+            size_t first = par < groups ? par : ngrid, end = par < groups ? ngrid : ny;
+            size_t stride = par < groups ? groups : 1;
 #pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
-            for (size_t by = par; by < ny; by += groups) {
+            for (size_t by = first; by < end; by += stride) {
                 size_t y0 = ys[by];
                 for (size_t bx = 0; bx < nx; bx++) {
                     size_t x0 = xs[bx];

@@ -1,0 +1,337 @@
+"""A second, independent statement of the reference's CPU develop maths in numpy float32.
+
+The C oracle (oracle/oracle.c) follows the reference's code structure: rows cached in strips,
+AHD in 64-row bands of 192-column chunks, plane loops with masks. The functions here state the
+same arithmetic per pixel over whole-frame planes, with no strips, chunks or masks, so that a
+mistake in either statement (a dropped term, a wrong neighbour, a swapped candidate, a border
+off by one) shows up as a difference between the two. numpy float32 arithmetic is IEEE single
+precision with correct rounding and no contraction, and every expression below keeps the
+reference's evaluation order, so the two statements must agree bit for bit.
+
+Test infrastructure only (imported by tests/).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+F32 = np.float32
+HALO = 6  # crates/flashraw/src/ahd.rs:37
+
+XTRANS = "GGRGGB/GGBGGR/BRGRBG/GGBGGR/GGRGGB/RBGBRG"
+_CODE = {"R": 0, "G": 1, "B": 2}
+BAYER = {"RGGB": [[0, 1], [1, 2]], "BGGR": [[2, 1], [1, 0]], "GRBG": [[1, 0], [2, 1]], "GBRG": [[1, 2], [0, 1]]}
+
+
+def parse_cfa(cfa):
+    """CfaPattern::parse of rows separated by '/', or a Bayer name (types.rs:222-252)."""
+    if isinstance(cfa, str) and cfa in BAYER:
+        return np.asarray(BAYER[cfa], dtype=np.int64)
+    if isinstance(cfa, str):
+        return np.asarray([[_CODE[c] for c in row] for row in cfa.split("/")], dtype=np.int64)
+    return np.asarray(cfa, dtype=np.int64)
+
+
+def colours(cfa, w, h, origin=(0, 0)):
+    """CFA colour of every frame pixel: the pattern at (x + origin.x, y + origin.y)."""
+    p = parse_cfa(cfa)
+    ys = (np.arange(h)[:, None] + origin[1]) % p.shape[0]
+    xs = (np.arange(w)[None, :] + origin[0]) % p.shape[1]
+    return p[ys, xs]
+
+
+def normalise(mos, cfa, origin=(0, 0), black=(1, 1, 1, [0.0]), white=(16383,), row_deltas=None, col_deltas=None):
+    """Calib::new + Calib::norm (process.rs:256-298) for every pixel of a one-sample mosaic:
+    (v - black[pos] (+ row delta)(+ col delta)) * scale[pos], scale = 1 / (white - black) in f32."""
+    h, w = mos.shape
+    p = parse_cfa(cfa)
+    brows, bcols, bsamples, bvals = black
+    brows, bcols, bsamples = max(brows, 1), max(bcols, 1), max(bsamples, 1)
+    bvals = np.asarray(bvals, dtype=F32)
+    pw = min(math.lcm(p.shape[1], bcols), 64)
+    ph = min(math.lcm(p.shape[0], brows), 64)
+    Y = (np.arange(h)[:, None] + origin[1]) + np.zeros((1, w), dtype=np.int64)
+    X = (np.arange(w)[None, :] + origin[0]) + np.zeros((h, 1), dtype=np.int64)
+    yy, xx = Y % ph, X % pw
+    idx = ((yy % brows) * bcols + xx % bcols) * bsamples
+    bl = np.where(idx < bvals.size, bvals[np.minimum(idx, bvals.size - 1)], F32(0.0)).astype(F32)
+    colour = p[yy % p.shape[0], xx % p.shape[1]]
+    wv = np.asarray(white, dtype=np.int64)
+    wsel = wv[np.minimum(colour, wv.size - 1)] if wv.size > 1 else np.full_like(colour, wv[0])
+    wf = wsel.astype(F32)
+    with np.errstate(divide="ignore"):
+        scale = np.where(wf > bl, F32(1.0) / (wf - bl), F32(1.0)).astype(F32)
+    b = bl.copy()
+    if row_deltas is not None and len(row_deltas):
+        rd = np.asarray(row_deltas, dtype=F32)
+        b = b + np.where(Y < rd.size, rd[np.minimum(Y, rd.size - 1)], F32(0.0)).astype(F32)
+    if col_deltas is not None and len(col_deltas):
+        cd = np.asarray(col_deltas, dtype=F32)
+        b = b + np.where(X < cd.size, cd[np.minimum(X, cd.size - 1)], F32(0.0)).astype(F32)
+    return ((mos.astype(F32) - b) * scale).astype(F32)
+
+
+def naive_bilinear(norm, col):
+    """The reference test's naive demosaic (process.rs:2405-2446): own colour is the centre,
+    the others the mean of the in-frame same-colour 3x3 samples, summed row-major, then s / n."""
+    h, w = norm.shape
+    out = np.zeros((h, w, 3), dtype=F32)
+    for y in range(h):
+        for x in range(w):
+            own = col[y, x]
+            for c in range(3):
+                if c == own:
+                    out[y, x, c] = norm[y, x]
+                    continue
+                s, n = F32(0.0), F32(0.0)
+                for dy in (-1, 0, 1):
+                    for dx in (-1, 0, 1):
+                        yy, xx = y + dy, x + dx
+                        if 0 <= yy < h and 0 <= xx < w and col[yy, xx] == c:
+                            s = F32(s + norm[yy, xx])
+                            n = F32(n + F32(1.0))
+                out[y, x, c] = F32(s / n) if n > 0 else F32(0.0)
+    return out
+
+
+def bilinear(norm, col, bayer=True):
+    """The reference's bilinear demosaic exactly (Kernel::row_cols, process.rs:1045-1129):
+    interior Bayer pixels by the fast-path formulas, other interior pixels by s * (1 / n) over
+    the neighbour list, border pixels by the naive row-major mean."""
+    h, w = norm.shape
+    out = naive_bilinear(norm, col)  # border pixels (and a start for the interior)
+    if h < 3 or w < 3:
+        return out
+    c = norm[1:-1, 1:-1]
+    l, r = norm[1:-1, :-2], norm[1:-1, 2:]
+    u, d = norm[:-2, 1:-1], norm[2:, 1:-1]
+    ul, ur, dl, dr = norm[:-2, :-2], norm[:-2, 2:], norm[2:, :-2], norm[2:, 2:]
+    cc = col[1:-1, 1:-1]
+    if bayer:
+        right = col[1:-1, 2:]
+        half_lr = (l + r) * F32(0.5)
+        half_ud = (u + d) * F32(0.5)
+        orth = (((l + r) + u) + d) * F32(0.25)
+        diag = (((ul + ur) + dl) + dr) * F32(0.25)
+        R = np.where(cc == 0, c, np.where(cc == 2, diag, np.where(right == 0, half_lr, half_ud)))
+        G = np.where(cc == 1, c, orth)
+        B = np.where(cc == 2, c, np.where(cc == 0, diag, np.where(right == 2, half_lr, half_ud)))
+        out[1:-1, 1:-1] = np.stack([R, G, B], axis=-1)
+        return out
+    for y in range(1, h - 1):
+        for x in range(1, w - 1):
+            for ch in range(3):
+                if col[y, x] == ch:
+                    out[y, x, ch] = norm[y, x]
+                    continue
+                s, n = F32(0.0), 0
+                for dy in (-1, 0, 1):
+                    for dx in (-1, 0, 1):
+                        if col[y + dy, x + dx] == ch:
+                            s = F32(s + norm[y + dy, x + dx])
+                            n += 1
+                out[y, x, ch] = F32(s * (F32(1.0) / F32(n))) if n else F32(0.0)
+    return out
+
+
+def ahd(norm, col, wb):
+    """The AHD demosaic (crates/flashraw/src/ahd.rs:140-392) stated per pixel. Returns camera RGB
+    planes (h, w, 3) valid for HALO <= y < h - HALO, HALO <= x < w - HALO (NaN elsewhere)."""
+    h, w = norm.shape
+    wbv = np.asarray(wb, dtype=F32)
+    m = np.minimum(norm * wbv[col], F32(1.0)).astype(F32)  # ahd.rs:149, no max(0)
+    nan = F32(np.nan)
+
+    def sh(a, dy, dx):
+        """a shifted so that out[y, x] = a[y + dy, x + dx] (NaN outside)."""
+        o = np.full_like(a, nan)
+        ys = slice(max(0, -dy), h - max(0, dy))
+        xs = slice(max(0, -dx), w - max(0, dx))
+        yd = slice(max(0, dy), h - max(0, -dy))
+        xd = slice(max(0, dx), w - max(0, -dx))
+        o[ys, xs] = a[yd, xd]
+        return o
+
+    green = col == 1
+    # 1. green along rows (H) and columns (V), limited to the range of the two nearest greens
+    l, r, l2, r2 = sh(m, 0, -1), sh(m, 0, 1), sh(m, 0, -2), sh(m, 0, 2)
+    u, d, u2, d2 = sh(m, -1, 0), sh(m, 1, 0), sh(m, -2, 0), sh(m, 2, 0)
+    with np.errstate(invalid="ignore"):
+        gh = np.minimum(np.maximum((l + r) * F32(0.5) + (F32(2.0) * m - l2 - r2) * F32(0.25), np.minimum(l, r)),
+                        np.maximum(l, r))
+        gv = np.minimum(np.maximum((u + d) * F32(0.5) + (F32(2.0) * m - u2 - d2) * F32(0.25), np.minimum(u, d)),
+                        np.maximum(u, d))
+    g = [np.where(green, m, gh), np.where(green, m, gv)]
+    # 2. red and blue by colour differences; square-root lightness and chroma
+    right = np.roll(col, -1, axis=1)  # colour of the right neighbour (wraps only outside the AHD region)
+    hr = green & (right == 0)
+    rgb, lab = [], []
+    for dd in range(2):
+        gd = g[dd]
+        df = m - gd
+        diag = gd + F32(0.25) * (((sh(df, -1, -1) + sh(df, -1, 1)) + sh(df, 1, -1)) + sh(df, 1, 1))
+        row = gd + F32(0.5) * (sh(df, 0, -1) + sh(df, 0, 1))
+        cl = gd + F32(0.5) * (sh(df, -1, 0) + sh(df, 1, 0))
+        red = np.where(col == 0, m, np.where(green, np.where(hr, row, cl), diag))
+        blue = np.where(col == 2, m, np.where(green, np.where(hr, cl, row), diag))
+        with np.errstate(invalid="ignore"):
+            sr, sg, sb = (np.sqrt(np.maximum(v, F32(0.0))) for v in (red, gd, blue))
+        rgb.append((red, gd, blue))
+        lab.append((((sr + F32(2.0) * sg) + sb) * F32(0.25), sr - sg, sb - sg))
+    # 3. thresholds: the smaller of the largest row steps of H and column steps of V
+    def lstep(L, dy, dx):
+        return np.abs(sh(L, dy, dx) - L)
+
+    def cstep(A, Bb, dy, dx):
+        da, db = sh(A, dy, dx) - A, sh(Bb, dy, dx) - Bb
+        return da * da + db * db
+
+    LH, AH, BH = lab[0]
+    LV, AV, BV = lab[1]
+    with np.errstate(invalid="ignore"):
+        el = np.minimum(np.maximum(lstep(LH, 0, -1), lstep(LH, 0, 1)), np.maximum(lstep(LV, -1, 0), lstep(LV, 1, 0)))
+        ec = np.minimum(np.maximum(cstep(AH, BH, 0, -1), cstep(AH, BH, 0, 1)),
+                        np.maximum(cstep(AV, BV, -1, 0), cstep(AV, BV, 1, 0)))
+        # 4. homogeneity: neighbours within both thresholds, summed over 3x3
+        hom = []
+        for dd in range(2):
+            L, A, Bb = lab[dd]
+            cnt = np.zeros((h, w), dtype=F32)
+            for dy in (-1, 0, 1):
+                for dx in (-1, 0, 1):
+                    if dy == 0 and dx == 0:
+                        continue
+                    ok = (lstep(L, dy, dx) <= el) & (cstep(A, Bb, dy, dx) <= ec)
+                    cnt = cnt + ok.astype(F32)
+            v = (sh(cnt, -1, 0) + cnt) + sh(cnt, 1, 0)
+            hom.append((sh(v, 0, -1) + v) + sh(v, 0, 1))
+    a, b = hom
+    s = a + b
+    with np.errstate(invalid="ignore", divide="ignore"):
+        wgt = np.where(s > 0, a / s, F32(0.5)).astype(F32)
+    out = np.stack([wgt * rgb[0][c] + (F32(1.0) - wgt) * rgb[1][c] for c in range(3)], axis=-1).astype(F32)
+    valid = np.zeros((h, w), dtype=bool)
+    valid[HALO:h - HALO, HALO:w - HALO] = True
+    out[~valid] = nan
+    return out
+
+
+def finish(rgb, wb, matrix, levels=None):
+    """finish_rgb with Highlights::Clip (process.rs:654-707): clamp(rgb * wb, 0, 1), then
+    (m0 * r + m1 * g) + m2 * b per row, then v * scale - offset with levels."""
+    wbv = np.asarray(wb, dtype=F32)
+    c = [np.clip(rgb[..., i] * wbv[i], F32(0.0), F32(1.0)).astype(F32) for i in range(3)]
+    m = np.asarray(matrix, dtype=F32)
+    o = np.stack([(m[k, 0] * c[0] + m[k, 1] * c[1]) + m[k, 2] * c[2] for k in range(3)], axis=-1).astype(F32)
+    if levels is not None:
+        o = (o * F32(levels[0]) - F32(levels[1])).astype(F32)
+    return o
+
+
+def develop(mos, cfa, *, origin=(0, 0), black=(1, 1, 1, [0.0]), white=(16383,), wb=(1, 1, 1, 1),
+            matrix=((1, 0, 0), (0, 1, 0), (0, 0, 1)), demosaic="bilinear", levels=None):
+    """Linear f32 develop of the full frame (crop = frame, orientation 1): the demosaic, then
+    finish. AHD pixels are already balanced (ahd.rs:149) and finish with unit WB
+    (process.rs:1205-1212); border pixels of AHD use bilinear."""
+    h, w = mos.shape
+    col = colours(cfa, w, h, origin)
+    norm = normalise(mos, cfa, origin, black, white)
+    is_bayer = parse_cfa(cfa).shape == (2, 2)
+    bil = bilinear(norm, col, bayer=is_bayer)
+    out = finish(bil, wb, matrix, levels)
+    if demosaic == "ahd" and h > 2 * HALO and w > 2 * HALO:
+        a = ahd(norm, col, wb)
+        fa = finish(a, (1.0, 1.0, 1.0), matrix, levels)
+        out[HALO:h - HALO, HALO:w - HALO] = fa[HALO:h - HALO, HALO:w - HALO]
+    return out
+
+
+def rgba8(lin, lut):
+    """Lut8 encode (process.rs:1679-1728): lut[min(trunc(clamp(v, 0, 1) * 4096), 4096)], alpha 255."""
+    v = np.clip(lin, F32(0.0), F32(1.0)).astype(F32) * F32(4096.0)
+    idx = np.minimum(v.astype(np.int64), 4096)
+    px = lut[idx]
+    return np.concatenate([px, np.full(px.shape[:-1] + (1,), 255, dtype=np.uint8)], axis=-1)
+
+
+def linear16(lin):
+    """(clamp(v, 0, 1) * 65535 + 0.5) as u16 (process.rs:1971-1973)."""
+    v = np.clip(lin, F32(0.0), F32(1.0)).astype(F32) * F32(65535.0) + F32(0.5)
+    return v.astype(np.uint16)
+
+
+def orient(img, o):
+    """EXIF orientation o (1..8) of an (h, w, c) image, stated with numpy flips and rotations."""
+    return {
+        1: lambda a: a,
+        2: lambda a: a[:, ::-1],
+        3: lambda a: a[::-1, ::-1],
+        4: lambda a: a[::-1, :],
+        5: lambda a: a.transpose(1, 0, 2),
+        6: lambda a: np.rot90(a, -1),
+        7: lambda a: a[::-1, ::-1].transpose(1, 0, 2),
+        8: lambda a: np.rot90(a, 1),
+    }[o](img)
+
+
+def lcg_fill(w, h):
+    """The reference tests' deterministic mosaic: sample i = (i * 7919) % 3900 + 100
+    (process.rs:2331-2334, 2451-2453)."""
+    i = np.arange(w * h, dtype=np.int64)
+    return ((i * 7919) % 3900 + 100).astype(np.uint16).reshape(h, w)
+
+
+def lcg_stream(w, h, seed=12345):
+    """The reference tests' LCG mosaic: s = s * 1664525 + 1013904223, sample 100 + (s >> 20) % 3900
+    (process.rs:2371-2375)."""
+    out = np.empty(w * h, dtype=np.uint16)
+    s = seed
+    for k in range(w * h):
+        s = (s * 1664525 + 1013904223) & 0xFFFFFFFF
+        out[k] = 100 + (s >> 20) % 3900
+    return out.reshape(h, w)
+
+
+def dct_matrix():
+    """Orthonormal DCT-II (denoise/mod.rs:626-635) from its closed form, float64."""
+    k = np.arange(8)[:, None]
+    i = np.arange(8)[None, :]
+    a = np.where(k == 0, math.sqrt(1 / 8), math.sqrt(2 / 8))
+    return a * np.cos(math.pi * (2 * i + 1) * k / 16)
+
+
+def dct_denoise(mos, black, sig_a, sig_b, thr=2.7, step=4):
+    """The CFA-plane DCT block denoiser in float64 with scipy's DCT-II (an implementation that
+    shares nothing with the oracle's matrix products): four 2x2 phase planes, 8x8 blocks at
+    stride `step` plus the last position, AC coefficients with |c| < thr * sigma zeroed,
+    sigma^2 = sig_a * max(mean, 0) + sig_b, uniform overlap-add. Returns the f32 mosaic in DN
+    above black."""
+    from scipy.fft import dctn, idctn
+    h, w = mos.shape
+    out = np.zeros((h, w), dtype=np.float64)
+    for plane in range(4):
+        dy, dx = plane >> 1, plane & 1
+        p = mos[dy::2, dx::2].astype(np.float64) - float(np.float32(black[plane]))
+        ph, pw = p.shape
+
+        def pos(n):
+            ps = list(range(0, n - 8 + 1, step))
+            if ps[-1] != n - 8:
+                ps.append(n - 8)
+            return ps
+
+        acc = np.zeros_like(p)
+        cnt = np.zeros_like(p)
+        for y0 in pos(ph):
+            for x0 in pos(pw):
+                c = dctn(p[y0:y0 + 8, x0:x0 + 8], type=2, norm="ortho")
+                mean = c[0, 0] / 8.0
+                t = float(np.float32(thr)) * math.sqrt(float(np.float32(sig_a)) * max(mean, 0.0)
+                                                       + float(np.float32(sig_b)))
+                keep = np.abs(c) >= t
+                keep[0, 0] = True
+                acc[y0:y0 + 8, x0:x0 + 8] += idctn(np.where(keep, c, 0.0), type=2, norm="ortho")
+                cnt[y0:y0 + 8, x0:x0 + 8] += 1.0
+        out[dy::2, dx::2] = acc / cnt
+    return out

@@ -239,3 +239,22 @@ def test_autotuned_plan(ctx, demosaic):
     assert np.array_equal(dst.cpu().numpy(), want)
     again = ctx.run(F.GpuFrame.from_u16(mos, w, h), info, F.GpuOp.Develop(spec)).image.download()
     assert np.array_equal(again, want)
+
+
+@pytest.mark.parametrize("cfa", ["RGGB", "BGGR", "GRBG", "GBRG"])
+@pytest.mark.parametrize("demosaic", ["bilinear", "ahd"])
+def test_golden_fixtures(ctx, cfa, demosaic):
+    """The committed fixtures (tests/golden/develop_small.npz: reference-test LCG mosaics, black
+    pattern 60/64/62/66, white 4095, origin (1, 0)) developed on the GPU, every byte."""
+    import pathlib
+    g = np.load(pathlib.Path(__file__).parent / "golden" / "develop_small.npz")
+    mos = g[f"mosaic_{cfa}"]
+    h, w = mos.shape
+    info = _info(w, h, cfa, (2, 2, 1, [float(v) for v in g["black"]]), tuple(int(v) for v in g["white"]))
+    m = tuple(tuple(float(v) for v in row) for row in g["matrix"])
+    for out, enc in ((F.DevelopOutput.Rgba8, "rgba8"), (F.DevelopOutput.LinearRgb16, "linear16")):
+        spec = F.DevelopSpec(F.Rect.full(w, h), wb=tuple(float(v) for v in g["wb"]), cam_to_rgb=m, output=out,
+                             edge=F.ahd_spec(info, (1, 0)) if demosaic == "ahd" else None)
+        frame = F.GpuFrame.from_u16(mos, w, h).with_origin((1, 0))
+        got = ctx.run(frame, info, F.GpuOp.Develop(spec)).image.download()
+        assert np.array_equal(got, g[f"{cfa}_{demosaic}_{enc}"]), enc
