@@ -6,6 +6,7 @@ camera->sRGB matrix, RGBA8 output. A step is one develop of that frame with its 
 resident in HBM and the output written to HBM. Prints one JSON line.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--demosaic ahd|bilinear] [--strip-rows R]
+  python bench.py --mode denoise [--precision f16|bf16|f16x3|f32]   # the CFA-plane DCT denoiser
   python bench.py --mode copy   # a plain device copy of 1 GiB (PMC byte-counter calibration)
 """
 import argparse
@@ -28,7 +29,10 @@ def _parse():
     p.add_argument("--steps", type=int, default=200)
     p.add_argument("--warmup", type=int, default=20)
     p.add_argument("--demosaic", default="ahd", choices=["ahd", "bilinear"])
-    p.add_argument("--mode", default="develop", choices=["develop", "copy"])
+    p.add_argument("--mode", default="develop", choices=["develop", "denoise", "copy"])
+    p.add_argument("--precision", default="f16", choices=["f16", "bf16", "f16x3", "f32"],
+                   help="denoise mode: MFMA operand precision, or the scalar f32 variant")
+    p.add_argument("--seg-rows", type=int, default=0, help="denoise mode: 8-row tile rows per wave (0: default)")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--strip-rows", type=int, default=0,
                    help="use this strip height instead of autotuning (profiled runs, where the autotuner's "
@@ -61,6 +65,89 @@ def copy_mode(args):
     print(json.dumps({"mode": "copy", "bytes_each_way": n, "ms": dt * 1e3, "GBps": 2 * n / dt / 1e9}))
 
 
+MFMA_PEAK_TFLOPS = {"f16": 2500.0, "bf16": 2500.0, "f16x3": 2500.0}
+
+
+def denoise_mode(args):
+    """The CFA-plane DCT denoiser on the 6048x4024 RGGB frame with read sigma 12 DN: u16 mosaic
+    in, u16 mosaic out, resident in HBM. A step is one denoise of the frame."""
+    import numpy as np
+    import torch
+
+    import flashraw_amd as F
+    from flashraw_amd import synth
+
+    ctx = F.context()
+    W, H = 6048, 4024
+    black = [512.0] * 4
+    sig_a, sig_b, thr = 2.0, 144.0, 2.7  # Poisson-Gaussian model of the frame: 1/gain DN/DN, read sigma 12 DN
+    mos = synth.frame_24mp_noisy()
+    prec = {"f16": F.DnPrecision.F16, "bf16": F.DnPrecision.BF16, "f16x3": F.DnPrecision.F16X3,
+            "f32": F.DnPrecision.F32}[args.precision]
+    spec = F.DenoiseSpec(W, H, black, sig_a, sig_b, thr, 4, prec, seg_rows=args.seg_rows)
+    d_in = torch.from_numpy(mos.view(np.int16)).to("cuda")
+    dst = torch.empty((H, W), dtype=torch.int16, device="cuda")
+    dstf = torch.empty((H, W), dtype=torch.float32, device="cuda")
+    stream = torch.cuda.Stream()
+    ctx.denoise_cfa(spec, d_in, None, dstf, stream=stream)  # f32 result for the accuracy report
+    for _ in range(args.warmup):
+        ctx.denoise_cfa(spec, d_in, dst, None, stream=stream)
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for s, e in evs:
+        s.record(stream)
+        ctx.denoise_cfa(spec, d_in, dst, None, stream=stream)
+        e.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kms = [s.elapsed_time(e) for s, e in evs]
+    k_avg = statistics.mean(kms)
+    ms_per_step = wall * 1e3 / args.steps
+    mp = W * H / 1e6
+    algo_bytes = W * H * 4  # u16 in, u16 out
+    blocks = 4 * ((W // 2 - 8) // 4 + 1) * ((H // 2 - 8) // 4 + 1)
+    flops = blocks * 4096  # forward and inverse separable 8x8 DCT: 4 products of 8x8x8 multiply-adds
+    got = dstf.cpu().numpy()
+    cpu = None
+    err = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+        times = []
+        want = None
+        for i in range(4):
+            t = time.perf_counter()
+            want, _ = O.dct_denoise(mos, black, sig_a, sig_b, thr, 4, precision=64, threads=threads)
+            if i:
+                times.append(time.perf_counter() - t)
+        d = np.abs(got.astype(np.float64) - want.astype(np.float64))
+        rmse = float(np.sqrt(np.mean(d * d)))
+        err = {"max_abs_dn": round(float(d.max()), 4), "mean_abs_dn": round(float(d.mean()), 6),
+               "psnr_db_vs_f64": round(20 * np.log10((16383 - 512) / rmse), 2) if rmse > 0 else None}
+        cpu = {"value": round(mp / statistics.median(times), 2), "unit": "MP/s", "cores": threads, "kind": "port",
+               "sample": "the same 6048x4024 frame, float64 oracle denoise, median of 3 runs after 1 warm-up"}
+    line = {
+        "metric": "MP/s CFA-plane DCT denoise (24 MP, stride 4, % HBM roofline)",
+        "value": round(mp / (ms_per_step / 1e3), 1), "unit": "MP/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_per_step, 5), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": args.precision, "data": "synthetic",
+        "config": {"workload": "6048x4024 RGGB 14-bit u16 -> u16, read sigma 12 DN, 8x8 DCT stride 4, 2.7 sigma",
+                   "blocks": blocks, "error_vs_f64_oracle": err},
+        "roofline": {"bound": "hbm", "achieved": round(algo_bytes / (k_avg / 1e3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(algo_bytes / (k_avg / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "traffic": _traffic(f"denoise_{args.precision}_6048x4024"), "kernel_ms_avg": round(k_avg, 5),
+                     "kernel_ms_p50": round(statistics.median(kms), 5), "algorithmic_bytes_per_launch": algo_bytes,
+                     "mfma": None if args.precision == "f32" else {
+                         "algorithmic_tflops": round(flops / (k_avg / 1e3) / 1e12, 2),
+                         "peak_tflops": MFMA_PEAK_TFLOPS[args.precision],
+                         "frac": round(flops / (k_avg / 1e3) / 1e12 / MFMA_PEAK_TFLOPS[args.precision], 5),
+                         "flops_per_launch": flops}},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
 def main():
     args = _parse()
     if args.gpus != 1:
@@ -69,6 +156,8 @@ def main():
         sys.exit(2)
     if args.mode == "copy":
         return copy_mode(args)
+    if args.mode == "denoise":
+        return denoise_mode(args)
 
     import numpy as np
     import torch

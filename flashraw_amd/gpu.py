@@ -175,6 +175,8 @@ def lib():
             L.frh_plan_destroy.argtypes = [vp]
             L.frh_context_stream.argtypes = [vp]
             L.frh_context_stream.restype = vp
+            L.frh_denoise_cfa.argtypes = [vp, ctypes.POINTER(_DnSpec), vp, vp, vp, vp]
+            L.frh_denoise_cfa.restype = i32
             _lib = L
         return _lib
 
@@ -397,6 +399,48 @@ class DevelopSpec:
         return s, keep
 
 
+class DnPrecision(enum.IntEnum):
+    """frh_dn_precision: MFMA with f16 or bf16 operands (f32 accumulation), or scalar f32."""
+    F16 = 0
+    BF16 = 1
+    F32 = 2
+    F16X3 = 3
+
+
+class _DnSpec(ctypes.Structure):
+    _fields_ = [("size", ctypes.c_uint32), ("width", ctypes.c_uint32), ("height", ctypes.c_uint32),
+                ("stride", ctypes.c_uint32), ("black", ctypes.c_float * 4), ("sig_a", ctypes.c_float),
+                ("sig_b", ctypes.c_float), ("threshold", ctypes.c_float), ("step", ctypes.c_uint32),
+                ("precision", ctypes.c_int32), ("seg_rows", ctypes.c_uint32)]
+
+
+@dataclass
+class DenoiseSpec:
+    """The CFA-plane DCT block denoiser's parameters (frh_dn_spec): 8x8 blocks at stride `step`
+    on each 2x2 phase plane, AC coefficients below threshold * sigma zeroed,
+    sigma^2 = sig_a * max(block mean, 0) + sig_b (DN), uniform overlap-add."""
+    width: int
+    height: int
+    black: Sequence[float] = (0.0, 0.0, 0.0, 0.0)
+    sig_a: float = 0.0
+    sig_b: float = 1.0
+    threshold: float = 2.7
+    step: int = 4
+    precision: DnPrecision = DnPrecision.F16
+    stride: int = 0
+    seg_rows: int = 0
+
+    def _c(self):
+        s = _DnSpec()
+        s.size = ctypes.sizeof(_DnSpec)
+        s.width, s.height, s.stride = self.width, self.height, self.stride
+        for i in range(4):
+            s.black[i] = float(self.black[i])
+        s.sig_a, s.sig_b, s.threshold = self.sig_a, self.sig_b, self.threshold
+        s.step, s.precision, s.seg_rows = self.step, int(self.precision), self.seg_rows
+        return s
+
+
 class GpuOp:
     """GpuOp (run.rs:219-225); only Develop is accepted by this path."""
 
@@ -580,6 +624,14 @@ class GpuContext:
         check(lib().frh_develop_into(self._h, ctypes.byref(f), ctypes.byref(i), ctypes.byref(s),
                                      _device_ptr(dst), n1, None if dst2 is None else _device_ptr(dst2), n2, sp))
         del k1, k2, k3
+
+    def denoise_cfa(self, spec: DenoiseSpec, src, dst=None, dst_f32=None, stream=None) -> None:
+        """Denoise the device u16 mosaic `src` into `dst` (u16) and/or `dst_f32` (f32, DN above
+        black), torch tensors or device addresses, on `stream` (frh_denoise_cfa)."""
+        s = spec._c()
+        check(lib().frh_denoise_cfa(self._h, ctypes.byref(s), _device_ptr(src),
+                                    None if dst is None else _device_ptr(dst),
+                                    None if dst_f32 is None else _device_ptr(dst_f32), self.stream_handle(stream)))
 
     def output_size(self, frame: GpuFrame, spec: DevelopSpec, output: DevelopOutput):
         """(width, height, channels, bytes) of `output` for this frame and spec."""

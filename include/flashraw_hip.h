@@ -183,6 +183,36 @@ void *frh_context_stream(const frh_context *ctx);
 int32_t frh_develop_output_size(const frh_frame *frame, const frh_develop_spec *spec, int32_t output,
                                 uint32_t *width, uint32_t *height, uint32_t *channels, uint64_t *bytes);
 
+/* ---- CFA-plane DCT block denoiser ----
+ * No reference counterpart as an entry point: the reference denoises developed linear RGB
+ * (GpuContext::denoise, crates/flashraw-gpu/src/denoise.rs:817). This one works on the mosaic's
+ * four 2x2 phase planes with the reference's block transform (the orthonormal DCT-II of
+ * crates/flashraw/src/denoise/mod.rs:621-635), hard threshold (dct_pass, mod.rs:921-974) and
+ * block positions (positions(), mod.rs:877-884): 8x8 blocks at stride `step` plus the last
+ * position, AC coefficients with |c| < threshold * sigma zeroed, sigma^2 = sig_a * max(block
+ * mean, 0) + sig_b (DN), uniform overlap-add. */
+enum frh_dn_precision {
+    FRH_DN_F16 = 0,  /* MFMA, f16 operands, f32 accumulation (stride 4, sides multiples of 8) */
+    FRH_DN_BF16 = 1, /* MFMA, bf16 operands, f32 accumulation (same limits) */
+    FRH_DN_F32 = 2,  /* scalar f32, any stride 1..8, even sides; synchronises `stream` */
+    FRH_DN_F16X3 = 3 /* MFMA, split f16 operands (value + rounded remainder, 3 products), f32 acc */
+};
+typedef struct frh_dn_spec {
+    uint32_t size;
+    uint32_t width, height, stride; /* mosaic size; stride in samples (0: width) */
+    float black[4];                 /* black level of the planes, (y & 1) * 2 + (x & 1) */
+    float sig_a, sig_b, threshold;  /* threshold: 2.7 in the reference's hard pass (mod.rs:47) */
+    uint32_t step;
+    int32_t precision;
+    uint32_t seg_rows; /* MFMA walker: 8-row tile rows per wave (0: 16) */
+} frh_dn_spec;
+/* Denoise the device u16 mosaic `src` into `dst` (u16 mosaic, black added back, rounded and
+ * clamped) and/or `dst_f32` (f32 mosaic in DN above black); either may be NULL, not both. The
+ * output must not overlap the input. Asynchronous on `stream` (NULL: the null stream) for the
+ * MFMA precisions. */
+int32_t frh_denoise_cfa(frh_context *ctx, const frh_dn_spec *spec, const void *src, void *dst, float *dst_f32,
+                        void *stream);
+
 /* ---- images (crates/flashraw-gpu/src/image.rs:52-306) ---- */
 
 enum frh_pixel_format { FRH_FORMAT_RGBA8 = 0, FRH_FORMAT_F32 = 1, FRH_FORMAT_U16 = 2 };
