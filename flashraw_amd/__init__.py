@@ -1,12 +1,12 @@
 """MI355X-native fused RAW develop: the reference's `flashraw-gpu` develop entry points over
 hand-written gfx950 HIP kernels (flashraw_amd/lib/libflashraw_hip.so)."""
-from .gpu import (BlackLevel, CfaColor, CfaPattern, DenoiseSpec, DevelopOutput, DnPrecision, DevelopSpec, EdgeSpec, FlashRawError,
+from .gpu import (Batch, BlackLevel, CfaColor, CfaPattern, DenoiseSpec, DevelopOutput, DnPrecision, DevelopSpec, EdgeSpec, FlashRawError,
                   FrameInfo, GpuContext, GpuFrame, GpuImage, GpuOp, GpuRun, GpuTimings, GpuUnavailable,
                   InvalidArgument, LimitExceeded, NotAvailable, Orientation, OutOfMemory, Rect, WhiteLevel,
                   ahd_spec, available, context, lib, srgb8_table)
 
 __all__ = [
-    "BlackLevel", "CfaColor", "CfaPattern", "DenoiseSpec", "DevelopOutput", "DnPrecision", "DevelopSpec", "EdgeSpec", "FlashRawError",
+    "Batch", "BlackLevel", "CfaColor", "CfaPattern", "DenoiseSpec", "DevelopOutput", "DnPrecision", "DevelopSpec", "EdgeSpec", "FlashRawError",
     "FrameInfo", "GpuContext", "GpuFrame", "GpuImage", "GpuOp", "GpuRun", "GpuTimings", "GpuUnavailable",
     "InvalidArgument", "LimitExceeded", "NotAvailable", "Orientation", "OutOfMemory", "Rect", "WhiteLevel",
     "ahd_spec", "available", "context", "lib", "srgb8_table",

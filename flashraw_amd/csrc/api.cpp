@@ -256,16 +256,7 @@ static uint32_t tuned_rows(const frh_context *ctx, bool ahd, const DevArgs &a, u
 }
 
 // Everything a develop launch needs, validated (run.rs:439-646).
-struct Plan {
-    DevArgs a;
-    bool ahd;
-    uint32_t out_w[2], out_h[2], ch[2];
-    uint64_t bytes[2];
-    int32_t fmt[2];
-    uint64_t in_row_bytes;
-};
-
-static int32_t plan_develop(const frh_context *ctx, const frh_frame *f, const frh_frame_info *info,
+int32_t frh::plan_develop(const frh_context *ctx, const frh_frame *f, const frh_frame_info *info,
                             const frh_develop_spec *spec, Plan &p) {
     if (!f || !info || !spec) return fail(FRH_ERROR_INVALID_ARGUMENT, "NULL descriptor");
     if (f->size < sizeof(frh_frame) || info->size < sizeof(frh_frame_info) || spec->size < sizeof(frh_develop_spec))

@@ -1,0 +1,436 @@
+// Batched develop: many frames of any size, Bayer phase, bit depth, white balance and colour
+// matrix in one persistent launch per demosaic kind.
+//
+// Each frame is split exactly as its single-frame launch splits it (the row walkers' strips
+// plus the bilinear border pixels, the same kernels' device code), and every strip or group of
+// 256 border pixels becomes an item of one work list. A persistent grid, sized to the resident
+// waves the kernel's occupancy allows, takes items from an atomic counter until the list is
+// empty. An item writes only its own pixels, so a frame's bytes are the same alone, in any
+// batch and in any order. The counter is reset by a one-thread kernel ahead of the list kernels
+// (no memset node: the launch sequence can be captured in a HIP graph and replayed).
+//
+// The per-frame strip height is the batch's tile shape; frh_batch_autotune times candidates on
+// the batch itself and keeps the fastest.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "ahd_walk.h"
+#include "bil_walk.h"
+#include "frh_internal.h"
+
+namespace frh {
+namespace {
+
+constexpr uint32_t BORDER_ITEM = 256;  // border pixels per item (4 per lane)
+
+struct BatchFrame {
+    DevArgs a;
+    WalkPlan wp;   // AHD frames
+    BWalkPlan bp;  // bilinear frames
+    int32_t rp;    // red position of the frame's Bayer phase
+    uint32_t walk_items, items;
+};
+
+__global__ void k_batch_reset(uint32_t *counters, uint32_t n) {
+    if (threadIdx.x < n) counters[threadIdx.x] = 0u;
+}
+
+// Work list of one demosaic kind: frames `ids` in order, item_off[k] the first item of ids[k]
+// (item_off[n] = total).
+struct BatchList {
+    uint32_t n, total;
+    uint32_t *counter;
+    const uint8_t *lut8;
+    const float *wtab;
+};
+// The descriptor and list pointers are kernel parameters of their own, read-only and unaliased:
+// a frame's arguments are then fetched with scalar loads, as a single-frame launch's are.
+
+__device__ __forceinline__ uint32_t next_item(uint32_t *counter) {
+    uint32_t v = 0;
+    if ((threadIdx.x & 63) == 0) v = atomicAdd(counter, 1u);
+    return __builtin_amdgcn_readfirstlane(v);
+}
+
+__device__ __forceinline__ uint32_t frame_of(const uint32_t *__restrict__ item_off, uint32_t n, uint32_t item) {
+    uint32_t lo = 0, hi = n;  // item_off[lo] <= item < item_off[hi]
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (item_off[mid] <= item) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ void load_tables(const BatchList &L, Lds &s, float *wt) {
+    const uint32_t *l = reinterpret_cast<const uint32_t *>(L.lut8);
+    for (int i = threadIdx.x; i < 1025; i += blockDim.x) s.lut[i] = l[i];
+    if (wt)
+        for (int i = threadIdx.x; i < 73 * 73; i += blockDim.x) wt[i] = L.wtab[i];
+    __syncthreads();
+}
+
+// Frames of a batch use the scalar calibration (at most a 2x2 black pattern, no deltas), so the
+// shared Lds holds only the sRGB table; its black/scale arrays are never read.
+template <bool FAST>
+__global__ __launch_bounds__(WALK_WAVES * 64) void k_batch_ahd(BatchList L, const BatchFrame *__restrict__ frames,
+                                                                    const uint32_t *__restrict__ ids,
+                                                                    const uint32_t *__restrict__ item_off) {
+    __shared__ Lds s;
+    __shared__ float wt[73 * 73];
+    load_tables(L, s, wt);
+    for (;;) {
+        const uint32_t item = next_item(L.counter);
+        if (item >= L.total) break;
+        const uint32_t k = __builtin_amdgcn_readfirstlane(frame_of(item_off, L.n, item));
+        const BatchFrame &F = frames[__builtin_amdgcn_readfirstlane(ids[k])];
+        const uint32_t local = item - item_off[k];
+        if (local < F.walk_items) {
+            switch (F.rp) {
+            case 0: walk_item<0, FAST>(F.a, F.wp, s, wt, (int)local); break;
+            case 1: walk_item<1, FAST>(F.a, F.wp, s, wt, (int)local); break;
+            case 2: walk_item<2, FAST>(F.a, F.wp, s, wt, (int)local); break;
+            default: walk_item<3, FAST>(F.a, F.wp, s, wt, (int)local); break;
+            }
+        } else {
+            const uint32_t base = (local - F.walk_items) * BORDER_ITEM + (threadIdx.x & 63);
+            for (uint32_t j = 0; j < BORDER_ITEM; j += 64)
+                if (base + j < F.wp.border.count) border_pixel(F.a, s, F.wp.border, base + j);
+        }
+    }
+}
+
+template <bool FAST>
+__global__ __launch_bounds__(BWALK_WAVES * 64) void k_batch_bilinear(BatchList L, const BatchFrame *__restrict__ frames,
+                                                                    const uint32_t *__restrict__ ids,
+                                                                    const uint32_t *__restrict__ item_off) {
+    __shared__ Lds s;
+    load_tables(L, s, nullptr);
+    for (;;) {
+        const uint32_t item = next_item(L.counter);
+        if (item >= L.total) break;
+        const uint32_t k = __builtin_amdgcn_readfirstlane(frame_of(item_off, L.n, item));
+        const BatchFrame &F = frames[__builtin_amdgcn_readfirstlane(ids[k])];
+        const uint32_t local = item - item_off[k];
+        if (local < F.walk_items) {
+            switch (F.rp) {
+            case 0: bwalk_item<0, FAST>(F.a, F.bp, s, (int)local); break;
+            case 1: bwalk_item<1, FAST>(F.a, F.bp, s, (int)local); break;
+            case 2: bwalk_item<2, FAST>(F.a, F.bp, s, (int)local); break;
+            default: bwalk_item<3, FAST>(F.a, F.bp, s, (int)local); break;
+            }
+        } else {
+            const uint32_t base = (local - F.walk_items) * BORDER_ITEM + (threadIdx.x & 63);
+            for (uint32_t j = 0; j < BORDER_ITEM; j += 64)
+                if (base + j < F.bp.border.count) border_pixel(F.a, s, F.bp.border, base + j);
+        }
+    }
+}
+
+}  // namespace
+}  // namespace frh
+
+using namespace frh;
+
+struct frh_batch {
+    frh_context *ctx = nullptr;
+    std::vector<Plan> plans;
+    std::vector<BatchFrame> frames;
+    uint32_t ahd_rows = 0, bil_rows = 0;  // 0: each frame's own default
+    // device copies
+    BatchFrame *d_frames = nullptr;
+    uint32_t *d_lists = nullptr;  // ids and offsets of both lists
+    uint32_t *d_counters = nullptr;
+    size_t n_list_words = 0;
+    uint32_t n_ahd = 0, n_bil = 0, total_ahd = 0, total_bil = 0;
+    bool fast_ahd = true, fast_bil = true;
+    hipGraphExec_t graph = nullptr;
+    hipGraph_t graph_def = nullptr;
+    hipStream_t graph_stream = nullptr;
+    ~frh_batch() {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        if (graph_def) (void)hipGraphDestroy(graph_def);
+        if (d_frames) (void)hipFree(d_frames);
+        if (d_lists) (void)hipFree(d_lists);
+        if (d_counters) (void)hipFree(d_counters);
+    }
+};
+
+// Build the work lists for the current strip heights and upload them (synchronously on the
+// context stream).
+static int32_t batch_build(frh_batch *b) {
+    frh_context *ctx = b->ctx;
+    const size_t n = b->plans.size();
+    b->frames.assign(n, BatchFrame{});
+    std::vector<uint32_t> ids_a, ids_b, off_a{0}, off_b{0};
+    b->fast_ahd = b->fast_bil = true;
+    for (size_t i = 0; i < n; i++) {
+        const Plan &p = b->plans[i];
+        BatchFrame &F = b->frames[i];
+        F.a = p.a;
+        F.a.band_y0 = 0;
+        F.a.band_y1 = p.a.dev_h;
+        F.rp = red_position(F.a);
+        if (p.ahd) {
+            F.wp = make_walk_plan(F.a, (int)(b->ahd_rows ? b->ahd_rows : p.a.ahd_rows));
+            F.walk_items = (uint32_t)(F.wp.strips * F.wp.chunks);
+            F.items = F.walk_items + (F.wp.border.count + BORDER_ITEM - 1) / BORDER_ITEM;
+            b->fast_ahd = b->fast_ahd && fast_rgba8(F.a);
+            ids_a.push_back((uint32_t)i);
+            off_a.push_back(off_a.back() + F.items);
+        } else {
+            F.bp = make_bwalk_plan(F.a, (int)(b->bil_rows ? b->bil_rows : p.a.bil_rows));
+            F.walk_items = (uint32_t)(F.bp.strips * F.bp.chunks);
+            F.items = F.walk_items + (F.bp.border.count + BORDER_ITEM - 1) / BORDER_ITEM;
+            b->fast_bil = b->fast_bil && fast_rgba8(F.a);
+            ids_b.push_back((uint32_t)i);
+            off_b.push_back(off_b.back() + F.items);
+        }
+    }
+    b->n_ahd = (uint32_t)ids_a.size();
+    b->n_bil = (uint32_t)ids_b.size();
+    b->total_ahd = off_a.back();
+    b->total_bil = off_b.back();
+    // layout: ids_a | off_a | ids_b | off_b
+    std::vector<uint32_t> words;
+    words.insert(words.end(), ids_a.begin(), ids_a.end());
+    words.insert(words.end(), off_a.begin(), off_a.end());
+    words.insert(words.end(), ids_b.begin(), ids_b.end());
+    words.insert(words.end(), off_b.begin(), off_b.end());
+    hipError_t e;
+    if (!b->d_frames && (e = hipMalloc(&b->d_frames, n * sizeof(BatchFrame))) != hipSuccess)
+        return hip_fail(e, "batch descriptors");
+    if (words.size() > b->n_list_words) {
+        if (b->d_lists) (void)hipFree(b->d_lists);
+        b->d_lists = nullptr;
+        if ((e = hipMalloc(&b->d_lists, words.size() * 4)) != hipSuccess) return hip_fail(e, "batch lists");
+        b->n_list_words = words.size();
+    }
+    if (!b->d_counters && (e = hipMalloc(&b->d_counters, 2 * sizeof(uint32_t))) != hipSuccess)
+        return hip_fail(e, "batch counters");
+    if ((e = hipMemcpyAsync(b->d_frames, b->frames.data(), n * sizeof(BatchFrame), hipMemcpyHostToDevice,
+                            ctx->stream)) != hipSuccess ||
+        (e = hipMemcpyAsync(b->d_lists, words.data(), words.size() * 4, hipMemcpyHostToDevice, ctx->stream)) !=
+            hipSuccess ||
+        (e = hipStreamSynchronize(ctx->stream)) != hipSuccess)
+        return hip_fail(e, "batch upload");
+    if (b->graph) {  // the lists changed: a captured graph is stale
+        (void)hipGraphExecDestroy(b->graph);
+        (void)hipGraphDestroy(b->graph_def);
+        b->graph = nullptr;
+        b->graph_def = nullptr;
+    }
+    return FRH_OK;
+}
+
+template <typename K>
+static uint32_t resident_blocks(K kernel, int threads, int num_cus) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+    return (uint32_t)per_cu * (uint32_t)num_cus;
+}
+
+// The launch sequence: counter reset, then the AHD list, then the bilinear list.
+static hipError_t batch_enqueue(frh_batch *b, hipStream_t st) {
+    frh_context *ctx = b->ctx;
+    hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, st, b->d_counters, 2u);
+    const uint32_t n = (uint32_t)b->plans.size();
+    BatchList L{};
+    L.lut8 = ctx->d_lut8;
+    L.wtab = ctx->d_wtab;
+    if (b->total_ahd) {
+        const uint32_t *ids = b->d_lists, *off = b->d_lists + b->n_ahd;
+        L.n = b->n_ahd;
+        L.total = b->total_ahd;
+        L.counter = b->d_counters;
+        const uint32_t waves = (b->total_ahd + 3) / 4;
+        if (b->fast_ahd) {
+            const uint32_t g = std::min(resident_blocks(k_batch_ahd<true>, WALK_WAVES * 64, ctx->num_cus), waves);
+            hipLaunchKernelGGL(k_batch_ahd<true>, dim3(g), dim3(WALK_WAVES * 64), 0, st, L, b->d_frames, ids, off);
+        } else {
+            const uint32_t g = std::min(resident_blocks(k_batch_ahd<false>, WALK_WAVES * 64, ctx->num_cus), waves);
+            hipLaunchKernelGGL(k_batch_ahd<false>, dim3(g), dim3(WALK_WAVES * 64), 0, st, L, b->d_frames, ids, off);
+        }
+    }
+    if (b->total_bil) {
+        const size_t base = (size_t)b->n_ahd + b->n_ahd + 1;
+        const uint32_t *ids = b->d_lists + base, *off = b->d_lists + base + b->n_bil;
+        L.n = b->n_bil;
+        L.total = b->total_bil;
+        L.counter = b->d_counters + 1;
+        const uint32_t waves = (b->total_bil + 3) / 4;
+        if (b->fast_bil) {
+            const uint32_t g =
+                std::min(resident_blocks(k_batch_bilinear<true>, BWALK_WAVES * 64, ctx->num_cus), waves);
+            hipLaunchKernelGGL(k_batch_bilinear<true>, dim3(g), dim3(BWALK_WAVES * 64), 0, st, L, b->d_frames, ids, off);
+        } else {
+            const uint32_t g =
+                std::min(resident_blocks(k_batch_bilinear<false>, BWALK_WAVES * 64, ctx->num_cus), waves);
+            hipLaunchKernelGGL(k_batch_bilinear<false>, dim3(g), dim3(BWALK_WAVES * 64), 0, st, L, b->d_frames, ids, off);
+        }
+    }
+    (void)n;
+    return hipGetLastError();
+}
+
+extern "C" int32_t frh_batch_create(frh_context *ctx, uint32_t n, const frh_frame *frames,
+                                    const frh_frame_info *infos, const frh_develop_spec *specs, void *const *dsts,
+                                    frh_batch **out) {
+    try {
+        if (!ctx || !out || (n && (!frames || !infos || !specs || !dsts)))
+            return fail(FRH_ERROR_INVALID_ARGUMENT, "NULL argument");
+        *out = nullptr;
+        if (n == 0) return fail(FRH_ERROR_INVALID_ARGUMENT, "empty batch");
+        std::unique_ptr<frh_batch> b(new frh_batch);
+        b->ctx = ctx;
+        b->plans.resize(n);
+        for (uint32_t i = 0; i < n; i++) {
+            Plan &p = b->plans[i];
+            int32_t rc = plan_develop(ctx, &frames[i], &infos[i], &specs[i], p);
+            if (rc) return rc;
+            if (!frames[i].data_on_device) return fail(FRH_ERROR_INVALID_ARGUMENT, "batched frames must be device-resident");
+            if (!p.a.simple_cal)
+                return fail(FRH_ERROR_NOT_AVAILABLE,
+                            "batched frames need a black level of at most a 2x2 pattern without row/column deltas");
+            if (specs[i].output2 != OUT_NONE) return fail(FRH_ERROR_NOT_AVAILABLE, "batched frames have one output");
+            if (!dsts[i]) return fail(FRH_ERROR_INVALID_ARGUMENT, "NULL output");
+            p.a.dst = dsts[i];
+            p.a.dst2 = nullptr;
+        }
+        (void)hipSetDevice(ctx->device);
+        int32_t rc = batch_build(b.get());
+        if (rc) return rc;
+        *out = b.release();
+        return FRH_OK;
+    } catch (const std::bad_alloc &) {
+        return fail(FRH_ERROR_OUT_OF_MEMORY, "host allocation failed");
+    } catch (...) {
+        return fail(FRH_ERROR_INTERNAL, "unexpected exception");
+    }
+}
+
+extern "C" int32_t frh_batch_launch(frh_batch *b, void *stream) {
+    if (!b) return fail(FRH_ERROR_INVALID_ARGUMENT, "NULL batch");
+    (void)hipSetDevice(b->ctx->device);
+    hipStream_t st = (hipStream_t)stream;
+    hipError_t e;
+    if (b->graph) {
+        if ((e = hipGraphLaunch(b->graph, st)) != hipSuccess) return hip_fail(e, "graph launch");
+        return FRH_OK;
+    }
+    if ((e = batch_enqueue(b, st)) != hipSuccess) return hip_fail(e, "batch launch");
+    return FRH_OK;
+}
+
+extern "C" int32_t frh_batch_capture(frh_batch *b, void *stream) {
+    if (!b) return fail(FRH_ERROR_INVALID_ARGUMENT, "NULL batch");
+    if (!stream) return fail(FRH_ERROR_INVALID_ARGUMENT, "capture needs a stream other than the null stream");
+    (void)hipSetDevice(b->ctx->device);
+    hipStream_t st = (hipStream_t)stream;
+    if (b->graph) {
+        (void)hipGraphExecDestroy(b->graph);
+        (void)hipGraphDestroy(b->graph_def);
+        b->graph = nullptr;
+        b->graph_def = nullptr;
+    }
+    hipError_t e;
+    if ((e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+        return hip_fail(e, "begin capture");
+    const hipError_t le = batch_enqueue(b, st);
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(st, &g);
+    if (le != hipSuccess || e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return hip_fail(le != hipSuccess ? le : e, "capture");
+    }
+    hipGraphExec_t x = nullptr;
+    if ((e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0)) != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return hip_fail(e, "graph instantiate");
+    }
+    b->graph = x;
+    b->graph_def = g;
+    return FRH_OK;
+}
+
+extern "C" int32_t frh_batch_autotune(frh_batch *b, void *stream, int32_t iters, uint32_t *best_ahd,
+                                      uint32_t *best_bil) {
+    try {
+        if (!b) return fail(FRH_ERROR_INVALID_ARGUMENT, "NULL batch");
+        if (iters < 1) iters = 1;
+        (void)hipSetDevice(b->ctx->device);
+        hipStream_t st = (hipStream_t)stream;
+        const bool had_graph = b->graph != nullptr;
+        hipEvent_t e0, e1;
+        if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+            return fail(FRH_ERROR_GPU_UNAVAILABLE, "hipEventCreate failed");
+        int32_t rc = FRH_OK;
+        auto time_it = [&](float *ms) -> int32_t {
+            int32_t r = batch_build(b);
+            if (r) return r;
+            hipError_t e = batch_enqueue(b, st);  // warm
+            (void)hipEventRecord(e0, st);
+            for (int i = 0; i < iters && e == hipSuccess; i++) e = batch_enqueue(b, st);
+            (void)hipEventRecord(e1, st);
+            if (e == hipSuccess) e = hipEventSynchronize(e1);
+            if (e != hipSuccess) return hip_fail(e, "autotune launch");
+            (void)hipEventElapsedTime(ms, e0, e1);
+            return FRH_OK;
+        };
+        // strip heights are tuned per demosaic kind, AHD first, each with the other at its best
+        uint32_t best_a = 64, best_b = 32;
+        const uint32_t cand_a[] = {24, 32, 40, 48, 56, 64, 72, 80, 96, 112, 128};
+        const uint32_t cand_b[] = {8, 12, 16, 24, 32, 48, 64};
+        b->bil_rows = best_b;
+        if (b->total_ahd) {
+            float best = 1e30f;
+            for (uint32_t r : cand_a) {
+                b->ahd_rows = r;
+                float ms = 0.0f;
+                if ((rc = time_it(&ms)) != FRH_OK) break;
+                if (ms < best) best = ms, best_a = r;
+            }
+        }
+        if (rc == FRH_OK && b->total_bil) {
+            b->ahd_rows = best_a;
+            float best = 1e30f;
+            for (uint32_t r : cand_b) {
+                b->bil_rows = r;
+                float ms = 0.0f;
+                if ((rc = time_it(&ms)) != FRH_OK) break;
+                if (ms < best) best = ms, best_b = r;
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (rc != FRH_OK) return rc;
+        b->ahd_rows = best_a;
+        b->bil_rows = best_b;
+        if ((rc = batch_build(b)) != FRH_OK) return rc;
+        if (best_ahd) *best_ahd = best_a;
+        if (best_bil) *best_bil = best_b;
+        if (had_graph && stream) return frh_batch_capture(b, stream);
+        return FRH_OK;
+    } catch (...) {
+        return fail(FRH_ERROR_INTERNAL, "unexpected exception");
+    }
+}
+
+extern "C" int32_t frh_batch_set_strip_rows(frh_batch *b, uint32_t ahd_rows, uint32_t bil_rows) {
+    if (!b) return fail(FRH_ERROR_INVALID_ARGUMENT, "NULL batch");
+    b->ahd_rows = ahd_rows;
+    b->bil_rows = bil_rows;
+    (void)hipSetDevice(b->ctx->device);
+    return batch_build(b);
+}
+
+extern "C" uint64_t frh_batch_items(const frh_batch *b) { return b ? (uint64_t)b->total_ahd + b->total_bil : 0; }
+
+extern "C" void frh_batch_destroy(frh_batch *b) {
+    if (b) (void)hipSetDevice(b->ctx->device);
+    delete b;
+}

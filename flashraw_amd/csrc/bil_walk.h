@@ -1,0 +1,165 @@
+// Bilinear develop, register row walker (gfx950).
+#pragma once
+
+#include "develop_common.h"
+
+namespace frh {
+
+// Bilinear develop, register row walker: lane j owns frame columns xs + 2j and xs + 2j + 1, the
+// wave walks down a strip keeping the three normalised rows the 3x3 neighbourhood needs in
+// registers, horizontal neighbours come from the adjacent lanes (DPP). With the Bayer phase a
+// template parameter and rows unrolled by two, every pixel evaluates only its own formulas
+// (process.rs:1076-1096). Lanes 1..62 hold results (124 columns per strip); pixels on the frame
+// border are developed by extra blocks with the in-frame neighbour means (process.rs:1109-1126).
+
+constexpr int BWALK_COLS = 124;
+constexpr int BWALK_WAVES = 4;
+constexpr int BPF = 4;  // rows fetched ahead (each slot is loaded and consumed in place, never moved)
+
+struct BWalk {
+    float n[3][2];           // normalised rows R-2, R-1, R; [k] even / odd column
+    float nl[3];             // odd column of lane - 1
+    float nr[3];             // even column of lane + 1
+    uint32_t pf[BPF];        // raw pairs fetched ahead, slot (row - start) % BPF
+};
+
+template <int RP, int PY, int SLOT, bool FAST, bool HOT>
+__device__ __forceinline__ void bwalk_row(const DevArgs &a, const Lds &s, BWalk &W, int R, int xs, int lane,
+                                          int out_row0, int out_row1, int out_x0, int out_x1, bool in0, bool in1,
+                                          __amdgpu_buffer_rsrc_t rs, __amdgpu_buffer_rsrc_t rd) {
+    const int x0 = xs + 2 * lane;
+#pragma unroll
+    for (int r = 0; r < 2; r++) {
+        W.n[r][0] = W.n[r + 1][0];
+        W.n[r][1] = W.n[r + 1][1];
+        W.nl[r] = W.nl[r + 1];
+        W.nr[r] = W.nr[r + 1];
+    }
+    const uint32_t raw = W.pf[SLOT];
+    W.pf[SLOT] = fetch_pair<HOT>(a, rs, x0, R + BPF);
+    norm_pair<PY, HOT>(a, s, raw, x0, R, R >= 0 && R < (int)a.height, in0, in1, W.n[2][0], W.n[2][1]);
+    W.nl[2] = from_left(W.n[2][1]);
+    W.nr[2] = from_right(W.n[2][0]);
+    const int ro = R - 1;
+    if (ro < out_row0 || ro >= out_row1 || lane < 1 || lane > 62) return;
+    constexpr int PO = PY ^ 1;
+    float o[2][3];
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        const float own = W.n[1][k];
+        const float l = k == 0 ? W.nl[1] : W.n[1][0], r = k == 0 ? W.n[1][1] : W.nr[1];
+        const float u = W.n[0][k], d = W.n[2][k];
+        float rgb[3];
+        const int col = site<RP>(PO, k);
+        if (col == 1) {
+            const int ch = site<RP>(PO, k ^ 1);  // the colour of this row's other sites
+            const float hv = (l + r) * 0.5f, vv = (u + d) * 0.5f;
+            rgb[0] = ch == 0 ? hv : vv;
+            rgb[1] = own;
+            rgb[2] = ch == 0 ? vv : hv;
+        } else {
+            const float ul = k == 0 ? W.nl[0] : W.n[0][0], ur = k == 0 ? W.n[0][1] : W.nr[0];
+            const float dl = k == 0 ? W.nl[2] : W.n[2][0], dr = k == 0 ? W.n[2][1] : W.nr[2];
+            const float orth = (l + r + u + d) * 0.25f;
+            const float diag = (ul + ur + dl + dr) * 0.25f;
+            rgb[0] = col == 0 ? own : diag;
+            rgb[1] = orth;
+            rgb[2] = col == 0 ? diag : own;
+        }
+        if (FAST) {
+            o[k][0] = __uint_as_float(finish_rgba8_fast(a, s, rgb[0], rgb[1], rgb[2], false));
+        } else {
+            finish(a, rgb[0], rgb[1], rgb[2], false, o[k][0], o[k][1], o[k][2]);
+        }
+    }
+    const int dx = x0 - (int)a.crop_x, dy = ro - (int)a.crop_y;
+    const bool st0 = x0 >= out_x0 && x0 < out_x1, st1 = x0 + 1 >= out_x0 && x0 + 1 < out_x1;
+    if (FAST) {
+        const uint32_t w0 = __float_as_uint(o[0][0]), w1 = __float_as_uint(o[1][0]);
+        if (st0 && st1) {
+            store_rgba8_pair(a, rd, (uint32_t)dx, (uint32_t)dy, w0, w1, 2);
+        } else if (st0) {
+            store_rgba8_pair(a, rd, (uint32_t)dx, (uint32_t)dy, w0, 0, 1);
+        } else if (st1) {
+            store_rgba8_pair(a, rd, (uint32_t)dx + 1, (uint32_t)dy, w1, 0, 1);
+        }
+    } else {
+        if (st0) store_both(a, s, (uint32_t)dx, (uint32_t)dy, o[0][0], o[0][1], o[0][2]);
+        if (st1) store_both(a, s, (uint32_t)dx + 1, (uint32_t)dy, o[1][0], o[1][1], o[1][2]);
+    }
+}
+
+// One wave's strip: rows from row0 - 2 (even) until the last output row out_row1 - 1 = R - 1,
+// four rows per trip so every prefetch slot is static.
+template <int RP, bool FAST, bool HOT>
+__device__ __forceinline__ void bwalk_strip(const DevArgs &a, const Lds &s, int xs, int lane, int row0, int out_row0,
+                                            int out_row1, int out_x0, int out_x1) {
+    BWalk W;
+    {
+        float *f = reinterpret_cast<float *>(&W);
+        for (int i = 0; i < (int)(sizeof(BWalk) / 4) - BPF; i++) f[i] = 0.0f;
+    }
+    const int x0 = xs + 2 * lane;
+    const bool in0 = x0 >= 0 && x0 < (int)a.width, in1 = x0 + 1 >= 0 && x0 + 1 < (int)a.width;
+    const __amdgpu_buffer_rsrc_t rs = make_rsrc(a.src, src_bytes(a));
+    const __amdgpu_buffer_rsrc_t rd = make_rsrc(a.dst, a.dst_w * a.dev_h * 4u);
+#pragma unroll
+    for (int i = 0; i < BPF; i++) W.pf[i] = fetch_pair<HOT>(a, rs, x0, row0 - 2 + i);
+    for (int R = row0 - 2; R < out_row1 + 1; R += BPF) {
+        bwalk_row<RP, 0, 0, FAST, HOT>(a, s, W, R, xs, lane, out_row0, out_row1, out_x0, out_x1, in0, in1, rs, rd);
+        bwalk_row<RP, 1, 1, FAST, HOT>(a, s, W, R + 1, xs, lane, out_row0, out_row1, out_x0, out_x1, in0, in1, rs,
+                                       rd);
+        bwalk_row<RP, 0, 2, FAST, HOT>(a, s, W, R + 2, xs, lane, out_row0, out_row1, out_x0, out_x1, in0, in1, rs,
+                                       rd);
+        bwalk_row<RP, 1, 3, FAST, HOT>(a, s, W, R + 3, xs, lane, out_row0, out_row1, out_x0, out_x1, in0, in1, rs,
+                                       rd);
+    }
+}
+
+struct BWalkPlan {
+    int x_first, x_end, y_first, y_end;
+    int xs0, ys0;
+    int strips, chunks, rows;
+    Border border;
+};
+
+// The bilinear walker's work split of the developed band of `a`.
+__host__ inline BWalkPlan make_bwalk_plan(const DevArgs &a, int rows_per_strip) {
+    BWalkPlan P;
+    P.border = make_border(a, 1);
+    P.rows = rows_per_strip < 2 ? 2 : (rows_per_strip & ~1);
+    if (P.border.ix1 > P.border.ix0) {
+        P.x_first = P.border.ix0 + (int)a.crop_x;
+        P.x_end = P.border.ix1 + (int)a.crop_x;
+        P.y_first = P.border.iy0 + (int)a.crop_y;
+        P.y_end = P.border.iy1 + (int)a.crop_y;
+        P.xs0 = P.x_first & ~1;
+        P.ys0 = P.y_first & ~1;
+        P.strips = (P.x_end - P.xs0 + BWALK_COLS - 1) / BWALK_COLS;
+        P.chunks = (P.y_end - P.ys0 + P.rows - 1) / P.rows;
+    } else {
+        P.x_first = P.x_end = P.y_first = P.y_end = P.xs0 = P.ys0 = 0;
+        P.strips = P.chunks = 0;
+    }
+    return P;
+}
+
+// Strip `item` of plan P, developed by the calling wave.
+template <int RP, bool FAST>
+__device__ __forceinline__ void bwalk_item(const DevArgs &a, const BWalkPlan &P, const Lds &s, int item) {
+    const int lane = threadIdx.x & 63;
+    const int sx = item % P.strips, sy = item / P.strips;
+    const int xs = P.xs0 + BWALK_COLS * sx - 2;  // lane 1 holds output column xs + 2
+    const int row0 = P.ys0 + P.rows * sy;
+    const int out_row0 = row0 > P.y_first ? row0 : P.y_first;
+    const int out_row1 = row0 + P.rows < P.y_end ? row0 + P.rows : P.y_end;
+    const int out_x0 = max(xs + 2, P.x_first), out_x1 = min(xs + 2 + BWALK_COLS, P.x_end);
+    const bool hot = a.simple_cal && xs >= 0 && xs + 128 <= (int)a.width && ((uintptr_t)a.src & 3u) == 0 &&
+                     (a.stride & 1u) == 0;
+    if (hot)
+        bwalk_strip<RP, FAST, true>(a, s, xs, lane, row0, out_row0, out_row1, out_x0, out_x1);
+    else
+        bwalk_strip<RP, FAST, false>(a, s, xs, lane, row0, out_row0, out_row1, out_x0, out_x1);
+}
+
+}  // namespace frh

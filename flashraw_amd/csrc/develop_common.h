@@ -212,6 +212,13 @@ __device__ __forceinline__ uint32_t fetch_pair(const DevArgs &a, __amdgpu_buffer
 
 // Calib::norm (process.rs:288-298) of raw sample v at frame (x, y), in the frame.
 __device__ __forceinline__ float norm_raw(const DevArgs &a, const Lds &s, float v, uint32_t x, uint32_t y) {
+    if (a.simple_cal) {
+        // at most a 2x2 pattern and no deltas: the per-parity values (the same numbers as the table)
+        const uint32_t i = (y & 1u) * 2u + (x & 1u);
+        const float b = i == 0 ? a.qb[0] : i == 1 ? a.qb[1] : i == 2 ? a.qb[2] : a.qb[3];
+        const float k = i == 0 ? a.qs[0] : i == 1 ? a.qs[1] : i == 2 ? a.qs[2] : a.qs[3];
+        return (v - b) * k;
+    }
     uint32_t X = x + a.ox, Y = y + a.oy;
     uint32_t i = a.cal_pow2 ? (Y & (a.cal_ph - 1u)) * a.cal_pw + (X & (a.cal_pw - 1u))
                             : (Y % a.cal_ph) * a.cal_pw + X % a.cal_pw;

@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "flashraw_hip.h"
+#include "frh_device.h"
 
 namespace frh {
 
@@ -54,6 +55,25 @@ struct frh_context {
     void release(void *p, uint64_t bytes);
     ~frh_context();
 };
+
+namespace frh {
+
+// A validated develop: kernel arguments (without the buffers a launch names) and the outputs'
+// geometry (run.rs:439-760).
+struct Plan {
+    DevArgs a;
+    bool ahd;
+    uint32_t out_w[2], out_h[2], ch[2];
+    uint64_t bytes[2];
+    int32_t fmt[2];
+    uint64_t in_row_bytes;
+};
+
+// Validate a develop request and fill `p` (status code; the message is the thread's last error).
+int32_t plan_develop(const frh_context *ctx, const frh_frame *f, const frh_frame_info *info,
+                     const frh_develop_spec *spec, Plan &p);
+
+}  // namespace frh
 
 struct frh_image {
     frh_context *ctx;

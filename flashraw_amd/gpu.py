@@ -177,6 +177,20 @@ def lib():
             L.frh_context_stream.restype = vp
             L.frh_denoise_cfa.argtypes = [vp, ctypes.POINTER(_DnSpec), vp, vp, vp, vp]
             L.frh_denoise_cfa.restype = i32
+            L.frh_batch_create.argtypes = [vp, u32, ctypes.POINTER(_Frame), ctypes.POINTER(_FrameInfo),
+                                           ctypes.POINTER(_DevelopSpec), ctypes.POINTER(vp), ctypes.POINTER(vp)]
+            L.frh_batch_create.restype = i32
+            L.frh_batch_launch.argtypes = [vp, vp]
+            L.frh_batch_launch.restype = i32
+            L.frh_batch_capture.argtypes = [vp, vp]
+            L.frh_batch_capture.restype = i32
+            L.frh_batch_autotune.argtypes = [vp, vp, i32, ctypes.POINTER(u32), ctypes.POINTER(u32)]
+            L.frh_batch_autotune.restype = i32
+            L.frh_batch_set_strip_rows.argtypes = [vp, u32, u32]
+            L.frh_batch_set_strip_rows.restype = i32
+            L.frh_batch_items.argtypes = [vp]
+            L.frh_batch_items.restype = u64
+            L.frh_batch_destroy.argtypes = [vp]
             _lib = L
         return _lib
 
@@ -625,6 +639,12 @@ class GpuContext:
                                      _device_ptr(dst), n1, None if dst2 is None else _device_ptr(dst2), n2, sp))
         del k1, k2, k3
 
+    def batch(self, jobs) -> "Batch":
+        """A batched develop of `jobs`, a sequence of (frame, info, spec, dst): device-resident
+        frames developed into the device buffers `dst` by one persistent launch per demosaic
+        kind (frh_batch_create)."""
+        return Batch(self, jobs)
+
     def denoise_cfa(self, spec: DenoiseSpec, src, dst=None, dst_f32=None, stream=None) -> None:
         """Denoise the device u16 mosaic `src` into `dst` (u16) and/or `dst_f32` (f32, DN above
         black), torch tensors or device addresses, on `stream` (frh_denoise_cfa)."""
@@ -646,6 +666,57 @@ class GpuContext:
         if getattr(self, "_h", None):
             try:
                 lib().frh_context_destroy(self._h)
+            except Exception:
+                pass
+            self._h = None
+
+
+class Batch:
+    """frh_batch: launch, capture into a HIP graph, autotune the strip heights."""
+
+    def __init__(self, ctx: "GpuContext", jobs):
+        self.ctx = ctx
+        n = len(jobs)
+        frames = (_Frame * n)()
+        infos = (_FrameInfo * n)()
+        specs = (_DevelopSpec * n)()
+        dsts = (ctypes.c_void_p * n)()
+        self._keep = []
+        for k, (frame, info, spec, dst) in enumerate(jobs):
+            f, k1 = frame._c()
+            i, k2 = info._c()
+            s, k3 = spec._c()
+            frames[k], infos[k], specs[k] = f, i, s
+            dsts[k] = _device_ptr(dst)
+            self._keep += [k1, k2, k3, dst, frame]
+        h = ctypes.c_void_p()
+        check(lib().frh_batch_create(ctx._h, n, frames, infos, specs, dsts, ctypes.byref(h)))
+        self._h = h.value
+
+    def launch(self, stream=None) -> None:
+        check(lib().frh_batch_launch(self._h, self.ctx.stream_handle(stream)))
+
+    def capture(self, stream) -> None:
+        """Capture the launch sequence on `stream` (a torch stream) into a HIP graph."""
+        check(lib().frh_batch_capture(self._h, self.ctx.stream_handle(stream)))
+
+    def autotune(self, stream=None, iters: int = 2):
+        a, b = ctypes.c_uint32(), ctypes.c_uint32()
+        check(lib().frh_batch_autotune(self._h, self.ctx.stream_handle(stream), iters, ctypes.byref(a),
+                                       ctypes.byref(b)))
+        return a.value, b.value
+
+    def set_strip_rows(self, ahd: int = 0, bilinear: int = 0) -> None:
+        check(lib().frh_batch_set_strip_rows(self._h, ahd, bilinear))
+
+    @property
+    def items(self) -> int:
+        return int(lib().frh_batch_items(self._h))
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            try:
+                lib().frh_batch_destroy(self._h)
             except Exception:
                 pass
             self._h = None

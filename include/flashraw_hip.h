@@ -183,6 +183,32 @@ void *frh_context_stream(const frh_context *ctx);
 int32_t frh_develop_output_size(const frh_frame *frame, const frh_develop_spec *spec, int32_t output,
                                 uint32_t *width, uint32_t *height, uint32_t *channels, uint64_t *bytes);
 
+/* ---- batched develop ----
+ * No reference counterpart (the reference develops one frame per GpuContext::run, run.rs:426):
+ * many device-resident frames of any size, Bayer phase, bit depth, white balance and colour
+ * matrix developed by one persistent launch per demosaic kind (AHD frames, bilinear frames),
+ * which can be captured once into a HIP graph and replayed. Every frame's bytes are those of its
+ * own single-frame develop. Frames use a black level of at most a 2x2 pattern without row or
+ * column deltas, and one output each. */
+typedef struct frh_batch frh_batch;
+/* Validate n develops (each as frh_develop_into would) into the device buffers dsts[i]. */
+int32_t frh_batch_create(frh_context *ctx, uint32_t n, const frh_frame *frames, const frh_frame_info *infos,
+                         const frh_develop_spec *specs, void *const *dsts, frh_batch **out);
+/* Enqueue the batch on `stream` (NULL: the null stream): the captured graph when there is one. */
+int32_t frh_batch_launch(frh_batch *batch, void *stream);
+/* Capture the batch's launch sequence on `stream` (not NULL) into a HIP graph that later
+ * frh_batch_launch calls replay. */
+int32_t frh_batch_capture(frh_batch *batch, void *stream);
+/* Time candidate strip heights (the walkers' tile shape) on the batch itself, `iters` launches
+ * each, and keep the fastest per demosaic kind (re-capturing a captured graph). */
+int32_t frh_batch_autotune(frh_batch *batch, void *stream, int32_t iters, uint32_t *best_ahd_rows,
+                           uint32_t *best_bilinear_rows);
+/* Fix the strip heights (0: each frame's default). */
+int32_t frh_batch_set_strip_rows(frh_batch *batch, uint32_t ahd_rows, uint32_t bilinear_rows);
+/* Work items (strips and border groups) in the batch's lists. */
+uint64_t frh_batch_items(const frh_batch *batch);
+void frh_batch_destroy(frh_batch *batch);
+
 /* ---- CFA-plane DCT block denoiser ----
  * No reference counterpart as an entry point: the reference denoises developed linear RGB
  * (GpuContext::denoise, crates/flashraw-gpu/src/denoise.rs:817). This one works on the mosaic's
