@@ -7,6 +7,7 @@ resident in HBM and the output written to HBM. Prints one JSON line.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--demosaic ahd|bilinear] [--strip-rows R]
   python bench.py --mode denoise [--precision f16|bf16|f16x3|f32]   # the CFA-plane DCT denoiser
+  python bench.py --mode batch [--frames 64]   # 64 mixed frames (12-61 MP) in one persistent launch
   python bench.py --mode copy   # a plain device copy of 1 GiB (PMC byte-counter calibration)
 """
 import argparse
@@ -29,7 +30,8 @@ def _parse():
     p.add_argument("--steps", type=int, default=200)
     p.add_argument("--warmup", type=int, default=20)
     p.add_argument("--demosaic", default="ahd", choices=["ahd", "bilinear"])
-    p.add_argument("--mode", default="develop", choices=["develop", "denoise", "copy"])
+    p.add_argument("--mode", default="develop", choices=["develop", "denoise", "batch", "copy"])
+    p.add_argument("--frames", type=int, default=64, help="batch mode: frames in the batch")
     p.add_argument("--precision", default="f16", choices=["f16", "bf16", "f16x3", "f32"],
                    help="denoise mode: MFMA operand precision, or the scalar f32 variant")
     p.add_argument("--seg-rows", type=int, default=0, help="denoise mode: 8-row tile rows per wave (0: default)")
@@ -148,6 +150,101 @@ def denoise_mode(args):
     print(json.dumps(line))
 
 
+def batch_frames(n, seed=3):
+    """The batched workload: n frames with sizes 12, 20, 24, 33, 45 or 61 MP at 3:2 or 4:3, a
+    uniformly drawn Bayer phase, 12 or 14 bits, WB R and B in [1.2, 2.8] and the colour matrix
+    jittered by up to 10 %, AHD to RGBA8 (sides rounded to even)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        mp = float(rng.choice([12, 20, 24, 33, 45, 61]))
+        aspect = 1.5 if rng.random() < 0.5 else 4.0 / 3.0
+        h = int(round((mp * 1e6 / aspect) ** 0.5 / 2)) * 2
+        w = int(round(h * aspect / 2)) * 2
+        out.append(dict(w=w, h=h, cfa=["RGGB", "BGGR", "GRBG", "GBRG"][int(rng.integers(0, 4))],
+                        bits=12 if rng.random() < 0.5 else 14, wb=(float(rng.uniform(1.2, 2.8)), 1.0,
+                                                                      float(rng.uniform(1.2, 2.8)), 1.0),
+                        jitter=rng.uniform(0.9, 1.1, (3, 3)), seed=1000 + k))
+    return out
+
+
+def batch_mode(args):
+    """Batched AHD develop of mixed frames: one persistent launch per Bayer phase, captured in a
+    HIP graph, against the same frames launched back to back one by one. A step is one develop
+    of every frame of the batch."""
+    import numpy as np
+    import torch
+
+    import flashraw_amd as F
+    from flashraw_amd import synth
+
+    ctx = F.context()
+    specs = batch_frames(args.frames)
+    jobs, plans, total_px = [], [], 0
+    for f in specs:
+        white = (1 << f["bits"]) - 1
+        black = 64.0 * (f["bits"] - 10)
+        mos = synth.mosaic(f["w"], f["h"], f["cfa"], black=black, white=float(white), seed=f["seed"])
+        info = F.FrameInfo(f["w"], f["h"], F.CfaPattern.bayer(f["cfa"]), F.BlackLevel.uniform(black),
+                           F.WhiteLevel.uniform(white))
+        matrix = tuple(tuple(float(v) for v in row) for row in np.asarray(synth.MATRIX) * f["jitter"])
+        spec = F.DevelopSpec(F.Rect.full(f["w"], f["h"]), wb=f["wb"], cam_to_rgb=matrix,
+                             output=F.DevelopOutput.Rgba8, edge=F.ahd_spec(info))
+        d_in = torch.from_numpy(mos.view(np.int16)).to("cuda")
+        dst = torch.empty((f["h"], f["w"], 4), dtype=torch.uint8, device="cuda")
+        frame = F.GpuFrame.from_u16(d_in, f["w"], f["h"])
+        jobs.append((frame, info, spec, dst))
+        plans.append((ctx.prepare(frame, info, spec), d_in, dst))
+        total_px += f["w"] * f["h"]
+        del mos
+    stream = torch.cuda.Stream()
+    batch = ctx.batch(jobs)
+    rows_ahd, _ = batch.autotune(stream=stream, iters=1)
+    batch.capture(stream)
+    # the single-frame baseline uses the single-frame autotuner's height for every frame
+    plans[0][0].autotune(plans[0][1], plans[0][2], stream=stream)
+
+    def timed(fn, steps):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps
+
+    def singles():
+        for p, d_in, dst in plans:
+            p.launch(d_in, dst, stream=stream)
+
+    steps = max(1, args.steps)
+    t_batch = timed(lambda: batch.launch(stream=stream), steps)
+    ref = [jobs[k][3].cpu().numpy().copy() for k in (0, len(jobs) - 1)]
+    t_single = timed(singles, steps)
+    same = all(np.array_equal(jobs[k][3].cpu().numpy(), r) for k, r in zip((0, len(jobs) - 1), ref))
+    mp = total_px / 1e6
+    algo_bytes = total_px * 6
+    line = {
+        "metric": "MP/s batched AHD develop (64 mixed frames 12-61 MP, one persistent launch per phase, HIP graph)",
+        "value": round(mp / t_batch, 1), "unit": "MP/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+        "ms_per_step": round(t_batch * 1e3, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{len(jobs)} frames, {mp:.0f} MP total, u16 -> RGBA8, AHD, mixed phase/bits/WB/matrix",
+                   "frames": len(jobs), "total_mp": round(mp, 1), "autotuned_ahd_rows": rows_ahd,
+                   "back_to_back_single_frame_mps": round(mp / t_single, 1),
+                   "back_to_back_ms": round(t_single * 1e3, 3),
+                   "batch_over_single": round(t_single / t_batch, 4),
+                   "batch_equals_single_frame_bytes": same},
+        "roofline": {"bound": "hbm", "achieved": round(algo_bytes / t_batch / 1e9, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(algo_bytes / t_batch / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
+                     "algorithmic_bytes_per_launch": algo_bytes},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line))
+
+
 def main():
     args = _parse()
     if args.gpus != 1:
@@ -158,6 +255,8 @@ def main():
         return copy_mode(args)
     if args.mode == "denoise":
         return denoise_mode(args)
+    if args.mode == "batch":
+        return batch_mode(args)
 
     import numpy as np
     import torch

@@ -1,5 +1,6 @@
 // Batched develop: many frames of any size, Bayer phase, bit depth, white balance and colour
-// matrix in one persistent launch per demosaic kind.
+// matrix in one persistent launch per demosaic kind and Bayer phase (the phase is a template
+// parameter of the walkers, as in the single-frame kernels).
 //
 // Each frame is split exactly as its single-frame launch splits it (the row walkers' strips
 // plus the bilinear border pixels, the same kernels' device code), and every strip or group of
@@ -75,7 +76,7 @@ __device__ __forceinline__ void load_tables(const BatchList &L, Lds &s, float *w
 
 // Frames of a batch use the scalar calibration (at most a 2x2 black pattern, no deltas), so the
 // shared Lds holds only the sRGB table; its black/scale arrays are never read.
-template <bool FAST>
+template <int RP, bool FAST>
 __global__ __launch_bounds__(WALK_WAVES * 64) void k_batch_ahd(BatchList L, const BatchFrame *__restrict__ frames,
                                                                     const uint32_t *__restrict__ ids,
                                                                     const uint32_t *__restrict__ item_off) {
@@ -89,12 +90,7 @@ __global__ __launch_bounds__(WALK_WAVES * 64) void k_batch_ahd(BatchList L, cons
         const BatchFrame &F = frames[__builtin_amdgcn_readfirstlane(ids[k])];
         const uint32_t local = item - item_off[k];
         if (local < F.walk_items) {
-            switch (F.rp) {
-            case 0: walk_item<0, FAST>(F.a, F.wp, s, wt, (int)local); break;
-            case 1: walk_item<1, FAST>(F.a, F.wp, s, wt, (int)local); break;
-            case 2: walk_item<2, FAST>(F.a, F.wp, s, wt, (int)local); break;
-            default: walk_item<3, FAST>(F.a, F.wp, s, wt, (int)local); break;
-            }
+            walk_item<RP, FAST>(F.a, F.wp, s, wt, (int)local);
         } else {
             const uint32_t base = (local - F.walk_items) * BORDER_ITEM + (threadIdx.x & 63);
             for (uint32_t j = 0; j < BORDER_ITEM; j += 64)
@@ -103,7 +99,7 @@ __global__ __launch_bounds__(WALK_WAVES * 64) void k_batch_ahd(BatchList L, cons
     }
 }
 
-template <bool FAST>
+template <int RP, bool FAST>
 __global__ __launch_bounds__(BWALK_WAVES * 64) void k_batch_bilinear(BatchList L, const BatchFrame *__restrict__ frames,
                                                                     const uint32_t *__restrict__ ids,
                                                                     const uint32_t *__restrict__ item_off) {
@@ -116,12 +112,7 @@ __global__ __launch_bounds__(BWALK_WAVES * 64) void k_batch_bilinear(BatchList L
         const BatchFrame &F = frames[__builtin_amdgcn_readfirstlane(ids[k])];
         const uint32_t local = item - item_off[k];
         if (local < F.walk_items) {
-            switch (F.rp) {
-            case 0: bwalk_item<0, FAST>(F.a, F.bp, s, (int)local); break;
-            case 1: bwalk_item<1, FAST>(F.a, F.bp, s, (int)local); break;
-            case 2: bwalk_item<2, FAST>(F.a, F.bp, s, (int)local); break;
-            default: bwalk_item<3, FAST>(F.a, F.bp, s, (int)local); break;
-            }
+            bwalk_item<RP, FAST>(F.a, F.bp, s, (int)local);
         } else {
             const uint32_t base = (local - F.walk_items) * BORDER_ITEM + (threadIdx.x & 63);
             for (uint32_t j = 0; j < BORDER_ITEM; j += 64)
@@ -135,6 +126,8 @@ __global__ __launch_bounds__(BWALK_WAVES * 64) void k_batch_bilinear(BatchList L
 
 using namespace frh;
 
+constexpr int NLIST = 8;  // (AHD, bilinear) x four Bayer phases
+
 struct frh_batch {
     frh_context *ctx = nullptr;
     std::vector<Plan> plans;
@@ -142,20 +135,25 @@ struct frh_batch {
     uint32_t ahd_rows = 0, bil_rows = 0;  // 0: each frame's own default
     // device copies
     BatchFrame *d_frames = nullptr;
-    uint32_t *d_lists = nullptr;  // ids and offsets of both lists
+    uint32_t *d_lists = nullptr;  // per list: ids, then offsets
     uint32_t *d_counters = nullptr;
     size_t n_list_words = 0;
-    uint32_t n_ahd = 0, n_bil = 0, total_ahd = 0, total_bil = 0;
-    bool fast_ahd = true, fast_bil = true;
+    uint32_t n[NLIST] = {}, total[NLIST] = {};
+    size_t base[NLIST] = {};  // word offset of each list's ids in d_lists
+    bool fast[NLIST] = {};
     hipGraphExec_t graph = nullptr;
     hipGraph_t graph_def = nullptr;
-    hipStream_t graph_stream = nullptr;
     ~frh_batch() {
         if (graph) (void)hipGraphExecDestroy(graph);
         if (graph_def) (void)hipGraphDestroy(graph_def);
         if (d_frames) (void)hipFree(d_frames);
         if (d_lists) (void)hipFree(d_lists);
         if (d_counters) (void)hipFree(d_counters);
+    }
+    uint64_t items(bool ahd) const {
+        uint64_t t = 0;
+        for (int l = ahd ? 0 : 4; l < (ahd ? 4 : 8); l++) t += total[l];
+        return t;
     }
 };
 
@@ -165,8 +163,11 @@ static int32_t batch_build(frh_batch *b) {
     frh_context *ctx = b->ctx;
     const size_t n = b->plans.size();
     b->frames.assign(n, BatchFrame{});
-    std::vector<uint32_t> ids_a, ids_b, off_a{0}, off_b{0};
-    b->fast_ahd = b->fast_bil = true;
+    std::vector<uint32_t> ids[NLIST], off[NLIST];
+    for (int l = 0; l < NLIST; l++) {
+        off[l].push_back(0);
+        b->fast[l] = true;
+    }
     for (size_t i = 0; i < n; i++) {
         const Plan &p = b->plans[i];
         BatchFrame &F = b->frames[i];
@@ -174,32 +175,29 @@ static int32_t batch_build(frh_batch *b) {
         F.a.band_y0 = 0;
         F.a.band_y1 = p.a.dev_h;
         F.rp = red_position(F.a);
+        const int l = (p.ahd ? 0 : 4) + F.rp;
         if (p.ahd) {
             F.wp = make_walk_plan(F.a, (int)(b->ahd_rows ? b->ahd_rows : p.a.ahd_rows));
             F.walk_items = (uint32_t)(F.wp.strips * F.wp.chunks);
             F.items = F.walk_items + (F.wp.border.count + BORDER_ITEM - 1) / BORDER_ITEM;
-            b->fast_ahd = b->fast_ahd && fast_rgba8(F.a);
-            ids_a.push_back((uint32_t)i);
-            off_a.push_back(off_a.back() + F.items);
         } else {
             F.bp = make_bwalk_plan(F.a, (int)(b->bil_rows ? b->bil_rows : p.a.bil_rows));
             F.walk_items = (uint32_t)(F.bp.strips * F.bp.chunks);
             F.items = F.walk_items + (F.bp.border.count + BORDER_ITEM - 1) / BORDER_ITEM;
-            b->fast_bil = b->fast_bil && fast_rgba8(F.a);
-            ids_b.push_back((uint32_t)i);
-            off_b.push_back(off_b.back() + F.items);
         }
+        if (F.items == 0) continue;
+        b->fast[l] = b->fast[l] && fast_rgba8(F.a);
+        ids[l].push_back((uint32_t)i);
+        off[l].push_back(off[l].back() + F.items);
     }
-    b->n_ahd = (uint32_t)ids_a.size();
-    b->n_bil = (uint32_t)ids_b.size();
-    b->total_ahd = off_a.back();
-    b->total_bil = off_b.back();
-    // layout: ids_a | off_a | ids_b | off_b
     std::vector<uint32_t> words;
-    words.insert(words.end(), ids_a.begin(), ids_a.end());
-    words.insert(words.end(), off_a.begin(), off_a.end());
-    words.insert(words.end(), ids_b.begin(), ids_b.end());
-    words.insert(words.end(), off_b.begin(), off_b.end());
+    for (int l = 0; l < NLIST; l++) {
+        b->n[l] = (uint32_t)ids[l].size();
+        b->total[l] = off[l].back();
+        b->base[l] = words.size();
+        words.insert(words.end(), ids[l].begin(), ids[l].end());
+        words.insert(words.end(), off[l].begin(), off[l].end());
+    }
     hipError_t e;
     if (!b->d_frames && (e = hipMalloc(&b->d_frames, n * sizeof(BatchFrame))) != hipSuccess)
         return hip_fail(e, "batch descriptors");
@@ -209,7 +207,7 @@ static int32_t batch_build(frh_batch *b) {
         if ((e = hipMalloc(&b->d_lists, words.size() * 4)) != hipSuccess) return hip_fail(e, "batch lists");
         b->n_list_words = words.size();
     }
-    if (!b->d_counters && (e = hipMalloc(&b->d_counters, 2 * sizeof(uint32_t))) != hipSuccess)
+    if (!b->d_counters && (e = hipMalloc(&b->d_counters, NLIST * sizeof(uint32_t))) != hipSuccess)
         return hip_fail(e, "batch counters");
     if ((e = hipMemcpyAsync(b->d_frames, b->frames.data(), n * sizeof(BatchFrame), hipMemcpyHostToDevice,
                             ctx->stream)) != hipSuccess ||
@@ -234,46 +232,48 @@ static uint32_t resident_blocks(K kernel, int threads, int num_cus) {
     return (uint32_t)per_cu * (uint32_t)num_cus;
 }
 
-// The launch sequence: counter reset, then the AHD list, then the bilinear list.
-static hipError_t batch_enqueue(frh_batch *b, hipStream_t st) {
+template <int RP, bool FAST>
+static void launch_list(frh_batch *b, int l, hipStream_t st) {
     frh_context *ctx = b->ctx;
-    hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, st, b->d_counters, 2u);
-    const uint32_t n = (uint32_t)b->plans.size();
     BatchList L{};
     L.lut8 = ctx->d_lut8;
     L.wtab = ctx->d_wtab;
-    if (b->total_ahd) {
-        const uint32_t *ids = b->d_lists, *off = b->d_lists + b->n_ahd;
-        L.n = b->n_ahd;
-        L.total = b->total_ahd;
-        L.counter = b->d_counters;
-        const uint32_t waves = (b->total_ahd + 3) / 4;
-        if (b->fast_ahd) {
-            const uint32_t g = std::min(resident_blocks(k_batch_ahd<true>, WALK_WAVES * 64, ctx->num_cus), waves);
-            hipLaunchKernelGGL(k_batch_ahd<true>, dim3(g), dim3(WALK_WAVES * 64), 0, st, L, b->d_frames, ids, off);
-        } else {
-            const uint32_t g = std::min(resident_blocks(k_batch_ahd<false>, WALK_WAVES * 64, ctx->num_cus), waves);
-            hipLaunchKernelGGL(k_batch_ahd<false>, dim3(g), dim3(WALK_WAVES * 64), 0, st, L, b->d_frames, ids, off);
+    L.n = b->n[l];
+    L.total = b->total[l];
+    L.counter = b->d_counters + l;
+    const uint32_t *ids = b->d_lists + b->base[l], *off = ids + b->n[l];
+    const uint32_t blocks = (b->total[l] + 3) / 4;
+    if (l < 4) {
+        const uint32_t g = std::min(resident_blocks(k_batch_ahd<RP, FAST>, WALK_WAVES * 64, ctx->num_cus), blocks);
+        hipLaunchKernelGGL((k_batch_ahd<RP, FAST>), dim3(g), dim3(WALK_WAVES * 64), 0, st, L, b->d_frames, ids, off);
+    } else {
+        const uint32_t g =
+            std::min(resident_blocks(k_batch_bilinear<RP, FAST>, BWALK_WAVES * 64, ctx->num_cus), blocks);
+        hipLaunchKernelGGL((k_batch_bilinear<RP, FAST>), dim3(g), dim3(BWALK_WAVES * 64), 0, st, L, b->d_frames,
+                           ids, off);
+    }
+}
+
+template <int RP>
+static void launch_list_rp(frh_batch *b, int l, hipStream_t st) {
+    if (b->fast[l])
+        launch_list<RP, true>(b, l, st);
+    else
+        launch_list<RP, false>(b, l, st);
+}
+
+// The launch sequence: counter reset, then each non-empty list (AHD lists first).
+static hipError_t batch_enqueue(frh_batch *b, hipStream_t st) {
+    hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, st, b->d_counters, (uint32_t)NLIST);
+    for (int l = 0; l < NLIST; l++) {
+        if (!b->total[l]) continue;
+        switch (l & 3) {
+        case 0: launch_list_rp<0>(b, l, st); break;
+        case 1: launch_list_rp<1>(b, l, st); break;
+        case 2: launch_list_rp<2>(b, l, st); break;
+        default: launch_list_rp<3>(b, l, st); break;
         }
     }
-    if (b->total_bil) {
-        const size_t base = (size_t)b->n_ahd + b->n_ahd + 1;
-        const uint32_t *ids = b->d_lists + base, *off = b->d_lists + base + b->n_bil;
-        L.n = b->n_bil;
-        L.total = b->total_bil;
-        L.counter = b->d_counters + 1;
-        const uint32_t waves = (b->total_bil + 3) / 4;
-        if (b->fast_bil) {
-            const uint32_t g =
-                std::min(resident_blocks(k_batch_bilinear<true>, BWALK_WAVES * 64, ctx->num_cus), waves);
-            hipLaunchKernelGGL(k_batch_bilinear<true>, dim3(g), dim3(BWALK_WAVES * 64), 0, st, L, b->d_frames, ids, off);
-        } else {
-            const uint32_t g =
-                std::min(resident_blocks(k_batch_bilinear<false>, BWALK_WAVES * 64, ctx->num_cus), waves);
-            hipLaunchKernelGGL(k_batch_bilinear<false>, dim3(g), dim3(BWALK_WAVES * 64), 0, st, L, b->d_frames, ids, off);
-        }
-    }
-    (void)n;
     return hipGetLastError();
 }
 
@@ -386,7 +386,7 @@ extern "C" int32_t frh_batch_autotune(frh_batch *b, void *stream, int32_t iters,
         const uint32_t cand_a[] = {24, 32, 40, 48, 56, 64, 72, 80, 96, 112, 128};
         const uint32_t cand_b[] = {8, 12, 16, 24, 32, 48, 64};
         b->bil_rows = best_b;
-        if (b->total_ahd) {
+        if (b->items(true)) {
             float best = 1e30f;
             for (uint32_t r : cand_a) {
                 b->ahd_rows = r;
@@ -395,7 +395,7 @@ extern "C" int32_t frh_batch_autotune(frh_batch *b, void *stream, int32_t iters,
                 if (ms < best) best = ms, best_a = r;
             }
         }
-        if (rc == FRH_OK && b->total_bil) {
+        if (rc == FRH_OK && b->items(false)) {
             b->ahd_rows = best_a;
             float best = 1e30f;
             for (uint32_t r : cand_b) {
@@ -428,7 +428,7 @@ extern "C" int32_t frh_batch_set_strip_rows(frh_batch *b, uint32_t ahd_rows, uin
     return batch_build(b);
 }
 
-extern "C" uint64_t frh_batch_items(const frh_batch *b) { return b ? (uint64_t)b->total_ahd + b->total_bil : 0; }
+extern "C" uint64_t frh_batch_items(const frh_batch *b) { return b ? b->items(true) + b->items(false) : 0; }
 
 extern "C" void frh_batch_destroy(frh_batch *b) {
     if (b) (void)hipSetDevice(b->ctx->device);
