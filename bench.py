@@ -249,7 +249,8 @@ def main():
     args = _parse()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (each frame develops on one "
-              "GPU; see DESIGN.md for how the path would shard)", file=sys.stderr)
+              "GPU; frames shard one per GPU and a single frame by row bands with halos, which this "
+              "one-GPU build does not launch)", file=sys.stderr)
         sys.exit(2)
     if args.mode == "copy":
         return copy_mode(args)
