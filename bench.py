@@ -8,6 +8,7 @@ resident in HBM and the output written to HBM. Prints one JSON line.
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--demosaic ahd|bilinear] [--strip-rows R]
   python bench.py --mode denoise [--precision f16|bf16|f16x3|f32]   # the CFA-plane DCT denoiser
   python bench.py --mode batch [--frames 64]   # 64 mixed frames (12-61 MP) in one persistent launch
+  python bench.py --mode serve [--rates 10,50,100,200] [--duration 15] [--serial]   # the serving loop
   python bench.py --mode copy   # a plain device copy of 1 GiB (PMC byte-counter calibration)
 """
 import argparse
@@ -30,8 +31,11 @@ def _parse():
     p.add_argument("--steps", type=int, default=200)
     p.add_argument("--warmup", type=int, default=20)
     p.add_argument("--demosaic", default="ahd", choices=["ahd", "bilinear"])
-    p.add_argument("--mode", default="develop", choices=["develop", "denoise", "batch", "copy"])
+    p.add_argument("--mode", default="develop", choices=["develop", "denoise", "batch", "serve", "copy"])
     p.add_argument("--frames", type=int, default=64, help="batch mode: frames in the batch")
+    p.add_argument("--rates", default="10,50,100,200", help="serve mode: request rates (req/s) to sweep")
+    p.add_argument("--duration", type=float, default=15.0, help="serve mode: seconds of arrivals per rate")
+    p.add_argument("--serial", action="store_true", help="serve mode: also run the serial single-frame loop")
     p.add_argument("--precision", default="f16", choices=["f16", "bf16", "f16x3", "f32"],
                    help="denoise mode: MFMA operand precision, or the scalar f32 variant")
     p.add_argument("--seg-rows", type=int, default=0, help="denoise mode: 8-row tile rows per wave (0: default)")
@@ -245,6 +249,184 @@ def batch_mode(args):
     print(json.dumps(line))
 
 
+N_SERVE_MOSAICS = 32
+
+
+def serve_mosaic_shapes(seed=5):
+    """Sizes of the pooled request mosaics: log-uniform in 12-102 MP at 3:2 or 4:3 (sides
+    multiples of 8), a Bayer phase and a bit depth each."""
+    import math
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(N_SERVE_MOSAICS):
+        mp = math.exp(float(rng.uniform(math.log(12.0), math.log(102.0))))
+        aspect = 1.5 if rng.random() < 0.5 else 4.0 / 3.0
+        h = max(16, int((mp * 1e6 / aspect) ** 0.5) // 8 * 8)
+        w = max(16, int(h * aspect) // 8 * 8)
+        out.append(dict(w=w, h=h, cfa=["RGGB", "BGGR", "GRBG", "GBRG"][int(rng.integers(0, 4))],
+                        bits=12 if rng.random() < 0.5 else 14, seed=100 + k))
+    return out
+
+
+def serve_requests(rate, duration, seed):
+    """Poisson arrivals at `rate` req/s for `duration` s. Each request takes one of the pooled
+    mosaics (so sizes follow the log-uniform 12-102 MP draw) with its own WB and matrix, 30 % AHD,
+    20 % denoised first."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    t, out = 0.0, []
+    while True:
+        t += float(rng.exponential(1.0 / rate))
+        if t > duration:
+            return out
+        out.append(dict(t=t, m=int(rng.integers(0, N_SERVE_MOSAICS)),
+                        wb=(float(rng.uniform(1.2, 2.8)), 1.0, float(rng.uniform(1.2, 2.8)), 1.0),
+                        jitter=rng.uniform(0.9, 1.1, (3, 3)), ahd=bool(rng.random() < 0.3),
+                        denoise=bool(rng.random() < 0.2)))
+
+
+def serve_mode(args):
+    """The serving loop under Poisson load: end-to-end latency (arrival -> RGBA8 in host memory)
+    and throughput per request rate, with the serial single-frame loop beside it (--serial)."""
+    import threading
+    import numpy as np
+
+    import flashraw_amd as F
+    from flashraw_amd import synth
+
+    ctx = F.context()
+    shapes = serve_mosaic_shapes()
+    pool = []
+    for sh in shapes:
+        white = (1 << sh["bits"]) - 1
+        black = 64.0 * (sh["bits"] - 10)
+        m = F.pinned_empty((sh["h"], sh["w"]), np.uint16)
+        m[:] = synth.mosaic(sh["w"], sh["h"], sh["cfa"], black=black, white=float(white), seed=sh["seed"])
+        pool.append(dict(sh, m=m, black=black, white=white))
+    max_px = max(sh["w"] * sh["h"] for sh in shapes)
+    n_out = 24
+    outs = [F.pinned_empty((max_px * 4,), np.uint8) for _ in range(n_out)]
+
+    def job_of(r):
+        p = pool[r["m"]]
+        w, h = p["w"], p["h"]
+        info = F.FrameInfo(w, h, F.CfaPattern.bayer(p["cfa"]), F.BlackLevel.uniform(p["black"]),
+                           F.WhiteLevel.uniform(p["white"]))
+        matrix = tuple(tuple(float(v) for v in row) for row in np.asarray(synth.MATRIX) * r["jitter"])
+        spec = F.DevelopSpec(F.Rect.full(w, h), wb=r["wb"], cam_to_rgb=matrix, output=F.DevelopOutput.Rgba8,
+                             edge=F.ahd_spec(info) if r["ahd"] else None)
+        frame = F.GpuFrame.from_u16(p["m"], w, h)
+        dn = F.DenoiseSpec(w, h, [p["black"]] * 4, 2.0, 144.0, 2.7, 4, F.DnPrecision.F16X3) if r["denoise"] else None
+        return frame, info, spec, dn, w * h
+
+    def run_server(reqs):
+        srv = ctx.server(device_budget_bytes=64 << 30)
+        free = list(range(n_out))
+        cv = threading.Condition()
+        lat, batch_sizes, done_t, parts = [], [], [], []
+        tickets = []
+        t0 = time.perf_counter()
+
+        def waiter():
+            k = 0
+            while True:
+                with cv:
+                    while k >= len(tickets):
+                        cv.wait()
+                    item = tickets[k]
+                if item is None:
+                    return
+                tk, slot, r = item
+                res = srv.wait(tk)
+                now = time.perf_counter() - t0
+                lat.append((now - r["t"]) * 1e3)
+                batch_sizes.append(res.batch_frames)
+                parts.append((res.queue_ms, res.upload_ms, res.batch_wait_ms))
+                done_t.append(now)
+                with cv:
+                    free.append(slot)
+                    cv.notify_all()
+                k += 1
+
+        th = threading.Thread(target=waiter)
+        th.start()
+        for r in reqs:
+            delay = r["t"] - (time.perf_counter() - t0)
+            if delay > 0:
+                time.sleep(delay)
+            frame, info, spec, dn, px = job_of(r)
+            with cv:
+                while not free:
+                    cv.wait()
+                slot = free.pop()
+            tk = srv.submit(frame, info, spec, outs[slot][: px * 4], dn)
+            with cv:
+                tickets.append((tk, slot, r))
+                cv.notify_all()
+        with cv:
+            tickets.append(None)
+            cv.notify_all()
+        th.join()
+        st = srv.stats()
+        srv.close()
+        return lat, done_t, batch_sizes, st, np.mean(np.asarray(parts), axis=0)
+
+    def run_serial(reqs):
+        import torch
+        lat, done_t = [], []
+        t0 = time.perf_counter()
+        for r in reqs:
+            delay = r["t"] - (time.perf_counter() - t0)
+            if delay > 0:
+                time.sleep(delay)
+            frame, info, spec, dn, px = job_of(r)
+            if dn is not None:
+                p = pool[r["m"]]
+                src = torch.from_numpy(p["m"].view(np.int16)).to("cuda")
+                dst = torch.empty_like(src)
+                ctx.denoise_cfa(dn, src, dst)
+                frame = F.GpuFrame.from_u16(dst, p["w"], p["h"])
+            run = ctx.run(frame, info, F.GpuOp.Develop(spec))
+            run.image.download_into(outs[0][: px * 4])
+            now = time.perf_counter() - t0
+            lat.append((now - r["t"]) * 1e3)
+            done_t.append(now)
+        return lat, done_t
+
+    results = []
+    for rate in [float(x) for x in args.rates.split(",")]:
+        reqs = serve_requests(rate, args.duration, seed=int(rate))
+        mp = sum(pool[r["m"]]["w"] * pool[r["m"]]["h"] for r in reqs) / 1e6
+        lat, done_t, bsz, st, parts = run_server(reqs)
+        entry = {"rate": rate, "requests": len(reqs), "mp": round(mp, 1),
+                 "served": {"mps": round(mp / max(done_t), 1), "p50_ms": round(float(np.percentile(lat, 50)), 2),
+                            "p99_ms": round(float(np.percentile(lat, 99)), 2),
+                            "mean_queue_ms": round(float(parts[0]), 2), "mean_upload_ms": round(float(parts[1]), 2),
+                            "mean_batch_wait_ms": round(float(parts[2]), 2),
+                            "mean_batch_frames": round(float(np.mean(bsz)), 2), "batches": st["batches"],
+                            "peak_device_gb": round(st["peak_device_bytes"] / 1e9, 2)}}
+        if args.serial:
+            slat, sdone = run_serial(reqs)
+            entry["serial"] = {"mps": round(mp / max(sdone), 1), "p50_ms": round(float(np.percentile(slat, 50)), 2),
+                               "p99_ms": round(float(np.percentile(slat, 99)), 2)}
+        results.append(entry)
+        print(json.dumps(entry), flush=True)
+    best = max(results, key=lambda e: e["served"]["mps"])
+    line = {
+        "metric": "served develop: MP/s and p50/p99 end-to-end latency at Poisson request rates",
+        "value": best["served"]["mps"], "unit": "MP/s", "n_gpus": 1, "steps": len(results), "warmup": 0,
+        "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "Poisson arrivals over 32 pooled mosaics of 12-102 MP (log-uniform, 3:2/4:3), random "
+                               "phase, 12/14-bit, own WB and matrix per request, 30% AHD / 70% bilinear, 20% denoised "
+                               "(split f16) first, host u16 in, RGBA8 to pinned host memory, 64 GB device budget",
+                   "duration_s": args.duration, "sweep": results},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(line))
+
+
 def main():
     args = _parse()
     if args.gpus != 1:
@@ -258,6 +440,8 @@ def main():
         return denoise_mode(args)
     if args.mode == "batch":
         return batch_mode(args)
+    if args.mode == "serve":
+        return serve_mode(args)
 
     import numpy as np
     import torch

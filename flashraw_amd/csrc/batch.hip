@@ -143,12 +143,15 @@ struct frh_batch {
     bool fast[NLIST] = {};
     hipGraphExec_t graph = nullptr;
     hipGraph_t graph_def = nullptr;
+    uint32_t capacity = 0;        // frames the device and staging buffers hold
+    uint8_t *h_stage = nullptr;   // pinned staging of descriptors and lists
     ~frh_batch() {
         if (graph) (void)hipGraphExecDestroy(graph);
         if (graph_def) (void)hipGraphDestroy(graph_def);
         if (d_frames) (void)hipFree(d_frames);
         if (d_lists) (void)hipFree(d_lists);
         if (d_counters) (void)hipFree(d_counters);
+        if (h_stage) (void)hipHostFree(h_stage);
     }
     uint64_t items(bool ahd) const {
         uint64_t t = 0;
@@ -157,11 +160,37 @@ struct frh_batch {
     }
 };
 
-// Build the work lists for the current strip heights and upload them (synchronously on the
-// context stream).
-static int32_t batch_build(frh_batch *b) {
-    frh_context *ctx = b->ctx;
+// Device and pinned staging buffers for `capacity` frames (allocated once; a batch object is
+// refilled many times by the serving loop without allocating).
+static int32_t batch_reserve(frh_batch *b, uint32_t capacity) {
+    if (capacity <= b->capacity) return FRH_OK;
+    hipError_t e;
+    if (b->d_frames) (void)hipFree(b->d_frames);
+    if (b->d_lists) (void)hipFree(b->d_lists);
+    if (b->h_stage) (void)hipHostFree(b->h_stage);
+    b->d_frames = nullptr;
+    b->d_lists = nullptr;
+    b->h_stage = nullptr;
+    b->capacity = 0;
+    const size_t words = (size_t)capacity + 2 * NLIST;
+    if ((e = hipMalloc(&b->d_frames, capacity * sizeof(BatchFrame))) != hipSuccess ||
+        (e = hipMalloc(&b->d_lists, words * 4)) != hipSuccess ||
+        (e = hipHostMalloc(reinterpret_cast<void **>(&b->h_stage), capacity * sizeof(BatchFrame) + words * 4,
+                           hipHostMallocDefault)) != hipSuccess)
+        return hip_fail(e, "batch buffers");
+    if (!b->d_counters && (e = hipMalloc(&b->d_counters, NLIST * sizeof(uint32_t))) != hipSuccess)
+        return hip_fail(e, "batch counters");
+    b->capacity = capacity;
+    b->n_list_words = words;
+    return FRH_OK;
+}
+
+// Build the work lists for the current strip heights and upload them on `st` (and wait for the
+// upload when `sync`).
+static int32_t batch_build(frh_batch *b, hipStream_t st, bool sync) {
     const size_t n = b->plans.size();
+    int32_t rc = batch_reserve(b, (uint32_t)n);
+    if (rc) return rc;
     b->frames.assign(n, BatchFrame{});
     std::vector<uint32_t> ids[NLIST], off[NLIST];
     for (int l = 0; l < NLIST; l++) {
@@ -199,21 +228,13 @@ static int32_t batch_build(frh_batch *b) {
         words.insert(words.end(), off[l].begin(), off[l].end());
     }
     hipError_t e;
-    if (!b->d_frames && (e = hipMalloc(&b->d_frames, n * sizeof(BatchFrame))) != hipSuccess)
-        return hip_fail(e, "batch descriptors");
-    if (words.size() > b->n_list_words) {
-        if (b->d_lists) (void)hipFree(b->d_lists);
-        b->d_lists = nullptr;
-        if ((e = hipMalloc(&b->d_lists, words.size() * 4)) != hipSuccess) return hip_fail(e, "batch lists");
-        b->n_list_words = words.size();
-    }
-    if (!b->d_counters && (e = hipMalloc(&b->d_counters, NLIST * sizeof(uint32_t))) != hipSuccess)
-        return hip_fail(e, "batch counters");
-    if ((e = hipMemcpyAsync(b->d_frames, b->frames.data(), n * sizeof(BatchFrame), hipMemcpyHostToDevice,
-                            ctx->stream)) != hipSuccess ||
-        (e = hipMemcpyAsync(b->d_lists, words.data(), words.size() * 4, hipMemcpyHostToDevice, ctx->stream)) !=
+    std::memcpy(b->h_stage, b->frames.data(), n * sizeof(BatchFrame));
+    std::memcpy(b->h_stage + b->capacity * sizeof(BatchFrame), words.data(), words.size() * 4);
+    if ((e = hipMemcpyAsync(b->d_frames, b->h_stage, n * sizeof(BatchFrame), hipMemcpyHostToDevice, st)) !=
             hipSuccess ||
-        (e = hipStreamSynchronize(ctx->stream)) != hipSuccess)
+        (e = hipMemcpyAsync(b->d_lists, b->h_stage + b->capacity * sizeof(BatchFrame), words.size() * 4,
+                            hipMemcpyHostToDevice, st)) != hipSuccess ||
+        (sync && (e = hipStreamSynchronize(st)) != hipSuccess))
         return hip_fail(e, "batch upload");
     if (b->graph) {  // the lists changed: a captured graph is stale
         (void)hipGraphExecDestroy(b->graph);
@@ -302,7 +323,7 @@ extern "C" int32_t frh_batch_create(frh_context *ctx, uint32_t n, const frh_fram
             p.a.dst2 = nullptr;
         }
         (void)hipSetDevice(ctx->device);
-        int32_t rc = batch_build(b.get());
+        int32_t rc = batch_build(b.get(), ctx->stream, true);
         if (rc) return rc;
         *out = b.release();
         return FRH_OK;
@@ -370,7 +391,7 @@ extern "C" int32_t frh_batch_autotune(frh_batch *b, void *stream, int32_t iters,
             return fail(FRH_ERROR_GPU_UNAVAILABLE, "hipEventCreate failed");
         int32_t rc = FRH_OK;
         auto time_it = [&](float *ms) -> int32_t {
-            int32_t r = batch_build(b);
+            int32_t r = batch_build(b, b->ctx->stream, true);
             if (r) return r;
             hipError_t e = batch_enqueue(b, st);  // warm
             (void)hipEventRecord(e0, st);
@@ -410,7 +431,7 @@ extern "C" int32_t frh_batch_autotune(frh_batch *b, void *stream, int32_t iters,
         if (rc != FRH_OK) return rc;
         b->ahd_rows = best_a;
         b->bil_rows = best_b;
-        if ((rc = batch_build(b)) != FRH_OK) return rc;
+        if ((rc = batch_build(b, b->ctx->stream, true)) != FRH_OK) return rc;
         if (best_ahd) *best_ahd = best_a;
         if (best_bil) *best_bil = best_b;
         if (had_graph && stream) return frh_batch_capture(b, stream);
@@ -425,7 +446,7 @@ extern "C" int32_t frh_batch_set_strip_rows(frh_batch *b, uint32_t ahd_rows, uin
     b->ahd_rows = ahd_rows;
     b->bil_rows = bil_rows;
     (void)hipSetDevice(b->ctx->device);
-    return batch_build(b);
+    return batch_build(b, b->ctx->stream, true);
 }
 
 extern "C" uint64_t frh_batch_items(const frh_batch *b) { return b ? b->items(true) + b->items(false) : 0; }
@@ -434,3 +455,19 @@ extern "C" void frh_batch_destroy(frh_batch *b) {
     if (b) (void)hipSetDevice(b->ctx->device);
     delete b;
 }
+
+// ---- internal interface of the serving loop (frh_internal.h) ----
+
+frh_batch *frh::batch_new(frh_context *ctx, uint32_t capacity) {
+    std::unique_ptr<frh_batch> b(new frh_batch);
+    b->ctx = ctx;
+    if (batch_reserve(b.get(), capacity) != FRH_OK) return nullptr;
+    return b.release();
+}
+
+int32_t frh::batch_assign(frh_batch *b, std::vector<Plan> &&plans, hipStream_t st) {
+    b->plans = std::move(plans);
+    return batch_build(b, st, false);
+}
+
+hipError_t frh::batch_run(frh_batch *b, hipStream_t st) { return batch_enqueue(b, st); }

@@ -73,6 +73,12 @@ struct Plan {
 int32_t plan_develop(const frh_context *ctx, const frh_frame *f, const frh_frame_info *info,
                      const frh_develop_spec *spec, Plan &p);
 
+// Batches reused by the serving loop: reserve buffers for `capacity` frames once, then assign a
+// new set of plans (descriptors uploaded on `st`, asynchronously) and run on `st`.
+frh_batch *batch_new(frh_context *ctx, uint32_t capacity);
+int32_t batch_assign(frh_batch *b, std::vector<Plan> &&plans, hipStream_t st);
+hipError_t batch_run(frh_batch *b, hipStream_t st);
+
 }  // namespace frh
 
 struct frh_image {

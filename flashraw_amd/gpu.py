@@ -191,6 +191,20 @@ def lib():
             L.frh_batch_items.argtypes = [vp]
             L.frh_batch_items.restype = u64
             L.frh_batch_destroy.argtypes = [vp]
+            L.frh_server_create.argtypes = [vp, ctypes.POINTER(_ServerConfig), ctypes.POINTER(vp)]
+            L.frh_server_create.restype = i32
+            L.frh_server_submit.argtypes = [vp, ctypes.POINTER(_Frame), ctypes.POINTER(_FrameInfo),
+                                            ctypes.POINTER(_DevelopSpec), ctypes.POINTER(_DnSpec), vp, u64,
+                                            ctypes.POINTER(u64)]
+            L.frh_server_submit.restype = i32
+            L.frh_server_wait.argtypes = [vp, u64, i32, ctypes.POINTER(_JobResult), ctypes.POINTER(i32)]
+            L.frh_server_wait.restype = i32
+            L.frh_server_get_stats.argtypes = [vp, ctypes.POINTER(_ServerStats)]
+            L.frh_server_get_stats.restype = i32
+            L.frh_server_destroy.argtypes = [vp]
+            L.frh_host_alloc.argtypes = [u64]
+            L.frh_host_alloc.restype = vp
+            L.frh_host_free.argtypes = [vp]
             _lib = L
         return _lib
 
@@ -455,6 +469,23 @@ class DenoiseSpec:
         return s
 
 
+class _ServerConfig(ctypes.Structure):
+    _fields_ = [("size", ctypes.c_uint32), ("device_budget_bytes", ctypes.c_uint64),
+                ("max_batch_frames", ctypes.c_uint32), ("max_inflight_batches", ctypes.c_uint32)]
+
+
+class _JobResult(ctypes.Structure):
+    _fields_ = [("size", ctypes.c_uint32), ("status", ctypes.c_int32), ("queue_ms", ctypes.c_double),
+                ("upload_ms", ctypes.c_double), ("batch_wait_ms", ctypes.c_double), ("latency_ms", ctypes.c_double),
+                ("batch_frames", ctypes.c_uint32)]
+
+
+class _ServerStats(ctypes.Structure):
+    _fields_ = [("size", ctypes.c_uint32)] + [(n, ctypes.c_uint64) for n in (
+        "jobs_submitted", "jobs_done", "jobs_failed", "batches", "batched_frames", "peak_device_bytes",
+        "device_budget_bytes")]
+
+
 class GpuOp:
     """GpuOp (run.rs:219-225); only Develop is accepted by this path."""
 
@@ -560,6 +591,13 @@ class GpuImage:
         check(L.frh_image_download(self._h, out.ctypes.data, out.nbytes))
         return out
 
+    def download_into(self, out: np.ndarray) -> None:
+        """Copy the pixels into an existing host array (pinned memory copies fastest)."""
+        n = lib().frh_image_bytes(self._h)
+        if out.nbytes < n:
+            raise InvalidArgument(10, "output array too small")
+        check(lib().frh_image_download(self._h, out.ctypes.data, n))
+
     def __del__(self):
         if getattr(self, "_h", None):
             try:
@@ -645,6 +683,10 @@ class GpuContext:
         kind (frh_batch_create)."""
         return Batch(self, jobs)
 
+    def server(self, device_budget_bytes: int = 0, max_batch_frames: int = 0, max_inflight_batches: int = 0) -> "Server":
+        """A serving loop on this context (frh_server_create)."""
+        return Server(self, device_budget_bytes, max_batch_frames, max_inflight_batches)
+
     def denoise_cfa(self, spec: DenoiseSpec, src, dst=None, dst_f32=None, stream=None) -> None:
         """Denoise the device u16 mosaic `src` into `dst` (u16) and/or `dst_f32` (f32, DN above
         black), torch tensors or device addresses, on `stream` (frh_denoise_cfa)."""
@@ -669,6 +711,86 @@ class GpuContext:
             except Exception:
                 pass
             self._h = None
+
+
+def pinned_empty(shape, dtype) -> np.ndarray:
+    """A numpy array in pinned host memory (frh_host_alloc), freed with the array."""
+    dt = np.dtype(dtype)
+    n = int(np.prod(shape)) * dt.itemsize
+    ptr = lib().frh_host_alloc(max(n, 1))
+    if not ptr:
+        raise OutOfMemory(7, "pinned host allocation failed")
+    buf = (ctypes.c_uint8 * max(n, 1)).from_address(ptr)
+    arr = np.frombuffer(buf, dtype=np.uint8, count=n).view(dt).reshape(shape)
+    import weakref
+    weakref.finalize(buf, lib().frh_host_free, ptr)
+    return arr
+
+
+@dataclass
+class JobResult:
+    status: int
+    queue_ms: float
+    upload_ms: float
+    batch_wait_ms: float
+    latency_ms: float
+    batch_frames: int
+
+
+class Server:
+    """frh_server: a request loop with device-memory admission and continuous batching."""
+
+    def __init__(self, ctx: "GpuContext", device_budget_bytes: int = 0, max_batch_frames: int = 0,
+                 max_inflight_batches: int = 0):
+        self.ctx = ctx
+        c = _ServerConfig(ctypes.sizeof(_ServerConfig), device_budget_bytes, max_batch_frames, max_inflight_batches)
+        h = ctypes.c_void_p()
+        check(lib().frh_server_create(ctx._h, ctypes.byref(c), ctypes.byref(h)))
+        self._h = h.value
+        self._keep = {}
+
+    def submit(self, frame: GpuFrame, info: FrameInfo, spec: DevelopSpec, out: np.ndarray,
+               denoise: Optional[DenoiseSpec] = None) -> int:
+        """Queue a job; `frame` holds host data, `out` receives the output (both must stay alive
+        until the job is collected by wait())."""
+        f, k1 = frame._c()
+        i, k2 = info._c()
+        sp, k3 = spec._c()
+        d = denoise._c() if denoise is not None else None
+        t = ctypes.c_uint64()
+        check(lib().frh_server_submit(self._h, ctypes.byref(f), ctypes.byref(i), ctypes.byref(sp),
+                                      None if d is None else ctypes.byref(d), out.ctypes.data, out.nbytes,
+                                      ctypes.byref(t)))
+        self._keep[t.value] = (frame, out)
+        return t.value
+
+    def wait(self, ticket: int, timeout_ms: int = -1) -> Optional[JobResult]:
+        r = _JobResult()
+        r.size = ctypes.sizeof(_JobResult)
+        done = ctypes.c_int32()
+        check(lib().frh_server_wait(self._h, ticket, timeout_ms, ctypes.byref(r), ctypes.byref(done)))
+        if not done.value:
+            return None
+        self._keep.pop(ticket, None)
+        return JobResult(r.status, r.queue_ms, r.upload_ms, r.batch_wait_ms, r.latency_ms, r.batch_frames)
+
+    def stats(self) -> dict:
+        st = _ServerStats()
+        st.size = ctypes.sizeof(_ServerStats)
+        check(lib().frh_server_get_stats(self._h, ctypes.byref(st)))
+        return {n: int(getattr(st, n)) for n, _ in _ServerStats._fields_[1:]}
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().frh_server_destroy(self._h)
+            self._h = None
+            self._keep.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Batch:

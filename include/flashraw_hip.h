@@ -239,6 +239,51 @@ typedef struct frh_dn_spec {
 int32_t frh_denoise_cfa(frh_context *ctx, const frh_dn_spec *spec, const void *src, void *dst, float *dst_f32,
                         void *stream);
 
+/* ---- serving loop ----
+ * No reference counterpart: a request loop over one context. Jobs (a host frame, its info and
+ * spec, an optional CFA-plane denoise, a host output buffer) are admitted in arrival order while
+ * their device buffers (input, denoised input, output) fit the device budget; uploads overlap
+ * running batches; every resident job joins the next batched develop (continuous batching);
+ * outputs are copied back and the buffers reused. Frames use a black level of at most a 2x2
+ * pattern without deltas and one output; the frame's host data and the output buffer must stay
+ * valid until the job is done (pinned host memory lets the copies run asynchronously). */
+typedef struct frh_server frh_server;
+typedef struct frh_server_config {
+    uint32_t size;
+    uint64_t device_budget_bytes;  /* admission bound on resident job buffers (0: 64 GiB) */
+    uint32_t max_batch_frames;     /* frames per batched launch (0: 64) */
+    uint32_t max_inflight_batches; /* batches queued on the device at once (0: 2) */
+} frh_server_config;
+typedef struct frh_job_result {
+    uint32_t size;
+    int32_t status;
+    double queue_ms;      /* submitted -> admitted (device memory granted) */
+    double upload_ms;     /* admitted -> input resident */
+    double batch_wait_ms; /* resident -> its batch enqueued */
+    double latency_ms;    /* submitted -> output in host memory */
+    uint32_t batch_frames; /* frames in the job's batch */
+} frh_job_result;
+typedef struct frh_server_stats {
+    uint32_t size;
+    uint64_t jobs_submitted, jobs_done, jobs_failed, batches, batched_frames;
+    uint64_t peak_device_bytes, device_budget_bytes;
+} frh_server_stats;
+int32_t frh_server_create(frh_context *ctx, const frh_server_config *cfg, frh_server **out);
+/* Queue a job; `denoise` NULL for none (otherwise its width/height/stride are the frame's). */
+int32_t frh_server_submit(frh_server *server, const frh_frame *frame, const frh_frame_info *info,
+                          const frh_develop_spec *spec, const frh_dn_spec *denoise, void *host_out,
+                          uint64_t host_out_bytes, uint64_t *ticket);
+/* Wait up to timeout_ms (< 0: forever) for a job; *done = 1 and `result` filled when finished
+ * (the ticket is then released). */
+int32_t frh_server_wait(frh_server *server, uint64_t ticket, int32_t timeout_ms, frh_job_result *result,
+                        int32_t *done);
+int32_t frh_server_get_stats(frh_server *server, frh_server_stats *stats);
+/* Finish every submitted job, then release the server. */
+void frh_server_destroy(frh_server *server);
+/* Pinned host memory for frames and outputs (hipHostMalloc). */
+void *frh_host_alloc(uint64_t bytes);
+void frh_host_free(void *p);
+
 /* ---- images (crates/flashraw-gpu/src/image.rs:52-306) ---- */
 
 enum frh_pixel_format { FRH_FORMAT_RGBA8 = 0, FRH_FORMAT_F32 = 1, FRH_FORMAT_U16 = 2 };
