@@ -117,7 +117,7 @@ __device__ __forceinline__ void walk_row(const DevArgs &a, const Lds &s, const f
 #pragma unroll
     for (int k = 0; k < 2; k++) {
         const float cm = W.m[M3][k];
-        const float sm = sqrtf(fmaxf(cm, 0.0f));  // the measured channel, the same in both candidates
+        const float sm = sqrt_max0(cm);  // the measured channel, the same in both candidates
         const int col = site<RP>(PL, k);
 #pragma unroll
         for (int d = 0; d < 2; d++) {
@@ -146,15 +146,15 @@ __device__ __forceinline__ void walk_row(const DevArgs &a, const Lds &s, const f
             float sr, sg, sb;
             if (col == 0) {
                 sr = sm;
-                sg = sqrtf(fmaxf(gg, 0.0f));
-                sb = sqrtf(fmaxf(blue, 0.0f));
+                sg = sqrt_max0(gg);
+                sb = sqrt_max0(blue);
             } else if (col == 1) {
-                sr = sqrtf(fmaxf(red, 0.0f));
+                sr = sqrt_max0(red);
                 sg = sm;
-                sb = sqrtf(fmaxf(blue, 0.0f));
+                sb = sqrt_max0(blue);
             } else {
-                sr = sqrtf(fmaxf(red, 0.0f));
-                sg = sqrtf(fmaxf(gg, 0.0f));
+                sr = sqrt_max0(red);
+                sg = sqrt_max0(gg);
                 sb = sm;
             }
             W.lab[LQ][d][0][k] = (sr + 2.0f * sg + sb) * 0.25f;

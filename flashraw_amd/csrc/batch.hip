@@ -172,7 +172,8 @@ static int32_t batch_reserve(frh_batch *b, uint32_t capacity) {
     b->d_lists = nullptr;
     b->h_stage = nullptr;
     b->capacity = 0;
-    const size_t words = (size_t)capacity + 2 * NLIST;
+    // per list: its frame ids and its item offsets (one more than its frames)
+    const size_t words = 2 * (size_t)capacity + NLIST;
     if ((e = hipMalloc(&b->d_frames, capacity * sizeof(BatchFrame))) != hipSuccess ||
         (e = hipMalloc(&b->d_lists, words * 4)) != hipSuccess ||
         (e = hipHostMalloc(reinterpret_cast<void **>(&b->h_stage), capacity * sizeof(BatchFrame) + words * 4,
@@ -227,6 +228,7 @@ static int32_t batch_build(frh_batch *b, hipStream_t st, bool sync) {
         words.insert(words.end(), ids[l].begin(), ids[l].end());
         words.insert(words.end(), off[l].begin(), off[l].end());
     }
+    if (words.size() > b->n_list_words) return fail(FRH_ERROR_INTERNAL, "batch lists exceed their buffer");
     hipError_t e;
     std::memcpy(b->h_stage, b->frames.data(), n * sizeof(BatchFrame));
     std::memcpy(b->h_stage + b->capacity * sizeof(BatchFrame), words.data(), words.size() * 4);

@@ -385,19 +385,42 @@ __host__ inline int red_position(const DevArgs &a) {
     return -1;
 }
 
+// Correctly rounded square root of x = max(v, 0) (IEEE sqrt, the reference's f32::sqrt). The
+// hardware estimate is within one ulp for x = 0, inf and x >= 2^-96; one residual test on each
+// side picks the correctly rounded neighbour, the same correction the compiler's own lowering
+// applies. Smaller positive inputs (never produced by normalised samples, whose non-zero
+// magnitudes are far larger) take the library path for the whole wave.
+__device__ __forceinline__ float sqrt_nonneg(float x) {
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float sd = __int_as_float(__float_as_int(s) - 1), su = __int_as_float(__float_as_int(s) + 1);
+    float r = __builtin_fmaf(-sd, s, x) <= 0.0f ? sd : s;
+    r = __builtin_fmaf(-su, s, x) > 0.0f ? su : r;
+    return r;
+}
+__device__ __forceinline__ float sqrt_max0(float v) {
+    const float x = fmaxf(v, 0.0f);
+    float r = sqrt_nonneg(x);
+    const bool tiny = x > 0.0f && x < 0x1p-96f;
+    if (__builtin_amdgcn_ballot_w64(tiny)) {
+        asm volatile("" ::: "memory");  // keep this rare path a branch, not speculated into every row
+        r = tiny ? sqrtf(x) : r;
+    }
+    return r;
+}
+
 // Value of the lane to the left (lane - 1) / right (lane + 1) of this one (DPP wave shifts; the
 // edge lanes read 0 and are halo lanes). Every lane must be active when these run.
 __device__ __forceinline__ float from_left(float v) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, false));
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, true));
 }
 __device__ __forceinline__ float from_right(float v) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, false));
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, true));
 }
 __device__ __forceinline__ uint32_t from_left_u(uint32_t v) {
-    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x138, 0xf, 0xf, false);
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x138, 0xf, 0xf, true);
 }
 __device__ __forceinline__ uint32_t from_right_u(uint32_t v) {
-    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x130, 0xf, 0xf, false);
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x130, 0xf, 0xf, true);
 }
 
 // Colour (0 R, 1 G, 2 B) of a sample with frame row parity py and column parity px, when red

@@ -170,3 +170,13 @@ def test_rejections(ctx):
         ctx.batch([(F.GpuFrame.from_u16(mos, 64, 48), plain, spec, dst)])
     with pytest.raises(F.NotAvailable):
         ctx.batch([(frame, plain, F.DevelopSpec(F.Rect.full(64, 48), output2=F.DevelopOutput.Rgba8), dst)])
+
+
+def test_many_small_frames(ctx):
+    """More frames than the list words of a small batch: the lists hold 2n + 8 words."""
+    import torch
+    jobs = _jobs(6, 40, max_side=120)
+    b = _batch(ctx, jobs)
+    b.launch()
+    torch.cuda.synchronize()
+    _check(ctx, jobs, "many")
