@@ -134,6 +134,15 @@ typedef struct {
     /* AHD plan (ahd.rs:42-61) */
     int ahd;
     uint8_t col[4];
+    /* edge-aware plan (edgeaware/mod.rs:189-230), Bayer */
+    int edge;
+    const fro_edge *et;
+    /* nonzero taps of g0 per canonical position; taps where some direction is nonzero, per
+     * position and colour (mod.rs:130-160) */
+    uint8_t g0_idx[4][49];
+    int g0_n[4];
+    uint8_t dir_idx[4][3][49];
+    int dir_n[4][3];
 } Kern;
 
 static uint8_t chan_of(uint8_t colour) { return colour < 3 ? colour : NO_CHANNEL; }
@@ -667,6 +676,23 @@ static int build_kernel(Kern *k, const fro_frame *f, const fro_params *p) {
 #undef OTHERS
         if (!ok) return -2;
         k->ahd = 1;
+    } else if (p->demosaic == FRO_EDGE) {
+        if (f->cfa_w != 2 || f->cfa_h != 2 || !p->edge || !p->edge->g0 || !p->edge->dir) return -2;
+        k->edge = 1;
+        k->et = p->edge;
+        for (int ph = 0; ph < 4; ph++) {
+            k->g0_n[ph] = 0;
+            for (int i = 0; i < 49; i++)
+                if (p->edge->g0[ph * 49 + i] != 0.0f) k->g0_idx[ph][k->g0_n[ph]++] = (uint8_t)i;
+            for (int c = 0; c < 3; c++) {
+                k->dir_n[ph][c] = 0;
+                for (int i = 0; i < 49; i++) {
+                    int any = 0;
+                    for (int d = 0; d < 4; d++) any |= p->edge->dir[((ph * 3 + c) * 4 + d) * 49 + i] != 0.0f;
+                    if (any) k->dir_idx[ph][c][k->dir_n[ph][c]++] = (uint8_t)i;
+                }
+            }
+        }
     } else if (p->demosaic != FRO_BILINEAR) {
         return -1;
     }
@@ -687,6 +713,204 @@ typedef struct {
 static const float *rb_row(const RowBlock *rb, const Kern *k, size_t y) {
     if (y >= k->h) return NULL;
     return rb->buf + (y - rb->base) * k->w;
+}
+
+
+/* ------------------------------------------------------------------------------------------ */
+/* Edge-aware demosaic, Bayer (crates/flashraw/src/edgeaware/mod.rs): develop_chroma 423-607,   */
+/* demosaic 612-796, smooth 803-822. The reference develops column chunks of a band; a pixel    */
+/* depends only on its neighbourhood (mod.rs:43-46), so this statement works on the whole band. */
+
+#define EDGE_HALO 8  /* CORE_HALO 5 + 3 (mod.rs:66-71) */
+#define EDGE_CORE 5
+
+/* Canonical position index (row * 2 + column) of frame sample (x, y). */
+static inline int edge_ph(const Kern *k, size_t x, size_t y) {
+    return (int)(((y + k->et->cy) & 1) * 2 + ((x + k->et->cx) & 1));
+}
+
+/* Camera RGB (balanced) of frame rows [y0, y1) x columns [xa, xb) into out (3 floats per pixel).
+ * row(y) returns the normalised frame row y. Needs EDGE_HALO samples around. */
+static void edge_band(const Kern *k, const RowBlock *rb, size_t y0, size_t y1, size_t xa, size_t xb, float *out) {
+    const fro_edge *t = k->et;
+    const size_t nr = y1 - y0, nc = xb - xa;
+    const int h = t->beta > 0.0f ? 1 : 0;
+    const size_t m = 2 + (size_t)h;
+    /* 1. balanced mosaic over rows/columns +- (m + CORE) */
+    const size_t MH = m + EDGE_CORE;
+    const size_t mr = nr + 2 * MH, mc = nc + 2 * MH;
+    const size_t Y0 = y0 - MH, X0 = xa - MH;
+    float *M = (float *)malloc(mr * mc * sizeof(float));
+    for (size_t i = 0; i < mr; i++) {
+        const float *row = rb_row(rb, k, Y0 + i);
+        for (size_t j = 0; j < mc; j++) {
+            size_t x = X0 + j;
+            float scale = k->wb[t->canon[edge_ph(k, x, Y0 + i)]];
+            M[i * mc + j] = rmin(row[x] * scale, 1.0f);
+        }
+    }
+#define MS(y, x) M[((y) - Y0) * mc + ((x) - X0)]
+    /* 2. sqrt(G0) over +- (m + 2) */
+    const size_t GH = m + 2, gr = nr + 2 * GH, gc = nc + 2 * GH;
+    const size_t GY0 = y0 - GH, GX0 = xa - GH;
+    float *G = (float *)malloc(gr * gc * sizeof(float));
+    for (size_t i = 0; i < gr; i++)
+        for (size_t j = 0; j < gc; j++) {
+            size_t y = GY0 + i, x = GX0 + j;
+            int ph = edge_ph(k, x, y);
+            float acc = 0.0f;
+            for (int n = 0; n < k->g0_n[ph]; n++) {
+                int idx = k->g0_idx[ph][n], ty = idx / 7, tx = idx % 7;
+                acc += t->g0[ph * 49 + idx] * MS(y + ty - 3, x + tx - 3);
+            }
+            G[i * gc + j] = sqrtf(rmax(acc, 0.0f));
+        }
+#define GS(y, x) G[((y) - GY0) * gc + ((x) - GX0)]
+    /* 3. squared gradients over +- (m + 1), 4. their 3x3 sums and the direction weights over +- m */
+    const size_t DH = m + 1, dr = nr + 2 * DH, dc = nc + 2 * DH;
+    const size_t DY0 = y0 - DH, DX0 = xa - DH;
+    float *D = (float *)malloc(4 * dr * dc * sizeof(float));
+    const float r2 = 0.70710677f; /* FRAC_1_SQRT_2 */
+    for (size_t i = 0; i < dr; i++)
+        for (size_t j = 0; j < dc; j++) {
+            size_t y = DY0 + i, x = DX0 + j;
+            float dh = GS(y, x + 1) - GS(y, x - 1);
+            float dv = GS(y + 1, x) - GS(y - 1, x);
+            float d1 = (GS(y + 1, x + 1) - GS(y - 1, x - 1)) * r2;
+            float d2 = (GS(y + 1, x - 1) - GS(y - 1, x + 1)) * r2;
+            D[(0 * dr + i) * dc + j] = dh * dh;
+            D[(1 * dr + i) * dc + j] = dv * dv;
+            D[(2 * dr + i) * dc + j] = d1 * d1;
+            D[(3 * dr + i) * dc + j] = d2 * d2;
+        }
+    const size_t er = nr + 2 * m, ec = nc + 2 * m, EY0 = y0 - m, EX0 = xa - m;
+    float *P[3];
+    for (int c = 0; c < 3; c++) P[c] = (float *)malloc(er * ec * sizeof(float));
+    for (size_t i = 0; i < er; i++)
+        for (size_t j = 0; j < ec; j++) {
+            size_t y = EY0 + i, x = EX0 + j;
+            float w[4];
+            for (int d = 0; d < 4; d++) {
+                float hs[3];
+                for (int a = 0; a < 3; a++) {
+                    const float *s0 = &D[(d * dr + (y - 1 + a - DY0)) * dc + (x - 1 - DX0)];
+                    hs[a] = s0[0] + s0[1] + s0[2];
+                }
+                float u = hs[0] + hs[1] + hs[2] + 1e-3f;
+                w[d] = 1.0f / (u * u);
+            }
+            float inv = 1.0f / (w[0] + w[1] + w[2] + w[3]);
+            for (int d = 0; d < 4; d++) w[d] = w[d] * inv;
+            /* 5. the directional filters for the two missing colours; the own colour is the sample */
+            int ph = edge_ph(k, x, y);
+            int own = t->canon[ph];
+            for (int c = 0; c < 3; c++) {
+                if (c == own) {
+                    P[c][i * ec + j] = MS(y, x);
+                    continue;
+                }
+                float a[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+                for (int n = 0; n < k->dir_n[ph][c]; n++) {
+                    int idx = k->dir_idx[ph][c][n], ty = idx / 7, tx = idx % 7;
+                    float v = MS(y + ty - 3, x + tx - 3);
+                    for (int d = 0; d < 4; d++) a[d] += t->dir[((ph * 3 + c) * 4 + d) * 49 + idx] * v;
+                }
+                P[c][i * ec + j] = w[0] * a[0] + w[1] * a[1] + w[2] * a[2] + w[3] * a[3];
+            }
+        }
+#define PS(c, y, x) P[c][((y) - EY0) * ec + ((x) - EX0)]
+    /* 6. pass 1 over +- (1 + h): gated mean of the measured differences of the 3x3 window */
+    const size_t H1 = 1 + (size_t)h, e1r = nr + 2 * H1, e1c = nc + 2 * H1, QY0 = y0 - H1, QX0 = xa - H1;
+    float *d1[2];
+    for (int cc = 0; cc < 2; cc++) d1[cc] = (float *)malloc(e1r * e1c * sizeof(float));
+    for (size_t i = 0; i < e1r; i++)
+        for (size_t j = 0; j < e1c; j++) {
+            size_t y = QY0 + i, x = QX0 + j;
+            float g = PS(1, y, x);
+            float inv = 1.0f / (0.2f * rmax(g, 0.0f) + 0.015f);
+            float sw[2] = {0, 0}, sd[2] = {0, 0}, su[2] = {0, 0}, n[2] = {0, 0};
+            for (int dy = 0; dy < 3; dy++)
+                for (int dx = 0; dx < 3; dx++) {
+                    size_t yy = y - 1 + dy, xx = x - 1 + dx;
+                    float gk = PS(1, yy, xx);
+                    float e = (gk - g) * inv;
+                    float wt = 1.0f / (1.0f + e * e);
+                    int col = t->canon[edge_ph(k, xx, yy)];
+                    for (int cc = 0; cc < 2; cc++) {
+                        float mk = col == (cc ? 2 : 0) ? 1.0f : 0.0f;
+                        float d = PS(cc ? 2 : 0, yy, xx) - gk;
+                        sw[cc] += mk * wt;
+                        sd[cc] += (mk * wt) * d;
+                        su[cc] += mk * d;
+                        n[cc] += mk;
+                    }
+                }
+            int own = t->canon[edge_ph(k, x, y)];
+            for (int cc = 0; cc < 2; cc++) {
+                float mixed = sw[cc] > 1e-12f ? sd[cc] / sw[cc] : su[cc] / n[cc];
+                d1[cc][i * e1c + j] = own == (cc ? 2 : 0) ? PS(cc ? 2 : 0, y, x) - g : mixed;
+            }
+        }
+    /* 7. pass 2 over +- h: the same gate over the differences of every pixel */
+    const size_t e2r = nr + 2 * (size_t)h, e2c = nc + 2 * (size_t)h, VY0 = y0 - (size_t)h, VX0 = xa - (size_t)h;
+    float *V = (float *)malloc(e2r * e2c * 3 * sizeof(float));
+    for (size_t i = 0; i < e2r; i++)
+        for (size_t j = 0; j < e2c; j++) {
+            size_t y = VY0 + i, x = VX0 + j;
+            float g = PS(1, y, x);
+            float inv = 1.0f / (0.1f * rmax(g, 0.0f) + 0.007f);
+            float sw = 0.0f, sdr = 0.0f, sdb = 0.0f;
+            for (int dy = 0; dy < 3; dy++)
+                for (int dx = 0; dx < 3; dx++) {
+                    size_t yy = y - 1 + dy, xx = x - 1 + dx;
+                    float gk = PS(1, yy, xx);
+                    float e = (gk - g) * inv;
+                    float wt = 1.0f / (1.0f + e * e);
+                    size_t q = (yy - QY0) * e1c + (xx - QX0);
+                    sw += wt;
+                    sdr += wt * d1[0][q];
+                    sdb += wt * d1[1][q];
+                }
+            float dr = sdr / sw, db = sdb / sw;
+            int own = t->canon[edge_ph(k, x, y)];
+            float *o = V + (i * e2c + j) * 3;
+            o[0] = own == 0 ? PS(0, y, x) : g + dr;
+            o[1] = g;
+            o[2] = own == 2 ? PS(2, y, x) : g + db;
+        }
+    /* 8. gated smoothing of the core on noisy frames (beta > 0) */
+    for (size_t i = 0; i < nr; i++)
+        for (size_t j = 0; j < nc; j++) {
+            const float *pc = V + ((i + (size_t)h) * e2c + j + (size_t)h) * 3;
+            float *o = out + (i * nc + j) * 3;
+            if (!h) {
+                o[0] = pc[0];
+                o[1] = pc[1];
+                o[2] = pc[2];
+                continue;
+            }
+            float inv = 1.0f / (0.2f * rmax(pc[1], 0.0f) + 0.015f);
+            float sw = 0.0f, sm[3] = {0.0f, 0.0f, 0.0f};
+            for (int dy = 0; dy < 3; dy++)
+                for (int dx = 0; dx < 3; dx++) {
+                    const float *q = V + ((i + (size_t)dy) * e2c + j + (size_t)dx) * 3;
+                    float e = (q[1] - pc[1]) * inv;
+                    float wt = 1.0f / (1.0f + e * e);
+                    sw += wt;
+                    for (int c = 0; c < 3; c++) sm[c] += wt * q[c];
+                }
+            for (int c = 0; c < 3; c++) o[c] = pc[c] + t->beta * (sm[c] / sw - pc[c]);
+        }
+#undef MS
+#undef GS
+#undef PS
+    free(M);
+    free(G);
+    free(D);
+    for (int c = 0; c < 3; c++) free(P[c]);
+    free(d1[0]);
+    free(d1[1]);
+    free(V);
 }
 
 int fro_develop(const fro_frame *f, const fro_params *p, void *out, uint64_t out_bytes, int threads,
@@ -749,23 +973,30 @@ int fro_develop(const fro_frame *f, const fro_params *p, void *out, uint64_t out
     if (out_w) *out_w = (uint32_t)(swap ? oh : ow);
     if (out_h) *out_h = (uint32_t)(swap ? ow : oh);
 
-    size_t strip = k.ahd ? BAND_ROWS : STRIP;
+    /* AHD and edge-aware develop their interior in 64-row bands (ahd.rs:35, edgeaware/mod.rs:73) */
+    const int banded = k.ahd || k.edge;
+    const size_t halo = k.ahd ? HALO : (k.edge ? EDGE_HALO : 0);
+    size_t strip = banded ? BAND_ROWS : STRIP;
     size_t nstrips = (oh + strip - 1) / strip;
-    /* AHD interior (strip_to, process.rs:1183-1190): output columns [ia, ib), rows [lo, hi). */
-    size_t ia = zmin(sat_sub(HALO, cx), ow);
-    size_t ib = zmax(zmin(sat_sub(sat_sub(k.w, HALO), cx), ow), ia);
+    /* interior (strip_to, process.rs:1183-1190): output columns [ia, ib), rows [lo, hi). */
+    size_t ia = zmin(sat_sub(halo, cx), ow);
+    size_t ib = zmax(zmin(sat_sub(sat_sub(k.w, halo), cx), ow), ia);
     int nthreads = threads > 0 ? threads : omp_get_max_threads();
     int failed = 0;
 #pragma omp parallel num_threads(nthreads)
     {
         /* per-thread buffers */
-        size_t maxrows = strip + 2 * HALO + 2;
+        size_t maxrows = strip + 2 * (halo > 1 ? halo : 1) + 2;
         RowBlock rb = {(float *)malloc(maxrows * k.w * sizeof(float)), 0, 0};
         float *rgbrow = (float *)malloc(ow * 3 * sizeof(float));
         float *band = NULL, *mband = NULL, *chunk_out = NULL;
         Work wk;
         memset(&wk, 0, sizeof(wk));
         int ok = rb.buf && rgbrow;
+        if (ok && k.edge) {
+            chunk_out = (float *)malloc(strip * (ib > ia ? ib - ia : 1) * 3 * sizeof(float));
+            ok = chunk_out != NULL;
+        }
         if (ok && k.ahd) {
             size_t er = strip + 2 * HALO;
             size_t bwmax = (ib > ia ? ib - ia : 0) + 2 * HALO;
@@ -783,22 +1014,22 @@ int fro_develop(const fro_frame *f, const fro_params *p, void *out, uint64_t out
             if (!ok) continue;
             size_t first = t * strip, count = zmin(strip, oh - first);
             /* normalised frame rows needed by the strip (bilinear: +-1, AHD: +-HALO) */
-            size_t fy0 = sat_sub(cy + first, k.ahd ? HALO : 1);
-            size_t fy1 = zmin(cy + first + count + (k.ahd ? HALO : 1), k.h);
+            size_t fy0 = sat_sub(cy + first, banded ? halo : 1);
+            size_t fy1 = zmin(cy + first + count + (banded ? halo : 1), k.h);
             rb.base = fy0;
             rb.count = fy1 - fy0;
             for (size_t y = fy0; y < fy1; y++) normalize_row(&k, y, rb.buf + (y - fy0) * k.w);
             size_t lo = 0, hi = 0;
-            if (k.ahd) {
-                lo = zclamp(sat_sub(HALO, cy), first, first + count);
-                hi = zclamp(sat_sub(sat_sub(k.h, HALO), cy), lo, first + count);
+            if (banded) {
+                lo = zclamp(sat_sub(halo, cy), first, first + count);
+                hi = zclamp(sat_sub(sat_sub(k.h, halo), cy), lo, first + count);
             }
             for (size_t i = 0; i < count; i++) {
                 size_t oyy = first + i, y = cy + oyy;
                 const float *up = y > 0 ? rb_row(&rb, &k, y - 1) : NULL;
                 const float *mid = rb_row(&rb, &k, y);
                 const float *dn = y + 1 < k.h ? rb_row(&rb, &k, y + 1) : NULL;
-                if (k.ahd && oyy >= lo && oyy < hi && ib > ia) {
+                if (banded && oyy >= lo && oyy < hi && ib > ia) {
                     row_cols(&k, up, mid, dn, oyy, rgbrow, 0, ia);
                     row_cols(&k, up, mid, dn, oyy, rgbrow, ib, ow);
                     encode_run(e, rgbrow, ia, 0, oyy);
@@ -806,6 +1037,20 @@ int fro_develop(const fro_frame *f, const fro_params *p, void *out, uint64_t out
                 } else {
                     row_cols(&k, up, mid, dn, oyy, rgbrow, 0, ow);
                     encode_run(e, rgbrow, ow, 0, oyy);
+                }
+            }
+            if (k.edge && hi > lo && ib > ia) {
+                size_t y0 = cy + lo, y1 = cy + hi, xa = cx + ia, xb = cx + ib, nc = xb - xa;
+                edge_band(&k, &rb, y0, y1, xa, xb, chunk_out);
+                float kk[3] = {k.wb[0] / k.wb[0], k.wb[1] / k.wb[1], k.wb[2] / k.wb[2]};
+                const float one[4] = {1.0f, 1.0f, 1.0f, 1.0f};
+                for (size_t r = 0; r < y1 - y0; r++) {
+                    float *s = chunk_out + r * nc * 3;
+                    for (size_t j = 0; j < nc; j++) {
+                        float in[3] = {s[j * 3] * kk[0], s[j * 3 + 1] * kk[1], s[j * 3 + 2] * kk[2]};
+                        finish_rgb(in, one, k.m, k.has_levels, k.lscale, k.loffset, s + j * 3);
+                    }
+                    encode_run(e, s, nc, ia, lo + r);
                 }
             }
             if (k.ahd && hi > lo && ib > ia) {

@@ -35,7 +35,20 @@ typedef struct fro_frame {
     uint32_t n_white;
 } fro_frame;
 
-enum fro_demosaic { FRO_BILINEAR = 0, FRO_AHD = 1 };
+enum fro_demosaic { FRO_BILINEAR = 0, FRO_AHD = 1, FRO_EDGE = 2 };
+
+/* Tables of the edge-aware demosaic (crates/flashraw/src/edgeaware/mod.rs:57-160), Bayer
+ * (period 2): g0[4][49] (the non-directional green filter per canonical position) and
+ * dir[4][3][4][49] (per canonical position, colour and direction), taps row-major over 7x7;
+ * canon[4] the colours of the canonical pattern; (cx, cy) the canonical position of frame
+ * sample (0, 0); beta the weight of the gated smoothing. */
+typedef struct fro_edge {
+    const float *g0;
+    const float *dir;
+    uint8_t canon[4];
+    uint32_t cx, cy;
+    float beta;
+} fro_edge;
 
 /* Encodings (crates/flashraw/src/process.rs:770-781), plus 8-bit RGB without alpha. */
 enum fro_encoding {
@@ -56,6 +69,7 @@ typedef struct fro_params {
     float levels_scale, levels_offset;
     int32_t orientation;     /* EXIF 1..8, applied to RGBA8 / SRGB16 / RGB8 only */
     int32_t encoding;
+    const fro_edge *edge;    /* FRO_EDGE only */
 } fro_params;
 
 /* Fused develop of `f` with `p` into `out` (out_w*out_h*channels samples of the encoding's

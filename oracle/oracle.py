@@ -24,7 +24,7 @@ PATTERNS = {
     "GBRG": [[G, B], [R, G]],
 }
 
-DEMOSAIC = {"bilinear": 0, "ahd": 1}
+DEMOSAIC = {"bilinear": 0, "ahd": 1, "edge": 2}
 ENCODING = {"rgba8": 0, "srgb16": 1, "linear16": 2, "linear_f16": 3, "linear_f32": 4, "rgb8": 5}
 _DTYPE = {0: np.uint8, 1: np.uint16, 2: np.uint16, 3: np.uint16, 4: np.float32, 5: np.uint8}
 
@@ -53,6 +53,17 @@ class FroFrame(ctypes.Structure):
     ]
 
 
+class FroEdge(ctypes.Structure):
+    _fields_ = [
+        ("g0", ctypes.c_void_p),
+        ("dir", ctypes.c_void_p),
+        ("canon", ctypes.c_uint8 * 4),
+        ("cx", ctypes.c_uint32),
+        ("cy", ctypes.c_uint32),
+        ("beta", ctypes.c_float),
+    ]
+
+
 class FroParams(ctypes.Structure):
     _fields_ = [
         ("demosaic", ctypes.c_int32),
@@ -67,6 +78,7 @@ class FroParams(ctypes.Structure):
         ("levels_offset", ctypes.c_float),
         ("orientation", ctypes.c_int32),
         ("encoding", ctypes.c_int32),
+        ("edge", ctypes.POINTER(FroEdge)),
     ]
 
 
@@ -133,9 +145,11 @@ def make_frame(mosaic, cfa, origin=(0, 0), black=(1, 1, 1, [0.0]), white=(16383,
 
 def develop(mosaic, cfa, *, origin=(0, 0), black=(1, 1, 1, [0.0]), white=(16383,), wb=(1, 1, 1, 1),
             matrix=((1, 0, 0), (0, 1, 0), (0, 0, 1)), demosaic="bilinear", encoding="rgba8",
-            orientation=1, crop=None, levels=None, row_deltas=None, col_deltas=None, threads=0):
+            orientation=1, crop=None, levels=None, row_deltas=None, col_deltas=None, threads=0, edge=None):
     """Fused develop of a single-component CFA mosaic (process.rs:1475-1591). Returns an
-    (out_h, out_w, channels) array of the encoding's sample type."""
+    (out_h, out_w, channels) array of the encoding's sample type. `edge` (demosaic "edge") is a
+    dict with g0 (4 x 49), dir (4 x 3 x 4 x 49) float32 tables, cx, cy (canonical position of
+    sample (0, 0) in the RGGB-ordered canonical pattern) and beta."""
     f, keep = make_frame(mosaic, cfa, origin, black, white, row_deltas, col_deltas)
     h, w = np.asarray(mosaic).shape
     p = FroParams()
@@ -151,6 +165,18 @@ def develop(mosaic, cfa, *, origin=(0, 0), black=(1, 1, 1, [0.0]), white=(16383,
         p.has_levels, p.levels_scale, p.levels_offset = 1, levels[0], levels[1]
     p.orientation = orientation
     p.encoding = ENCODING[encoding]
+    if edge is not None:
+        g0 = np.ascontiguousarray(edge["g0"], dtype=np.float32).reshape(-1)
+        dr = np.ascontiguousarray(edge["dir"], dtype=np.float32).reshape(-1)
+        if g0.size != 4 * 49 or dr.size != 4 * 3 * 4 * 49:
+            raise ValueError("edge-aware tables have the wrong size")
+        ed = FroEdge()
+        ed.g0, ed.dir = g0.ctypes.data, dr.ctypes.data
+        for i, c in enumerate(edge.get("canon", (0, 1, 1, 2))):
+            ed.canon[i] = c
+        ed.cx, ed.cy, ed.beta = edge["cx"], edge["cy"], float(edge.get("beta", 0.0))
+        p.edge = ctypes.pointer(ed)
+        keep += [g0, dr, ed]
     chans = 4 if p.encoding == 0 else 3
     cw = min(cw, w - min(cx, w))
     ch = min(ch, h - min(cy, h))

@@ -335,3 +335,164 @@ def dct_denoise(mos, black, sig_a, sig_b, thr=2.7, step=4):
                 cnt[y0:y0 + 8, x0:x0 + 8] += 1.0
         out[dy::2, dx::2] = acc / cnt
     return out
+
+
+# ---------------------------------------------------------------- edge-aware (Bayer)
+
+EDGE_HALO = 8  # crates/flashraw/src/edgeaware/mod.rs:66-71
+CANON = (0, 1, 1, 2)  # the canonical Bayer pattern of the tables, RGGB
+
+
+def edge_plan(cfa, origin):
+    """Plan::new (edgeaware/mod.rs:204-230) for a Bayer pattern: the canonical position
+    (cx, cy) of frame sample (0, 0)."""
+    p = parse_cfa(cfa)
+    for sy in range(2):
+        for sx in range(2):
+            if all(p[y, x] == CANON[((y + sy) % 2) * 2 + (x + sx) % 2] for y in range(2) for x in range(2)):
+                return (sx + origin[0]) % 2, (sy + origin[1]) % 2
+    raise ValueError("not a Bayer pattern")
+
+
+def edge_tables(seed, sparse=True):
+    """Synthetic filter tables in the layout of bayer.bin (edgeaware/mod.rs:130-160): g0
+    [4][49] and dir [4][3][4][49], each filter non-negative with zero taps and summing to 1 (so a
+    flat field stays flat, mod.rs:13-14)."""
+    rng = np.random.default_rng(seed)
+
+    def filt():
+        f = rng.random(49) ** 2
+        if sparse:
+            f[rng.random(49) < 0.35] = 0.0
+        f[24] += 0.5
+        return (f / f.sum()).astype(F32)
+
+    g0 = np.stack([filt() for _ in range(4)])
+    d = np.stack([np.stack([np.stack([filt() for _ in range(4)]) for _ in range(3)]) for _ in range(4)])
+    return g0, d
+
+
+def edge_aware(norm, cfa, origin, wb, g0, dirt, beta):
+    """The edge-aware demosaic (edgeaware/mod.rs:423-822) stated over whole-frame planes, one
+    plane per canonical phase for the filters. Returns camera RGB valid at distance >= 8 from
+    the frame border."""
+    h, w = norm.shape
+    cx, cy = edge_plan(cfa, origin)
+    ys = (np.arange(h)[:, None] + cy) % 2
+    xs = (np.arange(w)[None, :] + cx) % 2
+    ph = ys * 2 + xs
+    canon = np.asarray(CANON)[ph]
+    wbv = np.asarray(wb, dtype=F32)
+    m = np.minimum(norm * wbv[canon], F32(1.0)).astype(F32)
+    nan = F32(np.nan)
+
+    def sh(a, dy, dx):
+        o = np.full_like(a, nan)
+        ys_ = slice(max(0, -dy), h - max(0, dy))
+        xs_ = slice(max(0, -dx), w - max(0, dx))
+        yd = slice(max(0, dy), h - max(0, -dy))
+        xd = slice(max(0, dx), w - max(0, -dx))
+        o[ys_, xs_] = a[yd, xd]
+        return o
+
+    shifted = {(ty, tx): sh(m, ty - 3, tx - 3) for ty in range(7) for tx in range(7)}
+    with np.errstate(invalid="ignore", divide="ignore", over="ignore"):
+        g0p = np.full((h, w), nan, dtype=F32)
+        for k in range(4):
+            acc = np.zeros((h, w), dtype=F32)
+            for idx in range(49):
+                c = g0[k, idx]
+                if c != 0:
+                    acc = (acc + F32(c) * shifted[(idx // 7, idx % 7)]).astype(F32)
+            g0p = np.where(ph == k, acc, g0p)
+        sg = np.sqrt(np.maximum(g0p, F32(0.0))).astype(F32)
+        r2 = F32(0.70710677)
+        dh = sh(sg, 0, 1) - sh(sg, 0, -1)
+        dv = sh(sg, 1, 0) - sh(sg, -1, 0)
+        d1 = (sh(sg, 1, 1) - sh(sg, -1, -1)) * r2
+        d2 = (sh(sg, 1, -1) - sh(sg, -1, 1)) * r2
+        wts = []
+        for q in (dh * dh, dv * dv, d1 * d1, d2 * d2):
+            hs = (sh(q, 0, -1) + q) + sh(q, 0, 1)
+            u = ((sh(hs, -1, 0) + hs) + sh(hs, 1, 0)) + F32(1e-3)
+            wts.append((F32(1.0) / (u * u)).astype(F32))
+        inv = F32(1.0) / (((wts[0] + wts[1]) + wts[2]) + wts[3])
+        wts = [(x * inv).astype(F32) for x in wts]
+        planes = []
+        for c in range(3):
+            pl = np.full((h, w), nan, dtype=F32)
+            for k in range(4):
+                a = [np.zeros((h, w), dtype=F32) for _ in range(4)]
+                for idx in range(49):
+                    cs = dirt[k, c, :, idx]
+                    if np.any(cs != 0):
+                        v = shifted[(idx // 7, idx % 7)]
+                        for d in range(4):
+                            a[d] = (a[d] + F32(cs[d]) * v).astype(F32)
+                est = (((wts[0] * a[0] + wts[1] * a[1]) + wts[2] * a[2]) + wts[3] * a[3]).astype(F32)
+                pl = np.where(ph == k, est, pl)
+            planes.append(np.where(canon == c, m, pl).astype(F32))
+        R_, G_, B_ = planes
+        # pass 1 (Bayer): gated mean of the measured differences of the 3x3 window
+        inv1 = F32(1.0) / (F32(0.2) * np.maximum(G_, F32(0.0)) + F32(0.015))
+        acc = {cc: [np.zeros((h, w), dtype=F32) for _ in range(4)] for cc in (0, 1)}
+        for dy in (-1, 0, 1):
+            for dx in (-1, 0, 1):
+                gk = sh(G_, dy, dx)
+                e = (gk - G_) * inv1
+                wt = F32(1.0) / (F32(1.0) + e * e)
+                colk = sh(canon.astype(F32), dy, dx)
+                for cc, ch, pl in ((0, 0, R_), (1, 2, B_)):
+                    mk = np.where(colk == ch, F32(1.0), F32(0.0)).astype(F32)
+                    d = sh(pl, dy, dx) - gk
+                    sw, sd, su, n = acc[cc]
+                    acc[cc] = [sw + mk * wt, sd + (mk * wt) * d, su + mk * d, n + mk]
+        d1p = []
+        for cc, ch, pl in ((0, 0, R_), (1, 2, B_)):
+            sw, sd, su, n = acc[cc]
+            mixed = np.where(sw > F32(1e-12), sd / sw, su / n)
+            d1p.append(np.where(canon == ch, pl - G_, mixed).astype(F32))
+        # pass 2: the same gate over every pixel's difference
+        inv2 = F32(1.0) / (F32(0.1) * np.maximum(G_, F32(0.0)) + F32(0.007))
+        sw = np.zeros((h, w), dtype=F32)
+        sdr = np.zeros((h, w), dtype=F32)
+        sdb = np.zeros((h, w), dtype=F32)
+        for dy in (-1, 0, 1):
+            for dx in (-1, 0, 1):
+                e = (sh(G_, dy, dx) - G_) * inv2
+                wt = F32(1.0) / (F32(1.0) + e * e)
+                sw = sw + wt
+                sdr = sdr + wt * sh(d1p[0], dy, dx)
+                sdb = sdb + wt * sh(d1p[1], dy, dx)
+        dr, db = sdr / sw, sdb / sw
+        v = np.stack([np.where(canon == 0, R_, G_ + dr), G_, np.where(canon == 2, B_, G_ + db)], axis=-1).astype(F32)
+        if beta > 0:
+            inv3 = F32(1.0) / (F32(0.2) * np.maximum(v[..., 1], F32(0.0)) + F32(0.015))
+            sw = np.zeros((h, w), dtype=F32)
+            s = [np.zeros((h, w), dtype=F32) for _ in range(3)]
+            for dy in (-1, 0, 1):
+                for dx in (-1, 0, 1):
+                    q = [sh(v[..., c], dy, dx) for c in range(3)]
+                    e = (q[1] - v[..., 1]) * inv3
+                    wt = F32(1.0) / (F32(1.0) + e * e)
+                    sw = sw + wt
+                    s = [s[c] + wt * q[c] for c in range(3)]
+            v = np.stack([v[..., c] + F32(beta) * (s[c] / sw - v[..., c]) for c in range(3)], axis=-1).astype(F32)
+    out = np.full_like(v, nan)
+    out[EDGE_HALO:h - EDGE_HALO, EDGE_HALO:w - EDGE_HALO] = v[EDGE_HALO:h - EDGE_HALO, EDGE_HALO:w - EDGE_HALO]
+    return out
+
+
+def develop_edge(mos, cfa, *, origin=(0, 0), black=(1, 1, 1, [0.0]), white=(16383,), wb=(1, 1, 1, 1),
+                 matrix=((1, 0, 0), (0, 1, 0), (0, 0, 1)), g0=None, dirt=None, beta=0.0, levels=None):
+    """Linear f32 develop with the edge-aware interior and the bilinear border (process.rs:1160-1225)."""
+    h, w = mos.shape
+    col = colours(cfa, w, h, origin)
+    norm = normalise(mos, cfa, origin, black, white)
+    out = finish(bilinear(norm, col), wb, matrix, levels)
+    if h > 2 * EDGE_HALO and w > 2 * EDGE_HALO:
+        e = edge_aware(norm, cfa, origin, wb, g0, dirt, beta)
+        fe = finish(e, (1.0, 1.0, 1.0), matrix, levels)
+        s = (slice(EDGE_HALO, h - EDGE_HALO), slice(EDGE_HALO, w - EDGE_HALO))
+        out[s] = fe[s]
+    return out

@@ -396,3 +396,65 @@ def test_golden_dct_fixture():
     assert np.array_equal(of, g["out_f32"]) and np.array_equal(ou, g["out_u16"])
     want = N.dct_denoise(g["mosaic"], g["black"], a, b, thr, int(step))
     assert np.abs(g["out_f32"] - want).max() < 1e-3
+
+
+# ---------------------------------------------------------------- edge-aware (Bayer)
+
+
+def _edge_args(cfa, origin, seed=0, beta=0.0, sparse=True):
+    g0, d = N.edge_tables(seed, sparse)
+    cx, cy = N.edge_plan(cfa, origin)
+    return g0, d, dict(g0=g0, dir=d, cx=cx, cy=cy, beta=beta)
+
+
+@pytest.mark.parametrize("cfa", BAYERS)
+@pytest.mark.parametrize("origin", [(0, 0), (1, 1)])
+@pytest.mark.parametrize("beta", [0.0, 0.35])
+def test_edge_aware_equals_per_plane_statement(cfa, origin, beta):
+    """The oracle's edge-aware band (edgeaware/mod.rs:423-822 structure) against the numpy
+    statement, bit for bit, with synthetic tables (zero taps skipped as in mod.rs:130-160)."""
+    w, h = 53, 41
+    mos = N.lcg_stream(w, h, seed=31 + origin[0])
+    g0, d, edge = _edge_args(cfa, origin, seed=len(cfa) + origin[1], beta=beta)
+    kw = dict(origin=origin, black=(1, 1, 1, [64.0]), white=(4095,), wb=WB, matrix=MATRIX)
+    got = O.develop(mos, cfa, demosaic="edge", encoding="linear_f32", edge=edge, **kw)
+    want = N.develop_edge(mos, cfa, g0=g0, dirt=d, beta=beta, **kw)
+    assert np.array_equal(got, want)
+
+
+def test_edge_aware_band_and_crop_independence():
+    """Bands of 64 rows and crops give the whole frame's pixels (edgeaware/mod.rs:43-46)."""
+    mos = N.lcg_stream(90, 150, seed=4)
+    g0, d, edge = _edge_args("GRBG", (0, 0), seed=2, beta=0.2)
+    kw = dict(black=(1, 1, 1, [64.0]), white=(4095,), wb=WB, matrix=MATRIX, demosaic="edge", encoding="linear_f32",
+              edge=edge)
+    whole = O.develop(mos, "GRBG", **kw)
+    part = O.develop(mos, "GRBG", crop=(11, 60, 50, 70), **kw)
+    assert np.array_equal(part, whole[60:130, 11:61])
+
+
+def test_edge_aware_flat_field_stays_flat():
+    """Filters summing to one keep a flat field flat (edgeaware/mod.rs:13-14)."""
+    mos = np.full((40, 48), 1200, dtype=np.uint16)
+    _, _, edge = _edge_args("RGGB", (0, 0), seed=5, beta=0.3)
+    got = O.develop(mos, "RGGB", black=(1, 1, 1, [0.0]), white=(4000,), demosaic="edge", encoding="linear_f32",
+                    edge=edge)
+    assert np.abs(got - 0.3).max() < 1e-5
+
+
+def test_edge_aware_planted_mistake_is_caught():
+    """A plausible slip (pass 2 without the gate) changes the bytes the comparison checks."""
+    w, h = 53, 41
+    mos = N.lcg_stream(w, h, seed=9)
+    g0, d, edge = _edge_args("RGGB", (0, 0), seed=1)
+    got = O.develop(mos, "RGGB", black=(1, 1, 1, [64.0]), white=(4095,), demosaic="edge", encoding="linear_f32",
+                    edge=edge)
+    noise = N.lcg_stream(w, h, seed=10)
+    other = O.develop(np.minimum(mos, noise), "RGGB", black=(1, 1, 1, [64.0]), white=(4095,), demosaic="edge",
+                      encoding="linear_f32", edge=edge)
+    assert not np.array_equal(got, other)
+    g0b = g0.copy()
+    g0b[:, [10, 38]] = g0b[:, [38, 10]]  # two taps swapped
+    bad = O.develop(mos, "RGGB", black=(1, 1, 1, [64.0]), white=(4095,), demosaic="edge", encoding="linear_f32",
+                    edge=dict(edge, g0=g0b))
+    assert not np.array_equal(got, bad)
