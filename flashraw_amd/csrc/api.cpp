@@ -149,6 +149,7 @@ frh_context::~frh_context() {
     if (d_lut8) (void)hipFree(d_lut8);
     if (d_srgb16) (void)hipFree(d_srgb16);
     if (d_wtab) (void)hipFree(d_wtab);
+    for (auto &kv : edge_tabs) (void)hipFree(kv.second);
     if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -256,6 +257,61 @@ static uint32_t tuned_rows(const frh_context *ctx, bool ahd, const DevArgs &a, u
 }
 
 // Everything a develop launch needs, validated (run.rs:439-646).
+// The device tap lists of an edge-aware spec (built once per distinct table contents and kept by
+// the context): per canonical position, G0's non-zero taps and, per colour, the taps where some
+// direction is non-zero, in tap order (edgeaware/mod.rs:130-160).
+static int32_t edge_tables(frh_context *ctx, const frh_edge_spec *e, const EdgeTab **out) {
+    uint64_t hsh = 1469598103934665603ull;
+    auto mix = [&](const void *p, size_t n) {
+        const uint8_t *b = static_cast<const uint8_t *>(p);
+        for (size_t i = 0; i < n; i++) hsh = (hsh ^ b[i]) * 1099511628211ull;
+    };
+    mix(e->canon, 4);
+    mix(e->g0, 4 * 49 * sizeof(float));
+    mix(e->dir, 4 * 3 * 4 * 49 * sizeof(float));
+    std::lock_guard<std::mutex> g(ctx->mu);
+    auto it = ctx->edge_tabs.find(hsh);
+    if (it != ctx->edge_tabs.end()) {
+        *out = static_cast<const EdgeTab *>(it->second);
+        return FRH_OK;
+    }
+    EdgeTab t;
+    memset(&t, 0, sizeof(t));
+    for (int ph = 0; ph < 4; ph++) {
+        for (uint32_t i = 0; i < 49; i++) {
+            const float c = e->g0[ph * 49 + i];
+            if (c != 0.0f) {
+                t.g0_idx[ph][t.g0_n[ph]] = i;
+                t.g0_c[ph][t.g0_n[ph]] = c;
+                t.g0_n[ph]++;
+            }
+        }
+        for (int c = 0; c < 3; c++)
+            for (uint32_t i = 0; i < 49; i++) {
+                float v[4];
+                bool any = false;
+                for (int d = 0; d < 4; d++) {
+                    v[d] = e->dir[((ph * 3 + c) * 4 + d) * 49 + i];
+                    any = any || v[d] != 0.0f;
+                }
+                if (!any) continue;
+                const uint32_t n = t.dir_n[ph][c]++;
+                t.dir_idx[ph][c][n] = i;
+                for (int d = 0; d < 4; d++) t.dir_c[ph][c][n][d] = v[d];
+            }
+    }
+    void *d = nullptr;
+    hipError_t err;
+    if ((err = hipMalloc(&d, sizeof(EdgeTab))) != hipSuccess) return hip_fail(err, "edge tables");
+    if ((err = hipMemcpy(d, &t, sizeof(EdgeTab), hipMemcpyHostToDevice)) != hipSuccess) {
+        (void)hipFree(d);
+        return hip_fail(err, "edge tables");
+    }
+    ctx->edge_tabs[hsh] = d;
+    *out = static_cast<const EdgeTab *>(d);
+    return FRH_OK;
+}
+
 int32_t frh::plan_develop(const frh_context *ctx, const frh_frame *f, const frh_frame_info *info,
                             const frh_develop_spec *spec, Plan &p) {
     if (!f || !info || !spec) return fail(FRH_ERROR_INVALID_ARGUMENT, "NULL descriptor");
@@ -393,11 +449,8 @@ int32_t frh::plan_develop(const frh_context *ctx, const frh_frame *f, const frh_
     if (spec->edge) {
         const frh_edge_spec *e = spec->edge;
         if (e->size < sizeof(frh_edge_spec)) return fail(FRH_ERROR_INVALID_ARGUMENT, "edge spec size too small");
-        if (!e->ahd)
-            return fail(FRH_ERROR_NOT_AVAILABLE, "the edge-aware filter demosaic is not supported yet (AHD is)");
-        // run.rs:588-598: AHD needs period 2 and beta 0
-        if (e->period != 2 || e->beta != 0.0f) return fail(FRH_ERROR_INVALID_ARGUMENT, "inconsistent AHD spec");
-        if (e->halo < 5 || e->halo > 64) return fail(FRH_ERROR_INVALID_ARGUMENT, "AHD halo must be in [5, 64]");
+        if (e->period != 2)
+            return fail(FRH_ERROR_NOT_AVAILABLE, "only Bayer (period 2) edge specs are supported by this path");
         if (e->canon) {
             if (e->canon_len != 4) return fail(FRH_ERROR_INVALID_ARGUMENT, "inconsistent edge-aware filter tables");
             // The pattern the spec describes must be the frame's: colour of frame (x, y) is
@@ -406,17 +459,43 @@ int32_t frh::plan_develop(const frh_context *ctx, const frh_frame *f, const frh_
                 for (uint32_t x = 0; x < 2; x++) {
                     uint32_t want = a.chan[((y + a.oy) & 1) * 2 + ((x + a.ox) & 1)];
                     if (e->canon[((y + e->cy) % 2) * 2 + (x + e->cx) % 2] != want)
-                        return fail(FRH_ERROR_INVALID_ARGUMENT, "AHD pattern does not match the frame's CFA");
+                        return fail(FRH_ERROR_INVALID_ARGUMENT, "edge spec pattern does not match the frame's CFA");
                 }
         }
-        a.ahd_halo = e->halo;
-        // Implementation knobs (environment, like the reference's FLASHRAW_* variables): the LDS-tiled
-        // AHD kernel instead of the row walker, and the walker's rows per strip.
-        const char *impl = getenv("FLASHRAW_AHD_IMPL");
-        a.ahd_impl = (impl && strcmp(impl, "tiled") == 0) ? 1u : 0u;
-        const char *rows = getenv("FLASHRAW_AHD_ROWS");
-        a.ahd_rows = rows ? (uint32_t)atoi(rows) : tuned_rows(ctx, true, a, ctx->ahd_rows);
-        p.ahd = true;
+        if (e->ahd) {
+            // run.rs:588-598: AHD needs period 2 and beta 0
+            if (e->beta != 0.0f) return fail(FRH_ERROR_INVALID_ARGUMENT, "inconsistent AHD spec");
+            if (e->halo < 5 || e->halo > 64) return fail(FRH_ERROR_INVALID_ARGUMENT, "AHD halo must be in [5, 64]");
+            a.ahd_halo = e->halo;
+            // Implementation knobs (environment, like the reference's FLASHRAW_* variables): the
+            // LDS-tiled AHD kernel instead of the row walker, and the walker's rows per strip.
+            const char *impl = getenv("FLASHRAW_AHD_IMPL");
+            a.ahd_impl = (impl && strcmp(impl, "tiled") == 0) ? 1u : 0u;
+            const char *rows = getenv("FLASHRAW_AHD_ROWS");
+            a.ahd_rows = rows ? (uint32_t)atoi(rows) : tuned_rows(ctx, true, a, ctx->ahd_rows);
+            p.ahd = true;
+        } else {
+            // the edge-aware filters (edgeaware/mod.rs:57-160): tables of a Bayer plan
+            if (!e->canon || !e->g0 || !e->dir || e->g0_len != 4 * 49 || e->dir_len != 4 * 3 * 4 * 49)
+                return fail(FRH_ERROR_INVALID_ARGUMENT, "edge-aware spec needs canon[4], g0[4*49] and dir[4*3*4*49]");
+            if (!(e->beta >= 0.0f && e->beta <= 1.0f))
+                return fail(FRH_ERROR_INVALID_ARGUMENT, "edge-aware beta must be in [0, 1]");
+            if (e->halo < 8 || e->halo > 64)
+                return fail(FRH_ERROR_INVALID_ARGUMENT, "edge-aware halo must be in [8, 64]");
+            for (uint32_t i = 0; i < 4 * 49; i++)
+                if (!std::isfinite(e->g0[i])) return fail(FRH_ERROR_INVALID_ARGUMENT, "non-finite filter tap");
+            for (uint32_t i = 0; i < 4 * 3 * 4 * 49; i++)
+                if (!std::isfinite(e->dir[i])) return fail(FRH_ERROR_INVALID_ARGUMENT, "non-finite filter tap");
+            int32_t rc = edge_tables(const_cast<frh_context *>(ctx), e, &a.etab);
+            if (rc) return rc;
+            a.edge_aware = 1;
+            a.ecx = e->cx % 2;
+            a.ecy = e->cy % 2;
+            a.ebeta = e->beta;
+            a.ahd_halo = e->halo;
+            a.ecanon_bits = 0;
+            for (int i = 0; i < 4; i++) a.ecanon_bits |= (uint32_t)(e->canon[i] & 3u) << (2 * i);
+        }
     }
     int32_t modes[2] = {spec->output, spec->output2};
     for (int k = 0; k < 2; k++) {
@@ -436,7 +515,7 @@ int32_t frh::plan_develop(const frh_context *ctx, const frh_frame *f, const frh_
 // every band's input rows (+ halo) and output rows must fit; at least one 16-row tile.
 static uint32_t band_rows(const frh_context *ctx, const Plan &p) {
     if (!ctx->max_binding) return p.a.dev_h;
-    uint64_t halo = p.ahd ? p.a.ahd_halo : 1;
+    uint64_t halo = (p.ahd || p.a.edge_aware) ? p.a.ahd_halo : 1;
     uint64_t out_row = 0;
     for (int k = 0; k < 2; k++) out_row += p.bytes[k] / (p.out_h[k] ? p.out_h[k] : 1);
     uint64_t rows_in = ctx->max_binding / (p.in_row_bytes ? p.in_row_bytes : 1);

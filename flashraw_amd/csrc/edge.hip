@@ -1,0 +1,287 @@
+// Edge-aware demosaic (Bayer) fused with the finish and the encode (gfx950).
+//
+// The method of crates/flashraw/src/edgeaware/mod.rs:423-822 with the reference's f32 operations
+// in the reference's order: a 7x7 non-directional green estimate G0 per canonical position, the
+// square root of it, four directional gradient energies summed over 3x3 giving the weights
+// w_d = 1 / (E_d + 1e-3)^2 (normalised), one 7x7 filter per position, colour and direction for
+// the missing colours, two gated passes over the colour differences and, when beta > 0, a gated
+// smoothing. Taps whose coefficient is zero (in every direction, for the directional filters)
+// are skipped, as the reference's tap lists skip them (mod.rs:130-160).
+//
+// One block develops a 32 x 16 output tile. Each stage writes its plane (with the halo the next
+// stage needs) to LDS: balanced mosaic (+-8), sqrt(G0) (+-5), squared gradients (+-4), weights
+// and the three colour planes (+-3), pass-1 differences (+-2), pass-2 colours (+-1). The filter
+// stages run phase by phase so that every wave walks one tap list; the tap lists sit in the
+// constant address space and are read with scalar loads. Pixels nearer the frame border than the
+// halo are developed bilinearly by extra blocks of the same launch (strip_to,
+// process.rs:1160-1225).
+#include <hip/hip_runtime.h>
+
+#include "develop_common.h"
+#include "frh_internal.h"
+#include "frh_launch.h"
+
+namespace frh {
+namespace {
+
+constexpr int ETW = 32, ETH = 16;      // output tile
+constexpr int EH = 8;                  // halo of the mosaic plane
+constexpr int MW = ETW + 2 * EH, MHt = ETH + 2 * EH;   // mosaic plane 48 x 32
+constexpr int SW = ETW + 10, SH = ETH + 10;            // sqrt(G0) +-5
+constexpr int DW = ETW + 8, DH = ETH + 8;              // gradients +-4
+constexpr int PW = ETW + 6, PH = ETH + 6;              // weights, colour planes +-3
+constexpr int QW = ETW + 4, QH = ETH + 4;              // pass-1 differences +-2
+constexpr int VW = ETW + 2, VH = ETH + 2;              // pass-2 colours +-1
+
+typedef __attribute__((address_space(4))) const EdgeTab ctab_t;
+
+struct EdgeLds {
+    float m[MHt * MW];
+    union {
+        float sg[SH * SW];
+        float d1[2][QH * QW];
+    } r2;
+    union {
+        float dsq[4][DH * DW];
+        float v[3][VH * VW];
+    } r3;
+    float w[4][PH * PW];
+    float p[3][PH * PW];
+};
+
+__device__ __forceinline__ int edge_phase(const DevArgs &a, int x, int y) {
+    return (int)((((uint32_t)y + a.ecy) & 1u) * 2u + (((uint32_t)x + a.ecx) & 1u));
+}
+// Colour (0 R, 1 G, 2 B) of canonical position ph.
+__device__ __forceinline__ uint32_t canon_of(const DevArgs &a, int ph) { return (a.ecanon_bits >> (2 * ph)) & 3u; }
+
+// Pixels of canonical phase k in a plane of rw x rh starting at frame (x0, y0): the first row
+// and column of that phase and their counts.
+__device__ __forceinline__ void phase_grid(const DevArgs &a, int k, int x0, int y0, int rw, int rh, int &fr, int &fc,
+                                           int &nr, int &nc) {
+    fr = (int)((((uint32_t)(k >> 1)) - ((uint32_t)y0 + a.ecy)) & 1u);
+    fc = (int)((((uint32_t)(k & 1)) - ((uint32_t)x0 + a.ecx)) & 1u);
+    nr = rh > fr ? (rh - fr + 1) / 2 : 0;
+    nc = rw > fc ? (rw - fc + 1) / 2 : 0;
+}
+
+template <bool FAST>
+__global__ __launch_bounds__(256) void k_develop_edge(DevArgs a, Border B, int tiles_x, uint32_t tiles) {
+    __shared__ Lds s;
+    __shared__ EdgeLds E;
+    load_lds(a, s);
+    if (blockIdx.x >= tiles) {
+        __syncthreads();
+        const uint32_t i = (blockIdx.x - tiles) * blockDim.x + threadIdx.x;
+        if (i < B.count) border_pixel(a, s, B, i);
+        return;
+    }
+    const ctab_t *T = (const ctab_t *)(uintptr_t)a.etab;
+    const int tid = threadIdx.x;
+    // interior of this band in frame coordinates
+    const int ix0 = B.ix0 + (int)a.crop_x, ix1 = B.ix1 + (int)a.crop_x;
+    const int iy0 = B.iy0 + (int)a.crop_y, iy1 = B.iy1 + (int)a.crop_y;
+    const int X0 = ix0 + (int)(blockIdx.x % (uint32_t)tiles_x) * ETW;
+    const int Y0 = iy0 + (int)(blockIdx.x / (uint32_t)tiles_x) * ETH;
+    const int W = (int)a.width, H = (int)a.height;
+    __syncthreads();
+    // 1. balanced mosaic min(norm * wb[colour], 1) (mod.rs:627-640), clamped reads beyond the frame
+    for (int i = tid; i < MHt * MW; i += 256) {
+        const int y = min(max(Y0 - EH + i / MW, 0), H - 1), x = min(max(X0 - EH + i % MW, 0), W - 1);
+        const uint32_t c = canon_of(a, edge_phase(a, x, y));
+        const float wbv = c == 0 ? a.wb[0] : (c == 1 ? a.wb[1] : a.wb[2]);
+        E.m[i] = fminf(norm_at(a, s, (uint32_t)x, (uint32_t)y) * wbv, 1.0f);
+    }
+    __syncthreads();
+    // 2. sqrt(G0) over +-5, phase by phase (mod.rs:667-694)
+    for (int k = 0; k < 4; k++) {
+        int fr, fc, nr, nc;
+        phase_grid(a, k, X0 - 5, Y0 - 5, SW, SH, fr, fc, nr, nc);
+        const int n = T->g0_n[k];
+        for (int i = tid; i < nr * nc; i += 256) {
+            const int ry = fr + 2 * (i / nc), rx = fc + 2 * (i % nc);  // in the sqrt(G0) plane
+            const float *mp = &E.m[ry * MW + rx];  // tap (0, 0) of the 7x7 window: frame (x - 3, y - 3)
+            float acc = 0.0f;
+            for (int t = 0; t < n; t++) {
+                const uint32_t idx = T->g0_idx[k][t];
+                acc += T->g0_c[k][t] * mp[(idx / 7u) * MW + idx % 7u];
+            }
+            E.r2.sg[ry * SW + rx] = sqrt_max0(acc);
+        }
+    }
+    __syncthreads();
+    // 3. squared gradients of sqrt(G0) over +-4 (mod.rs:698-723)
+    const float r2 = 0.70710677f;
+    for (int i = tid; i < DH * DW; i += 256) {
+        const int y = i / DW + 1, x = i % DW + 1;  // in the sqrt(G0) plane
+        const float *g = E.r2.sg;
+        const float dh = g[y * SW + x + 1] - g[y * SW + x - 1];
+        const float dv = g[(y + 1) * SW + x] - g[(y - 1) * SW + x];
+        const float e1 = (g[(y + 1) * SW + x + 1] - g[(y - 1) * SW + x - 1]) * r2;
+        const float e2 = (g[(y + 1) * SW + x - 1] - g[(y - 1) * SW + x + 1]) * r2;
+        E.r3.dsq[0][i] = dh * dh;
+        E.r3.dsq[1][i] = dv * dv;
+        E.r3.dsq[2][i] = e1 * e1;
+        E.r3.dsq[3][i] = e2 * e2;
+    }
+    __syncthreads();
+    // 4. direction weights over +-3: 3x3 sums, 1 / (E + eps)^2, normalised (mod.rs:724-759)
+    for (int i = tid; i < PH * PW; i += 256) {
+        const int y = i / PW + 1, x = i % PW + 1;  // in the gradient plane
+        float w[4];
+#pragma unroll
+        for (int d = 0; d < 4; d++) {
+            const float *q = E.r3.dsq[d];
+            const float h0 = q[(y - 1) * DW + x - 1] + q[(y - 1) * DW + x] + q[(y - 1) * DW + x + 1];
+            const float h1 = q[y * DW + x - 1] + q[y * DW + x] + q[y * DW + x + 1];
+            const float h2 = q[(y + 1) * DW + x - 1] + q[(y + 1) * DW + x] + q[(y + 1) * DW + x + 1];
+            const float u = h0 + h1 + h2 + 1e-3f;
+            w[d] = 1.0f / (u * u);
+        }
+        const float inv = 1.0f / (w[0] + w[1] + w[2] + w[3]);
+#pragma unroll
+        for (int d = 0; d < 4; d++) E.w[d][i] = w[d] * inv;
+    }
+    __syncthreads();
+    // 5. the colour planes over +-3: the own colour is the sample, the missing ones the weighted
+    //    directional filters (mod.rs:762-795), phase by phase
+    for (int k = 0; k < 4; k++) {
+        const uint32_t own = canon_of(a, k);
+        int fr, fc, nr, nc;
+        phase_grid(a, k, X0 - 3, Y0 - 3, PW, PH, fr, fc, nr, nc);
+        for (int i = tid; i < nr * nc; i += 256) {
+            const int ry = fr + 2 * (i / nc), rx = fc + 2 * (i % nc);  // in the colour planes
+            const float *mp = &E.m[(ry + 5 - 3) * MW + (rx + 5 - 3)];
+            const int o = ry * PW + rx;
+            const float w0 = E.w[0][o], w1 = E.w[1][o], w2 = E.w[2][o], w3 = E.w[3][o];
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                if ((uint32_t)c == own) {
+                    E.p[c][o] = mp[3 * MW + 3];
+                    continue;
+                }
+                float acc0 = 0.0f, acc1 = 0.0f, acc2 = 0.0f, acc3 = 0.0f;
+                const int n = T->dir_n[k][c];
+                for (int t = 0; t < n; t++) {
+                    const uint32_t idx = T->dir_idx[k][c][t];
+                    const float v = mp[(idx / 7u) * MW + idx % 7u];
+                    acc0 += T->dir_c[k][c][t][0] * v;
+                    acc1 += T->dir_c[k][c][t][1] * v;
+                    acc2 += T->dir_c[k][c][t][2] * v;
+                    acc3 += T->dir_c[k][c][t][3] * v;
+                }
+                E.p[c][o] = w0 * acc0 + w1 * acc1 + w2 * acc2 + w3 * acc3;
+            }
+        }
+    }
+    __syncthreads();
+    // 6. pass 1 over +-2: gated mean of the measured differences of the 3x3 window
+    //    (mod.rs:476-530); the planes of the gradients are free again
+    for (int i = tid; i < QH * QW; i += 256) {
+        const int y = i / QW + 1, x = i % QW + 1;  // in the colour planes
+        const int fy = Y0 - 3 + y, fx = X0 - 3 + x;
+        const float g = E.p[1][y * PW + x];
+        const float inv = 1.0f / (0.2f * fmaxf(g, 0.0f) + 0.015f);
+        float sw0 = 0.0f, sd0 = 0.0f, su0 = 0.0f, n0 = 0.0f, sw1 = 0.0f, sd1 = 0.0f, su1 = 0.0f, n1 = 0.0f;
+#pragma unroll
+        for (int dy = -1; dy <= 1; dy++)
+#pragma unroll
+            for (int dx = -1; dx <= 1; dx++) {
+                const int q = (y + dy) * PW + x + dx;
+                const float gk = E.p[1][q];
+                const float e = (gk - g) * inv;
+                const float wt = 1.0f / (1.0f + e * e);
+                const uint32_t col = canon_of(a, edge_phase(a, fx + dx, fy + dy));
+                const float m0 = col == 0 ? 1.0f : 0.0f, m1 = col == 2 ? 1.0f : 0.0f;
+                const float d0 = E.p[0][q] - gk, d2 = E.p[2][q] - gk;
+                sw0 += m0 * wt;
+                sd0 += (m0 * wt) * d0;
+                su0 += m0 * d0;
+                n0 += m0;
+                sw1 += m1 * wt;
+                sd1 += (m1 * wt) * d2;
+                su1 += m1 * d2;
+                n1 += m1;
+            }
+        const uint32_t own = canon_of(a, edge_phase(a, fx, fy));
+        const float mixed0 = sw0 > 1e-12f ? sd0 / sw0 : su0 / n0;
+        const float mixed1 = sw1 > 1e-12f ? sd1 / sw1 : su1 / n1;
+        E.r2.d1[0][i] = own == 0 ? E.p[0][y * PW + x] - g : mixed0;
+        E.r2.d1[1][i] = own == 2 ? E.p[2][y * PW + x] - g : mixed1;
+    }
+    __syncthreads();
+    // 7. pass 2 over +-1: the same gate over every pixel's difference (mod.rs:534-598)
+    for (int i = tid; i < VH * VW; i += 256) {
+        const int y = i / VW + 1, x = i % VW + 1;  // in the pass-1 plane
+        const int py = y + 1, px = x + 1;          // in the colour planes
+        const float g = E.p[1][py * PW + px];
+        const float inv = 1.0f / (0.1f * fmaxf(g, 0.0f) + 0.007f);
+        float sw = 0.0f, sdr = 0.0f, sdb = 0.0f;
+#pragma unroll
+        for (int dy = -1; dy <= 1; dy++)
+#pragma unroll
+            for (int dx = -1; dx <= 1; dx++) {
+                const float gk = E.p[1][(py + dy) * PW + px + dx];
+                const float e = (gk - g) * inv;
+                const float wt = 1.0f / (1.0f + e * e);
+                const int q = (y + dy) * QW + x + dx;
+                sw += wt;
+                sdr += wt * E.r2.d1[0][q];
+                sdb += wt * E.r2.d1[1][q];
+            }
+        const float dr = sdr / sw, db = sdb / sw;
+        const uint32_t own = canon_of(a, edge_phase(a, X0 - 1 + x - 1, Y0 - 1 + y - 1));
+        E.r3.v[0][i] = own == 0 ? E.p[0][py * PW + px] : g + dr;
+        E.r3.v[1][i] = g;
+        E.r3.v[2][i] = own == 2 ? E.p[2][py * PW + px] : g + db;
+    }
+    __syncthreads();
+    // 8. gated smoothing (beta > 0, mod.rs:803-822), finish with unit balance, encode, store
+    const int xe = min(X0 + ETW, ix1), ye = min(Y0 + ETH, iy1);
+    for (int i = tid; i < ETW * ETH; i += 256) {
+        const int ty = i / ETW, tx = i % ETW;
+        const int fx = X0 + tx, fy = Y0 + ty;
+        if (fx >= xe || fy >= ye) continue;
+        const int c = (ty + 1) * VW + tx + 1;
+        float r = E.r3.v[0][c], g = E.r3.v[1][c], b = E.r3.v[2][c];
+        if (a.ebeta > 0.0f) {
+            const float inv = 1.0f / (0.2f * fmaxf(g, 0.0f) + 0.015f);
+            float sw = 0.0f, s0 = 0.0f, s1 = 0.0f, s2 = 0.0f;
+#pragma unroll
+            for (int dy = -1; dy <= 1; dy++)
+#pragma unroll
+                for (int dx = -1; dx <= 1; dx++) {
+                    const int q = c + dy * VW + dx;
+                    const float e = (E.r3.v[1][q] - g) * inv;
+                    const float wt = 1.0f / (1.0f + e * e);
+                    sw += wt;
+                    s0 += wt * E.r3.v[0][q];
+                    s1 += wt * E.r3.v[1][q];
+                    s2 += wt * E.r3.v[2][q];
+                }
+            r = r + a.ebeta * (s0 / sw - r);
+            const float g2 = g + a.ebeta * (s1 / sw - g);
+            b = b + a.ebeta * (s2 / sw - b);
+            g = g2;
+        }
+        float o0, o1, o2;
+        finish(a, r, g, b, true, o0, o1, o2);
+        store_both(a, s, (uint32_t)(fx - (int)a.crop_x), (uint32_t)(fy - (int)a.crop_y), o0, o1, o2);
+    }
+}
+
+}  // namespace
+
+hipError_t launch_develop_edge(const DevArgs &a, hipStream_t stream) {
+    const Border B = make_border(a, (int)a.ahd_halo);
+    const int iw = B.ix1 - B.ix0, ih = B.iy1 - B.iy0;
+    const int tiles_x = iw > 0 ? (iw + ETW - 1) / ETW : 0;
+    const uint32_t tiles = (iw > 0 && ih > 0) ? (uint32_t)(tiles_x * ((ih + ETH - 1) / ETH)) : 0u;
+    const uint32_t border_blocks = (B.count + 255) / 256;
+    const uint32_t grid = tiles + border_blocks;
+    if (grid == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_develop_edge<false>, dim3(grid), dim3(256), 0, stream, a, B, tiles_x, tiles);
+    return hipGetLastError();
+}
+
+}  // namespace frh

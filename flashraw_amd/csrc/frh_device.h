@@ -21,6 +21,18 @@ enum OutMode : int32_t {
     OUT_NONE = -1
 };
 
+// Tap lists of the edge-aware filters (edgeaware/mod.rs:130-160) for a Bayer pattern: per
+// canonical position, the taps of G0 whose coefficient is not zero and, per colour, the taps where
+// some direction is not zero (with all four directions' coefficients), in tap order.
+struct EdgeTab {
+    uint32_t g0_n[4];
+    uint32_t dir_n[4][3];
+    uint32_t g0_idx[4][49];
+    float g0_c[4][49];
+    uint32_t dir_idx[4][3][49];
+    float dir_c[4][3][49][4];
+};
+
 // Everything one develop launch reads, passed by value as the kernel argument.
 struct DevArgs {
     // input: `src` points at the window's sample (0, 0); `stride` samples per raster row
@@ -69,6 +81,13 @@ struct DevArgs {
     uint32_t ahd_rows;       // rows per walker strip (even)
     uint32_t bil_impl;       // bilinear: 0 register row walker, 1 LDS-tiled (the same bytes)
     uint32_t bil_rows;       // rows per bilinear walker strip (even)
+    // edge-aware demosaic: tap lists, canonical position of frame sample (0, 0), smoothing weight
+    // (the border width is ahd_halo)
+    uint32_t edge_aware;
+    const EdgeTab *etab;
+    uint32_t ecanon_bits;  // colour of canonical position i in bits 2i..2i+1
+    uint32_t ecx, ecy;
+    float ebeta;
 };
 
 // Rust `x as u32` for x below 2^32: truncation, with NaN and everything below 1 giving 0.

@@ -43,6 +43,8 @@ struct frh_context {
     uint32_t bil_rows = 32;  // rows per bilinear walker strip
     // strip heights found by frh_plan_autotune, by (AHD, developed width, rows, fast path)
     std::map<std::tuple<int, uint32_t, uint32_t, int>, uint32_t> tuned;
+    // device copies of edge-aware tap lists, by a hash of the tables they were built from
+    std::map<uint64_t, void *> edge_tabs;
     std::mutex mu;
     struct Block {
         void *ptr;

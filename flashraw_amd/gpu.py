@@ -356,7 +356,8 @@ class Orientation(enum.IntEnum):
 
 @dataclass
 class EdgeSpec:
-    """EdgeSpec (run.rs:141-160). The AHD form uses the pattern and the halo only."""
+    """EdgeSpec (run.rs:141-160). The AHD form uses the pattern and the halo only; the edge-aware
+    form (Bayer) also `g0` ([4][49]) and `dir` ([4][3][4][49]) and `beta`, with a halo of 8."""
     period: int = 2
     cx: int = 0
     cy: int = 0
@@ -366,6 +367,89 @@ class EdgeSpec:
     g0: Sequence[float] = ()
     dir: Sequence[float] = ()
     ahd: bool = False
+
+
+class NoiseReduction(enum.IntEnum):
+    """NoiseReduction (crates/flashraw/src/options.rs:341-353): the smoothing weight of the
+    edge-aware demosaic is the frame's estimate (Auto), zero (Off) or at least 0.25 / 0.5 / 0.8."""
+    Auto = 0
+    Off = 1
+    Low = 2
+    Medium = 3
+    High = 4
+
+
+_LEVEL_FLOOR = {NoiseReduction.Auto: None, NoiseReduction.Off: 0.0, NoiseReduction.Low: 0.25,
+                NoiseReduction.Medium: 0.5, NoiseReduction.High: 0.8}  # options.rs:70-78
+
+EDGE_CANON = (0, 1, 1, 2)  # canonical Bayer colours (R, G, G, B), edgeaware/mod.rs:165
+EDGE_HALO = 8              # edgeaware::HALO
+
+
+def edge_plan(info: FrameInfo, origin=(0, 0)) -> Optional[tuple]:
+    """Plan::new (edgeaware/mod.rs:204-230) for a Bayer frame: the canonical column and row
+    (cx, cy) of frame sample (0, 0), or None when the pattern is not Bayer."""
+    if info.cfa is None or not info.cfa.is_bayer() or info.cfa.width != 2 or info.cfa.height != 2:
+        return None
+    cols = tuple(info.cfa.colors)
+    for sy in range(2):
+        for sx in range(2):
+            if all(cols[y * 2 + x] == EDGE_CANON[((y + sy) % 2) * 2 + (x + sx) % 2] for y in range(2) for x in range(2)):
+                return (sx + origin[0]) % 2, (sy + origin[1]) % 2
+    return None
+
+
+def estimate_beta(width: int, height: int, g_gain: float, fetch, cx: int, cy: int) -> float:
+    """Plan::set_noise_from (edgeaware/mod.rs:369-409) for a Bayer plan: the smoothing weight
+    from the noise of 48 rows spread over the frame. `fetch(y)` returns row y of the normalised
+    mosaic (f32, `width` samples). Green pairs two samples apart with a mean in (0.03, 0.6) give
+    |a - b| / sqrt(mean); the lower quintile q gives sigma = q / (sqrt(2) * 0.2533) and
+    beta = 0.5 * clamp((sigma^2 - 2e-4) / 6e-4, 0, 1), all in f32."""
+    f32 = np.float32
+    if height < 16 or width < 16:
+        return 0.0
+    rows = 48
+    vals = []
+    for n in range(rows):
+        y = 4 + n * (height - 9) // (rows - 1)
+        row = np.asarray(fetch(y), dtype=f32).reshape(-1)[:width]
+        # green samples of row y: canon[((y + cy) % 2) * 2 + (x + cx) % 2] == 1, i.e. the column
+        # parity differs from the row parity; their green partners sit two samples to the right
+        xs = np.arange((y + cy + cx + 1) % 2, width - 2, 2)
+        a = row[xs] * f32(g_gain)
+        b = row[xs + 2] * f32(g_gain)
+        mid = f32(0.5) * (a + b)
+        ok = (mid > f32(0.03)) & (mid < f32(0.6))
+        vals.append(np.abs(a[ok] - b[ok]) / np.sqrt(mid[ok]))
+    v = np.concatenate(vals) if vals else np.zeros(0, f32)
+    if v.size < 1000:
+        return 0.0
+    k = v.size // 5
+    q = np.partition(v, k)[k]
+    sigma = f32(q) / (f32(np.sqrt(2.0)) * f32(0.2533))
+    nu = sigma * sigma
+    t = (nu - f32(2e-4)) / f32(6e-4)
+    return float(f32(0.5) * f32(min(max(t, f32(0.0)), f32(1.0))))
+
+
+def edge_spec(info: FrameInfo, origin, size, wb, fetch, tables, level=NoiseReduction.Auto) -> Optional[EdgeSpec]:
+    """flashraw::gpu::edge_spec (crates/flashraw/src/gpu.rs:115-128): the edge-aware spec of a
+    Bayer frame of `size` = (width, height) samples whose sample (0, 0) sits at sensor `origin`,
+    with the smoothing weight estimated from the frame (`fetch(y)` = normalised row y, green gain
+    wb[1]) and then raised to the floor of `level` (Plan::apply_level, mod.rs:413-419).
+    `tables` = (g0 [4][49], dir [4][3][4][49]) are the filter tables of the canonical pattern
+    (R, G, G, B). None for patterns other than Bayer."""
+    plan = edge_plan(info, origin)
+    if plan is None:
+        return None
+    cx, cy = plan
+    beta = estimate_beta(int(size[0]), int(size[1]), float(wb[1]), fetch, cx, cy)
+    floor = _LEVEL_FLOOR[NoiseReduction(level)]
+    if floor is not None:
+        beta = 0.0 if floor == 0.0 else max(beta, floor)
+    g0, dirt = tables
+    return EdgeSpec(period=2, cx=cx, cy=cy, beta=float(beta), halo=EDGE_HALO, canon=EDGE_CANON,
+                    g0=np.asarray(g0, np.float32), dir=np.asarray(dirt, np.float32), ahd=False)
 
 
 def ahd_spec(info: FrameInfo, origin=(0, 0)) -> Optional[EdgeSpec]:
@@ -422,6 +506,14 @@ class DevelopSpec:
                 c = np.ascontiguousarray(self.edge.canon, dtype=np.uint8)
                 keep.append(c)
                 e.canon, e.canon_len = c.ctypes.data, c.size
+            if len(self.edge.g0):
+                g0 = np.ascontiguousarray(self.edge.g0, dtype=np.float32).reshape(-1)
+                keep.append(g0)
+                e.g0, e.g0_len = g0.ctypes.data, g0.size
+            if len(self.edge.dir):
+                dr = np.ascontiguousarray(self.edge.dir, dtype=np.float32).reshape(-1)
+                keep.append(dr)
+                e.dir, e.dir_len = dr.ctypes.data, dr.size
             keep.append(e)
             s.edge = ctypes.pointer(e)
         return s, keep

@@ -62,3 +62,43 @@ def frame_24mp_noisy(seed=2, **kw):
 # matrix with a diagonal near 1.6 and off-diagonals in [-0.5, -0.1].
 WB = (2.1, 1.0, 1.6, 1.0)
 MATRIX = ((1.62, -0.38, -0.24), (-0.21, 1.58, -0.37), (-0.12, -0.45, 1.57))
+
+
+def edge_tables(seed=0):
+    """Synthetic edge-aware filter tables for a Bayer pattern with the canonical colours
+    (R, G, G, B), in the layout the edge-aware spec takes: g0 [4][49] and dir [4][3][4][49].
+
+    They have the sparsity of trained Bayer tables, which sets the kernel's cost: the first green
+    estimate is the sample itself at the green positions and a full 7x7 filter at red and blue;
+    the directional filters are full 7x7 for the two missing colours and zero for the own colour.
+    Every filter sums to one (a flat field stays flat) and leans along its direction, so the
+    result is a plausible demosaic; the coefficients are not trained."""
+    rng = np.random.default_rng(seed)
+    canon = (0, 1, 1, 2)
+    yy, xx = np.mgrid[-3:4, -3:4]
+    along = [(0, 1), (1, 0), (1, 1), (1, -1)]  # horizontal, vertical, and the two diagonals
+
+    def filt(d=None):
+        if d is None:
+            f = np.exp(-(xx ** 2 + yy ** 2) / 3.0)
+        else:
+            u = (xx * d[0] + yy * d[1]) / np.hypot(*d)     # distance across the direction
+            v = (xx * d[1] - yy * d[0]) / np.hypot(*d)     # distance along it
+            f = np.exp(-(u ** 2) / 0.6 - (v ** 2) / 4.0)
+        f = f * rng.uniform(0.8, 1.2, f.shape)
+        return (f / f.sum()).reshape(-1).astype(np.float32)
+
+    g0 = np.zeros((4, 49), np.float32)
+    dirt = np.zeros((4, 3, 4, 49), np.float32)
+    for ph in range(4):
+        if canon[ph] == 1:
+            g0[ph, 24] = 1.0
+        else:
+            g0[ph] = filt()
+        for c in range(3):
+            if c == canon[ph]:
+                continue
+            for d in range(4):
+                dirt[ph, c, d] = filt(along[d])
+    return g0, dirt
+

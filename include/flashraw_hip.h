@@ -85,8 +85,12 @@ enum frh_develop_output {
     FRH_OUTPUT_NONE = -1
 };
 
-/* EdgeSpec (run.rs:141-160). Only the AHD form (`ahd` = 1, period 2, beta 0) is accepted; the
- * filter tables of the edge-aware form are not used by it and may be NULL. */
+/* EdgeSpec (run.rs:141-160), Bayer (period 2) only. The AHD form (`ahd` = 1, beta 0) uses the
+ * pattern and the halo; its filter tables may be NULL. The edge-aware form (`ahd` = 0) needs
+ * canon[4], g0[4*49] (7x7 filter of the first green estimate per canonical position) and
+ * dir[4*3*4*49] (7x7 filters per position, colour and direction), beta in [0, 1] and a halo of
+ * at least 8; the device copy of its tap lists is built once per distinct table contents and
+ * kept by the context. Other periods (X-Trans) return FRH_ERROR_NOT_AVAILABLE. */
 typedef struct frh_edge_spec {
     uint32_t size;
     uint32_t period, cx, cy;
