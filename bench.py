@@ -30,7 +30,9 @@ def _parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=200)
     p.add_argument("--warmup", type=int, default=20)
-    p.add_argument("--demosaic", default="ahd", choices=["ahd", "bilinear"])
+    p.add_argument("--demosaic", default="ahd", choices=["ahd", "bilinear", "edge"])
+    p.add_argument("--beta", type=float, default=0.0,
+                   help="edge demosaic: weight of the gated smoothing (0 for clean frames)")
     p.add_argument("--mode", default="develop", choices=["develop", "denoise", "batch", "serve", "copy"])
     p.add_argument("--frames", type=int, default=64, help="batch mode: frames in the batch")
     p.add_argument("--rates", default="10,50,100,200", help="serve mode: request rates (req/s) to sweep")
@@ -455,7 +457,15 @@ def main():
     W, H = 6048, 4024
     mos = synth.frame_24mp()
     info = F.FrameInfo(W, H, F.CfaPattern.RGGB, F.BlackLevel.uniform(512.0), F.WhiteLevel.uniform(16383))
-    edge = F.ahd_spec(info) if args.demosaic == "ahd" else None
+    tables = synth.edge_tables(0)  # full 7x7 filters, the sparsity of trained Bayer tables
+    if args.demosaic == "ahd":
+        edge = F.ahd_spec(info)
+    elif args.demosaic == "edge":
+        cx, cy = F.edge_plan(info)
+        edge = F.EdgeSpec(period=2, cx=cx, cy=cy, beta=args.beta, halo=8, canon=F.EDGE_CANON, g0=tables[0],
+                          dir=tables[1])
+    else:
+        edge = None
     spec = F.DevelopSpec(F.Rect.full(W, H), wb=synth.WB, cam_to_rgb=synth.MATRIX, output=F.DevelopOutput.Rgba8,
                          edge=edge)
     d_in = torch.from_numpy(mos.view(np.int16)).to("cuda")
@@ -498,7 +508,9 @@ def main():
         for i in range(6):
             t = time.perf_counter()
             want = O.develop(mos, "RGGB", black=(1, 1, 1, [512.0]), white=(16383,), wb=synth.WB,
-                             matrix=synth.MATRIX, demosaic=args.demosaic, encoding="rgba8", threads=threads)
+                             matrix=synth.MATRIX, demosaic=args.demosaic, encoding="rgba8", threads=threads,
+                             edge=None if edge is None or edge.ahd else
+                             dict(g0=tables[0], dir=tables[1], cx=edge.cx, cy=edge.cy, beta=args.beta))
             if i:
                 times.append(time.perf_counter() - t)
         cpu_s = statistics.median(times)
@@ -506,7 +518,7 @@ def main():
         cpu = {"value": round(mp / cpu_s, 2), "unit": "MP/s", "cores": threads, "kind": "port",
                "sample": f"the same 6048x4024 frame, median of 5 runs after 1 warm-up, "
                          f"{args.demosaic} RGBA8, C restatement of the reference CPU path"}
-    tag = f"develop_{args.demosaic}_6048x4024_rgba8"
+    tag = f"develop_{args.demosaic}_6048x4024_rgba8" + (f"_beta{args.beta:g}" if args.demosaic == "edge" else "")
     line = {
         "metric": "MP/s RAW->sRGB develop (24 MP AHD, % HBM roofline)",
         "value": round(value, 1),
@@ -520,7 +532,9 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"6048x4024 RGGB 14-bit u16 -> RGBA8, {args.demosaic} demosaic, resident in HBM",
+        "config": {"workload": f"6048x4024 RGGB 14-bit u16 -> RGBA8, {args.demosaic} demosaic"
+                               + (f" (full 7x7 tables, beta {args.beta:g})" if args.demosaic == "edge" else "")
+                               + ", resident in HBM",
                    "frame_mp": round(mp, 3), "demosaic": args.demosaic, "output": "RGBA8",
                    "max_abs_diff_vs_oracle_lsb": max_diff, "strip_rows": rows, "strip_rows_autotuned": not args.strip_rows},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

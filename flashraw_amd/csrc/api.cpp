@@ -4,6 +4,7 @@
 // validation 439-646, bands 650-662, timings 227-239) and image.rs (GpuImage).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -286,12 +287,14 @@ static int32_t edge_tables(frh_context *ctx, const frh_edge_spec *e, const EdgeT
                 t.g0_n[ph]++;
             }
         }
+        for (uint32_t i = 0; i < 49; i++) t.g0d[ph][i] = e->g0[ph * 49 + i];
         for (int c = 0; c < 3; c++)
             for (uint32_t i = 0; i < 49; i++) {
                 float v[4];
                 bool any = false;
                 for (int d = 0; d < 4; d++) {
                     v[d] = e->dir[((ph * 3 + c) * 4 + d) * 49 + i];
+                    t.dird[ph][c][i][d] = v[d];
                     any = any || v[d] != 0.0f;
                 }
                 if (!any) continue;
@@ -310,6 +313,29 @@ static int32_t edge_tables(frh_context *ctx, const frh_edge_spec *e, const EdgeT
     ctx->edge_tabs[hsh] = d;
     *out = static_cast<const EdgeTab *>(d);
     return FRH_OK;
+}
+
+// True when the inputs bound every intermediate value of the edge-aware demosaic: with
+// |balanced sample| <= M and |tap| <= C, a 49-tap sum is at most 49 M C <= 1e15, so sqrt(G0) <=
+// 3.2e7, the gradient energies E <= 4e16, (E + eps)^2 <= 2e33 stays finite and 1 / that stays
+// normal, the colour planes stay below 1e16 and the gate terms (d * inv)^2 below 1e36. Every value
+// the kernel multiplies by a zero tap or a zero colour mask is then finite, so skipping those
+// products (or multiplying them) leaves every sum unchanged.
+static bool edge_tame(const DevArgs &a, const frh_frame_info *info, const frh_edge_spec *e) {
+    if (!a.bounded) return false;
+    double bmax = 0.0, smax = 0.0, wmax = 1.0, cmax = 0.0;
+    for (uint32_t i = 0; i < a.cal_pw * a.cal_ph; i++) {
+        bmax = std::max(bmax, (double)std::fabs(a.black[i]));
+        smax = std::max(smax, (double)std::fabs(a.scale[i]));
+    }
+    double rmax = 0.0, cdmax = 0.0;
+    for (uint32_t i = 0; i < a.n_rowd; i++) rmax = std::max(rmax, (double)std::fabs(info->black_row_deltas[i]));
+    for (uint32_t i = 0; i < a.n_cold; i++) cdmax = std::max(cdmax, (double)std::fabs(info->black_col_deltas[i]));
+    for (int i = 0; i < 3; i++) wmax = std::max(wmax, (double)std::fabs(a.wb[i]));
+    for (uint32_t i = 0; i < 4 * 49; i++) cmax = std::max(cmax, (double)std::fabs(e->g0[i]));
+    for (uint32_t i = 0; i < 4 * 3 * 4 * 49; i++) cmax = std::max(cmax, (double)std::fabs(e->dir[i]));
+    const double m = (65535.0 + bmax + rmax + cdmax) * smax * wmax * 1.001;
+    return 49.0 * m * cmax <= 1e15;
 }
 
 int32_t frh::plan_develop(const frh_context *ctx, const frh_frame *f, const frh_frame_info *info,
@@ -495,6 +521,7 @@ int32_t frh::plan_develop(const frh_context *ctx, const frh_frame *f, const frh_
             a.ahd_halo = e->halo;
             a.ecanon_bits = 0;
             for (int i = 0; i < 4; i++) a.ecanon_bits |= (uint32_t)(e->canon[i] & 3u) << (2 * i);
+            a.edge_tame = edge_tame(a, info, e) ? 1u : 0u;
         }
     }
     int32_t modes[2] = {spec->output, spec->output2};
@@ -654,6 +681,10 @@ extern "C" int32_t frh_plan_autotune(frh_plan *plan, const void *src, void *dst,
         if (!plan || !src || !dst) return fail(FRH_ERROR_INVALID_ARGUMENT, "NULL argument");
         if (plan->p.a.out_mode2 != OUT_NONE && !dst2) return fail(FRH_ERROR_INVALID_ARGUMENT, "second output is NULL");
         if (iters < 1) iters = 1;
+        if (plan->p.a.edge_aware) {  // one fixed tile shape: nothing to tune
+            if (best) *best = 0;
+            return FRH_OK;
+        }
         frh_context *ctx = plan->ctx;
         hipStream_t st = (hipStream_t)stream;
         const bool ahd = plan->p.ahd;

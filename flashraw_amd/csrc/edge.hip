@@ -270,17 +270,290 @@ __global__ __launch_bounds__(256) void k_develop_edge(DevArgs a, Border B, int t
     }
 }
 
+// ---- the kernel for tame inputs (edge_tame) ----
+//
+// Same stages and operations, arranged for throughput: every filter is a fully unrolled 7x7
+// window held in registers with the coefficients read as scalars from the dense tables (a zero tap
+// multiplies a finite value and adds +-0, which changes no sum), the direction weights are
+// computed where the colour filters need them (no weight planes), and pass 1 reads the colours of
+// the 3x3 neighbours from the phase, which is uniform per loop, so that the zero terms of its
+// masked sums are skipped. The tile is 58 x 26 so that each phase of the 64 x 32 colour-plane
+// region holds exactly 512 pixels: one per thread.
+template <int TW, int TH>
+struct EdgeGeom {
+    static constexpr int MW = TW + 16, MH = TH + 16;  // balanced mosaic, origin (X0 - 8, Y0 - 8)
+    static constexpr int SW = TW + 10, SH = TH + 10;  // sqrt(G0), origin - 5
+    static constexpr int DW = TW + 8, DH = TH + 8;    // squared gradients, origin - 4
+    static constexpr int PW = TW + 6, PH = TH + 6;    // colour planes, origin - 3
+    static constexpr int QW = TW + 4, QH = TH + 4;    // pass-1 differences, origin - 2
+    static constexpr int VW = TW + 2, VH = TH + 2;    // pass-2 colours, origin - 1
+};
+
+template <int TW, int TH>
+struct EdgeLds2 {
+    using G = EdgeGeom<TW, TH>;
+    union {
+        float m[G::MH * G::MW];
+        float d1[2][G::QH * G::QW];
+    } a;
+    union {
+        float sg[G::SH * G::SW];
+        float p[3][G::PH * G::PW];
+    } b;
+    union {
+        float dsq[4][G::DH * G::DW];
+        float v[3][G::VH * G::VW];
+    } c;
+};
+
+template <int TW, int TH, int NT>
+__global__ __launch_bounds__(NT) void k_develop_edge2(DevArgs a, Border B, int tiles_x, uint32_t tiles) {
+    using G = EdgeGeom<TW, TH>;
+    __shared__ Lds s;
+    __shared__ EdgeLds2<TW, TH> E;
+    load_lds(a, s);
+    if (blockIdx.x >= tiles) {
+        __syncthreads();
+        const uint32_t i = (blockIdx.x - tiles) * blockDim.x + threadIdx.x;
+        if (i < B.count) border_pixel(a, s, B, i);
+        return;
+    }
+    const ctab_t *T = (const ctab_t *)(uintptr_t)a.etab;
+    const int tid = threadIdx.x;
+    const int ix0 = B.ix0 + (int)a.crop_x, ix1 = B.ix1 + (int)a.crop_x;
+    const int iy0 = B.iy0 + (int)a.crop_y, iy1 = B.iy1 + (int)a.crop_y;
+    const int X0 = ix0 + (int)(blockIdx.x % (uint32_t)tiles_x) * TW;
+    const int Y0 = iy0 + (int)(blockIdx.x / (uint32_t)tiles_x) * TH;
+    const int W = (int)a.width, H = (int)a.height;
+    __syncthreads();
+    // 1. balanced mosaic min(norm * wb[colour], 1), clamped reads beyond the frame
+    for (int i = tid; i < G::MH * G::MW; i += NT) {
+        const int y = min(max(Y0 - 8 + i / G::MW, 0), H - 1), x = min(max(X0 - 8 + i % G::MW, 0), W - 1);
+        const uint32_t c = canon_of(a, edge_phase(a, x, y));
+        const float wbv = c == 0 ? a.wb[0] : (c == 1 ? a.wb[1] : a.wb[2]);
+        E.a.m[i] = fminf(norm_at(a, s, (uint32_t)x, (uint32_t)y) * wbv, 1.0f);
+    }
+    __syncthreads();
+    // 2. sqrt(G0) over +-5, phase by phase: the dense 7x7 window, or the few taps of an
+    //    identity-like filter
+#pragma unroll 1
+    for (int k = 0; k < 4; k++) {
+        int fr, fc, nr, nc;
+        phase_grid(a, k, X0 - 5, Y0 - 5, G::SW, G::SH, fr, fc, nr, nc);
+        const int n = T->g0_n[k];
+        if (n > 8) {
+            for (int i = tid; i < nr * nc; i += NT) {
+                const int ry = fr + 2 * (i / nc), rx = fc + 2 * (i % nc);
+                const float *mp = &E.a.m[ry * G::MW + rx];
+                float acc = 0.0f;
+#pragma unroll
+                for (int t = 0; t < 49; t++) acc += T->g0d[k][t] * mp[(t / 7) * G::MW + t % 7];
+                E.b.sg[ry * G::SW + rx] = sqrt_max0(acc);
+            }
+        } else {
+            for (int i = tid; i < nr * nc; i += NT) {
+                const int ry = fr + 2 * (i / nc), rx = fc + 2 * (i % nc);
+                const float *mp = &E.a.m[ry * G::MW + rx];
+                float acc = 0.0f;
+                for (int t = 0; t < n; t++) {
+                    const uint32_t idx = T->g0_idx[k][t];
+                    acc += T->g0_c[k][t] * mp[(idx / 7u) * G::MW + idx % 7u];
+                }
+                E.b.sg[ry * G::SW + rx] = sqrt_max0(acc);
+            }
+        }
+    }
+    __syncthreads();
+    // 3. squared gradients of sqrt(G0) over +-4
+    const float r2 = 0.70710677f;
+    for (int i = tid; i < G::DH * G::DW; i += NT) {
+        const int y = i / G::DW + 1, x = i % G::DW + 1;
+        const float *g = E.b.sg;
+        const float dh = g[y * G::SW + x + 1] - g[y * G::SW + x - 1];
+        const float dv = g[(y + 1) * G::SW + x] - g[(y - 1) * G::SW + x];
+        const float e1 = (g[(y + 1) * G::SW + x + 1] - g[(y - 1) * G::SW + x - 1]) * r2;
+        const float e2 = (g[(y + 1) * G::SW + x - 1] - g[(y - 1) * G::SW + x + 1]) * r2;
+        E.c.dsq[0][i] = dh * dh;
+        E.c.dsq[1][i] = dv * dv;
+        E.c.dsq[2][i] = e1 * e1;
+        E.c.dsq[3][i] = e2 * e2;
+    }
+    __syncthreads();
+    // 4-5. per pixel of the +-3 region, phase by phase: the direction weights from the 3x3 sums of
+    //      the squared gradients, then the two missing colours from the 49 samples of the window
+#pragma unroll 1
+    for (int k = 0; k < 4; k++) {
+        const uint32_t own = canon_of(a, k);
+        int fr, fc, nr, nc;
+        phase_grid(a, k, X0 - 3, Y0 - 3, G::PW, G::PH, fr, fc, nr, nc);
+        for (int i = tid; i < nr * nc; i += NT) {
+            const int ry = fr + 2 * (i / nc), rx = fc + 2 * (i % nc);
+            float w[4];
+#pragma unroll
+            for (int d = 0; d < 4; d++) {
+                const float *q = &E.c.dsq[d][ry * G::DW + rx];
+                const float h0 = q[0] + q[1] + q[2];
+                const float h1 = q[G::DW] + q[G::DW + 1] + q[G::DW + 2];
+                const float h2 = q[2 * G::DW] + q[2 * G::DW + 1] + q[2 * G::DW + 2];
+                const float u = h0 + h1 + h2 + 1e-3f;
+                w[d] = 1.0f / (u * u);
+            }
+            const float inv = 1.0f / (w[0] + w[1] + w[2] + w[3]);
+            const float w0 = w[0] * inv, w1 = w[1] * inv, w2 = w[2] * inv, w3 = w[3] * inv;
+            const float *mp = &E.a.m[(ry + 2) * G::MW + rx + 2];
+            float win[49];
+#pragma unroll
+            for (int t = 0; t < 49; t++) win[t] = mp[(t / 7) * G::MW + t % 7];
+            const int o = ry * G::PW + rx;
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                if ((uint32_t)c == own) {
+                    E.b.p[c][o] = win[24];
+                    continue;
+                }
+                float acc0 = 0.0f, acc1 = 0.0f, acc2 = 0.0f, acc3 = 0.0f;
+#pragma unroll
+                for (int t = 0; t < 49; t++) {
+                    acc0 += T->dird[k][c][t][0] * win[t];
+                    acc1 += T->dird[k][c][t][1] * win[t];
+                    acc2 += T->dird[k][c][t][2] * win[t];
+                    acc3 += T->dird[k][c][t][3] * win[t];
+                }
+                E.b.p[c][o] = w0 * acc0 + w1 * acc1 + w2 * acc2 + w3 * acc3;
+            }
+        }
+    }
+    __syncthreads();
+    // 6. pass 1 over +-2, phase by phase: the neighbours' colours are uniform, and a neighbour of
+    //    the other colours adds exact zeros to the masked sums (mod.rs:476-530)
+#pragma unroll 1
+    for (int k = 0; k < 4; k++) {
+        const uint32_t own = canon_of(a, k);
+        int fr, fc, nr, nc;
+        phase_grid(a, k, X0 - 2, Y0 - 2, G::QW, G::QH, fr, fc, nr, nc);
+        for (int i = tid; i < nr * nc; i += NT) {
+            const int qy = fr + 2 * (i / nc), qx = fc + 2 * (i % nc);  // in the pass-1 plane
+            const int y = qy + 1, x = qx + 1;                          // in the colour planes
+            const float g = E.b.p[1][y * G::PW + x];
+            const float inv = 1.0f / (0.2f * fmaxf(g, 0.0f) + 0.015f);
+            float sw0 = 0.0f, sd0 = 0.0f, su0 = 0.0f, n0 = 0.0f, sw1 = 0.0f, sd1 = 0.0f, su1 = 0.0f, n1 = 0.0f;
+#pragma unroll
+            for (int dy = -1; dy <= 1; dy++)
+#pragma unroll
+                for (int dx = -1; dx <= 1; dx++) {
+                    const uint32_t col = canon_of(a, k ^ (((dy & 1) << 1) | (dx & 1)));
+                    if (col == 1) continue;
+                    const int q = (y + dy) * G::PW + x + dx;
+                    const float gk = E.b.p[1][q];
+                    const float e = (gk - g) * inv;
+                    const float wt = 1.0f / (1.0f + e * e);
+                    if (col == 0) {
+                        const float d0 = E.b.p[0][q] - gk;
+                        sw0 += wt;
+                        sd0 += wt * d0;
+                        su0 += d0;
+                        n0 += 1.0f;
+                    } else {
+                        const float d2 = E.b.p[2][q] - gk;
+                        sw1 += wt;
+                        sd1 += wt * d2;
+                        su1 += d2;
+                        n1 += 1.0f;
+                    }
+                }
+            const int o = qy * G::QW + qx;
+            float r0, r1;
+            if (own == 0)
+                r0 = E.b.p[0][y * G::PW + x] - g;
+            else
+                r0 = sw0 > 1e-12f ? sd0 / sw0 : su0 / n0;
+            if (own == 2)
+                r1 = E.b.p[2][y * G::PW + x] - g;
+            else
+                r1 = sw1 > 1e-12f ? sd1 / sw1 : su1 / n1;
+            E.a.d1[0][o] = r0;
+            E.a.d1[1][o] = r1;
+        }
+    }
+    __syncthreads();
+    // 7. pass 2 over +-1 (mod.rs:534-598)
+    for (int i = tid; i < G::VH * G::VW; i += NT) {
+        const int y = i / G::VW + 1, x = i % G::VW + 1;  // in the pass-1 plane
+        const int py = y + 1, px = x + 1;                // in the colour planes
+        const float g = E.b.p[1][py * G::PW + px];
+        const float inv = 1.0f / (0.1f * fmaxf(g, 0.0f) + 0.007f);
+        float sw = 0.0f, sdr = 0.0f, sdb = 0.0f;
+#pragma unroll
+        for (int dy = -1; dy <= 1; dy++)
+#pragma unroll
+            for (int dx = -1; dx <= 1; dx++) {
+                const float gk = E.b.p[1][(py + dy) * G::PW + px + dx];
+                const float e = (gk - g) * inv;
+                const float wt = 1.0f / (1.0f + e * e);
+                const int q = (y + dy) * G::QW + x + dx;
+                sw += wt;
+                sdr += wt * E.a.d1[0][q];
+                sdb += wt * E.a.d1[1][q];
+            }
+        const float dr = sdr / sw, db = sdb / sw;
+        const uint32_t own = canon_of(a, edge_phase(a, X0 - 1 + x - 1, Y0 - 1 + y - 1));
+        E.c.v[0][i] = own == 0 ? E.b.p[0][py * G::PW + px] : g + dr;
+        E.c.v[1][i] = g;
+        E.c.v[2][i] = own == 2 ? E.b.p[2][py * G::PW + px] : g + db;
+    }
+    __syncthreads();
+    // 8. gated smoothing (beta > 0, mod.rs:803-822), finish with unit balance, encode, store
+    const int xe = min(X0 + TW, ix1), ye = min(Y0 + TH, iy1);
+    for (int i = tid; i < TW * TH; i += NT) {
+        const int ty = i / TW, tx = i % TW;
+        const int fx = X0 + tx, fy = Y0 + ty;
+        if (fx >= xe || fy >= ye) continue;
+        const int c = (ty + 1) * G::VW + tx + 1;
+        float r = E.c.v[0][c], g = E.c.v[1][c], b = E.c.v[2][c];
+        if (a.ebeta > 0.0f) {
+            const float inv = 1.0f / (0.2f * fmaxf(g, 0.0f) + 0.015f);
+            float sw = 0.0f, s0 = 0.0f, s1 = 0.0f, s2 = 0.0f;
+#pragma unroll
+            for (int dy = -1; dy <= 1; dy++)
+#pragma unroll
+                for (int dx = -1; dx <= 1; dx++) {
+                    const int q = c + dy * G::VW + dx;
+                    const float e = (E.c.v[1][q] - g) * inv;
+                    const float wt = 1.0f / (1.0f + e * e);
+                    sw += wt;
+                    s0 += wt * E.c.v[0][q];
+                    s1 += wt * E.c.v[1][q];
+                    s2 += wt * E.c.v[2][q];
+                }
+            r = r + a.ebeta * (s0 / sw - r);
+            const float g2 = g + a.ebeta * (s1 / sw - g);
+            b = b + a.ebeta * (s2 / sw - b);
+            g = g2;
+        }
+        float o0, o1, o2;
+        finish(a, r, g, b, true, o0, o1, o2);
+        store_both(a, s, (uint32_t)(fx - (int)a.crop_x), (uint32_t)(fy - (int)a.crop_y), o0, o1, o2);
+    }
+}
+
+constexpr int E2W = 58, E2H = 26, E2T = 512;
+static_assert(((E2W + 6) / 2) * ((E2H + 6) / 2) == E2T, "one colour-plane pixel per thread and phase");
+
 }  // namespace
 
 hipError_t launch_develop_edge(const DevArgs &a, hipStream_t stream) {
     const Border B = make_border(a, (int)a.ahd_halo);
     const int iw = B.ix1 - B.ix0, ih = B.iy1 - B.iy0;
-    const int tiles_x = iw > 0 ? (iw + ETW - 1) / ETW : 0;
-    const uint32_t tiles = (iw > 0 && ih > 0) ? (uint32_t)(tiles_x * ((ih + ETH - 1) / ETH)) : 0u;
-    const uint32_t border_blocks = (B.count + 255) / 256;
+    const int tw = a.edge_tame ? E2W : ETW, th = a.edge_tame ? E2H : ETH, nt = a.edge_tame ? E2T : 256;
+    const int tiles_x = iw > 0 ? (iw + tw - 1) / tw : 0;
+    const uint32_t tiles = (iw > 0 && ih > 0) ? (uint32_t)(tiles_x * ((ih + th - 1) / th)) : 0u;
+    const uint32_t border_blocks = (B.count + nt - 1) / nt;
     const uint32_t grid = tiles + border_blocks;
     if (grid == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_develop_edge<false>, dim3(grid), dim3(256), 0, stream, a, B, tiles_x, tiles);
+    if (a.edge_tame)
+        hipLaunchKernelGGL((k_develop_edge2<E2W, E2H, E2T>), dim3(grid), dim3(nt), 0, stream, a, B, tiles_x, tiles);
+    else
+        hipLaunchKernelGGL(k_develop_edge<false>, dim3(grid), dim3(nt), 0, stream, a, B, tiles_x, tiles);
     return hipGetLastError();
 }
 

@@ -23,8 +23,13 @@ enum OutMode : int32_t {
 
 // Tap lists of the edge-aware filters (edgeaware/mod.rs:130-160) for a Bayer pattern: per
 // canonical position, the taps of G0 whose coefficient is not zero and, per colour, the taps where
-// some direction is not zero (with all four directions' coefficients), in tap order.
+// some direction is not zero (with all four directions' coefficients), in tap order. The dense
+// copies (zero taps included) serve the kernel that multiplies every tap, which gives the same
+// bits when every value it multiplies is finite (a zero tap then adds +-0 to a sum that is never
+// -0).
 struct EdgeTab {
+    float g0d[4][49];
+    float dird[4][3][49][4];
     uint32_t g0_n[4];
     uint32_t dir_n[4][3];
     uint32_t g0_idx[4][49];
@@ -88,6 +93,9 @@ struct DevArgs {
     uint32_t ecanon_bits;  // colour of canonical position i in bits 2i..2i+1
     uint32_t ecx, ecy;
     float ebeta;
+    // The inputs bound every intermediate value of the edge-aware demosaic far from overflow
+    // (see edge_tables in api.cpp): dense taps and uniform colour masks give the same bits.
+    uint32_t edge_tame;
 };
 
 // Rust `x as u32` for x below 2^32: truncation, with NaN and everything below 1 giving 0.

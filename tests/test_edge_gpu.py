@@ -223,3 +223,21 @@ def test_24mp_full_frame(ctx):
                      encoding="rgba8")
     assert np.array_equal(got, want), np.count_nonzero(got != want)
     print(f"24 MP edge-aware: kernel {run.timings.kernel_ms:.3f} ms")
+
+
+def test_untamed_inputs(ctx):
+    """Inputs that do not bound the intermediate values (a white balance beyond 1e6, a tap of
+    1e13) take the kernel that follows the tap lists and the masked sums literally; it matches the
+    oracle bit for bit too."""
+    mos = _mosaic(150, 97, "GBRG", seed=8)
+    g0, d = _tables("dense", seed=6)
+    _compare(ctx, mos, "GBRG", tables=(g0, d), beta=0.3, output=F.DevelopOutput.LinearRgbF32,
+             wb=(2.5e6, 1.0, 1.5, 1.0), what="huge wb")
+    g0b, db = g0.copy(), d.copy()
+    g0b[0, 3] = 1e13
+    db[1, 0, 2, 10] = -1e13
+    for out in (F.DevelopOutput.LinearRgbF32, F.DevelopOutput.Rgba8):
+        _compare(ctx, mos, "GBRG", tables=(g0b, db), beta=0.0, output=out, what="huge taps")
+    sparse = _tables("sparse", seed=9)
+    _compare(ctx, mos, "GBRG", tables=sparse, beta=0.1, output=F.DevelopOutput.LinearRgbF32,
+             black=(1, 1, 1, [2e6]), what="huge black")
