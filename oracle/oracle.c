@@ -1243,3 +1243,65 @@ int fro_dct_denoise(const uint16_t *mosaic, uint32_t width, uint32_t height, con
     free(cnt);
     return 0;
 }
+
+/* ---- the edge-aware smoothing weight (crates/flashraw/src/edgeaware/mod.rs:369-419) ---- */
+
+int fro_normalize(const fro_frame *f, float *out) {
+    fro_params p;
+    memset(&p, 0, sizeof(p));
+    p.demosaic = FRO_BILINEAR;
+    Kern k;
+    if (build_kernel(&k, f, &p)) {
+        free_kernel(&k);
+        return -1;
+    }
+    for (size_t y = 0; y < k.h; y++) normalize_row(&k, y, out + y * k.w);
+    free_kernel(&k);
+    return 0;
+}
+
+static int cmp_f32(const void *a, const void *b) {
+    float x = *(const float *)a, y = *(const float *)b;
+    return (x > y) - (x < y);
+}
+
+/* Plan::set_noise_from (mod.rs:369-409) over a normalised frame: 48 rows y = 4 + n (h - 9) / 47;
+ * per green sample x < w - 2 of a row and each green partner x + d (d = 1, 2) of the same row,
+ * with a = row[x] g_gain, b = row[x + d] g_gain and mid = 0.5 (a + b) in (0.03, 0.6), the value
+ * |a - b| / sqrt(mid); fewer than 1000 values give 0. Otherwise the value of rank len / 5 (in the
+ * total order; every value here is finite and non-negative) is q, sigma = q / (sqrt(2) 0.2533),
+ * nu = sigma^2 and beta = 0.5 clamp((nu - 2e-4) / 6e-4, 0, 1). */
+float fro_edge_beta(const float *norm, uint32_t w, uint32_t h, const uint8_t canon[4], uint32_t cx, uint32_t cy,
+                    float g_gain) {
+    if (h < 16 || w < 16) return 0.0f;
+    const size_t rows = 48;
+    float *v = (float *)malloc(rows * (size_t)w * sizeof(float));
+    if (!v) return -1.0f;
+    size_t n = 0;
+    for (size_t r = 0; r < rows; r++) {
+        size_t y = 4 + r * (h - 9) / (rows - 1);
+        const float *row = norm + y * (size_t)w;
+        for (size_t x = 0; x + 2 < w; x++) {
+            if (canon[((y + cy) % 2) * 2 + (x + cx) % 2] != 1) continue;
+            for (size_t d = 1; d <= 2; d++) {
+                if (canon[((y + cy) % 2) * 2 + (x + d + cx) % 2] != 1) continue;
+                float a = row[x] * g_gain, b = row[x + d] * g_gain;
+                float mid = 0.5f * (a + b);
+                if (mid > 0.03f && mid < 0.6f) v[n++] = fabsf(a - b) / sqrtf(mid);
+            }
+        }
+    }
+    if (n < 1000) {
+        free(v);
+        return 0.0f;
+    }
+    qsort(v, n, sizeof(float), cmp_f32);
+    float q = v[n / 5];
+    free(v);
+    const float k = 1.41421356f * 0.2533f; /* std::f32::consts::SQRT_2 * 0.2533, in f32 */
+    float sigma = q / k;
+    float nu = sigma * sigma;
+    float t = (nu - 2e-4f) / 6e-4f;
+    t = t < 0.0f ? 0.0f : (t > 1.0f ? 1.0f : t);
+    return 0.5f * t;
+}

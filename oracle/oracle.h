@@ -84,6 +84,15 @@ void fro_srgb16_table(float out[16385]);
 /* IEEE half bits of x (process.rs:1978-1991). */
 uint16_t fro_f32_to_f16(float x);
 
+/* The normalised frame (Kernel::normalize_row, process.rs:881-945, for every row) into
+ * out[height][width]. Returns 0, or -1 for an invalid frame. */
+int fro_normalize(const fro_frame *f, float *out);
+/* Plan::set_noise_from (crates/flashraw/src/edgeaware/mod.rs:369-409) for a Bayer plan over the
+ * normalised frame `norm` (w x h): the smoothing weight of the edge-aware demosaic. canon[4] are
+ * the colours of the canonical pattern, (cx, cy) the canonical position of frame sample (0, 0). */
+float fro_edge_beta(const float *norm, uint32_t w, uint32_t h, const uint8_t canon[4], uint32_t cx, uint32_t cy,
+                    float g_gain);
+
 /* ---- CFA-plane DCT block denoiser (build-defined workload, no reference counterpart) ----
  * The mosaic is split into its four 2x2 phase planes (W/2 x H/2 each, black subtracted, in DN);
  * 8x8 blocks at stride `step` (positions include the last one) are transformed with the

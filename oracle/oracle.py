@@ -97,6 +97,11 @@ def lib():
         L.fro_srgb16_table.argtypes = [ctypes.c_void_p]
         L.fro_f32_to_f16.restype = ctypes.c_uint16
         L.fro_f32_to_f16.argtypes = [ctypes.c_float]
+        L.fro_normalize.restype = ctypes.c_int
+        L.fro_normalize.argtypes = [ctypes.POINTER(FroFrame), ctypes.c_void_p]
+        L.fro_edge_beta.restype = ctypes.c_float
+        L.fro_edge_beta.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p,
+                                    ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float]
         L.fro_dct_denoise.restype = ctypes.c_int
         L.fro_dct_denoise.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p,
                                       ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_uint32,
@@ -218,3 +223,21 @@ def dct_denoise(mosaic, black, sig_a, sig_b, thr=2.7, step=4, precision=64, thre
     if rc != 0:
         raise ValueError("oracle denoise rejected the request")
     return of, ou
+
+
+def normalize(mosaic, cfa, origin=(0, 0), black=(1, 1, 1, [0.0]), white=(16383,), row_deltas=None,
+              col_deltas=None):
+    """The normalised frame (normalize_row of every row, process.rs:881-945) as f32 (h, w)."""
+    f, keep = make_frame(mosaic, cfa, origin, black, white, row_deltas, col_deltas)
+    out = np.zeros(mosaic.shape, dtype=np.float32)
+    if lib().fro_normalize(ctypes.byref(f), out.ctypes.data) != 0:
+        raise ValueError("oracle rejected the frame")
+    return out
+
+
+def edge_beta(norm, cx, cy, g_gain, canon=(0, 1, 1, 2)):
+    """Plan::set_noise_from (edgeaware/mod.rs:369-409) over a normalised Bayer frame."""
+    norm = np.ascontiguousarray(norm, dtype=np.float32)
+    c = np.ascontiguousarray(canon, dtype=np.uint8)
+    h, w = norm.shape
+    return lib().fro_edge_beta(norm.ctypes.data, w, h, c.ctypes.data, cx, cy, g_gain)

@@ -241,3 +241,25 @@ def test_untamed_inputs(ctx):
     sparse = _tables("sparse", seed=9)
     _compare(ctx, mos, "GBRG", tables=sparse, beta=0.1, output=F.DevelopOutput.LinearRgbF32,
              black=(1, 1, 1, [2e6]), what="huge black")
+
+
+@pytest.mark.parametrize("cfa", PHASES)
+def test_edge_spec_end_to_end(ctx, cfa):
+    """edge_aware.rs:146-160: the spec of flashraw::gpu::edge_spec (smoothing weight estimated
+    from the normalised rows of the frame), then the GPU develop against the oracle with it."""
+    w, h = 203, 141
+    origin = (1, 3)
+    tables = _tables("dense", seed=3)
+    betas = []
+    for read_sigma in (0.5, 40.0):
+        mos = synth.mosaic(w, h, cfa, black=256.0, white=4095.0, gain=8.0, read_sigma=read_sigma, edges=30,
+                           discs=10, seed=11)
+        info = F.FrameInfo(w, h, F.CfaPattern.bayer(cfa), F.BlackLevel.uniform(256.0), F.WhiteLevel.uniform(4095))
+        norm = O.normalize(mos, cfa, origin=origin, black=(1, 1, 1, [256.0]), white=(4095,))
+        spec = F.edge_spec(info, origin, (w, h), (2.0, 1.0, 1.5, 1.0), lambda y: norm[y], tables)
+        assert (spec.cx, spec.cy) == F.edge_plan(info, origin)
+        betas.append(spec.beta)
+        for out in (F.DevelopOutput.LinearRgbF32, F.DevelopOutput.Rgba8):
+            _compare(ctx, mos, cfa, tables=tables, beta=spec.beta, output=out, origin=origin,
+                     what=f"{cfa} sigma {read_sigma} beta {spec.beta}")
+    assert betas[1] > betas[0]
