@@ -518,7 +518,7 @@ def main():
         cpu = {"value": round(mp / cpu_s, 2), "unit": "MP/s", "cores": threads, "kind": "port",
                "sample": f"the same 6048x4024 frame, median of 5 runs after 1 warm-up, "
                          f"{args.demosaic} RGBA8, C restatement of the reference CPU path"}
-    tag = f"develop_{args.demosaic}_6048x4024_rgba8" + (f"_beta{args.beta:g}" if args.demosaic == "edge" else "")
+    tag = f"develop_{args.demosaic}_6048x4024_rgba8" + (f"_beta{args.beta:g}" if args.beta else "")
     line = {
         "metric": "MP/s RAW->sRGB develop (24 MP AHD, % HBM roofline)",
         "value": round(value, 1),
