@@ -108,7 +108,7 @@ uint16_t fro_f32_to_f16(float x) {
 #define STRIP 16      /* process.rs:809 */
 
 typedef struct {
-    /* Calib of component 0 (process.rs:246-299) */
+    /* Calib of each component (process.rs:246-299): black[c * cal_pw * cal_ph + pos] */
     uint32_t cal_pw, cal_ph;
     float *black, *scale;
     const float *rowd, *cold;
@@ -124,7 +124,9 @@ typedef struct {
     int bayer;
     /* frame */
     const uint16_t *raw;
-    size_t w, h, ox, oy;
+    const float *rawf;
+    size_t w, h, ox, oy, cpp;
+    int has_cfa, half;
     /* params */
     float wb[4];
     float m[9];
@@ -153,31 +155,37 @@ static uint8_t cfa_colour(const Kern *k, size_t x, size_t y) {
 }
 
 /* Channel of frame sample (x, y) (process.rs:971-973). */
-static int colour_xy(const Kern *k, size_t x, size_t y) { return chan_of(cfa_colour(k, x + k->ox, y + k->oy)); }
+static int colour_xy(const Kern *k, size_t x, size_t y) {
+    return k->has_cfa ? chan_of(cfa_colour(k, x + k->ox, y + k->oy)) : 1;
+}
 
 static size_t gcd(size_t a, size_t b) { return b == 0 ? a : gcd(b, a % b); }
 static size_t lcm(size_t a, size_t b) { return a / gcd(a, b) * b; }
 
-/* Calib::new for component 0 (process.rs:256-286). */
+/* Calib::new for every component (process.rs:256-286). Without a CFA the pattern is 1x1 and
+ * the white level of component c is white[min(c, n - 1)]. */
 static int calib_new(Kern *k, const fro_frame *f) {
     size_t rows = f->black_rows ? f->black_rows : 1, cols = f->black_cols ? f->black_cols : 1;
     size_t samples = f->black_samples ? f->black_samples : 1;
-    size_t pw = zmin(lcm(f->cfa_w, cols), 64), ph = zmin(lcm(f->cfa_h, rows), 64);
+    size_t cw = f->cfa_w ? f->cfa_w : 1, ch = f->cfa_h ? f->cfa_h : 1;
+    size_t pw = zmin(lcm(cw, cols), 64), ph = zmin(lcm(ch, rows), 64);
     k->cal_pw = (uint32_t)pw;
     k->cal_ph = (uint32_t)ph;
-    k->black = (float *)malloc(pw * ph * sizeof(float));
-    k->scale = (float *)malloc(pw * ph * sizeof(float));
+    k->black = (float *)malloc(k->cpp * pw * ph * sizeof(float));
+    k->scale = (float *)malloc(k->cpp * pw * ph * sizeof(float));
     if (!k->black || !k->scale) return -1;
-    for (size_t y = 0; y < ph; y++) {
-        for (size_t x = 0; x < pw; x++) {
-            size_t idx = ((y % rows) * cols + x % cols) * samples + 0 % samples;
-            float bl = idx < f->n_black_values ? f->black_values[idx] : 0.0f;
-            size_t colour = f->cfa[(y % f->cfa_h) * f->cfa_w + x % f->cfa_w];
-            size_t wi = f->n_white > 1 ? zmin(colour, f->n_white - 1) : 0;
-            uint32_t wv = wi < f->n_white ? f->white_values[wi] : (f->n_white ? f->white_values[0] : 0);
-            float white = (float)wv;
-            k->black[y * pw + x] = bl;
-            k->scale[y * pw + x] = white > bl ? 1.0f / (white - bl) : 1.0f;
+    for (size_t c = 0; c < k->cpp; c++) {
+        for (size_t y = 0; y < ph; y++) {
+            for (size_t x = 0; x < pw; x++) {
+                size_t idx = ((y % rows) * cols + x % cols) * samples + c % samples;
+                float bl = idx < f->n_black_values ? f->black_values[idx] : 0.0f;
+                size_t colour = f->cfa_w ? f->cfa[(y % f->cfa_h) * f->cfa_w + x % f->cfa_w] : c;
+                size_t wi = f->n_white > 1 ? zmin(colour, f->n_white - 1) : 0;
+                uint32_t wv = wi < f->n_white ? f->white_values[wi] : (f->n_white ? f->white_values[0] : 0);
+                float white = (float)wv;
+                k->black[c * pw * ph + y * pw + x] = bl;
+                k->scale[c * pw * ph + y * pw + x] = white > bl ? 1.0f / (white - bl) : 1.0f;
+            }
         }
     }
     k->rowd = f->row_deltas;
@@ -187,42 +195,22 @@ static int calib_new(Kern *k, const fro_frame *f) {
     return 0;
 }
 
-/* Calib::norm (process.rs:288-298); x, y are sensor coordinates. */
-static inline float calib_norm(const Kern *k, float v, size_t x, size_t y) {
-    size_t i = (y % k->cal_ph) * k->cal_pw + x % k->cal_pw;
+/* Calib::norm (process.rs:288-298) of component c; x, y are sensor coordinates. */
+static inline float calib_norm(const Kern *k, size_t c, float v, size_t x, size_t y) {
+    size_t i = c * k->cal_pw * k->cal_ph + (y % k->cal_ph) * k->cal_pw + x % k->cal_pw;
     float b = k->black[i];
     if (k->n_rowd) b += (y < k->n_rowd) ? k->rowd[y] : 0.0f;
     if (k->n_cold) b += (x < k->n_cold) ? k->cold[x] : 0.0f;
     return (v - b) * k->scale[i];
 }
 
-/* Kernel::normalize_row for a single-component frame (process.rs:881-945): every branch
- * computes calib_norm's value for each sample. */
+/* Kernel::normalize_row (process.rs:881-945) into out[w * cpp]: each of its branches computes
+ * calib_norm's value for every sample, (f32(v) - black) * scale, u16 or f32 samples. */
 static void normalize_row(const Kern *k, size_t y, float *out) {
-    const uint16_t *restrict row = k->raw + y * k->w;
-    if (k->n_rowd || k->n_cold) {
-        for (size_t x = 0; x < k->w; x++) out[x] = calib_norm(k, (float)row[x], x + k->ox, y + k->oy);
-        return;
-    }
-    /* per-row black/scale over one pattern period, then a modulo-free loop (process.rs:918-944) */
-    size_t period = k->cal_pw, base = ((y + k->oy) % k->cal_ph) * k->cal_pw;
-    float blk[64], scl[64];
-    for (size_t i = 0; i < period; i++) {
-        blk[i] = k->black[base + (i + k->ox) % period];
-        scl[i] = k->scale[base + (i + k->ox) % period];
-    }
-    if (period == 2) {
-        float *restrict o = out;
-        for (size_t x = 0; x + 1 < k->w; x += 2) {
-            o[x] = ((float)row[x] - blk[0]) * scl[0];
-            o[x + 1] = ((float)row[x + 1] - blk[1]) * scl[1];
-        }
-        if (k->w & 1) o[k->w - 1] = ((float)row[k->w - 1] - blk[0]) * scl[0];
-        return;
-    }
-    for (size_t x = 0, j = 0; x < k->w; x++) {
-        out[x] = ((float)row[x] - blk[j]) * scl[j];
-        if (++j == period) j = 0;
+    const size_t n = k->w * k->cpp;
+    for (size_t i = 0; i < n; i++) {
+        float v = k->rawf ? k->rawf[y * n + i] : (float)k->raw[y * n + i];
+        out[i] = calib_norm(k, i % k->cpp, v, i / k->cpp + k->ox, y + k->oy);
     }
 }
 
@@ -246,6 +234,16 @@ static inline void finish_rgb(const float rgb[3], const float wb[4], const float
 static void row_cols(const Kern *k, const float *up, const float *mid, const float *dn, size_t oyy, float *out,
                      size_t i0, size_t i1) {
     size_t y = k->crop_y + oyy;
+    if (!(k->cpp == 1 && k->has_cfa)) {
+        /* no demosaic (process.rs:1038-1046): RGB components, or grey from component 0 */
+        for (size_t i = i0; i < i1; i++) {
+            size_t x = k->crop_x + i;
+            float rgb[3];
+            for (int c = 0; c < 3; c++) rgb[c] = mid[x * k->cpp + (k->cpp >= 3 ? (size_t)c : 0)];
+            finish_rgb(rgb, k->wb, k->m, k->has_levels, k->lscale, k->loffset, out + i * 3);
+        }
+        return;
+    }
     int interior_y = up && dn;
     const float *rows3[3] = {up, mid, dn};
     size_t py = (y + k->oy) % k->ph;
@@ -296,6 +294,46 @@ static void row_cols(const Kern *k, const float *up, const float *mid, const flo
                     rgb[c] = n > 0.0f ? s / n : 0.0f;
                 }
             }
+        }
+        finish_rgb(rgb, k->wb, k->m, k->has_levels, k->lscale, k->loffset, out + i * 3);
+    }
+}
+
+/* Kernel::row_cols, half size (process.rs:975-1036): output pixel i of row oyy averages, per
+ * colour, the samples of the 2x2 block at (crop_x + 2i, crop_y + 2 oyy) inside the frame (rows
+ * r0, r1; r1 NULL past the last row), in the order (0,0), (1,0), (0,1), (1,1). Monochrome frames
+ * average into green and replicate it; multi-component frames average their first three. */
+static void row_half(const Kern *k, const float *r0, const float *r1, size_t oyy, float *out, size_t ow) {
+    size_t y0 = k->crop_y + oyy * 2;
+    for (size_t i = 0; i < ow; i++) {
+        size_t x0 = k->crop_x + i * 2;
+        float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f}, n[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        for (size_t dy = 0; dy < 2; dy++) {
+            const float *row = dy ? r1 : r0;
+            if (!row) continue;
+            for (size_t dx = 0; dx < 2; dx++) {
+                size_t x = x0 + dx;
+                if (x >= k->w) continue;
+                if (k->cpp == 1 && k->has_cfa) {
+                    int c = colour_xy(k, x, y0 + dy);
+                    acc[c] += row[x];
+                    n[c] += 1.0f;
+                } else if (k->cpp == 1) {
+                    acc[1] += row[x];
+                    n[1] += 1.0f;
+                } else {
+                    for (size_t c = 0; c < zmin(k->cpp, 3); c++) {
+                        acc[c] += row[x * k->cpp + c];
+                        n[c] += 1.0f;
+                    }
+                }
+            }
+        }
+        float rgb[3];
+        for (int c = 0; c < 3; c++) rgb[c] = n[c] > 0.0f ? acc[c] / n[c] : 0.0f;
+        if (k->cpp == 1 && !k->has_cfa) {
+            float g = acc[1] / rmax(n[1], 1.0f);
+            rgb[0] = rgb[1] = rgb[2] = g;
         }
         finish_rgb(rgb, k->wb, k->m, k->has_levels, k->lscale, k->loffset, out + i * 3);
     }
@@ -622,18 +660,22 @@ static void encode_run(const Encoder *e, const float *restrict src, size_t n, si
 
 static int build_kernel(Kern *k, const fro_frame *f, const fro_params *p) {
     memset(k, 0, sizeof(*k));
-    if (!f->u16 || f->width == 0 || f->height == 0 || f->cfa_w == 0 || f->cfa_h == 0 || f->cfa_w > 8 ||
-        f->cfa_h > 8)
+    if ((!f->u16 && !f->f32) || f->width == 0 || f->height == 0 || f->cfa_w > 8 || f->cfa_h > 8 ||
+        (f->cfa_w == 0) != (f->cfa_h == 0))
         return -1;
     k->raw = f->u16;
+    k->rawf = f->f32;
+    k->cpp = f->cpp ? f->cpp : 1;
+    k->has_cfa = f->cfa_w != 0;
+    k->half = p->half_size != 0;
     k->w = f->width;
     k->h = f->height;
     k->ox = f->origin_x;
     k->oy = f->origin_y;
-    k->pw = f->cfa_w;
-    k->ph = f->cfa_h;
+    k->pw = k->has_cfa ? f->cfa_w : 1;
+    k->ph = k->has_cfa ? f->cfa_h : 1;
     k->cfa = f->cfa;
-    k->cfa_w = f->cfa_w;
+    k->cfa_w = k->pw;
     if (calib_new(k, f)) return -1;
     /* is_bayer (crates/flashraw-core/src/types.rs:243-252) */
     if (f->cfa_w == 2 && f->cfa_h == 2) {
@@ -665,6 +707,7 @@ static int build_kernel(Kern *k, const fro_frame *f, const fro_params *p) {
     k->has_levels = p->has_levels;
     k->lscale = p->levels_scale;
     k->loffset = p->levels_offset;
+    if ((p->demosaic == FRO_AHD || p->demosaic == FRO_EDGE) && (k->cpp != 1 || !k->has_cfa || k->half)) return -2;
     if (p->demosaic == FRO_AHD) {
         /* ahd::Plan::new (ahd.rs:49-61) with the frame origin; the pattern must be Bayer. */
         if (f->cfa_w != 2 || f->cfa_h != 2) return -2;
@@ -712,7 +755,7 @@ typedef struct {
 
 static const float *rb_row(const RowBlock *rb, const Kern *k, size_t y) {
     if (y >= k->h) return NULL;
-    return rb->buf + (y - rb->base) * k->w;
+    return rb->buf + (y - rb->base) * k->w * k->cpp;
 }
 
 
@@ -924,6 +967,10 @@ int fro_develop(const fro_frame *f, const fro_params *p, void *out, uint64_t out
     /* output_dims (process.rs:1384-1398); Rect::clamp_to */
     size_t cx = zmin(p->crop_x, k.w), cy = zmin(p->crop_y, k.h);
     size_t ow = zmin((size_t)p->crop_w, k.w - cx), oh = zmin((size_t)p->crop_h, k.h - cy);
+    if (k.half && ow && oh) { /* output_dims (process.rs:1384-1398) */
+        ow /= 2;
+        oh /= 2;
+    }
     if (ow == 0 || oh == 0) {
         free_kernel(&k);
         return -1;
@@ -986,8 +1033,8 @@ int fro_develop(const fro_frame *f, const fro_params *p, void *out, uint64_t out
 #pragma omp parallel num_threads(nthreads)
     {
         /* per-thread buffers */
-        size_t maxrows = strip + 2 * (halo > 1 ? halo : 1) + 2;
-        RowBlock rb = {(float *)malloc(maxrows * k.w * sizeof(float)), 0, 0};
+        size_t maxrows = 2 * strip + 2 * (halo > 1 ? halo : 1) + 2;
+        RowBlock rb = {(float *)malloc(maxrows * k.w * k.cpp * sizeof(float)), 0, 0};
         float *rgbrow = (float *)malloc(ow * 3 * sizeof(float));
         float *band = NULL, *mband = NULL, *chunk_out = NULL;
         Work wk;
@@ -1016,15 +1063,24 @@ int fro_develop(const fro_frame *f, const fro_params *p, void *out, uint64_t out
             /* normalised frame rows needed by the strip (bilinear: +-1, AHD: +-HALO) */
             size_t fy0 = sat_sub(cy + first, banded ? halo : 1);
             size_t fy1 = zmin(cy + first + count + (banded ? halo : 1), k.h);
+            if (k.half) { /* the 2x2 blocks of the strip's output rows */
+                fy0 = cy + 2 * first;
+                fy1 = zmin(cy + 2 * (first + count), k.h);
+            }
             rb.base = fy0;
             rb.count = fy1 - fy0;
-            for (size_t y = fy0; y < fy1; y++) normalize_row(&k, y, rb.buf + (y - fy0) * k.w);
+            for (size_t y = fy0; y < fy1; y++) normalize_row(&k, y, rb.buf + (y - fy0) * k.w * k.cpp);
             size_t lo = 0, hi = 0;
             if (banded) {
                 lo = zclamp(sat_sub(halo, cy), first, first + count);
                 hi = zclamp(sat_sub(sat_sub(k.h, halo), cy), lo, first + count);
             }
-            for (size_t i = 0; i < count; i++) {
+            for (size_t i = 0; i < count && k.half; i++) {
+                size_t oyy = first + i, y0 = cy + 2 * oyy;
+                row_half(&k, rb_row(&rb, &k, y0), y0 + 1 < k.h ? rb_row(&rb, &k, y0 + 1) : NULL, oyy, rgbrow, ow);
+                encode_run(e, rgbrow, ow, 0, oyy);
+            }
+            for (size_t i = 0; i < count && !k.half; i++) {
                 size_t oyy = first + i, y = cy + oyy;
                 const float *up = y > 0 ? rb_row(&rb, &k, y - 1) : NULL;
                 const float *mid = rb_row(&rb, &k, y);
@@ -1255,7 +1311,7 @@ int fro_normalize(const fro_frame *f, float *out) {
         free_kernel(&k);
         return -1;
     }
-    for (size_t y = 0; y < k.h; y++) normalize_row(&k, y, out + y * k.w);
+    for (size_t y = 0; y < k.h; y++) normalize_row(&k, y, out + y * k.w * k.cpp);
     free_kernel(&k);
     return 0;
 }
@@ -1304,4 +1360,63 @@ float fro_edge_beta(const float *norm, uint32_t w, uint32_t h, const uint8_t can
     float t = (nu - 2e-4f) / 6e-4f;
     t = t < 0.0f ? 0.0f : (t > 1.0f ? 1.0f : t);
     return 0.5f * t;
+}
+
+int fro_normalize_clamped(const fro_frame *f, float *out) {
+    if (fro_normalize(f, out)) return -1;
+    const size_t n = (size_t)f->width * f->height * (f->cpp ? f->cpp : 1);
+    for (size_t i = 0; i < n; i++) out[i] = rmax(out[i], 0.0f);
+    return 0;
+}
+
+/* ---- packed payloads (crates/flashraw-codecs/src/packed.rs) ---- */
+
+/* stream_bits (packed.rs): the 64-bit little-endian load reordered so that the payload's bit
+ * stream reads from the most significant bit (MSB orders) or the least (LSB bytes). */
+static uint64_t stream_bits(uint64_t le, int order) {
+    switch (order) {
+    case FRO_LSB_BYTES: return le;
+    case FRO_MSB_BYTES: return __builtin_bswap64(le);
+    case FRO_MSB32: return (le << 32) | (le >> 32);
+    default: {
+        uint64_t halves = (le << 32) | (le >> 32);
+        return ((halves & 0x0000FFFF0000FFFFull) << 16) | ((halves >> 16) & 0x0000FFFF0000FFFFull);
+    }
+    }
+}
+
+/* field (packed.rs): `bits` bits at bit offset `off` of the window. */
+static uint16_t field(uint64_t window, unsigned off, unsigned bits, int order) {
+    if (order == FRO_LSB_BYTES) return (uint16_t)((window >> off) & ((1ull << bits) - 1));
+    return (uint16_t)((window << off) >> (64 - bits));
+}
+
+int fro_unpack(const uint8_t *src, uint64_t len, uint32_t w, uint32_t h, uint32_t bits, int order, uint64_t stride,
+               uint16_t *out) {
+    if (bits < 1 || bits > 16 || order < 0 || order > 3) return -1;
+    const uint64_t row_bytes = ((uint64_t)w * bits + 7) / 8;
+    const size_t word = order == FRO_MSB16 ? 2 : (order == FRO_MSB32 ? 4 : 1);
+    for (uint64_t y = 0; y < h; y++) {
+        /* the row slice: src[y * stride ..][..min(row_bytes, rest)] */
+        uint64_t start = y * stride;
+        uint64_t n = start < len ? len - start : 0;
+        if (n > row_bytes) n = row_bytes;
+        const uint8_t *row = src + (start < len ? start : 0);
+        uint64_t pos = 0;
+        for (uint32_t x = 0; x < w; x++, pos += bits) {
+            /* samples_one_by_one: the window at the storage word holding the first bit,
+             * load_le64_padded (zeros past the slice) */
+            uint64_t at = (pos / 8) & ~(uint64_t)(word - 1);
+            uint64_t le = 0;
+            for (unsigned j = 0; j < 8; j++)
+                if (at + j < n) le |= (uint64_t)row[at + j] << (8 * j);
+            out[y * w + x] = field(stream_bits(le, order), (unsigned)(pos - at * 8), bits, order);
+        }
+    }
+    return 0;
+}
+
+void fro_linearize(uint16_t *v, uint64_t n, const uint16_t *lut, uint32_t lut_len) {
+    if (!lut || lut_len == 0) return;
+    for (uint64_t i = 0; i < n; i++) v[i] = lut[v[i] < lut_len ? v[i] : lut_len - 1];
 }

@@ -33,6 +33,10 @@ typedef struct fro_frame {
     uint32_t n_col_deltas;
     const uint32_t *white_values;
     uint32_t n_white;
+    /* f32 samples (raw values) in place of u16 when not NULL (RawImage PixelData::F32) */
+    const float *f32;
+    /* components per pixel (0 means 1); cfa_w == 0 means no CFA */
+    uint32_t cpp;
 } fro_frame;
 
 enum fro_demosaic { FRO_BILINEAR = 0, FRO_AHD = 1, FRO_EDGE = 2 };
@@ -70,6 +74,7 @@ typedef struct fro_params {
     int32_t orientation;     /* EXIF 1..8, applied to RGBA8 / SRGB16 / RGB8 only */
     int32_t encoding;
     const fro_edge *edge;    /* FRO_EDGE only */
+    int32_t half_size;       /* Demosaic::HalfSize: 2x2 blocks averaged per colour */
 } fro_params;
 
 /* Fused develop of `f` with `p` into `out` (out_w*out_h*channels samples of the encoding's
@@ -85,8 +90,21 @@ void fro_srgb16_table(float out[16385]);
 uint16_t fro_f32_to_f16(float x);
 
 /* The normalised frame (Kernel::normalize_row, process.rs:881-945, for every row) into
- * out[height][width]. Returns 0, or -1 for an invalid frame. */
+ * out[height][width * cpp]. Returns 0, or -1 for an invalid frame. */
 int fro_normalize(const fro_frame *f, float *out);
+/* process::normalize (process.rs:302-320): as fro_normalize, each value then max(v, 0). */
+int fro_normalize_clamped(const fro_frame *f, float *out);
+
+/* PackOrder (crates/flashraw-core/src/decoder.rs:203-212). */
+enum fro_pack_order { FRO_MSB_BYTES = 0, FRO_LSB_BYTES = 1, FRO_MSB16 = 2, FRO_MSB32 = 3 };
+/* unpack_row_scalar (crates/flashraw-codecs/src/packed.rs:137-180) over `h` rows of `w` samples
+ * of `bits` (1..16) bits: row y is src[y * stride .. min(y * stride + row_bytes, len)] with
+ * row_bytes = ceil(w * bits / 8), zeros beyond (the row semantics of the DNG decoders and of
+ * crates/flashraw-gpu/tests/equivalence.rs:52-63). Returns 0, or -1 for invalid arguments. */
+int fro_unpack(const uint8_t *src, uint64_t len, uint32_t w, uint32_t h, uint32_t bits, int order, uint64_t stride,
+               uint16_t *out);
+/* The linearisation curve of integer samples: v -> lut[min(v, len - 1)] (run.rs:38-39). */
+void fro_linearize(uint16_t *v, uint64_t n, const uint16_t *lut, uint32_t lut_len);
 /* Plan::set_noise_from (crates/flashraw/src/edgeaware/mod.rs:369-409) for a Bayer plan over the
  * normalised frame `norm` (w x h): the smoothing weight of the edge-aware demosaic. canon[4] are
  * the colours of the canonical pattern, (cx, cy) the canonical position of frame sample (0, 0). */

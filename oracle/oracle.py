@@ -50,6 +50,8 @@ class FroFrame(ctypes.Structure):
         ("n_col_deltas", ctypes.c_uint32),
         ("white_values", ctypes.c_void_p),
         ("n_white", ctypes.c_uint32),
+        ("f32", ctypes.c_void_p),
+        ("cpp", ctypes.c_uint32),
     ]
 
 
@@ -79,6 +81,7 @@ class FroParams(ctypes.Structure):
         ("orientation", ctypes.c_int32),
         ("encoding", ctypes.c_int32),
         ("edge", ctypes.POINTER(FroEdge)),
+        ("half_size", ctypes.c_int32),
     ]
 
 
@@ -99,6 +102,12 @@ def lib():
         L.fro_f32_to_f16.argtypes = [ctypes.c_float]
         L.fro_normalize.restype = ctypes.c_int
         L.fro_normalize.argtypes = [ctypes.POINTER(FroFrame), ctypes.c_void_p]
+        L.fro_normalize_clamped.restype = ctypes.c_int
+        L.fro_normalize_clamped.argtypes = [ctypes.POINTER(FroFrame), ctypes.c_void_p]
+        L.fro_unpack.restype = ctypes.c_int
+        L.fro_unpack.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                 ctypes.c_int, ctypes.c_uint64, ctypes.c_void_p]
+        L.fro_linearize.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_uint32]
         L.fro_edge_beta.restype = ctypes.c_float
         L.fro_edge_beta.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p,
                                     ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float]
@@ -115,25 +124,35 @@ def _ptr(a):
 
 
 def make_frame(mosaic, cfa, origin=(0, 0), black=(1, 1, 1, [0.0]), white=(16383,),
-               row_deltas=None, col_deltas=None):
-    """Build the frame descriptor. `black` is (rows, cols, samples, values); the arrays that
-    the descriptor points to are returned alongside it and must stay alive while it is used."""
-    mosaic = np.ascontiguousarray(mosaic, dtype=np.uint16)
-    if isinstance(cfa, str):
-        cfa = PATTERNS[cfa]
-    cfa = np.asarray(cfa, dtype=np.uint8)
+               row_deltas=None, col_deltas=None, cpp=1):
+    """Build the frame descriptor. `mosaic` is (h, w) or (h, w * cpp) of uint16 or float32 samples
+    (float32: raw values, RawImage PixelData::F32); `cfa` a pattern name, a 2-D array of CfaColor
+    codes, or None (no CFA); `black` is (rows, cols, samples, values). The arrays the descriptor
+    points to are returned alongside it and must stay alive while it is used."""
+    arr = np.asarray(mosaic)
     f = FroFrame()
-    f.u16 = mosaic.ctypes.data
-    f.height, f.width = mosaic.shape
+    if arr.dtype == np.float32:
+        arr = np.ascontiguousarray(arr)
+        f.f32 = arr.ctypes.data
+    else:
+        arr = np.ascontiguousarray(arr, dtype=np.uint16)
+        f.u16 = arr.ctypes.data
+    f.cpp = cpp
+    f.height = arr.shape[0]
+    f.width = arr.shape[1] // cpp
     f.origin_x, f.origin_y = origin
-    f.cfa_h, f.cfa_w = cfa.shape
-    for i, v in enumerate(cfa.reshape(-1)):
-        f.cfa[i] = int(v)
+    if cfa is not None:
+        if isinstance(cfa, str):
+            cfa = PATTERNS[cfa]
+        cfa = np.asarray(cfa, dtype=np.uint8)
+        f.cfa_h, f.cfa_w = cfa.shape
+        for i, v in enumerate(cfa.reshape(-1)):
+            f.cfa[i] = int(v)
     rows, cols, samples, values = black
     bv = np.ascontiguousarray(values, dtype=np.float32)
     f.black_rows, f.black_cols, f.black_samples = rows, cols, samples
     f.black_values, f.n_black_values = _ptr(bv), bv.size
-    keep = [mosaic, bv]
+    keep = [arr, bv]
     if row_deltas is not None:
         rd = np.ascontiguousarray(row_deltas, dtype=np.float32)
         f.row_deltas, f.n_row_deltas = _ptr(rd), rd.size
@@ -150,13 +169,15 @@ def make_frame(mosaic, cfa, origin=(0, 0), black=(1, 1, 1, [0.0]), white=(16383,
 
 def develop(mosaic, cfa, *, origin=(0, 0), black=(1, 1, 1, [0.0]), white=(16383,), wb=(1, 1, 1, 1),
             matrix=((1, 0, 0), (0, 1, 0), (0, 0, 1)), demosaic="bilinear", encoding="rgba8",
-            orientation=1, crop=None, levels=None, row_deltas=None, col_deltas=None, threads=0, edge=None):
-    """Fused develop of a single-component CFA mosaic (process.rs:1475-1591). Returns an
-    (out_h, out_w, channels) array of the encoding's sample type. `edge` (demosaic "edge") is a
-    dict with g0 (4 x 49), dir (4 x 3 x 4 x 49) float32 tables, cx, cy (canonical position of
-    sample (0, 0) in the RGGB-ordered canonical pattern) and beta."""
-    f, keep = make_frame(mosaic, cfa, origin, black, white, row_deltas, col_deltas)
-    h, w = np.asarray(mosaic).shape
+            orientation=1, crop=None, levels=None, row_deltas=None, col_deltas=None, threads=0, edge=None,
+            cpp=1, half_size=False):
+    """Fused develop (process.rs:1475-1591) of a frame of `cpp` components (u16 or f32 samples,
+    `cfa` None for none). Returns an (out_h, out_w, channels) array of the encoding's sample
+    type. `edge` (demosaic "edge") is a dict with g0 (4 x 49), dir (4 x 3 x 4 x 49) float32
+    tables, cx, cy (canonical position of sample (0, 0) in the RGGB-ordered canonical pattern)
+    and beta. `half_size` is Demosaic::HalfSize."""
+    f, keep = make_frame(mosaic, cfa, origin, black, white, row_deltas, col_deltas, cpp)
+    h, w = f.height, f.width
     p = FroParams()
     p.demosaic = DEMOSAIC[demosaic]
     cx, cy, cw, ch = crop if crop is not None else (0, 0, w, h)
@@ -170,6 +191,7 @@ def develop(mosaic, cfa, *, origin=(0, 0), black=(1, 1, 1, [0.0]), white=(16383,
         p.has_levels, p.levels_scale, p.levels_offset = 1, levels[0], levels[1]
     p.orientation = orientation
     p.encoding = ENCODING[encoding]
+    p.half_size = int(bool(half_size))
     if edge is not None:
         g0 = np.ascontiguousarray(edge["g0"], dtype=np.float32).reshape(-1)
         dr = np.ascontiguousarray(edge["dir"], dtype=np.float32).reshape(-1)
@@ -226,12 +248,35 @@ def dct_denoise(mosaic, black, sig_a, sig_b, thr=2.7, step=4, precision=64, thre
 
 
 def normalize(mosaic, cfa, origin=(0, 0), black=(1, 1, 1, [0.0]), white=(16383,), row_deltas=None,
-              col_deltas=None):
-    """The normalised frame (normalize_row of every row, process.rs:881-945) as f32 (h, w)."""
-    f, keep = make_frame(mosaic, cfa, origin, black, white, row_deltas, col_deltas)
-    out = np.zeros(mosaic.shape, dtype=np.float32)
-    if lib().fro_normalize(ctypes.byref(f), out.ctypes.data) != 0:
+              col_deltas=None, cpp=1, clamped=False):
+    """The normalised frame as f32 (h, w * cpp): normalize_row of every row (process.rs:881-945),
+    or with `clamped` process::normalize (process.rs:302-320, each value max(v, 0))."""
+    f, keep = make_frame(mosaic, cfa, origin, black, white, row_deltas, col_deltas, cpp)
+    out = np.zeros(np.asarray(mosaic).shape, dtype=np.float32)
+    fn = lib().fro_normalize_clamped if clamped else lib().fro_normalize
+    if fn(ctypes.byref(f), out.ctypes.data) != 0:
         raise ValueError("oracle rejected the frame")
+    return out
+
+
+PACK_ORDER = {"msb_bytes": 0, "lsb_bytes": 1, "msb16": 2, "msb32": 3}
+
+
+def unpack(src, w, h, bits, order, stride):
+    """unpack_row_scalar over h rows of w samples (crates/flashraw-codecs/src/packed.rs:137-180)."""
+    src = np.ascontiguousarray(src, dtype=np.uint8)
+    out = np.zeros((h, w), dtype=np.uint16)
+    o = PACK_ORDER[order] if isinstance(order, str) else int(order)
+    if lib().fro_unpack(src.ctypes.data, src.size, w, h, bits, o, stride, out.ctypes.data) != 0:
+        raise ValueError("oracle rejected the unpack")
+    return out
+
+
+def linearize(v, lut):
+    """v -> lut[min(v, len - 1)] (run.rs:38-39) on a copy of the uint16 array v."""
+    out = np.ascontiguousarray(v, dtype=np.uint16).copy()
+    lut = np.ascontiguousarray(lut, dtype=np.uint16)
+    lib().fro_linearize(out.ctypes.data, out.size, lut.ctypes.data, lut.size)
     return out
 
 

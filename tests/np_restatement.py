@@ -496,3 +496,136 @@ def develop_edge(mos, cfa, *, origin=(0, 0), black=(1, 1, 1, [0.0]), white=(1638
         s = (slice(EDGE_HALO, h - EDGE_HALO), slice(EDGE_HALO, w - EDGE_HALO))
         out[s] = fe[s]
     return out
+
+
+# ---------------------------------------------------------------- inputs and layouts
+
+
+def normalise_c(arr, cpp, cfa, origin=(0, 0), black=(1, 1, 1, [0.0]), white=(16383,), row_deltas=None,
+                col_deltas=None):
+    """Calib::new(info, c) + Calib::norm for every sample of a frame of `cpp` components
+    (process.rs:256-298; kernels.rs:216-248): component c of pixel x uses black values
+    [((y % rows) * cols + x % cols) * samples + c % samples] and, without a CFA, the white level
+    of index min(c, n - 1). `arr` is (h, w * cpp), u16 or f32 raw values."""
+    h, wc = arr.shape
+    w = wc // cpp
+    brows, bcols, bsamples, bvals = black
+    brows, bcols, bsamples = max(brows, 1), max(bcols, 1), max(bsamples, 1)
+    bvals = np.asarray(bvals, dtype=F32)
+    p = parse_cfa(cfa) if cfa is not None else None
+    cw, ch = (p.shape[1], p.shape[0]) if p is not None else (1, 1)
+    pw, ph = min(math.lcm(cw, bcols), 64), min(math.lcm(ch, brows), 64)
+    Y = (np.arange(h)[:, None] + origin[1]) + np.zeros((1, wc), dtype=np.int64)
+    X = (np.arange(wc)[None, :] // cpp + origin[0]) + np.zeros((h, 1), dtype=np.int64)
+    C = (np.arange(wc)[None, :] % cpp) + np.zeros((h, 1), dtype=np.int64)
+    yy, xx = Y % ph, X % pw
+    idx = ((yy % brows) * bcols + xx % bcols) * bsamples + C % bsamples
+    bl = np.where(idx < bvals.size, bvals[np.minimum(idx, bvals.size - 1)], F32(0.0)).astype(F32)
+    colour = p[yy % ch, xx % cw] if p is not None else C
+    wv = np.asarray(white, dtype=np.int64)
+    wsel = wv[np.minimum(colour, wv.size - 1)] if wv.size > 1 else np.full_like(colour, wv[0])
+    wf = wsel.astype(F32)
+    with np.errstate(divide="ignore"):
+        scale = np.where(wf > bl, F32(1.0) / (wf - bl), F32(1.0)).astype(F32)
+    b = bl.copy()
+    if row_deltas is not None and len(row_deltas):
+        rd = np.asarray(row_deltas, dtype=F32)
+        b = b + np.where(Y < rd.size, rd[np.minimum(Y, rd.size - 1)], F32(0.0)).astype(F32)
+    if col_deltas is not None and len(col_deltas):
+        cd = np.asarray(col_deltas, dtype=F32)
+        b = b + np.where(X < cd.size, cd[np.minimum(X, cd.size - 1)], F32(0.0)).astype(F32)
+    return ((arr.astype(F32) - b) * scale).astype(F32)
+
+
+def half_size(norm, cpp, col, crop):
+    """Demosaic::HalfSize (process.rs:975-1036): output pixel (i, j) averages per colour the
+    samples of the 2x2 block at (cx + 2i, cy + 2j) inside the frame, in the order (0,0), (1,0),
+    (0,1), (1,1); `col` the colour plane of a CFA frame (cpp 1) or None. Returns camera RGB."""
+    h, wc = norm.shape
+    w = wc // cpp
+    cx, cy, cw, ch = crop
+    ow, oh = cw // 2, ch // 2
+    acc = np.zeros((oh, ow, 4), F32)
+    n = np.zeros((oh, ow, 4), F32)
+    jj, ii = np.mgrid[0:oh, 0:ow]
+    for dy in range(2):
+        for dx in range(2):
+            y = cy + 2 * jj + dy
+            x = cx + 2 * ii + dx
+            ok = (y < h) & (x < w)
+            ys, xs = np.minimum(y, h - 1), np.minimum(x, w - 1)
+            if cpp == 1 and col is not None:
+                c = np.minimum(col[ys, xs], 3)
+                v = norm[ys, xs]
+                for k in range(4):
+                    m = ok & (c == k)
+                    acc[..., k] = np.where(m, acc[..., k] + v, acc[..., k]).astype(F32)
+                    n[..., k] = np.where(m, n[..., k] + F32(1.0), n[..., k]).astype(F32)
+            elif cpp == 1:
+                v = norm[ys, xs]
+                acc[..., 1] = np.where(ok, acc[..., 1] + v, acc[..., 1]).astype(F32)
+                n[..., 1] = np.where(ok, n[..., 1] + F32(1.0), n[..., 1]).astype(F32)
+            else:
+                for k in range(min(cpp, 3)):
+                    v = norm[ys, xs * cpp + k]
+                    acc[..., k] = np.where(ok, acc[..., k] + v, acc[..., k]).astype(F32)
+                    n[..., k] = np.where(ok, n[..., k] + F32(1.0), n[..., k]).astype(F32)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        rgb = np.where(n[..., :3] > 0, acc[..., :3] / n[..., :3], F32(0.0)).astype(F32)
+        if cpp == 1 and col is None:
+            g = (acc[..., 1] / np.maximum(n[..., 1], F32(1.0))).astype(F32)
+            rgb = np.stack([g, g, g], -1)
+    return rgb
+
+
+def packed_model(src, w, h, bits, order, stride):
+    """The GPU kernels' statement of a packed sample (crates/flashraw-gpu/src/kernels.rs:333-357,
+    common.wgsl packed_sample): the 3 logical bytes at bitpos / 8 of the row (word orders read the
+    physical byte k ^ 1 / k ^ 3; bytes at or past row_bytes, or past the payload, read 0), then a
+    shift and a mask."""
+    src = np.asarray(src, dtype=np.uint32)
+    row_bytes = (w * bits + 7) // 8
+    ys, xs = np.mgrid[0:h, 0:w].astype(np.int64)
+    bitpos = xs * bits
+    k, sh = bitpos >> 3, bitpos & 7
+
+    def byte(kk):
+        p = kk ^ 1 if order == 2 else (kk ^ 3 if order == 3 else kk)
+        a = ys * stride + p
+        ok = (p < row_bytes) & (a < src.size)
+        return np.where(ok, src[np.minimum(a, max(src.size - 1, 0))] if src.size else 0, 0).astype(np.uint32)
+
+    b0, b1, b2 = byte(k), byte(k + 1), byte(k + 2)
+    mask = np.uint32((1 << bits) - 1)
+    if order == 1:
+        v = ((b0 | (b1 << 8) | (b2 << 16)) >> sh.astype(np.uint32)) & mask
+    else:
+        v = (((b0 << 16) | (b1 << 8) | b2) >> (24 - sh - bits).astype(np.uint32)) & mask
+    return v.astype(np.uint16)
+
+
+def pack(samples, bits, order, stride):
+    """Pack rows of samples (h, w) by the format definitions (decoder.rs:203-212): the MSB-first
+    or LSB-first bit stream of each row, then for the word orders the MSB stream's bytes stored as
+    little-endian 16- or 32-bit words (the row's byte count must then be a multiple of the word);
+    rows start `stride` bytes apart, padding bytes 0xA5."""
+    h, w = samples.shape
+    row_bytes = (w * bits + 7) // 8
+    wsz = 2 if order == 2 else (4 if order == 3 else 1)
+    assert row_bytes % wsz == 0 and stride >= row_bytes
+    out = bytearray([0xA5]) * (stride * h)
+    for y in range(h):
+        acc = 0
+        if order == 1:
+            for x in range(w):
+                acc |= int(samples[y, x]) << (x * bits)
+            row = bytearray(acc.to_bytes(row_bytes, "little"))
+        else:
+            for x in range(w):
+                acc = (acc << bits) | int(samples[y, x])
+            acc <<= row_bytes * 8 - w * bits
+            row = bytearray(acc.to_bytes(row_bytes, "big"))
+            for i in range(0, row_bytes, wsz):
+                row[i:i + wsz] = row[i:i + wsz][::-1]
+        out[y * stride:y * stride + row_bytes] = row
+    return np.frombuffer(bytes(out), dtype=np.uint8)
