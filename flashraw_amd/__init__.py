@@ -2,14 +2,14 @@
 hand-written gfx950 HIP kernels (flashraw_amd/lib/libflashraw_hip.so)."""
 from .gpu import (Batch, BlackLevel, CfaColor, CfaPattern, DenoiseSpec, DevelopOutput, DnPrecision, DevelopSpec, EDGE_CANON, EDGE_HALO, EdgeSpec, FlashRawError,
                   FrameInfo, GpuContext, GpuFrame, GpuImage, GpuOp, GpuRun, GpuTimings, GpuUnavailable,
-                  InvalidArgument, JobResult, LimitExceeded, NotAvailable, Orientation, OutOfMemory, Rect, Server,
+                  InvalidArgument, JobResult, PackOrder, LimitExceeded, NotAvailable, Orientation, OutOfMemory, Rect, Server,
                   NoiseReduction, WhiteLevel, pinned_empty,
                   ahd_spec, available, context, edge_plan, edge_spec, estimate_beta, lib, srgb8_table)
 
 __all__ = [
     "Batch", "BlackLevel", "CfaColor", "CfaPattern", "DenoiseSpec", "DevelopOutput", "DnPrecision", "DevelopSpec", "EDGE_CANON", "EDGE_HALO", "EdgeSpec", "FlashRawError",
     "FrameInfo", "GpuContext", "GpuFrame", "GpuImage", "GpuOp", "GpuRun", "GpuTimings", "GpuUnavailable",
-    "InvalidArgument", "JobResult", "LimitExceeded", "NotAvailable", "Orientation", "OutOfMemory", "Rect", "Server",
+    "InvalidArgument", "JobResult", "PackOrder", "LimitExceeded", "NotAvailable", "Orientation", "OutOfMemory", "Rect", "Server",
     "NoiseReduction", "WhiteLevel", "pinned_empty",
     "ahd_spec", "available", "context", "edge_plan", "edge_spec", "estimate_beta", "lib", "srgb8_table",
 ]

@@ -151,6 +151,7 @@ frh_context::~frh_context() {
     if (d_srgb16) (void)hipFree(d_srgb16);
     if (d_wtab) (void)hipFree(d_wtab);
     for (auto &kv : edge_tabs) (void)hipFree(kv.second);
+    for (auto &kv : tables) (void)hipFree(kv.second);
     if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -258,6 +259,116 @@ static uint32_t tuned_rows(const frh_context *ctx, bool ahd, const DevArgs &a, u
 }
 
 // Everything a develop launch needs, validated (run.rs:439-646).
+// Calib::new (process.rs:256-286; kernels.rs:216-248) for every component into
+// gcal = black[cpp][pw * ph] then scale[cpp][pw * ph]; the pattern is lcm(CFA, black repeat),
+// at most 64 positions here.
+int32_t frh::build_calib(const frh_frame_info *info, uint32_t cpp, std::vector<float> &gcal, uint32_t &pw_out,
+                         uint32_t &ph_out) {
+    const bool has_cfa = info->cfa_width != 0 && info->cfa_height != 0;
+    const uint64_t rows = info->black_rows ? info->black_rows : 1, cols = info->black_cols ? info->black_cols : 1;
+    const uint64_t samples = info->black_samples ? info->black_samples : 1;
+    const uint64_t cw0 = has_cfa ? info->cfa_width : 1, ch0 = has_cfa ? info->cfa_height : 1;
+    uint64_t pw = cw0 / gcd64(cw0, cols) * cols, ph = ch0 / gcd64(ch0, rows) * rows;
+    pw = pw < 64 ? pw : 64;
+    ph = ph < 64 ? ph : 64;
+    if (pw * ph > 64) return fail(FRH_ERROR_NOT_AVAILABLE, "black-level pattern larger than 64 positions");
+    if ((info->black_values_len && !info->black_values) || (info->white_values_len && !info->white_values))
+        return fail(FRH_ERROR_INVALID_ARGUMENT, "NULL calibration array");
+    gcal.assign(2 * (size_t)cpp * pw * ph, 0.0f);
+    for (uint32_t c = 0; c < cpp; c++) {
+        for (uint64_t y = 0; y < ph; y++) {
+            for (uint64_t x = 0; x < pw; x++) {
+                uint64_t idx = ((y % rows) * cols + x % cols) * samples + c % samples;
+                float bl = idx < info->black_values_len ? info->black_values[idx] : 0.0f;
+                uint32_t colour = has_cfa ? info->cfa[(y % ch0) * cw0 + x % cw0] : c;
+                uint32_t nw = info->white_values_len;
+                uint32_t wi = nw > 1 ? (colour < nw - 1 ? colour : nw - 1) : 0;
+                uint32_t wv = wi < nw ? info->white_values[wi] : (nw ? info->white_values[0] : 0);
+                float white = (float)wv;
+                gcal[(c * ph + y) * pw + x] = bl;
+                gcal[((cpp + c) * ph + y) * pw + x] = white > bl ? 1.0f / (white - bl) : 1.0f;
+            }
+        }
+    }
+    pw_out = (uint32_t)pw;
+    ph_out = (uint32_t)ph;
+    return FRH_OK;
+}
+
+// GpuInput validation (run.rs:456-490) and the window's byte range in the raster.
+int32_t frh::check_input(const frh_frame *f, InputDesc &in) {
+    memset(&in, 0, sizeof(in));
+    const uint32_t w = f->window.width, h = f->window.height, cpp = f->cpp ? f->cpp : 1;
+    if (w == 0 || h == 0 || (uint64_t)f->window.x + w > f->raster_width ||
+        (uint64_t)f->window.y + h > f->raster_height)
+        return fail(FRH_ERROR_INVALID_ARGUMENT, "window outside the raster");
+    if (!f->data) return fail(FRH_ERROR_INVALID_ARGUMENT, "frame data is NULL");
+    in.kind = f->input;
+    in.w = w;
+    in.h = h;
+    in.cpp = cpp;
+    in.len = f->data_len;
+    const uint64_t raster_row = (uint64_t)f->raster_width * cpp;
+    const uint64_t raster_samples = raster_row * f->raster_height;
+    if (f->input == FRH_INPUT_PACKED) {
+        if (f->bits < 1 || f->bits > 16)
+            return fail(FRH_ERROR_INVALID_ARGUMENT, "unsupported packed bit depth " + std::to_string(f->bits));
+        if (f->order < FRH_PACK_MSB_BYTES || f->order > FRH_PACK_MSB32)
+            return fail(FRH_ERROR_INVALID_ARGUMENT, "unknown pack order");
+        in.bits = f->bits;
+        in.order = f->order;
+        in.row_bytes = (raster_row * f->bits + 7) / 8;
+        in.stride = f->row_stride;
+        if (in.row_bytes > 0xffffffffull || in.stride > 0xffffffffull)
+            return fail(FRH_ERROR_LIMIT_EXCEEDED, "packed row too long");
+        // the payload bytes of the window's rows (run.rs:785-796); bytes past the payload read 0
+        const uint64_t row_extent = std::max(in.stride, in.row_bytes);
+        const uint64_t start = std::min((uint64_t)f->window.y * in.stride, in.len);
+        const uint64_t end = std::min(((uint64_t)f->window.y + h - 1) * in.stride + row_extent, in.len);
+        in.first_byte = start;
+        in.upload_bytes = end > start ? end - start : 0;
+    } else if (f->input == FRH_INPUT_U16 || f->input == FRH_INPUT_F32) {
+        if (f->data_len < raster_samples)
+            return fail(FRH_ERROR_INVALID_ARGUMENT, f->input == FRH_INPUT_U16 ? "u16 raster shorter than its dimensions"
+                                                                            : "f32 raster shorter than its dimensions");
+        const uint64_t bps = f->input == FRH_INPUT_U16 ? 2 : 4;
+        in.stride = raster_row;
+        in.first_byte = (uint64_t)f->window.y * raster_row * bps;
+        in.upload_bytes = (uint64_t)h * raster_row * bps;
+    } else {
+        return fail(FRH_ERROR_INVALID_ARGUMENT, "unknown input kind");
+    }
+    if (f->linearization_len && !f->linearization) return fail(FRH_ERROR_INVALID_ARGUMENT, "NULL linearisation table");
+    // run.rs:491: the curve applies to integer samples only
+    in.lut = f->linearization && f->linearization_len && f->input != FRH_INPUT_F32;
+    in.staged = f->input == FRH_INPUT_PACKED || in.lut;
+    return FRH_OK;
+}
+
+const void *frh::device_table(frh_context *ctx, const void *host, uint64_t bytes, int32_t *rc) {
+    *rc = FRH_OK;
+    if (!bytes) return nullptr;
+    uint64_t hsh = 1469598103934665603ull;
+    const uint8_t *b = static_cast<const uint8_t *>(host);
+    for (uint64_t i = 0; i < bytes; i++) hsh = (hsh ^ b[i]) * 1099511628211ull;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    auto it = ctx->tables.find({hsh, bytes});
+    if (it != ctx->tables.end()) return it->second;
+    void *d = nullptr;
+    hipError_t e;
+    if ((e = hipMalloc(&d, bytes)) != hipSuccess) {
+        *rc = hip_fail(e, "table allocation");
+        return nullptr;
+    }
+    if ((e = hipMemcpy(d, host, bytes, hipMemcpyHostToDevice)) != hipSuccess) {
+        (void)hipFree(d);
+        *rc = hip_fail(e, "table upload");
+        return nullptr;
+    }
+    ctx->tables[{hsh, bytes}] = d;
+    return d;
+}
+
 // The device tap lists of an edge-aware spec (built once per distinct table contents and kept by
 // the context): per canonical position, G0's non-zero taps and, per colour, the taps where some
 // direction is non-zero, in tap order (edgeaware/mod.rs:130-160).
@@ -345,26 +456,18 @@ int32_t frh::plan_develop(const frh_context *ctx, const frh_frame *f, const frh_
         return fail(FRH_ERROR_INVALID_ARGUMENT, "descriptor size field too small");
     memset(&p, 0, sizeof(p));
     DevArgs &a = p.a;
-    const uint32_t w = f->window.width, h = f->window.height, cpp = f->cpp ? f->cpp : 1;
-    if (w == 0 || h == 0 || (uint64_t)f->window.x + w > f->raster_width ||
-        (uint64_t)f->window.y + h > f->raster_height)
-        return fail(FRH_ERROR_INVALID_ARGUMENT, "window outside the raster");
-    if (f->input != FRH_INPUT_U16)
-        return fail(FRH_ERROR_NOT_AVAILABLE, "only u16 input is supported by this develop path");
-    if (!f->data) return fail(FRH_ERROR_INVALID_ARGUMENT, "frame data is NULL");
-    if (f->data_len < (uint64_t)f->raster_width * f->raster_height * cpp)
-        return fail(FRH_ERROR_INVALID_ARGUMENT, "u16 raster shorter than its dimensions");
-    if (f->linearization) return fail(FRH_ERROR_NOT_AVAILABLE, "linearisation tables are not supported yet");
-    if (cpp != 1) return fail(FRH_ERROR_NOT_AVAILABLE, "only single-component mosaics are supported");
-    if (info->cfa_width != 2 || info->cfa_height != 2)
-        return fail(FRH_ERROR_NOT_AVAILABLE, "only Bayer mosaics are supported");
-    {
+    int32_t rc = check_input(f, p.in);
+    if (rc) return rc;
+    const uint32_t w = p.in.w, h = p.in.h, cpp = p.in.cpp;
+    const bool has_cfa = info->cfa_width != 0 && info->cfa_height != 0;
+    if (has_cfa && (info->cfa_width > 8 || info->cfa_height > 8))
+        return fail(FRH_ERROR_INVALID_ARGUMENT, "CFA pattern larger than 8x8");
+    bool bayer = false;
+    if (has_cfa && info->cfa_width == 2 && info->cfa_height == 2) {
         int cnt[256] = {0};
         for (int i = 0; i < 4; i++) cnt[info->cfa[i]]++;
-        if (!(cnt[0] == 1 && cnt[1] == 2 && cnt[2] == 1))
-            return fail(FRH_ERROR_NOT_AVAILABLE, "only Bayer mosaics are supported");
+        bayer = cnt[0] == 1 && cnt[1] == 2 && cnt[2] == 1;
     }
-    if (spec->half_size) return fail(FRH_ERROR_NOT_AVAILABLE, "half-size develop is not supported yet");
     if (spec->curve || spec->tone) return fail(FRH_ERROR_NOT_AVAILABLE, "output curves and tone are not supported");
     if (!valid_mode(spec->output)) return fail(FRH_ERROR_INVALID_ARGUMENT, "unknown output");
     if (spec->output2 != OUT_NONE && !valid_mode(spec->output2))
@@ -376,49 +479,80 @@ int32_t frh::plan_develop(const frh_context *ctx, const frh_frame *f, const frh_
     if (spec->has_levels && !(std::isfinite(spec->levels[0]) && std::isfinite(spec->levels[1])))
         return fail(FRH_ERROR_INVALID_ARGUMENT, "non-finite levels");
 
-    a.src = static_cast<const uint16_t *>(f->data) + (size_t)f->window.y * f->raster_width + f->window.x;
-    a.stride = f->raster_width;
+    // Samples: the window of the raster (the upload replaces the pointer for host input; staged
+    // input is replaced by its unpacked u16 window).
+    const uint64_t row_samples = (uint64_t)f->raster_width * cpp;
+    if (p.in.kind == FRH_INPUT_F32)
+        a.srcf = static_cast<const float *>(f->data) + (size_t)f->window.y * row_samples + (size_t)f->window.x * cpp;
+    else if (p.in.kind == FRH_INPUT_U16)
+        a.src = static_cast<const uint16_t *>(f->data) + (size_t)f->window.y * row_samples + (size_t)f->window.x * cpp;
+    a.stride = (uint32_t)(p.in.staged ? (uint64_t)w * cpp : row_samples);
+    a.cpp = cpp;
     a.width = w;
     a.height = h;
     a.ox = f->origin_x;
     a.oy = f->origin_y;
-    // Calib::new (process.rs:256-286); the kernels take at most 64 pattern positions.
-    const uint64_t rows = info->black_rows ? info->black_rows : 1, cols = info->black_cols ? info->black_cols : 1;
-    const uint64_t samples = info->black_samples ? info->black_samples : 1;
-    uint64_t pw = 2 / gcd64(2, cols) * cols, ph = 2 / gcd64(2, rows) * rows;
-    pw = pw < 64 ? pw : 64;
-    ph = ph < 64 ? ph : 64;
-    if (pw * ph > 64) return fail(FRH_ERROR_NOT_AVAILABLE, "black-level pattern larger than 64 positions");
-    if ((info->black_values_len && !info->black_values) || (info->white_values_len && !info->white_values))
-        return fail(FRH_ERROR_INVALID_ARGUMENT, "NULL calibration array");
-    a.cal_pw = (uint32_t)pw;
-    a.cal_ph = (uint32_t)ph;
+    // Calib::new (process.rs:256-286) for every component
+    std::vector<float> gcal;
+    uint32_t pw, ph;
+    if ((rc = build_calib(info, cpp, gcal, pw, ph))) return rc;
+    a.cal_pw = pw;
+    a.cal_ph = ph;
     a.cal_pow2 = ((pw & (pw - 1)) == 0 && (ph & (ph - 1)) == 0) ? 1u : 0u;
-    for (uint64_t y = 0; y < ph; y++) {
-        for (uint64_t x = 0; x < pw; x++) {
-            uint64_t idx = ((y % rows) * cols + x % cols) * samples;
-            float bl = idx < info->black_values_len ? info->black_values[idx] : 0.0f;
-            uint32_t colour = info->cfa[(y % 2) * 2 + x % 2];
-            uint32_t nw = info->white_values_len;
-            uint32_t wi = nw > 1 ? (colour < nw - 1 ? colour : nw - 1) : 0;
-            uint32_t wv = wi < nw ? info->white_values[wi] : (nw ? info->white_values[0] : 0);
-            float white = (float)wv;
-            a.black[y * pw + x] = bl;
-            a.scale[y * pw + x] = white > bl ? 1.0f / (white - bl) : 1.0f;
-        }
+    for (uint32_t i = 0; i < pw * ph; i++) {
+        a.black[i] = gcal[i];
+        a.scale[i] = gcal[(size_t)cpp * pw * ph + i];
     }
+    frh_context *mctx = const_cast<frh_context *>(ctx);
     if (info->black_row_deltas_len) {
         if (!info->black_row_deltas) return fail(FRH_ERROR_INVALID_ARGUMENT, "NULL row deltas");
         a.n_rowd = info->black_row_deltas_len;
+        a.rowd = static_cast<const float *>(device_table(mctx, info->black_row_deltas, a.n_rowd * 4ull, &rc));
+        if (rc) return rc;
     }
     if (info->black_col_deltas_len) {
         if (!info->black_col_deltas) return fail(FRH_ERROR_INVALID_ARGUMENT, "NULL column deltas");
         a.n_cold = info->black_col_deltas_len;
+        a.cold = static_cast<const float *>(device_table(mctx, info->black_col_deltas, a.n_cold * 4ull, &rc));
+        if (rc) return rc;
     }
-    for (int i = 0; i < 4; i++) {
-        uint32_t colour = info->cfa[i];  // pattern position (i & 1, i >> 1) of sensor parity
-        a.chan[i] = colour < 3 ? colour : 3;
-        a.chan_bits |= a.chan[i] << (2 * i);
+    // The generic kernel: every frame but a single-component Bayer mosaic at full size.
+    a.half = spec->half_size ? 1u : 0u;
+    a.generic = (cpp == 1 && bayer && !spec->half_size) ? 0u : 1u;
+    if (a.generic) {
+        a.gmode = (cpp == 1 && has_cfa) ? 1u : (cpp >= 3 ? 2u : 0u);
+        a.gcal = static_cast<const float *>(device_table(mctx, gcal.data(), gcal.size() * 4ull, &rc));
+        if (rc) return rc;
+        if (has_cfa) {
+            // cfa_sites (kernels.rs:260-287): colour (clamped to 3) and, per colour R, G, B, the 9-bit
+            // mask of 3x3 neighbours of that colour (the site's own colour: the centre only)
+            const uint32_t cw = info->cfa_width, chh = info->cfa_height;
+            std::vector<uint32_t> sites(cw * chh);
+            for (uint32_t py = 0; py < chh; py++)
+                for (uint32_t px = 0; px < cw; px++) {
+                    const uint32_t own = std::min<uint32_t>(info->cfa[py * cw + px], 3);
+                    uint32_t d = own;
+                    for (int dy = -1; dy <= 1; dy++)
+                        for (int dx = -1; dx <= 1; dx++) {
+                            const uint32_t c = std::min<uint32_t>(
+                                info->cfa[((py + dy + 4 * chh) % chh) * cw + (px + dx + 4 * cw) % cw], 3);
+                            if (c < 3 && (c != own || (dx == 0 && dy == 0)))
+                                d |= 1u << (2 + 9 * c + (uint32_t)((dy + 1) * 3 + dx + 1));
+                        }
+                    sites[py * cw + px] = d;
+                }
+            a.cfa_w = cw;
+            a.cfa_h = chh;
+            a.sites = static_cast<const uint32_t *>(device_table(mctx, sites.data(), sites.size() * 4ull, &rc));
+            if (rc) return rc;
+        }
+    }
+    if (bayer) {
+        for (int i = 0; i < 4; i++) {
+            uint32_t colour = info->cfa[i];  // pattern position (i & 1, i >> 1) of sensor parity
+            a.chan[i] = colour < 3 ? colour : 3;
+            a.chan_bits |= a.chan[i] << (2 * i);
+        }
     }
     // The walkers' scalar calibration (pattern of at most 2x2, no deltas) and the bound that lets
     // the RGBA8 fast path clamp with v_med3_f32.
@@ -448,8 +582,9 @@ int32_t frh::plan_develop(const frh_context *ctx, const frh_frame *f, const frh_
     chh = chh < h - cy ? chh : h - cy;
     a.crop_x = (uint32_t)cx;
     a.crop_y = (uint32_t)cy;
-    a.dev_w = (uint32_t)cw;
-    a.dev_h = (uint32_t)chh;
+    a.dev_w = (uint32_t)(spec->half_size ? cw / 2 : cw);
+    a.dev_h = (uint32_t)(spec->half_size ? chh / 2 : chh);
+    if (a.dev_w == 0 || a.dev_h == 0) return fail(FRH_ERROR_INVALID_ARGUMENT, "develop region too small");
     a.band_y0 = 0;
     a.band_y1 = a.dev_h;
     memcpy(a.wb, spec->wb, sizeof(a.wb));
@@ -472,9 +607,12 @@ int32_t frh::plan_develop(const frh_context *ctx, const frh_frame *f, const frh_
         const char *rows = getenv("FLASHRAW_BILINEAR_ROWS");
         a.bil_rows = rows ? (uint32_t)atoi(rows) : tuned_rows(ctx, false, a, ctx->bil_rows);
     }
-    if (spec->edge) {
+    // run.rs:545: the edge-aware and AHD demosaics apply to single-component CFA frames at full size
+    if (spec->edge && !spec->half_size && cpp == 1 && has_cfa) {
         const frh_edge_spec *e = spec->edge;
         if (e->size < sizeof(frh_edge_spec)) return fail(FRH_ERROR_INVALID_ARGUMENT, "edge spec size too small");
+        if (!bayer)
+            return fail(FRH_ERROR_NOT_AVAILABLE, "the AHD and edge-aware demosaics take Bayer mosaics on this path");
         if (e->period != 2)
             return fail(FRH_ERROR_NOT_AVAILABLE, "only Bayer (period 2) edge specs are supported by this path");
         if (e->canon) {
@@ -534,7 +672,13 @@ int32_t frh::plan_develop(const frh_context *ctx, const frh_frame *f, const frh_
         p.out_h[k] = oriented && swap ? a.dev_w : a.dev_h;
         p.bytes[k] = (uint64_t)p.out_w[k] * p.out_h[k] * p.ch[k] * bps;
     }
-    p.in_row_bytes = (uint64_t)f->raster_width * 2;
+    if (!a.generic && a.srcf) {
+        // the register walkers read u16 rows; f32 samples take the LDS-tiled kernels
+        a.bil_impl = 1;
+        a.ahd_impl = 1;
+    }
+    p.in_row_bytes = p.in.kind == FRH_INPUT_PACKED ? std::max(p.in.stride, p.in.row_bytes)
+                                                    : row_samples * (p.in.kind == FRH_INPUT_F32 ? 4 : 2);
     return FRH_OK;
 }
 
@@ -591,6 +735,64 @@ extern "C" int32_t frh_develop_output_size(const frh_frame *f, const frh_develop
     return FRH_OK;
 }
 
+// Put the frame's samples where the develop reads them: upload the window's rows of host input
+// into a pooled buffer (*d_in), and unpack packed or linearised input into a pooled u16 window
+// (*d_stage), on `st`. Points p.a.src / p.a.srcf at the result.
+static int32_t stage_frame(frh_context *ctx, const frh_frame *frame, Plan &p, hipStream_t st, void **d_in,
+                           uint64_t *in_bytes, void **d_stage, uint64_t *stage_bytes) {
+    const InputDesc &in = p.in;
+    hipError_t e;
+    const uint8_t *base = static_cast<const uint8_t *>(frame->data);
+    uint64_t first_row = frame->window.y;  // raster row at `base`
+    if (!frame->data_on_device) {
+        *in_bytes = in.upload_bytes ? in.upload_bytes : 4;
+        *d_in = ctx->alloc(*in_bytes, &e);
+        if (!*d_in) return hip_fail(e, "input allocation");
+        if (in.upload_bytes &&
+            (e = hipMemcpyAsync(*d_in, base + in.first_byte, in.upload_bytes, hipMemcpyHostToDevice, st)) != hipSuccess)
+            return hip_fail(e, "upload");
+        base = static_cast<const uint8_t *>(*d_in);
+        first_row = 0;
+    }
+    const uint64_t bps = in.kind == FRH_INPUT_F32 ? 4 : 2;
+    if (!in.staged) {
+        const uint64_t off = (first_row * in.stride + (uint64_t)frame->window.x * in.cpp) * bps;
+        if (in.kind == FRH_INPUT_F32) {
+            p.a.srcf = reinterpret_cast<const float *>(base + off);
+            p.a.src = nullptr;
+        } else {
+            p.a.src = reinterpret_cast<const uint16_t *>(base + off);
+        }
+        return FRH_OK;
+    }
+    InArgs a;
+    memset(&a, 0, sizeof(a));
+    a.kind = (uint32_t)in.kind;
+    a.src = base;
+    a.len = frame->data_on_device ? in.len : in.upload_bytes;
+    a.stride = in.stride;
+    a.row_bytes = (uint32_t)in.row_bytes;
+    a.bits = in.bits;
+    a.x0 = frame->window.x * in.cpp;
+    a.y0 = (uint32_t)first_row;
+    a.n = in.w * in.cpp;
+    a.rows = in.h;
+    if (in.lut) {
+        int32_t rc;
+        a.lut = static_cast<const uint16_t *>(device_table(ctx, frame->linearization, frame->linearization_len * 2ull, &rc));
+        if (rc) return rc;
+        a.lut_len = frame->linearization_len;
+    }
+    *stage_bytes = ((uint64_t)a.n * a.rows * 2 + 3) & ~3ull;
+    *d_stage = ctx->alloc(*stage_bytes, &e);
+    if (!*d_stage) return hip_fail(e, "unpack allocation");
+    if ((e = launch_unpack(a, in.order, static_cast<uint16_t *>(*d_stage), st)) != hipSuccess)
+        return hip_fail(e, "unpack launch");
+    p.a.src = static_cast<const uint16_t *>(*d_stage);
+    p.a.srcf = nullptr;
+    return FRH_OK;
+}
+
 extern "C" int32_t frh_develop_into(frh_context *ctx, const frh_frame *frame, const frh_frame_info *info,
                                     const frh_develop_spec *spec, void *dst, uint64_t dst_bytes, void *dst2,
                                     uint64_t dst2_bytes, void *stream) {
@@ -603,12 +805,21 @@ extern "C" int32_t frh_develop_into(frh_context *ctx, const frh_frame *frame, co
         if (!dst || dst_bytes < p.bytes[0]) return fail(FRH_ERROR_INVALID_ARGUMENT, "output buffer too small");
         if (spec->output2 != OUT_NONE && (!dst2 || dst2_bytes < p.bytes[1]))
             return fail(FRH_ERROR_INVALID_ARGUMENT, "second output buffer too small");
-        if ((p.a.n_rowd || p.a.n_cold))
-            return fail(FRH_ERROR_NOT_AVAILABLE, "black-level deltas need frh_run_develop (host tables)");
         p.a.dst = dst;
         p.a.dst2 = dst2;
         (void)hipSetDevice(ctx->device);
-        return launch_bands(ctx, p, (hipStream_t)stream, nullptr);
+        hipStream_t st = (hipStream_t)stream;
+        void *d_in = nullptr, *d_stage = nullptr;
+        uint64_t in_bytes = 0, stage_bytes = 0;
+        rc = stage_frame(ctx, frame, p, st, &d_in, &in_bytes, &d_stage, &stage_bytes);
+        if (rc == FRH_OK) rc = launch_bands(ctx, p, st, nullptr);
+        if (d_stage) {
+            // the unpacked window is pooled scratch: wait for the develop before returning it
+            hipError_t e = hipStreamSynchronize(st);
+            ctx->release(d_stage, stage_bytes);
+            if (rc == FRH_OK && e != hipSuccess) rc = hip_fail(e, "develop");
+        }
+        return rc;
     } catch (...) {
         return fail(FRH_ERROR_INTERNAL, "unexpected exception");
     }
@@ -621,11 +832,6 @@ struct frh_plan {
     frh_context *ctx;
     Plan p;
     size_t src_off;  // samples from the raster start to the window origin
-    void *d_rowd = nullptr, *d_cold = nullptr;
-    ~frh_plan() {
-        if (d_rowd) (void)hipFree(d_rowd);
-        if (d_cold) (void)hipFree(d_cold);
-    }
 };
 
 extern "C" void *frh_context_stream(const frh_context *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
@@ -639,23 +845,10 @@ extern "C" int32_t frh_plan_create(frh_context *ctx, const frh_frame *frame, con
         pl->ctx = ctx;
         int32_t rc = plan_develop(ctx, frame, info, spec, pl->p);
         if (rc) return rc;
-        pl->src_off = (size_t)frame->window.y * frame->raster_width + frame->window.x;
-        (void)hipSetDevice(ctx->device);
-        hipError_t e;
-        if (pl->p.a.n_rowd) {
-            if ((e = hipMalloc(&pl->d_rowd, pl->p.a.n_rowd * 4)) != hipSuccess) return hip_fail(e, "hipMalloc");
-            if ((e = hipMemcpy(pl->d_rowd, info->black_row_deltas, pl->p.a.n_rowd * 4, hipMemcpyHostToDevice)) !=
-                hipSuccess)
-                return hip_fail(e, "upload");
-            pl->p.a.rowd = static_cast<const float *>(pl->d_rowd);
-        }
-        if (pl->p.a.n_cold) {
-            if ((e = hipMalloc(&pl->d_cold, pl->p.a.n_cold * 4)) != hipSuccess) return hip_fail(e, "hipMalloc");
-            if ((e = hipMemcpy(pl->d_cold, info->black_col_deltas, pl->p.a.n_cold * 4, hipMemcpyHostToDevice)) !=
-                hipSuccess)
-                return hip_fail(e, "upload");
-            pl->p.a.cold = static_cast<const float *>(pl->d_cold);
-        }
+        if (pl->p.in.staged)
+            return fail(FRH_ERROR_NOT_AVAILABLE,
+                        "prepared plans take u16 or f32 samples: unpack packed or linearised input first (frh_unpack_into)");
+        pl->src_off = (size_t)frame->window.y * frame->raster_width * pl->p.in.cpp + (size_t)frame->window.x * pl->p.in.cpp;
         *out = pl.release();
         return FRH_OK;
     } catch (const std::bad_alloc &) {
@@ -669,7 +862,10 @@ extern "C" int32_t frh_plan_launch(const frh_plan *plan, const void *src, void *
     if (!plan || !src || !dst) return fail(FRH_ERROR_INVALID_ARGUMENT, "NULL argument");
     if (plan->p.a.out_mode2 != OUT_NONE && !dst2) return fail(FRH_ERROR_INVALID_ARGUMENT, "second output is NULL");
     Plan p = plan->p;
-    p.a.src = static_cast<const uint16_t *>(src) + plan->src_off;
+    if (p.in.kind == FRH_INPUT_F32)
+        p.a.srcf = static_cast<const float *>(src) + plan->src_off;
+    else
+        p.a.src = static_cast<const uint16_t *>(src) + plan->src_off;
     p.a.dst = dst;
     p.a.dst2 = dst2;
     return launch_bands(plan->ctx, p, (hipStream_t)stream, nullptr);
@@ -705,7 +901,10 @@ extern "C" int32_t frh_plan_autotune(frh_plan *plan, const void *src, void *dst,
                 p.a.ahd_rows = r;
             else
                 p.a.bil_rows = r;
-            p.a.src = static_cast<const uint16_t *>(src) + plan->src_off;
+            if (p.in.kind == FRH_INPUT_F32)
+                p.a.srcf = static_cast<const float *>(src) + plan->src_off;
+            else
+                p.a.src = static_cast<const uint16_t *>(src) + plan->src_off;
             p.a.dst = dst;
             p.a.dst2 = dst2;
             if ((rc = launch_bands(ctx, p, st, nullptr)) != FRH_OK) break;  // warm
@@ -767,42 +966,18 @@ extern "C" int32_t frh_run_develop(frh_context *ctx, const frh_frame *frame, con
         (void)hipSetDevice(ctx->device);
         hipError_t e;
         auto t0 = std::chrono::steady_clock::now();
-        // Upload: the window's rows of the raster (run.rs:915-923), unless resident.
-        void *d_in = nullptr;
-        uint64_t in_bytes = 0;
-        std::vector<void *> scratch;
+        // Upload: the window's rows of the raster (run.rs:915-923), unless resident; packed and
+        // linearised input is unpacked to u16 on the device first.
+        void *d_in = nullptr, *d_stage = nullptr;
+        uint64_t in_bytes = 0, stage_bytes = 0;
         auto cleanup = [&]() {
             if (d_in) ctx->release(d_in, in_bytes);
-            for (void *s : scratch) (void)hipFree(s);
+            if (d_stage) ctx->release(d_stage, stage_bytes);
         };
-        if (!frame->data_on_device) {
-            const uint64_t row = (uint64_t)frame->raster_width;
-            in_bytes = row * frame->window.height * 2;
-            d_in = ctx->alloc(in_bytes, &e);
-            if (!d_in) return hip_fail(e, "input allocation");
-            const uint16_t *src = static_cast<const uint16_t *>(frame->data) + (size_t)frame->window.y * row;
-            e = hipMemcpyAsync(d_in, src, in_bytes, hipMemcpyHostToDevice, ctx->stream);
-            if (e != hipSuccess) {
-                cleanup();
-                return hip_fail(e, "upload");
-            }
-            p.a.src = static_cast<const uint16_t *>(d_in) + frame->window.x;
-        }
-        if (p.a.n_rowd) {
-            void *d = nullptr;
-            if ((e = hipMalloc(&d, p.a.n_rowd * 4)) != hipSuccess) { cleanup(); return hip_fail(e, "hipMalloc"); }
-            scratch.push_back(d);
-            if ((e = hipMemcpyAsync(d, info->black_row_deltas, p.a.n_rowd * 4, hipMemcpyHostToDevice, ctx->stream)) !=
-                hipSuccess) { cleanup(); return hip_fail(e, "upload"); }
-            p.a.rowd = static_cast<const float *>(d);
-        }
-        if (p.a.n_cold) {
-            void *d = nullptr;
-            if ((e = hipMalloc(&d, p.a.n_cold * 4)) != hipSuccess) { cleanup(); return hip_fail(e, "hipMalloc"); }
-            scratch.push_back(d);
-            if ((e = hipMemcpyAsync(d, info->black_col_deltas, p.a.n_cold * 4, hipMemcpyHostToDevice, ctx->stream)) !=
-                hipSuccess) { cleanup(); return hip_fail(e, "upload"); }
-            p.a.cold = static_cast<const float *>(d);
+        rc = stage_frame(ctx, frame, p, ctx->stream, &d_in, &in_bytes, &d_stage, &stage_bytes);
+        if (rc) {
+            cleanup();
+            return rc;
         }
         if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) {
             cleanup();

@@ -320,6 +320,8 @@ extern "C" int32_t frh_batch_create(frh_context *ctx, uint32_t n, const frh_fram
                 return fail(FRH_ERROR_NOT_AVAILABLE,
                             "batched frames need a black level of at most a 2x2 pattern without row/column deltas");
             if (p.a.edge_aware) return fail(FRH_ERROR_NOT_AVAILABLE, "batched frames use bilinear or AHD");
+            if (p.a.generic || p.in.kind != FRH_INPUT_U16 || p.in.staged)
+                return fail(FRH_ERROR_NOT_AVAILABLE, "batched frames are full-size u16 Bayer mosaics");
             if (specs[i].output2 != OUT_NONE) return fail(FRH_ERROR_NOT_AVAILABLE, "batched frames have one output");
             if (!dsts[i]) return fail(FRH_ERROR_INVALID_ARGUMENT, "NULL output");
             p.a.dst = dsts[i];

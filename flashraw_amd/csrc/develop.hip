@@ -291,6 +291,7 @@ static uint32_t persistent_grid(uint32_t ntiles, int num_cus, int blocks_per_cu)
 }
 
 hipError_t launch_develop(const DevArgs &a, bool ahd, hipStream_t stream, int num_cus) {
+    if (a.generic) return launch_develop_generic(a, stream);
     if (a.edge_aware) return launch_develop_edge(a, stream);
     const uint32_t rows = a.band_y1 - a.band_y0;
     if (rows == 0 || a.dev_w == 0) return hipSuccess;

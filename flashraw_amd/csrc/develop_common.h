@@ -41,7 +41,8 @@ __device__ __forceinline__ float wb_of(const DevArgs &a, uint32_t c) {
 // Calib::norm (process.rs:288-298) of frame sample (x, y), in bounds.
 __device__ __forceinline__ float norm_raw(const DevArgs &a, const Lds &s, float v, uint32_t x, uint32_t y);
 __device__ __forceinline__ float norm_at(const DevArgs &a, const Lds &s, uint32_t x, uint32_t y) {
-    return norm_raw(a, s, (float)a.src[(size_t)y * a.stride + x], x, y);
+    const size_t i = (size_t)y * a.stride + x;
+    return norm_raw(a, s, a.srcf ? a.srcf[i] : (float)a.src[i], x, y);
 }
 
 // finish_rgb with Highlights::Clip (process.rs:654-707). `balanced` pixels come from the AHD

@@ -96,6 +96,42 @@ struct DevArgs {
     // The inputs bound every intermediate value of the edge-aware demosaic far from overflow
     // (see edge_tables in api.cpp): dense taps and uniform colour masks give the same bits.
     uint32_t edge_tame;
+    // Frames beyond a u16 Bayer mosaic. srcf (when not NULL) holds f32 raw values in place of src;
+    // cpp components per pixel (stride counts samples). The generic kernel develops every frame
+    // that is not a single-component Bayer mosaic at full size: mode 0 grey from component 0,
+    // 1 a single-component CFA of cfa_w x cfa_h (site descriptors `sites`), 2 RGB from
+    // components 0..2; `half` averages 2x2 blocks. gcal holds black[cpp][cal_pw * cal_ph] then
+    // scale[cpp][cal_pw * cal_ph].
+    const float *srcf;
+    uint32_t cpp;
+    uint32_t generic;
+    uint32_t gmode;
+    uint32_t half;
+    const float *gcal;
+    const uint32_t *sites;
+    uint32_t cfa_w, cfa_h;
+};
+
+// Window of a raster for the input kernels (unpack, linearise, normalise): packed payload bytes
+// (kind 2) or u16 / f32 samples (kinds 0 / 1), the window's first sample (x0 counts samples,
+// x * cpp) and row, and `n` samples per window row over `rows` rows.
+struct InArgs {
+    const void *src;
+    uint64_t len;        // packed: payload bytes
+    uint64_t stride;     // packed: bytes per raster row; u16 / f32: samples per raster row
+    uint32_t row_bytes;  // packed: bytes of sample data per raster row (bytes past it read 0)
+    uint32_t bits;       // packed: 1..16
+    uint32_t kind;
+    uint32_t x0, y0;
+    uint32_t n, rows;
+    const uint16_t *lut;  // linearisation of integer samples, NULL for none
+    uint32_t lut_len;
+    // normalise: origin of the window, component count and calibration (as DevArgs)
+    uint32_t ox, oy, cpp;
+    uint32_t cal_pw, cal_ph;
+    const float *gcal;
+    const float *rowd, *cold;
+    uint32_t n_rowd, n_cold;
 };
 
 // Rust `x as u32` for x below 2^32: truncation, with NaN and everything below 1 giving 0.

@@ -45,6 +45,9 @@ struct frh_context {
     std::map<std::tuple<int, uint32_t, uint32_t, int>, uint32_t> tuned;
     // device copies of edge-aware tap lists, by a hash of the tables they were built from
     std::map<uint64_t, void *> edge_tabs;
+    // device copies of small per-frame tables (calibration, CFA sites, black-level deltas,
+    // linearisation curves), by (hash of the contents, bytes); kept until the context ends
+    std::map<std::pair<uint64_t, uint64_t>, void *> tables;
     std::mutex mu;
     struct Block {
         void *ptr;
@@ -62,6 +65,30 @@ namespace frh {
 
 // A validated develop: kernel arguments (without the buffers a launch names) and the outputs'
 // geometry (run.rs:439-760).
+// A validated input (GpuInput + window, run.rs:456-490): what to upload and how to read it.
+struct InputDesc {
+    int32_t kind;          // FRH_INPUT_*
+    uint32_t bits;         // packed
+    int32_t order;         // packed
+    uint64_t stride;       // packed: bytes per raster row; u16 / f32: samples per raster row
+    uint64_t row_bytes;    // packed: bytes of sample data per raster row
+    uint64_t len;          // packed: payload bytes; u16 / f32: samples
+    uint32_t w, h, cpp;    // window
+    bool lut;              // a linearisation curve applies (integer input with a non-empty table)
+    bool staged;           // the develop reads u16 samples unpacked / linearised first
+    // bytes of the raster rows the window needs (uploaded for host input), from `first_byte`
+    uint64_t first_byte, upload_bytes;
+};
+
+// Validate the frame's input description (status code; the message is the thread's last error).
+int32_t check_input(const frh_frame *f, InputDesc &in);
+
+// Calibration tables of every component: black[cpp][pw * ph] then scale[cpp][pw * ph].
+int32_t build_calib(const frh_frame_info *info, uint32_t cpp, std::vector<float> &gcal, uint32_t &pw, uint32_t &ph);
+
+// A device copy of `bytes` bytes at `host`, cached by the context (NULL with *rc set on failure).
+const void *device_table(frh_context *ctx, const void *host, uint64_t bytes, int32_t *rc);
+
 struct Plan {
     DevArgs a;
     bool ahd;
@@ -69,6 +96,7 @@ struct Plan {
     uint64_t bytes[2];
     int32_t fmt[2];
     uint64_t in_row_bytes;
+    InputDesc in;
 };
 
 // Validate a develop request and fill `p` (status code; the message is the thread's last error).

@@ -16,4 +16,15 @@ hipError_t launch_develop_edge(const DevArgs &a, hipStream_t stream);
 // The AHD row walker over rows [a.band_y0, a.band_y1), `rows_per_strip` rows per wave.
 hipError_t launch_develop_ahd_walk(const DevArgs &a, hipStream_t stream, int rows_per_strip);
 
+// Unpack / re-window (with the linearisation curve) the window of `in` into u16 samples at `dst`
+// (in.n * in.rows samples, 4-byte aligned); PackOrder `order` for packed payloads.
+hipError_t launch_unpack(const InArgs &in, int order, uint16_t *dst, hipStream_t stream);
+
+// Normalised samples max((v - black) * scale, 0) of the window of `in` (u16 or f32) into `dst`.
+hipError_t launch_normalize(const InArgs &in, float *dst, hipStream_t stream);
+
+// The generic develop (any CFA, no CFA, several components, half size) over rows
+// [a.band_y0, a.band_y1).
+hipError_t launch_develop_generic(const DevArgs &a, hipStream_t stream);
+
 }  // namespace frh

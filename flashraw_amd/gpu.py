@@ -77,6 +77,7 @@ class _Frame(ctypes.Structure):
         ("raster_width", ctypes.c_uint32), ("raster_height", ctypes.c_uint32), ("cpp", ctypes.c_uint32),
         ("window", _Rect), ("origin_x", ctypes.c_uint32), ("origin_y", ctypes.c_uint32),
         ("linearization", ctypes.c_void_p), ("linearization_len", ctypes.c_uint32),
+        ("bits", ctypes.c_uint32), ("order", ctypes.c_int32), ("row_stride", ctypes.c_uint64),
     ]
 
 
@@ -147,6 +148,15 @@ def lib():
                                           ctypes.POINTER(_DevelopSpec), ctypes.POINTER(vp), ctypes.POINTER(vp),
                                           ctypes.POINTER(_Timings)]
             L.frh_run_develop.restype = i32
+            L.frh_run_unpack.argtypes = [vp, ctypes.POINTER(_Frame), ctypes.POINTER(vp), ctypes.POINTER(_Timings)]
+            L.frh_run_unpack.restype = i32
+            L.frh_run_normalize.argtypes = [vp, ctypes.POINTER(_Frame), ctypes.POINTER(_FrameInfo), ctypes.POINTER(vp),
+                                            ctypes.POINTER(_Timings)]
+            L.frh_run_normalize.restype = i32
+            L.frh_unpack_into.argtypes = [vp, ctypes.POINTER(_Frame), vp, u64, vp]
+            L.frh_unpack_into.restype = i32
+            L.frh_normalize_into.argtypes = [vp, ctypes.POINTER(_Frame), ctypes.POINTER(_FrameInfo), vp, u64, vp]
+            L.frh_normalize_into.restype = i32
             L.frh_develop_into.argtypes = [vp, ctypes.POINTER(_Frame), ctypes.POINTER(_FrameInfo),
                                            ctypes.POINTER(_DevelopSpec), vp, u64, vp, u64, vp]
             L.frh_develop_into.restype = i32
@@ -247,6 +257,14 @@ class CfaPattern:
     def bayer(name: str) -> "CfaPattern":
         m = {"R": 0, "G": 1, "B": 2}
         return CfaPattern(2, 2, tuple(m[c] for c in name))
+
+    @staticmethod
+    def parse(rows: str) -> "CfaPattern":
+        """Rows of colour letters separated by '/', e.g. the X-Trans
+        "GGRGGB/GGBGGR/BRGRBG/GGBGGR/GGRGGB/RBGBRG" (CfaPattern::parse, types.rs)."""
+        m = {"R": 0, "G": 1, "B": 2, "C": 3, "M": 4, "Y": 5, "W": 6, "E": 7}
+        rr = rows.split("/")
+        return CfaPattern(len(rr[0]), len(rr), tuple(m[c] for r in rr for c in r))
 
 
 CfaPattern.RGGB = CfaPattern.bayer("RGGB")
@@ -579,11 +597,26 @@ class _ServerStats(ctypes.Structure):
 
 
 class GpuOp:
-    """GpuOp (run.rs:219-225); only Develop is accepted by this path."""
+    """GpuOp (run.rs:219-225): Develop, Unpack (the window as u16 samples, packed payloads unpacked,
+    integer samples linearised) or Normalize (max((v - black) * scale, 0) as f32)."""
 
     @dataclass
     class Develop:
         spec: DevelopSpec
+
+    class Unpack:
+        pass
+
+    class Normalize:
+        pass
+
+
+class PackOrder(enum.IntEnum):
+    """PackOrder (crates/flashraw-core/src/decoder.rs:203-212)."""
+    MsbBytes = 0
+    LsbBytes = 1
+    Msb16 = 2
+    Msb32 = 3
 
 
 def _device_ptr(t):
@@ -595,8 +628,9 @@ def _device_ptr(t):
 
 @dataclass
 class GpuFrame:
-    """GpuFrame (run.rs:27-93). `data` is a numpy u16 array (host) or a torch tensor / int
-    device address (`on_device=True`, resident input)."""
+    """GpuFrame (run.rs:27-93). `data` is a numpy array (host) or a torch tensor / int device
+    address (`on_device=True`, resident input): u16 or f32 samples (`input` "u16" / "f32"), or a
+    packed payload of bytes (`input` "packed", with `bits`, `order` and `row_stride`)."""
     data: object
     raster_width: int
     raster_height: int
@@ -605,11 +639,32 @@ class GpuFrame:
     origin: tuple = (0, 0)
     on_device: bool = False
     linearization: object = None
+    input: str = "u16"
+    bits: int = 0
+    order: PackOrder = PackOrder.MsbBytes
+    row_stride: int = 0
+    data_len: Optional[int] = None
 
     @staticmethod
     def from_u16(data, width: int, height: int, cpp: int = 1) -> "GpuFrame":
         on_dev = not isinstance(data, np.ndarray)
         return GpuFrame(data, width, height, cpp, Rect.full(width, height), (0, 0), on_dev)
+
+    @staticmethod
+    def from_f32(data, width: int, height: int, cpp: int = 1) -> "GpuFrame":
+        on_dev = not isinstance(data, np.ndarray)
+        return GpuFrame(data, width, height, cpp, Rect.full(width, height), (0, 0), on_dev, input="f32")
+
+    @staticmethod
+    def from_packed(data, width: int, height: int, bits: int, order=PackOrder.MsbBytes, row_stride=None,
+                    data_len=None) -> "GpuFrame":
+        """GpuFrame::from_packed (run.rs:43-53): a single-component payload of `bits`-bit samples;
+        rows `row_stride` bytes apart (tight by default). `data_len` (bytes) is needed for device
+        payloads given as an address."""
+        on_dev = not isinstance(data, np.ndarray)
+        stride = row_stride if row_stride is not None else (width * bits + 7) // 8
+        return GpuFrame(data, width, height, 1, Rect.full(width, height), (0, 0), on_dev, input="packed", bits=bits,
+                        order=PackOrder(order), row_stride=stride, data_len=data_len)
 
     def with_window(self, r: Rect) -> "GpuFrame":
         w = self.window or Rect.full(self.raster_width, self.raster_height)
@@ -621,18 +676,33 @@ class GpuFrame:
         self.origin = tuple(origin)
         return self
 
+    def with_linearization(self, lut) -> "GpuFrame":
+        """GpuFrame::with_linearization (run.rs:84-87): integer samples v -> lut[min(v, len - 1)];
+        None or an empty table for none."""
+        self.linearization = None if lut is None or len(lut) == 0 else np.ascontiguousarray(lut, dtype=np.uint16)
+        return self
+
     def _c(self):
         f = _Frame()
         f.size = ctypes.sizeof(_Frame)
-        f.input = 0
+        f.input = {"u16": 0, "f32": 1, "packed": 2}[self.input]
         keep = []
-        n = self.raster_width * self.raster_height * self.cpp
+        if self.input == "packed":
+            dtype = np.uint8
+            n = self.data_len
+        else:
+            dtype = np.float32 if self.input == "f32" else np.uint16
+            n = self.raster_width * self.raster_height * self.cpp
         if self.on_device:
             f.data = _device_ptr(self.data)
             f.data_on_device = 1
+            if n is None:
+                n = self.data.numel() * self.data.element_size()
             f.data_len = n
         else:
-            a = np.ascontiguousarray(self.data, dtype=np.uint16)
+            a = np.ascontiguousarray(self.data)
+            if a.dtype != dtype:
+                a = a.view(dtype) if a.dtype.itemsize == np.dtype(dtype).itemsize else a.astype(dtype)
             keep.append(a)
             f.data = a.ctypes.data
             f.data_len = a.size
@@ -640,8 +710,11 @@ class GpuFrame:
         w = self.window or Rect.full(self.raster_width, self.raster_height)
         f.window = _Rect(w.x, w.y, w.width, w.height)
         f.origin_x, f.origin_y = self.origin
+        f.bits, f.order, f.row_stride = self.bits, int(self.order), self.row_stride
         if self.linearization is not None:
-            f.linearization = 1
+            lut = np.ascontiguousarray(self.linearization, dtype=np.uint16)
+            keep.append(lut)
+            f.linearization, f.linearization_len = lut.ctypes.data, lut.size
         return f, keep
 
 
@@ -729,9 +802,25 @@ class GpuContext:
         check(lib().frh_set_limits(self._h, max_binding_bytes))
 
     def run(self, frame: GpuFrame, info: FrameInfo, op) -> GpuRun:
-        """GpuContext::run (run.rs:426) with GpuOp::Develop; blocking (run_blocking, run.rs:1076)."""
+        """GpuContext::run (run.rs:426); blocking (run_blocking, run.rs:1076)."""
+        if op is GpuOp.Unpack or isinstance(op, GpuOp.Unpack) or op is GpuOp.Normalize or \
+                isinstance(op, GpuOp.Normalize):
+            normalize = op is GpuOp.Normalize or isinstance(op, GpuOp.Normalize)
+            f, k1 = frame._c()
+            out = ctypes.c_void_p()
+            t = _Timings()
+            if normalize:
+                i, k2 = info._c()
+                check(lib().frh_run_normalize(self._h, ctypes.byref(f), ctypes.byref(i), ctypes.byref(out),
+                                              ctypes.byref(t)))
+            else:
+                k2 = None
+                check(lib().frh_run_unpack(self._h, ctypes.byref(f), ctypes.byref(out), ctypes.byref(t)))
+            del k1, k2
+            timings = GpuTimings(t.upload_ms, t.compute_ms, t.kernel_ms if t.has_kernel_ms else None, t.bands)
+            return GpuRun(GpuImage(out.value), timings, None)
         if not isinstance(op, GpuOp.Develop):
-            raise NotAvailable(9, "only GpuOp.Develop is supported by this path")
+            raise InvalidArgument(10, "unknown GpuOp")
         f, k1 = frame._c()
         i, k2 = info._c()
         s, k3 = op.spec._c()
@@ -768,6 +857,22 @@ class GpuContext:
         check(lib().frh_develop_into(self._h, ctypes.byref(f), ctypes.byref(i), ctypes.byref(s),
                                      _device_ptr(dst), n1, None if dst2 is None else _device_ptr(dst2), n2, sp))
         del k1, k2, k3
+
+    def unpack_into(self, frame: GpuFrame, dst, stream=None):
+        """GpuOp::Unpack of a device-resident frame into the caller's device buffer (u16 samples)."""
+        f, k1 = frame._c()
+        n = dst.numel() * dst.element_size() if hasattr(dst, "numel") else (1 << 62)
+        check(lib().frh_unpack_into(self._h, ctypes.byref(f), _device_ptr(dst), n, self.stream_handle(stream)))
+        del k1
+
+    def normalize_into(self, frame: GpuFrame, info: FrameInfo, dst, stream=None):
+        """GpuOp::Normalize of a device-resident frame into the caller's device buffer (f32)."""
+        f, k1 = frame._c()
+        i, k2 = info._c()
+        n = dst.numel() * dst.element_size() if hasattr(dst, "numel") else (1 << 62)
+        check(lib().frh_normalize_into(self._h, ctypes.byref(f), ctypes.byref(i), _device_ptr(dst), n,
+                                       self.stream_handle(stream)))
+        del k1, k2
 
     def batch(self, jobs) -> "Batch":
         """A batched develop of `jobs`, a sequence of (frame, info, spec, dst): device-resident

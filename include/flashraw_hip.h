@@ -38,22 +38,34 @@ typedef struct frh_rect {
     uint32_t x, y, width, height;
 } frh_rect;
 
-/* GpuInput (crates/flashraw-gpu/src/run.rs:11-24). Packed payloads are not accepted yet. */
-enum frh_input_kind { FRH_INPUT_U16 = 0, FRH_INPUT_F32 = 1 };
+/* GpuInput (crates/flashraw-gpu/src/run.rs:11-24): decoded u16 or f32 samples, or a uniformly
+ * bit-packed payload unpacked on the device. */
+enum frh_input_kind { FRH_INPUT_U16 = 0, FRH_INPUT_F32 = 1, FRH_INPUT_PACKED = 2 };
 
-/* GpuFrame (run.rs:27-40). `data` holds raster_width * cpp samples per row; when
- * `data_on_device` is 1 it is a device pointer and nothing is uploaded (resident input). */
+/* PackOrder (crates/flashraw-core/src/decoder.rs:203-212). */
+enum frh_pack_order { FRH_PACK_MSB_BYTES = 0, FRH_PACK_LSB_BYTES = 1, FRH_PACK_MSB16 = 2, FRH_PACK_MSB32 = 3 };
+
+/* GpuFrame (run.rs:27-40). u16 / f32 input: `data` holds raster_width * cpp samples per row and
+ * `data_len` counts samples. Packed input: `data` holds the payload from its first row,
+ * `data_len` counts bytes (bytes past it read as 0), rows start `row_stride` bytes apart and hold
+ * ceil(raster_width * cpp * bits / 8) bytes of samples of `bits` (1..16) bits in PackOrder
+ * `order`. When `data_on_device` is 1, `data` is a device pointer and nothing is uploaded
+ * (resident input). `linearization` (NULL or empty for none) maps integer samples
+ * v -> table[min(v, len - 1)]. */
 typedef struct frh_frame {
     uint32_t size;
     int32_t input;
     const void *data;
-    uint64_t data_len; /* samples */
+    uint64_t data_len;
     int32_t data_on_device;
     uint32_t raster_width, raster_height, cpp;
     frh_rect window;
     uint32_t origin_x, origin_y;
-    const uint16_t *linearization; /* must be NULL (not supported yet) */
+    const uint16_t *linearization;
     uint32_t linearization_len;
+    uint32_t bits;
+    int32_t order;
+    uint64_t row_stride;
 } frh_frame;
 
 /* The FrameInfo fields the develop path reads (crates/flashraw-core/src/decoder.rs:25-74;
@@ -147,6 +159,25 @@ int32_t frh_context_info(const frh_context *ctx, char *buf, size_t len);
 int32_t frh_set_limits(frh_context *ctx, uint64_t max_binding_bytes);
 /* The message of the calling thread's last error (empty when none). */
 const char *frh_last_error_message(void);
+
+/* ---- input operations (run.rs:426 GpuContext::run with GpuOp::Unpack / GpuOp::Normalize,
+ * run.rs:503-517) ---- */
+
+/* GpuOp::Unpack: the frame's window as u16 samples (window.width * cpp per row), packed
+ * payloads unpacked and integer samples passed through the linearisation curve; an image of
+ * format U16 with `cpp` channels. f32 input is FRH_ERROR_INVALID_ARGUMENT. */
+int32_t frh_run_unpack(frh_context *ctx, const frh_frame *frame, frh_image **out, frh_timings *timings);
+/* GpuOp::Normalize: max((v - black) * scale, 0) of every sample of the window (Calib of each
+ * component, row and column deltas; normalize.wgsl), an image of format F32 with `cpp`
+ * channels. */
+int32_t frh_run_normalize(frh_context *ctx, const frh_frame *frame, const frh_frame_info *info, frh_image **out,
+                          frh_timings *timings);
+/* The same on `stream` for device-resident input into the caller's device buffer `dst` (4-byte
+ * aligned for unpack). Normalising packed or linearised input unpacks into pooled scratch and
+ * synchronises `stream` before returning it. */
+int32_t frh_unpack_into(frh_context *ctx, const frh_frame *frame, void *dst, uint64_t dst_bytes, void *stream);
+int32_t frh_normalize_into(frh_context *ctx, const frh_frame *frame, const frh_frame_info *info, void *dst,
+                           uint64_t dst_bytes, void *stream);
 
 /* ---- develop (run.rs:426 GpuContext::run with GpuOp::Develop; run_blocking run.rs:1076) ---- */
 
