@@ -33,7 +33,8 @@ def _parse():
     p.add_argument("--demosaic", default="ahd", choices=["ahd", "bilinear", "edge"])
     p.add_argument("--beta", type=float, default=0.0,
                    help="edge demosaic: weight of the gated smoothing (0 for clean frames)")
-    p.add_argument("--mode", default="develop", choices=["develop", "denoise", "batch", "serve", "copy"])
+    p.add_argument("--mode", default="develop", choices=["develop", "denoise", "batch", "serve", "copy", "unpack"])
+    p.add_argument("--bits", type=int, default=14, help="unpack mode: bits per packed sample")
     p.add_argument("--frames", type=int, default=64, help="batch mode: frames in the batch")
     p.add_argument("--rates", default="10,50,100,200", help="serve mode: request rates (req/s) to sweep")
     p.add_argument("--duration", type=float, default=15.0, help="serve mode: seconds of arrivals per rate")
@@ -254,6 +255,60 @@ def batch_mode(args):
 N_SERVE_MOSAICS = 32
 
 
+def unpack_mode(args):
+    """GpuOp::Unpack of the 6048x4024 frame as a resident MSB-first packed payload of --bits bits
+    per sample (tight rows) into u16 samples. A step is one unpack of the frame; algorithmic bytes
+    are the payload plus 2 B/px out."""
+    import numpy as np
+    import torch
+
+    import flashraw_amd as F
+    from flashraw_amd import synth
+
+    ctx = F.context()
+    W, H, bits = 6048, 4024, args.bits
+    mos = synth.frame_24mp() >> max(0, 14 - bits)
+    stride = (W * bits + 7) // 8
+    # MSB-first bit stream of each row
+    v = mos.astype(np.uint64)
+    bitsarr = ((v[:, :, None] >> np.arange(bits - 1, -1, -1, dtype=np.uint64)) & 1).astype(np.uint8).reshape(H, -1)
+    pad = stride * 8 - W * bits
+    if pad:
+        bitsarr = np.concatenate([bitsarr, np.zeros((H, pad), np.uint8)], axis=1)
+    payload = np.packbits(bitsarr, axis=1).reshape(-1)
+    d_src = torch.from_numpy(payload).to("cuda")
+    dst = torch.empty((H, W), dtype=torch.int16, device="cuda")
+    frame = F.GpuFrame.from_packed(d_src, W, H, bits, F.PackOrder.MsbBytes, stride, data_len=payload.size)
+    stream = torch.cuda.Stream()
+    for _ in range(args.warmup):
+        ctx.unpack_into(frame, dst, stream=stream)
+    torch.cuda.synchronize()
+    ok = bool(np.array_equal(dst.cpu().numpy().view(np.uint16), mos))
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for s_, e_ in evs:
+        s_.record(stream)
+        ctx.unpack_into(frame, dst, stream=stream)
+        e_.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kms = [a.elapsed_time(b) for a, b in evs]
+    k_avg = statistics.mean(kms)
+    algo = payload.size + W * H * 2
+    mp = W * H / 1e6
+    print(json.dumps({
+        "metric": "MP/s packed unpack (24 MP, % HBM roofline)", "value": round(mp / (wall / args.steps), 1),
+        "unit": "MP/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3 / args.steps, 5), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u16", "data": "synthetic",
+        "config": {"workload": f"6048x4024 {bits}-bit MSB-first packed payload -> u16, resident in HBM",
+                   "bit_exact_vs_source": ok},
+        "roofline": {"bound": "hbm", "achieved": round(algo / (k_avg / 1e3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(algo / (k_avg / 1e3) / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel_ms_avg": round(k_avg, 5), "algorithmic_bytes_per_launch": algo},
+        "cpu_baseline": None}))
+
+
 def serve_mosaic_shapes(seed=5):
     """Sizes of the pooled request mosaics: log-uniform in 12-102 MP at 3:2 or 4:3 (sides
     multiples of 8), a Bayer phase and a bit depth each."""
@@ -444,6 +499,8 @@ def main():
         return batch_mode(args)
     if args.mode == "serve":
         return serve_mode(args)
+    if args.mode == "unpack":
+        return unpack_mode(args)
 
     import numpy as np
     import torch
