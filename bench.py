@@ -555,6 +555,25 @@ def main():
     achieved = algo_bytes / (k_avg / 1e3) / 1e9
     got = dst.cpu().numpy()
 
+    # Host-buffer rate (not the metric): the same develop from a host mosaic through
+    # frh_run_develop (upload of the window's rows, develop, device image) and the download.
+    pinned_out = F.pinned_empty((H, W, 4), np.uint8)
+    host = []
+    for i in range(4):
+        t = time.perf_counter()
+        r = ctx.run(F.GpuFrame.from_u16(mos, W, H), info, F.GpuOp.Develop(spec))
+        t1 = time.perf_counter()
+        r.image.download_into(pinned_out)
+        t2 = time.perf_counter()
+        if i:
+            host.append((r.timings.upload_ms, r.timings.compute_ms, (t1 - t) * 1e3, (t2 - t1) * 1e3))
+    host_med = [statistics.median(x[k] for x in host) for k in range(4)]
+    host_rate = {"upload_ms": round(host_med[0], 3), "compute_ms": round(host_med[1], 3),
+                 "run_ms": round(host_med[2], 3), "download_ms": round(host_med[3], 3),
+                 "mp_per_s_upload_develop": round(mp / (host_med[2] / 1e3), 1),
+                 "mp_per_s_with_download": round(mp / ((host_med[2] + host_med[3]) / 1e3), 1),
+                 "note": "pageable host mosaic in, pinned host RGBA8 out; median of 3 after 1 warm-up"}
+
     cpu = None
     max_diff = None
     if not args.no_cpu:
@@ -599,6 +618,7 @@ def main():
                      "kernel_ms_avg": round(k_avg, 5), "kernel_ms_p50": round(statistics.median(kms), 5),
                      "algorithmic_bytes_per_launch": algo_bytes},
         "cpu_baseline": cpu,
+        "host_buffer_rate": host_rate,
     }
     print(json.dumps(line))
 
