@@ -468,7 +468,6 @@ int32_t frh::plan_develop(const frh_context *ctx, const frh_frame *f, const frh_
         for (int i = 0; i < 4; i++) cnt[info->cfa[i]]++;
         bayer = cnt[0] == 1 && cnt[1] == 2 && cnt[2] == 1;
     }
-    if (spec->curve || spec->tone) return fail(FRH_ERROR_NOT_AVAILABLE, "output curves and tone are not supported");
     if (!valid_mode(spec->output)) return fail(FRH_ERROR_INVALID_ARGUMENT, "unknown output");
     if (spec->output2 != OUT_NONE && !valid_mode(spec->output2))
         return fail(FRH_ERROR_INVALID_ARGUMENT, "unknown second output");
@@ -607,6 +606,43 @@ int32_t frh::plan_develop(const frh_context *ctx, const frh_frame *f, const frh_
         const char *rows = getenv("FLASHRAW_BILINEAR_ROWS");
         a.bil_rows = rows ? (uint32_t)atoi(rows) : tuned_rows(ctx, false, a, ctx->bil_rows);
     }
+    // The display outputs' tone and output curve (run.rs:626-675)
+    if (spec->tone) {
+        const frh_tone *t = spec->tone;
+        if (t->size < sizeof(frh_tone)) return fail(FRH_ERROR_INVALID_ARGUMENT, "tone size field too small");
+        if (t->table_len) {
+            if (!t->table) return fail(FRH_ERROR_INVALID_ARGUMENT, "NULL tone table");
+            a.tone = 1;
+            a.tone_tab = static_cast<const float2 *>(device_table(mctx, t->table, t->table_len * 8ull, &rc));
+            if (rc) return rc;
+            a.tone_n = t->table_len;
+            a.tone_xmax = t->xmax;
+            a.tone_scale = t->scale;
+            memcpy(a.tone_m, t->to_prophoto, sizeof(a.tone_m));
+            memcpy(a.tone_w, t->white, sizeof(a.tone_w));
+        }
+    }
+    if (spec->curve) {
+        const frh_curve *c = spec->curve;
+        if (c->size < sizeof(frh_curve)) return fail(FRH_ERROR_INVALID_ARGUMENT, "curve size field too small");
+        if (c->lut8_len > 1) {
+            if (!c->lut8) return fail(FRH_ERROR_INVALID_ARGUMENT, "NULL curve table");
+            a.curve8 = static_cast<const uint8_t *>(device_table(mctx, c->lut8, c->lut8_len, &rc));
+            if (rc) return rc;
+            a.curve8_n = c->lut8_len - 1;
+        }
+        if (c->lut16_len == 16385) {
+            if (!c->lut16) return fail(FRH_ERROR_INVALID_ARGUMENT, "NULL curve table");
+            // (value, step) pairs: the steps are the f32 differences srgb16 computes (process.rs:1953-1958)
+            std::vector<float> pairs(2 * 16384);
+            for (int i = 0; i < 16384; i++) {
+                pairs[2 * i] = c->lut16[i];
+                pairs[2 * i + 1] = c->lut16[i + 1] - c->lut16[i];
+            }
+            a.srgb16 = static_cast<const float2 *>(device_table(mctx, pairs.data(), pairs.size() * 4ull, &rc));
+            if (rc) return rc;
+        }
+    }
     // run.rs:545: the edge-aware and AHD demosaics apply to single-component CFA frames at full size
     if (spec->edge && !spec->half_size && cpp == 1 && has_cfa) {
         const frh_edge_spec *e = spec->edge;
@@ -672,8 +708,9 @@ int32_t frh::plan_develop(const frh_context *ctx, const frh_frame *f, const frh_
         p.out_h[k] = oriented && swap ? a.dev_w : a.dev_h;
         p.bytes[k] = (uint64_t)p.out_w[k] * p.out_h[k] * p.ch[k] * bps;
     }
-    if (!a.generic && a.srcf) {
-        // the register walkers read u16 rows; f32 samples take the LDS-tiled kernels
+    if (!a.generic && (a.srcf || a.tone || a.curve8)) {
+        // the register walkers read u16 rows and encode with the sRGB table: f32 samples, tone
+        // and output curves take the LDS-tiled kernels
         a.bil_impl = 1;
         a.ahd_impl = 1;
     }

@@ -94,7 +94,7 @@ __global__ __launch_bounds__(WALK_WAVES * 64) void k_batch_ahd(BatchList L, cons
         } else {
             const uint32_t base = (local - F.walk_items) * BORDER_ITEM + (threadIdx.x & 63);
             for (uint32_t j = 0; j < BORDER_ITEM; j += 64)
-                if (base + j < F.wp.border.count) border_pixel(F.a, s, F.wp.border, base + j);
+                if (base + j < F.wp.border.count) border_pixel<false>(F.a, s, F.wp.border, base + j);
         }
     }
 }
@@ -116,7 +116,7 @@ __global__ __launch_bounds__(BWALK_WAVES * 64) void k_batch_bilinear(BatchList L
         } else {
             const uint32_t base = (local - F.walk_items) * BORDER_ITEM + (threadIdx.x & 63);
             for (uint32_t j = 0; j < BORDER_ITEM; j += 64)
-                if (base + j < F.bp.border.count) border_pixel(F.a, s, F.bp.border, base + j);
+                if (base + j < F.bp.border.count) border_pixel<false>(F.a, s, F.bp.border, base + j);
         }
     }
 }
@@ -322,6 +322,8 @@ extern "C" int32_t frh_batch_create(frh_context *ctx, uint32_t n, const frh_fram
             if (p.a.edge_aware) return fail(FRH_ERROR_NOT_AVAILABLE, "batched frames use bilinear or AHD");
             if (p.a.generic || p.in.kind != FRH_INPUT_U16 || p.in.staged)
                 return fail(FRH_ERROR_NOT_AVAILABLE, "batched frames are full-size u16 Bayer mosaics");
+            if (specs[i].curve || specs[i].tone)
+                return fail(FRH_ERROR_NOT_AVAILABLE, "batched frames use the sRGB curve without tone");
             if (specs[i].output2 != OUT_NONE) return fail(FRH_ERROR_NOT_AVAILABLE, "batched frames have one output");
             if (!dsts[i]) return fail(FRH_ERROR_INVALID_ARGUMENT, "NULL output");
             p.a.dst = dsts[i];

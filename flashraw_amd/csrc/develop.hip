@@ -244,7 +244,7 @@ __global__ __launch_bounds__(BWALK_WAVES * 64) void k_develop_bilinear_walk(DevA
     __syncthreads();
     if (blockIdx.x >= walk_blocks) {
         const uint32_t i = (blockIdx.x - walk_blocks) * blockDim.x + threadIdx.x;
-        if (i < P.border.count) border_pixel(a, s, P.border, i);
+        if (i < P.border.count) border_pixel<false>(a, s, P.border, i);
         return;
     }
     // the wave index is the same for every lane; readfirstlane lets the compiler keep all the strip

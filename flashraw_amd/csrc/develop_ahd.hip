@@ -13,7 +13,7 @@ __global__ __launch_bounds__(WALK_WAVES * 64) void k_develop_ahd_walk(DevArgs a,
     if (blockIdx.x >= walk_blocks) {
         __syncthreads();
         const uint32_t i = (blockIdx.x - walk_blocks) * blockDim.x + threadIdx.x;
-        if (i < P.border.count) border_pixel(a, s, P.border, i);
+        if (i < P.border.count) border_pixel<false>(a, s, P.border, i);
         return;
     }
     for (int i = threadIdx.x; i < 73 * 73; i += blockDim.x) wt[i] = a.wtab[i];

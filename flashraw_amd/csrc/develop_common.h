@@ -40,9 +40,12 @@ __device__ __forceinline__ float wb_of(const DevArgs &a, uint32_t c) {
 
 // Calib::norm (process.rs:288-298) of frame sample (x, y), in bounds.
 __device__ __forceinline__ float norm_raw(const DevArgs &a, const Lds &s, float v, uint32_t x, uint32_t y);
+// FULL = false: the u16 register walkers' form (no f32 samples, tone or output curve: frames with
+// them take the other kernels).
+template <bool FULL = true>
 __device__ __forceinline__ float norm_at(const DevArgs &a, const Lds &s, uint32_t x, uint32_t y) {
     const size_t i = (size_t)y * a.stride + x;
-    return norm_raw(a, s, a.srcf ? a.srcf[i] : (float)a.src[i], x, y);
+    return norm_raw(a, s, (FULL && a.srcf) ? a.srcf[i] : (float)a.src[i], x, y);
 }
 
 // finish_rgb with Highlights::Clip (process.rs:654-707). `balanced` pixels come from the AHD
@@ -92,6 +95,44 @@ __device__ __forceinline__ uint32_t srgb16(const DevArgs &a, float v) {
 // linear16 (process.rs:1971-1973).
 __device__ __forceinline__ uint32_t lin16(float v) { return trunc_u32(rclamp01(v) * 65535.0f + 0.5f); }
 
+// ToneMap's max / min (crates/flashraw/src/tone.rs:98-108): plain comparisons (NaN -> the second).
+__device__ __forceinline__ float tmax(float a, float b) { return a > b ? a : b; }
+__device__ __forceinline__ float tmin(float a, float b) { return a < b ? a : b; }
+
+// ToneMap::position + read (tone.rs:148-162): the curve at linear x >= 0.
+__device__ __forceinline__ float tone_curve(const DevArgs &a, float x) {
+    const float t = sqrtf(tmin(x, a.tone_xmax)) * a.tone_scale;  // correctly rounded
+    const float t1 = tmin(t, (float)(a.tone_n - 1u));
+    const int i = (int)t1;
+    const float2 p = a.tone_tab[i];
+    return p.x + p.y * (t - (float)i);
+}
+
+// ToneMap::apply (tone.rs:170-207): one pixel of linear output RGB.
+__device__ __forceinline__ void tone_px(const DevArgs &a, float &r, float &g, float &b) {
+    const float *m = a.tone_m;
+    const float c0 = tmax(m[0] * r + m[1] * g + m[2] * b, 0.0f);
+    const float c1 = tmax(m[3] * r + m[4] * g + m[5] * b, 0.0f);
+    const float c2 = tmax(m[6] * r + m[7] * g + m[8] * b, 0.0f);
+    const float hi = tmax(tmax(c0, c1), c2), lo = tmin(tmin(c0, c1), c2);
+    const float fh = tone_curve(a, hi), fl = tone_curve(a, lo);
+    const float d = hi - lo;
+    const float k = d > 0.0f ? (fh - fl) / d : 0.0f;
+    const float base = fl - k * lo;
+    r = base * a.tone_w[0] + k * r;
+    g = base * a.tone_w[1] + k * g;
+    b = base * a.tone_w[2] + k * b;
+}
+
+// An 8-bit code of linear v: the sRGB table in LDS, or the output curve's table (Lut8,
+// process.rs:1679-1728: table[min(trunc(clamp(v, 0, 1) * n), n)]).
+__device__ __forceinline__ uint32_t code8(const DevArgs &a, const Lds &s, float v) {
+    if (!a.curve8) return lut8(s, lut8_index(v));
+    const float x = v * (float)a.curve8_n;
+    const uint32_t i = x >= 1.0f ? (uint32_t)fminf(x, (float)a.curve8_n) : 0u;
+    return a.curve8[i];
+}
+
 // Destination of developed pixel (dx, dy) under EXIF orientation o (the inverse of
 // encode_developed's source mapping, process.rs:2035-2044).
 __device__ __forceinline__ void orient(int o, uint32_t dx, uint32_t dy, uint32_t w, uint32_t h, uint32_t &px,
@@ -110,13 +151,16 @@ __device__ __forceinline__ void orient(int o, uint32_t dx, uint32_t dy, uint32_t
 
 // Encode and store linear output pixel (r, g, b) of developed pixel (dx, dy) (any encoding and
 // orientation).
+template <bool FULL = true>
 __device__ __forceinline__ void store_px(const DevArgs &a, const Lds &s, int mode, void *dst, uint32_t dx,
                                          uint32_t dy, float r, float g, float b) {
+    if (FULL && a.tone && (mode == OUT_RGBA8 || mode == OUT_SRGB16)) tone_px(a, r, g, b);
     if (mode == OUT_RGBA8) {
         uint32_t px, py;
         orient(a.orientation, dx, dy, a.dev_w, a.dev_h, px, py);
-        uint32_t v = lut8(s, lut8_index(r)) | (lut8(s, lut8_index(g)) << 8) | (lut8(s, lut8_index(b)) << 16) |
-                     0xff000000u;
+        uint32_t v = FULL ? (code8(a, s, r) | (code8(a, s, g) << 8) | (code8(a, s, b) << 16) | 0xff000000u)
+                          : (lut8(s, lut8_index(r)) | (lut8(s, lut8_index(g)) << 8) | (lut8(s, lut8_index(b)) << 16) |
+                             0xff000000u);
         reinterpret_cast<uint32_t *>(dst)[(size_t)py * a.dst_w + px] = v;
     } else if (mode == OUT_LINEAR_F32) {
         float *p = reinterpret_cast<float *>(dst) + ((size_t)dy * a.dev_w + dx) * 3;
@@ -143,10 +187,11 @@ __device__ __forceinline__ void store_px(const DevArgs &a, const Lds &s, int mod
     }
 }
 
+template <bool FULL = true>
 __device__ __forceinline__ void store_both(const DevArgs &a, const Lds &s, uint32_t dx, uint32_t dy, float r,
                                            float g, float b) {
-    store_px(a, s, a.out_mode, a.dst, dx, dy, r, g, b);
-    if (a.out_mode2 != OUT_NONE) store_px(a, s, a.out_mode2, a.dst2, dx, dy, r, g, b);
+    store_px<FULL>(a, s, a.out_mode, a.dst, dx, dy, r, g, b);
+    if (a.out_mode2 != OUT_NONE) store_px<FULL>(a, s, a.out_mode2, a.dst2, dx, dy, r, g, b);
 }
 
 // The RGBA8 word of linear output (r, g, b) (process.rs:1712-1728; alpha 255, R in the low byte).
@@ -157,7 +202,8 @@ __device__ __forceinline__ uint32_t rgba8_word(const Lds &s, float r, float g, f
 // True when a launch stores RGBA8 only, unrotated, without levels, with bounded parameters: the
 // kernels' inline fast path.
 __host__ __device__ __forceinline__ bool fast_rgba8(const DevArgs &a) {
-    return a.out_mode == OUT_RGBA8 && a.out_mode2 == OUT_NONE && a.orientation == 1 && a.bounded && !a.has_levels;
+    return a.out_mode == OUT_RGBA8 && a.out_mode2 == OUT_NONE && a.orientation == 1 && a.bounded && !a.has_levels &&
+           !a.tone && !a.curve8;
 }
 
 __device__ __forceinline__ float med01(float x) { return __builtin_amdgcn_fmed3f(x, 0.0f, 1.0f); }
@@ -274,13 +320,14 @@ __device__ __forceinline__ void store_rgba8_pair(const DevArgs &a, __amdgpu_buff
 // Kernel::row_cols for one pixel of a Bayer mosaic, read straight from the frame
 // (process.rs:1076-1126): the interior formulas, or at the frame border the mean of the
 // in-frame same-colour 3x3 samples in row-major order.
+template <bool FULL = true>
 __device__ __forceinline__ void bilinear_global(const DevArgs &a, const Lds &s, int x, int y, float rgb[3]) {
     const int w = (int)a.width, h = (int)a.height;
     uint32_t c0 = chan_at(a, x, y);
     if (x > 0 && y > 0 && x + 1 < w && y + 1 < h) {
-        float own = norm_at(a, s, x, y);
-        float l = norm_at(a, s, x - 1, y), r = norm_at(a, s, x + 1, y);
-        float u = norm_at(a, s, x, y - 1), d = norm_at(a, s, x, y + 1);
+        float own = norm_at<FULL>(a, s, x, y);
+        float l = norm_at<FULL>(a, s, x - 1, y), r = norm_at<FULL>(a, s, x + 1, y);
+        float u = norm_at<FULL>(a, s, x, y - 1), d = norm_at<FULL>(a, s, x, y + 1);
         if (c0 == 1) {
             uint32_t ch = chan_at(a, x + 1, y);
             float hv = (l + r) * 0.5f, vv = (u + d) * 0.5f;
@@ -289,8 +336,8 @@ __device__ __forceinline__ void bilinear_global(const DevArgs &a, const Lds &s, 
             rgb[2] = ch == 0 ? vv : hv;
         } else {
             float orth = (l + r + u + d) * 0.25f;
-            float diag = (norm_at(a, s, x - 1, y - 1) + norm_at(a, s, x + 1, y - 1) + norm_at(a, s, x - 1, y + 1) +
-                          norm_at(a, s, x + 1, y + 1)) *
+            float diag = (norm_at<FULL>(a, s, x - 1, y - 1) + norm_at<FULL>(a, s, x + 1, y - 1) + norm_at<FULL>(a, s, x - 1, y + 1) +
+                          norm_at<FULL>(a, s, x + 1, y + 1)) *
                          0.25f;
             rgb[0] = c0 == 0 ? own : diag;
             rgb[1] = orth;
@@ -301,7 +348,7 @@ __device__ __forceinline__ void bilinear_global(const DevArgs &a, const Lds &s, 
     for (uint32_t c = 0; c < 3; c++) {
         float sum = 0.0f, n = 0.0f;
         if (c == c0) {
-            sum = norm_at(a, s, x, y);
+            sum = norm_at<FULL>(a, s, x, y);
             n = 1.0f;
         } else {
             for (int dy = -1; dy <= 1; dy++) {
@@ -309,7 +356,7 @@ __device__ __forceinline__ void bilinear_global(const DevArgs &a, const Lds &s, 
                     int xx = x + dx, yy = y + dy;
                     if (chan_at(a, xx, yy) != c) continue;
                     if (xx < 0 || xx >= w || yy < 0 || yy >= h) continue;
-                    sum += norm_at(a, s, xx, yy);
+                    sum += norm_at<FULL>(a, s, xx, yy);
                     n += 1.0f;
                 }
             }
@@ -348,6 +395,7 @@ __host__ inline Border make_border(const DevArgs &a, int halo) {
     return b;
 }
 
+template <bool FULL = true>
 __device__ __forceinline__ void border_pixel(const DevArgs &a, const Lds &s, const Border &B, uint32_t i) {
     const int dw = (int)a.dev_w;
     const int b0 = (int)a.band_y0;
@@ -370,10 +418,10 @@ __device__ __forceinline__ void border_pixel(const DevArgs &a, const Lds &s, con
         dx = (int)(j % dw);
     }
     float rgb[3];
-    bilinear_global(a, s, (int)a.crop_x + dx, (int)a.crop_y + dy, rgb);
+    bilinear_global<FULL>(a, s, (int)a.crop_x + dx, (int)a.crop_y + dy, rgb);
     float o0, o1, o2;
     finish(a, rgb[0], rgb[1], rgb[2], false, o0, o1, o2);
-    store_both(a, s, (uint32_t)dx, (uint32_t)dy, o0, o1, o2);
+    store_both<FULL>(a, s, (uint32_t)dx, (uint32_t)dy, o0, o1, o2);
 }
 
 // Red's index in the frame-parity colour table fc[(y & 1) * 2 + (x & 1)] (the walkers' template

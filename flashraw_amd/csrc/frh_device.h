@@ -110,6 +110,17 @@ struct DevArgs {
     const float *gcal;
     const uint32_t *sites;
     uint32_t cfa_w, cfa_h;
+    // Display outputs (RGBA8, sRGB16): the standard tone (GpuTone: `tone_n` (value, step) pairs
+    // read at sqrt(min(x, xmax)) * scale, output RGB -> ProPhoto `tone_m`, ProPhoto white
+    // `tone_w`) and an RGBA8 output curve (`curve8_n` segments; NULL: the sRGB table). A curve's
+    // 16-bit table replaces `srgb16`.
+    uint32_t tone;
+    const float2 *tone_tab;
+    uint32_t tone_n;
+    float tone_xmax, tone_scale;
+    float tone_m[9], tone_w[3];
+    const uint8_t *curve8;
+    uint32_t curve8_n;
 };
 
 // Window of a raster for the input kernels (unpack, linearise, normalise): packed payload bytes

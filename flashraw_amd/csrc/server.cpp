@@ -386,6 +386,8 @@ extern "C" int32_t frh_server_submit(frh_server *s, const frh_frame *frame, cons
         if (p.a.edge_aware) return fail(FRH_ERROR_NOT_AVAILABLE, "served frames use bilinear or AHD");
         if (p.a.generic || p.in.kind != FRH_INPUT_U16 || p.in.staged)
             return fail(FRH_ERROR_NOT_AVAILABLE, "served frames are full-size u16 Bayer mosaics");
+        if (spec->curve || spec->tone)
+            return fail(FRH_ERROR_NOT_AVAILABLE, "served frames use the sRGB curve without tone");
         if (host_out_bytes < p.bytes[0]) return fail(FRH_ERROR_INVALID_ARGUMENT, "output buffer too small");
         auto j = std::make_shared<Job>();
         j->frame = *frame;

@@ -102,12 +102,23 @@ class _EdgeSpec(ctypes.Structure):
     ]
 
 
+class _Curve(ctypes.Structure):
+    _fields_ = [("size", ctypes.c_uint32), ("lut8", ctypes.c_void_p), ("lut8_len", ctypes.c_uint32),
+                ("lut16", ctypes.c_void_p), ("lut16_len", ctypes.c_uint32)]
+
+
+class _Tone(ctypes.Structure):
+    _fields_ = [("size", ctypes.c_uint32), ("table", ctypes.c_void_p), ("table_len", ctypes.c_uint32),
+                ("xmax", ctypes.c_float), ("scale", ctypes.c_float), ("to_prophoto", ctypes.c_float * 9),
+                ("white", ctypes.c_float * 3)]
+
+
 class _DevelopSpec(ctypes.Structure):
     _fields_ = [
         ("size", ctypes.c_uint32), ("crop", _Rect), ("half_size", ctypes.c_int32), ("wb", ctypes.c_float * 4),
         ("cam_to_rgb", ctypes.c_float * 9), ("orientation", ctypes.c_int32), ("output", ctypes.c_int32),
         ("output2", ctypes.c_int32), ("has_levels", ctypes.c_int32), ("levels", ctypes.c_float * 2),
-        ("edge", ctypes.POINTER(_EdgeSpec)), ("curve", ctypes.c_void_p), ("tone", ctypes.c_void_p),
+        ("edge", ctypes.POINTER(_EdgeSpec)), ("curve", ctypes.POINTER(_Curve)), ("tone", ctypes.POINTER(_Tone)),
     ]
 
 
@@ -480,6 +491,27 @@ def ahd_spec(info: FrameInfo, origin=(0, 0)) -> Optional[EdgeSpec]:
 
 
 @dataclass
+class GpuCurve:
+    """GpuCurve (run.rs:181-188): 8-bit codes at len(lut8) - 1 segments of [0, 1] (RGBA8) and
+    16-bit codes as f32 at 16385 points (sRGB16)."""
+    lut8: Sequence[int] = ()
+    lut16: Sequence[float] = ()
+
+
+@dataclass
+class GpuTone:
+    """GpuTone (run.rs:190-205): the standard tone of the display outputs. `table` holds
+    (value, step) pairs of the curve at evenly spaced sqrt(x / xmax); `scale` is
+    len(table) / sqrt(xmax); `to_prophoto` maps linear output RGB to linear ProPhoto and `white`
+    is ProPhoto white in the output space."""
+    table: Sequence = ()
+    xmax: float = 1.0
+    scale: float = 1024.0
+    to_prophoto: Sequence[Sequence[float]] = ((1.0, 0.0, 0.0), (0.0, 1.0, 0.0), (0.0, 0.0, 1.0))
+    white: Sequence[float] = (1.0, 1.0, 1.0)
+
+
+@dataclass
 class DevelopSpec:
     """DevelopSpec (run.rs:115-135). `output2` asks the same pass for a second encoding."""
     crop: Rect
@@ -511,9 +543,27 @@ class DevelopSpec:
             s.has_levels = 1
             s.levels[0], s.levels[1] = float(self.levels[0]), float(self.levels[1])
         keep = []
-        if self.curve is not None or self.tone is not None:
-            s.curve = 1 if self.curve is not None else None
-            s.tone = 1 if self.tone is not None else None
+        if self.curve is not None:
+            c = _Curve()
+            c.size = ctypes.sizeof(_Curve)
+            l8 = np.ascontiguousarray(self.curve.lut8, dtype=np.uint8)
+            l16 = np.ascontiguousarray(self.curve.lut16, dtype=np.float32)
+            keep += [l8, l16, c]
+            c.lut8, c.lut8_len = (l8.ctypes.data if l8.size else None), l8.size
+            c.lut16, c.lut16_len = (l16.ctypes.data if l16.size else None), l16.size
+            s.curve = ctypes.pointer(c)
+        if self.tone is not None:
+            t = _Tone()
+            t.size = ctypes.sizeof(_Tone)
+            tab = np.ascontiguousarray(self.tone.table, dtype=np.float32).reshape(-1)
+            keep += [tab, t]
+            t.table, t.table_len = (tab.ctypes.data if tab.size else None), tab.size // 2
+            t.xmax, t.scale = float(self.tone.xmax), float(self.tone.scale)
+            for i, v in enumerate(np.asarray(self.tone.to_prophoto, np.float32).reshape(-1)):
+                t.to_prophoto[i] = float(v)
+            for i, v in enumerate(np.asarray(self.tone.white, np.float32).reshape(-1)):
+                t.white[i] = float(v)
+            s.tone = ctypes.pointer(t)
         if self.edge is not None:
             e = _EdgeSpec()
             e.size = ctypes.sizeof(_EdgeSpec)

@@ -117,9 +117,35 @@ typedef struct frh_edge_spec {
     int32_t ahd;
 } frh_edge_spec;
 
+/* GpuCurve (run.rs:181-188): the encoder tables of a non-sRGB output curve (CurveTables,
+ * crates/flashraw/src/color.rs:343-367): `lut8` 8-bit codes at lut8_len - 1 segments of [0, 1]
+ * (RGBA8 output, used when lut8_len > 1) and `lut16` 16-bit codes as f32 at 16385 points (sRGB16
+ * output, used when lut16_len is 16385; the sRGB curve otherwise). */
+typedef struct frh_curve {
+    uint32_t size;
+    const uint8_t *lut8;
+    uint32_t lut8_len;
+    const float *lut16;
+    uint32_t lut16_len;
+} frh_curve;
+
+/* GpuTone (run.rs:190-205; ToneMap, crates/flashraw/src/tone.rs:81-207): the standard tone of
+ * the display outputs (RGBA8, sRGB16). `table` holds table_len (value, step) pairs of the curve
+ * at evenly spaced sqrt(x / xmax); in linear ProPhoto (`to_prophoto`, row-major, from linear
+ * output RGB) the largest and smallest channel go through the curve and the pixel becomes
+ * (fl - k lo) white + k p. table_len 0 means no tone. */
+typedef struct frh_tone {
+    uint32_t size;
+    const float *table;
+    uint32_t table_len;
+    float xmax, scale;
+    float to_prophoto[9];
+    float white[3];
+} frh_tone;
+
 /* DevelopSpec (run.rs:115-135). `output2` (FRH_OUTPUT_NONE for none) asks the same pass to
- * store a second encoding, e.g. linear 16-bit and RGBA8 together. `curve` and `tone` must be
- * NULL (sRGB curve, no tone). */
+ * store a second encoding, e.g. linear 16-bit and RGBA8 together. `curve` and `tone` (NULL for
+ * the sRGB curve and no tone) apply to the display encodings only. */
 typedef struct frh_develop_spec {
     uint32_t size;
     frh_rect crop;
@@ -132,8 +158,8 @@ typedef struct frh_develop_spec {
     int32_t has_levels;
     float levels[2]; /* scale, offset */
     const frh_edge_spec *edge;
-    const void *curve;
-    const void *tone;
+    const frh_curve *curve;
+    const frh_tone *tone;
 } frh_develop_spec;
 
 /* GpuTimings (run.rs:229-239). */

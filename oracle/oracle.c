@@ -208,6 +208,29 @@ static inline float calib_norm(const Kern *k, size_t c, float v, size_t x, size_
  * calib_norm's value for every sample, (f32(v) - black) * scale, u16 or f32 samples. */
 static void normalize_row(const Kern *k, size_t y, float *out) {
     const size_t n = k->w * k->cpp;
+    if (k->cpp == 1 && !k->n_rowd && !k->n_cold && k->raw) {
+        /* one pattern period of black / scale per row, then a modulo-free loop (process.rs:918-944) */
+        const uint16_t *restrict row = k->raw + y * k->w;
+        size_t period = k->cal_pw, base = ((y + k->oy) % k->cal_ph) * k->cal_pw;
+        float blk[64], scl[64];
+        for (size_t i = 0; i < period; i++) {
+            blk[i] = k->black[base + (i + k->ox) % period];
+            scl[i] = k->scale[base + (i + k->ox) % period];
+        }
+        if (period == 2) {
+            for (size_t x = 0; x + 1 < k->w; x += 2) {
+                out[x] = ((float)row[x] - blk[0]) * scl[0];
+                out[x + 1] = ((float)row[x + 1] - blk[1]) * scl[1];
+            }
+            if (k->w & 1) out[k->w - 1] = ((float)row[k->w - 1] - blk[0]) * scl[0];
+            return;
+        }
+        for (size_t x = 0, j = 0; x < k->w; x++) {
+            out[x] = ((float)row[x] - blk[j]) * scl[j];
+            if (++j == period) j = 0;
+        }
+        return;
+    }
     for (size_t i = 0; i < n; i++) {
         float v = k->rawf ? k->rawf[y * n + i] : (float)k->raw[y * n + i];
         out[i] = calib_norm(k, i % k->cpp, v, i / k->cpp + k->ox, y + k->oy);
@@ -566,7 +589,38 @@ typedef struct {
     uint8_t lut8[4097];
     float pairs[16384][2];
     int channels;
+    /* the 8-bit table in use (the sRGB table or a curve's) and its segments */
+    const uint8_t *t8;
+    uint32_t n8;
+    const fro_tone *tone;
 } Encoder;
+
+/* ToneMap's max / min (tone.rs:98-108): plain comparisons. */
+static inline float tmax(float a, float b) { return a > b ? a : b; }
+static inline float tmin(float a, float b) { return a < b ? a : b; }
+
+/* ToneMap::position + read (tone.rs:148-162): the curve at linear x >= 0. */
+static float tone_curve(const fro_tone *t, float x) {
+    float tt = sqrtf(tmin(x, t->xmax)) * t->scale;
+    float t1 = tmin(tt, (float)(t->n - 1));
+    int i = (int)t1;
+    float r = tt - (float)i;
+    return t->table[2 * i] + t->table[2 * i + 1] * r;
+}
+
+/* ToneMap::apply (tone.rs:170-207). */
+static void tone_apply(const fro_tone *t, const float p[3], float o[3]) {
+    const float *m = t->m;
+    float c0 = tmax(m[0] * p[0] + m[1] * p[1] + m[2] * p[2], 0.0f);
+    float c1 = tmax(m[3] * p[0] + m[4] * p[1] + m[5] * p[2], 0.0f);
+    float c2 = tmax(m[6] * p[0] + m[7] * p[1] + m[8] * p[2], 0.0f);
+    float hi = tmax(tmax(c0, c1), c2), lo = tmin(tmin(c0, c1), c2);
+    float fh = tone_curve(t, hi), fl = tone_curve(t, lo);
+    float d = hi - lo;
+    float k = d > 0.0f ? (fh - fl) / d : 0.0f;
+    float b = fl - k * lo;
+    for (int c = 0; c < 3; c++) o[c] = b * t->white[c] + k * p[c];
+}
 
 /* Destination pixel of developed pixel (sx, sy) under EXIF orientation o: the inverse of the
  * source mapping in encode_developed (process.rs:2035-2044). */
@@ -584,9 +638,15 @@ static inline void dest_of(const Encoder *e, size_t sx, size_t sy, size_t *ox, s
     }
 }
 
-static void encode_px(const Encoder *e, const float p[3], size_t sx, size_t sy) {
+static void encode_px(const Encoder *e, const float p_in[3], size_t sx, size_t sy) {
     size_t ox = sx, oy = sy;
     int oriented = e->enc == FRO_RGBA8 || e->enc == FRO_SRGB16 || e->enc == FRO_RGB8;
+    float toned[3];
+    const float *p = p_in;
+    if (e->tone && oriented) { /* the display outputs (develop.wgsl store_rgb) */
+        tone_apply(e->tone, p_in, toned);
+        p = toned;
+    }
     if (oriented) dest_of(e, sx, sy, &ox, &oy);
     size_t di = (oy * e->dw + ox) * (size_t)e->channels;
     switch (e->enc) {
@@ -594,8 +654,8 @@ static void encode_px(const Encoder *e, const float p[3], size_t sx, size_t sy) 
     case FRO_RGB8: {
         uint8_t *o = (uint8_t *)e->out + di;
         for (int c = 0; c < 3; c++) {
-            uint32_t ix = sat_u32(rclamp(p[c], 0.0f, 1.0f) * 4096.0f); /* lut8_run, process.rs:1712-1728 */
-            o[c] = e->lut8[ix < 4096 ? ix : 4096];
+            uint32_t ix = sat_u32(rclamp(p[c], 0.0f, 1.0f) * (float)e->n8); /* lut8_run, process.rs:1712-1728 */
+            o[c] = e->t8[ix < e->n8 ? ix : e->n8];
         }
         if (e->enc == FRO_RGBA8) o[3] = 255;
         break;
@@ -632,7 +692,7 @@ static void encode_px(const Encoder *e, const float p[3], size_t sx, size_t sy) 
 /* Encode developed pixels (sx, y) for sx in [x0, x0 + n) (a run of one developed row): the
  * same per-sample operations as encode_px, with the common unrotated cases as straight loops. */
 static void encode_run(const Encoder *e, const float *restrict src, size_t n, size_t x0, size_t y) {
-    if (e->orientation != 1) {
+    if (e->orientation != 1 || e->tone || e->n8 != 4096) {
         for (size_t i = 0; i < n; i++) encode_px(e, src + i * 3, x0 + i, y);
         return;
     }
@@ -1003,6 +1063,13 @@ int fro_develop(const fro_frame *f, const fro_params *p, void *out, uint64_t out
         return -1;
     }
     fro_srgb8_table(e->lut8);
+    e->t8 = e->lut8;
+    e->n8 = 4096;
+    if (p->curve8 && p->curve8_len > 1) { /* run.rs:672-675 */
+        e->t8 = p->curve8;
+        e->n8 = p->curve8_len - 1;
+    }
+    e->tone = (p->tone && p->tone->n) ? p->tone : NULL;
     {
         float *lut16 = (float *)malloc(16385 * sizeof(float));
         if (!lut16) {
@@ -1011,6 +1078,7 @@ int fro_develop(const fro_frame *f, const fro_params *p, void *out, uint64_t out
             return -1;
         }
         fro_srgb16_table(lut16);
+        if (p->curve16 && p->curve16_len == 16385) memcpy(lut16, p->curve16, 16385 * sizeof(float)); /* run.rs:639-643 */
         for (int i = 0; i < 16384; i++) {
             e->pairs[i][0] = lut16[i];
             e->pairs[i][1] = lut16[i + 1] - lut16[i];

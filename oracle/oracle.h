@@ -64,6 +64,18 @@ enum fro_encoding {
     FRO_RGB8 = 5
 };
 
+/* The standard tone of the display outputs (crates/flashraw/src/tone.rs:81-207, GpuTone): in
+ * linear ProPhoto (m: output RGB -> ProPhoto) the largest and smallest channel go through the
+ * curve table[i] = (value, step) read at t = sqrt(min(x, xmax)) * scale, and the result is
+ * (fl - k lo) white + k p. */
+typedef struct fro_tone {
+    const float *table; /* n pairs */
+    uint32_t n;
+    float xmax, scale;
+    float m[9];
+    float white[3];
+} fro_tone;
+
 typedef struct fro_params {
     int32_t demosaic;
     uint32_t crop_x, crop_y, crop_w, crop_h;
@@ -75,6 +87,13 @@ typedef struct fro_params {
     int32_t encoding;
     const fro_edge *edge;    /* FRO_EDGE only */
     int32_t half_size;       /* Demosaic::HalfSize: 2x2 blocks averaged per colour */
+    const fro_tone *tone;    /* display outputs (RGBA8, RGB8, SRGB16) only; NULL for none */
+    /* output curve (CurveTables, crates/flashraw/src/color.rs:343-367): 8-bit codes at
+     * curve8_len - 1 segments, 16-bit codes at 16385 points; NULL for the sRGB curve */
+    const uint8_t *curve8;
+    uint32_t curve8_len;
+    const float *curve16;
+    uint32_t curve16_len;
 } fro_params;
 
 /* Fused develop of `f` with `p` into `out` (out_w*out_h*channels samples of the encoding's

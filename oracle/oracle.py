@@ -66,6 +66,17 @@ class FroEdge(ctypes.Structure):
     ]
 
 
+class FroTone(ctypes.Structure):
+    _fields_ = [
+        ("table", ctypes.c_void_p),
+        ("n", ctypes.c_uint32),
+        ("xmax", ctypes.c_float),
+        ("scale", ctypes.c_float),
+        ("m", ctypes.c_float * 9),
+        ("white", ctypes.c_float * 3),
+    ]
+
+
 class FroParams(ctypes.Structure):
     _fields_ = [
         ("demosaic", ctypes.c_int32),
@@ -82,6 +93,11 @@ class FroParams(ctypes.Structure):
         ("encoding", ctypes.c_int32),
         ("edge", ctypes.POINTER(FroEdge)),
         ("half_size", ctypes.c_int32),
+        ("tone", ctypes.POINTER(FroTone)),
+        ("curve8", ctypes.c_void_p),
+        ("curve8_len", ctypes.c_uint32),
+        ("curve16", ctypes.c_void_p),
+        ("curve16_len", ctypes.c_uint32),
     ]
 
 
@@ -170,12 +186,14 @@ def make_frame(mosaic, cfa, origin=(0, 0), black=(1, 1, 1, [0.0]), white=(16383,
 def develop(mosaic, cfa, *, origin=(0, 0), black=(1, 1, 1, [0.0]), white=(16383,), wb=(1, 1, 1, 1),
             matrix=((1, 0, 0), (0, 1, 0), (0, 0, 1)), demosaic="bilinear", encoding="rgba8",
             orientation=1, crop=None, levels=None, row_deltas=None, col_deltas=None, threads=0, edge=None,
-            cpp=1, half_size=False):
+            cpp=1, half_size=False, tone=None, curve=None):
     """Fused develop (process.rs:1475-1591) of a frame of `cpp` components (u16 or f32 samples,
     `cfa` None for none). Returns an (out_h, out_w, channels) array of the encoding's sample
     type. `edge` (demosaic "edge") is a dict with g0 (4 x 49), dir (4 x 3 x 4 x 49) float32
     tables, cx, cy (canonical position of sample (0, 0) in the RGGB-ordered canonical pattern)
-    and beta. `half_size` is Demosaic::HalfSize."""
+    and beta. `half_size` is Demosaic::HalfSize. `tone` is a dict (table: (n, 2) float32 pairs,
+    xmax, scale, to_prophoto 3x3, white 3) for the display outputs; `curve` a pair (lut8 uint8,
+    lut16 float32) of output-curve tables."""
     f, keep = make_frame(mosaic, cfa, origin, black, white, row_deltas, col_deltas, cpp)
     h, w = f.height, f.width
     p = FroParams()
@@ -192,6 +210,23 @@ def develop(mosaic, cfa, *, origin=(0, 0), black=(1, 1, 1, [0.0]), white=(16383,
     p.orientation = orientation
     p.encoding = ENCODING[encoding]
     p.half_size = int(bool(half_size))
+    if tone is not None:
+        tt = np.ascontiguousarray(tone["table"], dtype=np.float32).reshape(-1)
+        to = FroTone()
+        to.table, to.n = tt.ctypes.data, tt.size // 2
+        to.xmax, to.scale = float(tone["xmax"]), float(tone["scale"])
+        for i, v in enumerate(np.asarray(tone["to_prophoto"], np.float32).reshape(-1)):
+            to.m[i] = float(v)
+        for i, v in enumerate(np.asarray(tone["white"], np.float32).reshape(-1)):
+            to.white[i] = float(v)
+        p.tone = ctypes.pointer(to)
+        keep += [tt, to]
+    if curve is not None:
+        c8 = np.ascontiguousarray(curve[0], dtype=np.uint8)
+        c16 = np.ascontiguousarray(curve[1], dtype=np.float32)
+        p.curve8, p.curve8_len = c8.ctypes.data, c8.size
+        p.curve16, p.curve16_len = c16.ctypes.data, c16.size
+        keep += [c8, c16]
     if edge is not None:
         g0 = np.ascontiguousarray(edge["g0"], dtype=np.float32).reshape(-1)
         dr = np.ascontiguousarray(edge["dir"], dtype=np.float32).reshape(-1)

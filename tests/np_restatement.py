@@ -629,3 +629,85 @@ def pack(samples, bits, order, stride):
                 row[i:i + wsz] = row[i:i + wsz][::-1]
         out[y * stride:y * stride + row_bytes] = row
     return np.frombuffer(bytes(out), dtype=np.uint8)
+
+
+# ---------------------------------------------------------------- display tone and curves
+
+
+def tone_apply(lin, table, xmax, scale, m, white):
+    """ToneMap::apply (crates/flashraw/src/tone.rs:148-207) per pixel of linear output RGB,
+    in float32 with the reference's order; max/min are plain comparisons."""
+    lin = lin.astype(F32)
+    tab = np.asarray(table, F32).reshape(-1, 2)
+    m = np.asarray(m, F32).reshape(3, 3)
+    w = np.asarray(white, F32)
+    p0, p1, p2 = lin[..., 0], lin[..., 1], lin[..., 2]
+
+    def mx(a, b):
+        return np.where(a > b, a, b).astype(F32)
+
+    def mn(a, b):
+        return np.where(a < b, a, b).astype(F32)
+
+    c = [mx(((m[k, 0] * p0 + m[k, 1] * p1) + m[k, 2] * p2).astype(F32), F32(0.0)) for k in range(3)]
+    hi, lo = mx(mx(c[0], c[1]), c[2]), mn(mn(c[0], c[1]), c[2])
+
+    def curve(x):
+        t = (np.sqrt(mn(x, F32(xmax))) * F32(scale)).astype(F32)
+        t1 = mn(t, F32(len(tab) - 1))
+        i = t1.astype(np.int64)
+        return (tab[i, 0] + tab[i, 1] * (t - i.astype(F32))).astype(F32)
+
+    fh, fl = curve(hi), curve(lo)
+    d = (hi - lo).astype(F32)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        k = np.where(d > 0, (fh - fl) / d, F32(0.0)).astype(F32)
+    b = (fl - k * lo).astype(F32)
+    return np.stack([(b * w[j] + k * lin[..., j]).astype(F32) for j in range(3)], -1)
+
+
+def curve8(lin, lut):
+    """Lut8 (process.rs:1679-1728): table[min(trunc(clamp(v, 0, 1) * (len - 1)), len - 1)]."""
+    n = len(lut) - 1
+    with np.errstate(invalid="ignore"):
+        x = np.clip(lin, F32(0.0), F32(1.0)) * F32(n)
+        i = np.where(np.isnan(x), 0, np.minimum(x, n)).astype(np.int64)
+    return np.asarray(lut, np.uint8)[i]
+
+
+def curve16(lin, lut):
+    """srgb16 with a curve's table (process.rs:1942-1950): (a + (b - a) t + 0.5) as u16."""
+    lut = np.asarray(lut, F32)
+    x = (np.clip(lin, F32(0.0), F32(1.0)) * F32(16384.0)).astype(F32)
+    i = np.minimum(np.where(np.isnan(x), 0, x).astype(np.int64), 16383)
+    t = (x - i.astype(F32)).astype(F32)
+    a, b = lut[i], lut[i + 1]
+    v = (a + (b - a) * t + F32(0.5)).astype(F32)
+    return np.clip(np.where(np.isnan(v), 0, v), 0, 65535).astype(np.uint16)
+
+
+def synthetic_tone(ev=0.7, n=1024):
+    """A tone in the GpuTone layout: an S-curve f over gamma-2.2 values tabulated at u = i / n as
+    f(u^2), (value, step) pairs; the curve reaches 1 at xmax = 2^-ev; a ProPhoto-like matrix with
+    unit row sums and its white."""
+    u = np.arange(n + 1, dtype=np.float64) / n
+    g = (u * u) ** (1 / 2.2)
+    s = 3 * g ** 2 - 2 * g ** 3
+    f = np.clip(0.6 * s + 0.4 * g, 0, 1) ** 2.2
+    f32 = f.astype(F32)
+    table = np.stack([f32[:-1], (f32[1:] - f32[:-1]).astype(F32)], -1)
+    xmax = 2.0 ** -ev
+    m = np.array([[0.53, 0.33, 0.14], [0.10, 0.82, 0.08], [0.02, 0.12, 0.86]])
+    white = np.linalg.inv(m).sum(axis=1)
+    return dict(table=table, xmax=F32(xmax), scale=F32(n / np.sqrt(xmax)), to_prophoto=m.astype(F32),
+                white=white.astype(F32))
+
+
+def synthetic_curve():
+    """CurveTables (crates/flashraw/src/color.rs:356-366) of the gamma 1/1.8 encoding: 16385
+    8-bit codes and 16385 16-bit codes."""
+    x = np.arange(16385, dtype=np.float64) / 16384
+    enc = x ** (1 / 1.8)
+    lut8 = np.clip(enc * 255.0 + 0.5, 0, 255).astype(np.uint8)
+    lut16 = (np.clip(enc, 0, 1) * 65535.0).astype(F32)
+    return lut8, lut16

@@ -347,3 +347,44 @@ def test_input_errors(ctx):
         ctx.run(F.GpuFrame.from_u16(d_in.cpu().numpy().view(np.uint16), w, h), xt,
                 F.GpuOp.Develop(F.DevelopSpec(F.Rect.full(w, h), edge=dataclasses.replace(
                     F.ahd_spec(info), canon=(0, 1, 1, 2)))))
+
+
+# ---------------------------------------------------------------- display tone and output curves
+
+
+@pytest.mark.parametrize("path", ["bilinear", "ahd", "edge", "xtrans", "half"])
+def test_tone_and_curve(ctx, path):
+    """GpuTone and GpuCurve on the display outputs of every develop path, bit for bit; linear
+    outputs (also as the second output of the same pass) are untouched."""
+    w, h = 140, 96
+    tone = N.synthetic_tone(ev=0.5)
+    lut8, lut16 = N.synthetic_curve()
+    gtone = F.GpuTone(tone["table"], float(tone["xmax"]), float(tone["scale"]), tone["to_prophoto"], tone["white"])
+    gcurve = F.GpuCurve(lut8, lut16)
+    cfa = XTRANS if path == "xtrans" else "GRBG"
+    info = _calibrated(w, h, _pat(cfa), 1)
+    arr = _synthetic(w, h, 1, 13)
+    origin = (1, 2)
+    tables = synth.edge_tables(3)
+    edge, oedge, demosaic = None, None, "bilinear"
+    if path == "ahd":
+        edge, demosaic = F.ahd_spec(info, origin), "ahd"
+    elif path == "edge":
+        cx, cy = F.edge_plan(info, origin)
+        edge = F.EdgeSpec(period=2, cx=cx, cy=cy, beta=0.1, halo=8, canon=F.EDGE_CANON, g0=tables[0], dir=tables[1])
+        oedge, demosaic = dict(g0=tables[0], dir=tables[1], cx=cx, cy=cy, beta=0.1), "edge"
+    frame = F.GpuFrame.from_u16(arr, w, h).with_origin(origin)
+    for curve in (None, gcurve):
+        for out, orientation in ((F.DevelopOutput.Rgba8, 1), (F.DevelopOutput.Rgba8, 6), (F.DevelopOutput.Rgb16, 3)):
+            spec = F.DevelopSpec(F.Rect.full(w, h), wb=WB, cam_to_rgb=MATRIX, output=out,
+                                 orientation=F.Orientation(orientation), tone=gtone, curve=curve,
+                                 half_size=path == "half", edge=edge, output2=F.DevelopOutput.LinearRgb16)
+            run = ctx.run(frame, info, F.GpuOp.Develop(spec))
+            kw = dict(origin=origin, wb=WB, matrix=MATRIX, demosaic=demosaic, edge=oedge, half_size=path == "half",
+                      **_oracle_kw(info))
+            want = O.develop(arr, _opat(cfa), orientation=orientation, encoding=ENC[out], tone=tone,
+                             curve=None if curve is None else (lut8, lut16), **kw)
+            got = run.image.download()
+            assert np.array_equal(got, want), (path, curve is not None, out.name, orientation)
+            want2 = O.develop(arr, _opat(cfa), encoding="linear16", **kw)
+            assert np.array_equal(run.image2.download(), want2), (path, "linear16 untouched")
