@@ -141,10 +141,11 @@ typedef struct {
     const fro_edge *et;
     /* nonzero taps of g0 per canonical position; taps where some direction is nonzero, per
      * position and colour (mod.rs:130-160) */
-    uint8_t g0_idx[4][49];
-    int g0_n[4];
-    uint8_t dir_idx[4][3][49];
-    int dir_n[4][3];
+    uint8_t g0_idx[36][49];
+    int g0_n[36];
+    uint8_t dir_idx[36][3][49];
+    int dir_n[36][3];
+    int period;
 } Kern;
 
 static uint8_t chan_of(uint8_t colour) { return colour < 3 ? colour : NO_CHANNEL; }
@@ -780,10 +781,14 @@ static int build_kernel(Kern *k, const fro_frame *f, const fro_params *p) {
         if (!ok) return -2;
         k->ahd = 1;
     } else if (p->demosaic == FRO_EDGE) {
-        if (f->cfa_w != 2 || f->cfa_h != 2 || !p->edge || !p->edge->g0 || !p->edge->dir) return -2;
+        const int P = p->edge && p->edge->period ? (int)p->edge->period : 2;
+        if ((P != 2 && P != 6) || (int)f->cfa_w != P || (int)f->cfa_h != P || !p->edge || !p->edge->g0 ||
+            !p->edge->dir)
+            return -2;
         k->edge = 1;
         k->et = p->edge;
-        for (int ph = 0; ph < 4; ph++) {
+        k->period = P;
+        for (int ph = 0; ph < P * P; ph++) {
             k->g0_n[ph] = 0;
             for (int i = 0; i < 49; i++)
                 if (p->edge->g0[ph * 49 + i] != 0.0f) k->g0_idx[ph][k->g0_n[ph]++] = (uint8_t)i;
@@ -829,7 +834,21 @@ static const float *rb_row(const RowBlock *rb, const Kern *k, size_t y) {
 
 /* Canonical position index (row * 2 + column) of frame sample (x, y). */
 static inline int edge_ph(const Kern *k, size_t x, size_t y) {
-    return (int)(((y + k->et->cy) & 1) * 2 + ((x + k->et->cx) & 1));
+    const size_t P = (size_t)k->period;
+    return (int)(((y + k->et->cy) % P) * P + (x + k->et->cx) % P);
+}
+
+/* The median of nine values (median_row, edgeaware/mod.rs:825-858: a selection, so no rounding). */
+static float median9(const float v[9]) {
+    float a[9];
+    memcpy(a, v, sizeof(a));
+    for (int i = 1; i < 9; i++)
+        for (int j = i; j > 0 && a[j - 1] > a[j]; j--) {
+            float t = a[j];
+            a[j] = a[j - 1];
+            a[j - 1] = t;
+        }
+    return a[4];
 }
 
 /* Camera RGB (balanced) of frame rows [y0, y1) x columns [xa, xb) into out (3 floats per pixel).
@@ -922,7 +941,8 @@ static void edge_band(const Kern *k, const RowBlock *rb, size_t y0, size_t y1, s
             }
         }
 #define PS(c, y, x) P[c][((y) - EY0) * ec + ((x) - EX0)]
-    /* 6. pass 1 over +- (1 + h): gated mean of the measured differences of the 3x3 window */
+    /* 6. pass 1 over +- (1 + h): Bayer, the gated mean of the measured differences of the 3x3
+     *    window; X-Trans, the median of the differences of the filtered planes (mod.rs:455-530) */
     const size_t H1 = 1 + (size_t)h, e1r = nr + 2 * H1, e1c = nc + 2 * H1, QY0 = y0 - H1, QX0 = xa - H1;
     float *d1[2];
     for (int cc = 0; cc < 2; cc++) d1[cc] = (float *)malloc(e1r * e1c * sizeof(float));
@@ -930,6 +950,18 @@ static void edge_band(const Kern *k, const RowBlock *rb, size_t y0, size_t y1, s
         for (size_t j = 0; j < e1c; j++) {
             size_t y = QY0 + i, x = QX0 + j;
             float g = PS(1, y, x);
+            int own = t->canon[edge_ph(k, x, y)];
+            if (k->period != 2) {
+                for (int cc = 0; cc < 2; cc++) {
+                    const int c = cc ? 2 : 0;
+                    float df[9];
+                    for (int dy = 0; dy < 3; dy++)
+                        for (int dx = 0; dx < 3; dx++)
+                            df[dy * 3 + dx] = PS(c, y - 1 + dy, x - 1 + dx) - PS(1, y - 1 + dy, x - 1 + dx);
+                    d1[cc][i * e1c + j] = own == c ? df[4] : median9(df);
+                }
+                continue;
+            }
             float inv = 1.0f / (0.2f * rmax(g, 0.0f) + 0.015f);
             float sw[2] = {0, 0}, sd[2] = {0, 0}, su[2] = {0, 0}, n[2] = {0, 0};
             for (int dy = 0; dy < 3; dy++)
@@ -948,7 +980,6 @@ static void edge_band(const Kern *k, const RowBlock *rb, size_t y0, size_t y1, s
                         n[cc] += mk;
                     }
                 }
-            int own = t->canon[edge_ph(k, x, y)];
             for (int cc = 0; cc < 2; cc++) {
                 float mixed = sw[cc] > 1e-12f ? sd[cc] / sw[cc] : su[cc] / n[cc];
                 d1[cc][i * e1c + j] = own == (cc ? 2 : 0) ? PS(cc ? 2 : 0, y, x) - g : mixed;
@@ -956,6 +987,7 @@ static void edge_band(const Kern *k, const RowBlock *rb, size_t y0, size_t y1, s
         }
     /* 7. pass 2 over +- h: the same gate over the differences of every pixel */
     const size_t e2r = nr + 2 * (size_t)h, e2c = nc + 2 * (size_t)h, VY0 = y0 - (size_t)h, VX0 = xa - (size_t)h;
+    const float xg = k->period == 6 ? rclamp(1.0f - 2.0f * t->beta, 0.0f, 1.0f) : 0.0f;
     float *V = (float *)malloc(e2r * e2c * 3 * sizeof(float));
     for (size_t i = 0; i < e2r; i++)
         for (size_t j = 0; j < e2c; j++) {
@@ -976,10 +1008,15 @@ static void edge_band(const Kern *k, const RowBlock *rb, size_t y0, size_t y1, s
                 }
             float dr = sdr / sw, db = sdb / sw;
             int own = t->canon[edge_ph(k, x, y)];
+            /* X-Trans: green at a red or blue site moves towards the measured sample minus the
+             * cleaned difference, by xg = clamp(1 - 2 beta, 0, 1) (mod.rs:547-592) */
+            float gv = g;
+            if (xg != 0.0f && own == 0) gv = g + xg * ((PS(0, y, x) - dr) - g);
+            if (xg != 0.0f && own == 2) gv = g + xg * ((PS(2, y, x) - db) - g);
             float *o = V + (i * e2c + j) * 3;
-            o[0] = own == 0 ? PS(0, y, x) : g + dr;
-            o[1] = g;
-            o[2] = own == 2 ? PS(2, y, x) : g + db;
+            o[0] = own == 0 ? PS(0, y, x) : gv + dr;
+            o[1] = gv;
+            o[2] = own == 2 ? PS(2, y, x) : gv + db;
         }
     /* 8. gated smoothing of the core on noisy frames (beta > 0) */
     for (size_t i = 0; i < nr; i++)
@@ -1395,20 +1432,20 @@ static int cmp_f32(const void *a, const void *b) {
  * |a - b| / sqrt(mid); fewer than 1000 values give 0. Otherwise the value of rank len / 5 (in the
  * total order; every value here is finite and non-negative) is q, sigma = q / (sqrt(2) 0.2533),
  * nu = sigma^2 and beta = 0.5 clamp((nu - 2e-4) / 6e-4, 0, 1). */
-float fro_edge_beta(const float *norm, uint32_t w, uint32_t h, const uint8_t canon[4], uint32_t cx, uint32_t cy,
-                    float g_gain) {
+float fro_edge_beta(const float *norm, uint32_t w, uint32_t h, const uint8_t *canon, uint32_t period, uint32_t cx,
+                    uint32_t cy, float g_gain) {
     if (h < 16 || w < 16) return 0.0f;
-    const size_t rows = 48;
-    float *v = (float *)malloc(rows * (size_t)w * sizeof(float));
+    const size_t rows = 48, P = period;
+    float *v = (float *)malloc(rows * (size_t)w * 2 * sizeof(float));
     if (!v) return -1.0f;
     size_t n = 0;
     for (size_t r = 0; r < rows; r++) {
         size_t y = 4 + r * (h - 9) / (rows - 1);
         const float *row = norm + y * (size_t)w;
         for (size_t x = 0; x + 2 < w; x++) {
-            if (canon[((y + cy) % 2) * 2 + (x + cx) % 2] != 1) continue;
+            if (canon[((y + cy) % P) * P + (x + cx) % P] != 1) continue;
             for (size_t d = 1; d <= 2; d++) {
-                if (canon[((y + cy) % 2) * 2 + (x + d + cx) % 2] != 1) continue;
+                if (canon[((y + cy) % P) * P + (x + d + cx) % P] != 1) continue;
                 float a = row[x] * g_gain, b = row[x + d] * g_gain;
                 float mid = 0.5f * (a + b);
                 if (mid > 0.03f && mid < 0.6f) v[n++] = fabsf(a - b) / sqrtf(mid);

@@ -41,17 +41,18 @@ typedef struct fro_frame {
 
 enum fro_demosaic { FRO_BILINEAR = 0, FRO_AHD = 1, FRO_EDGE = 2 };
 
-/* Tables of the edge-aware demosaic (crates/flashraw/src/edgeaware/mod.rs:57-160), Bayer
- * (period 2): g0[4][49] (the non-directional green filter per canonical position) and
- * dir[4][3][4][49] (per canonical position, colour and direction), taps row-major over 7x7;
- * canon[4] the colours of the canonical pattern; (cx, cy) the canonical position of frame
- * sample (0, 0); beta the weight of the gated smoothing. */
+/* Tables of the edge-aware demosaic (crates/flashraw/src/edgeaware/mod.rs:57-160) of a pattern
+ * of `period` 2 (Bayer) or 6 (X-Trans): g0[P*P][49] (the non-directional green filter per
+ * canonical position) and dir[P*P][3][4][49] (per canonical position, colour and direction),
+ * taps row-major over 7x7; canon[P*P] the colours of the canonical pattern; (cx, cy) the
+ * canonical position of frame sample (0, 0); beta the weight of the gated smoothing. */
 typedef struct fro_edge {
     const float *g0;
     const float *dir;
-    uint8_t canon[4];
+    uint8_t canon[36];
     uint32_t cx, cy;
     float beta;
+    uint32_t period; /* 0 means 2 */
 } fro_edge;
 
 /* Encodings (crates/flashraw/src/process.rs:770-781), plus 8-bit RGB without alpha. */
@@ -124,11 +125,11 @@ int fro_unpack(const uint8_t *src, uint64_t len, uint32_t w, uint32_t h, uint32_
                uint16_t *out);
 /* The linearisation curve of integer samples: v -> lut[min(v, len - 1)] (run.rs:38-39). */
 void fro_linearize(uint16_t *v, uint64_t n, const uint16_t *lut, uint32_t lut_len);
-/* Plan::set_noise_from (crates/flashraw/src/edgeaware/mod.rs:369-409) for a Bayer plan over the
- * normalised frame `norm` (w x h): the smoothing weight of the edge-aware demosaic. canon[4] are
- * the colours of the canonical pattern, (cx, cy) the canonical position of frame sample (0, 0). */
-float fro_edge_beta(const float *norm, uint32_t w, uint32_t h, const uint8_t canon[4], uint32_t cx, uint32_t cy,
-                    float g_gain);
+/* Plan::set_noise_from (crates/flashraw/src/edgeaware/mod.rs:369-409) over the normalised frame
+ * `norm` (w x h): the smoothing weight of the edge-aware demosaic. canon[period^2] are the
+ * colours of the canonical pattern, (cx, cy) the canonical position of frame sample (0, 0). */
+float fro_edge_beta(const float *norm, uint32_t w, uint32_t h, const uint8_t *canon, uint32_t period, uint32_t cx,
+                    uint32_t cy, float g_gain);
 
 /* ---- CFA-plane DCT block denoiser (build-defined workload, no reference counterpart) ----
  * The mosaic is split into its four 2x2 phase planes (W/2 x H/2 each, black subtracted, in DN);

@@ -59,10 +59,11 @@ class FroEdge(ctypes.Structure):
     _fields_ = [
         ("g0", ctypes.c_void_p),
         ("dir", ctypes.c_void_p),
-        ("canon", ctypes.c_uint8 * 4),
+        ("canon", ctypes.c_uint8 * 36),
         ("cx", ctypes.c_uint32),
         ("cy", ctypes.c_uint32),
         ("beta", ctypes.c_float),
+        ("period", ctypes.c_uint32),
     ]
 
 
@@ -126,7 +127,7 @@ def lib():
         L.fro_linearize.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_uint32]
         L.fro_edge_beta.restype = ctypes.c_float
         L.fro_edge_beta.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p,
-                                    ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float]
+                                    ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float]
         L.fro_dct_denoise.restype = ctypes.c_int
         L.fro_dct_denoise.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p,
                                       ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_uint32,
@@ -230,10 +231,12 @@ def develop(mosaic, cfa, *, origin=(0, 0), black=(1, 1, 1, [0.0]), white=(16383,
     if edge is not None:
         g0 = np.ascontiguousarray(edge["g0"], dtype=np.float32).reshape(-1)
         dr = np.ascontiguousarray(edge["dir"], dtype=np.float32).reshape(-1)
-        if g0.size != 4 * 49 or dr.size != 4 * 3 * 4 * 49:
+        period = int(edge.get("period", 2))
+        if g0.size != period * period * 49 or dr.size != period * period * 3 * 4 * 49:
             raise ValueError("edge-aware tables have the wrong size")
         ed = FroEdge()
         ed.g0, ed.dir = g0.ctypes.data, dr.ctypes.data
+        ed.period = period
         for i, c in enumerate(edge.get("canon", (0, 1, 1, 2))):
             ed.canon[i] = c
         ed.cx, ed.cy, ed.beta = edge["cx"], edge["cy"], float(edge.get("beta", 0.0))
@@ -316,8 +319,10 @@ def linearize(v, lut):
 
 
 def edge_beta(norm, cx, cy, g_gain, canon=(0, 1, 1, 2)):
-    """Plan::set_noise_from (edgeaware/mod.rs:369-409) over a normalised Bayer frame."""
+    """Plan::set_noise_from (edgeaware/mod.rs:369-409) over a normalised frame; `canon` the
+    canonical pattern (4 or 36 colours)."""
     norm = np.ascontiguousarray(norm, dtype=np.float32)
     c = np.ascontiguousarray(canon, dtype=np.uint8)
     h, w = norm.shape
-    return lib().fro_edge_beta(norm.ctypes.data, w, h, c.ctypes.data, cx, cy, g_gain)
+    period = 6 if c.size == 36 else 2
+    return lib().fro_edge_beta(norm.ctypes.data, w, h, c.ctypes.data, period, cx, cy, g_gain)

@@ -354,6 +354,31 @@ def edge_plan(cfa, origin):
     raise ValueError("not a Bayer pattern")
 
 
+def edge_tables_xtrans(seed):
+    """Synthetic X-Trans tables (g0 [36][49], dir [36][3][4][49]) with the trained tables'
+    layout: a 5x5 support inside the 7x7 (the outer ring zero), the own colour's directional
+    filters zero, every filter non-negative and summing to 1."""
+    rng = np.random.default_rng(seed)
+    canon = parse_cfa(XTRANS).reshape(-1)
+    ring = np.zeros((7, 7), bool)
+    ring[1:6, 1:6] = True
+    ring = ring.reshape(-1)
+
+    def filt():
+        f = np.where(ring, rng.random(49) ** 2, 0.0)
+        f[24] += 0.5
+        return (f / f.sum()).astype(F32)
+
+    g0 = np.stack([filt() for _ in range(36)])
+    d = np.zeros((36, 3, 4, 49), F32)
+    for k in range(36):
+        for c in range(3):
+            if c != canon[k]:
+                for dd in range(4):
+                    d[k, c, dd] = filt()
+    return g0, d
+
+
 def edge_tables(seed, sparse=True):
     """Synthetic filter tables in the layout of bayer.bin (edgeaware/mod.rs:130-160): g0
     [4][49] and dir [4][3][4][49], each filter non-negative with zero taps and summing to 1 (so a
@@ -372,16 +397,30 @@ def edge_tables(seed, sparse=True):
     return g0, d
 
 
+def edge_plan_p(cfa, origin):
+    """Plan::new (edgeaware/mod.rs:204-230) for a Bayer or X-Trans pattern: (cx, cy, canonical
+    colours, period)."""
+    p = parse_cfa(cfa)
+    P = p.shape[0]
+    canon = CANON if P == 2 else [c for c in parse_cfa(XTRANS).reshape(-1)]
+    for sy in range(P):
+        for sx in range(P):
+            if all(p[y, x] == canon[((y + sy) % P) * P + (x + sx) % P] for y in range(P) for x in range(P)):
+                return (sx + origin[0]) % P, (sy + origin[1]) % P, list(canon), P
+    raise ValueError("neither Bayer nor X-Trans")
+
+
 def edge_aware(norm, cfa, origin, wb, g0, dirt, beta):
     """The edge-aware demosaic (edgeaware/mod.rs:423-822) stated over whole-frame planes, one
-    plane per canonical phase for the filters. Returns camera RGB valid at distance >= 8 from
-    the frame border."""
+    plane per canonical phase for the filters; Bayer or X-Trans (pass 1: the 3x3 median of the
+    filtered differences; pass 2: the green re-estimate at red and blue sites). Returns camera
+    RGB valid at distance >= 8 from the frame border."""
     h, w = norm.shape
-    cx, cy = edge_plan(cfa, origin)
-    ys = (np.arange(h)[:, None] + cy) % 2
-    xs = (np.arange(w)[None, :] + cx) % 2
-    ph = ys * 2 + xs
-    canon = np.asarray(CANON)[ph]
+    cx, cy, canon_list, P = edge_plan_p(cfa, origin)
+    ys = (np.arange(h)[:, None] + cy) % P
+    xs = (np.arange(w)[None, :] + cx) % P
+    ph = ys * P + xs
+    canon = np.asarray(canon_list)[ph]
     wbv = np.asarray(wb, dtype=F32)
     m = np.minimum(norm * wbv[canon], F32(1.0)).astype(F32)
     nan = F32(np.nan)
@@ -398,7 +437,7 @@ def edge_aware(norm, cfa, origin, wb, g0, dirt, beta):
     shifted = {(ty, tx): sh(m, ty - 3, tx - 3) for ty in range(7) for tx in range(7)}
     with np.errstate(invalid="ignore", divide="ignore", over="ignore"):
         g0p = np.full((h, w), nan, dtype=F32)
-        for k in range(4):
+        for k in range(P * P):
             acc = np.zeros((h, w), dtype=F32)
             for idx in range(49):
                 c = g0[k, idx]
@@ -421,7 +460,7 @@ def edge_aware(norm, cfa, origin, wb, g0, dirt, beta):
         planes = []
         for c in range(3):
             pl = np.full((h, w), nan, dtype=F32)
-            for k in range(4):
+            for k in range(P * P):
                 a = [np.zeros((h, w), dtype=F32) for _ in range(4)]
                 for idx in range(49):
                     cs = dirt[k, c, :, idx]
@@ -449,8 +488,12 @@ def edge_aware(norm, cfa, origin, wb, g0, dirt, beta):
                     acc[cc] = [sw + mk * wt, sd + (mk * wt) * d, su + mk * d, n + mk]
         d1p = []
         for cc, ch, pl in ((0, 0, R_), (1, 2, B_)):
-            sw, sd, su, n = acc[cc]
-            mixed = np.where(sw > F32(1e-12), sd / sw, su / n)
+            if P == 2:
+                sw, sd, su, n = acc[cc]
+                mixed = np.where(sw > F32(1e-12), sd / sw, su / n)
+            else:  # X-Trans: the 3x3 median of the filtered differences
+                df = (pl - G_).astype(F32)
+                mixed = np.median(np.stack([sh(df, dy, dx) for dy in (-1, 0, 1) for dx in (-1, 0, 1)]), axis=0)
             d1p.append(np.where(canon == ch, pl - G_, mixed).astype(F32))
         # pass 2: the same gate over every pixel's difference
         inv2 = F32(1.0) / (F32(0.1) * np.maximum(G_, F32(0.0)) + F32(0.007))
@@ -465,7 +508,13 @@ def edge_aware(norm, cfa, origin, wb, g0, dirt, beta):
                 sdr = sdr + wt * sh(d1p[0], dy, dx)
                 sdb = sdb + wt * sh(d1p[1], dy, dx)
         dr, db = sdr / sw, sdb / sw
-        v = np.stack([np.where(canon == 0, R_, G_ + dr), G_, np.where(canon == 2, B_, G_ + db)], axis=-1).astype(F32)
+        gv = G_
+        if P == 6:
+            xg = F32(min(max(F32(1.0) - F32(2.0) * F32(beta), F32(0.0)), F32(1.0)))
+            if xg != 0:
+                gv = np.where(canon == 0, G_ + xg * ((R_ - dr) - G_),
+                              np.where(canon == 2, G_ + xg * ((B_ - db) - G_), G_)).astype(F32)
+        v = np.stack([np.where(canon == 0, R_, gv + dr), gv, np.where(canon == 2, B_, gv + db)], axis=-1).astype(F32)
         if beta > 0:
             inv3 = F32(1.0) / (F32(0.2) * np.maximum(v[..., 1], F32(0.0)) + F32(0.015))
             sw = np.zeros((h, w), dtype=F32)
@@ -489,7 +538,7 @@ def develop_edge(mos, cfa, *, origin=(0, 0), black=(1, 1, 1, [0.0]), white=(1638
     h, w = mos.shape
     col = colours(cfa, w, h, origin)
     norm = normalise(mos, cfa, origin, black, white)
-    out = finish(bilinear(norm, col), wb, matrix, levels)
+    out = finish(bilinear(norm, col, bayer=parse_cfa(cfa).shape == (2, 2)), wb, matrix, levels)
     if h > 2 * EDGE_HALO and w > 2 * EDGE_HALO:
         e = edge_aware(norm, cfa, origin, wb, g0, dirt, beta)
         fe = finish(e, (1.0, 1.0, 1.0), matrix, levels)

@@ -458,3 +458,42 @@ def test_edge_aware_planted_mistake_is_caught():
     bad = O.develop(mos, "RGGB", black=(1, 1, 1, [64.0]), white=(4095,), demosaic="edge", encoding="linear_f32",
                     edge=dict(edge, g0=g0b))
     assert not np.array_equal(got, bad)
+
+
+# ---------------------------------------------------------------- edge-aware (X-Trans)
+
+
+def _xt_args(origin, seed=0, beta=0.0):
+    g0, d = N.edge_tables_xtrans(seed)
+    cx, cy, canon, period = N.edge_plan_p(N.XTRANS, origin)
+    return g0, d, dict(g0=g0, dir=d, cx=cx, cy=cy, beta=beta, canon=canon, period=period)
+
+
+@pytest.mark.parametrize("origin", [(0, 0), (2, 3), (5, 1)])
+@pytest.mark.parametrize("beta", [0.0, 0.2, 0.6])
+def test_edge_aware_xtrans_equals_per_plane_statement(origin, beta):
+    """X-Trans: 36 canonical positions, pass 1 the median of the filtered differences, pass 2's
+    green re-estimate at red and blue sites with weight clamp(1 - 2 beta, 0, 1) (0 at beta 0.6)."""
+    w, h = 58, 47
+    mos = N.lcg_stream(w, h, seed=17 + origin[0])
+    g0, d, edge = _xt_args(origin, seed=origin[1], beta=beta)
+    kw = dict(origin=origin, black=(1, 1, 1, [64.0]), white=(4095,), wb=WB, matrix=MATRIX)
+    got = O.develop(mos, XTRANS, demosaic="edge", encoding="linear_f32", edge=edge, **kw)
+    want = N.develop_edge(mos, N.XTRANS, g0=g0, dirt=d, beta=beta, **kw)
+    assert np.array_equal(got, want)
+
+
+def test_edge_aware_xtrans_flat_field_and_bands():
+    mos = np.full((50, 60), 1500, dtype=np.uint16)
+    _, _, edge = _xt_args((0, 0), seed=4, beta=0.3)
+    got = O.develop(mos, XTRANS, black=(1, 1, 1, [0.0]), white=(4000,), demosaic="edge", encoding="linear_f32",
+                    edge=edge)
+    # the edge-aware interior (corner pixels of the bilinear border may have no red or blue neighbour)
+    assert np.abs(got[8:-8, 8:-8] - 0.375).max() < 1e-5
+    mos = N.lcg_stream(70, 150, seed=6)
+    _, _, edge = _xt_args((1, 4), seed=2, beta=0.1)
+    kw = dict(origin=(1, 4), black=(1, 1, 1, [64.0]), white=(4095,), wb=WB, matrix=MATRIX, demosaic="edge",
+              encoding="linear_f32", edge=edge)
+    whole = O.develop(mos, XTRANS, **kw)
+    part = O.develop(mos, XTRANS, crop=(9, 70, 45, 66), **kw)
+    assert np.array_equal(part, whole[70:136, 9:54])
