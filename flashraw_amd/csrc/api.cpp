@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <new>
 #include <string>
@@ -373,23 +374,27 @@ const void *frh::device_table(frh_context *ctx, const void *host, uint64_t bytes
 // the context): per canonical position, G0's non-zero taps and, per colour, the taps where some
 // direction is non-zero, in tap order (edgeaware/mod.rs:130-160).
 static int32_t edge_tables(frh_context *ctx, const frh_edge_spec *e, const EdgeTab **out) {
+    const uint32_t np = e->period * e->period;  // canonical positions: 4 (Bayer) or 36 (X-Trans)
     uint64_t hsh = 1469598103934665603ull;
     auto mix = [&](const void *p, size_t n) {
         const uint8_t *b = static_cast<const uint8_t *>(p);
         for (size_t i = 0; i < n; i++) hsh = (hsh ^ b[i]) * 1099511628211ull;
     };
-    mix(e->canon, 4);
-    mix(e->g0, 4 * 49 * sizeof(float));
-    mix(e->dir, 4 * 3 * 4 * 49 * sizeof(float));
+    mix(&np, sizeof(np));
+    mix(e->canon, np);
+    mix(e->g0, np * 49 * sizeof(float));
+    mix(e->dir, np * 3 * 4 * 49 * sizeof(float));
     std::lock_guard<std::mutex> g(ctx->mu);
     auto it = ctx->edge_tabs.find(hsh);
     if (it != ctx->edge_tabs.end()) {
         *out = static_cast<const EdgeTab *>(it->second);
         return FRH_OK;
     }
-    EdgeTab t;
+    std::unique_ptr<EdgeTab> hold(new EdgeTab());
+    EdgeTab &t = *hold;
     memset(&t, 0, sizeof(t));
-    for (int ph = 0; ph < 4; ph++) {
+    for (uint32_t ph = 0; ph < np; ph++) {
+        t.canon[ph] = std::min<uint32_t>(e->canon[ph], 3);
         for (uint32_t i = 0; i < 49; i++) {
             const float c = e->g0[ph * 49 + i];
             if (c != 0.0f) {
@@ -397,15 +402,15 @@ static int32_t edge_tables(frh_context *ctx, const frh_edge_spec *e, const EdgeT
                 t.g0_c[ph][t.g0_n[ph]] = c;
                 t.g0_n[ph]++;
             }
+            if (ph < 4) t.g0d[ph][i] = c;
         }
-        for (uint32_t i = 0; i < 49; i++) t.g0d[ph][i] = e->g0[ph * 49 + i];
         for (int c = 0; c < 3; c++)
             for (uint32_t i = 0; i < 49; i++) {
                 float v[4];
                 bool any = false;
                 for (int d = 0; d < 4; d++) {
                     v[d] = e->dir[((ph * 3 + c) * 4 + d) * 49 + i];
-                    t.dird[ph][c][i][d] = v[d];
+                    if (ph < 4) t.dird[ph][c][i][d] = v[d];
                     any = any || v[d] != 0.0f;
                 }
                 if (!any) continue;
@@ -443,8 +448,9 @@ static bool edge_tame(const DevArgs &a, const frh_frame_info *info, const frh_ed
     for (uint32_t i = 0; i < a.n_rowd; i++) rmax = std::max(rmax, (double)std::fabs(info->black_row_deltas[i]));
     for (uint32_t i = 0; i < a.n_cold; i++) cdmax = std::max(cdmax, (double)std::fabs(info->black_col_deltas[i]));
     for (int i = 0; i < 3; i++) wmax = std::max(wmax, (double)std::fabs(a.wb[i]));
-    for (uint32_t i = 0; i < 4 * 49; i++) cmax = std::max(cmax, (double)std::fabs(e->g0[i]));
-    for (uint32_t i = 0; i < 4 * 3 * 4 * 49; i++) cmax = std::max(cmax, (double)std::fabs(e->dir[i]));
+    const uint32_t np = e->period * e->period;
+    for (uint32_t i = 0; i < np * 49; i++) cmax = std::max(cmax, (double)std::fabs(e->g0[i]));
+    for (uint32_t i = 0; i < np * 3 * 4 * 49; i++) cmax = std::max(cmax, (double)std::fabs(e->dir[i]));
     const double m = (65535.0 + bmax + rmax + cdmax) * smax * wmax * 1.001;
     return 49.0 * m * cmax <= 1e15;
 }
@@ -647,18 +653,22 @@ int32_t frh::plan_develop(const frh_context *ctx, const frh_frame *f, const frh_
     if (spec->edge && !spec->half_size && cpp == 1 && has_cfa) {
         const frh_edge_spec *e = spec->edge;
         if (e->size < sizeof(frh_edge_spec)) return fail(FRH_ERROR_INVALID_ARGUMENT, "edge spec size too small");
-        if (!bayer)
-            return fail(FRH_ERROR_NOT_AVAILABLE, "the AHD and edge-aware demosaics take Bayer mosaics on this path");
-        if (e->period != 2)
-            return fail(FRH_ERROR_NOT_AVAILABLE, "only Bayer (period 2) edge specs are supported by this path");
+        const uint32_t P = e->period;
+        if (e->ahd && P != 2) return fail(FRH_ERROR_NOT_AVAILABLE, "the AHD demosaic takes period-2 specs");
+        if (e->ahd ? !bayer : !(P == 2 ? bayer : (P == 6 && info->cfa_width == 6 && info->cfa_height == 6)))
+            return fail(FRH_ERROR_NOT_AVAILABLE,
+                        "the AHD demosaic takes Bayer mosaics, the edge-aware one Bayer (period 2) or 6x6 "
+                        "(period 6) mosaics on this path");
         if (e->canon) {
-            if (e->canon_len != 4) return fail(FRH_ERROR_INVALID_ARGUMENT, "inconsistent edge-aware filter tables");
+            if (e->canon_len != P * P) return fail(FRH_ERROR_INVALID_ARGUMENT, "inconsistent edge-aware filter tables");
             // The pattern the spec describes must be the frame's: colour of frame (x, y) is
-            // canon[((y + cy) % 2) * 2 + (x + cx) % 2].
-            for (uint32_t y = 0; y < 2; y++)
-                for (uint32_t x = 0; x < 2; x++) {
-                    uint32_t want = a.chan[((y + a.oy) & 1) * 2 + ((x + a.ox) & 1)];
-                    if (e->canon[((y + e->cy) % 2) * 2 + (x + e->cx) % 2] != want)
+            // canon[((y + cy) % P) * P + (x + cx) % P].
+            for (uint32_t y = 0; y < P; y++)
+                for (uint32_t x = 0; x < P; x++) {
+                    const uint32_t want =
+                        P == 2 ? a.chan[((y + a.oy) & 1) * 2 + ((x + a.ox) & 1)]
+                               : std::min<uint32_t>(info->cfa[((y + a.oy) % 6) * 6 + (x + a.ox) % 6], 3);
+                    if (std::min<uint32_t>(e->canon[((y + e->cy) % P) * P + (x + e->cx) % P], 3) != want)
                         return fail(FRH_ERROR_INVALID_ARGUMENT, "edge spec pattern does not match the frame's CFA");
                 }
         }
@@ -675,27 +685,37 @@ int32_t frh::plan_develop(const frh_context *ctx, const frh_frame *f, const frh_
             a.ahd_rows = rows ? (uint32_t)atoi(rows) : tuned_rows(ctx, true, a, ctx->ahd_rows);
             p.ahd = true;
         } else {
-            // the edge-aware filters (edgeaware/mod.rs:57-160): tables of a Bayer plan
-            if (!e->canon || !e->g0 || !e->dir || e->g0_len != 4 * 49 || e->dir_len != 4 * 3 * 4 * 49)
-                return fail(FRH_ERROR_INVALID_ARGUMENT, "edge-aware spec needs canon[4], g0[4*49] and dir[4*3*4*49]");
+            // the edge-aware filters (edgeaware/mod.rs:57-160): P * P canonical positions
+            if (!e->canon || !e->g0 || !e->dir || e->g0_len != P * P * 49 || e->dir_len != P * P * 3 * 4 * 49)
+                return fail(FRH_ERROR_INVALID_ARGUMENT,
+                            "edge-aware spec needs canon[P*P], g0[P*P*49] and dir[P*P*3*4*49]");
             if (!(e->beta >= 0.0f && e->beta <= 1.0f))
                 return fail(FRH_ERROR_INVALID_ARGUMENT, "edge-aware beta must be in [0, 1]");
             if (e->halo < 8 || e->halo > 64)
                 return fail(FRH_ERROR_INVALID_ARGUMENT, "edge-aware halo must be in [8, 64]");
-            for (uint32_t i = 0; i < 4 * 49; i++)
+            for (uint32_t i = 0; i < P * P * 49; i++)
                 if (!std::isfinite(e->g0[i])) return fail(FRH_ERROR_INVALID_ARGUMENT, "non-finite filter tap");
-            for (uint32_t i = 0; i < 4 * 3 * 4 * 49; i++)
+            for (uint32_t i = 0; i < P * P * 3 * 4 * 49; i++)
                 if (!std::isfinite(e->dir[i])) return fail(FRH_ERROR_INVALID_ARGUMENT, "non-finite filter tap");
             int32_t rc = edge_tables(const_cast<frh_context *>(ctx), e, &a.etab);
             if (rc) return rc;
             a.edge_aware = 1;
-            a.ecx = e->cx % 2;
-            a.ecy = e->cy % 2;
+            a.eperiod = P;
+            a.ecx = e->cx % P;
+            a.ecy = e->cy % P;
             a.ebeta = e->beta;
+            // xg = (1 - 2 beta).clamp(0, 1) (edgeaware/mod.rs:547-550), f32 as the reference
+            {
+                const float xg = 1.0f - 2.0f * e->beta;
+                a.exg = xg < 0.0f ? 0.0f : (xg > 1.0f ? 1.0f : xg);
+            }
             a.ahd_halo = e->halo;
             a.ecanon_bits = 0;
-            for (int i = 0; i < 4; i++) a.ecanon_bits |= (uint32_t)(e->canon[i] & 3u) << (2 * i);
-            a.edge_tame = edge_tame(a, info, e) ? 1u : 0u;
+            if (P == 2)
+                for (int i = 0; i < 4; i++) a.ecanon_bits |= (uint32_t)(e->canon[i] & 3u) << (2 * i);
+            // 6x6 mosaics: the border pixels take the generic CFA average (sites, gcal stay set)
+            a.generic = 0;
+            a.edge_tame = (P == 2 && edge_tame(a, info, e)) ? 1u : 0u;
         }
     }
     int32_t modes[2] = {spec->output, spec->output2};

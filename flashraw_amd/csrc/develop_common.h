@@ -395,15 +395,15 @@ __host__ inline Border make_border(const DevArgs &a, int halo) {
     return b;
 }
 
-template <bool FULL = true>
-__device__ __forceinline__ void border_pixel(const DevArgs &a, const Lds &s, const Border &B, uint32_t i) {
+// Developed coordinates of border pixel i of a band: the rows above the interior, the columns
+// left and right of it, the rows below.
+__device__ __forceinline__ void border_xy(const DevArgs &a, const Border &B, uint32_t i, int &dx, int &dy) {
     const int dw = (int)a.dev_w;
     const int b0 = (int)a.band_y0;
     const int top = B.iy0 - b0;
     const int mid = B.iy1 - B.iy0;
     const int side = B.ix0 + (dw - B.ix1);
     const uint32_t ntop = (uint32_t)top * dw, nmid = (uint32_t)mid * side;
-    int dx, dy;
     if (i < ntop) {
         dy = b0 + (int)(i / dw);
         dx = (int)(i % dw);
@@ -417,11 +417,68 @@ __device__ __forceinline__ void border_pixel(const DevArgs &a, const Lds &s, con
         dy = B.iy1 + (int)(j / dw);
         dx = (int)(j % dw);
     }
+}
+
+template <bool FULL = true>
+__device__ __forceinline__ void border_pixel(const DevArgs &a, const Lds &s, const Border &B, uint32_t i) {
+    int dx, dy;
+    border_xy(a, B, i, dx, dy);
     float rgb[3];
     bilinear_global<FULL>(a, s, (int)a.crop_x + dx, (int)a.crop_y + dy, rgb);
     float o0, o1, o2;
     finish(a, rgb[0], rgb[1], rgb[2], false, o0, o1, o2);
     store_both<FULL>(a, s, (uint32_t)dx, (uint32_t)dy, o0, o1, o2);
+}
+
+// Calib::norm (process.rs:288-298) of component c of window sample (x, y) from the generic table
+// gcal (black[cpp][pattern] then scale[cpp][pattern]).
+__device__ __forceinline__ float gen_norm(const DevArgs &a, uint32_t x, uint32_t y, uint32_t c) {
+    const size_t at = (size_t)y * a.stride + (size_t)x * a.cpp + c;
+    const float v = a.srcf ? a.srcf[at] : (float)a.src[at];
+    const uint32_t X = x + a.ox, Y = y + a.oy;
+    const uint32_t P = a.cal_pw * a.cal_ph;
+    const uint32_t i = c * P + (Y % a.cal_ph) * a.cal_pw + X % a.cal_pw;
+    float b = a.gcal[i];
+    if (a.n_rowd) b += Y < a.n_rowd ? a.rowd[Y] : 0.0f;
+    if (a.n_cold) b += X < a.n_cold ? a.cold[X] : 0.0f;
+    return (v - b) * a.gcal[a.cpp * P + i];
+}
+
+// Site descriptor of window sample (x, y): own colour in bits 0..1, then per colour R, G, B the
+// 9-bit mask of its 3x3 neighbours of that colour.
+__device__ __forceinline__ uint32_t site_at(const DevArgs &a, uint32_t x, uint32_t y) {
+    return a.sites[((y + a.oy) % a.cfa_h) * a.cfa_w + (x + a.ox) % a.cfa_w];
+}
+
+// Kernel::row_cols for a single-component CFA other than Bayer (process.rs:1098-1126): each colour
+// is the mean of the same-colour samples of the 3x3 neighbourhood, sum * (1 / n) inside the frame
+// and sum / n over the neighbours inside at the border, neighbours in row-major order.
+__device__ __forceinline__ void cfa_mean_rgb(const DevArgs &a, uint32_t x, uint32_t y, float &r, float &g, float &b) {
+    const uint32_t site = site_at(a, x, y);
+    const bool inside = x > 0 && x + 1 < a.width && y > 0 && y + 1 < a.height;
+    float v[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const uint32_t m = (site >> (2 + 9 * c)) & 0x1ffu;
+        float sum = 0.0f, n = 0.0f;
+#pragma unroll
+        for (int t = 0; t < 9; t++) {
+            if (!((m >> t) & 1u)) continue;
+            const int xx = (int)x + t % 3 - 1, yy = (int)y + t / 3 - 1;
+            if (!inside && (xx < 0 || xx >= (int)a.width || yy < 0 || yy >= (int)a.height)) continue;
+            sum += gen_norm(a, (uint32_t)xx, (uint32_t)yy, 0);
+            n += 1.0f;
+        }
+        if (inside) {
+            const uint32_t cnt = __builtin_popcount(m);
+            v[c] = sum * (cnt ? 1.0f / (float)cnt : 0.0f);
+        } else {
+            v[c] = n > 0.0f ? sum / n : 0.0f;
+        }
+    }
+    r = v[0];
+    g = v[1];
+    b = v[2];
 }
 
 // Red's index in the frame-parity colour table fc[(y & 1) * 2 + (x & 1)] (the walkers' template

@@ -65,7 +65,74 @@ __device__ __forceinline__ void phase_grid(const DevArgs &a, int k, int x0, int 
     nc = rw > fc ? (rw - fc + 1) / 2 : 0;
 }
 
-template <bool FAST>
+// 6x6 mosaics (X-Trans): canonical position of frame sample (x, y), x, y >= 0.
+__device__ __forceinline__ int edge_phase6(const DevArgs &a, int x, int y) {
+    return (int)((((uint32_t)y + a.ecy) % 6u) * 6u + ((uint32_t)x + a.ecx) % 6u);
+}
+
+// Colour of frame sample (x, y): Bayer from the packed canonical colours, 6x6 from the table.
+template <int P>
+__device__ __forceinline__ uint32_t colour_at(const DevArgs &a, const ctab_t *T, int x, int y) {
+    if constexpr (P == 2)
+        return canon_of(a, edge_phase(a, x, y));
+    else
+        return T->canon[edge_phase6(a, x, y)];
+}
+
+// Phase-major enumeration of a plane of RW x RH starting at frame (x0, y0) for period 6: item i is
+// pixel j of canonical position k = i / (NX * NY), NX x NY being the most columns and rows one
+// position has in the plane; returns false for the items beyond a position's own count. A wave
+// then walks at most two positions' tap lists.
+template <int RW, int RH>
+struct Phase6 {
+    static constexpr int NX = (RW + 5) / 6, NY = (RH + 5) / 6, ITEMS = 36 * NX * NY;
+    __device__ static __forceinline__ bool at(const DevArgs &a, int i, int x0, int y0, int &k, int &ry, int &rx) {
+        k = i / (NX * NY);
+        const int j = i - k * (NX * NY);
+        const int fr = (int)((uint32_t)(k / 6) + 6u - ((uint32_t)y0 + a.ecy) % 6u) % 6;
+        const int fc = (int)((uint32_t)(k % 6) + 6u - ((uint32_t)x0 + a.ecx) % 6u) % 6;
+        ry = fr + 6 * (j / NX);
+        rx = fc + 6 * (j % NX);
+        return ry < RH && rx < RW;
+    }
+};
+
+// Median of three (med3, edgeaware/mod.rs:857-860).
+__device__ __forceinline__ float med3(float a, float b, float c) { return fmaxf(fminf(a, b), fminf(fmaxf(a, b), c)); }
+
+// Median of the 3x3 window of differences centred on (y, x) of the colour planes, by the column
+// scheme of median_row (edgeaware/mod.rs:825-855): each column of three sorted, then the median of
+// (largest column minimum, median of the column medians, smallest column maximum). A median is
+// one of the inputs, so no rounding is involved.
+__device__ __forceinline__ float median_diff(const float *pc, const float *pg, int o, int pw) {
+    float lo[3], mi[3], hi[3];
+#pragma unroll
+    for (int dx = 0; dx < 3; dx++) {
+        const int q = o - pw - 1 + dx;
+        const float a0 = pc[q] - pg[q], b0 = pc[q + pw] - pg[q + pw], c0 = pc[q + 2 * pw] - pg[q + 2 * pw];
+        const float x1 = fminf(a0, b0), y1 = fmaxf(a0, b0);
+        const float y2 = fminf(y1, c0), z2 = fmaxf(y1, c0);
+        lo[dx] = fminf(x1, y2);
+        mi[dx] = fmaxf(x1, y2);
+        hi[dx] = z2;
+    }
+    const float l = fmaxf(fmaxf(lo[0], lo[1]), lo[2]);
+    const float h = fminf(fminf(hi[0], hi[1]), hi[2]);
+    return med3(l, med3(mi[0], mi[1], mi[2]), h);
+}
+
+// Border pixel i of a 6x6 mosaic: the generic CFA mean (process.rs:1098-1126).
+__device__ __forceinline__ void border_pixel_cfa(const DevArgs &a, const Lds &s, const Border &B, uint32_t i) {
+    int dx, dy;
+    border_xy(a, B, i, dx, dy);
+    float r, g, b;
+    cfa_mean_rgb(a, a.crop_x + (uint32_t)dx, a.crop_y + (uint32_t)dy, r, g, b);
+    float o0, o1, o2;
+    finish(a, r, g, b, false, o0, o1, o2);
+    store_both(a, s, (uint32_t)dx, (uint32_t)dy, o0, o1, o2);
+}
+
+template <int P>
 __global__ __launch_bounds__(256) void k_develop_edge(DevArgs a, Border B, int tiles_x, uint32_t tiles) {
     __shared__ Lds s;
     __shared__ EdgeLds E;
@@ -73,7 +140,12 @@ __global__ __launch_bounds__(256) void k_develop_edge(DevArgs a, Border B, int t
     if (blockIdx.x >= tiles) {
         __syncthreads();
         const uint32_t i = (blockIdx.x - tiles) * blockDim.x + threadIdx.x;
-        if (i < B.count) border_pixel(a, s, B, i);
+        if (i < B.count) {
+            if constexpr (P == 2)
+                border_pixel(a, s, B, i);
+            else
+                border_pixel_cfa(a, s, B, i);
+        }
         return;
     }
     const ctab_t *T = (const ctab_t *)(uintptr_t)a.etab;
@@ -88,25 +160,33 @@ __global__ __launch_bounds__(256) void k_develop_edge(DevArgs a, Border B, int t
     // 1. balanced mosaic min(norm * wb[colour], 1) (mod.rs:627-640), clamped reads beyond the frame
     for (int i = tid; i < MHt * MW; i += 256) {
         const int y = min(max(Y0 - EH + i / MW, 0), H - 1), x = min(max(X0 - EH + i % MW, 0), W - 1);
-        const uint32_t c = canon_of(a, edge_phase(a, x, y));
+        const uint32_t c = colour_at<P>(a, T, x, y);
         const float wbv = c == 0 ? a.wb[0] : (c == 1 ? a.wb[1] : a.wb[2]);
         E.m[i] = fminf(norm_at(a, s, (uint32_t)x, (uint32_t)y) * wbv, 1.0f);
     }
     __syncthreads();
     // 2. sqrt(G0) over +-5, phase by phase (mod.rs:667-694)
-    for (int k = 0; k < 4; k++) {
-        int fr, fc, nr, nc;
-        phase_grid(a, k, X0 - 5, Y0 - 5, SW, SH, fr, fc, nr, nc);
+    auto g0_px = [&](int k, int ry, int rx) {  // (ry, rx) in the sqrt(G0) plane
+        const float *mp = &E.m[ry * MW + rx];  // tap (0, 0) of the 7x7 window: frame (x - 3, y - 3)
         const int n = T->g0_n[k];
-        for (int i = tid; i < nr * nc; i += 256) {
-            const int ry = fr + 2 * (i / nc), rx = fc + 2 * (i % nc);  // in the sqrt(G0) plane
-            const float *mp = &E.m[ry * MW + rx];  // tap (0, 0) of the 7x7 window: frame (x - 3, y - 3)
-            float acc = 0.0f;
-            for (int t = 0; t < n; t++) {
-                const uint32_t idx = T->g0_idx[k][t];
-                acc += T->g0_c[k][t] * mp[(idx / 7u) * MW + idx % 7u];
-            }
-            E.r2.sg[ry * SW + rx] = sqrt_max0(acc);
+        float acc = 0.0f;
+        for (int t = 0; t < n; t++) {
+            const uint32_t idx = T->g0_idx[k][t];
+            acc += T->g0_c[k][t] * mp[(idx / 7u) * MW + idx % 7u];
+        }
+        E.r2.sg[ry * SW + rx] = sqrt_max0(acc);
+    };
+    if constexpr (P == 2) {
+        for (int k = 0; k < 4; k++) {
+            int fr, fc, nr, nc;
+            phase_grid(a, k, X0 - 5, Y0 - 5, SW, SH, fr, fc, nr, nc);
+            for (int i = tid; i < nr * nc; i += 256) g0_px(k, fr + 2 * (i / nc), fc + 2 * (i % nc));
+        }
+    } else {
+        using G6 = Phase6<SW, SH>;
+        for (int i = tid; i < G6::ITEMS; i += 256) {
+            int k, ry, rx;
+            if (G6::at(a, i, X0 - 5, Y0 - 5, k, ry, rx)) g0_px(k, ry, rx);
         }
     }
     __syncthreads();
@@ -145,41 +225,57 @@ __global__ __launch_bounds__(256) void k_develop_edge(DevArgs a, Border B, int t
     __syncthreads();
     // 5. the colour planes over +-3: the own colour is the sample, the missing ones the weighted
     //    directional filters (mod.rs:762-795), phase by phase
-    for (int k = 0; k < 4; k++) {
-        const uint32_t own = canon_of(a, k);
-        int fr, fc, nr, nc;
-        phase_grid(a, k, X0 - 3, Y0 - 3, PW, PH, fr, fc, nr, nc);
-        for (int i = tid; i < nr * nc; i += 256) {
-            const int ry = fr + 2 * (i / nc), rx = fc + 2 * (i % nc);  // in the colour planes
-            const float *mp = &E.m[(ry + 5 - 3) * MW + (rx + 5 - 3)];
-            const int o = ry * PW + rx;
-            const float w0 = E.w[0][o], w1 = E.w[1][o], w2 = E.w[2][o], w3 = E.w[3][o];
+    auto colour_px = [&](int k, int ry, int rx) {  // (ry, rx) in the colour planes
+        const uint32_t own = P == 2 ? canon_of(a, k) : T->canon[k];
+        const float *mp = &E.m[(ry + 5 - 3) * MW + (rx + 5 - 3)];
+        const int o = ry * PW + rx;
+        const float w0 = E.w[0][o], w1 = E.w[1][o], w2 = E.w[2][o], w3 = E.w[3][o];
 #pragma unroll
-            for (int c = 0; c < 3; c++) {
-                if ((uint32_t)c == own) {
-                    E.p[c][o] = mp[3 * MW + 3];
-                    continue;
-                }
-                float acc0 = 0.0f, acc1 = 0.0f, acc2 = 0.0f, acc3 = 0.0f;
-                const int n = T->dir_n[k][c];
-                for (int t = 0; t < n; t++) {
-                    const uint32_t idx = T->dir_idx[k][c][t];
-                    const float v = mp[(idx / 7u) * MW + idx % 7u];
-                    acc0 += T->dir_c[k][c][t][0] * v;
-                    acc1 += T->dir_c[k][c][t][1] * v;
-                    acc2 += T->dir_c[k][c][t][2] * v;
-                    acc3 += T->dir_c[k][c][t][3] * v;
-                }
-                E.p[c][o] = w0 * acc0 + w1 * acc1 + w2 * acc2 + w3 * acc3;
+        for (int c = 0; c < 3; c++) {
+            if ((uint32_t)c == own) {
+                E.p[c][o] = mp[3 * MW + 3];
+                continue;
             }
+            float acc0 = 0.0f, acc1 = 0.0f, acc2 = 0.0f, acc3 = 0.0f;
+            const int n = T->dir_n[k][c];
+            for (int t = 0; t < n; t++) {
+                const uint32_t idx = T->dir_idx[k][c][t];
+                const float v = mp[(idx / 7u) * MW + idx % 7u];
+                acc0 += T->dir_c[k][c][t][0] * v;
+                acc1 += T->dir_c[k][c][t][1] * v;
+                acc2 += T->dir_c[k][c][t][2] * v;
+                acc3 += T->dir_c[k][c][t][3] * v;
+            }
+            E.p[c][o] = w0 * acc0 + w1 * acc1 + w2 * acc2 + w3 * acc3;
+        }
+    };
+    if constexpr (P == 2) {
+        for (int k = 0; k < 4; k++) {
+            int fr, fc, nr, nc;
+            phase_grid(a, k, X0 - 3, Y0 - 3, PW, PH, fr, fc, nr, nc);
+            for (int i = tid; i < nr * nc; i += 256) colour_px(k, fr + 2 * (i / nc), fc + 2 * (i % nc));
+        }
+    } else {
+        using G6 = Phase6<PW, PH>;
+        for (int i = tid; i < G6::ITEMS; i += 256) {
+            int k, ry, rx;
+            if (G6::at(a, i, X0 - 3, Y0 - 3, k, ry, rx)) colour_px(k, ry, rx);
         }
     }
     __syncthreads();
-    // 6. pass 1 over +-2: gated mean of the measured differences of the 3x3 window
-    //    (mod.rs:476-530); the planes of the gradients are free again
+    // 6. pass 1 over +-2: Bayer, the gated mean of the measured differences of the 3x3 window
+    //    (mod.rs:476-530); 6x6, the median of the differences of the filtered planes over the 3x3
+    //    window (mod.rs:455-474), the measured difference at the colour's own sites
     for (int i = tid; i < QH * QW; i += 256) {
         const int y = i / QW + 1, x = i % QW + 1;  // in the colour planes
         const int fy = Y0 - 3 + y, fx = X0 - 3 + x;
+        if constexpr (P == 6) {
+            const uint32_t own = colour_at<P>(a, T, fx, fy);
+            const int o = y * PW + x;
+            E.r2.d1[0][i] = own == 0 ? E.p[0][o] - E.p[1][o] : median_diff(E.p[0], E.p[1], o, PW);
+            E.r2.d1[1][i] = own == 2 ? E.p[2][o] - E.p[1][o] : median_diff(E.p[2], E.p[1], o, PW);
+            continue;
+        }
         const float g = E.p[1][y * PW + x];
         const float inv = 1.0f / (0.2f * fmaxf(g, 0.0f) + 0.015f);
         float sw0 = 0.0f, sd0 = 0.0f, su0 = 0.0f, n0 = 0.0f, sw1 = 0.0f, sd1 = 0.0f, su1 = 0.0f, n1 = 0.0f;
@@ -191,7 +287,7 @@ __global__ __launch_bounds__(256) void k_develop_edge(DevArgs a, Border B, int t
                 const float gk = E.p[1][q];
                 const float e = (gk - g) * inv;
                 const float wt = 1.0f / (1.0f + e * e);
-                const uint32_t col = canon_of(a, edge_phase(a, fx + dx, fy + dy));
+                const uint32_t col = colour_at<P>(a, T, fx + dx, fy + dy);
                 const float m0 = col == 0 ? 1.0f : 0.0f, m1 = col == 2 ? 1.0f : 0.0f;
                 const float d0 = E.p[0][q] - gk, d2 = E.p[2][q] - gk;
                 sw0 += m0 * wt;
@@ -203,7 +299,7 @@ __global__ __launch_bounds__(256) void k_develop_edge(DevArgs a, Border B, int t
                 su1 += m1 * d2;
                 n1 += m1;
             }
-        const uint32_t own = canon_of(a, edge_phase(a, fx, fy));
+        const uint32_t own = colour_at<P>(a, T, fx, fy);
         const float mixed0 = sw0 > 1e-12f ? sd0 / sw0 : su0 / n0;
         const float mixed1 = sw1 > 1e-12f ? sd1 / sw1 : su1 / n1;
         E.r2.d1[0][i] = own == 0 ? E.p[0][y * PW + x] - g : mixed0;
@@ -230,10 +326,16 @@ __global__ __launch_bounds__(256) void k_develop_edge(DevArgs a, Border B, int t
                 sdb += wt * E.r2.d1[1][q];
             }
         const float dr = sdr / sw, db = sdb / sw;
-        const uint32_t own = canon_of(a, edge_phase(a, X0 - 1 + x - 1, Y0 - 1 + y - 1));
-        E.r3.v[0][i] = own == 0 ? E.p[0][py * PW + px] : g + dr;
-        E.r3.v[1][i] = g;
-        E.r3.v[2][i] = own == 2 ? E.p[2][py * PW + px] : g + db;
+        const uint32_t own = colour_at<P>(a, T, X0 - 1 + x - 1, Y0 - 1 + y - 1);
+        const float p0 = E.p[0][py * PW + px], p2 = E.p[2][py * PW + px];
+        // 6x6: green at a red or blue site moves towards the measured sample minus the cleaned
+        // difference by xg = clamp(1 - 2 beta, 0, 1) (mod.rs:547-592)
+        float gv = g;
+        if (P == 6 && a.exg != 0.0f && own == 0) gv = g + a.exg * ((p0 - dr) - g);
+        if (P == 6 && a.exg != 0.0f && own == 2) gv = g + a.exg * ((p2 - db) - g);
+        E.r3.v[0][i] = own == 0 ? p0 : gv + dr;
+        E.r3.v[1][i] = gv;
+        E.r3.v[2][i] = own == 2 ? p2 : gv + db;
     }
     __syncthreads();
     // 8. gated smoothing (beta > 0, mod.rs:803-822), finish with unit balance, encode, store
@@ -552,8 +654,10 @@ hipError_t launch_develop_edge(const DevArgs &a, hipStream_t stream) {
     if (grid == 0) return hipSuccess;
     if (a.edge_tame)
         hipLaunchKernelGGL((k_develop_edge2<E2W, E2H, E2T>), dim3(grid), dim3(nt), 0, stream, a, B, tiles_x, tiles);
+    else if (a.eperiod == 6)
+        hipLaunchKernelGGL(k_develop_edge<6>, dim3(grid), dim3(nt), 0, stream, a, B, tiles_x, tiles);
     else
-        hipLaunchKernelGGL(k_develop_edge<false>, dim3(grid), dim3(nt), 0, stream, a, B, tiles_x, tiles);
+        hipLaunchKernelGGL(k_develop_edge<2>, dim3(grid), dim3(nt), 0, stream, a, B, tiles_x, tiles);
     return hipGetLastError();
 }
 

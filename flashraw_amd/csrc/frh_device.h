@@ -28,14 +28,15 @@ enum OutMode : int32_t {
 // bits when every value it multiplies is finite (a zero tap then adds +-0 to a sum that is never
 // -0).
 struct EdgeTab {
-    float g0d[4][49];
+    float g0d[4][49];           // dense Bayer taps (the tame kernel)
     float dird[4][3][49][4];
-    uint32_t g0_n[4];
-    uint32_t dir_n[4][3];
-    uint32_t g0_idx[4][49];
-    float g0_c[4][49];
-    uint32_t dir_idx[4][3][49];
-    float dir_c[4][3][49][4];
+    uint32_t canon[36];         // colour of each canonical position
+    uint32_t g0_n[36];          // per canonical position: G0's non-zero taps
+    uint32_t dir_n[36][3];      // per position and colour: taps non-zero in some direction
+    uint32_t g0_idx[36][49];
+    float g0_c[36][49];
+    uint32_t dir_idx[36][3][49];
+    float dir_c[36][3][49][4];
 };
 
 // Everything one develop launch reads, passed by value as the kernel argument.
@@ -93,6 +94,8 @@ struct DevArgs {
     uint32_t ecanon_bits;  // colour of canonical position i in bits 2i..2i+1
     uint32_t ecx, ecy;
     float ebeta;
+    uint32_t eperiod;  // 2 (Bayer) or 6 (X-Trans)
+    float exg;         // X-Trans green re-estimate weight clamp(1 - 2 beta, 0, 1)
     // The inputs bound every intermediate value of the edge-aware demosaic far from overflow
     // (see edge_tables in api.cpp): dense taps and uniform colour masks give the same bits.
     uint32_t edge_tame;

@@ -14,23 +14,6 @@
 namespace frh {
 namespace {
 
-// Calib::norm (process.rs:288-298) of component c of window sample (x, y).
-__device__ __forceinline__ float gen_norm(const DevArgs &a, uint32_t x, uint32_t y, uint32_t c) {
-    const size_t at = (size_t)y * a.stride + (size_t)x * a.cpp + c;
-    const float v = a.srcf ? a.srcf[at] : (float)a.src[at];
-    const uint32_t X = x + a.ox, Y = y + a.oy;
-    const uint32_t P = a.cal_pw * a.cal_ph;
-    const uint32_t i = c * P + (Y % a.cal_ph) * a.cal_pw + X % a.cal_pw;
-    float b = a.gcal[i];
-    if (a.n_rowd) b += Y < a.n_rowd ? a.rowd[Y] : 0.0f;
-    if (a.n_cold) b += X < a.n_cold ? a.cold[X] : 0.0f;
-    return (v - b) * a.gcal[a.cpp * P + i];
-}
-
-__device__ __forceinline__ uint32_t site_at(const DevArgs &a, uint32_t x, uint32_t y) {
-    return a.sites[((y + a.oy) % a.cfa_h) * a.cfa_w + (x + a.ox) % a.cfa_w];
-}
-
 __device__ __forceinline__ void add_to(uint32_t c, float v, float &a0, float &a1, float &a2, float &n0, float &n1,
                                        float &n2) {
     if (c == 0) {
@@ -87,31 +70,7 @@ __global__ __launch_bounds__(256) void k_develop_generic(DevArgs a) {
     } else {
         const uint32_t x = a.crop_x + ox, y = a.crop_y + oy;
         if (a.gmode == 1) {
-            const uint32_t site = site_at(a, x, y);
-            const bool inside = x > 0 && x + 1 < a.width && y > 0 && y + 1 < a.height;
-            float v[3];
-#pragma unroll
-            for (int c = 0; c < 3; c++) {
-                const uint32_t m = (site >> (2 + 9 * c)) & 0x1ffu;
-                float sum = 0.0f, n = 0.0f;
-#pragma unroll
-                for (int t = 0; t < 9; t++) {
-                    if (!((m >> t) & 1u)) continue;
-                    const int xx = (int)x + t % 3 - 1, yy = (int)y + t / 3 - 1;
-                    if (!inside && (xx < 0 || xx >= (int)a.width || yy < 0 || yy >= (int)a.height)) continue;
-                    sum += gen_norm(a, (uint32_t)xx, (uint32_t)yy, 0);
-                    n += 1.0f;
-                }
-                if (inside) {
-                    const uint32_t cnt = __builtin_popcount(m);
-                    v[c] = sum * (cnt ? 1.0f / (float)cnt : 0.0f);
-                } else {
-                    v[c] = n > 0.0f ? sum / n : 0.0f;
-                }
-            }
-            r = v[0];
-            g = v[1];
-            b = v[2];
+            cfa_mean_rgb(a, x, y, r, g, b);
         } else if (a.gmode == 2) {
             r = gen_norm(a, x, y, 0);
             g = gen_norm(a, x, y, 1);

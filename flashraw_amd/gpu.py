@@ -385,8 +385,9 @@ class Orientation(enum.IntEnum):
 
 @dataclass
 class EdgeSpec:
-    """EdgeSpec (run.rs:141-160). The AHD form uses the pattern and the halo only; the edge-aware
-    form (Bayer) also `g0` ([4][49]) and `dir` ([4][3][4][49]) and `beta`, with a halo of 8."""
+    """EdgeSpec (run.rs:141-160). The AHD form (Bayer) uses the pattern and the halo only; the
+    edge-aware form, Bayer (period 2) or X-Trans (period 6), also `canon` ([P*P]), `g0`
+    ([P*P][49]), `dir` ([P*P][3][4][49]) and `beta`, with a halo of 8."""
     period: int = 2
     cx: int = 0
     cy: int = 0
@@ -411,45 +412,58 @@ class NoiseReduction(enum.IntEnum):
 _LEVEL_FLOOR = {NoiseReduction.Auto: None, NoiseReduction.Off: 0.0, NoiseReduction.Low: 0.25,
                 NoiseReduction.Medium: 0.5, NoiseReduction.High: 0.8}  # options.rs:70-78
 
-EDGE_CANON = (0, 1, 1, 2)  # canonical Bayer colours (R, G, G, B), edgeaware/mod.rs:165
+EDGE_CANON = (0, 1, 1, 2)  # canonical Bayer colours (R, G, G, B), edgeaware/mod.rs:163-166
+# canonical X-Trans colours GGRGGB / GGBGGR / BRGRBG / GGBGGR / GGRGGB / RBGBRG (mod.rs:168-185)
+EDGE_CANON_XTRANS = tuple({"R": 0, "G": 1, "B": 2}[c] for c in "GGRGGBGGBGGRBRGRBGGGBGGRGGRGGBRBGBRG")
 EDGE_HALO = 8              # edgeaware::HALO
 
 
-def edge_plan(info: FrameInfo, origin=(0, 0)) -> Optional[tuple]:
-    """Plan::new (edgeaware/mod.rs:204-230) for a Bayer frame: the canonical column and row
-    (cx, cy) of frame sample (0, 0), or None when the pattern is not Bayer."""
-    if info.cfa is None or not info.cfa.is_bayer() or info.cfa.width != 2 or info.cfa.height != 2:
+def edge_plan(info: FrameInfo, origin=(0, 0), with_pattern=False) -> Optional[tuple]:
+    """Plan::new (edgeaware/mod.rs:207-232): the canonical column and row (cx, cy) of frame sample
+    (0, 0) when the pattern is a shifted copy of the canonical Bayer (2x2) or X-Trans (6x6) one,
+    else None. With `with_pattern`, (cx, cy, canonical colours, period)."""
+    if info.cfa is None:
         return None
     cols = tuple(info.cfa.colors)
-    for sy in range(2):
-        for sx in range(2):
-            if all(cols[y * 2 + x] == EDGE_CANON[((y + sy) % 2) * 2 + (x + sx) % 2] for y in range(2) for x in range(2)):
-                return (sx + origin[0]) % 2, (sy + origin[1]) % 2
+    for canon, p in ((EDGE_CANON, 2), (EDGE_CANON_XTRANS, 6)):
+        if info.cfa.width != p or info.cfa.height != p:
+            continue
+        for sy in range(p):
+            for sx in range(p):
+                if all(cols[y * p + x] == canon[((y + sy) % p) * p + (x + sx) % p]
+                       for y in range(p) for x in range(p)):
+                    cx, cy = (sx + origin[0]) % p, (sy + origin[1]) % p
+                    return (cx, cy, canon, p) if with_pattern else (cx, cy)
     return None
 
 
-def estimate_beta(width: int, height: int, g_gain: float, fetch, cx: int, cy: int) -> float:
-    """Plan::set_noise_from (edgeaware/mod.rs:369-409) for a Bayer plan: the smoothing weight
-    from the noise of 48 rows spread over the frame. `fetch(y)` returns row y of the normalised
-    mosaic (f32, `width` samples). Green pairs two samples apart with a mean in (0.03, 0.6) give
-    |a - b| / sqrt(mean); the lower quintile q gives sigma = q / (sqrt(2) * 0.2533) and
-    beta = 0.5 * clamp((sigma^2 - 2e-4) / 6e-4, 0, 1), all in f32."""
+def estimate_beta(width: int, height: int, g_gain: float, fetch, cx: int, cy: int, canon=EDGE_CANON) -> float:
+    """Plan::set_noise_from (edgeaware/mod.rs:369-409): the smoothing weight from the noise of
+    48 rows spread over the frame. `fetch(y)` returns row y of the normalised mosaic (f32,
+    `width` samples); `canon` is the canonical pattern (4 or 36 colours). Green pairs one or two
+    samples apart (x < width - 2) with a mean in (0.03, 0.6) give |a - b| / sqrt(mean); the lower
+    quintile q gives sigma = q / (sqrt(2) * 0.2533) and beta = 0.5 * clamp((sigma^2 - 2e-4) /
+    6e-4, 0, 1), all in f32."""
     f32 = np.float32
     if height < 16 or width < 16:
         return 0.0
+    canon = np.asarray(canon, dtype=np.int64)
+    P = 6 if canon.size == 36 else 2
     rows = 48
     vals = []
+    xs = np.arange(width)
     for n in range(rows):
         y = 4 + n * (height - 9) // (rows - 1)
         row = np.asarray(fetch(y), dtype=f32).reshape(-1)[:width]
-        # green samples of row y: canon[((y + cy) % 2) * 2 + (x + cx) % 2] == 1, i.e. the column
-        # parity differs from the row parity; their green partners sit two samples to the right
-        xs = np.arange((y + cy + cx + 1) % 2, width - 2, 2)
-        a = row[xs] * f32(g_gain)
-        b = row[xs + 2] * f32(g_gain)
-        mid = f32(0.5) * (a + b)
-        ok = (mid > f32(0.03)) & (mid < f32(0.6))
-        vals.append(np.abs(a[ok] - b[ok]) / np.sqrt(mid[ok]))
+        green = canon[((y + cy) % P) * P + (xs + cx) % P] == 1
+        for d in (1, 2):
+            x = xs[:width - 2]
+            sel = x[green[x] & green[x + d]]
+            a = row[sel] * f32(g_gain)
+            b = row[sel + d] * f32(g_gain)
+            mid = f32(0.5) * (a + b)
+            ok = (mid > f32(0.03)) & (mid < f32(0.6))
+            vals.append(np.abs(a[ok] - b[ok]) / np.sqrt(mid[ok]))
     v = np.concatenate(vals) if vals else np.zeros(0, f32)
     if v.size < 1000:
         return 0.0
@@ -463,21 +477,22 @@ def estimate_beta(width: int, height: int, g_gain: float, fetch, cx: int, cy: in
 
 def edge_spec(info: FrameInfo, origin, size, wb, fetch, tables, level=NoiseReduction.Auto) -> Optional[EdgeSpec]:
     """flashraw::gpu::edge_spec (crates/flashraw/src/gpu.rs:115-128): the edge-aware spec of a
-    Bayer frame of `size` = (width, height) samples whose sample (0, 0) sits at sensor `origin`,
-    with the smoothing weight estimated from the frame (`fetch(y)` = normalised row y, green gain
-    wb[1]) and then raised to the floor of `level` (Plan::apply_level, mod.rs:413-419).
-    `tables` = (g0 [4][49], dir [4][3][4][49]) are the filter tables of the canonical pattern
-    (R, G, G, B). None for patterns other than Bayer."""
-    plan = edge_plan(info, origin)
+    Bayer or X-Trans frame of `size` = (width, height) samples whose sample (0, 0) sits at sensor
+    `origin`, with the smoothing weight estimated from the frame (`fetch(y)` = normalised row y,
+    green gain wb[1]) and then raised to the floor of `level` (Plan::apply_level,
+    mod.rs:413-419). `tables` = (g0 [P*P][49], dir [P*P][3][4][49]) are the filter tables of the
+    canonical pattern (R, G, G, B for Bayer; the X-Trans arrangement of EDGE_CANON_XTRANS).
+    None for other patterns."""
+    plan = edge_plan(info, origin, with_pattern=True)
     if plan is None:
         return None
-    cx, cy = plan
-    beta = estimate_beta(int(size[0]), int(size[1]), float(wb[1]), fetch, cx, cy)
+    cx, cy, canon, period = plan
+    beta = estimate_beta(int(size[0]), int(size[1]), float(wb[1]), fetch, cx, cy, canon)
     floor = _LEVEL_FLOOR[NoiseReduction(level)]
     if floor is not None:
         beta = 0.0 if floor == 0.0 else max(beta, floor)
     g0, dirt = tables
-    return EdgeSpec(period=2, cx=cx, cy=cy, beta=float(beta), halo=EDGE_HALO, canon=EDGE_CANON,
+    return EdgeSpec(period=period, cx=cx, cy=cy, beta=float(beta), halo=EDGE_HALO, canon=canon,
                     g0=np.asarray(g0, np.float32), dir=np.asarray(dirt, np.float32), ahd=False)
 
 

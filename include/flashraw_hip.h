@@ -97,12 +97,14 @@ enum frh_develop_output {
     FRH_OUTPUT_NONE = -1
 };
 
-/* EdgeSpec (run.rs:141-160), Bayer (period 2) only. The AHD form (`ahd` = 1, beta 0) uses the
- * pattern and the halo; its filter tables may be NULL. The edge-aware form (`ahd` = 0) needs
- * canon[4], g0[4*49] (7x7 filter of the first green estimate per canonical position) and
- * dir[4*3*4*49] (7x7 filters per position, colour and direction), beta in [0, 1] and a halo of
- * at least 8; the device copy of its tap lists is built once per distinct table contents and
- * kept by the context. Other periods (X-Trans) return FRH_ERROR_NOT_AVAILABLE. */
+/* EdgeSpec (run.rs:141-160). The AHD form (`ahd` = 1, beta 0, period 2) uses the pattern and the
+ * halo; its filter tables may be NULL. The edge-aware form (`ahd` = 0) takes Bayer (period 2) and
+ * 6x6 (period 6, X-Trans) mosaics: with P = period it needs canon[P*P] (colour of each canonical
+ * position), g0[P*P*49] (7x7 filter of the first green estimate per canonical position) and
+ * dir[P*P*3*4*49] (7x7 filters per position, colour and direction), beta in [0, 1] and a halo of
+ * at least 8; frame sample (x, y) sits at canonical position ((y + cy) % P) * P + (x + cx) % P.
+ * The device copy of its tap lists is built once per distinct table contents and kept by the
+ * context. Other periods return FRH_ERROR_NOT_AVAILABLE. */
 typedef struct frh_edge_spec {
     uint32_t size;
     uint32_t period, cx, cy;

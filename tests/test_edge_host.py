@@ -157,3 +157,124 @@ def test_edge_spec_levels():
     assert (spec.cx, spec.cy) == N.edge_plan("RGGB", (1, 0))
     assert F.edge_spec(F.FrameInfo(w, h, F.CfaPattern(2, 2, (0, 1, 1, 1))), (0, 0), (w, h), (1, 1, 1, 1),
                        lambda y: norm[y], tables) is None
+
+
+# ---------------------------------------------------------------- X-Trans (period 6)
+
+XT = N.XTRANS
+
+
+def _xt_info(w, h):
+    return F.FrameInfo(w, h, F.CfaPattern.parse(XT), F.BlackLevel.uniform(256.0), F.WhiteLevel.uniform(4095))
+
+
+def test_xtrans_canon_is_the_reference_arrangement():
+    """mod.rs:168-185: GGRGGB / GGBGGR / BRGRBG / GGBGGR / GGRGGB / RBGBRG with R, G, B = 0, 1, 2."""
+    rows = ["GGRGGB", "GGBGGR", "BRGRBG", "GGBGGR", "GGRGGB", "RBGBRG"]
+    assert F.EDGE_CANON_XTRANS == tuple("RGB".index(c) for r in rows for c in r)
+    assert list(F.EDGE_CANON_XTRANS) == N.edge_plan_p(XT, (0, 0))[2]
+
+
+@pytest.mark.parametrize("origin", [(0, 0), (1, 0), (0, 1), (2, 3), (5, 5), (7, 13)])
+def test_xtrans_edge_plan(origin):
+    cx, cy, canon, period = F.edge_plan(_xt_info(12, 12), origin, with_pattern=True)
+    assert (cx, cy, list(canon), period) == tuple(N.edge_plan_p(XT, origin))
+    # the plan puts every frame sample at a canonical position of its own colour
+    col = N.colours(XT, 12, 12, origin)
+    for y in range(12):
+        for x in range(12):
+            assert canon[((y + cy) % 6) * 6 + (x + cx) % 6] == col[y, x]
+
+
+def test_xtrans_edge_plan_shifted_patterns():
+    """Plan::new accepts every shifted copy of the canonical arrangement (mod.rs:211-216) and
+    nothing else of period 6."""
+    p = N.parse_cfa(XT)
+    for sy in range(6):
+        for sx in range(6):
+            shifted = np.roll(np.roll(p, -sy, 0), -sx, 1)
+            info = F.FrameInfo(12, 12, F.CfaPattern(6, 6, tuple(int(v) for v in shifted.reshape(-1))))
+            cx, cy = F.edge_plan(info, (0, 0))
+            assert all(F.EDGE_CANON_XTRANS[((y + cy) % 6) * 6 + (x + cx) % 6] == shifted[y, x]
+                       for y in range(6) for x in range(6))
+    other = p.copy()
+    other[2, 1], other[2, 2] = other[2, 2], other[2, 1]  # R and G swapped at one site
+    assert F.edge_plan(F.FrameInfo(12, 12, F.CfaPattern(6, 6, tuple(int(v) for v in other.reshape(-1)))), (0, 0)) is None
+
+
+def _pairs_brute(norm, cx, cy, canon, P, g_gain):
+    """Plan::set_noise_from's values written out sample by sample (tiny frames only)."""
+    h, w = norm.shape
+    f32 = np.float32
+    vals = []
+    for n in range(48):
+        y = 4 + n * (h - 9) // 47
+        for x in range(w - 2):
+            if canon[((y + cy) % P) * P + (x + cx) % P] != 1:
+                continue
+            for d in (1, 2):
+                if canon[((y + cy) % P) * P + (x + d + cx) % P] != 1:
+                    continue
+                a, b = f32(norm[y, x]) * f32(g_gain), f32(norm[y, x + d]) * f32(g_gain)
+                mid = f32(0.5) * (a + b)
+                if f32(0.03) < mid < f32(0.6):
+                    vals.append(f32(abs(a - b)) / f32(np.sqrt(mid)))
+    return vals
+
+
+@pytest.mark.parametrize("origin", [(0, 0), (3, 1)])
+@pytest.mark.parametrize("read_sigma", [2.0, 20.0])
+def test_xtrans_beta_brute_force(origin, read_sigma):
+    """The oracle and the package against the sample-by-sample statement: the same values and the
+    value of rank len / 5."""
+    w, h = 160, 90
+    rng = np.random.default_rng(int(read_sigma))
+    norm = (0.3 + read_sigma * 1e-3 * rng.standard_normal((h, w))).astype(F32)
+    cx, cy, canon, P = N.edge_plan_p(XT, origin)
+    vals = _pairs_brute(norm, cx, cy, canon, P, 1.3)
+    assert len(vals) >= 1000
+    # both kinds of pair are present: one and two samples apart
+    q = np.sort(np.asarray(vals, F32))[len(vals) // 5]
+    sigma = F32(q) / (F32(np.sqrt(2.0)) * F32(0.2533))
+    want = F32(0.5) * F32(min(max((sigma * sigma - F32(2e-4)) / F32(6e-4), F32(0)), F32(1)))
+    got_o = O.edge_beta(norm, cx, cy, 1.3, canon=canon)
+    got_p = F.estimate_beta(w, h, 1.3, lambda y: norm[y], cx, cy, canon)
+    assert F32(got_o) == want and F32(got_p) == want, (got_o, got_p, want)
+
+
+def test_xtrans_beta_closed_form_and_pair_kinds():
+    """Greens alternating lo / hi along each row's chain of green samples: every pair (one or two
+    apart) differs by d with mean g, so q = d / sqrt(g) and beta has its closed form; keeping only
+    pairs two apart (the Bayer rule) halves the count, which drops below the 1000 values the
+    estimate needs on this frame size."""
+    canon = np.asarray(F.EDGE_CANON_XTRANS).reshape(6, 6)
+    mid, d = 0.25, 0.01
+    w, h = 56, 64
+    f = np.zeros((h, w), F32)
+    for y in range(h):
+        greens = [x for x in range(w) if canon[y % 6, x % 6] == 1]
+        for i, x in enumerate(greens):
+            f[y, x] = mid - d / 2 if i % 2 == 0 else mid + d / 2
+    vals = _pairs_brute(f, 0, 0, list(F.EDGE_CANON_XTRANS), 6, 1.0)
+    n1 = sum(1 for y in range(h) for x in range(w - 2) if canon[y % 6, x % 6] == 1 and canon[y % 6, (x + 1) % 6] == 1)
+    assert 1000 <= len(vals) < 2000 and n1 > 0
+    got = O.edge_beta(f, 0, 0, 1.0, canon=F.EDGE_CANON_XTRANS)
+    assert F32(got) == F32(F.estimate_beta(w, h, 1.0, lambda y: f[y], 0, 0, F.EDGE_CANON_XTRANS))
+    assert abs(got - _beta_closed_form(d / np.sqrt(mid))) < 2e-3, got
+    # every value is equal, so the rank cannot tell the pair kinds apart; the count can
+    two_apart = sum(1 for n in range(48) for x in range(w - 2)
+                    if canon[(4 + n * (h - 9) // 47) % 6, x % 6] == 1
+                    and canon[(4 + n * (h - 9) // 47) % 6, (x + 2) % 6] == 1)
+    assert two_apart < 1000 <= len(vals)
+
+
+def test_xtrans_edge_spec():
+    w, h = 300, 200
+    rng = np.random.default_rng(3)
+    norm = (0.3 + 0.02 * rng.standard_normal((h, w))).astype(F32)
+    g0, d = N.edge_tables_xtrans(0)
+    spec = F.edge_spec(_xt_info(w, h), (2, 1), (w, h), (2.0, 1.2, 1.5, 1.0), lambda y: norm[y], (g0, d))
+    cx, cy, canon, P = N.edge_plan_p(XT, (2, 1))
+    assert (spec.period, spec.cx, spec.cy, list(spec.canon), spec.halo, spec.ahd) == (6, cx, cy, canon, 8, False)
+    assert F32(spec.beta) == F32(O.edge_beta(norm, cx, cy, 1.2, canon=canon)) and spec.beta > 0
+    assert spec.g0.shape == (36, 49) and spec.dir.shape == (36, 3, 4, 49)
