@@ -143,6 +143,30 @@ int fro_dct_denoise(const uint16_t *mosaic, uint32_t width, uint32_t height, con
                     float sig_a, float sig_b, float thr, uint32_t step, int precision, float *out_f32,
                     uint16_t *out_u16, int threads);
 
+
+/* ---- luma / chroma noise reduction of developed linear RGB (crates/flashraw/src/denoise/mod.rs,
+ * nr.c) ---- */
+
+/* denoise::apply on interleaved RGB (3 floats per pixel), in place. */
+int fro_nr_apply(float *rgb, uint32_t w, uint32_t h, float luma, float chroma);
+/* Pyramid levels (level_count) and the noise estimate's block step (estimate_step, 0: none). */
+uint32_t fro_nr_level_count(uint32_t w, uint32_t h, uint32_t max);
+uint32_t fro_nr_estimate_step(uint32_t w, uint32_t h);
+/* The measured blocks of a plane (per block: guide mean, then 63 squared coefficients; entry
+ * l * 8 + k is horizontal frequency l, vertical k). Returns the count; out NULL only counts. */
+uint32_t fro_nr_stats(const float *p, const float *guide, uint32_t w, uint32_t h, float *out);
+/* Model::from_stats: bins (at most 12) means and 64 deviations per bin; -1 if too few blocks. */
+int fro_nr_model_from_stats(const float *stats, uint32_t count, uint32_t *bins, float *means, float *sigmas);
+/* Pieces: the 2-D DCT of an 8x8 block and its inverse, box_mean, down, add_up, one DCT pass. */
+void fro_nr_dct_block(const float *p, uint32_t w, uint32_t x0, uint32_t y0, float *out);
+void fro_nr_inverse_block(const float *coef, float *out);
+int fro_nr_box_mean(const float *in, uint32_t w, uint32_t h, uint32_t r, float *out);
+int fro_nr_down(const float *in, uint32_t w, uint32_t h, float *out);
+int fro_nr_add_up(float *d, uint32_t w, uint32_t h, const float *coarse);
+int fro_nr_dct_pass(const float *src, const float *guide, const float *pilot, uint32_t w, uint32_t h,
+                    uint32_t bins, const float *means, const float *sigmas, float gain, float hard, uint32_t step,
+                    float *out);
+
 #ifdef __cplusplus
 }
 #endif

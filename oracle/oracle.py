@@ -132,6 +132,24 @@ def lib():
         L.fro_dct_denoise.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p,
                                       ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_uint32,
                                       ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
+        P, U, F = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_float
+        L.fro_nr_apply.restype = ctypes.c_int
+        L.fro_nr_apply.argtypes = [P, U, U, F, F]
+        L.fro_nr_level_count.restype = U
+        L.fro_nr_level_count.argtypes = [U, U, U]
+        L.fro_nr_estimate_step.restype = U
+        L.fro_nr_estimate_step.argtypes = [U, U]
+        L.fro_nr_stats.restype = U
+        L.fro_nr_stats.argtypes = [P, P, U, U, P]
+        L.fro_nr_model_from_stats.restype = ctypes.c_int
+        L.fro_nr_model_from_stats.argtypes = [P, U, P, P, P]
+        L.fro_nr_dct_block.argtypes = [P, U, U, U, P]
+        L.fro_nr_inverse_block.argtypes = [P, P]
+        L.fro_nr_box_mean.argtypes = [P, U, U, U, P]
+        L.fro_nr_down.argtypes = [P, U, U, P]
+        L.fro_nr_add_up.argtypes = [P, U, U, P]
+        L.fro_nr_dct_pass.restype = ctypes.c_int
+        L.fro_nr_dct_pass.argtypes = [P, P, P, U, U, U, P, P, F, F, U, P]
         _LIB = L
     return _LIB
 
@@ -326,3 +344,101 @@ def edge_beta(norm, cx, cy, g_gain, canon=(0, 1, 1, 2)):
     h, w = norm.shape
     period = 6 if c.size == 36 else 2
     return lib().fro_edge_beta(norm.ctypes.data, w, h, c.ctypes.data, period, cx, cy, g_gain)
+
+
+# ---- luma / chroma noise reduction (crates/flashraw/src/denoise/mod.rs, nr.c) ----
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def nr_apply(rgb, luma, chroma):
+    """denoise::apply on a copy of linear RGB (h, w, 3) float32."""
+    out = _f32(rgb).copy()
+    h, w = out.shape[:2]
+    if lib().fro_nr_apply(out.ctypes.data, w, h, luma, chroma) != 0:
+        raise ValueError("oracle rejected the noise reduction")
+    return out
+
+
+def nr_level_count(w, h, mx):
+    return lib().fro_nr_level_count(w, h, mx)
+
+
+def nr_estimate_step(w, h):
+    return lib().fro_nr_estimate_step(w, h)
+
+
+def nr_stats(p, guide):
+    """The measured blocks of plane p binned by guide: (n, 64) float32 (entry 0 the guide mean)."""
+    p, guide = _f32(p), _f32(guide)
+    h, w = p.shape
+    n = lib().fro_nr_stats(p.ctypes.data, guide.ctypes.data, w, h, None)
+    out = np.zeros((max(n, 1), 64), np.float32)
+    if n:
+        lib().fro_nr_stats(p.ctypes.data, guide.ctypes.data, w, h, out.ctypes.data)
+    return out[:n]
+
+
+def nr_model_from_stats(stats):
+    """Model::from_stats: (means (bins,), sigmas (bins, 64)) or None."""
+    st = _f32(stats).reshape(-1, 64)
+    bins = ctypes.c_uint32(0)
+    means = np.zeros(12, np.float32)
+    sig = np.zeros((12, 64), np.float32)
+    if lib().fro_nr_model_from_stats(st.ctypes.data, st.shape[0], ctypes.byref(bins), means.ctypes.data,
+                                     sig.ctypes.data) != 0:
+        return None
+    return means[:bins.value].copy(), sig[:bins.value].copy()
+
+
+def nr_dct_block(block):
+    """Dct::block of an 8x8 array: (8, 8) with [l, k] = horizontal l, vertical k."""
+    b = _f32(block)
+    out = np.zeros(64, np.float32)
+    lib().fro_nr_dct_block(b.ctypes.data, 8, 0, 0, out.ctypes.data)
+    return out.reshape(8, 8)
+
+
+def nr_inverse_block(coef):
+    c = _f32(coef)
+    out = np.zeros(64, np.float32)
+    lib().fro_nr_inverse_block(c.ctypes.data, out.ctypes.data)
+    return out.reshape(8, 8)
+
+
+def nr_box_mean(p, r):
+    p = _f32(p)
+    h, w = p.shape
+    out = np.zeros_like(p)
+    lib().fro_nr_box_mean(p.ctypes.data, w, h, r, out.ctypes.data)
+    return out
+
+
+def nr_down(p):
+    p = _f32(p)
+    h, w = p.shape
+    out = np.zeros(((h + 1) // 2, (w + 1) // 2), np.float32)
+    lib().fro_nr_down(p.ctypes.data, w, h, out.ctypes.data)
+    return out
+
+
+def nr_add_up(d, coarse):
+    out = _f32(d).copy()
+    h, w = out.shape
+    c = _f32(coarse)
+    assert c.shape == ((h + 1) // 2, (w + 1) // 2)
+    lib().fro_nr_add_up(out.ctypes.data, w, h, c.ctypes.data)
+    return out
+
+
+def nr_dct_pass(src, guide, pilot, means, sigmas, gain=1.0, hard=2.7, step=2):
+    src, guide = _f32(src), _f32(guide)
+    h, w = src.shape
+    pil = None if pilot is None else _f32(pilot)
+    m, s = _f32(means), _f32(sigmas).reshape(-1, 64)
+    out = np.zeros_like(src)
+    if lib().fro_nr_dct_pass(src.ctypes.data, guide.ctypes.data, None if pil is None else pil.ctypes.data, w, h,
+                             m.size, m.ctypes.data, s.ctypes.data, gain, hard, step, out.ctypes.data) != 0:
+        raise ValueError("oracle rejected the pass")
+    return out
