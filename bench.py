@@ -33,7 +33,9 @@ def _parse():
     p.add_argument("--demosaic", default="ahd", choices=["ahd", "bilinear", "edge"])
     p.add_argument("--beta", type=float, default=0.0,
                    help="edge demosaic: weight of the gated smoothing (0 for clean frames)")
-    p.add_argument("--mode", default="develop", choices=["develop", "denoise", "batch", "serve", "copy", "unpack"])
+    p.add_argument("--mode", default="develop", choices=["develop", "denoise", "nr", "batch", "serve", "copy", "unpack"])
+    p.add_argument("--luma", type=float, default=0.5, help="nr mode: luma noise reduction strength")
+    p.add_argument("--chroma", type=float, default=0.5, help="nr mode: chroma noise reduction strength")
     p.add_argument("--bits", type=int, default=14, help="unpack mode: bits per packed sample")
     p.add_argument("--frames", type=int, default=64, help="batch mode: frames in the batch")
     p.add_argument("--rates", default="10,50,100,200", help="serve mode: request rates (req/s) to sweep")
@@ -93,7 +95,7 @@ def denoise_mode(args):
     mos = synth.frame_24mp_noisy()
     prec = {"f16": F.DnPrecision.F16, "bf16": F.DnPrecision.BF16, "f16x3": F.DnPrecision.F16X3,
             "f32": F.DnPrecision.F32}[args.precision]
-    spec = F.DenoiseSpec(W, H, black, sig_a, sig_b, thr, 4, prec, seg_rows=args.seg_rows)
+    spec = F.CfaDenoiseSpec(W, H, black, sig_a, sig_b, thr, 4, prec, seg_rows=args.seg_rows)
     d_in = torch.from_numpy(mos.view(np.int16)).to("cuda")
     dst = torch.empty((H, W), dtype=torch.int16, device="cuda")
     dstf = torch.empty((H, W), dtype=torch.float32, device="cuda")
@@ -152,6 +154,77 @@ def denoise_mode(args):
                          "peak_tflops": MFMA_PEAK_TFLOPS[args.precision],
                          "frac": round(flops / (k_avg / 1e3) / 1e12 / MFMA_PEAK_TFLOPS[args.precision], 5),
                          "flops_per_launch": flops}},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+def nr_mode(args):
+    """The luma / chroma noise reduction (GpuContext::denoise) of the 6048x4024 frame with read
+    sigma 12 DN developed bilinearly to linear f32 RGB, encoded to RGBA8, resident in HBM. A step
+    is one whole noise reduction: pyramids, the measured blocks read back, the host's noise models,
+    every DCT pass, the guided filter and the encode."""
+    import numpy as np
+    import torch
+
+    import flashraw_amd as F
+    from flashraw_amd import synth
+
+    ctx = F.context()
+    W, H = 6048, 4024
+    mos = synth.frame_24mp_noisy()
+    info = F.FrameInfo(W, H, F.CfaPattern.RGGB, F.BlackLevel.uniform(512.0), F.WhiteLevel.uniform(16383))
+    dev = F.DevelopSpec(F.Rect.full(W, H), wb=synth.WB, cam_to_rgb=synth.MATRIX, output=F.DevelopOutput.LinearRgbF32)
+    d_in = torch.from_numpy(mos.view(np.int16)).to("cuda")
+    lin = torch.empty(H * W * 3, dtype=torch.float32, device="cuda")
+    ctx.develop_into(F.GpuFrame.from_u16(d_in, W, H), info, dev, lin)
+    torch.cuda.synchronize()
+    spec = F.denoise_spec(W, H, args.luma, args.chroma, F.DevelopOutput.Rgba8)
+    dst = torch.empty(H * W * 4, dtype=torch.uint8, device="cuda")
+    stream = torch.cuda.current_stream()
+    for _ in range(args.warmup):
+        ctx.denoise_into(lin, W, H, spec, dst, stream=stream)
+    torch.cuda.synchronize()
+    times = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        t = time.perf_counter()
+        ctx.denoise_into(lin, W, H, spec, dst, stream=stream)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t)
+    wall = time.perf_counter() - t0
+    ms_per_step = wall * 1e3 / args.steps
+    mp = W * H / 1e6
+    algo_bytes = W * H * (12 + 4)  # linear f32 RGB in, RGBA8 out
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+        sw, sh = 2048, 1536
+        sample = lin.view(H, W, 3)[:sh, :sw].cpu().numpy()
+        ct = []
+        for i in range(3):
+            t = time.perf_counter()
+            O.nr_apply(sample, args.luma, args.chroma)
+            if i:
+                ct.append(time.perf_counter() - t)
+        cpu = {"value": round(sw * sh / 1e6 / statistics.median(ct), 2), "unit": "MP/s", "cores": threads,
+               "kind": "port", "sample": f"a {sw}x{sh} crop of the same linear RGB frame, the oracle's "
+                                         "denoise::apply, median of 2 runs after 1 warm-up"}
+    line = {
+        "metric": "MP/s luma/chroma noise reduction (24 MP linear RGB -> RGBA8)",
+        "value": round(mp / (ms_per_step / 1e3), 1), "unit": "MP/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"6048x4024 linear f32 RGB (bilinear develop of the read-sigma-12 frame) -> RGBA8, "
+                               f"luma {args.luma} chroma {args.chroma}",
+                   "luma_levels": spec.luma.levels if spec.luma else 0,
+                   "chroma_levels": spec.chroma.levels if spec.chroma else 0,
+                   "ms_p50": round(statistics.median(times) * 1e3, 4)},
+        "roofline": {"bound": "valu", "achieved": round(algo_bytes / (ms_per_step / 1e3) / 1e9, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(algo_bytes / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
+                     "algorithmic_bytes_per_launch": algo_bytes},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))
@@ -374,7 +447,7 @@ def serve_mode(args):
         spec = F.DevelopSpec(F.Rect.full(w, h), wb=r["wb"], cam_to_rgb=matrix, output=F.DevelopOutput.Rgba8,
                              edge=F.ahd_spec(info) if r["ahd"] else None)
         frame = F.GpuFrame.from_u16(p["m"], w, h)
-        dn = F.DenoiseSpec(w, h, [p["black"]] * 4, 2.0, 144.0, 2.7, 4, F.DnPrecision.F16X3) if r["denoise"] else None
+        dn = F.CfaDenoiseSpec(w, h, [p["black"]] * 4, 2.0, 144.0, 2.7, 4, F.DnPrecision.F16X3) if r["denoise"] else None
         return frame, info, spec, dn, w * h
 
     def run_server(reqs):
@@ -495,6 +568,8 @@ def main():
         return copy_mode(args)
     if args.mode == "denoise":
         return denoise_mode(args)
+    if args.mode == "nr":
+        return nr_mode(args)
     if args.mode == "batch":
         return batch_mode(args)
     if args.mode == "serve":

@@ -237,7 +237,7 @@ extern "C" void frh_image_free(frh_image *img) {
 static uint64_t gcd64(uint64_t a, uint64_t b) { return b == 0 ? a : gcd64(b, a % b); }
 
 // Bytes per sample and samples per pixel of an output encoding.
-static void out_format(int32_t mode, uint32_t *ch, uint32_t *bps, int32_t *fmt) {
+void frh::out_format(int32_t mode, uint32_t *ch, uint32_t *bps, int32_t *fmt) {
     switch (mode) {
     case OUT_RGBA8: *ch = 4; *bps = 1; *fmt = FRH_FORMAT_RGBA8; break;
     case OUT_LINEAR_F32: *ch = 3; *bps = 4; *fmt = FRH_FORMAT_F32; break;
@@ -245,8 +245,8 @@ static void out_format(int32_t mode, uint32_t *ch, uint32_t *bps, int32_t *fmt) 
     }
 }
 
-static bool valid_mode(int32_t m) { return m >= OUT_RGBA8 && m <= OUT_LINEAR_F16; }
-static bool display_mode(int32_t m) { return m == OUT_RGBA8 || m == OUT_SRGB16; }
+bool frh::valid_mode(int32_t m) { return m >= OUT_RGBA8 && m <= OUT_LINEAR_F16; }
+bool frh::display_mode(int32_t m) { return m == OUT_RGBA8 || m == OUT_SRGB16; }
 
 static std::tuple<int, uint32_t, uint32_t, int> tune_key(bool ahd, const DevArgs &a) {
     return std::make_tuple(ahd ? 1 : 0, a.dev_w, a.dev_h, (a.out_mode == OUT_RGBA8 && a.out_mode2 == OUT_NONE) ? 1 : 0);
@@ -455,6 +455,48 @@ static bool edge_tame(const DevArgs &a, const frh_frame_info *info, const frh_ed
     return 49.0 * m * cmax <= 1e15;
 }
 
+// The display outputs' tone and output curve (run.rs:626-675) into a's table pointers.
+int32_t frh::output_tables(frh_context *ctx, const frh_tone *tone, const frh_curve *curve, DevArgs &a) {
+    int32_t rc = FRH_OK;
+    if (tone) {
+        const frh_tone *t = tone;
+        if (t->size < sizeof(frh_tone)) return fail(FRH_ERROR_INVALID_ARGUMENT, "tone size field too small");
+        if (t->table_len) {
+            if (!t->table) return fail(FRH_ERROR_INVALID_ARGUMENT, "NULL tone table");
+            a.tone = 1;
+            a.tone_tab = static_cast<const float2 *>(device_table(ctx, t->table, t->table_len * 8ull, &rc));
+            if (rc) return rc;
+            a.tone_n = t->table_len;
+            a.tone_xmax = t->xmax;
+            a.tone_scale = t->scale;
+            memcpy(a.tone_m, t->to_prophoto, sizeof(a.tone_m));
+            memcpy(a.tone_w, t->white, sizeof(a.tone_w));
+        }
+    }
+    if (curve) {
+        const frh_curve *c = curve;
+        if (c->size < sizeof(frh_curve)) return fail(FRH_ERROR_INVALID_ARGUMENT, "curve size field too small");
+        if (c->lut8_len > 1) {
+            if (!c->lut8) return fail(FRH_ERROR_INVALID_ARGUMENT, "NULL curve table");
+            a.curve8 = static_cast<const uint8_t *>(device_table(ctx, c->lut8, c->lut8_len, &rc));
+            if (rc) return rc;
+            a.curve8_n = c->lut8_len - 1;
+        }
+        if (c->lut16_len == 16385) {
+            if (!c->lut16) return fail(FRH_ERROR_INVALID_ARGUMENT, "NULL curve table");
+            // (value, step) pairs: the steps are the f32 differences srgb16 computes (process.rs:1953-1958)
+            std::vector<float> pairs(2 * 16384);
+            for (int i = 0; i < 16384; i++) {
+                pairs[2 * i] = c->lut16[i];
+                pairs[2 * i + 1] = c->lut16[i + 1] - c->lut16[i];
+            }
+            a.srgb16 = static_cast<const float2 *>(device_table(ctx, pairs.data(), pairs.size() * 4ull, &rc));
+            if (rc) return rc;
+        }
+    }
+    return FRH_OK;
+}
+
 int32_t frh::plan_develop(const frh_context *ctx, const frh_frame *f, const frh_frame_info *info,
                             const frh_develop_spec *spec, Plan &p) {
     if (!f || !info || !spec) return fail(FRH_ERROR_INVALID_ARGUMENT, "NULL descriptor");
@@ -613,42 +655,7 @@ int32_t frh::plan_develop(const frh_context *ctx, const frh_frame *f, const frh_
         a.bil_rows = rows ? (uint32_t)atoi(rows) : tuned_rows(ctx, false, a, ctx->bil_rows);
     }
     // The display outputs' tone and output curve (run.rs:626-675)
-    if (spec->tone) {
-        const frh_tone *t = spec->tone;
-        if (t->size < sizeof(frh_tone)) return fail(FRH_ERROR_INVALID_ARGUMENT, "tone size field too small");
-        if (t->table_len) {
-            if (!t->table) return fail(FRH_ERROR_INVALID_ARGUMENT, "NULL tone table");
-            a.tone = 1;
-            a.tone_tab = static_cast<const float2 *>(device_table(mctx, t->table, t->table_len * 8ull, &rc));
-            if (rc) return rc;
-            a.tone_n = t->table_len;
-            a.tone_xmax = t->xmax;
-            a.tone_scale = t->scale;
-            memcpy(a.tone_m, t->to_prophoto, sizeof(a.tone_m));
-            memcpy(a.tone_w, t->white, sizeof(a.tone_w));
-        }
-    }
-    if (spec->curve) {
-        const frh_curve *c = spec->curve;
-        if (c->size < sizeof(frh_curve)) return fail(FRH_ERROR_INVALID_ARGUMENT, "curve size field too small");
-        if (c->lut8_len > 1) {
-            if (!c->lut8) return fail(FRH_ERROR_INVALID_ARGUMENT, "NULL curve table");
-            a.curve8 = static_cast<const uint8_t *>(device_table(mctx, c->lut8, c->lut8_len, &rc));
-            if (rc) return rc;
-            a.curve8_n = c->lut8_len - 1;
-        }
-        if (c->lut16_len == 16385) {
-            if (!c->lut16) return fail(FRH_ERROR_INVALID_ARGUMENT, "NULL curve table");
-            // (value, step) pairs: the steps are the f32 differences srgb16 computes (process.rs:1953-1958)
-            std::vector<float> pairs(2 * 16384);
-            for (int i = 0; i < 16384; i++) {
-                pairs[2 * i] = c->lut16[i];
-                pairs[2 * i + 1] = c->lut16[i + 1] - c->lut16[i];
-            }
-            a.srgb16 = static_cast<const float2 *>(device_table(mctx, pairs.data(), pairs.size() * 4ull, &rc));
-            if (rc) return rc;
-        }
-    }
+    if ((rc = output_tables(mctx, spec->tone, spec->curve, a))) return rc;
     // run.rs:545: the edge-aware and AHD demosaics apply to single-component CFA frames at full size
     if (spec->edge && !spec->half_size && cpp == 1 && has_cfa) {
         const frh_edge_spec *e = spec->edge;

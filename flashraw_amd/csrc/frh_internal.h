@@ -99,6 +99,14 @@ struct Plan {
     InputDesc in;
 };
 
+// Output encodings: channels, bytes per sample and image format of `mode`; whether it is one;
+// whether it is a display output (oriented, toned).
+void out_format(int32_t mode, uint32_t *ch, uint32_t *bps, int32_t *fmt);
+bool valid_mode(int32_t m);
+bool display_mode(int32_t m);
+// The display outputs' tone and output curve into a's table pointers (device copies cached by
+// the context).
+int32_t output_tables(frh_context *ctx, const frh_tone *tone, const frh_curve *curve, DevArgs &a);
 // Validate a develop request and fill `p` (status code; the message is the thread's last error).
 int32_t plan_develop(const frh_context *ctx, const frh_frame *f, const frh_frame_info *info,
                      const frh_develop_spec *spec, Plan &p);

@@ -1,0 +1,64 @@
+// Luma / chroma noise reduction of developed linear RGB: device tables and kernel launchers
+// shared by the kernels (nr.hip) and the host pipeline (nr_api.cpp).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "frh_device.h"
+
+namespace frh {
+
+constexpr int NR_BINS = 12;  // most bins of a noise model (Model::from_stats, denoise/mod.rs:815)
+
+// The noise model of one pyramid level (denoise.rs:30-39): per bin of luma mean, the deviation
+// of each DCT frequency (entry l * 8 + k: horizontal l, vertical k; entry 0 unused).
+struct NrModel {
+    uint32_t n;
+    float m[NR_BINS];
+    float s[NR_BINS][64];
+};
+
+// Per-pixel noise variance by luma mean (Variance, denoise/mod.rs:255-276); n = 0: no model.
+struct NrVar {
+    uint32_t n;
+    float m[NR_BINS];
+    float v[NR_BINS];
+};
+
+// One DCT pass over a w x h plane padded symmetrically by 8 (blocks at the positions 0, step,
+// 2 step, ... of each padded side, and always its last position `xlast` / `ylast`). Tiles of 64 x
+// 64 block positions accumulate their blocks' weighted samples in a 72 x 72 partial each.
+struct NrPass {
+    const float *src, *guide, *pilot;
+    uint32_t w, h;
+    uint32_t step;
+    uint32_t xlast, ylast;
+    uint32_t tiles_x, tiles_y;
+    const NrModel *model;
+    const float *dct;  // the 8 x 8 DCT-II matrix, dct[k * 8 + i]
+    float thr;         // hard threshold times the plane's gain
+    float n2scale;     // gain squared
+    float *pacc, *pws;
+};
+
+constexpr int NR_TILE = 64;
+constexpr int NR_EXT = NR_TILE + 8;
+
+hipError_t nr_split(const float *rgb, uint32_t w, uint32_t h, float *y, float *u, float *v, hipStream_t st);
+hipError_t nr_down(const float *src, uint32_t w, uint32_t h, float *dst, const float *minus, hipStream_t st);
+hipError_t nr_add_up(float *d, uint32_t w, uint32_t h, const float *coarse, hipStream_t st);
+hipError_t nr_blend(float *d, const float *orig, float keep, uint64_t n, hipStream_t st);
+hipError_t nr_mul(const float *a, const float *b, float *out, uint64_t n, hipStream_t st);
+hipError_t nr_box(const float *a, uint32_t w, uint32_t h, uint32_t r, float *tmp, float *out, hipStream_t st);
+hipError_t nr_den(float *var, const float *mean, const NrVar *nv, uint64_t n, hipStream_t st);
+hipError_t nr_gf_coef(const float *cov, const float *mp, const float *mean, const float *den, float *a, float *b,
+                      uint64_t n, hipStream_t st);
+hipError_t nr_gf_apply(const float *ma, const float *mb, const float *g, uint32_t w, uint32_t h, float *out,
+                       hipStream_t st);
+hipError_t nr_stats(const float *p, const float *g, uint32_t w, uint32_t step, uint32_t cols, uint32_t rows,
+                    const float *dct, float *out, hipStream_t st);
+hipError_t nr_pass(const NrPass &P, float *out, hipStream_t st);
+hipError_t nr_encode(const DevArgs &a, const float *y, const float *u, const float *v, hipStream_t st);
+
+}  // namespace frh

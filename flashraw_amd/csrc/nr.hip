@@ -1,0 +1,551 @@
+// Luma / chroma noise reduction of developed linear RGB (gfx950): the kernels of the method of
+// crates/flashraw/src/denoise/mod.rs on f32 planes in HBM.
+//
+// Pointwise and separable steps are one thread per sample: split / merge into luminance and the
+// two colour differences (433-464), 2x2 means (down 467-486), bilinear upsampling (tap / up_row
+// 488-515, add_up, up_affine), strength blends, box means (280-335, direct sums instead of the
+// CPU's running sums) and the guided filter's coefficients (337-382). The noise estimate measures
+// one block per thread (estimate 761-788). The DCT passes (dct_pass / run_group_impl 900-1131)
+// run in tiles of 64 x 64 block positions, one wave per tile: the wave's 64 lanes take the 64
+// blocks of one position class (positions congruent modulo 8 do not overlap), transform them in
+// registers (the reference's even / odd 8-point factorisation, vertical then horizontal),
+// threshold or Wiener-filter them against the model's deviations at the block's luma mean, and
+// add the weighted inverse blocks to the tile's 72 x 72 sums in LDS, class after class, so no
+// two lanes add to one sample at once and the sums run in a fixed order. A gather kernel adds the
+// (at most four) tile partials of every sample in a fixed order and divides by the weights.
+#include <hip/hip_runtime.h>
+
+#include "develop_common.h"
+#include "nr.h"
+
+namespace frh {
+namespace {
+
+constexpr float KR = 0.2126f, KG = 0.7152f, KB = 0.0722f;
+
+__device__ __forceinline__ float dot4(float c0, float c1, float c2, float c3, float x0, float x1, float x2,
+                                      float x3) {
+    return (c0 * x0 + c1 * x1) + (c2 * x2 + c3 * x3);
+}
+
+// The DCT matrix entries the factorisation reads (Dct::fwd1 / inv1, denoise/mod.rs:643-686).
+struct D8 {
+    float m00, m20, m21, m40, m60, m61;
+    float o[4][4];  // odd rows 1, 3, 5, 7, columns 0..3
+};
+
+__device__ __forceinline__ D8 load_d8(const float *__restrict__ d) {
+    D8 r;
+    r.m00 = d[0];
+    r.m20 = d[16];
+    r.m21 = d[17];
+    r.m40 = d[32];
+    r.m60 = d[48];
+    r.m61 = d[49];
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) r.o[q][i] = d[(2 * q + 1) * 8 + i];
+    return r;
+}
+
+// out[k] = sum_i m[k][i] x[i]
+__device__ __forceinline__ void fwd1(const D8 &D, const float x[8], float out[8]) {
+    float s[4], d[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        s[i] = x[i] + x[7 - i];
+        d[i] = x[i] - x[7 - i];
+    }
+    const float e0 = s[0] + s[3], e1 = s[1] + s[2], e2 = s[0] - s[3], e3 = s[1] - s[2];
+    out[0] = D.m00 * (e0 + e1);
+    out[1] = dot4(D.o[0][0], D.o[0][1], D.o[0][2], D.o[0][3], d[0], d[1], d[2], d[3]);
+    out[2] = D.m20 * e2 + D.m21 * e3;
+    out[3] = dot4(D.o[1][0], D.o[1][1], D.o[1][2], D.o[1][3], d[0], d[1], d[2], d[3]);
+    out[4] = D.m40 * (e0 - e1);
+    out[5] = dot4(D.o[2][0], D.o[2][1], D.o[2][2], D.o[2][3], d[0], d[1], d[2], d[3]);
+    out[6] = D.m60 * e2 + D.m61 * e3;
+    out[7] = dot4(D.o[3][0], D.o[3][1], D.o[3][2], D.o[3][3], d[0], d[1], d[2], d[3]);
+}
+
+// out[i] = sum_k m[k][i] x[k]
+__device__ __forceinline__ void inv1(const D8 &D, const float x[8], float out[8]) {
+    const float p = D.m00 * (x[0] + x[4]);
+    const float q = D.m00 * (x[0] - x[4]);
+    const float u = D.m20 * x[2] + D.m60 * x[6];
+    const float v = D.m21 * x[2] + D.m61 * x[6];
+    const float e[4] = {p + u, q + v, q - v, p - u};
+    float o[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) o[i] = dot4(D.o[0][i], D.o[1][i], D.o[2][i], D.o[3][i], x[1], x[3], x[5], x[7]);
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        out[i] = e[i] + o[i];
+        out[7 - i] = e[i] - o[i];
+    }
+}
+
+// The 2-D transform of a block given its rows as addresses: c[l][k] is horizontal frequency l,
+// vertical frequency k (Dct::block: the columns' vertical transforms, then the horizontal one).
+__device__ __forceinline__ void block_fwd(const D8 &D, const float *__restrict__ p, const uint32_t row[8],
+                                          const uint32_t col[8], float c[8][8]) {
+    float v[8][8];  // v[k][j]
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        float x[8], o[8];
+#pragma unroll
+        for (int i = 0; i < 8; i++) x[i] = p[row[i] + col[j]];
+        fwd1(D, x, o);
+#pragma unroll
+        for (int k = 0; k < 8; k++) v[k][j] = o[k];
+    }
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        float o[8];
+        fwd1(D, v[k], o);
+#pragma unroll
+        for (int l = 0; l < 8; l++) c[l][k] = o[l];
+    }
+}
+
+// pad_map (denoise/mod.rs:871-886): plane index of padded position q of a side of n samples.
+__device__ __forceinline__ uint32_t reflect8(int q, int n) {
+    const int i = q - 8;
+    const int r = i < 0 ? -i - 1 : (i >= n ? 2 * n - i - 1 : i);
+    return (uint32_t)min(max(r, 0), n - 1);
+}
+
+// tap (denoise/mod.rs:488-494): the two taps and the second one's weight of bilinear upsampling
+// to position i from a level of side n.
+__device__ __forceinline__ void tap(uint32_t i, uint32_t n, uint32_t &a, uint32_t &b, float &f) {
+    const float s = fmaxf(((float)i + 0.5f) * 0.5f - 0.5f, 0.0f);
+    a = min((uint32_t)s, n - 1);
+    b = min(a + 1, n - 1);
+    f = s - (float)a;
+}
+
+__device__ __forceinline__ float up_at(const float *__restrict__ p, uint32_t pw, uint32_t ph, uint32_t x, uint32_t y) {
+    uint32_t x0, x1, y0, y1;
+    float fx, fy;
+    tap(x, pw, x0, x1, fx);
+    tap(y, ph, y0, y1, fy);
+    const float *r0 = p + (size_t)y0 * pw, *r1 = p + (size_t)y1 * pw;
+    const float a = r0[x0] + (r0[x1] - r0[x0]) * fx;
+    const float b = r1[x0] + (r1[x1] - r1[x0]) * fx;
+    return a + (b - a) * fy;
+}
+
+__global__ __launch_bounds__(256) void k_split(const float *__restrict__ rgb, uint64_t n, float *__restrict__ y,
+                                               float *__restrict__ u, float *__restrict__ v) {
+    const uint64_t k = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (k >= n) return;
+    const float r = rgb[3 * k], g = rgb[3 * k + 1], b = rgb[3 * k + 2];
+    const float l = KR * r + KG * g + KB * b;
+    y[k] = l;
+    u[k] = b - l;
+    v[k] = r - l;
+}
+
+// 2x2 mean (the last row or column repeated on odd sides); with `minus`, minus - mean.
+__global__ __launch_bounds__(256) void k_down(const float *__restrict__ p, uint32_t w, uint32_t h,
+                                              float *__restrict__ o, const float *__restrict__ minus) {
+    const uint32_t dw = (w + 1) / 2, dh = (h + 1) / 2;
+    const uint32_t x = blockIdx.x * 256 + threadIdx.x, y = blockIdx.y;
+    if (x >= dw || y >= dh) return;
+    const uint32_t y0 = 2 * y, y1 = min(2 * y + 1, h - 1);
+    const float *r0 = p + (size_t)y0 * w, *r1 = p + (size_t)y1 * w;
+    float m;
+    if (2 * x + 1 < w) {
+        m = 0.25f * (r0[2 * x] + r0[2 * x + 1] + r1[2 * x] + r1[2 * x + 1]);
+    } else {
+        const uint32_t c = w - 1;
+        m = 0.25f * (r0[c] + r0[c] + r1[c] + r1[c]);
+    }
+    const size_t k = (size_t)y * dw + x;
+    o[k] = minus ? minus[k] - m : m;
+}
+
+__global__ __launch_bounds__(256) void k_add_up(float *__restrict__ d, uint32_t w, uint32_t h,
+                                                const float *__restrict__ c) {
+    const uint32_t x = blockIdx.x * 256 + threadIdx.x, y = blockIdx.y;
+    if (x >= w || y >= h) return;
+    d[(size_t)y * w + x] += up_at(c, (w + 1) / 2, (h + 1) / 2, x, y);
+}
+
+__global__ __launch_bounds__(256) void k_blend(float *__restrict__ d, const float *__restrict__ o, float keep,
+                                               uint64_t n) {
+    const uint64_t k = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (k < n) d[k] += keep * (o[k] - d[k]);
+}
+
+__global__ __launch_bounds__(256) void k_mul(const float *__restrict__ a, const float *__restrict__ b,
+                                             float *__restrict__ o, uint64_t n) {
+    const uint64_t k = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (k < n) o[k] = a[k] * b[k];
+}
+
+// Box sums of radius r with repeated edges: vertical, then horizontal divided by (2r + 1)^2.
+__global__ __launch_bounds__(256) void k_box_v(const float *__restrict__ a, uint32_t w, uint32_t h, uint32_t r,
+                                               float *__restrict__ o) {
+    const uint32_t x = blockIdx.x * 256 + threadIdx.x, y = blockIdx.y;
+    if (x >= w || y >= h) return;
+    float s = 0.0f;
+    for (int d = -(int)r; d <= (int)r; d++) {
+        const int yy = min(max((int)y + d, 0), (int)h - 1);
+        s += a[(size_t)yy * w + x];
+    }
+    o[(size_t)y * w + x] = s;
+}
+
+__global__ __launch_bounds__(256) void k_box_h(const float *__restrict__ a, uint32_t w, uint32_t h, uint32_t r,
+                                               float n, float *__restrict__ o) {
+    const uint32_t x = blockIdx.x * 256 + threadIdx.x, y = blockIdx.y;
+    if (x >= w || y >= h) return;
+    const float *row = a + (size_t)y * w;
+    float s = 0.0f;
+    for (int d = -(int)r; d <= (int)r; d++) s += row[min(max((int)x + d, 0), (int)w - 1)];
+    o[(size_t)y * w + x] = s / n;
+}
+
+// Variance::at (denoise/mod.rs:261-275)
+__device__ __forceinline__ float var_at(const NrVar *__restrict__ nv, float m) {
+    const uint32_t n = nv->n;
+    uint32_t i = 0;
+    while (i < n && nv->m[i] < m) i++;
+    if (i == 0) return nv->v[0];
+    if (i >= n) return nv->v[n - 1];
+    const float a = nv->m[i - 1], b = nv->m[i];
+    const float t = b > a ? (m - a) / (b - a) : 0.0f;
+    return nv->v[i - 1] + (nv->v[i] - nv->v[i - 1]) * t;
+}
+
+// The guided filter's denominator: max(E[i^2] - E[i]^2, 0) + max(eps(E[i]), 1e-9), eps the luma
+// noise variance per pixel at the window's mean (1e-6 without a model).
+__global__ __launch_bounds__(256) void k_den(float *__restrict__ var, const float *__restrict__ mean,
+                                             const NrVar *__restrict__ nv, uint64_t n) {
+    const uint64_t k = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (k >= n) return;
+    const float m = mean[k];
+    const float v = fmaxf(var[k] - m * m, 0.0f);
+    const float e = nv->n ? var_at(nv, m) : 1e-6f;
+    var[k] = v + fmaxf(e, 1e-9f);
+}
+
+__global__ __launch_bounds__(256) void k_gf_coef(const float *__restrict__ cov, const float *__restrict__ mp,
+                                                 const float *__restrict__ mean, const float *__restrict__ den,
+                                                 float *__restrict__ a, float *__restrict__ b, uint64_t n) {
+    const uint64_t k = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (k >= n) return;
+    const float av = (cov[k] - mean[k] * mp[k]) / den[k];
+    a[k] = av;
+    b[k] = mp[k] - av * mean[k];
+}
+
+// up(mb) + up(ma) * g (up_affine, denoise/mod.rs:537-555)
+__global__ __launch_bounds__(256) void k_gf_apply(const float *__restrict__ ma, const float *__restrict__ mb,
+                                                  const float *__restrict__ g, uint32_t w, uint32_t h,
+                                                  float *__restrict__ o) {
+    const uint32_t x = blockIdx.x * 256 + threadIdx.x, y = blockIdx.y;
+    if (x >= w || y >= h) return;
+    const uint32_t cw = (w + 1) / 2, ch = (h + 1) / 2;
+    const size_t k = (size_t)y * w + x;
+    o[k] = up_at(mb, cw, ch, x, y) + up_at(ma, cw, ch, x, y) * g[k];
+}
+
+// The measured blocks (estimate, denoise/mod.rs:761-788): block (c, r) of the grid of `step`
+// blocks; the guide's mean over the block (row-major sum / 64), then the squared coefficients.
+__global__ __launch_bounds__(256) void k_stats(const float *__restrict__ p, const float *__restrict__ g, uint32_t w,
+                                               uint32_t step, uint32_t cols, uint32_t rows,
+                                               const float *__restrict__ dct, float *__restrict__ out) {
+    const uint32_t idx = blockIdx.x * 256 + threadIdx.x;
+    if (idx >= cols * rows) return;
+    const D8 D = load_d8(dct);
+    const uint32_t x0 = (idx % cols) * step * 8, y0 = (idx / cols) * step * 8;
+    uint32_t row[8], col[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        row[i] = (y0 + i) * w;
+        col[i] = x0 + i;
+    }
+    float mean = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 8; i++)
+#pragma unroll
+        for (int j = 0; j < 8; j++) mean += g[row[i] + col[j]];
+    float c[8][8];
+    block_fwd(D, p, row, col, c);
+    float *o = out + (size_t)idx * 64;
+#pragma unroll
+    for (int l = 0; l < 8; l++)
+#pragma unroll
+        for (int k = 0; k < 8; k++) o[l * 8 + k] = (l | k) ? c[l][k] * c[l][k] : mean / 64.0f;
+}
+
+// One DCT pass, one wave per tile of 64 x 64 block positions (see the file comment).
+template <bool WIENER>
+__global__ __launch_bounds__(64) void k_pass(NrPass P) {
+    __shared__ float acc[NR_EXT * NR_EXT], wsum[NR_EXT * NR_EXT];
+    __shared__ float sm[NR_BINS], ss[NR_BINS * 64];
+    const int lane = threadIdx.x;
+    const int tx = (int)(blockIdx.x % P.tiles_x), ty = (int)(blockIdx.x / P.tiles_x);
+    const int TX = tx * NR_TILE, TY = ty * NR_TILE;
+    for (int i = lane; i < NR_EXT * NR_EXT; i += 64) acc[i] = wsum[i] = 0.0f;
+    const uint32_t nb = P.model->n;
+    for (int i = lane; i < (int)nb * 64; i += 64) ss[i] = (&P.model->s[0][0])[i];
+    if (lane < (int)nb) sm[lane] = P.model->m[lane];
+    __syncthreads();
+    const D8 D = load_d8(P.dct);
+    const int step = (int)P.step, ncls = 8 / step;
+    const int xlast = (int)P.xlast, ylast = (int)P.ylast;
+    const bool xirr = xlast % step != 0, yirr = ylast % step != 0;
+    const int w = (int)P.w, h = (int)P.h;
+    const int bi = lane & 7, bj = lane >> 3;
+    for (int cy = 0; cy <= ncls; cy++) {
+        if (cy == ncls && !yirr) break;
+        for (int cx = 0; cx <= ncls; cx++) {
+            if (cx == ncls && !xirr) break;
+            int x0, y0;
+            bool valid;
+            if (cx < ncls) {
+                x0 = TX + cx * step + 8 * bi;
+                valid = x0 <= xlast;
+            } else {
+                x0 = xlast;
+                valid = bi == 0 && xlast >= TX && xlast < TX + NR_TILE;
+            }
+            if (cy < ncls) {
+                y0 = TY + cy * step + 8 * bj;
+                valid = valid && y0 <= ylast;
+            } else {
+                y0 = ylast;
+                valid = valid && bj == 0 && ylast >= TY && ylast < TY + NR_TILE;
+            }
+            if (valid) {
+                uint32_t row[8], col[8];
+#pragma unroll
+                for (int i = 0; i < 8; i++) {
+                    row[i] = reflect8(y0 + i, h) * (uint32_t)w;
+                    col[i] = reflect8(x0 + i, w);
+                }
+                float c[8][8];
+                block_fwd(D, P.src, row, col, c);
+                // the guide's mean over the block: column sums, then their sum / 64
+                float gs = 0.0f;
+                {
+                    float gcol[8];
+#pragma unroll
+                    for (int j = 0; j < 8; j++) gcol[j] = 0.0f;
+#pragma unroll
+                    for (int i = 0; i < 8; i++)
+#pragma unroll
+                        for (int j = 0; j < 8; j++) gcol[j] += P.guide[row[i] + col[j]];
+#pragma unroll
+                    for (int j = 0; j < 8; j++) gs += gcol[j];
+                }
+                const float mean = gs / 64.0f;
+                // Model::sigmas: the bins around the mean and the weight of the upper one
+                uint32_t bin = 0;
+                while (bin < nb && sm[bin] < mean) bin++;
+                uint32_t ia, ib;
+                float t;
+                if (bin == 0 || nb == 1) {
+                    ia = ib = 0;
+                    t = 0.0f;
+                } else if (bin >= nb) {
+                    ia = ib = nb - 1;
+                    t = 0.0f;
+                } else {
+                    ia = bin - 1;
+                    ib = bin;
+                    const float a = sm[ia], b = sm[ib];
+                    t = b > a ? (mean - a) / (b - a) : 0.0f;
+                }
+                const float *sa = ss + ia * 64, *sb = ss + ib * 64;
+                float weight;
+                if (WIENER) {
+                    float pc[8][8];
+                    block_fwd(D, P.pilot, row, col, pc);
+                    const float dc = c[0][0];
+                    float e[8];
+#pragma unroll
+                    for (int k = 0; k < 8; k++) e[k] = 0.0f;
+#pragma unroll
+                    for (int l = 0; l < 8; l++)
+#pragma unroll
+                        for (int k = 0; k < 8; k++) {
+                            const float a = sa[l * 8 + k], b = sb[l * 8 + k];
+                            const float sg = a + (b - a) * t;
+                            const float n2 = sg * sg * P.n2scale;
+                            const float p2 = pc[l][k] * pc[l][k];
+                            const float s = p2 / (p2 + n2 + 1e-30f);
+                            c[l][k] = c[l][k] * s;
+                            e[k] = e[k] + s * s * n2;
+                        }
+                    c[0][0] = dc;
+                    const float es = ((e[0] + e[4]) + (e[1] + e[5])) + ((e[2] + e[6]) + (e[3] + e[7]));
+                    weight = 1.0f / (es + 1e-20f);
+                } else {
+                    uint32_t kept = 0;
+#pragma unroll
+                    for (int l = 0; l < 8; l++)
+#pragma unroll
+                        for (int k = 0; k < 8; k++) {
+                            const float a = sa[l * 8 + k], b = sb[l * 8 + k];
+                            const float sg = a + (b - a) * t;
+                            const bool keep = fabsf(c[l][k]) >= sg * P.thr;
+                            kept += keep ? 1u : 0u;
+                            c[l][k] = keep ? c[l][k] : 0.0f;
+                        }
+                    weight = 1.0f / (float)kept;
+                }
+                // weighted inverse: horizontal (per vertical frequency), then vertical (per column)
+                float tcol[8][8];  // tcol[j][k]
+#pragma unroll
+                for (int k = 0; k < 8; k++) {
+                    float x[8], o[8];
+#pragma unroll
+                    for (int l = 0; l < 8; l++) x[l] = c[l][k] * weight;
+                    inv1(D, x, o);
+#pragma unroll
+                    for (int j = 0; j < 8; j++) tcol[j][k] = o[j];
+                }
+                const int ox = x0 - TX, oy = y0 - TY;
+#pragma unroll
+                for (int j = 0; j < 8; j++) {
+                    float o[8];
+                    inv1(D, tcol[j], o);
+#pragma unroll
+                    for (int i = 0; i < 8; i++) {
+                        const int q = (oy + i) * NR_EXT + ox + j;
+                        acc[q] += o[i];
+                        wsum[q] += weight;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    float *pa = P.pacc + (size_t)blockIdx.x * (NR_EXT * NR_EXT);
+    float *pw = P.pws + (size_t)blockIdx.x * (NR_EXT * NR_EXT);
+    for (int i = lane; i < NR_EXT * NR_EXT; i += 64) {
+        pa[i] = acc[i];
+        pw[i] = wsum[i];
+    }
+}
+
+// Every sample: the partials of the tiles whose blocks cover it, in tile order, then the
+// weighted mean (the sample itself where no block has weight).
+__global__ __launch_bounds__(256) void k_gather(NrPass P, float *__restrict__ out) {
+    const uint32_t x = blockIdx.x * 256 + threadIdx.x, y = blockIdx.y;
+    if (x >= P.w || y >= P.h) return;
+    const int px = (int)x + 8, py = (int)y + 8;
+    const int tx1 = px / NR_TILE, ty1 = py / NR_TILE;
+    float a = 0.0f, ws = 0.0f;
+    for (int ty = ty1 - 1; ty <= ty1; ty++) {
+        if (ty < 0 || ty >= (int)P.tiles_y || py - ty * NR_TILE >= NR_EXT) continue;
+        for (int tx = tx1 - 1; tx <= tx1; tx++) {
+            if (tx < 0 || tx >= (int)P.tiles_x || px - tx * NR_TILE >= NR_EXT) continue;
+            const size_t o = (size_t)(ty * (int)P.tiles_x + tx) * (NR_EXT * NR_EXT) +
+                             (size_t)((py - ty * NR_TILE) * NR_EXT + (px - tx * NR_TILE));
+            a += P.pacc[o];
+            ws += P.pws[o];
+        }
+    }
+    const size_t k = (size_t)y * P.w + x;
+    out[k] = ws > 0.0f ? a / ws : P.src[k];
+}
+
+// merge (denoise/mod.rs:450-464) and the develop's encoders (tone, curve, orientation)
+__global__ __launch_bounds__(256) void k_encode(DevArgs a, const float *__restrict__ py,
+                                                const float *__restrict__ pu, const float *__restrict__ pv) {
+    __shared__ Lds s;
+    load_lds(a, s);
+    __syncthreads();
+    const uint32_t x = blockIdx.x * 256 + threadIdx.x, y = blockIdx.y;
+    if (x >= a.dev_w || y >= a.dev_h) return;
+    const size_t k = (size_t)y * a.dev_w + x;
+    const float yv = py[k];
+    const float r = pv[k] + yv, b = pu[k] + yv;
+    const float g = (yv - KR * r - KB * b) / KG;
+    store_both(a, s, x, y, r, g, b);
+}
+
+inline dim3 grid2(uint32_t w, uint32_t h) { return dim3((w + 255) / 256, h); }
+inline uint32_t grid1(uint64_t n) { return (uint32_t)((n + 255) / 256); }
+
+}  // namespace
+
+hipError_t nr_split(const float *rgb, uint32_t w, uint32_t h, float *y, float *u, float *v, hipStream_t st) {
+    const uint64_t n = (uint64_t)w * h;
+    hipLaunchKernelGGL(k_split, dim3(grid1(n)), dim3(256), 0, st, rgb, n, y, u, v);
+    return hipGetLastError();
+}
+
+hipError_t nr_down(const float *src, uint32_t w, uint32_t h, float *dst, const float *minus, hipStream_t st) {
+    hipLaunchKernelGGL(k_down, grid2((w + 1) / 2, (h + 1) / 2), dim3(256), 0, st, src, w, h, dst, minus);
+    return hipGetLastError();
+}
+
+hipError_t nr_add_up(float *d, uint32_t w, uint32_t h, const float *coarse, hipStream_t st) {
+    hipLaunchKernelGGL(k_add_up, grid2(w, h), dim3(256), 0, st, d, w, h, coarse);
+    return hipGetLastError();
+}
+
+hipError_t nr_blend(float *d, const float *orig, float keep, uint64_t n, hipStream_t st) {
+    hipLaunchKernelGGL(k_blend, dim3(grid1(n)), dim3(256), 0, st, d, orig, keep, n);
+    return hipGetLastError();
+}
+
+hipError_t nr_mul(const float *a, const float *b, float *out, uint64_t n, hipStream_t st) {
+    hipLaunchKernelGGL(k_mul, dim3(grid1(n)), dim3(256), 0, st, a, b, out, n);
+    return hipGetLastError();
+}
+
+hipError_t nr_box(const float *a, uint32_t w, uint32_t h, uint32_t r, float *tmp, float *out, hipStream_t st) {
+    hipLaunchKernelGGL(k_box_v, grid2(w, h), dim3(256), 0, st, a, w, h, r, tmp);
+    const float n = (float)((2 * r + 1) * (2 * r + 1));
+    hipLaunchKernelGGL(k_box_h, grid2(w, h), dim3(256), 0, st, tmp, w, h, r, n, out);
+    return hipGetLastError();
+}
+
+hipError_t nr_den(float *var, const float *mean, const NrVar *nv, uint64_t n, hipStream_t st) {
+    hipLaunchKernelGGL(k_den, dim3(grid1(n)), dim3(256), 0, st, var, mean, nv, n);
+    return hipGetLastError();
+}
+
+hipError_t nr_gf_coef(const float *cov, const float *mp, const float *mean, const float *den, float *a, float *b,
+                      uint64_t n, hipStream_t st) {
+    hipLaunchKernelGGL(k_gf_coef, dim3(grid1(n)), dim3(256), 0, st, cov, mp, mean, den, a, b, n);
+    return hipGetLastError();
+}
+
+hipError_t nr_gf_apply(const float *ma, const float *mb, const float *g, uint32_t w, uint32_t h, float *out,
+                       hipStream_t st) {
+    hipLaunchKernelGGL(k_gf_apply, grid2(w, h), dim3(256), 0, st, ma, mb, g, w, h, out);
+    return hipGetLastError();
+}
+
+hipError_t nr_stats(const float *p, const float *g, uint32_t w, uint32_t step, uint32_t cols, uint32_t rows,
+                    const float *dct, float *out, hipStream_t st) {
+    const uint64_t n = (uint64_t)cols * rows;
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_stats, dim3(grid1(n)), dim3(256), 0, st, p, g, w, step, cols, rows, dct, out);
+    return hipGetLastError();
+}
+
+hipError_t nr_pass(const NrPass &P, float *out, hipStream_t st) {
+    const uint32_t tiles = P.tiles_x * P.tiles_y;
+    if (P.pilot)
+        hipLaunchKernelGGL(k_pass<true>, dim3(tiles), dim3(64), 0, st, P);
+    else
+        hipLaunchKernelGGL(k_pass<false>, dim3(tiles), dim3(64), 0, st, P);
+    hipLaunchKernelGGL(k_gather, grid2(P.w, P.h), dim3(256), 0, st, P, out);
+    return hipGetLastError();
+}
+
+hipError_t nr_encode(const DevArgs &a, const float *y, const float *u, const float *v, hipStream_t st) {
+    hipLaunchKernelGGL(k_encode, grid2(a.dev_w, a.dev_h), dim3(256), 0, st, a, y, u, v);
+    return hipGetLastError();
+}
+
+}  // namespace frh

@@ -127,6 +127,18 @@ class _Timings(ctypes.Structure):
                 ("has_kernel_ms", ctypes.c_int32), ("bands", ctypes.c_uint32)]
 
 
+class _NrPlane(ctypes.Structure):
+    _fields_ = [("levels", ctypes.c_uint32), ("gain", ctypes.c_float), ("steps", ctypes.c_uint32 * 2),
+                ("hard_only0", ctypes.c_int32), ("keep", ctypes.c_float)]
+
+
+class _NrSpec(ctypes.Structure):
+    _fields_ = [("size", ctypes.c_uint32), ("has_luma", ctypes.c_int32), ("has_chroma", ctypes.c_int32),
+                ("luma", _NrPlane), ("chroma", _NrPlane), ("chroma_radius", ctypes.c_uint32),
+                ("hard", ctypes.c_float), ("max_blocks", ctypes.c_uint32), ("output", ctypes.c_int32),
+                ("orientation", ctypes.c_int32), ("tone", ctypes.POINTER(_Tone)), ("curve", ctypes.POINTER(_Curve))]
+
+
 _lib = None
 _lib_lock = threading.Lock()
 
@@ -198,6 +210,14 @@ def lib():
             L.frh_context_stream.restype = vp
             L.frh_denoise_cfa.argtypes = [vp, ctypes.POINTER(_DnSpec), vp, vp, vp, vp]
             L.frh_denoise_cfa.restype = i32
+            L.frh_nr_spec_for.argtypes = [u32, u32, ctypes.c_float, ctypes.c_float, i32, ctypes.POINTER(_NrSpec)]
+            L.frh_nr_spec_for.restype = i32
+            L.frh_run_denoise.argtypes = [vp, vp, ctypes.POINTER(_NrSpec), ctypes.POINTER(vp), ctypes.POINTER(_Timings)]
+            L.frh_run_denoise.restype = i32
+            L.frh_denoise_into.argtypes = [vp, vp, u32, u32, ctypes.POINTER(_NrSpec), vp, u64, vp]
+            L.frh_denoise_into.restype = i32
+            L.frh_nr_model_from_stats.argtypes = [vp, u32, ctypes.POINTER(u32), vp, vp]
+            L.frh_nr_model_from_stats.restype = i32
             L.frh_batch_create.argtypes = [vp, u32, ctypes.POINTER(_Frame), ctypes.POINTER(_FrameInfo),
                                            ctypes.POINTER(_DevelopSpec), ctypes.POINTER(vp), ctypes.POINTER(vp)]
             L.frh_batch_create.restype = i32
@@ -526,6 +546,33 @@ class GpuTone:
     white: Sequence[float] = (1.0, 1.0, 1.0)
 
 
+def _tone_curve_c(tone, curve, keep):
+    """Pointers to the C forms of a GpuTone and a GpuCurve (or NULL); `keep` holds their arrays."""
+    tp, cp = ctypes.POINTER(_Tone)(), ctypes.POINTER(_Curve)()
+    if curve is not None:
+        c = _Curve()
+        c.size = ctypes.sizeof(_Curve)
+        l8 = np.ascontiguousarray(curve.lut8, dtype=np.uint8)
+        l16 = np.ascontiguousarray(curve.lut16, dtype=np.float32)
+        keep += [l8, l16, c]
+        c.lut8, c.lut8_len = (l8.ctypes.data if l8.size else None), l8.size
+        c.lut16, c.lut16_len = (l16.ctypes.data if l16.size else None), l16.size
+        cp = ctypes.pointer(c)
+    if tone is not None:
+        t = _Tone()
+        t.size = ctypes.sizeof(_Tone)
+        tab = np.ascontiguousarray(tone.table, dtype=np.float32).reshape(-1)
+        keep += [tab, t]
+        t.table, t.table_len = (tab.ctypes.data if tab.size else None), tab.size // 2
+        t.xmax, t.scale = float(tone.xmax), float(tone.scale)
+        for i, v in enumerate(np.asarray(tone.to_prophoto, np.float32).reshape(-1)):
+            t.to_prophoto[i] = float(v)
+        for i, v in enumerate(np.asarray(tone.white, np.float32).reshape(-1)):
+            t.white[i] = float(v)
+        tp = ctypes.pointer(t)
+    return tp, cp
+
+
 @dataclass
 class DevelopSpec:
     """DevelopSpec (run.rs:115-135). `output2` asks the same pass for a second encoding."""
@@ -558,27 +605,7 @@ class DevelopSpec:
             s.has_levels = 1
             s.levels[0], s.levels[1] = float(self.levels[0]), float(self.levels[1])
         keep = []
-        if self.curve is not None:
-            c = _Curve()
-            c.size = ctypes.sizeof(_Curve)
-            l8 = np.ascontiguousarray(self.curve.lut8, dtype=np.uint8)
-            l16 = np.ascontiguousarray(self.curve.lut16, dtype=np.float32)
-            keep += [l8, l16, c]
-            c.lut8, c.lut8_len = (l8.ctypes.data if l8.size else None), l8.size
-            c.lut16, c.lut16_len = (l16.ctypes.data if l16.size else None), l16.size
-            s.curve = ctypes.pointer(c)
-        if self.tone is not None:
-            t = _Tone()
-            t.size = ctypes.sizeof(_Tone)
-            tab = np.ascontiguousarray(self.tone.table, dtype=np.float32).reshape(-1)
-            keep += [tab, t]
-            t.table, t.table_len = (tab.ctypes.data if tab.size else None), tab.size // 2
-            t.xmax, t.scale = float(self.tone.xmax), float(self.tone.scale)
-            for i, v in enumerate(np.asarray(self.tone.to_prophoto, np.float32).reshape(-1)):
-                t.to_prophoto[i] = float(v)
-            for i, v in enumerate(np.asarray(self.tone.white, np.float32).reshape(-1)):
-                t.white[i] = float(v)
-            s.tone = ctypes.pointer(t)
+        s.tone, s.curve = _tone_curve_c(self.tone, self.curve, keep)
         if self.edge is not None:
             e = _EdgeSpec()
             e.size = ctypes.sizeof(_EdgeSpec)
@@ -618,7 +645,7 @@ class _DnSpec(ctypes.Structure):
 
 
 @dataclass
-class DenoiseSpec:
+class CfaDenoiseSpec:
     """The CFA-plane DCT block denoiser's parameters (frh_dn_spec): 8x8 blocks at stride `step`
     on each 2x2 phase plane, AC coefficients below threshold * sigma zeroed,
     sigma^2 = sig_a * max(block mean, 0) + sig_b (DN), uniform overlap-add."""
@@ -792,6 +819,80 @@ class GpuTimings:
     bands: int
 
 
+@dataclass
+class PlaneSpec:
+    """PlaneSpec (crates/flashraw-gpu/src/denoise.rs:44-57): how one plane of the noise
+    reduction is denoised."""
+    levels: int
+    gain: float = 1.0
+    steps: Sequence[int] = (2, 2)
+    hard_only0: bool = False
+    keep: float = 0.0
+
+
+@dataclass
+class DenoiseSpec:
+    """DenoiseSpec (denoise.rs:59-80) of GpuContext.denoise: the luma and colour planes (None
+    leaves a plane unchanged), the guided filter's radius at half resolution, the hard threshold,
+    the noise estimate's block budget and the output encoding. The noise models are measured and
+    computed by the library (the reference's `models` callback, flashraw::denoise::gpu_models)."""
+    luma: Optional[PlaneSpec] = None
+    chroma: Optional[PlaneSpec] = None
+    chroma_radius: int = 1
+    hard: float = 2.7
+    max_blocks: int = 50000
+    output: "DevelopOutput" = None
+    orientation: "Orientation" = None
+    tone: Optional[GpuTone] = None
+    curve: Optional[GpuCurve] = None
+
+    def _c(self):
+        s = _NrSpec()
+        s.size = ctypes.sizeof(_NrSpec)
+        for name, p in (("luma", self.luma), ("chroma", self.chroma)):
+            setattr(s, "has_" + name, int(p is not None))
+            if p is not None:
+                c = getattr(s, name)
+                c.levels, c.gain, c.hard_only0, c.keep = int(p.levels), float(p.gain), int(p.hard_only0), float(p.keep)
+                c.steps[0], c.steps[1] = int(p.steps[0]), int(p.steps[1])
+        s.chroma_radius, s.hard, s.max_blocks = int(self.chroma_radius), float(self.hard), int(self.max_blocks)
+        s.output = int(DevelopOutput.LinearRgbF32 if self.output is None else self.output)
+        s.orientation = int(Orientation.Normal if self.orientation is None else self.orientation)
+        keep = []
+        s.tone, s.curve = _tone_curve_c(self.tone, self.curve, keep)
+        return s, keep
+
+
+def denoise_spec(width: int, height: int, luma: float, chroma: float, output=None) -> DenoiseSpec:
+    """flashraw::denoise::gpu_spec (crates/flashraw/src/denoise/mod.rs:155-193): the spec of
+    strengths `luma` and `chroma` (clamped to [0, 1]; 0 leaves that part alone) for a
+    width x height image, from the library (frh_nr_spec_for)."""
+    s = _NrSpec()
+    out = DevelopOutput.LinearRgbF32 if output is None else output
+    check(lib().frh_nr_spec_for(width, height, luma, chroma, int(out), ctypes.byref(s)))
+
+    def plane(c):
+        return PlaneSpec(c.levels, c.gain, (c.steps[0], c.steps[1]), bool(c.hard_only0), c.keep)
+
+    return DenoiseSpec(plane(s.luma) if s.has_luma else None, plane(s.chroma) if s.has_chroma else None,
+                       s.chroma_radius, s.hard, s.max_blocks, DevelopOutput(s.output), Orientation(s.orientation))
+
+
+def nr_model_from_stats(stats) -> Optional[tuple]:
+    """Model::from_stats (mod.rs:803-848) in the library: (means, sigmas) of the measured blocks
+    (n, 64) (entry 0 the luma mean, then the squared coefficients), or None below 32 blocks."""
+    st = np.ascontiguousarray(stats, dtype=np.float32).reshape(-1, 64)
+    bins = ctypes.c_uint32(0)
+    means = np.zeros(12, np.float32)
+    sig = np.zeros((12, 64), np.float32)
+    rc = lib().frh_nr_model_from_stats(st.ctypes.data, st.shape[0], ctypes.byref(bins), means.ctypes.data,
+                                       sig.ctypes.data)
+    if rc == 9:
+        return None
+    check(rc)
+    return means[:bins.value].copy(), sig[:bins.value].copy()
+
+
 _FMT_DTYPE = {0: np.uint8, 1: np.float32, 2: np.uint16}
 
 
@@ -949,7 +1050,30 @@ class GpuContext:
         """A serving loop on this context (frh_server_create)."""
         return Server(self, device_budget_bytes, max_batch_frames, max_inflight_batches)
 
-    def denoise_cfa(self, spec: DenoiseSpec, src, dst=None, dst_f32=None, stream=None) -> None:
+    def denoise(self, image: "GpuImage", spec: DenoiseSpec) -> GpuRun:
+        """GpuContext::denoise (crates/flashraw-gpu/src/denoise.rs:817): the luma / chroma noise
+        reduction of a linear f32 RGB image (a develop to DevelopOutput.LinearRgbF32), encoded to
+        spec.output; blocking (denoise_blocking, denoise.rs:1180)."""
+        s, keep = spec._c()
+        out = ctypes.c_void_p()
+        t = _Timings()
+        check(lib().frh_run_denoise(self._h, image._h, ctypes.byref(s), ctypes.byref(out), ctypes.byref(t)))
+        del keep
+        timings = GpuTimings(t.upload_ms, t.compute_ms, t.kernel_ms if t.has_kernel_ms else None, t.bands)
+        return GpuRun(GpuImage(out.value), timings, None)
+
+    denoise_blocking = denoise
+
+    def denoise_into(self, rgb, width: int, height: int, spec: DenoiseSpec, dst, stream=None) -> None:
+        """The noise reduction of device-resident linear RGB (width * height * 3 floats) into the
+        caller's device buffer `dst` (torch tensors or addresses) on `stream`."""
+        s, keep = spec._c()
+        n = dst.numel() * dst.element_size() if hasattr(dst, "numel") else (1 << 62)
+        check(lib().frh_denoise_into(self._h, _device_ptr(rgb), width, height, ctypes.byref(s), _device_ptr(dst), n,
+                                     self.stream_handle(stream)))
+        del keep
+
+    def denoise_cfa(self, spec: CfaDenoiseSpec, src, dst=None, dst_f32=None, stream=None) -> None:
         """Denoise the device u16 mosaic `src` into `dst` (u16) and/or `dst_f32` (f32, DN above
         black), torch tensors or device addresses, on `stream` (frh_denoise_cfa)."""
         s = spec._c()
@@ -1012,7 +1136,7 @@ class Server:
         self._keep = {}
 
     def submit(self, frame: GpuFrame, info: FrameInfo, spec: DevelopSpec, out: np.ndarray,
-               denoise: Optional[DenoiseSpec] = None) -> int:
+               denoise: Optional[CfaDenoiseSpec] = None) -> int:
         """Queue a job; `frame` holds host data, `out` receives the output (both must stay alive
         until the job is collected by wait())."""
         f, k1 = frame._c()

@@ -302,6 +302,50 @@ typedef struct frh_dn_spec {
 int32_t frh_denoise_cfa(frh_context *ctx, const frh_dn_spec *spec, const void *src, void *dst, float *dst_f32,
                         void *stream);
 
+/* ---- luma / chroma noise reduction of developed linear RGB (GpuContext::denoise,
+ * crates/flashraw-gpu/src/denoise.rs:817; the method of crates/flashraw/src/denoise/mod.rs) ---- */
+
+/* PlaneSpec (denoise.rs:44-57): how one plane is denoised. */
+typedef struct frh_nr_plane {
+    uint32_t levels;    /* pyramid levels, the full image counting as one (1..8) */
+    float gain;         /* multiplier of the noise deviations (thresholds and Wiener noise) */
+    uint32_t steps[2];  /* block step of the two finest levels (2 beyond); 1, 2, 4 or 8 */
+    int32_t hard_only0; /* the full-resolution level gets the hard-threshold pass only */
+    float keep;         /* fraction of the difference to the input put back */
+} frh_nr_plane;
+
+/* DenoiseSpec (denoise.rs:59-80). The noise models the reference takes from the caller
+ * (`models`, flashraw::denoise::gpu_models, mod.rs:196-222) are computed by the library from the
+ * blocks it measures: frh_nr_model_from_stats, each coarse level capped by the finer one. */
+typedef struct frh_nr_spec {
+    uint32_t size;
+    int32_t has_luma, has_chroma; /* 0: that plane unchanged */
+    frh_nr_plane luma, chroma;
+    uint32_t chroma_radius; /* guided filter window radius at half resolution */
+    float hard;             /* hard threshold of the first pass, in noise deviations */
+    uint32_t max_blocks;    /* most blocks the noise estimate measures per level */
+    int32_t output;         /* frh_output */
+    int32_t orientation;    /* EXIF 1..8, display outputs */
+    const frh_tone *tone;
+    const frh_curve *curve;
+} frh_nr_spec;
+
+/* flashraw::denoise::gpu_spec (mod.rs:155-193): the spec of strengths `luma` and `chroma`
+ * (clamped to [0, 1]; 0 leaves that part unchanged) for a w x h image. */
+int32_t frh_nr_spec_for(uint32_t w, uint32_t h, float luma, float chroma, int32_t output, frh_nr_spec *spec);
+/* GpuContext::denoise: denoise the linear RGB image `input` (format F32, 3 channels, as a develop
+ * to FRH_OUTPUT_LINEAR_RGB_F32 returns it) and encode it to spec->output. */
+int32_t frh_run_denoise(frh_context *ctx, const frh_image *input, const frh_nr_spec *spec, frh_image **out,
+                        frh_timings *timings);
+/* The same on device buffers: `rgb` (w * h * 3 floats) into `dst` (the output's bytes), on
+ * `stream` (NULL: the context's); synchronises `stream` once to read the measured blocks back. */
+int32_t frh_denoise_into(frh_context *ctx, const float *rgb, uint32_t w, uint32_t h, const frh_nr_spec *spec,
+                         void *dst, uint64_t dst_bytes, void *stream);
+/* Model::from_stats (mod.rs:803-848): the noise model of `count` measured blocks (64 floats each:
+ * the luma mean, then the squared coefficients l * 8 + k) into at most 12 bins (`means`,
+ * `sigmas` of 12 * 64 floats). FRH_ERROR_NOT_AVAILABLE when fewer than 32 blocks are finite. */
+int32_t frh_nr_model_from_stats(const float *stats, uint32_t count, uint32_t *bins, float *means, float *sigmas);
+
 /* ---- serving loop ----
  * No reference counterpart: a request loop over one context. Jobs (a host frame, its info and
  * spec, an optional CFA-plane denoise, a host output buffer) are admitted in arrival order while

@@ -39,7 +39,7 @@ def _run(ctx, mos, precision, sig_a=2.0, sig_b=144.0, thr=2.7, step=4, seg_rows=
     src = torch.from_numpy(mos.view(np.int16)).to("cuda")
     dst = torch.zeros((h, w), dtype=torch.int16, device="cuda")
     dstf = torch.zeros((h, w), dtype=torch.float32, device="cuda")
-    spec = F.DenoiseSpec(w, h, BLACK, sig_a, sig_b, thr, step, precision, seg_rows=seg_rows)
+    spec = F.CfaDenoiseSpec(w, h, BLACK, sig_a, sig_b, thr, step, precision, seg_rows=seg_rows)
     ctx.denoise_cfa(spec, src, dst, dstf)
     torch.cuda.synchronize()
     return dst.cpu().numpy().view(np.uint16), dstf.cpu().numpy()
@@ -134,10 +134,10 @@ def test_rejects_unsupported(ctx):
     src = torch.zeros((60, 60), dtype=torch.int16, device="cuda")
     dst = torch.zeros_like(src)
     with pytest.raises(F.NotAvailable):
-        ctx.denoise_cfa(F.DenoiseSpec(60, 60, step=4), src, dst)  # sides not multiples of 8
+        ctx.denoise_cfa(F.CfaDenoiseSpec(60, 60, step=4), src, dst)  # sides not multiples of 8
     with pytest.raises(F.NotAvailable):
-        ctx.denoise_cfa(F.DenoiseSpec(64, 48, step=2), src, dst)
+        ctx.denoise_cfa(F.CfaDenoiseSpec(64, 48, step=2), src, dst)
     with pytest.raises(F.InvalidArgument):
-        ctx.denoise_cfa(F.DenoiseSpec(64, 48), src)  # no output
+        ctx.denoise_cfa(F.CfaDenoiseSpec(64, 48), src)  # no output
     with pytest.raises(F.InvalidArgument):
-        ctx.denoise_cfa(F.DenoiseSpec(14, 48, precision=F.DnPrecision.F32), src, dst)
+        ctx.denoise_cfa(F.CfaDenoiseSpec(14, 48, precision=F.DnPrecision.F32), src, dst)

@@ -33,7 +33,7 @@ def _jobs(seed, n, max_side=640):
                              output=F.DevelopOutput.Rgba8, edge=F.ahd_spec(info, origin) if ahd else None)
         dn = None
         if rng.random() < 0.3:
-            dn = F.DenoiseSpec(w, h, [black] * 4, 2.0, 144.0, 2.7, 4, F.DnPrecision.F16X3)
+            dn = F.CfaDenoiseSpec(w, h, [black] * 4, 2.0, 144.0, 2.7, 4, F.DnPrecision.F16X3)
         frame = F.GpuFrame.from_u16(raster, w + 8, h + 6).with_window(win).with_origin(origin)
         out.append(dict(frame=frame, info=info, spec=spec, dn=dn, raster=raster, win=win, origin=origin, w=w, h=h))
     return out
@@ -118,7 +118,7 @@ def test_submit_rejections(ctx):
     spec = F.DevelopSpec(F.Rect.full(60, 40))
     out = np.zeros((40, 60, 4), dtype=np.uint8)
     with pytest.raises(F.NotAvailable):  # the MFMA denoiser needs sides that are multiples of 8
-        srv.submit(F.GpuFrame.from_u16(mos, 60, 40), info, spec, out, F.DenoiseSpec(60, 40, [512.0] * 4))
+        srv.submit(F.GpuFrame.from_u16(mos, 60, 40), info, spec, out, F.CfaDenoiseSpec(60, 40, [512.0] * 4))
     d = torch.from_numpy(mos.view(np.int16)).to("cuda")
     with pytest.raises(F.InvalidArgument):
         srv.submit(F.GpuFrame.from_u16(d, 60, 40), info, spec, out)
