@@ -153,6 +153,7 @@ frh_context::~frh_context() {
     if (d_wtab) (void)hipFree(d_wtab);
     for (auto &kv : edge_tabs) (void)hipFree(kv.second);
     for (auto &kv : tables) (void)hipFree(kv.second);
+    if (h_stage) (void)hipHostFree(h_stage);
     if (stream) (void)hipStreamDestroy(stream);
 }
 

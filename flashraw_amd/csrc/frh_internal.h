@@ -55,6 +55,10 @@ struct frh_context {
     };
     std::vector<Block> pool;
     uint64_t pooled = 0;
+    // pinned host staging for device read-backs (the noise estimate's blocks), grown on demand
+    std::mutex stage_mu;
+    void *h_stage = nullptr;
+    uint64_t h_stage_bytes = 0;
 
     void *alloc(uint64_t bytes, hipError_t *err);
     void release(void *p, uint64_t bytes);

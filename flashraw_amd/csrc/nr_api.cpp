@@ -5,9 +5,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 #include "flashraw_hip.h"
@@ -23,66 +25,154 @@ constexpr float SELECTION_BIAS = 1.10f;
 constexpr float LEVEL_CAP = 1.5f;
 constexpr uint32_t MIN_LEVEL_SIDE = 48;
 
-// f32::total_cmp as an integer key
+// f32::total_cmp as an integer key. The map is an involution: applied to a key it gives back the
+// float's bits, so order statistics of the keys are those of the floats in total order.
 inline int32_t total_key(float f) {
     int32_t i;
     memcpy(&i, &f, 4);
     return i ^ (int32_t)(((uint32_t)(i >> 31)) >> 1);
 }
+inline float key_float(int32_t k) {
+    const int32_t i = k ^ (int32_t)(((uint32_t)(k >> 31)) >> 1);
+    float f;
+    memcpy(&f, &i, 4);
+    return f;
+}
+
+// Runs fn(0) .. fn(n - 1) on up to 16 host threads (tasks taken from a shared counter).
+template <class Fn>
+void parallel_for(size_t n, Fn fn) {
+    const size_t hw = std::max<size_t>(std::thread::hardware_concurrency(), 1);
+    const size_t nt = std::min<size_t>({hw, (size_t)16, n});
+    if (nt <= 1) {
+        for (size_t i = 0; i < n; i++) fn(i);
+        return;
+    }
+    std::atomic<size_t> next{0};
+    auto work = [&]() {
+        for (size_t i; (i = next.fetch_add(1)) < n;) fn(i);
+    };
+    std::vector<std::thread> pool;
+    for (size_t t = 1; t < nt; t++) pool.emplace_back(work);
+    work();
+    for (auto &t : pool) t.join();
+}
+
+// The finite blocks of one level in Model::from_stats's order (mod.rs:803-822): ascending luma
+// mean in total order, ties in block order (the stable sort), cut into `bins` bins of `per`.
+struct Binned {
+    const float *stats = nullptr;
+    std::vector<uint32_t> idx;
+    std::vector<float> act;  // the summed squared AC coefficients of idx[i]
+    size_t bins = 0, per = 0;
+};
+
+bool sort_blocks(const float *stats, size_t count, Binned &b) {
+    b.stats = stats;
+    // activities, eight blocks at a time (each block's sum in coefficient order)
+    std::vector<float> act(count);
+    size_t i0 = 0;
+    for (; i0 + 8 <= count; i0 += 8) {
+        float a[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        const float *s = stats + i0 * 64;
+        for (int k = 1; k < 64; k++)
+            for (int j = 0; j < 8; j++) a[j] = a[j] + s[j * 64 + k];
+        for (int j = 0; j < 8; j++) act[i0 + j] = a[j];
+    }
+    for (; i0 < count; i0++) {
+        const float *s = stats + i0 * 64;
+        float a = 0.0f;
+        for (int k = 1; k < 64; k++) a = a + s[k];
+        act[i0] = a;
+    }
+    std::vector<uint64_t> keyed;
+    keyed.reserve(count);
+    for (size_t i = 0; i < count; i++)
+        if (std::isfinite(stats[i * 64]) && std::isfinite(act[i]))
+            keyed.push_back(((uint64_t)((uint32_t)total_key(stats[i * 64]) ^ 0x80000000u) << 32) | (uint64_t)i);
+    if (keyed.size() < 32) return false;
+    std::sort(keyed.begin(), keyed.end());  // (mean key, index): the stable order by mean
+    const size_t n = keyed.size();
+    b.idx.resize(n);
+    b.act.resize(n);
+    for (size_t i = 0; i < n; i++) {
+        const uint32_t j = (uint32_t)keyed[i];
+        b.idx[i] = j;
+        b.act[i] = act[j];
+    }
+    b.bins = std::min<size_t>(std::max<size_t>(n / 96, 1), NR_BINS);
+    b.per = (n + b.bins - 1) / b.bins;
+    return true;
+}
+
+// Bin `bin` (mod.rs:823-840): the blocks whose activity is at most the bin's median activity, the
+// median of each squared coefficient over them, as a deviation; and the bin's median luma mean.
+void bin_model(const Binned &b, size_t bin, float *mean, float sig[64]) {
+    const size_t b0 = bin * b.per, len = std::min(b.per, b.idx.size() - b0);
+    std::vector<int32_t> act(len);
+    for (size_t i = 0; i < len; i++) act[i] = total_key(b.act[b0 + i]);
+    std::nth_element(act.begin(), act.begin() + len / 2, act.end());
+    const float cut = key_float(act[len / 2]);
+    std::vector<uint32_t> flat;
+    flat.reserve(len);
+    for (size_t i = 0; i < len; i++)
+        if (b.act[b0 + i] <= cut) flat.push_back(b.idx[b0 + i]);
+    const size_t nf = flat.size();
+    std::vector<int32_t> t(63 * nf);  // frequency-major keys of the flat blocks
+    for (size_t i = 0; i < nf; i++) {
+        const float *s = b.stats + (size_t)flat[i] * 64;
+        for (int k = 1; k < 64; k++) t[(size_t)(k - 1) * nf + i] = total_key(s[k]);
+    }
+    sig[0] = 0.0f;
+    for (int k = 1; k < 64; k++) {
+        int32_t *v = t.data() + (size_t)(k - 1) * nf;
+        std::nth_element(v, v + nf / 2, v + nf);
+        sig[k] = std::sqrt(key_float(v[nf / 2]) / CHI2_MEDIAN * SELECTION_BIAS);
+    }
+    *mean = b.stats[(size_t)b.idx[b0 + len / 2] * 64];
+}
+
+// The model's deviations smoothed across neighbouring bins (mod.rs:841-847).
+void smooth_bins(NrModel &md) {
+    if (md.n <= 2) return;
+    float orig[NR_BINS][64];
+    memcpy(orig, md.s, sizeof(orig));
+    for (uint32_t i = 0; i < md.n; i++) {
+        const uint32_t a = i ? i - 1 : 0, b = std::min(i + 1, md.n - 1);
+        for (int k = 0; k < 64; k++) md.s[i][k] = 0.25f * orig[a][k] + 0.5f * orig[i][k] + 0.25f * orig[b][k];
+    }
+}
+
+// Model::from_stats (mod.rs:803-848) of several levels at once (the bins of every level in
+// parallel); ok[i] false when level i has fewer than 32 finite blocks.
+void models_from_stats(const std::vector<const float *> &stats, const std::vector<size_t> &counts,
+                       std::vector<NrModel> &md, std::vector<char> &ok) {
+    const size_t nl = stats.size();
+    std::vector<Binned> binned(nl);
+    md.assign(nl, NrModel{});
+    ok.assign(nl, 0);
+    parallel_for(nl, [&](size_t i) { ok[i] = sort_blocks(stats[i], counts[i], binned[i]) ? 1 : 0; });
+    std::vector<std::pair<size_t, size_t>> tasks;
+    for (size_t i = 0; i < nl; i++)
+        if (ok[i]) {
+            md[i].n = (uint32_t)((binned[i].idx.size() + binned[i].per - 1) / binned[i].per);
+            for (size_t bin = 0; bin < md[i].n; bin++) tasks.push_back({i, bin});
+        }
+    parallel_for(tasks.size(), [&](size_t t) {
+        const size_t i = tasks[t].first, bin = tasks[t].second;
+        bin_model(binned[i], bin, &md[i].m[bin], md[i].s[bin]);
+    });
+    for (size_t i = 0; i < nl; i++)
+        if (ok[i]) smooth_bins(md[i]);
+}
 
 // Model::from_stats (mod.rs:803-848): false when fewer than 32 blocks are finite.
 bool from_stats(const float *stats, size_t count, NrModel &md) {
-    struct Ord {
-        float mean, act;
-        uint32_t idx;
-    };
-    std::vector<Ord> order;
-    order.reserve(count);
-    for (size_t i = 0; i < count; i++) {
-        const float *s = stats + i * 64;
-        float a = 0.0f;
-        for (int k = 1; k < 64; k++) a = a + s[k];
-        if (std::isfinite(s[0]) && std::isfinite(a)) order.push_back({s[0], a, (uint32_t)i});
-    }
-    if (order.size() < 32) return false;
-    std::stable_sort(order.begin(), order.end(),
-                     [](const Ord &x, const Ord &y) { return total_key(x.mean) < total_key(y.mean); });
-    const size_t n = order.size();
-    const size_t bins = std::min<size_t>(std::max<size_t>(n / 96, 1), NR_BINS);
-    const size_t per = (n + bins - 1) / bins;
-    auto by_total = [](float x, float y) { return total_key(x) < total_key(y); };
-    std::vector<float> act, vals;
-    std::vector<const float *> flat;
-    md.n = 0;
-    for (size_t b0 = 0; b0 < n; b0 += per) {
-        const size_t len = std::min(per, n - b0);
-        act.resize(len);
-        for (size_t i = 0; i < len; i++) act[i] = order[b0 + i].act;
-        std::nth_element(act.begin(), act.begin() + len / 2, act.end(), by_total);
-        const float cut = act[len / 2];
-        flat.clear();
-        for (size_t i = 0; i < len; i++)
-            if (order[b0 + i].act <= cut) flat.push_back(stats + (size_t)order[b0 + i].idx * 64);
-        float *sig = md.s[md.n];
-        sig[0] = 0.0f;
-        for (int k = 1; k < 64; k++) {
-            vals.resize(flat.size());
-            for (size_t i = 0; i < flat.size(); i++) vals[i] = flat[i][k];
-            std::nth_element(vals.begin(), vals.begin() + vals.size() / 2, vals.end(), by_total);
-            sig[k] = std::sqrt(vals[vals.size() / 2] / CHI2_MEDIAN * SELECTION_BIAS);
-        }
-        md.m[md.n] = order[b0 + len / 2].mean;
-        md.n++;
-    }
-    if (md.n > 2) {
-        float orig[NR_BINS][64];
-        memcpy(orig, md.s, sizeof(orig));
-        for (uint32_t i = 0; i < md.n; i++) {
-            const uint32_t a = i ? i - 1 : 0, b = std::min(i + 1, md.n - 1);
-            for (int k = 0; k < 64; k++) md.s[i][k] = 0.25f * orig[a][k] + 0.5f * orig[i][k] + 0.25f * orig[b][k];
-        }
-    }
-    return true;
+    std::vector<NrModel> m;
+    std::vector<char> ok;
+    models_from_stats({stats}, {count}, m, ok);
+    md = m[0];
+    return ok[0] != 0;
 }
 
 float var_at(const NrVar &v, float m) {
@@ -372,7 +462,27 @@ int32_t denoise(frh_context *ctx, const float *rgb, uint32_t w, uint32_t h, cons
             regions.push_back({pi, l, total, cols, rows, step});
             total += (uint64_t)cols * rows;
         }
-    std::vector<float> stats(total * 64);
+    // the blocks come back through the context's pinned staging buffer when it is free (another
+    // run on this context holding it: a pageable buffer of this run's own)
+    std::unique_lock<std::mutex> stage(ctx->stage_mu, std::try_to_lock);
+    std::vector<float> pageable;
+    float *stats = nullptr;
+    if (stage.owns_lock()) {
+        if (ctx->h_stage_bytes < total * 256) {
+            if (ctx->h_stage) (void)hipHostFree(ctx->h_stage);
+            ctx->h_stage = nullptr;
+            ctx->h_stage_bytes = 0;
+            if (hipHostMalloc(&ctx->h_stage, std::max<uint64_t>(total * 256, 1)) == hipSuccess)
+                ctx->h_stage_bytes = std::max<uint64_t>(total * 256, 1);
+            else
+                ctx->h_stage = nullptr;
+        }
+        stats = static_cast<float *>(ctx->h_stage);
+    }
+    if (!stats) {
+        pageable.resize(total * 64);
+        stats = pageable.data();
+    }
     if (total) {
         float *d_stats = R.ar.get(total * 64);
         if (!d_stats) return hip_fail(R.ar.err, "noise estimate buffer");
@@ -380,22 +490,36 @@ int32_t denoise(frh_context *ctx, const float *rgb, uint32_t w, uint32_t h, cons
             const Level &p = (*planes[r.pi])[r.l];
             R.check(nr_stats(p.p, ylev[r.l].p, p.w, r.step, r.cols, r.rows, R.dct, d_stats + r.first * 64, st));
         }
-        R.check(hipMemcpyAsync(stats.data(), d_stats, total * 256, hipMemcpyDeviceToHost, st));
+        R.check(hipMemcpyAsync(stats, d_stats, total * 256, hipMemcpyDeviceToHost, st));
         R.check(hipStreamSynchronize(st));
         R.ar.put(d_stats);
         if (!R.ok()) return hip_fail(R.e, "noise estimate");
     }
-    // gpu_models: each level's model from its blocks, a coarse level capped by the finer one
+    // gpu_models: each level's model from its blocks (every level at once), a coarse level capped
+    // by the finer one
+    std::vector<NrModel> rmod;
+    std::vector<char> rok;
+    {
+        std::vector<const float *> rp;
+        std::vector<size_t> rn;
+        for (const Region &r : regions) {
+            rp.push_back(stats + r.first * 64);
+            rn.push_back((size_t)r.cols * r.rows);
+        }
+        models_from_stats(rp, rn, rmod, rok);
+    }
+    if (stage.owns_lock()) stage.unlock();
     std::vector<NrModel> mods[3];
     std::vector<bool> have[3];
     for (uint32_t pi = 0; pi < 3; pi++) {
         mods[pi].resize(want[pi]);
         have[pi].assign(want[pi], false);
         for (uint32_t l = 0; l < want[pi]; l++) {
-            const Region *reg = nullptr;
-            for (const Region &r : regions)
-                if (r.pi == pi && r.l == l) reg = &r;
-            bool ok = reg && from_stats(stats.data() + reg->first * 64, (size_t)reg->cols * reg->rows, mods[pi][l]);
+            size_t ri = regions.size();
+            for (size_t i = 0; i < regions.size(); i++)
+                if (regions[i].pi == pi && regions[i].l == l) ri = i;
+            bool ok = ri < regions.size() && rok[ri];
+            if (ok) mods[pi][l] = rmod[ri];
             if (l > 0) {
                 if (ok && have[pi][l - 1])
                     cap(mods[pi][l], next_level(mods[pi][l - 1]), LEVEL_CAP);
