@@ -281,15 +281,27 @@ __global__ __launch_bounds__(256) void k_stats(const float *__restrict__ p, cons
         for (int k = 0; k < 8; k++) o[l * 8 + k] = (l | k) ? c[l][k] * c[l][k] : mean / 64.0f;
 }
 
+// The pass of a launch's list that item `i` (a tile or a gather chunk) belongs to: the last one
+// whose first item is at most i (`first` the member holding it). Wave-uniform.
+template <uint32_t NrPass::*first>
+__device__ __forceinline__ const NrPass &pass_of(const NrPass *__restrict__ list, uint32_t n, uint32_t i) {
+    uint32_t p = 0;
+    while (p + 1 < n && i >= list[p + 1].*first) p++;
+    return list[__builtin_amdgcn_readfirstlane(p)];
+}
+
 // One DCT pass, one wave per tile of 64 x 64 block positions (see the file comment).
 template <bool WIENER>
-__global__ __launch_bounds__(64) void k_pass(NrPass P) {
-    __shared__ float acc[NR_EXT * NR_EXT], wsum[NR_EXT * NR_EXT];
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_pass(const NrPass *__restrict__ list,
+                                                                                    uint32_t npass) {
+    __shared__ float acc[NR_EXT * NR_EXT];
     __shared__ float sm[NR_BINS], ss[NR_BINS * 64];
+    const NrPass &P = pass_of<&NrPass::tile0>(list, npass, blockIdx.x);
+    const uint32_t tile = blockIdx.x - P.tile0;
     const int lane = threadIdx.x;
-    const int tx = (int)(blockIdx.x % P.tiles_x), ty = (int)(blockIdx.x / P.tiles_x);
+    const int tx = (int)(tile % P.tiles_x), ty = (int)(tile / P.tiles_x);
     const int TX = tx * NR_TILE, TY = ty * NR_TILE;
-    for (int i = lane; i < NR_EXT * NR_EXT; i += 64) acc[i] = wsum[i] = 0.0f;
+    for (int i = lane; i < NR_EXT * NR_EXT; i += 64) acc[i] = 0.0f;
     const uint32_t nb = P.model->n;
     for (int i = lane; i < (int)nb * 64; i += 64) ss[i] = (&P.model->s[0][0])[i];
     if (lane < (int)nb) sm[lane] = P.model->m[lane];
@@ -327,8 +339,6 @@ __global__ __launch_bounds__(64) void k_pass(NrPass P) {
                     row[i] = reflect8(y0 + i, h) * (uint32_t)w;
                     col[i] = reflect8(x0 + i, w);
                 }
-                float c[8][8];
-                block_fwd(D, P.src, row, col, c);
                 // the guide's mean over the block: column sums, then their sum / 64
                 float gs = 0.0f;
                 {
@@ -361,11 +371,13 @@ __global__ __launch_bounds__(64) void k_pass(NrPass P) {
                     t = b > a ? (mean - a) / (b - a) : 0.0f;
                 }
                 const float *sa = ss + ia * 64, *sb = ss + ib * 64;
+                float c[8][8];
                 float weight;
                 if (WIENER) {
-                    float pc[8][8];
-                    block_fwd(D, P.pilot, row, col, pc);
-                    const float dc = c[0][0];
+                    // the shrink factors and the block's weight come from the pilot alone, so they
+                    // are ready before the source block is transformed
+                    float sh[8][8];
+                    block_fwd(D, P.pilot, row, col, sh);
                     float e[8];
 #pragma unroll
                     for (int k = 0; k < 8; k++) e[k] = 0.0f;
@@ -376,15 +388,22 @@ __global__ __launch_bounds__(64) void k_pass(NrPass P) {
                             const float a = sa[l * 8 + k], b = sb[l * 8 + k];
                             const float sg = a + (b - a) * t;
                             const float n2 = sg * sg * P.n2scale;
-                            const float p2 = pc[l][k] * pc[l][k];
-                            const float s = p2 / (p2 + n2 + 1e-30f);
-                            c[l][k] = c[l][k] * s;
-                            e[k] = e[k] + s * s * n2;
+                            const float p2 = sh[l][k] * sh[l][k];
+                            const float f = p2 / (p2 + n2 + 1e-30f);
+                            sh[l][k] = f;
+                            e[k] = e[k] + f * f * n2;
                         }
-                    c[0][0] = dc;
                     const float es = ((e[0] + e[4]) + (e[1] + e[5])) + ((e[2] + e[6]) + (e[3] + e[7]));
                     weight = 1.0f / (es + 1e-20f);
+                    block_fwd(D, P.src, row, col, c);
+                    const float dc = c[0][0];
+#pragma unroll
+                    for (int l = 0; l < 8; l++)
+#pragma unroll
+                        for (int k = 0; k < 8; k++) c[l][k] = c[l][k] * sh[l][k];
+                    c[0][0] = dc;
                 } else {
+                    block_fwd(D, P.src, row, col, c);
                     uint32_t kept = 0;
 #pragma unroll
                     for (int l = 0; l < 8; l++)
@@ -397,6 +416,12 @@ __global__ __launch_bounds__(64) void k_pass(NrPass P) {
                             c[l][k] = keep ? c[l][k] : 0.0f;
                         }
                     weight = 1.0f / (float)kept;
+                }
+                // the block's weight in the grid of block positions
+                {
+                    const uint32_t ix = cx < ncls ? (uint32_t)x0 / (uint32_t)step : P.gx - 1;
+                    const uint32_t iy = cy < ncls ? (uint32_t)y0 / (uint32_t)step : P.gy - 1;
+                    P.wgrid[(size_t)iy * P.gx + ix] = weight;
                 }
                 // weighted inverse: horizontal (per vertical frequency), then vertical (per column)
                 float tcol[8][8];  // tcol[j][k]
@@ -415,44 +440,60 @@ __global__ __launch_bounds__(64) void k_pass(NrPass P) {
                     float o[8];
                     inv1(D, tcol[j], o);
 #pragma unroll
-                    for (int i = 0; i < 8; i++) {
-                        const int q = (oy + i) * NR_EXT + ox + j;
-                        acc[q] += o[i];
-                        wsum[q] += weight;
-                    }
+                    for (int i = 0; i < 8; i++) acc[(oy + i) * NR_EXT + ox + j] += o[i];
                 }
             }
             __syncthreads();
         }
     }
-    float *pa = P.pacc + (size_t)blockIdx.x * (NR_EXT * NR_EXT);
-    float *pw = P.pws + (size_t)blockIdx.x * (NR_EXT * NR_EXT);
-    for (int i = lane; i < NR_EXT * NR_EXT; i += 64) {
-        pa[i] = acc[i];
-        pw[i] = wsum[i];
-    }
+    float *pa = P.pacc + (size_t)tile * (NR_EXT * NR_EXT);
+    for (int i = lane; i < NR_EXT * NR_EXT; i += 64) pa[i] = acc[i];
 }
 
-// Every sample: the partials of the tiles whose blocks cover it, in tile order, then the
-// weighted mean (the sample itself where no block has weight).
-__global__ __launch_bounds__(256) void k_gather(NrPass P, float *__restrict__ out) {
-    const uint32_t x = blockIdx.x * 256 + threadIdx.x, y = blockIdx.y;
+// The block positions along one padded side that cover padded coordinate c: the regular ones
+// lo..hi (hi < lo: none) and, when the last position is off the step and covers c, index `last_i`.
+struct Cover {
+    int lo, hi;
+    bool last;
+};
+__device__ __forceinline__ Cover cover(int c, int last, int step) {
+    Cover r;
+    r.lo = c >= 7 ? (c - 7 + step - 1) / step : 0;
+    r.hi = min(c / step, last / step);
+    r.last = last % step != 0 && c >= last && c <= last + 7;
+    return r;
+}
+
+// Every sample: the partials of the tiles whose blocks cover it, in tile order, over the weights
+// of the blocks that cover it, in position order (the sample itself where no block has weight).
+__global__ __launch_bounds__(256) void k_gather(const NrPass *__restrict__ list, uint32_t npass) {
+    const NrPass &P = pass_of<&NrPass::chunk0>(list, npass, blockIdx.x);
+    const uint32_t cpr = (P.w + 255) / 256, local = blockIdx.x - P.chunk0;
+    const uint32_t y = local / cpr, x = (local % cpr) * 256 + threadIdx.x;
     if (x >= P.w || y >= P.h) return;
     const int px = (int)x + 8, py = (int)y + 8;
     const int tx1 = px / NR_TILE, ty1 = py / NR_TILE;
-    float a = 0.0f, ws = 0.0f;
+    float a = 0.0f;
     for (int ty = ty1 - 1; ty <= ty1; ty++) {
         if (ty < 0 || ty >= (int)P.tiles_y || py - ty * NR_TILE >= NR_EXT) continue;
         for (int tx = tx1 - 1; tx <= tx1; tx++) {
             if (tx < 0 || tx >= (int)P.tiles_x || px - tx * NR_TILE >= NR_EXT) continue;
-            const size_t o = (size_t)(ty * (int)P.tiles_x + tx) * (NR_EXT * NR_EXT) +
-                             (size_t)((py - ty * NR_TILE) * NR_EXT + (px - tx * NR_TILE));
-            a += P.pacc[o];
-            ws += P.pws[o];
+            a += P.pacc[(size_t)(ty * (int)P.tiles_x + tx) * (NR_EXT * NR_EXT) +
+                        (size_t)((py - ty * NR_TILE) * NR_EXT + (px - tx * NR_TILE))];
         }
     }
+    const int step = (int)P.step;
+    const Cover cxr = cover(px, (int)P.xlast, step), cyr = cover(py, (int)P.ylast, step);
+    float ws = 0.0f;
+    auto row = [&](uint32_t iy) {
+        const float *g = P.wgrid + (size_t)iy * P.gx;
+        for (int ix = cxr.lo; ix <= cxr.hi; ix++) ws += g[ix];
+        if (cxr.last) ws += g[P.gx - 1];
+    };
+    for (int iy = cyr.lo; iy <= cyr.hi; iy++) row((uint32_t)iy);
+    if (cyr.last) row(P.gy - 1);
     const size_t k = (size_t)y * P.w + x;
-    out[k] = ws > 0.0f ? a / ws : P.src[k];
+    P.out[k] = ws > 0.0f ? a / ws : P.src[k];
 }
 
 // merge (denoise/mod.rs:450-464) and the develop's encoders (tone, curve, orientation)
@@ -533,13 +574,13 @@ hipError_t nr_stats(const float *p, const float *g, uint32_t w, uint32_t step, u
     return hipGetLastError();
 }
 
-hipError_t nr_pass(const NrPass &P, float *out, hipStream_t st) {
-    const uint32_t tiles = P.tiles_x * P.tiles_y;
-    if (P.pilot)
-        hipLaunchKernelGGL(k_pass<true>, dim3(tiles), dim3(64), 0, st, P);
+hipError_t nr_passes(const NrPass *d_list, uint32_t n, uint32_t tiles, uint32_t chunks, bool wiener, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    if (wiener)
+        hipLaunchKernelGGL(k_pass<true>, dim3(tiles), dim3(64), 0, st, d_list, n);
     else
-        hipLaunchKernelGGL(k_pass<false>, dim3(tiles), dim3(64), 0, st, P);
-    hipLaunchKernelGGL(k_gather, grid2(P.w, P.h), dim3(256), 0, st, P, out);
+        hipLaunchKernelGGL(k_pass<false>, dim3(tiles), dim3(64), 0, st, d_list, n);
+    hipLaunchKernelGGL(k_gather, dim3(chunks), dim3(256), 0, st, d_list, n);
     return hipGetLastError();
 }
 

@@ -9,6 +9,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <list>
 #include <thread>
 #include <vector>
 
@@ -295,6 +296,8 @@ struct Level {
 struct Run {
     frh_context *ctx;
     hipStream_t st;
+    // host copies of the pass descriptors, alive until the arena's destructor has synchronised
+    std::list<std::vector<NrPass>> keep;
     Arena ar;
     const float *dct;
     hipError_t e = hipSuccess;
@@ -319,9 +322,16 @@ struct Run {
         return o;
     }
 
-    // One DCT pass (hard thresholding without `pilot`) into a new plane.
-    float *pass(const Level &src, const float *guide, const float *pilot, const NrModel *model, float gain, float hard,
-                uint32_t step) {
+    // The passes of one launch: descriptors, and tiles / gather chunks so far.
+    struct Stage {
+        std::vector<NrPass> list;
+        uint32_t tiles = 0, chunks = 0;
+    };
+
+    // Add a DCT pass (hard thresholding without `pilot`, Wiener with it) of `src` to stage S; its
+    // result plane, or nullptr when a buffer could not be had.
+    float *add_pass(Stage &S, const Level &src, const float *guide, const float *pilot, const NrModel *model,
+                    float gain, float hard, uint32_t step) {
         NrPass P{};
         P.src = src.p;
         P.guide = guide;
@@ -333,55 +343,110 @@ struct Run {
         P.ylast = src.h + 8;
         P.tiles_x = (P.xlast + NR_TILE) / NR_TILE;
         P.tiles_y = (P.ylast + NR_TILE) / NR_TILE;
+        P.gx = P.xlast / step + 1 + (P.xlast % step ? 1 : 0);
+        P.gy = P.ylast / step + 1 + (P.ylast % step ? 1 : 0);
+        P.tile0 = S.tiles;
+        P.chunk0 = S.chunks;
         P.model = model;
         P.dct = dct;
         P.thr = hard * gain;
         P.n2scale = gain * gain;
-        const uint64_t part = (uint64_t)P.tiles_x * P.tiles_y * NR_EXT * NR_EXT;
-        P.pacc = ar.get(part);
-        P.pws = ar.get(part);
-        float *out = ar.get((uint64_t)src.w * src.h);
-        if (P.pacc && P.pws && out) check(nr_pass(P, out, st));
-        ar.put(P.pacc);
-        ar.put(P.pws);
-        return out;
+        P.pacc = ar.get((uint64_t)P.tiles_x * P.tiles_y * NR_EXT * NR_EXT);
+        P.wgrid = ar.get((uint64_t)P.gx * P.gy);
+        P.out = ar.get((uint64_t)src.w * src.h);
+        if (!P.pacc || !P.wgrid || !P.out) return nullptr;
+        S.tiles += P.tiles_x * P.tiles_y;
+        S.chunks += (src.w + 255) / 256 * src.h;
+        S.list.push_back(P);
+        return P.out;
     }
 
-    // Run::denoise_plane (denoise.rs:595-633): the levels from the coarsest, each a hard pass and
-    // a Wiener pass (or the level itself without a model), recomposed with the coarser result.
-    float *denoise_plane(const std::vector<Level> &levels, const std::vector<Level> &guides,
-                         const std::vector<const NrModel *> &models, const frh_nr_plane &ps, float hard) {
-        float *result = nullptr;
-        for (size_t li = levels.size(); li-- > 0 && ok();) {
-            const Level &src = levels[li];
-            float *d;
-            if (models[li]) {
-                const uint32_t step = li < 2 ? ps.steps[li] : 2;
-                float *pilot = pass(src, guides[li].p, nullptr, models[li], ps.gain, hard, step);
-                if (li == 0 && ps.hard_only0) {
-                    d = pilot;
-                } else {
-                    d = pass(src, guides[li].p, pilot, models[li], ps.gain, hard, step);
-                    ar.put(pilot);
-                }
-            } else {
-                d = ar.get((uint64_t)src.w * src.h);
-                if (d) check(hipMemcpyAsync(d, src.p, (size_t)src.w * src.h * 4, hipMemcpyDeviceToDevice, st));
-            }
-            if (!d) return nullptr;
-            if (result) {
-                // the detail of this level over the denoised coarser level
-                const uint32_t cw = (src.w + 1) / 2, ch = (src.h + 1) / 2;
-                float *diff = ar.get((uint64_t)cw * ch);
-                if (!diff) return nullptr;
-                check(nr_down(d, src.w, src.h, diff, result, st));
-                check(nr_add_up(d, src.w, src.h, diff, st));
-                ar.put(diff);
-                ar.put(result);
-            }
-            result = d;
+    // Launch every pass of S (descriptors copied to the device; the host copy is kept until the
+    // run ends), then hand its partial buffers back.
+    void run_stage(Stage &S, bool wiener) {
+        if (S.list.empty() || !ok()) return;
+        keep.push_back(S.list);
+        const uint64_t bytes = S.list.size() * sizeof(NrPass);
+        NrPass *d = reinterpret_cast<NrPass *>(ar.get((bytes + 3) / 4));
+        if (!d) return;
+        check(hipMemcpyAsync(d, keep.back().data(), bytes, hipMemcpyHostToDevice, st));
+        check(nr_passes(d, (uint32_t)S.list.size(), S.tiles, S.chunks, wiener, st));
+        for (const NrPass &P : S.list) {
+            ar.put(P.pacc);
+            ar.put(P.wgrid);
         }
-        return result;
+        ar.put(reinterpret_cast<float *>(d));
+    }
+
+    // One plane of Run::denoise_plane (denoise.rs:595-633): its pyramid levels, the guide's
+    // levels, a device model per level (nullptr: none) and the plane's settings.
+    struct Plane {
+        const std::vector<Level> *levels, *guides;
+        std::vector<const NrModel *> models;
+        const frh_nr_plane *ps;
+        float *result = nullptr;
+    };
+
+    // Run::denoise_plane for several planes at once: every level with a model takes a hard pass
+    // and then a Wiener pass (the full-resolution level only the hard pass when hard_only0); a
+    // level without a model stays as it is. Each pass reads only its own pyramid level, so each
+    // stage is one launch over every plane and level. Then each plane is recomposed from its
+    // coarsest level: a finer level's detail over the denoised coarser one.
+    bool denoise_planes(std::vector<Plane> &planes, float hard) {
+        const size_t np = planes.size();
+        std::vector<std::vector<float *>> pil(np), den(np);
+        Stage s1, s2;
+        for (size_t j = 0; j < np; j++) {
+            const Plane &pl = planes[j];
+            const size_t nl = pl.levels->size();
+            pil[j].assign(nl, nullptr);
+            den[j].assign(nl, nullptr);
+            for (size_t li = 0; li < nl; li++) {
+                if (!pl.models[li]) continue;
+                const uint32_t step = li < 2 ? pl.ps->steps[li] : 2;
+                pil[j][li] = add_pass(s1, (*pl.levels)[li], (*pl.guides)[li].p, nullptr, pl.models[li], pl.ps->gain,
+                                      hard, step);
+                if (!pil[j][li]) return false;
+            }
+        }
+        run_stage(s1, false);
+        for (size_t j = 0; j < np; j++) {
+            const Plane &pl = planes[j];
+            for (size_t li = 0; li < pl.levels->size(); li++) {
+                if (!pil[j][li] || (li == 0 && pl.ps->hard_only0)) continue;
+                const uint32_t step = li < 2 ? pl.ps->steps[li] : 2;
+                den[j][li] = add_pass(s2, (*pl.levels)[li], (*pl.guides)[li].p, pil[j][li], pl.models[li], pl.ps->gain,
+                                      hard, step);
+                if (!den[j][li]) return false;
+            }
+        }
+        run_stage(s2, true);
+        for (size_t j = 0; j < np && ok(); j++) {
+            const std::vector<Level> &levels = *planes[j].levels;
+            float *result = nullptr;
+            for (size_t li = levels.size(); li-- > 0 && ok();) {
+                const Level &src = levels[li];
+                float *d = den[j][li] ? den[j][li] : pil[j][li];
+                if (den[j][li]) ar.put(pil[j][li]);
+                if (!d) {
+                    d = ar.get((uint64_t)src.w * src.h);
+                    if (!d) return false;
+                    check(hipMemcpyAsync(d, src.p, (size_t)src.w * src.h * 4, hipMemcpyDeviceToDevice, st));
+                }
+                if (result) {
+                    const uint32_t cw = (src.w + 1) / 2, ch = (src.h + 1) / 2;
+                    float *diff = ar.get((uint64_t)cw * ch);
+                    if (!diff) return false;
+                    check(nr_down(d, src.w, src.h, diff, result, st));
+                    check(nr_add_up(d, src.w, src.h, diff, st));
+                    ar.put(diff);
+                    ar.put(result);
+                }
+                result = d;
+            }
+            planes[j].result = result;
+        }
+        return ok();
     }
 };
 
@@ -557,21 +622,25 @@ int32_t denoise(frh_context *ctx, const float *rgb, uint32_t w, uint32_t h, cons
         return v;
     };
     // ---- the passes ----
-    float *y2 = nullptr;
-    if (luma) {
-        const uint32_t nlv = spec->luma.levels;
-        std::vector<Level> lv(ylev.begin(), ylev.begin() + nlv);
-        y2 = R.denoise_plane(lv, lv, models_of(0, nlv), spec->luma, spec->hard);
-        if (y2 && spec->luma.keep > 0.0f) R.check(nr_blend(y2, Y.p, spec->luma.keep, n, st));
+    // every plane's passes together: luma (its own levels as guides), then the two colour planes
+    // (guided by the luma levels)
+    const uint32_t nlv = luma ? spec->luma.levels : 0u, ncl = chroma ? spec->chroma.levels : 0u;
+    std::vector<Level> lv(ylev.begin(), ylev.begin() + nlv), gl(ylev.begin(), ylev.begin() + ncl);
+    std::vector<Level> ul(ulev.begin(), ulev.begin() + ncl), vl(vlev.begin(), vlev.begin() + ncl);
+    std::vector<Run::Plane> jobs;
+    if (luma) jobs.push_back(Run::Plane{&lv, &lv, models_of(0, nlv), &spec->luma});
+    if (chroma) {
+        jobs.push_back(Run::Plane{&ul, &gl, models_of(1, ncl), &spec->chroma});
+        jobs.push_back(Run::Plane{&vl, &gl, models_of(2, ncl), &spec->chroma});
     }
+    if (!jobs.empty() && !R.denoise_planes(jobs, spec->hard))
+        return R.e != hipSuccess ? hip_fail(R.e, "noise reduction") : hip_fail(R.ar.err, "noise reduction");
+    float *y2 = luma ? jobs[0].result : nullptr;
+    if (y2 && spec->luma.keep > 0.0f) R.check(nr_blend(y2, Y.p, spec->luma.keep, n, st));
     const float *y2r = y2 ? y2 : Y.p;
     const float *u2r = U.p, *v2r = V.p;
     if (chroma && R.ok()) {
-        const uint32_t ncl = spec->chroma.levels;
-        std::vector<Level> gl(ylev.begin(), ylev.begin() + ncl);
-        std::vector<Level> ul(ulev.begin(), ulev.begin() + ncl), vl(vlev.begin(), vlev.begin() + ncl);
-        float *ud = R.denoise_plane(ul, gl, models_of(1, ncl), spec->chroma, spec->hard);
-        float *vd = R.ok() ? R.denoise_plane(vl, gl, models_of(2, ncl), spec->chroma, spec->hard) : nullptr;
+        float *ud = jobs[luma ? 1 : 0].result, *vd = jobs[luma ? 2 : 1].result;
         if (!ud || !vd || !R.ok())
             return R.e != hipSuccess ? hip_fail(R.e, "noise reduction") : hip_fail(R.ar.err, "noise reduction");
         // the guided filter (Guide, mod.rs:337-382), coefficients at half resolution
