@@ -6,7 +6,10 @@ camera->sRGB matrix, RGBA8 output. A step is one develop of that frame with its 
 resident in HBM and the output written to HBM. Prints one JSON line.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--demosaic ahd|bilinear] [--strip-rows R]
+  python bench.py --mode large   # 45 MP BGGR AHD develop into linear RGB16 + RGBA8 in one pass
   python bench.py --mode denoise [--precision f16|bf16|f16x3|f32]   # the CFA-plane DCT denoiser
+  python bench.py --mode nr [--luma L] [--chroma C]   # luma / chroma noise reduction of linear RGB
+  python bench.py --mode unpack [--bits B]   # packed 24 MP payload -> u16 samples
   python bench.py --mode batch [--frames 64]   # 64 mixed frames (12-61 MP) in one persistent launch
   python bench.py --mode serve [--rates 10,50,100,200] [--duration 15] [--serial]   # the serving loop
   python bench.py --mode copy   # a plain device copy of 1 GiB (PMC byte-counter calibration)
@@ -33,7 +36,7 @@ def _parse():
     p.add_argument("--demosaic", default="ahd", choices=["ahd", "bilinear", "edge"])
     p.add_argument("--beta", type=float, default=0.0,
                    help="edge demosaic: weight of the gated smoothing (0 for clean frames)")
-    p.add_argument("--mode", default="develop", choices=["develop", "denoise", "nr", "batch", "serve", "copy", "unpack"])
+    p.add_argument("--mode", default="develop", choices=["develop", "large", "denoise", "nr", "batch", "serve", "copy", "unpack"])
     p.add_argument("--luma", type=float, default=0.5, help="nr mode: luma noise reduction strength")
     p.add_argument("--chroma", type=float, default=0.5, help="nr mode: chroma noise reduction strength")
     p.add_argument("--bits", type=int, default=14, help="unpack mode: bits per packed sample")
@@ -51,13 +54,18 @@ def _parse():
     return p.parse_args()
 
 
-def _traffic(kernel_tag):
-    """HBM bytes per launch from the committed PMC summary, when there is one."""
+def _profiled(kernel_tag, key):
+    """A figure (`key: value`) of the committed PMC summary of a kernel, when there is one."""
     path = os.path.join(ROOT, "profiles", f"pmc_{kernel_tag}.md")
     if not os.path.exists(path):
         return None
-    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read())
+    m = re.search(key + r":\s*([0-9.eE+]+)", open(path).read())
     return float(m.group(1)) if m else None
+
+
+def _traffic(kernel_tag):
+    """HBM bytes per launch from the committed PMC summary, when there is one."""
+    return _profiled(kernel_tag, "traffic_bytes_per_launch")
 
 
 def copy_mode(args):
@@ -153,7 +161,8 @@ def denoise_mode(args):
                          "algorithmic_tflops": round(flops / (k_avg / 1e3) / 1e12, 2),
                          "peak_tflops": MFMA_PEAK_TFLOPS[args.precision],
                          "frac": round(flops / (k_avg / 1e3) / 1e12 / MFMA_PEAK_TFLOPS[args.precision], 5),
-                         "flops_per_launch": flops}},
+                         "flops_per_launch": flops,
+                         "busy_frac_rocprof": _profiled(f"denoise_{args.precision}_6048x4024", "mfma_utilisation")}},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))
@@ -557,6 +566,83 @@ def serve_mode(args):
     print(json.dumps(line))
 
 
+def large_mode(args):
+    """AHD develop of the 8256x5504 BGGR 14-bit frame with dense 2-6 px stripe patches into both
+    linear RGB16 (3 x u16 per pixel) and RGBA8 in one pass, resident in HBM. A step is one develop
+    of the frame with both outputs."""
+    import numpy as np
+    import torch
+
+    import flashraw_amd as F
+    from flashraw_amd import synth
+
+    ctx = F.context()
+    W, H = 8256, 5504
+    mos = synth.frame_45mp_stripes()
+    info = F.FrameInfo(W, H, F.CfaPattern.BGGR, F.BlackLevel.uniform(512.0), F.WhiteLevel.uniform(16383))
+    spec = F.DevelopSpec(F.Rect.full(W, H), wb=synth.WB, cam_to_rgb=synth.MATRIX, output=F.DevelopOutput.LinearRgb16,
+                         output2=F.DevelopOutput.Rgba8, edge=F.ahd_spec(info))
+    d_in = torch.from_numpy(mos.view(np.int16)).to("cuda")
+    lin = torch.empty((H, W, 3), dtype=torch.int16, device="cuda")
+    rgba = torch.empty((H, W, 4), dtype=torch.uint8, device="cuda")
+    if args.strip_rows:
+        os.environ["FLASHRAW_AHD_ROWS"] = str(args.strip_rows)
+    plan = ctx.prepare(F.GpuFrame.from_u16(d_in, W, H), info, spec)
+    stream = torch.cuda.Stream()
+    rows = args.strip_rows or plan.autotune(d_in, lin, rgba, stream=stream)
+    for _ in range(args.warmup):
+        plan.launch(d_in, lin, rgba, stream=stream)
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for s, e in evs:
+        s.record(stream)
+        plan.launch(d_in, lin, rgba, stream=stream)
+        e.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kms = [s.elapsed_time(e) for s, e in evs]
+    k_avg = statistics.mean(kms)
+    ms_per_step = wall * 1e3 / args.steps
+    mp = W * H / 1e6
+    algo_bytes = W * H * (2 + 6 + 4)  # u16 in, RGB16 and RGBA8 out
+    err = None
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+        kw = dict(black=(1, 1, 1, [512.0]), white=(16383,), wb=synth.WB, matrix=synth.MATRIX, demosaic="ahd",
+                  threads=threads)
+        times = []
+        for i in range(4):
+            t = time.perf_counter()
+            w16 = O.develop(mos, "BGGR", encoding="linear16", **kw)
+            w8 = O.develop(mos, "BGGR", encoding="rgba8", **kw)
+            if i:
+                times.append(time.perf_counter() - t)
+        d16 = np.abs(lin.cpu().numpy().view(np.uint16).astype(np.int64) - w16.astype(np.int64))
+        d8 = np.abs(rgba.cpu().numpy().astype(np.int64) - w8.astype(np.int64))
+        err = {"rgb16_max_abs": int(d16.max()), "rgb16_mean_abs": float(d16.mean()), "rgba8_max_abs": int(d8.max())}
+        cpu = {"value": round(mp / statistics.median(times), 2), "unit": "MP/s", "cores": threads, "kind": "port",
+               "sample": "the same 8256x5504 frame, both encodings (two oracle develops), median of 3 runs after "
+                         "1 warm-up"}
+    line = {
+        "metric": "MP/s RAW->sRGB develop (45 MP AHD, RGB16 + RGBA8, % HBM roofline)",
+        "value": round(mp / (ms_per_step / 1e3), 1), "unit": "MP/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_per_step, 5), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "8256x5504 BGGR 14-bit u16 with 500 stripe patches (periods 2-6 px) -> linear RGB16 "
+                               "+ RGBA8 in one pass, ahd demosaic, resident in HBM",
+                   "frame_mp": round(mp, 3), "strip_rows": rows, "error_vs_oracle": err},
+        "roofline": {"bound": "hbm", "achieved": round(algo_bytes / (k_avg / 1e3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(algo_bytes / (k_avg / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "traffic": None, "kernel_ms_avg": round(k_avg, 5), "kernel_ms_p50": round(statistics.median(kms), 5),
+                     "algorithmic_bytes_per_launch": algo_bytes},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
 def main():
     args = _parse()
     if args.gpus != 1:
@@ -566,6 +652,8 @@ def main():
         sys.exit(2)
     if args.mode == "copy":
         return copy_mode(args)
+    if args.mode == "large":
+        return large_mode(args)
     if args.mode == "denoise":
         return denoise_mode(args)
     if args.mode == "nr":
