@@ -93,6 +93,23 @@ extern "C" int32_t frh_srgb8_table(uint8_t *out, size_t len) {
     return FRH_OK;
 }
 
+extern "C" int32_t frh_check_sqrt(frh_context *ctx, uint64_t *mismatches) {
+    if (!ctx || !mismatches) return fail(FRH_ERROR_INVALID_ARGUMENT, "NULL argument");
+    (void)hipSetDevice(ctx->device);
+    hipError_t e = hipSuccess;
+    void *d = ctx->alloc(8, &e);
+    if (!d) return hip_fail(e, "check buffer");
+    unsigned long long n = 0;
+    if ((e = hipMemsetAsync(d, 0, 8, ctx->stream)) == hipSuccess &&
+        (e = check_sqrt(static_cast<unsigned long long *>(d), ctx->stream)) == hipSuccess &&
+        (e = hipMemcpyAsync(&n, d, 8, hipMemcpyDeviceToHost, ctx->stream)) == hipSuccess)
+        e = hipStreamSynchronize(ctx->stream);
+    ctx->release(d, 8);
+    if (e != hipSuccess) return hip_fail(e, "square root check");
+    *mismatches = n;
+    return FRH_OK;
+}
+
 // ------------------------------------------------------------------------------------------
 // Context (context.rs:116-150, 775-842)
 

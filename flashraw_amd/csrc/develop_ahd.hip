@@ -36,6 +36,32 @@ static void launch_ahd_rp(const DevArgs &a, const WalkPlan &P, uint32_t walk_blo
                            walk_blocks);
 }
 
+// Every float in [2^-96, 2^128), and +-0, through sqrt_nonneg against the compiler's correctly
+// rounded sqrtf; the mismatches are added to *count. 16 values per thread.
+constexpr uint32_t SQRT_LO = 0x0F800000u, SQRT_HI = 0x7F800000u;  // 2^-96, +inf
+__global__ __launch_bounds__(256) void k_check_sqrt(unsigned long long *count) {
+    const uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    uint32_t bad = 0;
+    for (int j = 0; j < 16; j++) {
+        const uint64_t b = SQRT_LO + t * 16 + (uint64_t)j;
+        if (b < SQRT_HI) {
+            const float x = __uint_as_float((uint32_t)b);
+            bad += __float_as_uint(sqrt_nonneg(x)) != __float_as_uint(sqrtf(x)) ? 1u : 0u;
+        }
+    }
+    if (t == 0) {
+        bad += __float_as_uint(sqrt_nonneg(0.0f)) != __float_as_uint(sqrtf(0.0f)) ? 1u : 0u;
+        bad += __float_as_uint(sqrt_nonneg(-0.0f)) != __float_as_uint(sqrtf(-0.0f)) ? 1u : 0u;
+    }
+    if (bad) atomicAdd(count, (unsigned long long)bad);
+}
+
+hipError_t check_sqrt(unsigned long long *count, hipStream_t stream) {
+    const uint64_t threads = ((uint64_t)(SQRT_HI - SQRT_LO) + 15) / 16;
+    hipLaunchKernelGGL(k_check_sqrt, dim3((uint32_t)((threads + 255) / 256)), dim3(256), 0, stream, count);
+    return hipGetLastError();
+}
+
 hipError_t launch_develop_ahd_walk(const DevArgs &a, hipStream_t stream, int rows_per_strip) {
     const int rp = red_position(a);
     if (rp < 0) return hipErrorInvalidValue;

@@ -491,25 +491,27 @@ __host__ inline int red_position(const DevArgs &a) {
     return -1;
 }
 
-// Correctly rounded square root of x = max(v, 0) (IEEE sqrt, the reference's f32::sqrt). The
-// hardware estimate is within one ulp for x = 0, inf and x >= 2^-96; one residual test on each
-// side picks the correctly rounded neighbour, the same correction the compiler's own lowering
-// applies. Smaller positive inputs (never produced by normalised samples, whose non-zero
-// magnitudes are far larger) take the library path for the whole wave.
+// Correctly rounded square root of x = max(v, 0) (IEEE sqrt, the reference's f32::sqrt), for x = +-0
+// and x >= 2^-96: the hardware reciprocal square root estimate y, g = x y, then one refinement
+// g + (x - g^2) y / 2 with both steps fused (the residual x - g^2 is exact in an fma). On gfx950 this
+// equals the correctly rounded square root for every float in [2^-96, 2^128), checked exhaustively
+// against the compiler's correctly rounded sqrtf (frh_check_sqrt; a GPU test repeats the check).
+// At x = +-0 the estimate of 2^-96 keeps g = x and the result x. Smaller positive inputs (never
+// produced by normalised samples, whose non-zero magnitudes are far larger) take the library path
+// for the whole wave (sqrt_max0), and so does +inf (where g would be inf * 0).
 __device__ __forceinline__ float sqrt_nonneg(float x) {
-    const float s = __builtin_amdgcn_sqrtf(x);
-    const float sd = __int_as_float(__float_as_int(s) - 1), su = __int_as_float(__float_as_int(s) + 1);
-    float r = __builtin_fmaf(-sd, s, x) <= 0.0f ? sd : s;
-    r = __builtin_fmaf(-su, s, x) > 0.0f ? su : r;
-    return r;
+    const float y = __builtin_amdgcn_rsqf(fmaxf(x, 0x1p-96f));
+    const float g = x * y;
+    const float r = __builtin_fmaf(-g, g, x);
+    return __builtin_fmaf(r, 0.5f * y, g);
 }
 __device__ __forceinline__ float sqrt_max0(float v) {
     const float x = fmaxf(v, 0.0f);
     float r = sqrt_nonneg(x);
-    const bool tiny = x > 0.0f && x < 0x1p-96f;
-    if (__builtin_amdgcn_ballot_w64(tiny)) {
+    const bool rare = (x > 0.0f && x < 0x1p-96f) || x == __builtin_inff();
+    if (__builtin_amdgcn_ballot_w64(rare)) {
         asm volatile("" ::: "memory");  // keep this rare path a branch, not speculated into every row
-        r = tiny ? sqrtf(x) : r;
+        r = rare ? sqrtf(x) : r;
     }
     return r;
 }

@@ -15,6 +15,9 @@ hipError_t launch_develop_edge(const DevArgs &a, hipStream_t stream);
 
 // The AHD row walker over rows [a.band_y0, a.band_y1), `rows_per_strip` rows per wave.
 hipError_t launch_develop_ahd_walk(const DevArgs &a, hipStream_t stream, int rows_per_strip);
+// Adds to *count the floats (all of [2^-96, 2^128) and +-0) where the kernels' square root differs from
+// the correctly rounded one.
+hipError_t check_sqrt(unsigned long long *count, hipStream_t stream);
 
 // Unpack / re-window (with the linearisation curve) the window of `in` into u16 samples at `dst`
 // (in.n * in.rows samples, 4-byte aligned); PackOrder `order` for packed payloads.

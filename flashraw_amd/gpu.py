@@ -196,6 +196,8 @@ def lib():
             L.frh_image_free.argtypes = [vp]
             L.frh_srgb8_table.argtypes = [vp, ctypes.c_size_t]
             L.frh_srgb8_table.restype = i32
+            L.frh_check_sqrt.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64)]
+            L.frh_check_sqrt.restype = i32
             L.frh_plan_create.argtypes = [vp, ctypes.POINTER(_Frame), ctypes.POINTER(_FrameInfo),
                                           ctypes.POINTER(_DevelopSpec), ctypes.POINTER(vp)]
             L.frh_plan_create.restype = i32
@@ -962,6 +964,13 @@ class GpuContext:
         buf = ctypes.create_string_buffer(256)
         check(lib().frh_context_info(self._h, buf, 256))
         return buf.value.decode()
+
+    def check_sqrt(self) -> int:
+        """How many floats in [2^-96, 2^128) (and +-0) the kernels' square root rounds differently
+        from IEEE sqrt on this device (frh_check_sqrt); 0 where the develop is exact."""
+        n = ctypes.c_uint64()
+        check(lib().frh_check_sqrt(self._h, ctypes.byref(n)))
+        return int(n.value)
 
     def override_limits(self, max_binding_bytes: int) -> None:
         """context.rs:704: cap the bytes one band binds (forces banded runs); 0 resets."""

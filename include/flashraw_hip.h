@@ -410,6 +410,14 @@ void frh_image_free(frh_image *img);
 /* The 4097-entry sRGB 8-bit table the kernels encode with (process.rs:1903-1915). */
 int32_t frh_srgb8_table(uint8_t *out, size_t len);
 
+/* ---- diagnostics ----
+ * No reference counterpart. The kernels' f32 square root (the AHD Lab and the edge-aware gradients,
+ * f32::sqrt in the reference) is the hardware reciprocal square root refined by one fma step, which
+ * is correctly rounded on gfx950 for every input the kernels give it. This runs every float in
+ * [2^-96, 2^128), and +-0, through it on the context's device and reports in *mismatches how many
+ * differ from the correctly rounded square root (0 on a device where the develop is exact). */
+int32_t frh_check_sqrt(frh_context *ctx, uint64_t *mismatches);
+
 #ifdef __cplusplus
 }
 #endif

@@ -63,6 +63,12 @@ def _compare(ctx, mos, cfa, *, demosaic="bilinear", output=F.DevelopOutput.Rgba8
     return run
 
 
+def test_square_root_is_correctly_rounded(ctx):
+    """The kernels' square root (reciprocal square root estimate and one fma refinement) equals
+    IEEE sqrt for every float in [2^-96, 2^128) and +-0 on this device: all 1.88e9 of them."""
+    assert ctx.check_sqrt() == 0
+
+
 @pytest.mark.parametrize("cfa", ["RGGB", "BGGR", "GRBG", "GBRG"])
 @pytest.mark.parametrize("origin", [(0, 0), (1, 0), (0, 1), (3, 5)])
 @pytest.mark.parametrize("demosaic", ["bilinear", "ahd"])
