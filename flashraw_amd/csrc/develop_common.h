@@ -14,6 +14,15 @@ struct Lds {
     uint32_t lut[1025];  // 4097 sRGB codes, bytes packed four per word
 };
 
+// A kernel's DevArgs read in place from the kernel argument segment, where `args`, the kernel's
+// first argument, starts at offset 0. Kernels whose code indexes the struct's arrays at run time
+// take their arguments this way: naming the by-value argument itself there makes the compiler
+// copy all of it (about 1 KB) to private memory, per lane, on every launch.
+__device__ __forceinline__ const DevArgs &kernel_args(const DevArgs &args) {
+    (void)args;
+    return *(const DevArgs *)__builtin_amdgcn_kernarg_segment_ptr();
+}
+
 __device__ __forceinline__ void load_lds(const DevArgs &a, Lds &s) {
     for (int i = threadIdx.x; i < 64; i += blockDim.x) {
         s.black[i] = a.black[i];

@@ -133,7 +133,8 @@ __device__ __forceinline__ void border_pixel_cfa(const DevArgs &a, const Lds &s,
 }
 
 template <int P>
-__global__ __launch_bounds__(256) void k_develop_edge(DevArgs a, Border B, int tiles_x, uint32_t tiles) {
+__global__ __launch_bounds__(256) void k_develop_edge(DevArgs args, Border B, int tiles_x, uint32_t tiles) {
+    const DevArgs &a = kernel_args(args);
     __shared__ Lds s;
     __shared__ EdgeLds E;
     load_lds(a, s);
@@ -409,7 +410,8 @@ struct EdgeLds2 {
 };
 
 template <int TW, int TH, int NT>
-__global__ __launch_bounds__(NT) void k_develop_edge2(DevArgs a, Border B, int tiles_x, uint32_t tiles) {
+__global__ __launch_bounds__(NT) void k_develop_edge2(DevArgs args, Border B, int tiles_x, uint32_t tiles) {
+    const DevArgs &a = kernel_args(args);
     using G = EdgeGeom<TW, TH>;
     __shared__ Lds s;
     __shared__ EdgeLds2<TW, TH> E;
