@@ -4,7 +4,7 @@ from .gpu import (Batch, BlackLevel, CfaColor, CfaPattern, CfaDenoiseSpec, Denoi
                   FrameInfo, GpuContext, GpuCurve, GpuFrame, GpuImage, GpuOp, GpuRun, GpuTimings, GpuTone, GpuUnavailable,
                   InvalidArgument, JobResult, PackOrder, LimitExceeded, NotAvailable, Orientation, OutOfMemory, Rect, Server,
                   NoiseReduction, PlaneSpec, WhiteLevel, pinned_empty,
-                  ahd_spec, available, context, denoise_spec, edge_plan, edge_spec, estimate_beta, lib, nr_model_from_stats,
+                  ahd_spec, available, context, denoise_spec, edge_plan, edge_spec, estimate_beta, lib, nr_model_fit, nr_model_from_stats,
                   srgb8_table)
 
 __all__ = [
@@ -13,5 +13,5 @@ __all__ = [
     "InvalidArgument", "JobResult", "PackOrder", "LimitExceeded", "NotAvailable", "Orientation", "OutOfMemory", "Rect", "Server",
     "NoiseReduction", "PlaneSpec", "WhiteLevel", "pinned_empty",
     "ahd_spec", "available", "context", "denoise_spec", "edge_plan", "edge_spec", "estimate_beta", "lib",
-    "nr_model_from_stats", "srgb8_table",
+    "nr_model_fit", "nr_model_from_stats", "srgb8_table",
 ]

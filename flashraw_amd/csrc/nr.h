@@ -50,6 +50,24 @@ struct NrPass {
     float *out;        // the w x h result
 };
 
+// The noise model of one level fitted on the device (Model::from_stats, denoise/mod.rs:803-848):
+// the level's measured blocks are stats[first .. first + count) (64 floats each: the luma mean,
+// then the squared coefficients). The kernels sort them by mean (rank selections on total-order
+// keys, ties in block order), cut them into bins of `per`, find each bin's median activity and
+// the median of each squared coefficient over its quieter blocks.
+struct NrRegion {
+    uint32_t first, count;
+};
+constexpr uint32_t NR_FIT_LIST = 4608;  // most blocks per bin the fit kernels hold in LDS
+struct NrFit {
+    uint32_t n;                 // finite blocks
+    uint32_t nbins, per;        // bins, blocks per bin (the last may hold fewer)
+    uint32_t start[NR_BINS];    // block (region-local) of rank b * per
+    uint32_t nflat[NR_BINS];    // blocks at most the bin's median activity
+    float mean[NR_BINS];        // the luma mean of the bin's median block
+    int32_t sig[NR_BINS][64];   // total-order keys of the median squared coefficients (entry 0 unused)
+};
+
 constexpr int NR_TILE = 64;
 constexpr int NR_EXT = NR_TILE + 8;
 
@@ -68,6 +86,10 @@ hipError_t nr_stats(const float *p, const float *g, uint32_t w, uint32_t step, u
                     const float *dct, float *out, hipStream_t st);
 // The passes d_list[0 .. n - 1] (all Wiener passes when `wiener`, else all hard-threshold passes),
 // `tiles` tiles and `chunks` gather chunks in all.
+// Fit the noise model of every region (fit[i] zeroed first) from `stats`; key, act and flat are
+// scratch of `total` entries each.
+hipError_t nr_fit(const float *stats, uint32_t total, const NrRegion *reg, uint32_t nreg, uint32_t *key, float *act,
+                  uint32_t *flat, NrFit *fit, hipStream_t st);
 hipError_t nr_passes(const NrPass *d_list, uint32_t n, uint32_t tiles, uint32_t chunks, bool wiener, hipStream_t st);
 hipError_t nr_encode(const DevArgs &a, const float *y, const float *u, const float *v, hipStream_t st);
 

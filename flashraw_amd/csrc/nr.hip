@@ -514,6 +514,206 @@ __global__ __launch_bounds__(256) void k_encode(DevArgs a, const float *__restri
 inline dim3 grid2(uint32_t w, uint32_t h) { return dim3((w + 255) / 256, h); }
 inline uint32_t grid1(uint64_t n) { return (uint32_t)((n + 255) / 256); }
 
+
+// ---- the noise model fit (Model::from_stats, denoise/mod.rs:803-848, on the device) ----
+
+// f32::total_cmp order as unsigned keys (the float's total-order key with its sign bit flipped),
+// and back.
+__device__ __forceinline__ uint32_t order_key(float f) {
+    const int32_t i = __float_as_int(f);
+    return (uint32_t)(i ^ (int32_t)(((uint32_t)(i >> 31)) >> 1)) ^ 0x80000000u;
+}
+__device__ __forceinline__ float order_float(uint32_t k) {
+    const int32_t t = (int32_t)(k ^ 0x80000000u);
+    return __int_as_float(t ^ (int32_t)(((uint32_t)(t >> 31)) >> 1));
+}
+
+constexpr int FT = 256;  // threads per block of the fit kernels
+
+// Bins of n finite blocks: at most 12, n / 96 of them, `per` blocks each (the last may hold
+// fewer, and there may be fewer bins than asked).
+__device__ __forceinline__ void fit_bins(uint32_t n, uint32_t &nb, uint32_t &per) {
+    const uint32_t bins = min(max(n / 96u, 1u), (uint32_t)NR_BINS);
+    per = (n + bins - 1) / bins;
+    nb = (n + per - 1) / per;
+}
+
+// The key of rank q (0-based; q below the number of candidates) among get(i), i in [0, n), for
+// which keep(i) holds: four passes over 8-bit digits, most significant first. *rest receives the
+// rank among the keys equal to the result. Every thread of the block calls it; `hist` is 258
+// words of LDS.
+template <class Get, class Keep>
+__device__ uint32_t select_rank(Get get, Keep keep, uint32_t n, uint32_t q, uint32_t *hist, uint32_t *rest) {
+    uint32_t prefix = 0, mask = 0;
+    for (int shift = 24; shift >= 0; shift -= 8) {
+        for (int i = threadIdx.x; i < 256; i += FT) hist[i] = 0;
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < n; i += FT) {
+            if (!keep(i)) continue;
+            const uint32_t k = get(i);
+            if ((k & mask) == prefix) atomicAdd(&hist[(k >> shift) & 255u], 1u);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t cum = 0, d = 0;
+            while (d < 255 && cum + hist[d] <= q) cum += hist[d++];
+            hist[256] = d;
+            hist[257] = q - cum;
+        }
+        __syncthreads();
+        prefix |= hist[256] << shift;
+        mask |= 255u << shift;
+        q = hist[257];
+        __syncthreads();
+    }
+    *rest = q;
+    return prefix;
+}
+
+// Per block: the activity (the squared AC coefficients summed in order), the mean's key (all ones
+// for a block whose mean or activity is not finite) and the region's count of finite blocks.
+__global__ __launch_bounds__(256) void k_fit_keys(const float *__restrict__ stats, uint32_t total,
+                                                  const NrRegion *__restrict__ reg, uint32_t nreg,
+                                                  uint32_t *__restrict__ key, float *__restrict__ act, NrFit *fit) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= total) return;
+    const float *s = stats + (size_t)i * 64;
+    float a = 0.0f;
+    for (int k = 1; k < 64; k++) a = a + s[k];
+    const bool fin = __builtin_isfinite(s[0]) && __builtin_isfinite(a);
+    key[i] = fin ? order_key(s[0]) : 0xffffffffu;
+    act[i] = a;
+    if (fin) {
+        uint32_t r = 0;
+        while (r + 1 < nreg && i >= reg[r + 1].first) r++;
+        atomicAdd(&fit[r].n, 1u);
+    }
+}
+
+// Block (r, q): the block of rank b * per (q = b) or the bin's median rank b * per + len / 2
+// (q = NR_BINS + b) in the order by mean, ties in block order.
+__global__ __launch_bounds__(FT) void k_fit_bounds(const float *__restrict__ stats, const NrRegion *__restrict__ reg,
+                                                   const uint32_t *__restrict__ key, NrFit *fit) {
+    __shared__ uint32_t hist[258], part[FT + 1];
+    const NrRegion R = reg[blockIdx.x];
+    NrFit &F = fit[blockIdx.x];
+    const uint32_t n = F.n;
+    if (n < 32) return;
+    uint32_t nb, per;
+    fit_bins(n, nb, per);
+    const uint32_t b = blockIdx.y % NR_BINS;
+    const bool med = blockIdx.y >= NR_BINS;
+    if (b >= nb) return;
+    if (blockIdx.y == 0 && threadIdx.x == 0) {
+        F.nbins = nb;
+        F.per = per;
+    }
+    const uint32_t len = min(per, n - b * per);
+    const uint32_t *kr = key + R.first;
+    uint32_t rest;
+    const uint32_t K = select_rank([&](uint32_t i) { return kr[i]; }, [](uint32_t) { return true; }, R.count,
+                                   b * per + (med ? len / 2 : 0), hist, &rest);
+    // the rest-th block, in block order, whose key is K
+    const uint32_t chunk = (R.count + FT - 1) / FT, lo = min(threadIdx.x * chunk, R.count);
+    const uint32_t hi = min(lo + chunk, R.count);
+    uint32_t c = 0;
+    for (uint32_t i = lo; i < hi; i++) c += kr[i] == K ? 1u : 0u;
+    part[threadIdx.x] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (int t = 0; t < FT; t++) {
+            const uint32_t v = part[t];
+            part[t] = run;
+            run += v;
+        }
+    }
+    __syncthreads();
+    const uint32_t before = part[threadIdx.x];
+    if (rest >= before && rest < before + c) {
+        uint32_t m = rest - before;
+        for (uint32_t i = lo; i < hi; i++) {
+            if (kr[i] != K) continue;
+            if (m == 0) {
+                if (med)
+                    F.mean[b] = stats[(size_t)(R.first + i) * 64];
+                else
+                    F.start[b] = i;
+                break;
+            }
+            m--;
+        }
+    }
+}
+
+// Bin (r, b): its blocks, their median activity, and the blocks at most that (the quieter half)
+// into flat[first + b * per ...].
+__global__ __launch_bounds__(FT) void k_fit_flat(const NrRegion *__restrict__ reg, const uint32_t *__restrict__ key,
+                                                 const float *__restrict__ act, uint32_t *__restrict__ flat,
+                                                 NrFit *fit) {
+    __shared__ uint32_t hist[258], lidx[NR_FIT_LIST], lkey[NR_FIT_LIST], cnt[2];
+    const NrRegion R = reg[blockIdx.x];
+    NrFit &F = fit[blockIdx.x];
+    const uint32_t n = F.n;
+    if (n < 32) return;
+    uint32_t nb, per;
+    fit_bins(n, nb, per);
+    const uint32_t b = blockIdx.y;
+    if (b >= nb) return;
+    const uint32_t *kr = key + R.first;
+    const float *ar = act + R.first;
+    const uint32_t i0 = b > 0 ? F.start[b] : 0u, k0 = b > 0 ? kr[i0] : 0u;
+    const uint32_t i1 = b + 1 < nb ? F.start[b + 1] : 0u, k1 = b + 1 < nb ? kr[i1] : 0u;
+    if (threadIdx.x < 2) cnt[threadIdx.x] = 0;
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j < R.count; j += FT) {
+        const uint32_t k = kr[j];
+        const bool in = k != 0xffffffffu && (b == 0 || k > k0 || (k == k0 && j >= i0)) &&
+                        (b + 1 == nb || k < k1 || (k == k1 && j < i1));
+        if (in) {
+            const uint32_t m = atomicAdd(&cnt[0], 1u);
+            if (m < NR_FIT_LIST) {
+                lidx[m] = j;
+                lkey[m] = order_key(ar[j]);
+            }
+        }
+    }
+    __syncthreads();
+    const uint32_t len = min(cnt[0], NR_FIT_LIST);
+    uint32_t rest;
+    const float cut = order_float(
+        select_rank([&](uint32_t i) { return lkey[i]; }, [](uint32_t) { return true; }, len, len / 2, hist, &rest));
+    uint32_t *out = flat + R.first + b * per;
+    for (uint32_t m = threadIdx.x; m < len; m += FT) {
+        const uint32_t j = lidx[m];
+        if (ar[j] <= cut) out[atomicAdd(&cnt[1], 1u)] = j;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) F.nflat[b] = cnt[1];
+}
+
+// (r, b, k): the median of squared coefficient k over bin b's quieter blocks.
+__global__ __launch_bounds__(FT) void k_fit_sigma(const float *__restrict__ stats, const NrRegion *__restrict__ reg,
+                                                  const uint32_t *__restrict__ flat, NrFit *fit) {
+    __shared__ uint32_t hist[258], v[NR_FIT_LIST];
+    const NrRegion R = reg[blockIdx.x];
+    NrFit &F = fit[blockIdx.x];
+    const uint32_t n = F.n;
+    if (n < 32) return;
+    uint32_t nb, per;
+    fit_bins(n, nb, per);
+    const uint32_t b = blockIdx.y, k = blockIdx.z + 1;
+    if (b >= nb) return;
+    const uint32_t nf = min(F.nflat[b], NR_FIT_LIST);
+    const uint32_t *fl = flat + R.first + b * per;
+    for (uint32_t m = threadIdx.x; m < nf; m += FT) v[m] = order_key(stats[(size_t)(R.first + fl[m]) * 64 + k]);
+    __syncthreads();
+    uint32_t rest;
+    const uint32_t sel =
+        select_rank([&](uint32_t i) { return v[i]; }, [](uint32_t) { return true; }, nf, nf / 2, hist, &rest);
+    if (threadIdx.x == 0) F.sig[b][k] = (int32_t)(sel ^ 0x80000000u);
+}
+
 }  // namespace
 
 hipError_t nr_split(const float *rgb, uint32_t w, uint32_t h, float *y, float *u, float *v, hipStream_t st) {
@@ -571,6 +771,18 @@ hipError_t nr_stats(const float *p, const float *g, uint32_t w, uint32_t step, u
     const uint64_t n = (uint64_t)cols * rows;
     if (!n) return hipSuccess;
     hipLaunchKernelGGL(k_stats, dim3(grid1(n)), dim3(256), 0, st, p, g, w, step, cols, rows, dct, out);
+    return hipGetLastError();
+}
+
+hipError_t nr_fit(const float *stats, uint32_t total, const NrRegion *reg, uint32_t nreg, uint32_t *key, float *act,
+                  uint32_t *flat, NrFit *fit, hipStream_t st) {
+    if (nreg == 0 || total == 0) return hipSuccess;
+    hipError_t e = hipMemsetAsync(fit, 0, nreg * sizeof(NrFit), st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_fit_keys, dim3((total + 255) / 256), dim3(256), 0, st, stats, total, reg, nreg, key, act, fit);
+    hipLaunchKernelGGL(k_fit_bounds, dim3(nreg, 2 * NR_BINS), dim3(FT), 0, st, stats, reg, key, fit);
+    hipLaunchKernelGGL(k_fit_flat, dim3(nreg, NR_BINS), dim3(FT), 0, st, reg, key, act, flat, fit);
+    hipLaunchKernelGGL(k_fit_sigma, dim3(nreg, NR_BINS, 63), dim3(FT), 0, st, stats, reg, flat, fit);
     return hipGetLastError();
 }
 

@@ -176,6 +176,60 @@ bool from_stats(const float *stats, size_t count, NrModel &md) {
     return ok[0] != 0;
 }
 
+// Model::from_stats of every region of the device blocks `d_stats` (64 floats each), fitted on the
+// device: the regions' models (ok[i] false below 32 finite blocks). False (nothing launched) when a
+// region holds more blocks than the fit kernels keep per bin, or on a device error (*err).
+bool device_models(frh_context *ctx, hipStream_t st, const float *d_stats, const std::vector<NrRegion> &reg,
+                   std::vector<NrModel> &md, std::vector<char> &ok, hipError_t *err) {
+    *err = hipSuccess;
+    uint64_t total = 0;
+    for (const NrRegion &r : reg) {
+        if ((uint64_t)r.count > 12ull * NR_FIT_LIST) return false;
+        total = std::max<uint64_t>(total, (uint64_t)r.first + r.count);
+    }
+    md.assign(reg.size(), NrModel{});
+    ok.assign(reg.size(), 0);
+    if (reg.empty() || total == 0) return true;
+    const uint64_t fit_bytes = reg.size() * sizeof(NrFit), reg_bytes = reg.size() * sizeof(NrRegion);
+    void *d_reg = ctx->alloc(reg_bytes, err);
+    void *d_fit = d_reg ? ctx->alloc(fit_bytes, err) : nullptr;
+    void *d_key = d_fit ? ctx->alloc(total * 4, err) : nullptr;
+    void *d_act = d_key ? ctx->alloc(total * 4, err) : nullptr;
+    void *d_flat = d_act ? ctx->alloc(total * 4, err) : nullptr;
+    std::vector<NrFit> fits(reg.size());
+    hipError_t e = d_flat ? hipSuccess : (*err != hipSuccess ? *err : hipErrorOutOfMemory);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_reg, reg.data(), reg_bytes, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+        e = nr_fit(d_stats, (uint32_t)total, static_cast<const NrRegion *>(d_reg), (uint32_t)reg.size(),
+                   static_cast<uint32_t *>(d_key), static_cast<float *>(d_act), static_cast<uint32_t *>(d_flat),
+                   static_cast<NrFit *>(d_fit), st);
+    if (e == hipSuccess) e = hipMemcpyAsync(fits.data(), d_fit, fit_bytes, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (d_flat) ctx->release(d_flat, total * 4);
+    if (d_act) ctx->release(d_act, total * 4);
+    if (d_key) ctx->release(d_key, total * 4);
+    if (d_fit) ctx->release(d_fit, fit_bytes);
+    if (d_reg) ctx->release(d_reg, reg_bytes);
+    if (e != hipSuccess) {
+        *err = e;
+        return false;
+    }
+    for (size_t i = 0; i < reg.size(); i++) {
+        const NrFit &f = fits[i];
+        if (f.n < 32) continue;
+        NrModel &m = md[i];
+        m.n = f.nbins;
+        for (uint32_t b = 0; b < f.nbins; b++) {
+            m.m[b] = f.mean[b];
+            m.s[b][0] = 0.0f;
+            for (int k = 1; k < 64; k++) m.s[b][k] = std::sqrt(key_float(f.sig[b][k]) / CHI2_MEDIAN * SELECTION_BIAS);
+        }
+        smooth_bins(m);
+        ok[i] = 1;
+    }
+    return true;
+}
+
 float var_at(const NrVar &v, float m) {
     uint32_t i = 0;
     while (i < v.n && v.m[i] < m) i++;
@@ -507,7 +561,7 @@ int32_t denoise(frh_context *ctx, const float *rgb, uint32_t w, uint32_t h, cons
         vlev.push_back(R.down(vlev.back()));
     }
     if (!R.ok()) return R.e != hipSuccess ? hip_fail(R.e, "noise reduction") : hip_fail(R.ar.err, "noise reduction");
-    // ---- the noise estimate: blocks on the device, models on the host ----
+    // ---- the noise estimate: blocks measured and models fitted on the device ----
     const uint32_t nyl = luma ? spec->luma.levels : 1u;
     const uint32_t want[3] = {(luma || chroma) ? nyl : 0u, nc, nc};
     const std::vector<Level> *planes[3] = {&ylev, &ulev, &vlev};
@@ -527,27 +581,12 @@ int32_t denoise(frh_context *ctx, const float *rgb, uint32_t w, uint32_t h, cons
             regions.push_back({pi, l, total, cols, rows, step});
             total += (uint64_t)cols * rows;
         }
-    // the blocks come back through the context's pinned staging buffer when it is free (another
-    // run on this context holding it: a pageable buffer of this run's own)
-    std::unique_lock<std::mutex> stage(ctx->stage_mu, std::try_to_lock);
-    std::vector<float> pageable;
-    float *stats = nullptr;
-    if (stage.owns_lock()) {
-        if (ctx->h_stage_bytes < total * 256) {
-            if (ctx->h_stage) (void)hipHostFree(ctx->h_stage);
-            ctx->h_stage = nullptr;
-            ctx->h_stage_bytes = 0;
-            if (hipHostMalloc(&ctx->h_stage, std::max<uint64_t>(total * 256, 1)) == hipSuccess)
-                ctx->h_stage_bytes = std::max<uint64_t>(total * 256, 1);
-            else
-                ctx->h_stage = nullptr;
-        }
-        stats = static_cast<float *>(ctx->h_stage);
-    }
-    if (!stats) {
-        pageable.resize(total * 64);
-        stats = pageable.data();
-    }
+    // gpu_models: each level's model from its blocks, a coarse level capped by the finer one. The
+    // models are fitted on the device; when a level has more blocks than the fit kernels hold per
+    // bin, the blocks come back to the host (through the context's pinned staging buffer when it is
+    // free, else a pageable buffer of this run's own) and every level is fitted there at once.
+    std::vector<NrModel> rmod;
+    std::vector<char> rok;
     if (total) {
         float *d_stats = R.ar.get(total * 64);
         if (!d_stats) return hip_fail(R.ar.err, "noise estimate buffer");
@@ -555,25 +594,45 @@ int32_t denoise(frh_context *ctx, const float *rgb, uint32_t w, uint32_t h, cons
             const Level &p = (*planes[r.pi])[r.l];
             R.check(nr_stats(p.p, ylev[r.l].p, p.w, r.step, r.cols, r.rows, R.dct, d_stats + r.first * 64, st));
         }
-        R.check(hipMemcpyAsync(stats, d_stats, total * 256, hipMemcpyDeviceToHost, st));
-        R.check(hipStreamSynchronize(st));
-        R.ar.put(d_stats);
         if (!R.ok()) return hip_fail(R.e, "noise estimate");
-    }
-    // gpu_models: each level's model from its blocks (every level at once), a coarse level capped
-    // by the finer one
-    std::vector<NrModel> rmod;
-    std::vector<char> rok;
-    {
-        std::vector<const float *> rp;
-        std::vector<size_t> rn;
-        for (const Region &r : regions) {
-            rp.push_back(stats + r.first * 64);
-            rn.push_back((size_t)r.cols * r.rows);
+        std::vector<NrRegion> nreg;
+        for (const Region &r : regions) nreg.push_back(NrRegion{(uint32_t)r.first, r.cols * r.rows});
+        hipError_t fe;
+        const bool fitted = device_models(ctx, st, d_stats, nreg, rmod, rok, &fe);
+        if (fe != hipSuccess) return hip_fail(fe, "noise model fit");
+        if (!fitted) {
+            std::unique_lock<std::mutex> stage(ctx->stage_mu, std::try_to_lock);
+            std::vector<float> pageable;
+            float *stats = nullptr;
+            if (stage.owns_lock()) {
+                if (ctx->h_stage_bytes < total * 256) {
+                    if (ctx->h_stage) (void)hipHostFree(ctx->h_stage);
+                    ctx->h_stage = nullptr;
+                    ctx->h_stage_bytes = 0;
+                    if (hipHostMalloc(&ctx->h_stage, total * 256) == hipSuccess)
+                        ctx->h_stage_bytes = total * 256;
+                    else
+                        ctx->h_stage = nullptr;
+                }
+                stats = static_cast<float *>(ctx->h_stage);
+            }
+            if (!stats) {
+                pageable.resize(total * 64);
+                stats = pageable.data();
+            }
+            R.check(hipMemcpyAsync(stats, d_stats, total * 256, hipMemcpyDeviceToHost, st));
+            R.check(hipStreamSynchronize(st));
+            if (!R.ok()) return hip_fail(R.e, "noise estimate");
+            std::vector<const float *> rp;
+            std::vector<size_t> rn;
+            for (const Region &r : regions) {
+                rp.push_back(stats + r.first * 64);
+                rn.push_back((size_t)r.cols * r.rows);
+            }
+            models_from_stats(rp, rn, rmod, rok);
         }
-        models_from_stats(rp, rn, rmod, rok);
+        R.ar.put(d_stats);
     }
-    if (stage.owns_lock()) stage.unlock();
     std::vector<NrModel> mods[3];
     std::vector<bool> have[3];
     for (uint32_t pi = 0; pi < 3; pi++) {
@@ -746,6 +805,44 @@ extern "C" int32_t frh_nr_model_from_stats(const float *stats, uint32_t count, u
         if (!stats || !bins || !means || !sigmas) return fail(FRH_ERROR_INVALID_ARGUMENT, "NULL argument");
         NrModel md{};
         if (!from_stats(stats, count, md)) return fail(FRH_ERROR_NOT_AVAILABLE, "fewer than 32 measured blocks");
+        *bins = md.n;
+        memcpy(means, md.m, md.n * sizeof(float));
+        memcpy(sigmas, md.s, (size_t)md.n * 64 * sizeof(float));
+        return FRH_OK;
+    } catch (...) {
+        return fail(FRH_ERROR_INTERNAL, "unexpected exception");
+    }
+}
+
+extern "C" int32_t frh_nr_model_fit(frh_context *ctx, const float *stats, uint32_t count, uint32_t *bins,
+                                    float *means, float *sigmas) {
+    try {
+        if (!ctx || !stats || !bins || !means || !sigmas) return fail(FRH_ERROR_INVALID_ARGUMENT, "NULL argument");
+        (void)hipSetDevice(ctx->device);
+        NrModel md{};
+        bool ok = false;
+        if (count > 0) {
+            hipError_t e = hipSuccess;
+            const uint64_t bytes = (uint64_t)count * 256;
+            void *d = ctx->alloc(bytes, &e);
+            if (!d) return hip_fail(e, "noise model blocks");
+            e = hipMemcpy(d, stats, bytes, hipMemcpyHostToDevice);
+            std::vector<NrModel> m;
+            std::vector<char> k;
+            hipError_t fe = e;
+            const bool fitted =
+                e == hipSuccess && device_models(ctx, ctx->stream, static_cast<const float *>(d), {NrRegion{0, count}},
+                                                 m, k, &fe);
+            ctx->release(d, bytes);
+            if (fe != hipSuccess) return hip_fail(fe, "noise model fit");
+            if (fitted) {
+                ok = k[0] != 0;
+                md = m[0];
+            } else {
+                ok = from_stats(stats, count, md);  // more blocks per bin than the device fit holds
+            }
+        }
+        if (!ok) return fail(FRH_ERROR_NOT_AVAILABLE, "fewer than 32 measured blocks");
         *bins = md.n;
         memcpy(means, md.m, md.n * sizeof(float));
         memcpy(sigmas, md.s, (size_t)md.n * 64 * sizeof(float));

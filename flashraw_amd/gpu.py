@@ -220,6 +220,8 @@ def lib():
             L.frh_denoise_into.restype = i32
             L.frh_nr_model_from_stats.argtypes = [vp, u32, ctypes.POINTER(u32), vp, vp]
             L.frh_nr_model_from_stats.restype = i32
+            L.frh_nr_model_fit.argtypes = [vp, vp, u32, ctypes.POINTER(u32), vp, vp]
+            L.frh_nr_model_fit.restype = i32
             L.frh_batch_create.argtypes = [vp, u32, ctypes.POINTER(_Frame), ctypes.POINTER(_FrameInfo),
                                            ctypes.POINTER(_DevelopSpec), ctypes.POINTER(vp), ctypes.POINTER(vp)]
             L.frh_batch_create.restype = i32
@@ -878,6 +880,21 @@ def denoise_spec(width: int, height: int, luma: float, chroma: float, output=Non
 
     return DenoiseSpec(plane(s.luma) if s.has_luma else None, plane(s.chroma) if s.has_chroma else None,
                        s.chroma_radius, s.hard, s.max_blocks, DevelopOutput(s.output), Orientation(s.orientation))
+
+
+def nr_model_fit(ctx: "GpuContext", stats) -> Optional[tuple]:
+    """nr_model_from_stats fitted on the context's device, as GpuContext.denoise fits its levels
+    (frh_nr_model_fit)."""
+    st = np.ascontiguousarray(stats, dtype=np.float32).reshape(-1, 64)
+    bins = ctypes.c_uint32(0)
+    means = np.zeros(12, np.float32)
+    sig = np.zeros((12, 64), np.float32)
+    rc = lib().frh_nr_model_fit(ctx._h, st.ctypes.data, st.shape[0], ctypes.byref(bins), means.ctypes.data,
+                                sig.ctypes.data)
+    if rc == 9:
+        return None
+    check(rc)
+    return means[:bins.value].copy(), sig[:bins.value].copy()
 
 
 def nr_model_from_stats(stats) -> Optional[tuple]:

@@ -345,6 +345,10 @@ int32_t frh_denoise_into(frh_context *ctx, const float *rgb, uint32_t w, uint32_
  * the luma mean, then the squared coefficients l * 8 + k) into at most 12 bins (`means`,
  * `sigmas` of 12 * 64 floats). FRH_ERROR_NOT_AVAILABLE when fewer than 32 blocks are finite. */
 int32_t frh_nr_model_from_stats(const float *stats, uint32_t count, uint32_t *bins, float *means, float *sigmas);
+/* The same model fitted on the context's device, as frh_run_denoise fits every level (the host
+ * statement above when a bin would hold more than 4608 blocks). */
+int32_t frh_nr_model_fit(frh_context *ctx, const float *stats, uint32_t count, uint32_t *bins, float *means,
+                         float *sigmas);
 
 /* ---- serving loop ----
  * No reference counterpart: a request loop over one context. Jobs (a host frame, its info and

@@ -99,6 +99,29 @@ def test_models_equal_the_oracle(ctx):
     assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1])
 
 
+@pytest.mark.parametrize("n", [20, 40, 500, 1153, 5000, 42336, 55300])
+def test_device_fit_equals_the_oracle(ctx, n):
+    """Model::from_stats fitted on the device (the rank selections GpuContext.denoise runs) equals
+    the oracle's bit for bit: below 32 blocks there is no model; 1153 blocks make 12 bins of 97 (the
+    last shorter); 55300 blocks exceed what the fit kernels hold per bin and take the host form.
+    Repeated means (ties broken by block order), zero coefficients and a few non-finite blocks are
+    mixed in."""
+    rng = np.random.default_rng(n)
+    st = (rng.standard_normal((n, 64)) ** 2).astype(np.float32) * 1e-4
+    st[:, 0] = rng.choice(np.linspace(0.05, 0.9, 37), n).astype(np.float32)  # many equal means
+    st[rng.random((n, 64)) < 0.01] = 0.0
+    if n > 100:
+        st[3, 0] = np.nan
+        st[7, 9] = np.inf
+    want = O.nr_model_from_stats(st)
+    got = F.nr_model_fit(ctx, st)
+    if want is None:
+        assert got is None
+        return
+    assert got is not None
+    assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1])
+
+
 def test_resident_form_equals_the_image_form(ctx):
     import torch
     w, h = 203, 150
