@@ -23,9 +23,11 @@ int32_t hip_fail(hipError_t e, const char *what);
 // The 4097-entry sRGB 8-bit table (process.rs:1903-1915).
 void build_srgb8(uint8_t out[4097]);
 
-// Bytes of freed device buffers a context keeps for reuse (the reference's POOL_MAX_BYTES,
-// crates/flashraw-gpu/src/context.rs:153).
-constexpr uint64_t POOL_MAX_BYTES = 1ull << 30;
+// Bytes of freed device buffers a context keeps for reuse. The reference keeps 1 GiB
+// (POOL_MAX_BYTES, crates/flashraw-gpu/src/context.rs:153); a 24 MP noise reduction alone holds
+// about 2 GB of planes and partial sums at once, and an MI355X has 288 GB, so the pool keeps 8 GiB
+// and repeated runs reuse their buffers instead of freeing and allocating them again.
+constexpr uint64_t POOL_MAX_BYTES = 8ull << 30;
 
 }  // namespace frh
 

@@ -28,13 +28,21 @@ __device__ __forceinline__ float dot4(float c0, float c1, float c2, float c3, fl
     return (c0 * x0 + c1 * x1) + (c2 * x2 + c3 * x3);
 }
 
+// Global-memory views of the planes a pass descriptor (itself read from memory) points at: named
+// as such, their loads and stores are global ones instead of flat ones, which would also count
+// against the LDS wait counter.
+typedef __attribute__((address_space(1))) float gfloat;
+__device__ __forceinline__ const gfloat *gl(const float *p) { return (const gfloat *)p; }
+__device__ __forceinline__ gfloat *gl(float *p) { return (gfloat *)p; }
+
 // The DCT matrix entries the factorisation reads (Dct::fwd1 / inv1, denoise/mod.rs:643-686).
 struct D8 {
     float m00, m20, m21, m40, m60, m61;
     float o[4][4];  // odd rows 1, 3, 5, 7, columns 0..3
 };
 
-__device__ __forceinline__ D8 load_d8(const float *__restrict__ d) {
+template <class T>
+__device__ __forceinline__ D8 load_d8(const T *__restrict__ d) {
     D8 r;
     r.m00 = d[0];
     r.m20 = d[16];
@@ -87,7 +95,8 @@ __device__ __forceinline__ void inv1(const D8 &D, const float x[8], float out[8]
 
 // The 2-D transform of a block given its rows as addresses: c[l][k] is horizontal frequency l,
 // vertical frequency k (Dct::block: the columns' vertical transforms, then the horizontal one).
-__device__ __forceinline__ void block_fwd(const D8 &D, const float *__restrict__ p, const uint32_t row[8],
+template <class T>
+__device__ __forceinline__ void block_fwd(const D8 &D, const T *__restrict__ p, const uint32_t row[8],
                                           const uint32_t col[8], float c[8][8]) {
     float v[8][8];  // v[k][j]
 #pragma unroll
@@ -302,11 +311,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
     const int tx = (int)(tile % P.tiles_x), ty = (int)(tile / P.tiles_x);
     const int TX = tx * NR_TILE, TY = ty * NR_TILE;
     for (int i = lane; i < NR_EXT * NR_EXT; i += 64) acc[i] = 0.0f;
-    const uint32_t nb = P.model->n;
-    for (int i = lane; i < (int)nb * 64; i += 64) ss[i] = (&P.model->s[0][0])[i];
-    if (lane < (int)nb) sm[lane] = P.model->m[lane];
+    const __attribute__((address_space(1))) NrModel *model = (const __attribute__((address_space(1))) NrModel *)P.model;
+    const uint32_t nb = model->n;
+    for (int i = lane; i < (int)nb * 64; i += 64) ss[i] = model->s[i / 64][i % 64];
+    if (lane < (int)nb) sm[lane] = model->m[lane];
     __syncthreads();
-    const D8 D = load_d8(P.dct);
+    const D8 D = load_d8(gl(P.dct));
     const int step = (int)P.step, ncls = 8 / step;
     const int xlast = (int)P.xlast, ylast = (int)P.ylast;
     const bool xirr = xlast % step != 0, yirr = ylast % step != 0;
@@ -348,7 +358,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
 #pragma unroll
                     for (int i = 0; i < 8; i++)
 #pragma unroll
-                        for (int j = 0; j < 8; j++) gcol[j] += P.guide[row[i] + col[j]];
+                        for (int j = 0; j < 8; j++) gcol[j] += gl(P.guide)[row[i] + col[j]];
 #pragma unroll
                     for (int j = 0; j < 8; j++) gs += gcol[j];
                 }
@@ -377,7 +387,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
                     // the shrink factors and the block's weight come from the pilot alone, so they
                     // are ready before the source block is transformed
                     float sh[8][8];
-                    block_fwd(D, P.pilot, row, col, sh);
+                    block_fwd(D, gl(P.pilot), row, col, sh);
                     float e[8];
 #pragma unroll
                     for (int k = 0; k < 8; k++) e[k] = 0.0f;
@@ -395,7 +405,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
                         }
                     const float es = ((e[0] + e[4]) + (e[1] + e[5])) + ((e[2] + e[6]) + (e[3] + e[7]));
                     weight = 1.0f / (es + 1e-20f);
-                    block_fwd(D, P.src, row, col, c);
+                    block_fwd(D, gl(P.src), row, col, c);
                     const float dc = c[0][0];
 #pragma unroll
                     for (int l = 0; l < 8; l++)
@@ -403,7 +413,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
                         for (int k = 0; k < 8; k++) c[l][k] = c[l][k] * sh[l][k];
                     c[0][0] = dc;
                 } else {
-                    block_fwd(D, P.src, row, col, c);
+                    block_fwd(D, gl(P.src), row, col, c);
                     uint32_t kept = 0;
 #pragma unroll
                     for (int l = 0; l < 8; l++)
@@ -421,7 +431,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
                 {
                     const uint32_t ix = cx < ncls ? (uint32_t)x0 / (uint32_t)step : P.gx - 1;
                     const uint32_t iy = cy < ncls ? (uint32_t)y0 / (uint32_t)step : P.gy - 1;
-                    P.wgrid[(size_t)iy * P.gx + ix] = weight;
+                    gl(P.wgrid)[(size_t)iy * P.gx + ix] = weight;
                 }
                 // weighted inverse: horizontal (per vertical frequency), then vertical (per column)
                 float tcol[8][8];  // tcol[j][k]
@@ -446,7 +456,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
             __syncthreads();
         }
     }
-    float *pa = P.pacc + (size_t)tile * (NR_EXT * NR_EXT);
+    gfloat *pa = gl(P.pacc) + (size_t)tile * (NR_EXT * NR_EXT);
     for (int i = lane; i < NR_EXT * NR_EXT; i += 64) pa[i] = acc[i];
 }
 
@@ -478,7 +488,7 @@ __global__ __launch_bounds__(256) void k_gather(const NrPass *__restrict__ list,
         if (ty < 0 || ty >= (int)P.tiles_y || py - ty * NR_TILE >= NR_EXT) continue;
         for (int tx = tx1 - 1; tx <= tx1; tx++) {
             if (tx < 0 || tx >= (int)P.tiles_x || px - tx * NR_TILE >= NR_EXT) continue;
-            a += P.pacc[(size_t)(ty * (int)P.tiles_x + tx) * (NR_EXT * NR_EXT) +
+            a += gl(P.pacc)[(size_t)(ty * (int)P.tiles_x + tx) * (NR_EXT * NR_EXT) +
                         (size_t)((py - ty * NR_TILE) * NR_EXT + (px - tx * NR_TILE))];
         }
     }
@@ -486,14 +496,14 @@ __global__ __launch_bounds__(256) void k_gather(const NrPass *__restrict__ list,
     const Cover cxr = cover(px, (int)P.xlast, step), cyr = cover(py, (int)P.ylast, step);
     float ws = 0.0f;
     auto row = [&](uint32_t iy) {
-        const float *g = P.wgrid + (size_t)iy * P.gx;
+        const gfloat *g = gl(P.wgrid) + (size_t)iy * P.gx;
         for (int ix = cxr.lo; ix <= cxr.hi; ix++) ws += g[ix];
         if (cxr.last) ws += g[P.gx - 1];
     };
     for (int iy = cyr.lo; iy <= cyr.hi; iy++) row((uint32_t)iy);
     if (cyr.last) row(P.gy - 1);
     const size_t k = (size_t)y * P.w + x;
-    P.out[k] = ws > 0.0f ? a / ws : P.src[k];
+    gl(P.out)[k] = ws > 0.0f ? a / ws : gl(P.src)[k];
 }
 
 // merge (denoise/mod.rs:450-464) and the develop's encoders (tone, curve, orientation)
