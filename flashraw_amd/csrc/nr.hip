@@ -93,6 +93,35 @@ __device__ __forceinline__ void inv1(const D8 &D, const float x[8], float out[8]
     }
 }
 
+// Four consecutive floats from a 4-byte-aligned address (one dwordx4 load).
+typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
+typedef __attribute__((address_space(1))) f4u gf4u;
+__device__ __forceinline__ f4u ld4(const gfloat *q) { return *(const gf4u *)q; }
+__device__ __forceinline__ f4u ld4(const float *q) { return *(const f4u *)q; }
+
+// Columns 4h .. 4h + 3 of a block, b[i][j] at p[row[i] + col[4h + j]]: one 16-byte load per row
+// when the block's columns are consecutive (every block off the reflected border), else one load
+// per sample.
+template <class T>
+__device__ __forceinline__ void load_half(const T *__restrict__ p, const uint32_t row[8], const uint32_t col[8], int h,
+                                          float b[8][4]) {
+    if (col[7] == col[0] + 7) {
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const f4u q = ld4(p + row[i] + col[0] + 4 * h);
+            b[i][0] = q.x;
+            b[i][1] = q.y;
+            b[i][2] = q.z;
+            b[i][3] = q.w;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 8; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) b[i][j] = p[row[i] + col[4 * h + j]];
+    }
+}
+
 // The 2-D transform of a block given its rows as addresses: c[l][k] is horizontal frequency l,
 // vertical frequency k (Dct::block: the columns' vertical transforms, then the horizontal one).
 template <class T>
@@ -100,13 +129,18 @@ __device__ __forceinline__ void block_fwd(const D8 &D, const T *__restrict__ p, 
                                           const uint32_t col[8], float c[8][8]) {
     float v[8][8];  // v[k][j]
 #pragma unroll
-    for (int j = 0; j < 8; j++) {
-        float x[8], o[8];
+    for (int h = 0; h < 2; h++) {
+        float b[8][4];
+        load_half(p, row, col, h, b);
 #pragma unroll
-        for (int i = 0; i < 8; i++) x[i] = p[row[i] + col[j]];
-        fwd1(D, x, o);
+        for (int j = 0; j < 4; j++) {
+            float x[8], o[8];
 #pragma unroll
-        for (int k = 0; k < 8; k++) v[k][j] = o[k];
+            for (int i = 0; i < 8; i++) x[i] = b[i][j];
+            fwd1(D, x, o);
+#pragma unroll
+            for (int k = 0; k < 8; k++) v[k][4 * h + j] = o[k];
+        }
     }
 #pragma unroll
     for (int k = 0; k < 8; k++) {
@@ -263,31 +297,38 @@ __global__ __launch_bounds__(256) void k_gf_apply(const float *__restrict__ ma, 
 
 // The measured blocks (estimate, denoise/mod.rs:761-788): block (c, r) of the grid of `step`
 // blocks; the guide's mean over the block (row-major sum / 64), then the squared coefficients.
-__global__ __launch_bounds__(256) void k_stats(const float *__restrict__ p, const float *__restrict__ g, uint32_t w,
-                                               uint32_t step, uint32_t cols, uint32_t rows,
-                                               const float *__restrict__ dct, float *__restrict__ out) {
-    const uint32_t idx = blockIdx.x * 256 + threadIdx.x;
-    if (idx >= cols * rows) return;
-    const D8 D = load_d8(dct);
-    const uint32_t x0 = (idx % cols) * step * 8, y0 = (idx / cols) * step * 8;
-    uint32_t row[8], col[8];
+__global__ __launch_bounds__(64) void k_stats(const float *__restrict__ p, const float *__restrict__ g, uint32_t w,
+                                              uint32_t step, uint32_t cols, uint32_t rows,
+                                              const float *__restrict__ dct, float *__restrict__ out) {
+    // one block per lane; the 64 blocks' rows go out through LDS, so each store instruction writes
+    // consecutive words (the row pitch 65 keeps the lanes' LDS writes on distinct banks)
+    __shared__ float o[64][65];
+    const uint32_t b0 = blockIdx.x * 64, idx = b0 + threadIdx.x, n = cols * rows;
+    if (idx < n) {
+        const D8 D = load_d8(dct);
+        const uint32_t x0 = (idx % cols) * step * 8, y0 = (idx / cols) * step * 8;
+        uint32_t row[8], col[8];
 #pragma unroll
-    for (int i = 0; i < 8; i++) {
-        row[i] = (y0 + i) * w;
-        col[i] = x0 + i;
+        for (int i = 0; i < 8; i++) {
+            row[i] = (y0 + i) * w;
+            col[i] = x0 + i;
+        }
+        float mean = 0.0f;
+#pragma unroll
+        for (int i = 0; i < 8; i++)
+#pragma unroll
+            for (int j = 0; j < 8; j++) mean += g[row[i] + col[j]];
+        float c[8][8];
+        block_fwd(D, p, row, col, c);
+#pragma unroll
+        for (int l = 0; l < 8; l++)
+#pragma unroll
+            for (int k = 0; k < 8; k++) o[threadIdx.x][l * 8 + k] = (l | k) ? c[l][k] * c[l][k] : mean / 64.0f;
     }
-    float mean = 0.0f;
-#pragma unroll
-    for (int i = 0; i < 8; i++)
-#pragma unroll
-        for (int j = 0; j < 8; j++) mean += g[row[i] + col[j]];
-    float c[8][8];
-    block_fwd(D, p, row, col, c);
-    float *o = out + (size_t)idx * 64;
-#pragma unroll
-    for (int l = 0; l < 8; l++)
-#pragma unroll
-        for (int k = 0; k < 8; k++) o[l * 8 + k] = (l | k) ? c[l][k] * c[l][k] : mean / 64.0f;
+    __syncthreads();
+    const uint32_t nb = min(64u, n - b0);
+    float *dst = out + (size_t)b0 * 64;
+    for (uint32_t t = threadIdx.x; t < nb * 64; t += 64) dst[t] = o[t / 64][t % 64];
 }
 
 // The pass of a launch's list that item `i` (a tile or a gather chunk) belongs to: the last one
@@ -356,9 +397,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
 #pragma unroll
                     for (int j = 0; j < 8; j++) gcol[j] = 0.0f;
 #pragma unroll
-                    for (int i = 0; i < 8; i++)
+                    for (int h = 0; h < 2; h++) {
+                        float gb[8][4];
+                        load_half(gl(P.guide), row, col, h, gb);
 #pragma unroll
-                        for (int j = 0; j < 8; j++) gcol[j] += gl(P.guide)[row[i] + col[j]];
+                        for (int i = 0; i < 8; i++)
+#pragma unroll
+                            for (int j = 0; j < 4; j++) gcol[4 * h + j] += gb[i][j];
+                    }
 #pragma unroll
                     for (int j = 0; j < 8; j++) gs += gcol[j];
                 }
@@ -392,7 +438,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
 #pragma unroll
                     for (int k = 0; k < 8; k++) e[k] = 0.0f;
 #pragma unroll
-                    for (int l = 0; l < 8; l++)
+                    for (int l = 0; l < 8; l++) {
 #pragma unroll
                         for (int k = 0; k < 8; k++) {
                             const float a = sa[l * 8 + k], b = sb[l * 8 + k];
@@ -403,6 +449,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
                             sh[l][k] = f;
                             e[k] = e[k] + f * f * n2;
                         }
+                    }
                     const float es = ((e[0] + e[4]) + (e[1] + e[5])) + ((e[2] + e[6]) + (e[3] + e[7]));
                     weight = 1.0f / (es + 1e-20f);
                     block_fwd(D, gl(P.src), row, col, c);
@@ -581,21 +628,34 @@ __device__ uint32_t select_rank(Get get, Keep keep, uint32_t n, uint32_t q, uint
 }
 
 // Per block: the activity (the squared AC coefficients summed in order), the mean's key (all ones
-// for a block whose mean or activity is not finite) and the region's count of finite blocks.
+// for a block whose mean or activity is not finite) and the region's count of finite blocks. 64
+// blocks per workgroup, read in consecutive words into LDS (row pitch 65), summed one per lane.
 __global__ __launch_bounds__(256) void k_fit_keys(const float *__restrict__ stats, uint32_t total,
                                                   const NrRegion *__restrict__ reg, uint32_t nreg,
                                                   uint32_t *__restrict__ key, float *__restrict__ act, NrFit *fit) {
-    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= total) return;
-    const float *s = stats + (size_t)i * 64;
+    __shared__ float rowsl[64][65];
+    const uint32_t b0 = blockIdx.x * 64, nb = min(64u, total - b0);
+    const float *src = stats + (size_t)b0 * 64;
+    for (uint32_t t = threadIdx.x; t < nb * 64; t += 256) rowsl[t / 64][t % 64] = src[t];
+    __syncthreads();
+    if (threadIdx.x >= nb) return;
+    const uint32_t i = b0 + threadIdx.x;
+    const float *s = rowsl[threadIdx.x];
     float a = 0.0f;
     for (int k = 1; k < 64; k++) a = a + s[k];
     const bool fin = __builtin_isfinite(s[0]) && __builtin_isfinite(a);
     key[i] = fin ? order_key(s[0]) : 0xffffffffu;
     act[i] = a;
-    if (fin) {
-        uint32_t r = 0;
-        while (r + 1 < nreg && i >= reg[r + 1].first) r++;
+    uint32_t r = 0;
+    while (r + 1 < nreg && i >= reg[r + 1].first) r++;
+    // one atomic per wave when its blocks lie in one region (the count's address is shared by
+    // the whole region, so per-block atomics would serialise)
+    const uint32_t r0 = __builtin_amdgcn_readfirstlane(r);
+    const uint64_t finite = __ballot(fin);
+    if (__ballot(r != r0) == 0) {
+        if (threadIdx.x == (uint32_t)__builtin_amdgcn_readfirstlane(threadIdx.x) && finite)
+            atomicAdd(&fit[r0].n, (uint32_t)__popcll(finite));
+    } else if (fin) {
         atomicAdd(&fit[r].n, 1u);
     }
 }
@@ -780,7 +840,7 @@ hipError_t nr_stats(const float *p, const float *g, uint32_t w, uint32_t step, u
                     const float *dct, float *out, hipStream_t st) {
     const uint64_t n = (uint64_t)cols * rows;
     if (!n) return hipSuccess;
-    hipLaunchKernelGGL(k_stats, dim3(grid1(n)), dim3(256), 0, st, p, g, w, step, cols, rows, dct, out);
+    hipLaunchKernelGGL(k_stats, dim3((uint32_t)((n + 63) / 64)), dim3(64), 0, st, p, g, w, step, cols, rows, dct, out);
     return hipGetLastError();
 }
 
@@ -789,7 +849,7 @@ hipError_t nr_fit(const float *stats, uint32_t total, const NrRegion *reg, uint3
     if (nreg == 0 || total == 0) return hipSuccess;
     hipError_t e = hipMemsetAsync(fit, 0, nreg * sizeof(NrFit), st);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_fit_keys, dim3((total + 255) / 256), dim3(256), 0, st, stats, total, reg, nreg, key, act, fit);
+    hipLaunchKernelGGL(k_fit_keys, dim3((total + 63) / 64), dim3(256), 0, st, stats, total, reg, nreg, key, act, fit);
     hipLaunchKernelGGL(k_fit_bounds, dim3(nreg, 2 * NR_BINS), dim3(FT), 0, st, stats, reg, key, fit);
     hipLaunchKernelGGL(k_fit_flat, dim3(nreg, NR_BINS), dim3(FT), 0, st, reg, key, act, flat, fit);
     hipLaunchKernelGGL(k_fit_sigma, dim3(nreg, NR_BINS, 63), dim3(FT), 0, st, stats, reg, flat, fit);
