@@ -30,9 +30,11 @@ struct NrVar {
 // 2 step, ... of each padded side, and always its last position `xlast` / `ylast`). Tiles of 64 x
 // 64 block positions accumulate their blocks' weighted samples in a 72 x 72 partial each; every
 // block's weight goes to a grid of gy x gx block positions (the last position, when it is off the
-// step, in the last column / row). The passes of one stage run in one launch: their descriptors
-// sit in device memory, and `tile0` / `chunk0` are the pass's first tile and first 256-sample
-// gather chunk in that launch.
+// step, in the last column / row). Samples whose padded coordinates share px / step and py / step
+// are covered by the same blocks, so the weights' sums are formed once per such cell (`wsum`,
+// (ylast / step + 1) x (xlast / step + 1) cells). The passes of one stage run in one launch: their
+// descriptors sit in device memory, and `tile0` / `cell0` / `chunk0` are the pass's first tile,
+// first 256-cell block and first 256-sample gather chunk in that launch.
 struct NrPass {
     const float *src, *guide, *pilot;
     uint32_t w, h;
@@ -40,13 +42,14 @@ struct NrPass {
     uint32_t xlast, ylast;
     uint32_t tiles_x, tiles_y;
     uint32_t gx, gy;
-    uint32_t tile0, chunk0;
+    uint32_t tile0, cell0, chunk0;
     const NrModel *model;
     const float *dct;  // the 8 x 8 DCT-II matrix, dct[k * 8 + i]
     float thr;         // hard threshold times the plane's gain
     float n2scale;     // gain squared
     float *pacc;       // tiles_x * tiles_y partials of NR_EXT x NR_EXT
     float *wgrid;      // gy x gx block weights
+    float *wsum;       // the weights covering each cell
     float *out;        // the w x h result
 };
 
@@ -84,13 +87,14 @@ hipError_t nr_gf_apply(const float *ma, const float *mb, const float *g, uint32_
                        hipStream_t st);
 hipError_t nr_stats(const float *p, const float *g, uint32_t w, uint32_t step, uint32_t cols, uint32_t rows,
                     const float *dct, float *out, hipStream_t st);
-// The passes d_list[0 .. n - 1] (all Wiener passes when `wiener`, else all hard-threshold passes),
-// `tiles` tiles and `chunks` gather chunks in all.
 // Fit the noise model of every region (fit[i] zeroed first) from `stats`; key, act and flat are
 // scratch of `total` entries each.
 hipError_t nr_fit(const float *stats, uint32_t total, const NrRegion *reg, uint32_t nreg, uint32_t *key, float *act,
                   uint32_t *flat, NrFit *fit, hipStream_t st);
-hipError_t nr_passes(const NrPass *d_list, uint32_t n, uint32_t tiles, uint32_t chunks, bool wiener, hipStream_t st);
+// The passes d_list[0 .. n - 1] (all Wiener passes when `wiener`, else all hard-threshold passes),
+// `tiles` tiles, `cell_blocks` blocks of 256 cells and `chunks` gather chunks in all.
+hipError_t nr_passes(const NrPass *d_list, uint32_t n, uint32_t tiles, uint32_t cell_blocks, uint32_t chunks,
+                     bool wiener, hipStream_t st);
 hipError_t nr_encode(const DevArgs &a, const float *y, const float *u, const float *v, hipStream_t st);
 
 }  // namespace frh

@@ -521,8 +521,8 @@ __device__ __forceinline__ Cover cover(int c, int last, int step) {
     return r;
 }
 
-// Every sample: the partials of the tiles whose blocks cover it, in tile order, over the weights
-// of the blocks that cover it, in position order (the sample itself where no block has weight).
+// Every sample: the partials of the tiles whose blocks cover it, in tile order, over its cell's
+// sum of the covering blocks' weights (the sample itself where no block has weight).
 __global__ __launch_bounds__(256) void k_gather(const NrPass *__restrict__ list, uint32_t npass) {
     const NrPass &P = pass_of<&NrPass::chunk0>(list, npass, blockIdx.x);
     const uint32_t cpr = (P.w + 255) / 256, local = blockIdx.x - P.chunk0;
@@ -539,8 +539,23 @@ __global__ __launch_bounds__(256) void k_gather(const NrPass *__restrict__ list,
                         (size_t)((py - ty * NR_TILE) * NR_EXT + (px - tx * NR_TILE))];
         }
     }
-    const int step = (int)P.step;
-    const Cover cxr = cover(px, (int)P.xlast, step), cyr = cover(py, (int)P.ylast, step);
+    const uint32_t step = P.step, ncx = P.xlast / step + 1;
+    const float ws = gl(P.wsum)[(size_t)((uint32_t)py / step) * ncx + (uint32_t)px / step];
+    const size_t k = (size_t)y * P.w + x;
+    gl(P.out)[k] = ws > 0.0f ? a / ws : gl(P.src)[k];
+}
+
+// The weights of the blocks covering each cell of samples (padded coordinates with the same
+// px / step, py / step), summed in position order: rows of block positions from the top, each row
+// from the left. The last position, off the step, covers only padding beyond the plane (it
+// starts at w + 8, past the last sample's w + 7), so it adds to no cell a sample reads.
+__global__ __launch_bounds__(256) void k_wsum(const NrPass *__restrict__ list, uint32_t npass) {
+    const NrPass &P = pass_of<&NrPass::cell0>(list, npass, blockIdx.x);
+    const uint32_t step = P.step, ncx = P.xlast / step + 1, ncy = P.ylast / step + 1;
+    const uint32_t c = (blockIdx.x - P.cell0) * 256 + threadIdx.x;
+    if (c >= ncx * ncy) return;
+    const Cover cxr = cover((int)((c % ncx) * step), (int)P.xlast, (int)step);
+    const Cover cyr = cover((int)((c / ncx) * step), (int)P.ylast, (int)step);
     float ws = 0.0f;
     auto row = [&](uint32_t iy) {
         const gfloat *g = gl(P.wgrid) + (size_t)iy * P.gx;
@@ -549,8 +564,7 @@ __global__ __launch_bounds__(256) void k_gather(const NrPass *__restrict__ list,
     };
     for (int iy = cyr.lo; iy <= cyr.hi; iy++) row((uint32_t)iy);
     if (cyr.last) row(P.gy - 1);
-    const size_t k = (size_t)y * P.w + x;
-    gl(P.out)[k] = ws > 0.0f ? a / ws : gl(P.src)[k];
+    gl(P.wsum)[c] = ws;
 }
 
 // merge (denoise/mod.rs:450-464) and the develop's encoders (tone, curve, orientation)
@@ -856,12 +870,14 @@ hipError_t nr_fit(const float *stats, uint32_t total, const NrRegion *reg, uint3
     return hipGetLastError();
 }
 
-hipError_t nr_passes(const NrPass *d_list, uint32_t n, uint32_t tiles, uint32_t chunks, bool wiener, hipStream_t st) {
+hipError_t nr_passes(const NrPass *d_list, uint32_t n, uint32_t tiles, uint32_t cell_blocks, uint32_t chunks,
+                     bool wiener, hipStream_t st) {
     if (n == 0) return hipSuccess;
     if (wiener)
         hipLaunchKernelGGL(k_pass<true>, dim3(tiles), dim3(64), 0, st, d_list, n);
     else
         hipLaunchKernelGGL(k_pass<false>, dim3(tiles), dim3(64), 0, st, d_list, n);
+    hipLaunchKernelGGL(k_wsum, dim3(cell_blocks), dim3(256), 0, st, d_list, n);
     hipLaunchKernelGGL(k_gather, dim3(chunks), dim3(256), 0, st, d_list, n);
     return hipGetLastError();
 }

@@ -379,7 +379,7 @@ struct Run {
     // The passes of one launch: descriptors, and tiles / gather chunks so far.
     struct Stage {
         std::vector<NrPass> list;
-        uint32_t tiles = 0, chunks = 0;
+        uint32_t tiles = 0, cell_blocks = 0, chunks = 0;
     };
 
     // Add a DCT pass (hard thresholding without `pilot`, Wiener with it) of `src` to stage S; its
@@ -400,6 +400,7 @@ struct Run {
         P.gx = P.xlast / step + 1 + (P.xlast % step ? 1 : 0);
         P.gy = P.ylast / step + 1 + (P.ylast % step ? 1 : 0);
         P.tile0 = S.tiles;
+        P.cell0 = S.cell_blocks;
         P.chunk0 = S.chunks;
         P.model = model;
         P.dct = dct;
@@ -407,9 +408,12 @@ struct Run {
         P.n2scale = gain * gain;
         P.pacc = ar.get((uint64_t)P.tiles_x * P.tiles_y * NR_EXT * NR_EXT);
         P.wgrid = ar.get((uint64_t)P.gx * P.gy);
+        const uint64_t cells = (uint64_t)(P.xlast / step + 1) * (P.ylast / step + 1);
+        P.wsum = ar.get(cells);
         P.out = ar.get((uint64_t)src.w * src.h);
-        if (!P.pacc || !P.wgrid || !P.out) return nullptr;
+        if (!P.pacc || !P.wgrid || !P.wsum || !P.out) return nullptr;
         S.tiles += P.tiles_x * P.tiles_y;
+        S.cell_blocks += (uint32_t)((cells + 255) / 256);
         S.chunks += (src.w + 255) / 256 * src.h;
         S.list.push_back(P);
         return P.out;
@@ -424,10 +428,11 @@ struct Run {
         NrPass *d = reinterpret_cast<NrPass *>(ar.get((bytes + 3) / 4));
         if (!d) return;
         check(hipMemcpyAsync(d, keep.back().data(), bytes, hipMemcpyHostToDevice, st));
-        check(nr_passes(d, (uint32_t)S.list.size(), S.tiles, S.chunks, wiener, st));
+        check(nr_passes(d, (uint32_t)S.list.size(), S.tiles, S.cell_blocks, S.chunks, wiener, st));
         for (const NrPass &P : S.list) {
             ar.put(P.pacc);
             ar.put(P.wgrid);
+            ar.put(P.wsum);
         }
         ar.put(reinterpret_cast<float *>(d));
     }
