@@ -124,10 +124,10 @@ __device__ __forceinline__ void load_half(const T *__restrict__ p, const uint32_
 
 // The 2-D transform of a block given its rows as addresses: c[l][k] is horizontal frequency l,
 // vertical frequency k (Dct::block: the columns' vertical transforms, then the horizontal one).
+// The vertical stage of a block's transform: v[k][j], vertical frequency k of column j.
 template <class T>
-__device__ __forceinline__ void block_fwd(const D8 &D, const T *__restrict__ p, const uint32_t row[8],
-                                          const uint32_t col[8], float c[8][8]) {
-    float v[8][8];  // v[k][j]
+__device__ __forceinline__ void block_vert(const D8 &D, const T *__restrict__ p, const uint32_t row[8],
+                                           const uint32_t col[8], float v[8][8]) {
 #pragma unroll
     for (int h = 0; h < 2; h++) {
         float b[8][4];
@@ -142,6 +142,13 @@ __device__ __forceinline__ void block_fwd(const D8 &D, const T *__restrict__ p, 
             for (int k = 0; k < 8; k++) v[k][4 * h + j] = o[k];
         }
     }
+}
+
+template <class T>
+__device__ __forceinline__ void block_fwd(const D8 &D, const T *__restrict__ p, const uint32_t row[8],
+                                          const uint32_t col[8], float c[8][8]) {
+    float v[8][8];  // v[k][j]
+    block_vert(D, p, row, col, v);
 #pragma unroll
     for (int k = 0; k < 8; k++) {
         float o[8];
@@ -430,35 +437,33 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
                 float c[8][8];
                 float weight;
                 if (WIENER) {
-                    // the shrink factors and the block's weight come from the pilot alone, so they
-                    // are ready before the source block is transformed
-                    float sh[8][8];
-                    block_fwd(D, gl(P.pilot), row, col, sh);
+                    // the source block's coefficients first; then the pilot's, one vertical
+                    // frequency k at a time (its horizontal transform), each shrinking the source
+                    // coefficients of that k as it appears, so at most 64 + 64 values are live
+                    block_fwd(D, gl(P.src), row, col, c);
+                    const float dc = c[0][0];
+                    float vp[8][8];
+                    block_vert(D, gl(P.pilot), row, col, vp);
                     float e[8];
 #pragma unroll
-                    for (int k = 0; k < 8; k++) e[k] = 0.0f;
+                    for (int k = 0; k < 8; k++) {
+                        float pc[8];
+                        fwd1(D, vp[k], pc);  // pc[l]: the pilot's coefficient (l, k)
+                        e[k] = 0.0f;
 #pragma unroll
-                    for (int l = 0; l < 8; l++) {
-#pragma unroll
-                        for (int k = 0; k < 8; k++) {
+                        for (int l = 0; l < 8; l++) {
                             const float a = sa[l * 8 + k], b = sb[l * 8 + k];
                             const float sg = a + (b - a) * t;
                             const float n2 = sg * sg * P.n2scale;
-                            const float p2 = sh[l][k] * sh[l][k];
+                            const float p2 = pc[l] * pc[l];
                             const float f = p2 / (p2 + n2 + 1e-30f);
-                            sh[l][k] = f;
+                            c[l][k] = c[l][k] * f;
                             e[k] = e[k] + f * f * n2;
                         }
                     }
+                    c[0][0] = dc;
                     const float es = ((e[0] + e[4]) + (e[1] + e[5])) + ((e[2] + e[6]) + (e[3] + e[7]));
                     weight = 1.0f / (es + 1e-20f);
-                    block_fwd(D, gl(P.src), row, col, c);
-                    const float dc = c[0][0];
-#pragma unroll
-                    for (int l = 0; l < 8; l++)
-#pragma unroll
-                        for (int k = 0; k < 8; k++) c[l][k] = c[l][k] * sh[l][k];
-                    c[0][0] = dc;
                 } else {
                     block_fwd(D, gl(P.src), row, col, c);
                     uint32_t kept = 0;
