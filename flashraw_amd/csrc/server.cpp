@@ -71,7 +71,11 @@ struct InFlight {
 struct frh_server {
     frh_context *ctx = nullptr;
     frh_server_config cfg{};
-    static constexpr int NCOPY = 2;  // copy streams per direction (the DMA engines run them concurrently)
+    // One copy stream per direction: with the context's stream and the compute stream that makes
+    // four streams, one per hardware queue (a process gets four), so uploads, downloads and
+    // batches never wait in one another's queue; the host link then carries both directions at
+    // once (97 GB/s measured together against 57 GB/s either way alone).
+    static constexpr int NCOPY = 1;
     hipStream_t up[NCOPY] = {}, down[NCOPY] = {}, comp = nullptr;
     uint32_t next_up = 0, next_down = 0;
     std::thread worker;
