@@ -10,6 +10,7 @@ resident in HBM and the output written to HBM. Prints one JSON line.
   python bench.py --mode denoise [--precision f16|bf16|f16x3|f32]   # the CFA-plane DCT denoiser
   python bench.py --mode nr [--luma L] [--chroma C]   # luma / chroma noise reduction of linear RGB
   python bench.py --mode unpack [--bits B]   # packed 24 MP payload -> u16 samples
+  python bench.py --mode link   # host-link bandwidth, each direction alone and both at once
   python bench.py --mode batch [--frames 64]   # 64 mixed frames (12-61 MP) in one persistent launch
   python bench.py --mode serve [--rates 10,50,100,200] [--duration 15] [--serial]   # the serving loop
   python bench.py --mode copy   # a plain device copy of 1 GiB (PMC byte-counter calibration)
@@ -36,7 +37,7 @@ def _parse():
     p.add_argument("--demosaic", default="ahd", choices=["ahd", "bilinear", "edge"])
     p.add_argument("--beta", type=float, default=0.0,
                    help="edge demosaic: weight of the gated smoothing (0 for clean frames)")
-    p.add_argument("--mode", default="develop", choices=["develop", "large", "denoise", "nr", "batch", "serve", "copy", "unpack"])
+    p.add_argument("--mode", default="develop", choices=["develop", "large", "denoise", "nr", "batch", "serve", "copy", "unpack", "link"])
     p.add_argument("--luma", type=float, default=0.5, help="nr mode: luma noise reduction strength")
     p.add_argument("--chroma", type=float, default=0.5, help="nr mode: chroma noise reduction strength")
     p.add_argument("--bits", type=int, default=14, help="unpack mode: bits per packed sample")
@@ -572,6 +573,39 @@ def serve_mode(args):
     print(json.dumps(line))
 
 
+def link_mode(args):
+    """Host-link bandwidth: 1 GiB pinned host <-> device copies, each direction alone and both at
+    once on two streams (the bound of the serving loop, which moves 6 B per pixel)."""
+    import torch
+
+    n = 1 << 30
+    h_up = torch.empty(n, dtype=torch.uint8).pin_memory()
+    h_dn = torch.empty(n, dtype=torch.uint8).pin_memory()
+    d_up = torch.empty(n, dtype=torch.uint8, device="cuda")
+    d_dn = torch.ones(n, dtype=torch.uint8, device="cuda")
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+
+    def run(up, down, reps=3):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            if up:
+                with torch.cuda.stream(s1):
+                    d_up.copy_(h_up, non_blocking=True)
+            if down:
+                with torch.cuda.stream(s2):
+                    h_dn.copy_(d_dn, non_blocking=True)
+        torch.cuda.synchronize()
+        return reps * n * (int(up) + int(down)) / (time.perf_counter() - t) / 1e9
+
+    run(True, True, 1)
+    up, down, both = run(True, False), run(False, True), run(True, True)
+    print(json.dumps({"metric": "host link GB/s (pinned 1 GiB copies)", "value": round(both, 1), "unit": "GB/s",
+                      "n_gpus": 1, "higher_is_better": True, "data": "synthetic",
+                      "config": {"h2d_alone_gbs": round(up, 1), "d2h_alone_gbs": round(down, 1),
+                                 "both_directions_gbs": round(both, 1)}}))
+
+
 def large_mode(args):
     """AHD develop of the 8256x5504 BGGR 14-bit frame with dense 2-6 px stripe patches into both
     linear RGB16 (3 x u16 per pixel) and RGBA8 in one pass, resident in HBM. A step is one develop
@@ -660,6 +694,8 @@ def main():
         return copy_mode(args)
     if args.mode == "large":
         return large_mode(args)
+    if args.mode == "link":
+        return link_mode(args)
     if args.mode == "denoise":
         return denoise_mode(args)
     if args.mode == "nr":
