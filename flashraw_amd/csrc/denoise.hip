@@ -451,7 +451,7 @@ extern "C" int32_t frh_denoise_cfa(frh_context *ctx, const frh_dn_spec *spec, co
             a.nt = (pw + 7) / 8;
             a.nr = (ph + 7) / 8;
             a.strips = (a.nt + DN_NB - 1) / DN_NB;
-            a.seg_rows = spec->seg_rows ? spec->seg_rows : 16;
+            a.seg_rows = spec->seg_rows ? spec->seg_rows : 12;
             a.segs = (a.nr + a.seg_rows - 1) / a.seg_rows;
             for (int i = 0; i < 4; i++) a.black[i] = spec->black[i];
             a.sig_a = spec->sig_a;

@@ -293,7 +293,7 @@ typedef struct frh_dn_spec {
     float sig_a, sig_b, threshold;  /* threshold: 2.7 in the reference's hard pass (mod.rs:47) */
     uint32_t step;
     int32_t precision;
-    uint32_t seg_rows; /* MFMA walker: 8-row tile rows per wave (0: 16) */
+    uint32_t seg_rows; /* MFMA walker: 8-row tile rows per wave (0: 12) */
 } frh_dn_spec;
 /* Denoise the device u16 mosaic `src` into `dst` (u16 mosaic, black added back, rounded and
  * clamped) and/or `dst_f32` (f32 mosaic in DN above black); either may be NULL, not both. The
