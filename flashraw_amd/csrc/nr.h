@@ -34,7 +34,7 @@ struct NrVar {
 // are covered by the same blocks, so the weights' sums are formed once per such cell (`wsum`,
 // (ylast / step + 1) x (xlast / step + 1) cells). The passes of one stage run in one launch: their
 // descriptors sit in device memory, and `tile0` / `cell0` / `chunk0` are the pass's first tile,
-// first 256-cell block and first 256-sample gather chunk in that launch.
+// first 256-cell block and first 1024-sample gather chunk in that launch.
 struct NrPass {
     const float *src, *guide, *pilot;
     uint32_t w, h;

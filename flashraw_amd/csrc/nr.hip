@@ -527,27 +527,34 @@ __device__ __forceinline__ Cover cover(int c, int last, int step) {
 }
 
 // Every sample: the partials of the tiles whose blocks cover it, in tile order, over its cell's
-// sum of the covering blocks' weights (the sample itself where no block has weight).
+// sum of the covering blocks' weights (the sample itself where no block has weight). Four
+// consecutive samples of a row per thread (a chunk is 1024 samples of one row).
 __global__ __launch_bounds__(256) void k_gather(const NrPass *__restrict__ list, uint32_t npass) {
     const NrPass &P = pass_of<&NrPass::chunk0>(list, npass, blockIdx.x);
-    const uint32_t cpr = (P.w + 255) / 256, local = blockIdx.x - P.chunk0;
-    const uint32_t y = local / cpr, x = (local % cpr) * 256 + threadIdx.x;
-    if (x >= P.w || y >= P.h) return;
-    const int px = (int)x + 8, py = (int)y + 8;
-    const int tx1 = px / NR_TILE, ty1 = py / NR_TILE;
-    float a = 0.0f;
-    for (int ty = ty1 - 1; ty <= ty1; ty++) {
-        if (ty < 0 || ty >= (int)P.tiles_y || py - ty * NR_TILE >= NR_EXT) continue;
-        for (int tx = tx1 - 1; tx <= tx1; tx++) {
-            if (tx < 0 || tx >= (int)P.tiles_x || px - tx * NR_TILE >= NR_EXT) continue;
-            a += gl(P.pacc)[(size_t)(ty * (int)P.tiles_x + tx) * (NR_EXT * NR_EXT) +
-                        (size_t)((py - ty * NR_TILE) * NR_EXT + (px - tx * NR_TILE))];
-        }
-    }
+    const uint32_t cpr = (P.w + 1023) / 1024, local = blockIdx.x - P.chunk0;
+    const uint32_t y = local / cpr, xb = (local % cpr) * 1024 + threadIdx.x * 4;
+    if (y >= P.h) return;
+    const int py = (int)y + 8, ty1 = py / NR_TILE;
     const uint32_t step = P.step, ncx = P.xlast / step + 1;
-    const float ws = gl(P.wsum)[(size_t)((uint32_t)py / step) * ncx + (uint32_t)px / step];
-    const size_t k = (size_t)y * P.w + x;
-    gl(P.out)[k] = ws > 0.0f ? a / ws : gl(P.src)[k];
+    const gfloat *wrow = gl(P.wsum) + (size_t)((uint32_t)py / step) * ncx;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const uint32_t x = xb + (uint32_t)q;
+        if (x >= P.w) break;
+        const int px = (int)x + 8, tx1 = px / NR_TILE;
+        float a = 0.0f;
+        for (int ty = ty1 - 1; ty <= ty1; ty++) {
+            if (ty < 0 || ty >= (int)P.tiles_y || py - ty * NR_TILE >= NR_EXT) continue;
+            for (int tx = tx1 - 1; tx <= tx1; tx++) {
+                if (tx < 0 || tx >= (int)P.tiles_x || px - tx * NR_TILE >= NR_EXT) continue;
+                a += gl(P.pacc)[(size_t)(ty * (int)P.tiles_x + tx) * (NR_EXT * NR_EXT) +
+                                (size_t)((py - ty * NR_TILE) * NR_EXT + (px - tx * NR_TILE))];
+            }
+        }
+        const float ws = wrow[(uint32_t)px / step];
+        const size_t k = (size_t)y * P.w + x;
+        gl(P.out)[k] = ws > 0.0f ? a / ws : gl(P.src)[k];
+    }
 }
 
 // The weights of the blocks covering each cell of samples (padded coordinates with the same

@@ -414,7 +414,7 @@ struct Run {
         if (!P.pacc || !P.wgrid || !P.wsum || !P.out) return nullptr;
         S.tiles += P.tiles_x * P.tiles_y;
         S.cell_blocks += (uint32_t)((cells + 255) / 256);
-        S.chunks += (src.w + 255) / 256 * src.h;
+        S.chunks += (src.w + 1023) / 1024 * src.h;
         S.list.push_back(P);
         return P.out;
     }
