@@ -240,25 +240,6 @@ def nr_mode(args):
     print(json.dumps(line))
 
 
-def batch_frames(n, seed=3):
-    """The batched workload: n frames with sizes 12, 20, 24, 33, 45 or 61 MP at 3:2 or 4:3, a
-    uniformly drawn Bayer phase, 12 or 14 bits, WB R and B in [1.2, 2.8] and the colour matrix
-    jittered by up to 10 %, AHD to RGBA8 (sides rounded to even)."""
-    import numpy as np
-    rng = np.random.default_rng(seed)
-    out = []
-    for k in range(n):
-        mp = float(rng.choice([12, 20, 24, 33, 45, 61]))
-        aspect = 1.5 if rng.random() < 0.5 else 4.0 / 3.0
-        h = int(round((mp * 1e6 / aspect) ** 0.5 / 2)) * 2
-        w = int(round(h * aspect / 2)) * 2
-        out.append(dict(w=w, h=h, cfa=["RGGB", "BGGR", "GRBG", "GBRG"][int(rng.integers(0, 4))],
-                        bits=12 if rng.random() < 0.5 else 14, wb=(float(rng.uniform(1.2, 2.8)), 1.0,
-                                                                      float(rng.uniform(1.2, 2.8)), 1.0),
-                        jitter=rng.uniform(0.9, 1.1, (3, 3)), seed=1000 + k))
-    return out
-
-
 def batch_mode(args):
     """Batched AHD develop of mixed frames: one persistent launch per Bayer phase, captured in a
     HIP graph, against the same frames launched back to back one by one. A step is one develop
@@ -270,7 +251,7 @@ def batch_mode(args):
     from flashraw_amd import synth
 
     ctx = F.context()
-    specs = batch_frames(args.frames)
+    specs = synth.batch_frames(args.frames)
     jobs, plans, total_px = [], [], 0
     for f in specs:
         white = (1 << f["bits"]) - 1
@@ -335,9 +316,6 @@ def batch_mode(args):
     print(json.dumps(line))
 
 
-N_SERVE_MOSAICS = 32
-
-
 def unpack_mode(args):
     """GpuOp::Unpack of the 6048x4024 frame as a resident MSB-first packed payload of --bits bits
     per sample (tight rows) into u16 samples. A step is one unpack of the frame; algorithmic bytes
@@ -392,40 +370,6 @@ def unpack_mode(args):
         "cpu_baseline": None}))
 
 
-def serve_mosaic_shapes(seed=5):
-    """Sizes of the pooled request mosaics: log-uniform in 12-102 MP at 3:2 or 4:3 (sides
-    multiples of 8), a Bayer phase and a bit depth each."""
-    import math
-    import numpy as np
-    rng = np.random.default_rng(seed)
-    out = []
-    for k in range(N_SERVE_MOSAICS):
-        mp = math.exp(float(rng.uniform(math.log(12.0), math.log(102.0))))
-        aspect = 1.5 if rng.random() < 0.5 else 4.0 / 3.0
-        h = max(16, int((mp * 1e6 / aspect) ** 0.5) // 8 * 8)
-        w = max(16, int(h * aspect) // 8 * 8)
-        out.append(dict(w=w, h=h, cfa=["RGGB", "BGGR", "GRBG", "GBRG"][int(rng.integers(0, 4))],
-                        bits=12 if rng.random() < 0.5 else 14, seed=100 + k))
-    return out
-
-
-def serve_requests(rate, duration, seed):
-    """Poisson arrivals at `rate` req/s for `duration` s. Each request takes one of the pooled
-    mosaics (so sizes follow the log-uniform 12-102 MP draw) with its own WB and matrix, 30 % AHD,
-    20 % denoised first."""
-    import numpy as np
-    rng = np.random.default_rng(seed)
-    t, out = 0.0, []
-    while True:
-        t += float(rng.exponential(1.0 / rate))
-        if t > duration:
-            return out
-        out.append(dict(t=t, m=int(rng.integers(0, N_SERVE_MOSAICS)),
-                        wb=(float(rng.uniform(1.2, 2.8)), 1.0, float(rng.uniform(1.2, 2.8)), 1.0),
-                        jitter=rng.uniform(0.9, 1.1, (3, 3)), ahd=bool(rng.random() < 0.3),
-                        denoise=bool(rng.random() < 0.2)))
-
-
 def serve_mode(args):
     """The serving loop under Poisson load: end-to-end latency (arrival -> RGBA8 in host memory)
     and throughput per request rate, with the serial single-frame loop beside it (--serial)."""
@@ -436,7 +380,7 @@ def serve_mode(args):
     from flashraw_amd import synth
 
     ctx = F.context()
-    shapes = serve_mosaic_shapes()
+    shapes = synth.serve_mosaic_shapes()
     pool = []
     for sh in shapes:
         white = (1 << sh["bits"]) - 1
@@ -536,13 +480,13 @@ def serve_mode(args):
 
     # warm-up, not reported: one second of requests at 200 req/s through a server (and the serial
     # loop), so device buffers, pinned staging and code are in place before the measured rates
-    warm = serve_requests(200.0, 1.0, seed=1)
+    warm = synth.serve_requests(200.0, 1.0, seed=1)
     run_server(warm)
     if args.serial:
         run_serial(warm)
     results = []
     for rate in [float(x) for x in args.rates.split(",")]:
-        reqs = serve_requests(rate, args.duration, seed=int(rate))
+        reqs = synth.serve_requests(rate, args.duration, seed=int(rate))
         mp = sum(pool[r["m"]]["w"] * pool[r["m"]]["h"] for r in reqs) / 1e6
         lat, done_t, bsz, st, parts = run_server(reqs)
         entry = {"rate": rate, "requests": len(reqs), "mp": round(mp, 1),

@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <cctype>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -39,6 +41,61 @@ int32_t frh::hip_fail(hipError_t e, const char *what) {
 }
 
 extern "C" const char *frh_last_error_message(void) { return g_last_error.c_str(); }
+
+// ------------------------------------------------------------------------------------------
+// Cancellation (CancelToken, crates/flashraw-core/src/cancel.rs:26-76; the C API's
+// flashraw_cancel_token, crates/flashraw-capi/include/flashraw.h:1247-1275)
+
+struct frh_cancel_token {
+    mutable std::atomic<bool> flag{false};
+    int32_t (*poll)(void *) = nullptr;
+    void *user = nullptr;
+};
+
+extern "C" frh_cancel_token *frh_cancel_token_new(void) { return new (std::nothrow) frh_cancel_token(); }
+
+extern "C" frh_cancel_token *frh_cancel_token_with_poll(int32_t (*poll)(void *user), void *user) {
+    frh_cancel_token *t = new (std::nothrow) frh_cancel_token();
+    if (t) {
+        t->poll = poll;
+        t->user = user;
+    }
+    return t;
+}
+
+extern "C" void frh_cancel_token_cancel(const frh_cancel_token *t) {
+    if (t) t->flag.store(true, std::memory_order_release);
+}
+
+bool frh::cancelled(const frh_cancel_token *t) {
+    if (!t) return false;
+    if (t->flag.load(std::memory_order_acquire)) return true;
+    if (t->poll && t->poll(t->user)) {
+        t->flag.store(true, std::memory_order_release);
+        return true;
+    }
+    return false;
+}
+
+extern "C" int32_t frh_cancel_token_is_cancelled(const frh_cancel_token *t) { return frh::cancelled(t) ? 1 : 0; }
+
+extern "C" void frh_cancel_token_free(frh_cancel_token *t) { delete t; }
+
+int32_t frh::cancel_fail() { return fail(FRH_ERROR_CANCELLED, "cancelled"); }
+
+const frh_cancel_token *frh::spec_cancel(const frh_develop_spec *s) {
+    return s && s->size >= sizeof(frh_develop_spec) ? s->cancel : nullptr;
+}
+
+// FLASHRAW_GPU=0|off|false|none|cpu (any case) disables the device backend (disabled_by_env,
+// crates/flashraw-gpu/src/context.rs:734-747).
+static bool disabled_by_env() {
+    const char *v = getenv("FLASHRAW_GPU");
+    if (!v) return false;
+    std::string s(v);
+    for (char &c : s) c = (char)tolower((unsigned char)c);
+    return s == "0" || s == "off" || s == "false" || s == "none" || s == "cpu";
+}
 
 // ------------------------------------------------------------------------------------------
 // Tables (crates/flashraw-gpu/src/kernels.rs:348-371; process.rs:1903-1934)
@@ -114,6 +171,7 @@ extern "C" int32_t frh_check_sqrt(frh_context *ctx, uint64_t *mismatches) {
 // Context (context.rs:116-150, 775-842)
 
 extern "C" int32_t frh_available(void) {
+    if (disabled_by_env()) return 0;
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess) return 0;
     return n > 0 ? 1 : 0;
@@ -122,6 +180,7 @@ extern "C" int32_t frh_available(void) {
 extern "C" int32_t frh_context_create(int32_t device, frh_context **out) {
     if (!out) return fail(FRH_ERROR_INVALID_ARGUMENT, "out is NULL");
     *out = nullptr;
+    if (disabled_by_env()) return fail(FRH_ERROR_GPU_UNAVAILABLE, "disabled by `FLASHRAW_GPU`");
     try {
         int n = 0;
         hipError_t e = hipGetDeviceCount(&n);
@@ -168,8 +227,7 @@ frh_context::~frh_context() {
     if (d_lut8) (void)hipFree(d_lut8);
     if (d_srgb16) (void)hipFree(d_srgb16);
     if (d_wtab) (void)hipFree(d_wtab);
-    for (auto &kv : edge_tabs) (void)hipFree(kv.second);
-    for (auto &kv : tables) (void)hipFree(kv.second);
+    for (auto &kv : tables) (void)hipFree(kv.second.d);
     if (h_stage) (void)hipHostFree(h_stage);
     if (stream) (void)hipStreamDestroy(stream);
 }
@@ -364,28 +422,102 @@ int32_t frh::check_input(const frh_frame *f, InputDesc &in) {
     return FRH_OK;
 }
 
-const void *frh::device_table(frh_context *ctx, const void *host, uint64_t bytes, int32_t *rc) {
+static uint64_t fnv1a(uint64_t h, const void *p, uint64_t n) {
+    const uint8_t *b = static_cast<const uint8_t *>(p);
+    for (uint64_t i = 0; i < n; i++) h = (h ^ b[i]) * 1099511628211ull;
+    return h;
+}
+
+// Free least recently used, unpinned entries until `incoming` more bytes fit under the cache's
+// bounds. Entries used within the last 16 lookups are kept: a develop looks up at most 8 tables
+// between its first lookup and its launch, so a concurrent caller's tables are not freed under
+// it. Called with ctx->mu held.
+static void evict_tables(frh_context *ctx, uint64_t incoming) {
+    std::vector<void *> victims;
+    while (ctx->table_bytes + incoming > TABLE_CACHE_BYTES || ctx->tables.size() + 1 > TABLE_CACHE_ENTRIES) {
+        auto lru = ctx->tables.end();
+        for (auto it = ctx->tables.begin(); it != ctx->tables.end(); ++it)
+            if (it->second.pins == 0 && it->second.last_use + 16 < ctx->table_tick &&
+                (lru == ctx->tables.end() || it->second.last_use < lru->second.last_use))
+                lru = it;
+        if (lru == ctx->tables.end()) break;  // everything left is pinned or in use: grow past the bound
+        victims.push_back(lru->second.d);
+        ctx->table_bytes -= lru->second.dev_bytes;
+        ctx->tables.erase(lru);
+    }
+    if (victims.empty()) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipDeviceSynchronize();  // launches queued earlier may still read the evicted tables
+    for (void *d : victims) (void)hipFree(d);
+}
+
+const void *frh::cached_table(frh_context *ctx, int kind, const void *key, uint64_t key_bytes,
+                              const std::function<bool(std::vector<uint8_t> &)> &make, int32_t *rc) {
     *rc = FRH_OK;
-    if (!bytes) return nullptr;
-    uint64_t hsh = 1469598103934665603ull;
-    const uint8_t *b = static_cast<const uint8_t *>(host);
-    for (uint64_t i = 0; i < bytes; i++) hsh = (hsh ^ b[i]) * 1099511628211ull;
+    const uint64_t hsh = fnv1a(1469598103934665603ull, key, key_bytes);
     std::lock_guard<std::mutex> g(ctx->mu);
-    auto it = ctx->tables.find({hsh, bytes});
-    if (it != ctx->tables.end()) return it->second;
+    const uint64_t tick = ++ctx->table_tick;
+    auto range = ctx->tables.equal_range(std::make_tuple(kind, hsh, key_bytes));
+    for (auto it = range.first; it != range.second; ++it)
+        if (memcmp(it->second.key.data(), key, key_bytes) == 0) {
+            it->second.last_use = tick;
+            return it->second.d;
+        }
+    std::vector<uint8_t> dev;
+    if (!make(dev)) {
+        *rc = fail(FRH_ERROR_INTERNAL, "table construction failed");
+        return nullptr;
+    }
+    evict_tables(ctx, dev.size());
     void *d = nullptr;
     hipError_t e;
-    if ((e = hipMalloc(&d, bytes)) != hipSuccess) {
+    (void)hipSetDevice(ctx->device);
+    if ((e = hipMalloc(&d, dev.size())) != hipSuccess) {
         *rc = hip_fail(e, "table allocation");
         return nullptr;
     }
-    if ((e = hipMemcpy(d, host, bytes, hipMemcpyHostToDevice)) != hipSuccess) {
+    if ((e = hipMemcpy(d, dev.data(), dev.size(), hipMemcpyHostToDevice)) != hipSuccess) {
         (void)hipFree(d);
         *rc = hip_fail(e, "table upload");
         return nullptr;
     }
-    ctx->tables[{hsh, bytes}] = d;
+    TableEntry t{d, dev.size(), std::vector<uint8_t>(static_cast<const uint8_t *>(key),
+                                                      static_cast<const uint8_t *>(key) + key_bytes),
+                 tick, 0};
+    ctx->tables.emplace(std::make_tuple(kind, hsh, key_bytes), std::move(t));
+    ctx->table_bytes += dev.size();
     return d;
+}
+
+const void *frh::device_table(frh_context *ctx, const void *host, uint64_t bytes, int32_t *rc) {
+    *rc = FRH_OK;
+    if (!bytes) return nullptr;
+    return cached_table(ctx, 0, host, bytes, [&](std::vector<uint8_t> &dev) {
+        dev.assign(static_cast<const uint8_t *>(host), static_cast<const uint8_t *>(host) + bytes);
+        return true;
+    }, rc);
+}
+
+void frh::pin_tables(frh_context *ctx, const DevArgs &a, int delta) {
+    const void *named[] = {a.rowd, a.cold, a.gcal, a.sites, a.tone_tab, a.curve8, a.etab,
+                           a.srgb16 == reinterpret_cast<const float2 *>(ctx->d_srgb16) ? nullptr : a.srgb16};
+    std::lock_guard<std::mutex> g(ctx->mu);
+    for (const void *p : named) {
+        if (!p) continue;
+        for (auto &kv : ctx->tables)
+            if (kv.second.d == p) {
+                kv.second.pins = (uint32_t)((int)kv.second.pins + delta);
+                break;
+            }
+    }
+}
+
+extern "C" int32_t frh_context_table_cache(const frh_context *ctx, uint64_t *entries, uint64_t *bytes) {
+    if (!ctx) return fail(FRH_ERROR_INVALID_ARGUMENT, "NULL context");
+    std::lock_guard<std::mutex> g(const_cast<frh_context *>(ctx)->mu);
+    if (entries) *entries = ctx->tables.size();
+    if (bytes) *bytes = ctx->table_bytes;
+    return FRH_OK;
 }
 
 // The device tap lists of an edge-aware spec (built once per distinct table contents and kept by
@@ -393,58 +525,45 @@ const void *frh::device_table(frh_context *ctx, const void *host, uint64_t bytes
 // direction is non-zero, in tap order (edgeaware/mod.rs:130-160).
 static int32_t edge_tables(frh_context *ctx, const frh_edge_spec *e, const EdgeTab **out) {
     const uint32_t np = e->period * e->period;  // canonical positions: 4 (Bayer) or 36 (X-Trans)
-    uint64_t hsh = 1469598103934665603ull;
-    auto mix = [&](const void *p, size_t n) {
-        const uint8_t *b = static_cast<const uint8_t *>(p);
-        for (size_t i = 0; i < n; i++) hsh = (hsh ^ b[i]) * 1099511628211ull;
-    };
-    mix(&np, sizeof(np));
-    mix(e->canon, np);
-    mix(e->g0, np * 49 * sizeof(float));
-    mix(e->dir, np * 3 * 4 * 49 * sizeof(float));
-    std::lock_guard<std::mutex> g(ctx->mu);
-    auto it = ctx->edge_tabs.find(hsh);
-    if (it != ctx->edge_tabs.end()) {
-        *out = static_cast<const EdgeTab *>(it->second);
-        return FRH_OK;
-    }
-    std::unique_ptr<EdgeTab> hold(new EdgeTab());
-    EdgeTab &t = *hold;
-    memset(&t, 0, sizeof(t));
-    for (uint32_t ph = 0; ph < np; ph++) {
-        t.canon[ph] = std::min<uint32_t>(e->canon[ph], 3);
-        for (uint32_t i = 0; i < 49; i++) {
-            const float c = e->g0[ph * 49 + i];
-            if (c != 0.0f) {
-                t.g0_idx[ph][t.g0_n[ph]] = i;
-                t.g0_c[ph][t.g0_n[ph]] = c;
-                t.g0_n[ph]++;
-            }
-            if (ph < 4) t.g0d[ph][i] = c;
-        }
-        for (int c = 0; c < 3; c++)
+    // the key: position count, canonical colours, G0 and directional filters as given
+    std::vector<uint8_t> key(4 + np + (size_t)np * 49 * 4 + (size_t)np * 3 * 4 * 49 * 4);
+    memcpy(key.data(), &np, 4);
+    memcpy(key.data() + 4, e->canon, np);
+    memcpy(key.data() + 4 + np, e->g0, (size_t)np * 49 * 4);
+    memcpy(key.data() + 4 + np + (size_t)np * 49 * 4, e->dir, (size_t)np * 3 * 4 * 49 * 4);
+    int32_t rc = FRH_OK;
+    const void *d = cached_table(ctx, 1, key.data(), key.size(), [&](std::vector<uint8_t> &dev) {
+        dev.assign(sizeof(EdgeTab), 0);
+        EdgeTab &t = *reinterpret_cast<EdgeTab *>(dev.data());
+        for (uint32_t ph = 0; ph < np; ph++) {
+            t.canon[ph] = std::min<uint32_t>(e->canon[ph], 3);
             for (uint32_t i = 0; i < 49; i++) {
-                float v[4];
-                bool any = false;
-                for (int d = 0; d < 4; d++) {
-                    v[d] = e->dir[((ph * 3 + c) * 4 + d) * 49 + i];
-                    if (ph < 4) t.dird[ph][c][i][d] = v[d];
-                    any = any || v[d] != 0.0f;
+                const float c = e->g0[ph * 49 + i];
+                if (c != 0.0f) {
+                    t.g0_idx[ph][t.g0_n[ph]] = i;
+                    t.g0_c[ph][t.g0_n[ph]] = c;
+                    t.g0_n[ph]++;
                 }
-                if (!any) continue;
-                const uint32_t n = t.dir_n[ph][c]++;
-                t.dir_idx[ph][c][n] = i;
-                for (int d = 0; d < 4; d++) t.dir_c[ph][c][n][d] = v[d];
+                if (ph < 4) t.g0d[ph][i] = c;
             }
-    }
-    void *d = nullptr;
-    hipError_t err;
-    if ((err = hipMalloc(&d, sizeof(EdgeTab))) != hipSuccess) return hip_fail(err, "edge tables");
-    if ((err = hipMemcpy(d, &t, sizeof(EdgeTab), hipMemcpyHostToDevice)) != hipSuccess) {
-        (void)hipFree(d);
-        return hip_fail(err, "edge tables");
-    }
-    ctx->edge_tabs[hsh] = d;
+            for (int c = 0; c < 3; c++)
+                for (uint32_t i = 0; i < 49; i++) {
+                    float v[4];
+                    bool any = false;
+                    for (int dd = 0; dd < 4; dd++) {
+                        v[dd] = e->dir[((ph * 3 + c) * 4 + dd) * 49 + i];
+                        if (ph < 4) t.dird[ph][c][i][dd] = v[dd];
+                        any = any || v[dd] != 0.0f;
+                    }
+                    if (!any) continue;
+                    const uint32_t n = t.dir_n[ph][c]++;
+                    t.dir_idx[ph][c][n] = i;
+                    for (int dd = 0; dd < 4; dd++) t.dir_c[ph][c][n][dd] = v[dd];
+                }
+        }
+        return true;
+    }, &rc);
+    if (rc) return rc;
     *out = static_cast<const EdgeTab *>(d);
     return FRH_OK;
 }
@@ -518,7 +637,8 @@ int32_t frh::output_tables(frh_context *ctx, const frh_tone *tone, const frh_cur
 int32_t frh::plan_develop(const frh_context *ctx, const frh_frame *f, const frh_frame_info *info,
                             const frh_develop_spec *spec, Plan &p) {
     if (!f || !info || !spec) return fail(FRH_ERROR_INVALID_ARGUMENT, "NULL descriptor");
-    if (f->size < sizeof(frh_frame) || info->size < sizeof(frh_frame_info) || spec->size < sizeof(frh_develop_spec))
+    if (f->size < sizeof(frh_frame) || info->size < sizeof(frh_frame_info) ||
+        spec->size < offsetof(frh_develop_spec, cancel))
         return fail(FRH_ERROR_INVALID_ARGUMENT, "descriptor size field too small");
     memset(&p, 0, sizeof(p));
     DevArgs &a = p.a;
@@ -741,6 +861,8 @@ int32_t frh::plan_develop(const frh_context *ctx, const frh_frame *f, const frh_
             // 6x6 mosaics: the border pixels take the generic CFA average (sites, gcal stay set)
             a.generic = 0;
             a.edge_tame = (P == 2 && edge_tame(a, info, e)) ? 1u : 0u;
+            const char *impl = getenv("FLASHRAW_EDGE_IMPL");  // "general": the tap-list kernel always
+            if (impl && strcmp(impl, "general") == 0) a.edge_tame = 0;
         }
     }
     int32_t modes[2] = {spec->output, spec->output2};
@@ -780,12 +902,15 @@ static uint32_t band_rows(const frh_context *ctx, const Plan &p) {
     return n < p.a.dev_h ? (uint32_t)n : p.a.dev_h;
 }
 
-static int32_t launch_bands(frh_context *ctx, Plan &p, hipStream_t stream, uint32_t *nbands) {
+// Launch the develop band by band; `cancel` is checked before every band.
+static int32_t launch_bands(frh_context *ctx, Plan &p, hipStream_t stream, uint32_t *nbands,
+                            const frh_cancel_token *cancel = nullptr) {
     if (p.a.orientation >= 5 && ctx->max_binding)
         return fail(FRH_ERROR_LIMIT_EXCEEDED, "banded runs of rotated outputs are not supported");
     uint32_t n = band_rows(ctx, p);
     uint32_t bands = 0;
     for (uint32_t y0 = 0; y0 < p.a.dev_h; y0 += n) {
+        if (cancelled(cancel)) return cancel_fail();
         DevArgs a = p.a;
         a.band_y0 = y0;
         a.band_y1 = y0 + n < p.a.dev_h ? y0 + n : p.a.dev_h;
@@ -889,12 +1014,14 @@ extern "C" int32_t frh_develop_into(frh_context *ctx, const frh_frame *frame, co
             return fail(FRH_ERROR_INVALID_ARGUMENT, "second output buffer too small");
         p.a.dst = dst;
         p.a.dst2 = dst2;
+        const frh_cancel_token *cancel = spec_cancel(spec);
+        if (cancelled(cancel)) return cancel_fail();
         (void)hipSetDevice(ctx->device);
         hipStream_t st = (hipStream_t)stream;
         void *d_in = nullptr, *d_stage = nullptr;
         uint64_t in_bytes = 0, stage_bytes = 0;
         rc = stage_frame(ctx, frame, p, st, &d_in, &in_bytes, &d_stage, &stage_bytes);
-        if (rc == FRH_OK) rc = launch_bands(ctx, p, st, nullptr);
+        if (rc == FRH_OK) rc = launch_bands(ctx, p, st, nullptr, cancel);
         if (d_stage) {
             // the unpacked window is pooled scratch: wait for the develop before returning it
             hipError_t e = hipStreamSynchronize(st);
@@ -931,6 +1058,7 @@ extern "C" int32_t frh_plan_create(frh_context *ctx, const frh_frame *frame, con
             return fail(FRH_ERROR_NOT_AVAILABLE,
                         "prepared plans take u16 or f32 samples: unpack packed or linearised input first (frh_unpack_into)");
         pl->src_off = (size_t)frame->window.y * frame->raster_width * pl->p.in.cpp + (size_t)frame->window.x * pl->p.in.cpp;
+        pin_tables(ctx, pl->p.a, +1);  // the plan's tables stay cached while it lives
         *out = pl.release();
         return FRH_OK;
     } catch (const std::bad_alloc &) {
@@ -1028,7 +1156,19 @@ extern "C" uint64_t frh_plan_output_bytes(const frh_plan *plan, int32_t which) {
     return plan->p.bytes[which];
 }
 
-extern "C" void frh_plan_destroy(frh_plan *plan) { delete plan; }
+extern "C" int32_t frh_plan_kernel(const frh_plan *plan) {
+    if (!plan) return fail(FRH_ERROR_INVALID_ARGUMENT, "NULL plan");
+    const DevArgs &a = plan->p.a;
+    if (a.generic) return FRH_KERNEL_GENERIC;
+    if (a.edge_aware) return a.edge_tame ? FRH_KERNEL_EDGE_TAME : FRH_KERNEL_EDGE_GENERAL;
+    if (plan->p.ahd) return a.ahd_impl ? FRH_KERNEL_AHD_TILED : FRH_KERNEL_AHD_WALK;
+    return a.bil_impl ? FRH_KERNEL_BILINEAR_TILED : FRH_KERNEL_BILINEAR_WALK;
+}
+
+extern "C" void frh_plan_destroy(frh_plan *plan) {
+    if (plan) pin_tables(plan->ctx, plan->p.a, -1);
+    delete plan;
+}
 
 static double ms_since(std::chrono::steady_clock::time_point t0) {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -1045,6 +1185,8 @@ extern "C" int32_t frh_run_develop(frh_context *ctx, const frh_frame *frame, con
         int32_t rc = plan_develop(ctx, frame, info, spec, p);
         if (rc) return rc;
         if (spec->output2 != OUT_NONE && !out2) return fail(FRH_ERROR_INVALID_ARGUMENT, "second output needs out2");
+        const frh_cancel_token *cancel = spec_cancel(spec);
+        if (cancelled(cancel)) return cancel_fail();
         (void)hipSetDevice(ctx->device);
         hipError_t e;
         auto t0 = std::chrono::steady_clock::now();
@@ -1066,6 +1208,10 @@ extern "C" int32_t frh_run_develop(frh_context *ctx, const frh_frame *frame, con
             return hip_fail(e, "upload");
         }
         double upload_ms = ms_since(t0);
+        if (cancelled(cancel)) {  // between the upload and the develop (crates/flashraw/src/gpu.rs:341-344)
+            cleanup();
+            return cancel_fail();
+        }
         auto t1 = std::chrono::steady_clock::now();
         frh_image *imgs[2] = {nullptr, nullptr};
         for (int k = 0; k < 2; k++) {
@@ -1084,7 +1230,7 @@ extern "C" int32_t frh_run_develop(frh_context *ctx, const frh_frame *frame, con
         const bool timed = hipEventCreate(&ev0) == hipSuccess && hipEventCreate(&ev1) == hipSuccess;
         if (timed) (void)hipEventRecord(ev0, ctx->stream);
         uint32_t bands = 0;
-        rc = launch_bands(ctx, p, ctx->stream, &bands);
+        rc = launch_bands(ctx, p, ctx->stream, &bands, cancel);
         if (timed) (void)hipEventRecord(ev1, ctx->stream);
         e = hipStreamSynchronize(ctx->stream);
         float kms = 0.0f;
@@ -1093,6 +1239,7 @@ extern "C" int32_t frh_run_develop(frh_context *ctx, const frh_frame *frame, con
         if (ev1) (void)hipEventDestroy(ev1);
         cleanup();
         if (rc == FRH_OK && e != hipSuccess) rc = hip_fail(e, "develop");
+        if (rc == FRH_OK && cancelled(cancel)) rc = cancel_fail();  // after the develop (gpu.rs:475)
         if (rc != FRH_OK) {
             frh_image_free(imgs[0]);
             frh_image_free(imgs[1]);

@@ -128,6 +128,27 @@ using namespace frh;
 
 constexpr int NLIST = 8;  // (AHD, bilinear) x four Bayer phases
 
+// The work lists of a set of frames: per list its frame count, its item total, the word offset of
+// its frame ids in the list words (its item offsets follow them) and whether every frame of it
+// takes the RGBA8 fast path.
+struct Lists {
+    uint32_t n[NLIST] = {}, total[NLIST] = {};
+    size_t base[NLIST] = {};
+    bool fast[NLIST] = {};
+    uint64_t items(bool ahd) const {
+        uint64_t t = 0;
+        for (int l = ahd ? 0 : 4; l < (ahd ? 4 : 8); l++) t += total[l];
+        return t;
+    }
+};
+
+// The device memory the list kernels read: descriptors, list words and one counter per list.
+struct BatchDev {
+    const BatchFrame *frames;
+    const uint32_t *lists;
+    uint32_t *counters;
+};
+
 struct frh_batch {
     frh_context *ctx = nullptr;
     std::vector<Plan> plans;
@@ -138,26 +159,26 @@ struct frh_batch {
     uint32_t *d_lists = nullptr;  // per list: ids, then offsets
     uint32_t *d_counters = nullptr;
     size_t n_list_words = 0;
-    uint32_t n[NLIST] = {}, total[NLIST] = {};
-    size_t base[NLIST] = {};  // word offset of each list's ids in d_lists
-    bool fast[NLIST] = {};
+    Lists lists;
     hipGraphExec_t graph = nullptr;
     hipGraph_t graph_def = nullptr;
     uint32_t capacity = 0;        // frames the device and staging buffers hold
     uint8_t *h_stage = nullptr;   // pinned staging of descriptors and lists
+    // recorded after each frh_batch_launch: rebuilding the lists waits for it, so a launch still
+    // reading the descriptors is never overwritten (frh_batch_set_strip_rows, frh_batch_autotune)
+    hipEvent_t last = nullptr;
     ~frh_batch() {
         if (graph) (void)hipGraphExecDestroy(graph);
         if (graph_def) (void)hipGraphDestroy(graph_def);
+        if (last) (void)hipEventSynchronize(last);
+        if (last) (void)hipEventDestroy(last);
         if (d_frames) (void)hipFree(d_frames);
         if (d_lists) (void)hipFree(d_lists);
         if (d_counters) (void)hipFree(d_counters);
         if (h_stage) (void)hipHostFree(h_stage);
     }
-    uint64_t items(bool ahd) const {
-        uint64_t t = 0;
-        for (int l = ahd ? 0 : 4; l < (ahd ? 4 : 8); l++) t += total[l];
-        return t;
-    }
+    uint64_t items(bool ahd) const { return lists.items(ahd); }
+    BatchDev dev() const { return BatchDev{d_frames, d_lists, d_counters}; }
 };
 
 // Device and pinned staging buffers for `capacity` frames (allocated once; a batch object is
@@ -186,50 +207,61 @@ static int32_t batch_reserve(frh_batch *b, uint32_t capacity) {
     return FRH_OK;
 }
 
-// Build the work lists for the current strip heights and upload them on `st` (and wait for the
-// upload when `sync`).
-static int32_t batch_build(frh_batch *b, hipStream_t st, bool sync) {
-    const size_t n = b->plans.size();
-    int32_t rc = batch_reserve(b, (uint32_t)n);
-    if (rc) return rc;
-    b->frames.assign(n, BatchFrame{});
+// The descriptors and work lists of `plans` at strip heights ahd_rows / bil_rows (0: each frame's
+// own): `frames` gets one descriptor per plan, `words` every list's frame ids then item offsets.
+static void build_lists(const std::vector<Plan> &plans, uint32_t ahd_rows, uint32_t bil_rows,
+                        std::vector<BatchFrame> &frames, std::vector<uint32_t> &words, Lists &L) {
+    const size_t n = plans.size();
+    frames.assign(n, BatchFrame{});
     std::vector<uint32_t> ids[NLIST], off[NLIST];
     for (int l = 0; l < NLIST; l++) {
         off[l].push_back(0);
-        b->fast[l] = true;
+        L.fast[l] = true;
     }
     for (size_t i = 0; i < n; i++) {
-        const Plan &p = b->plans[i];
-        BatchFrame &F = b->frames[i];
+        const Plan &p = plans[i];
+        BatchFrame &F = frames[i];
         F.a = p.a;
         F.a.band_y0 = 0;
         F.a.band_y1 = p.a.dev_h;
         F.rp = red_position(F.a);
         const int l = (p.ahd ? 0 : 4) + F.rp;
         if (p.ahd) {
-            F.wp = make_walk_plan(F.a, (int)(b->ahd_rows ? b->ahd_rows : p.a.ahd_rows));
+            F.wp = make_walk_plan(F.a, (int)(ahd_rows ? ahd_rows : p.a.ahd_rows));
             F.walk_items = (uint32_t)(F.wp.strips * F.wp.chunks);
             F.items = F.walk_items + (F.wp.border.count + BORDER_ITEM - 1) / BORDER_ITEM;
         } else {
-            F.bp = make_bwalk_plan(F.a, (int)(b->bil_rows ? b->bil_rows : p.a.bil_rows));
+            F.bp = make_bwalk_plan(F.a, (int)(bil_rows ? bil_rows : p.a.bil_rows));
             F.walk_items = (uint32_t)(F.bp.strips * F.bp.chunks);
             F.items = F.walk_items + (F.bp.border.count + BORDER_ITEM - 1) / BORDER_ITEM;
         }
         if (F.items == 0) continue;
-        b->fast[l] = b->fast[l] && fast_rgba8(F.a);
+        L.fast[l] = L.fast[l] && fast_rgba8(F.a);
         ids[l].push_back((uint32_t)i);
         off[l].push_back(off[l].back() + F.items);
     }
-    std::vector<uint32_t> words;
+    words.clear();
     for (int l = 0; l < NLIST; l++) {
-        b->n[l] = (uint32_t)ids[l].size();
-        b->total[l] = off[l].back();
-        b->base[l] = words.size();
+        L.n[l] = (uint32_t)ids[l].size();
+        L.total[l] = off[l].back();
+        L.base[l] = words.size();
         words.insert(words.end(), ids[l].begin(), ids[l].end());
         words.insert(words.end(), off[l].begin(), off[l].end());
     }
+}
+
+// Build the work lists for the current strip heights and upload them on `st` (and wait for the
+// upload when `sync`). The upload waits for the batch's last launch, which may still be reading
+// the previous lists.
+static int32_t batch_build(frh_batch *b, hipStream_t st, bool sync) {
+    const size_t n = b->plans.size();
+    int32_t rc = batch_reserve(b, (uint32_t)n);
+    if (rc) return rc;
+    std::vector<uint32_t> words;
+    build_lists(b->plans, b->ahd_rows, b->bil_rows, b->frames, words, b->lists);
     if (words.size() > b->n_list_words) return fail(FRH_ERROR_INTERNAL, "batch lists exceed their buffer");
     hipError_t e;
+    if (b->last && (e = hipStreamWaitEvent(st, b->last, 0)) != hipSuccess) return hip_fail(e, "batch upload");
     std::memcpy(b->h_stage, b->frames.data(), n * sizeof(BatchFrame));
     std::memcpy(b->h_stage + b->capacity * sizeof(BatchFrame), words.data(), words.size() * 4);
     if ((e = hipMemcpyAsync(b->d_frames, b->h_stage, n * sizeof(BatchFrame), hipMemcpyHostToDevice, st)) !=
@@ -256,49 +288,50 @@ static uint32_t resident_blocks(K kernel, int threads, int num_cus) {
 }
 
 template <int RP, bool FAST>
-static void launch_list(frh_batch *b, int l, hipStream_t st) {
-    frh_context *ctx = b->ctx;
+static void launch_list(const frh_context *ctx, const Lists &Ls, const BatchDev &D, int l, hipStream_t st) {
     BatchList L{};
     L.lut8 = ctx->d_lut8;
     L.wtab = ctx->d_wtab;
-    L.n = b->n[l];
-    L.total = b->total[l];
-    L.counter = b->d_counters + l;
-    const uint32_t *ids = b->d_lists + b->base[l], *off = ids + b->n[l];
-    const uint32_t blocks = (b->total[l] + 3) / 4;
+    L.n = Ls.n[l];
+    L.total = Ls.total[l];
+    L.counter = D.counters + l;
+    const uint32_t *ids = D.lists + Ls.base[l], *off = ids + Ls.n[l];
+    const uint32_t blocks = (Ls.total[l] + 3) / 4;
     if (l < 4) {
         const uint32_t g = std::min(resident_blocks(k_batch_ahd<RP, FAST>, WALK_WAVES * 64, ctx->num_cus), blocks);
-        hipLaunchKernelGGL((k_batch_ahd<RP, FAST>), dim3(g), dim3(WALK_WAVES * 64), 0, st, L, b->d_frames, ids, off);
+        hipLaunchKernelGGL((k_batch_ahd<RP, FAST>), dim3(g), dim3(WALK_WAVES * 64), 0, st, L, D.frames, ids, off);
     } else {
         const uint32_t g =
             std::min(resident_blocks(k_batch_bilinear<RP, FAST>, BWALK_WAVES * 64, ctx->num_cus), blocks);
-        hipLaunchKernelGGL((k_batch_bilinear<RP, FAST>), dim3(g), dim3(BWALK_WAVES * 64), 0, st, L, b->d_frames,
-                           ids, off);
+        hipLaunchKernelGGL((k_batch_bilinear<RP, FAST>), dim3(g), dim3(BWALK_WAVES * 64), 0, st, L, D.frames, ids,
+                           off);
     }
 }
 
 template <int RP>
-static void launch_list_rp(frh_batch *b, int l, hipStream_t st) {
-    if (b->fast[l])
-        launch_list<RP, true>(b, l, st);
+static void launch_list_rp(const frh_context *ctx, const Lists &Ls, const BatchDev &D, int l, hipStream_t st) {
+    if (Ls.fast[l])
+        launch_list<RP, true>(ctx, Ls, D, l, st);
     else
-        launch_list<RP, false>(b, l, st);
+        launch_list<RP, false>(ctx, Ls, D, l, st);
 }
 
 // The launch sequence: counter reset, then each non-empty list (AHD lists first).
-static hipError_t batch_enqueue(frh_batch *b, hipStream_t st) {
-    hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, st, b->d_counters, (uint32_t)NLIST);
+static hipError_t enqueue_lists(const frh_context *ctx, const Lists &Ls, const BatchDev &D, hipStream_t st) {
+    hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, st, D.counters, (uint32_t)NLIST);
     for (int l = 0; l < NLIST; l++) {
-        if (!b->total[l]) continue;
+        if (!Ls.total[l]) continue;
         switch (l & 3) {
-        case 0: launch_list_rp<0>(b, l, st); break;
-        case 1: launch_list_rp<1>(b, l, st); break;
-        case 2: launch_list_rp<2>(b, l, st); break;
-        default: launch_list_rp<3>(b, l, st); break;
+        case 0: launch_list_rp<0>(ctx, Ls, D, l, st); break;
+        case 1: launch_list_rp<1>(ctx, Ls, D, l, st); break;
+        case 2: launch_list_rp<2>(ctx, Ls, D, l, st); break;
+        default: launch_list_rp<3>(ctx, Ls, D, l, st); break;
         }
     }
     return hipGetLastError();
 }
+
+static hipError_t batch_enqueue(frh_batch *b, hipStream_t st) { return enqueue_lists(b->ctx, b->lists, b->dev(), st); }
 
 extern "C" int32_t frh_batch_create(frh_context *ctx, uint32_t n, const frh_frame *frames,
                                     const frh_frame_info *infos, const frh_develop_spec *specs, void *const *dsts,
@@ -348,9 +381,16 @@ extern "C" int32_t frh_batch_launch(frh_batch *b, void *stream) {
     hipError_t e;
     if (b->graph) {
         if ((e = hipGraphLaunch(b->graph, st)) != hipSuccess) return hip_fail(e, "graph launch");
-        return FRH_OK;
+    } else if ((e = batch_enqueue(b, st)) != hipSuccess) {
+        return hip_fail(e, "batch launch");
     }
-    if ((e = batch_enqueue(b, st)) != hipSuccess) return hip_fail(e, "batch launch");
+    // remember the launch (not while the caller captures `st` into a graph of its own)
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(st, &cs) == hipSuccess && cs == hipStreamCaptureStatusNone) {
+        if (!b->last && (e = hipEventCreateWithFlags(&b->last, hipEventDisableTiming)) != hipSuccess)
+            return hip_fail(e, "batch event");
+        if ((e = hipEventRecord(b->last, st)) != hipSuccess) return hip_fail(e, "batch event");
+    }
     return FRH_OK;
 }
 

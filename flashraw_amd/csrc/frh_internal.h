@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
 #include <map>
 #include <tuple>
 #include <memory>
@@ -23,11 +24,35 @@ int32_t hip_fail(hipError_t e, const char *what);
 // The 4097-entry sRGB 8-bit table (process.rs:1903-1915).
 void build_srgb8(uint8_t out[4097]);
 
+// Whether `t` (NULL: never) has been cancelled; a poll answering non-zero cancels it for good.
+bool cancelled(const frh_cancel_token *t);
+// FRH_ERROR_CANCELLED with its message.
+int32_t cancel_fail();
+// The token of a develop spec, NULL when its `size` ends before the field.
+const frh_cancel_token *spec_cancel(const frh_develop_spec *s);
+
 // Bytes of freed device buffers a context keeps for reuse. The reference keeps 1 GiB
 // (POOL_MAX_BYTES, crates/flashraw-gpu/src/context.rs:153); a 24 MP noise reduction alone holds
 // about 2 GB of planes and partial sums at once, and an MI355X has 288 GB, so the pool keeps 8 GiB
 // and repeated runs reuse their buffers instead of freeing and allocating them again.
 constexpr uint64_t POOL_MAX_BYTES = 8ull << 30;
+
+// The per-frame table cache (calibration, CFA sites, black-level deltas, linearisation and tone
+// tables, edge-aware tap lists) keeps at most this many bytes and entries. Beyond either, the
+// least recently used entries that no prepared plan holds are freed, after a device
+// synchronisation so that no queued launch still reads them.
+constexpr uint64_t TABLE_CACHE_BYTES = 64ull << 20;
+constexpr uint64_t TABLE_CACHE_ENTRIES = 4096;
+
+// A device copy of a table, found by (kind, hash of its key bytes, key length) and confirmed by
+// comparing the key bytes kept on the host. `pins` counts the prepared plans that hold it.
+struct TableEntry {
+    void *d;
+    uint64_t dev_bytes;
+    std::vector<uint8_t> key;
+    uint64_t last_use;
+    uint32_t pins;
+};
 
 }  // namespace frh
 
@@ -45,11 +70,11 @@ struct frh_context {
     uint32_t bil_rows = 32;  // rows per bilinear walker strip
     // strip heights found by frh_plan_autotune, by (AHD, developed width, rows, fast path)
     std::map<std::tuple<int, uint32_t, uint32_t, int>, uint32_t> tuned;
-    // device copies of edge-aware tap lists, by a hash of the tables they were built from
-    std::map<uint64_t, void *> edge_tabs;
     // device copies of small per-frame tables (calibration, CFA sites, black-level deltas,
-    // linearisation curves), by (hash of the contents, bytes); kept until the context ends
-    std::map<std::pair<uint64_t, uint64_t>, void *> tables;
+    // linearisation curves, tone and curve tables: kind 0) and of edge-aware tap lists (kind 1),
+    // by (kind, hash of the key bytes, key length); bounded by TABLE_CACHE_BYTES / _ENTRIES
+    std::multimap<std::tuple<int, uint64_t, uint64_t>, frh::TableEntry> tables;
+    uint64_t table_bytes = 0, table_tick = 0;
     std::mutex mu;
     struct Block {
         void *ptr;
@@ -93,7 +118,15 @@ int32_t check_input(const frh_frame *f, InputDesc &in);
 int32_t build_calib(const frh_frame_info *info, uint32_t cpp, std::vector<float> &gcal, uint32_t &pw, uint32_t &ph);
 
 // A device copy of `bytes` bytes at `host`, cached by the context (NULL with *rc set on failure).
+// The copy stays valid for launches queued before the next call that evicts it (which
+// synchronises the device first); a prepared plan pins the tables it names.
 const void *device_table(frh_context *ctx, const void *host, uint64_t bytes, int32_t *rc);
+// The cached device copy for (kind, key), built by `make` (which fills the device bytes to upload)
+// on a miss.
+const void *cached_table(frh_context *ctx, int kind, const void *key, uint64_t key_bytes,
+                         const std::function<bool(std::vector<uint8_t> &)> &make, int32_t *rc);
+// Pin (+1) or unpin (-1) the cached tables a develop's arguments name.
+void pin_tables(frh_context *ctx, const DevArgs &a, int delta);
 
 struct Plan {
     DevArgs a;

@@ -511,7 +511,7 @@ struct Run {
 
 int32_t check_spec(const frh_nr_spec *s) {
     if (!s) return fail(FRH_ERROR_INVALID_ARGUMENT, "NULL spec");
-    if (s->size < sizeof(frh_nr_spec)) return fail(FRH_ERROR_INVALID_ARGUMENT, "spec size field too small");
+    if (s->size < offsetof(frh_nr_spec, cancel)) return fail(FRH_ERROR_INVALID_ARGUMENT, "spec size field too small");
     if (!valid_mode(s->output)) return fail(FRH_ERROR_INVALID_ARGUMENT, "unknown output");
     for (int k = 0; k < 2; k++) {
         const frh_nr_plane &p = k ? s->chroma : s->luma;
@@ -541,6 +541,10 @@ void out_geometry(const frh_nr_spec *s, uint32_t w, uint32_t h, uint32_t &ow, ui
 // GpuContext::denoise (denoise.rs:817-1176) on device buffers.
 int32_t denoise(frh_context *ctx, const float *rgb, uint32_t w, uint32_t h, const frh_nr_spec *spec, void *dst,
                 hipStream_t st) {
+    // the token is checked between the stages: split and pyramids, noise estimate, passes, guided
+    // filter, encode (queued work finishes before the arena returns its buffers)
+    const frh_cancel_token *cancel = spec->size >= sizeof(frh_nr_spec) ? spec->cancel : nullptr;
+    if (cancelled(cancel)) return cancel_fail();
     Run R(ctx, st);
     int32_t rc = FRH_OK;
     R.dct = nullptr;
@@ -566,6 +570,7 @@ int32_t denoise(frh_context *ctx, const float *rgb, uint32_t w, uint32_t h, cons
         vlev.push_back(R.down(vlev.back()));
     }
     if (!R.ok()) return R.e != hipSuccess ? hip_fail(R.e, "noise reduction") : hip_fail(R.ar.err, "noise reduction");
+    if (cancelled(cancel)) return cancel_fail();
     // ---- the noise estimate: blocks measured and models fitted on the device ----
     const uint32_t nyl = luma ? spec->luma.levels : 1u;
     const uint32_t want[3] = {(luma || chroma) ? nyl : 0u, nc, nc};
@@ -685,6 +690,7 @@ int32_t denoise(frh_context *ctx, const float *rgb, uint32_t w, uint32_t h, cons
             v.push_back(l < index[pi].size() && index[pi][l] >= 0 ? d_models + index[pi][l] : nullptr);
         return v;
     };
+    if (cancelled(cancel)) return cancel_fail();
     // ---- the passes ----
     // every plane's passes together: luma (its own levels as guides), then the two colour planes
     // (guided by the luma levels)
@@ -703,6 +709,7 @@ int32_t denoise(frh_context *ctx, const float *rgb, uint32_t w, uint32_t h, cons
     if (y2 && spec->luma.keep > 0.0f) R.check(nr_blend(y2, Y.p, spec->luma.keep, n, st));
     const float *y2r = y2 ? y2 : Y.p;
     const float *u2r = U.p, *v2r = V.p;
+    if (cancelled(cancel)) return cancel_fail();
     if (chroma && R.ok()) {
         float *ud = jobs[luma ? 1 : 0].result, *vd = jobs[luma ? 2 : 1].result;
         if (!ud || !vd || !R.ok())
@@ -753,6 +760,7 @@ int32_t denoise(frh_context *ctx, const float *rgb, uint32_t w, uint32_t h, cons
         v2r = outs[1];
     }
     if (!R.ok()) return R.e != hipSuccess ? hip_fail(R.e, "noise reduction") : hip_fail(R.ar.err, "noise reduction");
+    if (cancelled(cancel)) return cancel_fail();
     // merge and encode (denoise_encode.wgsl: the develop's tone, encoders and orientation)
     DevArgs a;
     memset(&a, 0, sizeof(a));
@@ -773,6 +781,7 @@ int32_t denoise(frh_context *ctx, const float *rgb, uint32_t w, uint32_t h, cons
     // the arena's buffers go back to the pool once the stream has used them
     R.check(hipStreamSynchronize(st));
     if (!R.ok()) return hip_fail(R.e, "noise reduction");
+    if (cancelled(cancel)) return cancel_fail();
     return FRH_OK;
 }
 

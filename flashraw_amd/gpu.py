@@ -119,6 +119,7 @@ class _DevelopSpec(ctypes.Structure):
         ("cam_to_rgb", ctypes.c_float * 9), ("orientation", ctypes.c_int32), ("output", ctypes.c_int32),
         ("output2", ctypes.c_int32), ("has_levels", ctypes.c_int32), ("levels", ctypes.c_float * 2),
         ("edge", ctypes.POINTER(_EdgeSpec)), ("curve", ctypes.POINTER(_Curve)), ("tone", ctypes.POINTER(_Tone)),
+        ("cancel", ctypes.c_void_p),
     ]
 
 
@@ -136,11 +137,41 @@ class _NrSpec(ctypes.Structure):
     _fields_ = [("size", ctypes.c_uint32), ("has_luma", ctypes.c_int32), ("has_chroma", ctypes.c_int32),
                 ("luma", _NrPlane), ("chroma", _NrPlane), ("chroma_radius", ctypes.c_uint32),
                 ("hard", ctypes.c_float), ("max_blocks", ctypes.c_uint32), ("output", ctypes.c_int32),
-                ("orientation", ctypes.c_int32), ("tone", ctypes.POINTER(_Tone)), ("curve", ctypes.POINTER(_Curve))]
+                ("orientation", ctypes.c_int32), ("tone", ctypes.POINTER(_Tone)), ("curve", ctypes.POINTER(_Curve)),
+                ("cancel", ctypes.c_void_p)]
 
 
 _lib = None
 _lib_lock = threading.Lock()
+_POLL_FN = ctypes.CFUNCTYPE(ctypes.c_int32, ctypes.c_void_p)
+
+
+class CancelToken:
+    """CancelToken (crates/flashraw-core/src/cancel.rs:26-76) over frh_cancel_token: a shared
+    flag that stops a develop or noise reduction between its stages (the call raises Cancelled).
+    `poll`, when given, is asked on every check while the token is not cancelled (with_poll)."""
+
+    def __init__(self, poll=None):
+        L = lib()
+        if poll is None:
+            self._cb = None
+            self._h = L.frh_cancel_token_new()
+        else:
+            self._cb = _POLL_FN(lambda _user: 1 if poll() else 0)
+            self._h = L.frh_cancel_token_with_poll(self._cb, None)
+        if not self._h:
+            raise OutOfMemory(7, "cancel token")
+
+    def cancel(self) -> None:
+        lib().frh_cancel_token_cancel(self._h)
+
+    def is_cancelled(self) -> bool:
+        return bool(lib().frh_cancel_token_is_cancelled(self._h))
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.frh_cancel_token_free(self._h)
+            self._h = None
 
 
 def lib():
@@ -250,6 +281,17 @@ def lib():
             L.frh_host_alloc.argtypes = [u64]
             L.frh_host_alloc.restype = vp
             L.frh_host_free.argtypes = [vp]
+            L.frh_cancel_token_new.restype = vp
+            L.frh_cancel_token_with_poll.argtypes = [_POLL_FN, vp]
+            L.frh_cancel_token_with_poll.restype = vp
+            L.frh_cancel_token_cancel.argtypes = [vp]
+            L.frh_cancel_token_is_cancelled.argtypes = [vp]
+            L.frh_cancel_token_is_cancelled.restype = i32
+            L.frh_cancel_token_free.argtypes = [vp]
+            L.frh_context_table_cache.argtypes = [vp, ctypes.POINTER(u64), ctypes.POINTER(u64)]
+            L.frh_context_table_cache.restype = i32
+            L.frh_plan_kernel.argtypes = [vp]
+            L.frh_plan_kernel.restype = i32
             _lib = L
         return _lib
 
@@ -591,6 +633,7 @@ class DevelopSpec:
     tone: object = None
     edge: Optional[EdgeSpec] = None
     output2: Optional[DevelopOutput] = None
+    cancel: Optional[CancelToken] = None
 
     def _c(self):
         s = _DevelopSpec()
@@ -610,6 +653,9 @@ class DevelopSpec:
             s.levels[0], s.levels[1] = float(self.levels[0]), float(self.levels[1])
         keep = []
         s.tone, s.curve = _tone_curve_c(self.tone, self.curve, keep)
+        if self.cancel is not None:
+            s.cancel = self.cancel._h
+            keep.append(self.cancel)
         if self.edge is not None:
             e = _EdgeSpec()
             e.size = ctypes.sizeof(_EdgeSpec)
@@ -849,6 +895,7 @@ class DenoiseSpec:
     orientation: "Orientation" = None
     tone: Optional[GpuTone] = None
     curve: Optional[GpuCurve] = None
+    cancel: Optional[CancelToken] = None
 
     def _c(self):
         s = _NrSpec()
@@ -864,6 +911,9 @@ class DenoiseSpec:
         s.orientation = int(Orientation.Normal if self.orientation is None else self.orientation)
         keep = []
         s.tone, s.curve = _tone_curve_c(self.tone, self.curve, keep)
+        if self.cancel is not None:
+            s.cancel = self.cancel._h
+            keep.append(self.cancel)
         return s, keep
 
 
@@ -988,6 +1038,12 @@ class GpuContext:
         n = ctypes.c_uint64()
         check(lib().frh_check_sqrt(self._h, ctypes.byref(n)))
         return int(n.value)
+
+    def table_cache(self):
+        """(entries, device bytes) of the context's per-frame table cache (frh_context_table_cache)."""
+        n, b = ctypes.c_uint64(), ctypes.c_uint64()
+        check(lib().frh_context_table_cache(self._h, ctypes.byref(n), ctypes.byref(b)))
+        return int(n.value), int(b.value)
 
     def override_limits(self, max_binding_bytes: int) -> None:
         """context.rs:704: cap the bytes one band binds (forces banded runs); 0 resets."""
@@ -1286,6 +1342,16 @@ class Plan:
                                       None if dst2 is None else _device_ptr(dst2), self.ctx.stream_handle(stream),
                                       iters, ctypes.byref(best)))
         return best.value
+
+    KERNELS = ("bilinear_walk", "bilinear_tiled", "ahd_walk", "ahd_tiled", "edge_tame", "edge_general", "generic")
+
+    @property
+    def kernel(self) -> str:
+        """The kernel family the plan's launches take (frh_plan_kernel)."""
+        k = lib().frh_plan_kernel(self._h)
+        if k < 0 or k >= len(self.KERNELS):
+            check(k)
+        return self.KERNELS[k]
 
     def __del__(self):
         if getattr(self, "_h", None):

@@ -102,3 +102,86 @@ def edge_tables(seed=0):
                 dirt[ph, c, d] = filt(along[d])
     return g0, dirt
 
+
+_GOLDEN = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "tests", "golden")
+
+
+def trained_edge_tables(kind="bayer"):
+    """The reference's trained edge-aware filter tables (crates/flashraw/src/edgeaware/bayer.bin,
+    xtrans.bin, kept as test data under tests/golden/edge_{bayer,xtrans}.bin): little-endian f32,
+    g0 [P*P][49] then dir [P*P][3][4][49] (load, edgeaware/mod.rs:131-135), P = 2 or 6. Returns
+    (g0, dir) shaped that way."""
+    p = 2 if kind == "bayer" else 6
+    n = p * p
+    f = np.fromfile(os.path.join(_GOLDEN, f"edge_{kind}.bin"), dtype="<f4")
+    if f.size != n * 49 + n * 3 * 4 * 49:
+        raise ValueError(f"edge_{kind}.bin has {f.size} floats")
+    return f[: n * 49].reshape(n, 49).copy(), f[n * 49:].reshape(n, 3, 4, 49).copy()
+
+
+# ---- the batched and served workloads ----
+
+def batch_frames(n, seed=3):
+    """The batched workload: n frames with sizes 12, 20, 24, 33, 45 or 61 MP at 3:2 or 4:3, a
+    uniformly drawn Bayer phase, 12 or 14 bits, WB R and B in [1.2, 2.8] and the colour matrix
+    jittered by up to 10 %, AHD to RGBA8 (sides rounded to even). Each entry carries the seed of
+    its mosaic (black 64 * (bits - 10), white 2^bits - 1)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        mp = float(rng.choice([12, 20, 24, 33, 45, 61]))
+        aspect = 1.5 if rng.random() < 0.5 else 4.0 / 3.0
+        h = int(round((mp * 1e6 / aspect) ** 0.5 / 2)) * 2
+        w = int(round(h * aspect / 2)) * 2
+        out.append(dict(w=w, h=h, cfa=["RGGB", "BGGR", "GRBG", "GBRG"][int(rng.integers(0, 4))],
+                        bits=12 if rng.random() < 0.5 else 14, wb=(float(rng.uniform(1.2, 2.8)), 1.0,
+                                                                      float(rng.uniform(1.2, 2.8)), 1.0),
+                        jitter=rng.uniform(0.9, 1.1, (3, 3)), seed=1000 + k))
+    return out
+
+
+def batch_mosaic(f):
+    """The mosaic of a batch_frames entry."""
+    white = (1 << f["bits"]) - 1
+    return mosaic(f["w"], f["h"], f["cfa"], black=64.0 * (f["bits"] - 10), white=float(white), seed=f["seed"])
+
+
+def jittered_matrix(jitter):
+    """MATRIX scaled element-wise by `jitter` (3x3), as a tuple of rows."""
+    return tuple(tuple(float(v) for v in row) for row in np.asarray(MATRIX) * jitter)
+
+
+N_SERVE_MOSAICS = 32
+
+
+def serve_mosaic_shapes(seed=5, n=N_SERVE_MOSAICS):
+    """Sizes of the served request mosaics: log-uniform in 12-102 MP at 3:2 or 4:3 (sides
+    multiples of 8), a Bayer phase and a bit depth each."""
+    import math
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        mp = math.exp(float(rng.uniform(math.log(12.0), math.log(102.0))))
+        aspect = 1.5 if rng.random() < 0.5 else 4.0 / 3.0
+        h = max(16, int((mp * 1e6 / aspect) ** 0.5) // 8 * 8)
+        w = max(16, int(h * aspect) // 8 * 8)
+        out.append(dict(w=w, h=h, cfa=["RGGB", "BGGR", "GRBG", "GBRG"][int(rng.integers(0, 4))],
+                        bits=12 if rng.random() < 0.5 else 14, seed=100 + k))
+    return out
+
+
+def serve_requests(rate, duration, seed, n_mosaics=N_SERVE_MOSAICS):
+    """Poisson arrivals at `rate` req/s for `duration` s. Each request takes one of the pooled
+    mosaics (so sizes follow the log-uniform 12-102 MP draw) with its own WB and matrix, 30 % AHD,
+    20 % denoised first."""
+    rng = np.random.default_rng(seed)
+    t, out = 0.0, []
+    while True:
+        t += float(rng.exponential(1.0 / rate))
+        if t > duration:
+            return out
+        out.append(dict(t=t, m=int(rng.integers(0, n_mosaics)),
+                        wb=(float(rng.uniform(1.2, 2.8)), 1.0, float(rng.uniform(1.2, 2.8)), 1.0),
+                        jitter=rng.uniform(0.9, 1.1, (3, 3)), ahd=bool(rng.random() < 0.3),
+                        denoise=bool(rng.random() < 0.2)))
+

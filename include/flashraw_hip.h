@@ -33,6 +33,30 @@ enum frh_status {
 typedef struct frh_context frh_context;
 typedef struct frh_image frh_image;
 
+/* ---- cancellation (flashraw_cancel_token, crates/flashraw-capi/include/flashraw.h:818-824 and
+ * 1247-1275; CancelToken, crates/flashraw-core/src/cancel.rs:26-76) ----
+ * A shared flag that asks a running develop or noise reduction to stop. The entry points that take
+ * one (frh_run_develop, frh_develop_into, frh_run_denoise, frh_denoise_into, through the spec's
+ * `cancel` field) check it between their stages and launches, as the reference's GPU facade does
+ * between its stages (crates/flashraw/src/gpu.rs:292-299): work already queued on the device runs
+ * to its end, nothing further is queued, and the call returns FRH_ERROR_CANCELLED with no result
+ * (the caller's output buffers then hold unspecified bytes). Cancel and is_cancelled may be called
+ * from any thread, also while a call uses the token; a token must not be freed while a call holds
+ * it. A cancelled token stays cancelled. */
+typedef struct frh_cancel_token frh_cancel_token;
+/* flashraw_cancel_token_new: a token that is not cancelled. */
+frh_cancel_token *frh_cancel_token_new(void);
+/* CancelToken::with_poll (cancel.rs:40-45): a token that also asks poll(user) on every check
+ * while it is not cancelled, and is cancelled for good once poll answers non-zero. poll runs on the
+ * thread that checks and must be cheap. */
+frh_cancel_token *frh_cancel_token_with_poll(int32_t (*poll)(void *user), void *user);
+/* flashraw_cancel_token_cancel: does nothing for NULL. */
+void frh_cancel_token_cancel(const frh_cancel_token *t);
+/* flashraw_cancel_token_is_cancelled: non-zero once cancelled (0 for NULL). */
+int32_t frh_cancel_token_is_cancelled(const frh_cancel_token *t);
+/* flashraw_cancel_token_free: does nothing for NULL. */
+void frh_cancel_token_free(frh_cancel_token *t);
+
 /* flashraw_core::Rect */
 typedef struct frh_rect {
     uint32_t x, y, width, height;
@@ -147,7 +171,10 @@ typedef struct frh_tone {
 
 /* DevelopSpec (run.rs:115-135). `output2` (FRH_OUTPUT_NONE for none) asks the same pass to
  * store a second encoding, e.g. linear 16-bit and RGBA8 together. `curve` and `tone` (NULL for
- * the sRGB curve and no tone) apply to the display encodings only. */
+ * the sRGB curve and no tone) apply to the display encodings only. `cancel` (NULL: the develop
+ * cannot be stopped) is DecodeOptions::cancel (crates/flashraw/src/options.rs:501-508), read by
+ * frh_run_develop and frh_develop_into; plans, batches and the serving loop ignore it. A spec
+ * whose `size` ends before `cancel` (the layout without it) is accepted and has no token. */
 typedef struct frh_develop_spec {
     uint32_t size;
     frh_rect crop;
@@ -162,6 +189,7 @@ typedef struct frh_develop_spec {
     const frh_edge_spec *edge;
     const frh_curve *curve;
     const frh_tone *tone;
+    const frh_cancel_token *cancel;
 } frh_develop_spec;
 
 /* GpuTimings (run.rs:229-239). */
@@ -175,9 +203,12 @@ typedef struct frh_timings {
 
 /* ---- context (crates/flashraw-gpu/src/context.rs:775-842) ---- */
 
-/* flashraw_gpu::available() */
+/* flashraw_gpu::available(). 0 when the environment variable FLASHRAW_GPU is 0, off, false, none
+ * or cpu (any case; disabled_by_env, context.rs:734-747), without touching the device. */
 int32_t frh_available(void);
-/* flashraw_gpu::init() / context(): a context on HIP device `device`. */
+/* flashraw_gpu::init() / context(): a context on HIP device `device`. FRH_ERROR_GPU_UNAVAILABLE
+ * ("disabled by FLASHRAW_GPU") under the same environment switch (GpuContext::new,
+ * context.rs:203-206). */
 int32_t frh_context_create(int32_t device, frh_context **out);
 void frh_context_destroy(frh_context *ctx);
 /* GpuContext::info() / adapter_info(): a description of the device into `buf`. */
@@ -238,6 +269,21 @@ int32_t frh_plan_autotune(frh_plan *plan, const void *src, void *dst, void *dst2
                           uint32_t *best);
 /* Bytes of the plan's first and second outputs. */
 uint64_t frh_plan_output_bytes(const frh_plan *plan, int32_t which);
+/* No reference counterpart (the reference picks one WGSL pipeline per demosaic, run.rs:545-600):
+ * which kernel family the plan's launches take. The edge-aware demosaic has two kernels that give
+ * the same bytes: one for inputs that bound every intermediate value (dense 7x7 taps in registers)
+ * and the general one (tap lists); FLASHRAW_EDGE_IMPL=general forces the latter, as
+ * FLASHRAW_AHD_IMPL=tiled and FLASHRAW_BILINEAR_IMPL=tiled force the LDS-tiled demosaics. */
+enum frh_kernel {
+    FRH_KERNEL_BILINEAR_WALK = 0,
+    FRH_KERNEL_BILINEAR_TILED = 1,
+    FRH_KERNEL_AHD_WALK = 2,
+    FRH_KERNEL_AHD_TILED = 3,
+    FRH_KERNEL_EDGE_TAME = 4,
+    FRH_KERNEL_EDGE_GENERAL = 5,
+    FRH_KERNEL_GENERIC = 6
+};
+int32_t frh_plan_kernel(const frh_plan *plan);
 void frh_plan_destroy(frh_plan *plan);
 /* The context's own stream (a hipStream_t). */
 void *frh_context_stream(const frh_context *ctx);
@@ -328,6 +374,7 @@ typedef struct frh_nr_spec {
     int32_t orientation;    /* EXIF 1..8, display outputs */
     const frh_tone *tone;
     const frh_curve *curve;
+    const frh_cancel_token *cancel; /* NULL: cannot be stopped; checked between the stages */
 } frh_nr_spec;
 
 /* flashraw::denoise::gpu_spec (mod.rs:155-193): the spec of strengths `luma` and `chroma`
@@ -421,6 +468,12 @@ int32_t frh_srgb8_table(uint8_t *out, size_t len);
  * [2^-96, 2^128), and +-0, through it on the context's device and reports in *mismatches how many
  * differ from the correctly rounded square root (0 on a device where the develop is exact). */
 int32_t frh_check_sqrt(frh_context *ctx, uint64_t *mismatches);
+/* No reference counterpart. The context's cache of per-frame device tables (calibration, CFA
+ * sites, black-level deltas, linearisation, tone and curve tables, edge-aware tap lists): entries
+ * and device bytes. Hits compare the full table contents; the cache keeps at most 64 MiB and 4096
+ * entries, freeing the least recently used tables no prepared plan holds (after synchronising the
+ * device, so that no queued launch still reads them) when a new table would not fit. */
+int32_t frh_context_table_cache(const frh_context *ctx, uint64_t *entries, uint64_t *bytes);
 
 #ifdef __cplusplus
 }
