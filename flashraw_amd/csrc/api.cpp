@@ -189,6 +189,9 @@ extern "C" int32_t frh_context_create(int32_t device, frh_context **out) {
         std::unique_ptr<frh_context> c(new frh_context());
         c->device = device;
         if ((e = hipSetDevice(device)) != hipSuccess) return hip_fail(e, "hipSetDevice");
+        // this device's copy of the no-allocation batch's scratch array
+        if ((e = noalloc_scratch_address(&c->noalloc_scratch)) != hipSuccess)
+            return hip_fail(e, "scratch array address");
         hipDeviceProp_t prop;
         if ((e = hipGetDeviceProperties(&prop, device)) != hipSuccess) return hip_fail(e, "hipGetDeviceProperties");
         c->num_cus = prop.multiProcessorCount;

@@ -518,3 +518,127 @@ int32_t frh::batch_assign(frh_batch *b, std::vector<Plan> &&plans, hipStream_t s
 }
 
 hipError_t frh::batch_run(frh_batch *b, hipStream_t st) { return batch_enqueue(b, st); }
+
+// ---- the batched develop that allocates nothing ----
+//
+// frh_batch_develop_noalloc validates, lists and launches a batch without any device allocation:
+// the descriptors, the work lists and the list counters live in `scratch`, a fixed array of this
+// library in device memory (one copy per GPU, held until the process ends). The host fills it
+// through the arguments of small copy launches on the caller's stream, so the upload is ordered
+// with the launches that read it and allocates nothing either.
+
+namespace {
+constexpr uint32_t NOALLOC_MAX_FRAMES = 256;
+constexpr size_t NA_FRAMES_BYTES = ((size_t)NOALLOC_MAX_FRAMES * sizeof(BatchFrame) + 255) & ~(size_t)255;
+constexpr size_t NA_LIST_WORDS = 2 * (size_t)NOALLOC_MAX_FRAMES + NLIST;
+constexpr size_t NA_LISTS_BYTES = (NA_LIST_WORDS * 4 + 255) & ~(size_t)255;
+constexpr size_t NA_SCRATCH_BYTES = NA_FRAMES_BYTES + NA_LISTS_BYTES + 256;  // + the list counters
+
+// Words one copy launch carries in its arguments (about 2 KB of the kernel argument segment).
+constexpr uint32_t PUT_WORDS = 480;
+struct PutWords {
+    uint32_t *dst;
+    uint32_t count;
+    uint32_t w[PUT_WORDS];
+};
+
+__global__ __launch_bounds__(256) void k_put_words(PutWords p) {
+    // read in place from the argument segment (indexing the by-value copy would move it to
+    // private memory per lane)
+    (void)p;
+    const PutWords &q = *(const PutWords *)__builtin_amdgcn_kernarg_segment_ptr();
+    for (uint32_t i = threadIdx.x; i < q.count; i += blockDim.x) q.dst[i] = q.w[i];
+}
+
+// Copy `count` words from host memory to device `dst` on `st` through launch arguments.
+hipError_t put_words(uint32_t *dst, const uint32_t *src, size_t count, hipStream_t st) {
+    PutWords p;
+    for (size_t i = 0; i < count; i += PUT_WORDS) {
+        p.dst = dst + i;
+        p.count = (uint32_t)std::min<size_t>(PUT_WORDS, count - i);
+        std::memcpy(p.w, src + i, p.count * 4);
+        hipLaunchKernelGGL(k_put_words, dim3(1), dim3(256), 0, st, p);
+    }
+    return hipGetLastError();
+}
+}  // namespace
+
+static __device__ __align__(256) uint8_t scratch[NA_SCRATCH_BYTES];
+
+// The device address of `scratch` on the current device (its host-side name is not one).
+hipError_t frh::noalloc_scratch_address(void **ptr) {
+    return hipGetSymbolAddress(ptr, HIP_SYMBOL(scratch));
+}
+
+extern "C" int32_t frh_batch_develop_noalloc(frh_context *ctx, uint32_t n, const frh_frame *frames,
+                                             const frh_frame_info *infos, const frh_develop_spec *specs,
+                                             void *const *dsts, uint32_t ahd_rows, uint32_t bilinear_rows,
+                                             void *stream) {
+    try {
+        if (!ctx || (n && (!frames || !infos || !specs || !dsts)))
+            return fail(FRH_ERROR_INVALID_ARGUMENT, "NULL argument");
+        if (n == 0) return fail(FRH_ERROR_INVALID_ARGUMENT, "empty batch");
+        if (n > NOALLOC_MAX_FRAMES)
+            return fail(FRH_ERROR_LIMIT_EXCEEDED, "frh_batch_develop_noalloc takes at most " +
+                                                      std::to_string(NOALLOC_MAX_FRAMES) + " frames per call");
+        if (!ctx->noalloc_scratch) return fail(FRH_ERROR_GPU_UNAVAILABLE, "the device scratch array is not set up");
+        std::vector<Plan> plans(n);
+        for (uint32_t i = 0; i < n; i++) {
+            const frh_frame &f = frames[i];
+            const frh_frame_info &inf = infos[i];
+            const frh_develop_spec &s = specs[i];
+            if (f.size < sizeof(frh_frame) || inf.size < sizeof(frh_frame_info) ||
+                s.size < offsetof(frh_develop_spec, cancel))
+                return fail(FRH_ERROR_INVALID_ARGUMENT, "descriptor size field too small");
+            // Refused before validation, which would copy their tables to the device: frames
+            // other than device-resident u16 Bayer mosaics at full size, black-level deltas,
+            // linearisation curves, tone, output curves, edge-aware filters, second outputs.
+            bool bayer = inf.cfa_width == 2 && inf.cfa_height == 2;
+            if (bayer) {
+                int cnt[4] = {0, 0, 0, 0};
+                for (int k = 0; k < 4; k++) cnt[std::min<int>(inf.cfa[k], 3)]++;
+                bayer = cnt[0] == 1 && cnt[1] == 2 && cnt[2] == 1;
+            }
+            if (!f.data_on_device || f.input != FRH_INPUT_U16 || f.cpp > 1 || f.linearization_len || !bayer ||
+                s.half_size)
+                return fail(FRH_ERROR_NOT_AVAILABLE, "frh_batch_develop_noalloc takes device-resident u16 Bayer "
+                                                     "mosaics at full size without a linearisation curve");
+            if (inf.black_row_deltas_len || inf.black_col_deltas_len || s.tone || s.curve ||
+                s.output2 != OUT_NONE || (s.edge && (s.edge->size < sizeof(frh_edge_spec) || !s.edge->ahd)))
+                return fail(FRH_ERROR_NOT_AVAILABLE, "frh_batch_develop_noalloc takes bilinear or AHD develops "
+                                                     "with one output, no black-level deltas, tone or curve");
+            Plan &p = plans[i];
+            int32_t rc = plan_develop(ctx, &f, &inf, &s, p);
+            if (rc) return rc;
+            if (!p.a.simple_cal || p.a.generic || p.a.edge_aware || p.in.staged)
+                return fail(FRH_ERROR_NOT_AVAILABLE, "batched frames need a black level of at most a 2x2 pattern");
+            if (!dsts[i]) return fail(FRH_ERROR_INVALID_ARGUMENT, "NULL output");
+            p.a.dst = dsts[i];
+            p.a.dst2 = nullptr;
+        }
+        std::vector<BatchFrame> bf;
+        std::vector<uint32_t> words;
+        Lists L;
+        build_lists(plans, ahd_rows, bilinear_rows, bf, words, L);
+        if (words.size() > NA_LIST_WORDS) return fail(FRH_ERROR_INTERNAL, "batch lists exceed the scratch array");
+        uint8_t *base = static_cast<uint8_t *>(ctx->noalloc_scratch);
+        BatchFrame *d_frames = reinterpret_cast<BatchFrame *>(base);
+        uint32_t *d_lists = reinterpret_cast<uint32_t *>(base + NA_FRAMES_BYTES);
+        uint32_t *d_counters = reinterpret_cast<uint32_t *>(base + NA_FRAMES_BYTES + NA_LISTS_BYTES);
+        static_assert(sizeof(BatchFrame) % 4 == 0, "descriptors are copied as words");
+        (void)hipSetDevice(ctx->device);
+        hipStream_t st = (hipStream_t)stream;
+        hipError_t e;
+        if ((e = put_words(reinterpret_cast<uint32_t *>(d_frames), reinterpret_cast<const uint32_t *>(bf.data()),
+                           bf.size() * sizeof(BatchFrame) / 4, st)) != hipSuccess ||
+            (e = put_words(d_lists, words.data(), words.size(), st)) != hipSuccess)
+            return hip_fail(e, "descriptor upload");
+        if ((e = enqueue_lists(ctx, L, BatchDev{d_frames, d_lists, d_counters}, st)) != hipSuccess)
+            return hip_fail(e, "batch launch");
+        return FRH_OK;
+    } catch (const std::bad_alloc &) {
+        return fail(FRH_ERROR_OUT_OF_MEMORY, "host allocation failed");
+    } catch (...) {
+        return fail(FRH_ERROR_INTERNAL, "unexpected exception");
+    }
+}

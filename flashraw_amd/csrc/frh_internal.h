@@ -86,6 +86,9 @@ struct frh_context {
     std::mutex stage_mu;
     void *h_stage = nullptr;
     uint64_t h_stage_bytes = 0;
+    // device address of the library's fixed scratch array on this context's device (taken once
+    // at creation; the array's host-side name is not a device address)
+    void *noalloc_scratch = nullptr;
 
     void *alloc(uint64_t bytes, hipError_t *err);
     void release(void *p, uint64_t bytes);
@@ -155,6 +158,8 @@ int32_t plan_develop(const frh_context *ctx, const frh_frame *f, const frh_frame
 frh_batch *batch_new(frh_context *ctx, uint32_t capacity);
 int32_t batch_assign(frh_batch *b, std::vector<Plan> &&plans, hipStream_t st);
 hipError_t batch_run(frh_batch *b, hipStream_t st);
+// The device address of the no-allocation batch's scratch array on the current device.
+hipError_t noalloc_scratch_address(void **ptr);
 
 }  // namespace frh
 

@@ -292,6 +292,9 @@ def lib():
             L.frh_context_table_cache.restype = i32
             L.frh_plan_kernel.argtypes = [vp]
             L.frh_plan_kernel.restype = i32
+            L.frh_batch_develop_noalloc.argtypes = [vp, u32, ctypes.POINTER(_Frame), ctypes.POINTER(_FrameInfo),
+                                                    ctypes.POINTER(_DevelopSpec), ctypes.POINTER(vp), u32, u32, vp]
+            L.frh_batch_develop_noalloc.restype = i32
             _lib = L
         return _lib
 
@@ -1127,6 +1130,24 @@ class GpuContext:
         frames developed into the device buffers `dst` by one persistent launch per demosaic
         kind (frh_batch_create)."""
         return Batch(self, jobs)
+
+    def batch_develop_noalloc(self, jobs, ahd_rows: int = 0, bilinear_rows: int = 0, stream=None) -> None:
+        """The batched develop of `jobs` ((frame, info, spec, dst) as for batch()) in one call that
+        allocates no device memory (frh_batch_develop_noalloc); asynchronous on `stream`."""
+        n = len(jobs)
+        frames, infos, specs = (_Frame * n)(), (_FrameInfo * n)(), (_DevelopSpec * n)()
+        dsts = (ctypes.c_void_p * n)()
+        keep = []
+        for k, (frame, info, spec, dst) in enumerate(jobs):
+            f, k1 = frame._c()
+            i, k2 = info._c()
+            s, k3 = spec._c()
+            frames[k], infos[k], specs[k] = f, i, s
+            dsts[k] = _device_ptr(dst)
+            keep += [k1, k2, k3]
+        check(lib().frh_batch_develop_noalloc(self._h, n, frames, infos, specs, dsts, ahd_rows, bilinear_rows,
+                                              self.stream_handle(stream)))
+        del keep
 
     def server(self, device_budget_bytes: int = 0, max_batch_frames: int = 0, max_inflight_batches: int = 0) -> "Server":
         """A serving loop on this context (frh_server_create)."""

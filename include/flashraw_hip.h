@@ -317,6 +317,22 @@ int32_t frh_batch_set_strip_rows(frh_batch *batch, uint32_t ahd_rows, uint32_t b
 /* Work items (strips and border groups) in the batch's lists. */
 uint64_t frh_batch_items(const frh_batch *batch);
 void frh_batch_destroy(frh_batch *batch);
+/* No counterpart in the reference. For callers that cannot allocate device memory (code being
+ * captured into a HIP graph, tight serving loops, harnesses that forbid hipMalloc): the batched
+ * develop of frh_batch_create + frh_batch_launch in one call that allocates nothing. Descriptors,
+ * work lists and list counters go to a fixed device array of the library (about 290 KB per GPU,
+ * held until the process ends), filled by copy launches on `stream` (NULL: the null stream)
+ * ahead of the list kernels; the call does not synchronise. At most 256 frames per call
+ * (FRH_ERROR_LIMIT_EXCEEDED beyond). Frames are device-resident u16 Bayer mosaics at full size
+ * with a black level of at most a 2x2 pattern, no deltas, linearisation, tone, curve or second
+ * output, bilinear or AHD (FRH_ERROR_NOT_AVAILABLE otherwise, refused before anything reaches the
+ * device). Strip heights ahd_rows / bilinear_rows (0: each frame's default). Every call on a GPU
+ * shares that GPU's array, which keeps the last call's contents (the counters are cleared by each
+ * call): calls that overlap on one GPU (different streams or threads) race; order them on one
+ * stream. Each frame's bytes equal its frh_batch_launch / single-frame develop. */
+int32_t frh_batch_develop_noalloc(frh_context *ctx, uint32_t n, const frh_frame *frames, const frh_frame_info *infos,
+                                  const frh_develop_spec *specs, void *const *dsts, uint32_t ahd_rows,
+                                  uint32_t bilinear_rows, void *stream);
 
 /* ---- CFA-plane DCT block denoiser ----
  * No reference counterpart as an entry point: the reference denoises developed linear RGB
