@@ -271,7 +271,43 @@ static void box_row(const float *a, size_t w, size_t r, float n, float *out) {
 #undef AT
 }
 
-static Plane box_mean(const Plane *f, size_t r) {
+/* Direct window sums in the order of the reference's GPU kernels (dn_box_v / dn_box_h,
+ * crates/flashraw-gpu/shaders/denoise.wgsl:191-218): per sample, the vertical sum over
+ * d = -r..r from 0, then the horizontal sum of those over d = -r..r from 0, divided by n. The
+ * reference's CPU path (box_mean below) slides running sums instead, whose rounding drifts along
+ * its segments; the two orders give different bits. */
+static Plane box_mean_direct(const Plane *f, size_t r) {
+    const size_t w = f->w, h = f->h;
+    const float n = (float)((2 * r + 1) * (2 * r + 1));
+    Plane vs = plane_new(w, h), out = plane_new(w, h);
+#pragma omp parallel for schedule(static)
+    for (size_t y = 0; y < h; y++)
+        for (size_t x = 0; x < w; x++) {
+            float s = 0.0f;
+            for (long d = -(long)r; d <= (long)r; d++) {
+                long yy = (long)y + d;
+                yy = yy < 0 ? 0 : (yy > (long)h - 1 ? (long)h - 1 : yy);
+                s += f->v[(size_t)yy * w + x];
+            }
+            vs.v[y * w + x] = s;
+        }
+#pragma omp parallel for schedule(static)
+    for (size_t y = 0; y < h; y++)
+        for (size_t x = 0; x < w; x++) {
+            float s = 0.0f;
+            for (long d = -(long)r; d <= (long)r; d++) {
+                long xx = (long)x + d;
+                xx = xx < 0 ? 0 : (xx > (long)w - 1 ? (long)w - 1 : xx);
+                s += vs.v[y * w + (size_t)xx];
+            }
+            out.v[y * w + x] = s / n;
+        }
+    plane_free(&vs);
+    return out;
+}
+
+static Plane box_mean(const Plane *f, size_t r, int direct) {
+    if (direct) return box_mean_direct(f, r);
     const size_t w = f->w, h = f->h;
     const float n = (float)((2 * r + 1) * (2 * r + 1));
     Plane out = plane_new(w, h);
@@ -813,6 +849,10 @@ static size_t chroma_radius(float s) {
 
 /* denoise::apply (mod.rs:75-130) on interleaved RGB of w x h, in place. */
 int fro_nr_apply(float *rgb, uint32_t wi, uint32_t hi, float luma, float chroma) {
+    return fro_nr_apply_order(rgb, wi, hi, luma, chroma, 0);
+}
+
+int fro_nr_apply_order(float *rgb, uint32_t wi, uint32_t hi, float luma, float chroma, int box_direct) {
     dct_init();
     luma = rclampf(luma, 0.0f, 1.0f);
     chroma = rclampf(chroma, 0.0f, 1.0f);
@@ -864,10 +904,10 @@ int fro_nr_apply(float *rgb, uint32_t wi, uint32_t hi, float luma, float chroma)
         Variance noise;
         if (have_ym) noise = pixel_variance(&ym);
         Plane gi = down(&y2);
-        Plane mean = box_mean(&gi, r);
+        Plane mean = box_mean(&gi, r, box_direct);
         Plane sq = plane_new(gi.w, gi.h);
         for (size_t k = 0; k < gi.w * gi.h; k++) sq.v[k] = gi.v[k] * gi.v[k];
-        Plane var = box_mean(&sq, r);
+        Plane var = box_mean(&sq, r, box_direct);
         for (size_t k = 0; k < var.w * var.h; k++) var.v[k] = rmaxf(var.v[k] - mean.v[k] * mean.v[k], 0.0f);
         Plane den = plane_new(gi.w, gi.h);
         for (size_t k = 0; k < den.w * den.h; k++) {
@@ -877,15 +917,15 @@ int fro_nr_apply(float *rgb, uint32_t wi, uint32_t hi, float luma, float chroma)
         Plane outp[2];
         for (int c = 0; c < 2; c++) {
             Plane ps = down(&dn[c]);
-            Plane mp = box_mean(&ps, r);
+            Plane mp = box_mean(&ps, r, box_direct);
             Plane prod = plane_new(gi.w, gi.h);
             for (size_t k = 0; k < gi.w * gi.h; k++) prod.v[k] = gi.v[k] * ps.v[k];
-            Plane a = box_mean(&prod, r);
+            Plane a = box_mean(&prod, r, box_direct);
             for (size_t k = 0; k < a.w * a.h; k++) a.v[k] = (a.v[k] - mean.v[k] * mp.v[k]) / den.v[k];
             Plane bb = plane_new(gi.w, gi.h);
             for (size_t k = 0; k < gi.w * gi.h; k++) bb.v[k] = mp.v[k] - a.v[k] * mean.v[k];
-            Plane mb = box_mean(&bb, r);
-            Plane ma = box_mean(&a, r);
+            Plane mb = box_mean(&bb, r, box_direct);
+            Plane ma = box_mean(&a, r, box_direct);
             outp[c] = up_affine(&ma, &mb, &y2);
             plane_free(&ps);
             plane_free(&mp);
@@ -947,8 +987,12 @@ void fro_nr_inverse_block(const float *coef, float *out) {
 }
 
 int fro_nr_box_mean(const float *in, uint32_t w, uint32_t h, uint32_t r, float *out) {
+    return fro_nr_box_mean_order(in, w, h, r, out, 0);
+}
+
+int fro_nr_box_mean_order(const float *in, uint32_t w, uint32_t h, uint32_t r, float *out, int direct) {
     Plane p = {w, h, (float *)in};
-    Plane o = box_mean(&p, r);
+    Plane o = box_mean(&p, r, direct);
     memcpy(out, o.v, (size_t)w * h * sizeof(float));
     plane_free(&o);
     return 0;

@@ -149,6 +149,9 @@ int fro_dct_denoise(const uint16_t *mosaic, uint32_t width, uint32_t height, con
 
 /* denoise::apply on interleaved RGB (3 floats per pixel), in place. */
 int fro_nr_apply(float *rgb, uint32_t w, uint32_t h, float luma, float chroma);
+/* The same with the guided filter's box means as direct window sums in the order of the
+ * reference's GPU kernels (box_direct 1) instead of the CPU path's running sums (0). */
+int fro_nr_apply_order(float *rgb, uint32_t w, uint32_t h, float luma, float chroma, int box_direct);
 /* Pyramid levels (level_count) and the noise estimate's block step (estimate_step, 0: none). */
 uint32_t fro_nr_level_count(uint32_t w, uint32_t h, uint32_t max);
 uint32_t fro_nr_estimate_step(uint32_t w, uint32_t h);
@@ -161,6 +164,7 @@ int fro_nr_model_from_stats(const float *stats, uint32_t count, uint32_t *bins, 
 void fro_nr_dct_block(const float *p, uint32_t w, uint32_t x0, uint32_t y0, float *out);
 void fro_nr_inverse_block(const float *coef, float *out);
 int fro_nr_box_mean(const float *in, uint32_t w, uint32_t h, uint32_t r, float *out);
+int fro_nr_box_mean_order(const float *in, uint32_t w, uint32_t h, uint32_t r, float *out, int direct);
 int fro_nr_down(const float *in, uint32_t w, uint32_t h, float *out);
 int fro_nr_add_up(float *d, uint32_t w, uint32_t h, const float *coarse);
 int fro_nr_dct_pass(const float *src, const float *guide, const float *pilot, uint32_t w, uint32_t h,

@@ -135,6 +135,10 @@ def lib():
         P, U, F = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_float
         L.fro_nr_apply.restype = ctypes.c_int
         L.fro_nr_apply.argtypes = [P, U, U, F, F]
+        L.fro_nr_apply_order.restype = ctypes.c_int
+        L.fro_nr_apply_order.argtypes = [P, U, U, F, F, ctypes.c_int]
+        L.fro_nr_box_mean_order.restype = ctypes.c_int
+        L.fro_nr_box_mean_order.argtypes = [P, U, U, U, P, ctypes.c_int]
         L.fro_nr_level_count.restype = U
         L.fro_nr_level_count.argtypes = [U, U, U]
         L.fro_nr_estimate_step.restype = U
@@ -352,11 +356,13 @@ def _f32(a):
     return np.ascontiguousarray(a, dtype=np.float32)
 
 
-def nr_apply(rgb, luma, chroma):
-    """denoise::apply on a copy of linear RGB (h, w, 3) float32."""
+def nr_apply(rgb, luma, chroma, box_direct=False):
+    """denoise::apply on a copy of linear RGB (h, w, 3) float32. box_direct: the guided filter's
+    box means as direct window sums in the reference GPU kernels' order (denoise.wgsl:191-218)
+    instead of the CPU path's running sums (mod.rs:280-335)."""
     out = _f32(rgb).copy()
     h, w = out.shape[:2]
-    if lib().fro_nr_apply(out.ctypes.data, w, h, luma, chroma) != 0:
+    if lib().fro_nr_apply_order(out.ctypes.data, w, h, luma, chroma, int(box_direct)) != 0:
         raise ValueError("oracle rejected the noise reduction")
     return out
 
@@ -407,11 +413,11 @@ def nr_inverse_block(coef):
     return out.reshape(8, 8)
 
 
-def nr_box_mean(p, r):
+def nr_box_mean(p, r, direct=False):
     p = _f32(p)
     h, w = p.shape
     out = np.zeros_like(p)
-    lib().fro_nr_box_mean(p.ctypes.data, w, h, r, out.ctypes.data)
+    lib().fro_nr_box_mean_order(p.ctypes.data, w, h, r, out.ctypes.data, int(direct))
     return out
 
 

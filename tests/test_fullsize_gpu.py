@@ -11,7 +11,9 @@
   develop of its input (of the device denoiser's output for denoised requests) and its develop
   alone.
 - The luma / chroma noise reduction at 24 MP and in rows wider than one gather chunk, within the
-  reference's GPU-vs-CPU contract (equivalence.rs:960-1003).
+  reference's GPU-vs-CPU contract (equivalence.rs:960-1003) against the oracle whose guided-filter
+  box means are summed as the reference's GPU kernels sum them, and within a stated bound against
+  the CPU path's running sums.
 """
 import numpy as np
 import pytest
@@ -144,13 +146,25 @@ def _nr_case(ctx, w, h, luma, chroma, mos=None):
     img = ctx.run(F.GpuFrame.from_u16(mos, w, h), info, F.GpuOp.Develop(spec)).image
     lin = img.download()
     got = ctx.denoise(img, F.denoise_spec(w, h, luma, chroma)).image.download()
-    want = O.nr_apply(lin, luma, chroma)
-    mean, p999, mx = _nr_stats(got, want)
-    moved = _nr_stats(want, lin)[0]
-    print(f"{w}x{h} luma {luma} chroma {chroma}: mean {mean:.2e} p99.9 {p999:.2e} max {mx:.2e} (moves {moved:.2e})")
     assert got.shape == (h, w, 3)
+    # the oracle with the guided filter's box means summed as the reference's GPU kernels sum them
+    # (direct window sums, denoise.wgsl:191-218): the reference's own contract
+    direct = O.nr_apply(lin, luma, chroma, box_direct=True)
+    mean, p999, mx = _nr_stats(got, direct)
+    moved = _nr_stats(direct, lin)[0]
+    print(f"{w}x{h} luma {luma} chroma {chroma} vs direct-sum oracle: mean {mean:.2e} p99.9 {p999:.2e} "
+          f"max {mx:.2e} (moves {moved:.2e})")
     assert mean <= 2e-6 and p999 <= 1e-4, (w, h, luma, chroma, mean, p999)
     assert moved > 1e-3
+    # the oracle with the CPU path's running sums (mod.rs:280-335), whose rounding drifts along row
+    # segments of w / 8 samples at half resolution: the reference's 99.9th-percentile bound, and a
+    # mean bound of 1.5x the 4.3e-6 measured at 24 MP (the reference states 2e-6 for its 347x229
+    # test frame, whose segments are 43 samples long)
+    want = O.nr_apply(lin, luma, chroma)
+    mean, p999, mx = _nr_stats(got, want)
+    print(f"{w}x{h} luma {luma} chroma {chroma} vs running-sum oracle: mean {mean:.2e} p99.9 {p999:.2e} "
+          f"max {mx:.2e}")
+    assert mean <= 6.5e-6 and p999 <= 1e-4, (w, h, luma, chroma, mean, p999)
 
 
 @pytest.mark.parametrize("w,h", [(2500, 200), (4100, 96)])

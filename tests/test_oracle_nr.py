@@ -55,6 +55,44 @@ def test_box_mean_is_the_uniform_filter(r, shape):
     assert np.abs(got - want).max() < 2e-6
 
 
+@pytest.mark.parametrize("r", [1, 3])
+def test_box_mean_direct_order_is_the_gpu_kernels_sum(r):
+    """The direct-sum order (the reference GPU's dn_box_v / dn_box_h, denoise.wgsl:191-218):
+    vertical sums over d = -r..r from 0, then horizontal sums of those, divided by (2r + 1)^2 --
+    restated sample by sample in float32, bit for bit; within float error of the uniform filter."""
+    rng = np.random.default_rng(7 + r)
+    p = rng.random((37, 53)).astype(F32)
+    h, w = p.shape
+    got = O.nr_box_mean(p, r, direct=True)
+    vs = np.zeros_like(p)
+    for d in range(-r, r + 1):
+        vs = (vs + p[np.clip(np.arange(h) + d, 0, h - 1), :]).astype(F32)
+    hs = np.zeros_like(p)
+    for d in range(-r, r + 1):
+        hs = (hs + vs[:, np.clip(np.arange(w) + d, 0, w - 1)]).astype(F32)
+    want = (hs / F32((2 * r + 1) ** 2)).astype(F32)
+    assert np.array_equal(got, want)
+    ref = scipy.ndimage.uniform_filter(p.astype(np.float64), size=2 * r + 1, mode="nearest")
+    assert np.abs(got - ref).max() < 2e-6
+
+
+def test_box_mean_running_sums_drift_along_long_rows():
+    """The CPU path's running sums (mod.rs:280-335) drift along a row segment while direct sums do
+    not: on a long row of values around 0.5 the running-sum error grows with the segment length.
+    The two orders therefore differ by more on wide images, which is why the full-size GPU
+    comparison states its mean bound against both."""
+    rng = np.random.default_rng(3)
+    w = 4096
+    p = (0.5 + 0.1 * rng.standard_normal((4, w))).astype(F32)
+    ref = scipy.ndimage.uniform_filter(p.astype(np.float64), size=5, mode="nearest")
+    run = np.abs(O.nr_box_mean(p, 2) - ref)
+    direct = np.abs(O.nr_box_mean(p, 2, direct=True) - ref)
+    seg = w // 4
+    early, late = run[:, :64].mean(), run[:, seg - 64:seg].mean()
+    assert late > 2 * early  # the running sum's error grows along its segment
+    assert direct.max() < 1e-6 and direct[:, seg - 64:seg].mean() < late / 4
+
+
 @pytest.mark.parametrize("shape", [(10, 12), (11, 13), (1, 7), (48, 49)])
 def test_down_is_the_2x2_mean(shape):
     rng = np.random.default_rng(1)
