@@ -34,9 +34,11 @@
 
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
+#include "dn_kron.h"
 #include "frh_internal.h"
 
 namespace frh {
@@ -439,6 +441,44 @@ extern "C" int32_t frh_denoise_cfa(frh_context *ctx, const frh_dn_spec *spec, co
             if ((reinterpret_cast<uintptr_t>(src) & 15) || (stride % 8) ||
                 (dst && (reinterpret_cast<uintptr_t>(dst) & 3)) || (dst_f32 && (reinterpret_cast<uintptr_t>(dst_f32) & 7)))
                 return fail(FRH_ERROR_INVALID_ARGUMENT, "source must be 16-byte aligned with a row stride that is a multiple of 8");
+            const char *impl = getenv("FLASHRAW_DN_IMPL");  // "walker": the separable 16x16x16 kernel
+            const bool walker = impl && strcmp(impl, "walker") == 0;
+            if (spec->precision != FRH_DN_F16X3 && !walker) {
+                // the kernel's stores address the outputs with 32-bit byte offsets
+                const uint64_t samples = (uint64_t)H * stride;
+                if ((dst && samples * 2 >= (1ull << 31)) || (dst_f32 && samples * 4 >= (1ull << 31)))
+                    return fail(FRH_ERROR_NOT_AVAILABLE, "the MFMA denoiser's outputs must be below 2 GiB");
+                // f16 / bf16: the Kronecker-form kernel (dn_kron.hip)
+                const bool bf = spec->precision == FRH_DN_BF16;
+                float d[64];
+                dct_matrix(d);
+                std::vector<uint16_t> kf, ki;
+                dn_kron_tables(d, bf, kf, ki);
+                int32_t rc = FRH_OK;
+                DnkArgs k{};
+                k.kf = static_cast<const uint16_t *>(device_table(ctx, kf.data(), kf.size() * 2, &rc));
+                if (rc) return rc;
+                k.ki = static_cast<const uint16_t *>(device_table(ctx, ki.data(), ki.size() * 2, &rc));
+                if (rc) return rc;
+                k.src = static_cast<const uint16_t *>(src);
+                k.dst = static_cast<uint16_t *>(dst);
+                k.dst_f = dst_f32;
+                k.stride = stride;
+                k.pw = (int)pw;
+                k.ph = (int)ph;
+                k.plx = (int)pw - 8;
+                k.ply = (int)ph - 8;
+                k.strips = (int)((pw + DNK_STRIP - 1) / DNK_STRIP);
+                k.nsteps = (int)((ph + 7) / 8);
+                k.seg_steps = spec->seg_rows ? (int)spec->seg_rows : dn_kron_seg_steps(k.strips, k.nsteps, ctx->num_cus);
+                k.segs = (k.nsteps + k.seg_steps - 1) / k.seg_steps;
+                for (int i = 0; i < 4; i++) k.black[i] = spec->black[i];
+                const float t = spec->threshold / 16.0f;  // the forward's coefficients are scaled by 1/16
+                k.t2a = t * t * spec->sig_a;
+                k.t2b = t * t * spec->sig_b;
+                if ((e = launch_dn_kron(k, bf, st)) != hipSuccess) return hip_fail(e, "denoise launch");
+                return FRH_OK;
+            }
             DnArgs a{};
             a.src = static_cast<const uint16_t *>(src);
             a.dst = static_cast<uint16_t *>(dst);
