@@ -450,16 +450,24 @@ extern "C" int32_t frh_denoise_cfa(frh_context *ctx, const frh_dn_spec *spec, co
                     return fail(FRH_ERROR_NOT_AVAILABLE, "the MFMA denoiser's outputs must be below 2 GiB");
                 // f16 / bf16: the Kronecker-form kernel (dn_kron.hip)
                 const bool bf = spec->precision == FRH_DN_BF16;
-                float d[64];
-                dct_matrix(d);
-                std::vector<uint16_t> kf, ki;
-                dn_kron_tables(d, bf, kf, ki);
+                // the forward and inverse operand tables, one device allocation per operand type,
+                // looked up by a short key (the tables depend only on the operand type)
+                const uint32_t key[2] = {0x4b4e4444u /* "DDNK" */, bf ? 1u : 0u};
                 int32_t rc = FRH_OK;
+                const void *tabs = cached_table(ctx, 2, key, sizeof key, [&](std::vector<uint8_t> &dev) {
+                    float d[64];
+                    dct_matrix(d);
+                    std::vector<uint16_t> kf, ki;
+                    dn_kron_tables(d, bf, kf, ki);
+                    dev.resize((kf.size() + ki.size()) * 2);
+                    std::memcpy(dev.data(), kf.data(), kf.size() * 2);
+                    std::memcpy(dev.data() + kf.size() * 2, ki.data(), ki.size() * 2);
+                    return true;
+                }, &rc);
+                if (rc) return rc;
                 DnkArgs k{};
-                k.kf = static_cast<const uint16_t *>(device_table(ctx, kf.data(), kf.size() * 2, &rc));
-                if (rc) return rc;
-                k.ki = static_cast<const uint16_t *>(device_table(ctx, ki.data(), ki.size() * 2, &rc));
-                if (rc) return rc;
+                k.kf = static_cast<const uint16_t *>(tabs);
+                k.ki = k.kf + 64 * 64;
                 k.src = static_cast<const uint16_t *>(src);
                 k.dst = static_cast<uint16_t *>(dst);
                 k.dst_f = dst_f32;

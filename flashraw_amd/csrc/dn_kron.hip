@@ -336,9 +336,10 @@ __global__ __launch_bounds__(256) void k_dn_kron(DnkArgs A) {
         u4 wB[2], wC[2];
         planes(nB, wB);
         planes(nC, wC);
-        // (past the last step these are clamped rows, never used)
-        nB = issue(y0 + 12);
-        nC = issue(y0 + 16);
+        // (after the last step: this step's rows again, already in the caches, never used)
+        const bool more = s + 1 < nsteps;
+        nB = issue(more ? y0 + 12 : y0 + 4);
+        nC = issue(more ? y0 + 16 : y0 + 8);
         uint32_t off2[2];
         float offf[2];
         step_offset(wB, off2, offf);

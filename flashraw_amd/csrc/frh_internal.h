@@ -71,8 +71,9 @@ struct frh_context {
     // strip heights found by frh_plan_autotune, by (AHD, developed width, rows, fast path)
     std::map<std::tuple<int, uint32_t, uint32_t, int>, uint32_t> tuned;
     // device copies of small per-frame tables (calibration, CFA sites, black-level deltas,
-    // linearisation curves, tone and curve tables: kind 0) and of edge-aware tap lists (kind 1),
-    // by (kind, hash of the key bytes, key length); bounded by TABLE_CACHE_BYTES / _ENTRIES
+    // linearisation curves, tone and curve tables: kind 0), of edge-aware tap lists (kind 1) and
+    // of the Kronecker denoiser's operand tables (kind 2), by (kind, hash of the key bytes, key
+    // length); bounded by TABLE_CACHE_BYTES / _ENTRIES
     std::multimap<std::tuple<int, uint64_t, uint64_t>, frh::TableEntry> tables;
     uint64_t table_bytes = 0, table_tick = 0;
     std::mutex mu;

@@ -10,6 +10,7 @@
  * Gaussian read noise of `read_sigma` DN, plus black, rounded and clipped to [0, white].
  * The parameters of every object come from one SplitMix64 stream of `seed`; the noise of pixel i
  * from a counter-based stream of (seed, i), so the mosaic does not depend on the thread count.
+ * frs_scene_clean writes the same mosaic before noise and rounding (black + signal, f32 DN).
  */
 #include <math.h>
 #include <omp.h>
@@ -79,10 +80,12 @@ static double gauss(uint64_t *s) {
     return sqrt(-2.0 * log(u1 > 1e-300 ? u1 : 1e-300)) * cos(2.0 * M_PI * u2);
 }
 
-/* cfa: colour (0 R, 1 G, 2 B) of sensor parity (y & 1) * 2 + (x & 1). Returns 0 or -1. */
-int frs_scene(uint16_t *out, uint32_t w, uint32_t h, const uint8_t cfa[4], float black, float white, float gain,
-              float read_sigma, uint32_t n_edges, uint32_t n_discs, uint32_t n_stripes, uint64_t seed, int threads) {
-    if (!out || w == 0 || h == 0 || !(white > black) || gain <= 0.0f) return -1;
+/* cfa: colour (0 R, 1 G, 2 B) of sensor parity (y & 1) * 2 + (x & 1). One of out (noisy,
+ * rounded) and clean (noise-free, f32) is written. Returns 0 or -1. */
+static int make_scene(uint16_t *out, float *clean, uint32_t w, uint32_t h, const uint8_t cfa[4], float black, float white,
+                      float gain, float read_sigma, uint32_t n_edges, uint32_t n_discs, uint32_t n_stripes, uint64_t seed,
+                      int threads) {
+    if ((!out && !clean) || w == 0 || h == 0 || !(white > black) || gain <= 0.0f) return -1;
     int nt = threads > 0 ? threads : omp_get_max_threads();
     uint64_t rng = seed * 0x2545f4914f6cdd1dull + 1;
     /* smooth gradients */
@@ -199,6 +202,10 @@ int frs_scene(uint16_t *out, uint32_t w, uint32_t h, const uint8_t cfa[4], float
             int c = cfa[(y & 1) * 2 + (x & 1)];
             double s01 = 0.05 + 0.9 * (scene[i * 3 + c] - lo) / span;
             double sig = s01 * range;
+            if (clean) {
+                clean[i] = (float)(black + sig);
+                continue;
+            }
             uint64_t st = seed ^ (i * 0xd1b54a32d192ed03ull) ^ 0x8bb84b93962eacc9ull;
             double e = poisson(&st, sig * gain) / gain;
             double v = black + e + read_sigma * gauss(&st);
@@ -212,4 +219,16 @@ int frs_scene(uint16_t *out, uint32_t w, uint32_t h, const uint8_t cfa[4], float
     free(stripes);
     free(scene);
     return 0;
+}
+
+int frs_scene(uint16_t *out, uint32_t w, uint32_t h, const uint8_t cfa[4], float black, float white, float gain,
+              float read_sigma, uint32_t n_edges, uint32_t n_discs, uint32_t n_stripes, uint64_t seed, int threads) {
+    if (!out) return -1;
+    return make_scene(out, NULL, w, h, cfa, black, white, gain, read_sigma, n_edges, n_discs, n_stripes, seed, threads);
+}
+
+int frs_scene_clean(float *clean, uint32_t w, uint32_t h, const uint8_t cfa[4], float black, float white, float gain,
+                    uint32_t n_edges, uint32_t n_discs, uint32_t n_stripes, uint64_t seed, int threads) {
+    if (!clean) return -1;
+    return make_scene(NULL, clean, w, h, cfa, black, white, gain, 0.0f, n_edges, n_discs, n_stripes, seed, threads);
 }

@@ -27,6 +27,10 @@ def _L():
         L.frs_scene.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_float,
                                 ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_uint32, ctypes.c_uint32,
                                 ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int]
+        L.frs_scene_clean.restype = ctypes.c_int
+        L.frs_scene_clean.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p,
+                                      ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_uint32,
+                                      ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -38,6 +42,18 @@ def mosaic(width, height, cfa="RGGB", black=512.0, white=16383.0, gain=0.5, read
     c = np.asarray(BAYER[cfa] if isinstance(cfa, str) else cfa, dtype=np.uint8)
     rc = _L().frs_scene(out.ctypes.data, width, height, c.ctypes.data, black, white, gain, read_sigma, edges, discs,
                         stripes, seed, threads)
+    if rc != 0:
+        raise ValueError("invalid scene parameters")
+    return out
+
+
+def clean_mosaic(width, height, cfa="RGGB", black=512.0, white=16383.0, gain=0.5, edges=200, discs=50, stripes=0,
+                 seed=0, threads=0) -> np.ndarray:
+    """The mosaic of the same seeded scene before noise and rounding: black + signal, f32 DN."""
+    out = np.empty((height, width), dtype=np.float32)
+    c = np.asarray(BAYER[cfa] if isinstance(cfa, str) else cfa, dtype=np.uint8)
+    rc = _L().frs_scene_clean(out.ctypes.data, width, height, c.ctypes.data, black, white, gain, edges, discs,
+                              stripes, seed, threads)
     if rc != 0:
         raise ValueError("invalid scene parameters")
     return out
@@ -56,6 +72,11 @@ def frame_45mp_stripes(seed=1, **kw):
 def frame_24mp_noisy(seed=2, **kw):
     """The 6048x4024 RGGB frame with read sigma 12 DN (denoise workload)."""
     return mosaic(6048, 4024, "RGGB", 512.0, 16383.0, 0.5, 12.0, 200, 50, 0, seed, **kw)
+
+
+def frame_24mp_clean(seed=2, **kw):
+    """The denoise workload's scene before noise (f32 DN, black included)."""
+    return clean_mosaic(6048, 4024, "RGGB", 512.0, 16383.0, 0.5, 200, 50, 0, seed, **kw)
 
 
 # Develop parameters of the workloads: WB (2.1, 1, 1.6) and a row-normalised camera->sRGB
