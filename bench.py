@@ -555,6 +555,7 @@ def serve_mode(args):
         reqs = synth.serve_requests(rate, args.duration, seed=int(rate))
         mp = sum(pool[r["m"]]["w"] * pool[r["m"]]["h"] for r in reqs) / 1e6
         lat, done_t, bsz, st, parts = run_server(reqs)
+        print(f"rate {rate:g}: {len(reqs)} requests served", file=sys.stderr, flush=True)
         entry = {"rate": rate, "requests": len(reqs), "mp": round(mp, 1),
                  "served": {"mps": round(mp / max(done_t), 1), "p50_ms": round(float(np.percentile(lat, 50)), 2),
                             "p99_ms": round(float(np.percentile(lat, 99)), 2),
