@@ -57,11 +57,20 @@ struct Walk {
     uint32_t pf[2];         // raw pairs fetched two rows ahead, slot = row parity (consumed in place)
 };
 
+// Per-lane constants of a strip: its columns x0, x0 + 1 are inside the frame (in0, in1) and are
+// output columns of the strip (st0, st1); pair_even[p]: at trip position parity p the lane's two
+// output pixels start at an even destination pixel (one 8-byte store).
+struct StripCols {
+    bool in0, in1, st0, st1;
+    bool pair_even[2];
+    float inf;  // +inf (opaque_inf)
+};
+
 // One walker row at trip position T: load mosaic row R (parity T & 1) and advance every stage.
-template <int RP, int T, bool FAST, bool HOT>
+template <int RP, int T, bool FAST, bool HOT, bool TAME>
 __device__ __forceinline__ void walk_row(const DevArgs &a, const Lds &s, const float *wt, Walk &W, int R, int xs,
-                                         int lane, int out_row0, int out_row1, int out_x0, int out_x1, bool in0,
-                                         bool in1, __amdgpu_buffer_rsrc_t rs, __amdgpu_buffer_rsrc_t rd) {
+                                         int lane, int out_row0, int out_row1, const StripCols &C,
+                                         __amdgpu_buffer_rsrc_t rs, __amdgpu_buffer_rsrc_t rd) {
     constexpr int PY = T & 1;
     // ring slots
     constexpr int M0 = T % 6, M1 = (T + 5) % 6, M2 = (T + 4) % 6, M3 = (T + 3) % 6, M4 = (T + 2) % 6,
@@ -78,9 +87,9 @@ __device__ __forceinline__ void walk_row(const DevArgs &a, const Lds &s, const f
         const uint32_t raw = W.pf[PY];
         W.pf[PY] = fetch_pair<HOT>(a, rs, x0, R + 2);
         float n0, n1;
-        norm_pair<PY, HOT>(a, s, raw, x0, R, R >= 0 && R < (int)a.height, in0, in1, n0, n1);
-        W.m[M0][0] = fminf(n0 * a.wb[site<RP>(PY, 0)], 1.0f);
-        W.m[M0][1] = fminf(n1 * a.wb[site<RP>(PY, 1)], 1.0f);
+        norm_pair<PY, HOT>(a, s, raw, x0, R, R >= 0 && R < (int)a.height, C.in0, C.in1, n0, n1);
+        W.m[M0][0] = fmin_nn<TAME>(n0 * a.wb[site<RP>(PY, 0)], 1.0f, C.inf);
+        W.m[M0][1] = fmin_nn<TAME>(n1 * a.wb[site<RP>(PY, 1)], 1.0f, C.inf);
     }
 
     // ---- green candidates of row R-2 (ahd.rs:225-244) ----
@@ -97,8 +106,17 @@ __device__ __forceinline__ void walk_row(const DevArgs &a, const Lds &s, const f
                 const float l = k == 0 ? po : me, r = k == 0 ? mo : ne;
                 const float l2 = k == 0 ? pe : po, r2 = k == 0 ? ne : no;
                 const float u = W.m[M3][k], dn = W.m[M1][k], u2 = W.m[M4][k], d2 = W.m[M0][k];
-                gh = fminf(fmaxf((l + r) * 0.5f + (2.0f * c - l2 - r2) * 0.25f, fminf(l, r)), fmaxf(l, r));
-                gv = fminf(fmaxf((u + dn) * 0.5f + (2.0f * c - u2 - d2) * 0.25f, fminf(u, dn)), fmaxf(u, dn));
+                const float eh = (l + r) * 0.5f + (2.0f * c - l2 - r2) * 0.25f;
+                const float ev = (u + dn) * 0.5f + (2.0f * c - u2 - d2) * 0.25f;
+                if (TAME) {
+                    // clamp(e, min(l, r), max(l, r)) is the median of e, l, r (no NaN; no -0 arises
+                    // in a tame frame, so the choice between equal zeros cannot differ)
+                    gh = __builtin_amdgcn_fmed3f(eh, l, r);
+                    gv = __builtin_amdgcn_fmed3f(ev, u, dn);
+                } else {
+                    gh = fminf(fmaxf(eh, fminf(l, r)), fmaxf(l, r));
+                    gv = fminf(fmaxf(ev, fminf(u, dn)), fmaxf(u, dn));
+                }
             }
             W.g[G2][0][k] = gh;
             W.g[G2][1][k] = gv;
@@ -117,7 +135,7 @@ __device__ __forceinline__ void walk_row(const DevArgs &a, const Lds &s, const f
 #pragma unroll
     for (int k = 0; k < 2; k++) {
         const float cm = W.m[M3][k];
-        const float sm = sqrt_max0(cm);  // the measured channel, the same in both candidates
+        const float sm = sqrt_m0<TAME>(cm, C.inf);  // the measured channel, the same in both candidates
         const int col = site<RP>(PL, k);
 #pragma unroll
         for (int d = 0; d < 2; d++) {
@@ -146,15 +164,15 @@ __device__ __forceinline__ void walk_row(const DevArgs &a, const Lds &s, const f
             float sr, sg, sb;
             if (col == 0) {
                 sr = sm;
-                sg = sqrt_max0(gg);
-                sb = sqrt_max0(blue);
+                sg = sqrt_m0<TAME>(gg, C.inf);
+                sb = sqrt_m0<TAME>(blue, C.inf);
             } else if (col == 1) {
-                sr = sqrt_max0(red);
+                sr = sqrt_m0<TAME>(red, C.inf);
                 sg = sm;
-                sb = sqrt_max0(blue);
+                sb = sqrt_m0<TAME>(blue, C.inf);
             } else {
-                sr = sqrt_max0(red);
-                sg = sqrt_max0(gg);
+                sr = sqrt_m0<TAME>(red, C.inf);
+                sg = sqrt_m0<TAME>(gg, C.inf);
                 sb = sm;
             }
             W.lab[LQ][d][0][k] = (sr + 2.0f * sg + sb) * 0.25f;
@@ -217,8 +235,10 @@ __device__ __forceinline__ void walk_row(const DevArgs &a, const Lds &s, const f
         // thresholds: the smaller of the largest steps along the row in H (left, right) and along
         // the column in V (up, down)
         const Step hl = nb[0][k][3], hrr = nb[0][k][4], vu = nb[1][k][1], vd = nb[1][k][6];
-        const float el = fminf(fmaxf(hl.l, hrr.l), fmaxf(vu.l, vd.l));
-        const float ec = fminf(fmaxf(hl.c, hrr.c), fmaxf(vu.c, vd.c));
+        const float el =
+            fmin_nn<TAME>(fmax_nn<TAME>(hl.l, hrr.l, C.inf), fmax_nn<TAME>(vu.l, vd.l, C.inf), C.inf);
+        const float ec =
+            fmin_nn<TAME>(fmax_nn<TAME>(hl.c, hrr.c, C.inf), fmax_nn<TAME>(vu.c, vd.c, C.inf), C.inf);
         uint32_t ch = 0, cv = 0;
 #pragma unroll
         for (int n = 0; n < 8; n++) {
@@ -237,54 +257,67 @@ __device__ __forceinline__ void walk_row(const DevArgs &a, const Lds &s, const f
     // would hand its neighbour a 0)
     const uint32_t sums[2] = {from_left_u(v1) + v0 + v1, v0 + v1 + from_right_u(v0)};
     const int ro = R - 5;
-    if (ro >= out_row0 && ro < out_row1 && lane >= 3 && lane <= 60) {
-        constexpr int PO = PY ^ 1;
+    constexpr int PO = PY ^ 1;
+    auto blend = [&](int k, float &r, float &g, float &b) {
+        // w = sh / (sh + sv), 0.5 when both are 0 (ahd.rs:388-389), read from the exact table
+        const float w = wt[(sums[k] & 0xffffu) * 73u + (sums[k] >> 16)];
+        const float cm = W.m[M5][k];
+        const int col = site<RP>(PO, k);
+        float rgb[2][3];
+#pragma unroll
+        for (int d = 0; d < 2; d++) {
+            const float u0 = W.u[U5][d][k][0], u1 = W.u[U5][d][k][1];
+            rgb[d][0] = col == 0 ? cm : (col == 1 ? u0 : u1);
+            rgb[d][1] = col == 1 ? cm : u0;
+            rgb[d][2] = col == 2 ? cm : u1;
+        }
+        const float iw = 1.0f - w;
+        r = w * rgb[0][0] + iw * rgb[1][0];
+        g = w * rgb[0][1] + iw * rgb[1][1];
+        b = w * rgb[0][2] + iw * rgb[1][2];
+    };
+    if (FAST) {
+        // Every lane finishes its two pixels (the halo lanes' counts are still 0..8, so their table
+        // and code indexes stay in range), and the stores of pixels outside the strip's output, or
+        // of rows outside it (pipeline fill, the rows past the strip's end), address past the
+        // destination resource, where the hardware drops them: no branch splits the trip.
+        uint32_t o[2];
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            float r, g, b;
+            blend(k, r, g, b);
+            o[k] = TAME ? finish_rgba8_scaled(a, s, r, g, b) : finish_rgba8_fast(a, s, r, g, b, true);
+        }
+        const bool rv = ro >= out_row0 && ro < out_row1;
+        const uint32_t off = ((uint32_t)(ro - (int)a.crop_y) * a.dst_w + (uint32_t)(x0 - (int)a.crop_x)) * 4u;
+        const bool s0 = rv && C.st0, s1 = rv && C.st1, pe = C.pair_even[PY];
+        constexpr uint32_t PAST = 0xfffffff0u;
+        typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+        u32x2 v;
+        v.x = o[0];
+        v.y = o[1];
+        __builtin_amdgcn_raw_buffer_store_b64(v, rd, (pe && s0 && s1) ? (int)off : (int)PAST, 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b32(o[0], rd, (s0 && !(pe && s1)) ? (int)off : (int)PAST, 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b32(o[1], rd, (s1 && !(pe && s0)) ? (int)(off + 4u) : (int)PAST, 0, 0);
+    } else if (ro >= out_row0 && ro < out_row1 && lane >= 3 && lane <= 60) {
         float o[2][3];
 #pragma unroll
         for (int k = 0; k < 2; k++) {
-            // w = sh / (sh + sv), 0.5 when both are 0 (ahd.rs:388-389), read from the exact table
-            const float w = wt[(sums[k] & 0xffffu) * 73u + (sums[k] >> 16)];
-            const float cm = W.m[M5][k];
-            const int col = site<RP>(PO, k);
-            float rgb[2][3];
-#pragma unroll
-            for (int d = 0; d < 2; d++) {
-                const float u0 = W.u[U5][d][k][0], u1 = W.u[U5][d][k][1];
-                rgb[d][0] = col == 0 ? cm : (col == 1 ? u0 : u1);
-                rgb[d][1] = col == 1 ? cm : u0;
-                rgb[d][2] = col == 2 ? cm : u1;
-            }
-            const float iw = 1.0f - w;
-            const float r = w * rgb[0][0] + iw * rgb[1][0];
-            const float g = w * rgb[0][1] + iw * rgb[1][1];
-            const float b = w * rgb[0][2] + iw * rgb[1][2];
-            if (FAST) {
-                o[k][0] = __uint_as_float(finish_rgba8_fast(a, s, r, g, b, true));
-            } else {
-                finish(a, r, g, b, true, o[k][0], o[k][1], o[k][2]);
-            }
+            float r, g, b;
+            blend(k, r, g, b);
+            finish(a, r, g, b, true, o[k][0], o[k][1], o[k][2]);
         }
         const int dx = x0 - (int)a.crop_x, dy = ro - (int)a.crop_y;
-        const bool st0 = x0 >= out_x0 && x0 < out_x1, st1 = x0 + 1 >= out_x0 && x0 + 1 < out_x1;
-        if (FAST) {
-            const uint32_t w0 = __float_as_uint(o[0][0]), w1 = __float_as_uint(o[1][0]);
-            if (st0 && st1) {
-                store_rgba8_pair(a, rd, (uint32_t)dx, (uint32_t)dy, w0, w1, 2);
-            } else if (st0) {
-                store_rgba8_pair(a, rd, (uint32_t)dx, (uint32_t)dy, w0, 0, 1);
-            } else if (st1) {
-                store_rgba8_pair(a, rd, (uint32_t)dx + 1, (uint32_t)dy, w1, 0, 1);
-            }
-        } else {
-            if (st0) store_both<false>(a, s, (uint32_t)dx, (uint32_t)dy, o[0][0], o[0][1], o[0][2]);
-            if (st1) store_both<false>(a, s, (uint32_t)dx + 1, (uint32_t)dy, o[1][0], o[1][1], o[1][2]);
-        }
+        if (C.st0) store_both<false>(a, s, (uint32_t)dx, (uint32_t)dy, o[0][0], o[0][1], o[0][2]);
+        if (C.st1) store_both<false>(a, s, (uint32_t)dx + 1, (uint32_t)dy, o[1][0], o[1][1], o[1][2]);
     }
 }
 
-// One wave's strip: mosaic row R runs from row0 - 6 (even) until the last output row
-// out_row1 - 1 = R - 5, in trips of six rows (prefetch slot = row parity).
-template <int RP, bool FAST, bool HOT>
+// One wave's strip: mosaic row R runs from row0 - 6 (even) past the last output row
+// out_row1 - 1 = R - 5, in whole trips of six rows (prefetch slot = row parity), so a trip is
+// straight-line code the compiler schedules as one block; the rows walked past the strip's end
+// read clamped rows and store nothing.
+template <int RP, bool FAST, bool HOT, bool TAME>
 __device__ __forceinline__ void walk_strip(const DevArgs &a, const Lds &s, const float *wt, int xs, int lane,
                                            int row0, int out_row0, int out_row1, int out_x0, int out_x1) {
     Walk W;
@@ -294,17 +327,26 @@ __device__ __forceinline__ void walk_strip(const DevArgs &a, const Lds &s, const
         W.hom[0][0] = W.hom[0][1] = W.hom[1][0] = W.hom[1][1] = 0u;
     }
     const int x0 = xs + 2 * lane;
-    const bool in0 = x0 >= 0 && x0 < (int)a.width, in1 = x0 + 1 >= 0 && x0 + 1 < (int)a.width;
+    StripCols C;
+    C.in0 = x0 >= 0 && x0 < (int)a.width;
+    C.in1 = x0 + 1 >= 0 && x0 + 1 < (int)a.width;
+    C.st0 = x0 >= out_x0 && x0 < out_x1;
+    C.st1 = x0 + 1 >= out_x0 && x0 + 1 < out_x1;
+    C.inf = TAME ? opaque_inf() : __builtin_inff();
+    // an 8-byte store needs an even destination pixel: the column parity is the strip's, the row's
+    // adds dst_w times the row parity (row R - 5 has parity PY ^ 1 at trip position PY)
+    const uint32_t cpar = (uint32_t)(xs - (int)a.crop_x) & 1u;
+#pragma unroll
+    for (int py = 0; py < 2; py++)
+        C.pair_even[py] = ((cpar + (uint32_t)(py ^ 1 ^ (int)(a.crop_y & 1u)) * a.dst_w) & 1u) == 0;
     const __amdgpu_buffer_rsrc_t rs = make_rsrc(a.src, src_bytes(a));
     const __amdgpu_buffer_rsrc_t rd = make_rsrc(a.dst, a.dst_w * a.dev_h * 4u);
 #pragma unroll
     for (int i = 0; i < 2; i++) W.pf[i] = fetch_pair<HOT>(a, rs, x0, row0 - 6 + i);
-    const int r_end = out_row1 + 5;
-    // the last trip stops at r_end (a wave-uniform test between rows)
-    for (int R = row0 - 6; R < r_end; R += 6) {
-#define WALK_ROW(t)                                                                                            \
-    if (t > 0 && R + t >= r_end) break;                                                                       \
-    walk_row<RP, t, FAST, HOT>(a, s, wt, W, R + t, xs, lane, out_row0, out_row1, out_x0, out_x1, in0, in1, rs, rd);
+    const int trips = (out_row1 + 5 - (row0 - 6) + 5) / 6;
+    int R = row0 - 6;
+    for (int t = 0; t < trips; t++, R += 6) {
+#define WALK_ROW(t) walk_row<RP, t, FAST, HOT, TAME>(a, s, wt, W, R + t, xs, lane, out_row0, out_row1, C, rs, rd);
         WALK_ROW(0)
         WALK_ROW(1)
         WALK_ROW(2)
@@ -346,7 +388,7 @@ __host__ inline WalkPlan make_walk_plan(const DevArgs &a, int rows_per_strip) {
 }
 
 // Strip `item` (0 .. strips * chunks - 1) of plan P, developed by the calling wave.
-template <int RP, bool FAST>
+template <int RP, bool FAST, bool TAME>
 __device__ __forceinline__ void walk_item(const DevArgs &a, const WalkPlan &P, const Lds &s, const float *wt,
                                           int item) {
     const int lane = threadIdx.x & 63;
@@ -359,9 +401,9 @@ __device__ __forceinline__ void walk_item(const DevArgs &a, const WalkPlan &P, c
     const bool hot = a.simple_cal && xs >= 0 && xs + 128 <= (int)a.width && ((uintptr_t)a.src & 3u) == 0 &&
                      (a.stride & 1u) == 0;
     if (hot)
-        walk_strip<RP, FAST, true>(a, s, wt, xs, lane, row0, out_row0, out_row1, out_x0, out_x1);
+        walk_strip<RP, FAST, true, TAME>(a, s, wt, xs, lane, row0, out_row0, out_row1, out_x0, out_x1);
     else
-        walk_strip<RP, FAST, false>(a, s, wt, xs, lane, row0, out_row0, out_row1, out_x0, out_x1);
+        walk_strip<RP, FAST, false, TAME>(a, s, wt, xs, lane, row0, out_row0, out_row1, out_x0, out_x1);
 }
 
 }  // namespace frh

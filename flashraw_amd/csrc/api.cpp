@@ -571,6 +571,46 @@ static int32_t edge_tables(frh_context *ctx, const frh_edge_spec *e, const EdgeT
     return FRH_OK;
 }
 
+// True when the scalar calibration, the white balance and the matrix bound the values of the AHD
+// walker (ahd.rs:143-298) so that
+//  - every value it takes a square root of is 0 or at least 2^-96, and finite, and
+//  - every product of the RGBA8 finish (matrix coefficient times clipped channel) is 0 or normal.
+// The proof: a sample v (an integer) minus a black level b with j fractional bits is a multiple of
+// 2^-j, so a non-zero normalised and balanced sample m = min((v - b) * scale * wb, 1) has
+// |m| >= 2^-j |scale| |wb| (1 - 2^-20) >= 2^-40, and every float of that size is a multiple of
+// q = 2^-63. Sums and differences of multiples of q round to multiples of q, and the walker's
+// only other operations before the square roots are clamps and the exact scalings by 2, 0.5 and
+// 0.25, at most four halvings deep (the diagonal colour difference), so every square-root
+// operand is a multiple of 2^-67: 0 or >= 2^-67. The blend w * a + (1 - w) * b of two such
+// candidates, w = 0 or >= 1/72, is 0 or >= 2^-97, so a matrix coefficient of at least 2^-20
+// times a non-zero channel is at least 2^-117, a normal float; then scaling the coefficients
+// by 4096 scales every product and sum exactly. The bound 2^60 on |m| keeps every value (sums
+// of a few m, their square roots, squared differences) far from overflow. With positive scales
+// and white balance every zero is +0 (x - x is +0, and nothing before the finish multiplies by a
+// negative number), so choosing between equal values, as min, max and clamp do, cannot flip the
+// sign of a zero either.
+static bool ahd_tame(const DevArgs &a) {
+    if (!a.bounded || !a.simple_cal) return false;
+    double wmin = 1e300, wmax = 0.0;
+    for (int c = 0; c < 3; c++) {
+        if (!(a.wb[c] > 0.0f)) return false;
+        wmin = std::min(wmin, (double)a.wb[c]);
+        wmax = std::max(wmax, (double)a.wb[c]);
+    }
+    for (int i = 0; i < 4; i++) {
+        const double b = a.qb[i], k = a.qs[i];
+        if (!std::isfinite(b) || !std::isfinite(k) || !(k > 0.0)) return false;
+        int j = 0;
+        for (double f = b; f != std::floor(f); f *= 2.0)
+            if (++j > 40) return false;
+        if (std::ldexp(k * wmin * (1.0 - 0x1p-20), -j) < 0x1p-40) return false;
+        if ((65535.0 + std::fabs(b)) * k * wmax > 0x1p60) return false;
+    }
+    for (int i = 0; i < 9; i++)
+        if (a.m[i] != 0.0f && std::fabs(a.m[i]) < 0x1p-20f) return false;
+    return true;
+}
+
 // True when the inputs bound every intermediate value of the edge-aware demosaic: with
 // |balanced sample| <= M and |tap| <= C, a 49-tap sum is at most 49 M C <= 1e15, so sqrt(G0) <=
 // 3.2e7, the gradient energies E <= 4e16, (E + eps)^2 <= 2e33 stays finite and 1 / that stays
@@ -777,6 +817,13 @@ int32_t frh::plan_develop(const frh_context *ctx, const frh_frame *f, const frh_
     a.band_y1 = a.dev_h;
     memcpy(a.wb, spec->wb, sizeof(a.wb));
     memcpy(a.m, spec->cam_to_rgb, sizeof(a.m));
+    a.unit_k = 1u;
+    for (int c = 0; c < 3; c++) {
+        a.kk[c] = a.wb[c] / a.wb[c];
+        if (a.kk[c] != 1.0f) a.unit_k = 0u;
+    }
+    a.ahd_tame = ahd_tame(a) ? 1u : 0u;
+    for (int i = 0; i < 9; i++) a.m4k[i] = a.m[i] * 4096.0f;
     a.has_levels = spec->has_levels ? 1 : 0;
     a.lscale = spec->levels[0];
     a.loffset = spec->levels[1];

@@ -76,8 +76,8 @@ __device__ __forceinline__ void load_tables(const BatchList &L, Lds &s, float *w
 
 // Frames of a batch use the scalar calibration (at most a 2x2 black pattern, no deltas), so the
 // shared Lds holds only the sRGB table; its black/scale arrays are never read.
-template <int RP, bool FAST>
-__global__ __launch_bounds__(WALK_WAVES * 64) void k_batch_ahd(BatchList L, const BatchFrame *__restrict__ frames,
+template <int RP, bool FAST, bool TAME>
+__global__ __launch_bounds__(WALK_WAVES * 64) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_batch_ahd(BatchList L, const BatchFrame *__restrict__ frames,
                                                                     const uint32_t *__restrict__ ids,
                                                                     const uint32_t *__restrict__ item_off) {
     __shared__ Lds s;
@@ -90,7 +90,7 @@ __global__ __launch_bounds__(WALK_WAVES * 64) void k_batch_ahd(BatchList L, cons
         const BatchFrame &F = frames[__builtin_amdgcn_readfirstlane(ids[k])];
         const uint32_t local = item - item_off[k];
         if (local < F.walk_items) {
-            walk_item<RP, FAST>(F.a, F.wp, s, wt, (int)local);
+            walk_item<RP, FAST, TAME>(F.a, F.wp, s, wt, (int)local);
         } else {
             const uint32_t base = (local - F.walk_items) * BORDER_ITEM + (threadIdx.x & 63);
             for (uint32_t j = 0; j < BORDER_ITEM; j += 64)
@@ -135,6 +135,7 @@ struct Lists {
     uint32_t n[NLIST] = {}, total[NLIST] = {};
     size_t base[NLIST] = {};
     bool fast[NLIST] = {};
+    bool tame[NLIST] = {};  // every AHD frame of the list is AHD-tame (DevArgs::ahd_tame)
     uint64_t items(bool ahd) const {
         uint64_t t = 0;
         for (int l = ahd ? 0 : 4; l < (ahd ? 4 : 8); l++) t += total[l];
@@ -217,6 +218,7 @@ static void build_lists(const std::vector<Plan> &plans, uint32_t ahd_rows, uint3
     for (int l = 0; l < NLIST; l++) {
         off[l].push_back(0);
         L.fast[l] = true;
+        L.tame[l] = true;
     }
     for (size_t i = 0; i < n; i++) {
         const Plan &p = plans[i];
@@ -237,6 +239,7 @@ static void build_lists(const std::vector<Plan> &plans, uint32_t ahd_rows, uint3
         }
         if (F.items == 0) continue;
         L.fast[l] = L.fast[l] && fast_rgba8(F.a);
+        L.tame[l] = L.tame[l] && F.a.ahd_tame != 0;
         ids[l].push_back((uint32_t)i);
         off[l].push_back(off[l].back() + F.items);
     }
@@ -287,7 +290,7 @@ static uint32_t resident_blocks(K kernel, int threads, int num_cus) {
     return (uint32_t)per_cu * (uint32_t)num_cus;
 }
 
-template <int RP, bool FAST>
+template <int RP, bool FAST, bool TAME>
 static void launch_list(const frh_context *ctx, const Lists &Ls, const BatchDev &D, int l, hipStream_t st) {
     BatchList L{};
     L.lut8 = ctx->d_lut8;
@@ -298,8 +301,10 @@ static void launch_list(const frh_context *ctx, const Lists &Ls, const BatchDev 
     const uint32_t *ids = D.lists + Ls.base[l], *off = ids + Ls.n[l];
     const uint32_t blocks = (Ls.total[l] + 3) / 4;
     if (l < 4) {
-        const uint32_t g = std::min(resident_blocks(k_batch_ahd<RP, FAST>, WALK_WAVES * 64, ctx->num_cus), blocks);
-        hipLaunchKernelGGL((k_batch_ahd<RP, FAST>), dim3(g), dim3(WALK_WAVES * 64), 0, st, L, D.frames, ids, off);
+        const uint32_t g =
+            std::min(resident_blocks(k_batch_ahd<RP, FAST, TAME>, WALK_WAVES * 64, ctx->num_cus), blocks);
+        hipLaunchKernelGGL((k_batch_ahd<RP, FAST, TAME>), dim3(g), dim3(WALK_WAVES * 64), 0, st, L, D.frames, ids,
+                           off);
     } else {
         const uint32_t g =
             std::min(resident_blocks(k_batch_bilinear<RP, FAST>, BWALK_WAVES * 64, ctx->num_cus), blocks);
@@ -310,10 +315,16 @@ static void launch_list(const frh_context *ctx, const Lists &Ls, const BatchDev 
 
 template <int RP>
 static void launch_list_rp(const frh_context *ctx, const Lists &Ls, const BatchDev &D, int l, hipStream_t st) {
-    if (Ls.fast[l])
-        launch_list<RP, true>(ctx, Ls, D, l, st);
+    // the tame form only changes the AHD lists (l < 4); the bilinear lists take TAME = false
+    const bool tame = l < 4 && Ls.tame[l];
+    if (Ls.fast[l] && tame)
+        launch_list<RP, true, true>(ctx, Ls, D, l, st);
+    else if (Ls.fast[l])
+        launch_list<RP, true, false>(ctx, Ls, D, l, st);
+    else if (tame)
+        launch_list<RP, false, true>(ctx, Ls, D, l, st);
     else
-        launch_list<RP, false>(ctx, Ls, D, l, st);
+        launch_list<RP, false, false>(ctx, Ls, D, l, st);
 }
 
 // The launch sequence: counter reset, then each non-empty list (AHD lists first).

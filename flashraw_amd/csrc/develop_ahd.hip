@@ -5,8 +5,8 @@
 
 namespace frh {
 
-template <int RP, bool FAST>
-__global__ __launch_bounds__(WALK_WAVES * 64) void k_develop_ahd_walk(DevArgs a, WalkPlan P, uint32_t walk_blocks) {
+template <int RP, bool FAST, bool TAME>
+__global__ __launch_bounds__(WALK_WAVES * 64) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_develop_ahd_walk(DevArgs a, WalkPlan P, uint32_t walk_blocks) {
     __shared__ Lds s;
     __shared__ float wt[73 * 73];
     load_lds(a, s);
@@ -22,18 +22,25 @@ __global__ __launch_bounds__(WALK_WAVES * 64) void k_develop_ahd_walk(DevArgs a,
     // bookkeeping in scalar registers
     const int item = (int)uniform(blockIdx.x * WALK_WAVES + (threadIdx.x >> 6));
     if (item >= P.strips * P.chunks) return;
-    walk_item<RP, FAST>(a, P, s, wt, item);
+    walk_item<RP, FAST, TAME>(a, P, s, wt, item);
 }
 
 template <int RP>
 static void launch_ahd_rp(const DevArgs &a, const WalkPlan &P, uint32_t walk_blocks, uint32_t grid,
                           hipStream_t stream) {
-    if (fast_rgba8(a))
-        hipLaunchKernelGGL((k_develop_ahd_walk<RP, true>), dim3(grid), dim3(WALK_WAVES * 64), 0, stream, a, P,
+    const bool fast = fast_rgba8(a), tame = a.ahd_tame != 0;
+    if (fast && tame)
+        hipLaunchKernelGGL((k_develop_ahd_walk<RP, true, true>), dim3(grid), dim3(WALK_WAVES * 64), 0, stream, a, P,
                            walk_blocks);
+    else if (fast)
+        hipLaunchKernelGGL((k_develop_ahd_walk<RP, true, false>), dim3(grid), dim3(WALK_WAVES * 64), 0, stream, a,
+                           P, walk_blocks);
+    else if (tame)
+        hipLaunchKernelGGL((k_develop_ahd_walk<RP, false, true>), dim3(grid), dim3(WALK_WAVES * 64), 0, stream, a,
+                           P, walk_blocks);
     else
-        hipLaunchKernelGGL((k_develop_ahd_walk<RP, false>), dim3(grid), dim3(WALK_WAVES * 64), 0, stream, a, P,
-                           walk_blocks);
+        hipLaunchKernelGGL((k_develop_ahd_walk<RP, false, false>), dim3(grid), dim3(WALK_WAVES * 64), 0, stream, a,
+                           P, walk_blocks);
 }
 
 // Every float in [2^-96, 2^128), and +-0, through sqrt_nonneg against the compiler's correctly

@@ -65,6 +65,10 @@ __device__ __forceinline__ void finish(const DevArgs &a, float r, float g, float
         r = r * a.wb[0];
         g = g * a.wb[1];
         b = b * a.wb[2];
+    } else if (!a.unit_k) {
+        r = r * a.kk[0];
+        g = g * a.kk[1];
+        b = b * a.kk[2];
     }
     r = rclamp01(r);
     g = rclamp01(g);
@@ -212,7 +216,7 @@ __device__ __forceinline__ uint32_t rgba8_word(const Lds &s, float r, float g, f
 // kernels' inline fast path.
 __host__ __device__ __forceinline__ bool fast_rgba8(const DevArgs &a) {
     return a.out_mode == OUT_RGBA8 && a.out_mode2 == OUT_NONE && a.orientation == 1 && a.bounded && !a.has_levels &&
-           !a.tone && !a.curve8;
+           !a.tone && !a.curve8 && a.unit_k;
 }
 
 __device__ __forceinline__ float med01(float x) { return __builtin_amdgcn_fmed3f(x, 0.0f, 1.0f); }
@@ -234,6 +238,22 @@ __device__ __forceinline__ uint32_t finish_rgba8_fast(const DevArgs &a, const Ld
     const float o2 = a.m[6] * r + a.m[7] * g + a.m[8] * b;
     const uint32_t i0 = (uint32_t)(med01(o0) * 4096.0f), i1 = (uint32_t)(med01(o1) * 4096.0f),
                    i2 = (uint32_t)(med01(o2) * 4096.0f);
+    return lut8(s, i0) | (lut8(s, i1) << 8) | (lut8(s, i2) << 16) | 0xff000000u;
+}
+
+// finish_rgba8_fast of balanced pixels when the frame is AHD-tame (DevArgs::ahd_tame): the matrix
+// rows pre-scaled by 4096 give exactly 4096 times each output (every product is 0 or normal), so
+// clamping to [0, 4096] equals clamping to [0, 1] and scaling by 4096.
+__device__ __forceinline__ uint32_t finish_rgba8_scaled(const DevArgs &a, const Lds &s, float r, float g, float b) {
+    r = med01(r);
+    g = med01(g);
+    b = med01(b);
+    const float o0 = a.m4k[0] * r + a.m4k[1] * g + a.m4k[2] * b;
+    const float o1 = a.m4k[3] * r + a.m4k[4] * g + a.m4k[5] * b;
+    const float o2 = a.m4k[6] * r + a.m4k[7] * g + a.m4k[8] * b;
+    const uint32_t i0 = (uint32_t)__builtin_amdgcn_fmed3f(o0, 0.0f, 4096.0f),
+                   i1 = (uint32_t)__builtin_amdgcn_fmed3f(o1, 0.0f, 4096.0f),
+                   i2 = (uint32_t)__builtin_amdgcn_fmed3f(o2, 0.0f, 4096.0f);
     return lut8(s, i0) | (lut8(s, i1) << 8) | (lut8(s, i2) << 16) | 0xff000000u;
 }
 
@@ -523,6 +543,34 @@ __device__ __forceinline__ float sqrt_max0(float v) {
         r = rare ? sqrtf(x) : r;
     }
     return r;
+}
+// max / min; NN: neither operand is a NaN, so v_med3_f32 with an infinite third operand gives the
+// same value and, unlike v_max_f32 / v_min_f32 in IEEE mode, needs no quieting of its operands.
+// `inf` is +inf held in a register the compiler cannot see through (opaque_inf), or the median
+// would be folded back into a max / min.
+template <bool NN>
+__device__ __forceinline__ float fmax_nn(float a, float b, float inf) {
+    return NN ? __builtin_amdgcn_fmed3f(a, b, inf) : fmaxf(a, b);
+}
+template <bool NN>
+__device__ __forceinline__ float fmin_nn(float a, float b, float inf) {
+    return NN ? __builtin_amdgcn_fmed3f(a, b, -inf) : fminf(a, b);
+}
+__device__ __forceinline__ float opaque_inf() {
+    float r;
+    asm volatile("s_mov_b32 %0, 0x7f800000" : "=s"(r));
+    return r;
+}
+// sqrt(max(v, 0)); TAME: v is known to be 0 or at least 2^-96 and finite (DevArgs::ahd_tame), so
+// max(max(v, 0), 2^-96) = max(v, 2^-96) feeds the estimate directly.
+template <bool TAME>
+__device__ __forceinline__ float sqrt_m0(float v, float inf) {
+    if (!TAME) return sqrt_max0(v);
+    const float x = fmax_nn<true>(v, 0.0f, inf);
+    const float y = __builtin_amdgcn_rsqf(fmax_nn<true>(v, 0x1p-96f, inf));
+    const float g = x * y;
+    const float r = __builtin_fmaf(-g, g, x);
+    return __builtin_fmaf(r, 0.5f * y, g);
 }
 
 // Value of the lane to the left (lane - 1) / right (lane + 1) of this one (DPP wave shifts; the

@@ -66,6 +66,17 @@ struct DevArgs {
     // fast path may clamp with v_med3_f32 (identical to f32::clamp on finite values; the sign of a
     // zero cannot change an 8-bit code).
     uint32_t bounded;
+    // The calibration, white balance and matrix bound the AHD walker's values (see ahd_tame in
+    // api.cpp): every value it takes a square root of is 0 or at least 2^-96 and finite, and every
+    // product of the RGBA8 finish is a normal float, so the square root needs no test for tiny or
+    // infinite inputs and the finish may take the matrix pre-scaled by 4096 (`m4k`).
+    uint32_t ahd_tame;
+    float m4k[9];
+    // The AHD and edge-aware finish multiplies the balanced pixel by wb / wb per channel
+    // (process.rs:1209): 1 for every multiplier except 0, whose channel becomes NaN. unit_k: all
+    // three are 1 and the multiply is skipped.
+    uint32_t unit_k;
+    float kk[3];
     // develop region and band
     uint32_t crop_x, crop_y, dev_w, dev_h;
     uint32_t band_y0, band_y1;  // developed rows [band_y0, band_y1) of this launch
