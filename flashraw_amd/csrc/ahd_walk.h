@@ -175,7 +175,11 @@ __device__ __forceinline__ void walk_row(const DevArgs &a, const Lds &s, const f
                 sg = sqrt_m0<TAME>(gg, C.inf);
                 sb = sm;
             }
-            W.lab[LQ][d][0][k] = (sr + 2.0f * sg + sb) * 0.25f;
+            // Lightness (sr + 2 sg + sb) / 4. The tame form keeps 4 L: lightness only enters the
+            // homogeneity test as differences compared with maxima of differences, and scaling by
+            // 4 commutes with their rounding (no value of a tame frame comes near the subnormals).
+            const float l4 = sr + 2.0f * sg + sb;
+            W.lab[LQ][d][0][k] = TAME ? l4 : l4 * 0.25f;
             W.lab[LQ][d][1][k] = sr - sg;
             W.lab[LQ][d][2][k] = sb - sg;
             W.u[U3][d][k][0] = u0;

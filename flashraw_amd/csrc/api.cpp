@@ -588,7 +588,9 @@ static int32_t edge_tables(frh_context *ctx, const frh_edge_spec *e, const EdgeT
 // of a few m, their square roots, squared differences) far from overflow. With positive scales
 // and white balance every zero is +0 (x - x is +0, and nothing before the finish multiplies by a
 // negative number), so choosing between equal values, as min, max and clamp do, cannot flip the
-// sign of a zero either.
+// sign of a zero either. The bilinear walker's RGBA8 finish relies on the same bound: its channels
+// are means of at most four normalised samples times the white balance, so a non-zero one is at
+// least 2^-66 and its products with the pre-scaled matrix are exact too.
 static bool ahd_tame(const DevArgs &a) {
     if (!a.bounded || !a.simple_cal) return false;
     double wmin = 1e300, wmax = 0.0;

@@ -99,7 +99,7 @@ __global__ __launch_bounds__(WALK_WAVES * 64) __attribute__((amdgpu_waves_per_eu
     }
 }
 
-template <int RP, bool FAST>
+template <int RP, bool FAST, bool TAME>
 __global__ __launch_bounds__(BWALK_WAVES * 64) void k_batch_bilinear(BatchList L, const BatchFrame *__restrict__ frames,
                                                                     const uint32_t *__restrict__ ids,
                                                                     const uint32_t *__restrict__ item_off) {
@@ -112,7 +112,7 @@ __global__ __launch_bounds__(BWALK_WAVES * 64) void k_batch_bilinear(BatchList L
         const BatchFrame &F = frames[__builtin_amdgcn_readfirstlane(ids[k])];
         const uint32_t local = item - item_off[k];
         if (local < F.walk_items) {
-            bwalk_item<RP, FAST>(F.a, F.bp, s, (int)local);
+            bwalk_item<RP, FAST, TAME>(F.a, F.bp, s, (int)local);
         } else {
             const uint32_t base = (local - F.walk_items) * BORDER_ITEM + (threadIdx.x & 63);
             for (uint32_t j = 0; j < BORDER_ITEM; j += 64)
@@ -307,16 +307,17 @@ static void launch_list(const frh_context *ctx, const Lists &Ls, const BatchDev 
                            off);
     } else {
         const uint32_t g =
-            std::min(resident_blocks(k_batch_bilinear<RP, FAST>, BWALK_WAVES * 64, ctx->num_cus), blocks);
-        hipLaunchKernelGGL((k_batch_bilinear<RP, FAST>), dim3(g), dim3(BWALK_WAVES * 64), 0, st, L, D.frames, ids,
-                           off);
+            std::min(resident_blocks(k_batch_bilinear<RP, FAST, TAME>, BWALK_WAVES * 64, ctx->num_cus), blocks);
+        hipLaunchKernelGGL((k_batch_bilinear<RP, FAST, TAME>), dim3(g), dim3(BWALK_WAVES * 64), 0, st, L, D.frames,
+                           ids, off);
     }
 }
 
 template <int RP>
 static void launch_list_rp(const frh_context *ctx, const Lists &Ls, const BatchDev &D, int l, hipStream_t st) {
-    // the tame form only changes the AHD lists (l < 4); the bilinear lists take TAME = false
-    const bool tame = l < 4 && Ls.tame[l];
+    // tame lists: every frame's values are bounded as ahd_tame in api.cpp states (the AHD walker's
+    // square roots and clamps, and both walkers' finish with the matrix pre-scaled by 4096)
+    const bool tame = Ls.tame[l];
     if (Ls.fast[l] && tame)
         launch_list<RP, true, true>(ctx, Ls, D, l, st);
     else if (Ls.fast[l])

@@ -23,9 +23,17 @@ struct BWalk {
     uint32_t pf[BPF];        // raw pairs fetched ahead, slot (row - start) % BPF
 };
 
-template <int RP, int PY, int SLOT, bool FAST, bool HOT>
+// Per-lane constants of a strip: columns x0, x0 + 1 inside the frame (in0, in1) and output
+// columns of the strip (st0, st1); pair_even[p]: at row parity p the lane's two output pixels start
+// at an even destination pixel (one 8-byte store).
+struct BStripCols {
+    bool in0, in1, st0, st1;
+    bool pair_even[2];
+};
+
+template <int RP, int PY, int SLOT, bool FAST, bool HOT, bool TAME>
 __device__ __forceinline__ void bwalk_row(const DevArgs &a, const Lds &s, BWalk &W, int R, int xs, int lane,
-                                          int out_row0, int out_row1, int out_x0, int out_x1, bool in0, bool in1,
+                                          int out_row0, int out_row1, const BStripCols &C,
                                           __amdgpu_buffer_rsrc_t rs, __amdgpu_buffer_rsrc_t rd) {
     const int x0 = xs + 2 * lane;
 #pragma unroll
@@ -36,12 +44,15 @@ __device__ __forceinline__ void bwalk_row(const DevArgs &a, const Lds &s, BWalk 
         W.nr[r] = W.nr[r + 1];
     }
     const uint32_t raw = W.pf[SLOT];
-    W.pf[SLOT] = fetch_pair<HOT>(a, rs, x0, R + BPF);
-    norm_pair<PY, HOT>(a, s, raw, x0, R, R >= 0 && R < (int)a.height, in0, in1, W.n[2][0], W.n[2][1]);
+    // rows past out_row1 feed no output: their prefetch re-reads row out_row1 (a cache hit)
+    W.pf[SLOT] = fetch_pair<HOT>(a, rs, x0, min(R + BPF, out_row1));
+    norm_pair<PY, HOT>(a, s, raw, x0, R, R >= 0 && R < (int)a.height, C.in0, C.in1, W.n[2][0], W.n[2][1]);
     W.nl[2] = from_left(W.n[2][1]);
     W.nr[2] = from_right(W.n[2][0]);
     const int ro = R - 1;
-    if (ro < out_row0 || ro >= out_row1 || lane < 1 || lane > 62) return;
+    // rows outside the strip's output (the two it starts above, those a trip walks past its end)
+    // skip the finish: a uniform branch
+    if (ro < out_row0 || ro >= out_row1 || (!FAST && (lane < 1 || lane > 62))) return;
     constexpr int PO = PY ^ 1;
     float o[2][3];
 #pragma unroll
@@ -67,31 +78,36 @@ __device__ __forceinline__ void bwalk_row(const DevArgs &a, const Lds &s, BWalk 
             rgb[2] = col == 0 ? diag : own;
         }
         if (FAST) {
-            o[k][0] = __uint_as_float(finish_rgba8_fast(a, s, rgb[0], rgb[1], rgb[2], false));
+            o[k][0] = __uint_as_float(TAME ? finish_rgba8_scaled(a, s, rgb[0] * a.wb[0], rgb[1] * a.wb[1],
+                                                                 rgb[2] * a.wb[2])
+                                           : finish_rgba8_fast(a, s, rgb[0], rgb[1], rgb[2], false));
         } else {
             finish(a, rgb[0], rgb[1], rgb[2], false, o[k][0], o[k][1], o[k][2]);
         }
     }
-    const int dx = x0 - (int)a.crop_x, dy = ro - (int)a.crop_y;
-    const bool st0 = x0 >= out_x0 && x0 < out_x1, st1 = x0 + 1 >= out_x0 && x0 + 1 < out_x1;
     if (FAST) {
-        const uint32_t w0 = __float_as_uint(o[0][0]), w1 = __float_as_uint(o[1][0]);
-        if (st0 && st1) {
-            store_rgba8_pair(a, rd, (uint32_t)dx, (uint32_t)dy, w0, w1, 2);
-        } else if (st0) {
-            store_rgba8_pair(a, rd, (uint32_t)dx, (uint32_t)dy, w0, 0, 1);
-        } else if (st1) {
-            store_rgba8_pair(a, rd, (uint32_t)dx + 1, (uint32_t)dy, w1, 0, 1);
-        }
+        // every lane finishes its pixels; stores of columns outside the strip's output address past
+        // the destination resource, where the hardware drops them (no divergent branch)
+        const uint32_t off = ((uint32_t)(ro - (int)a.crop_y) * a.dst_w + (uint32_t)(x0 - (int)a.crop_x)) * 4u;
+        const bool s0 = C.st0, s1 = C.st1, pe = C.pair_even[PY];
+        constexpr uint32_t PAST = 0xfffffff0u;
+        typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+        u32x2 v;
+        v.x = __float_as_uint(o[0][0]);
+        v.y = __float_as_uint(o[1][0]);
+        __builtin_amdgcn_raw_buffer_store_b64(v, rd, (pe && s0 && s1) ? (int)off : (int)PAST, 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b32(v.x, rd, (s0 && !(pe && s1)) ? (int)off : (int)PAST, 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b32(v.y, rd, (s1 && !(pe && s0)) ? (int)(off + 4u) : (int)PAST, 0, 0);
     } else {
-        if (st0) store_both<false>(a, s, (uint32_t)dx, (uint32_t)dy, o[0][0], o[0][1], o[0][2]);
-        if (st1) store_both<false>(a, s, (uint32_t)dx + 1, (uint32_t)dy, o[1][0], o[1][1], o[1][2]);
+        const int dx = x0 - (int)a.crop_x, dy = ro - (int)a.crop_y;
+        if (C.st0) store_both<false>(a, s, (uint32_t)dx, (uint32_t)dy, o[0][0], o[0][1], o[0][2]);
+        if (C.st1) store_both<false>(a, s, (uint32_t)dx + 1, (uint32_t)dy, o[1][0], o[1][1], o[1][2]);
     }
 }
 
 // One wave's strip: rows from row0 - 2 (even) until the last output row out_row1 - 1 = R - 1,
 // four rows per trip so every prefetch slot is static.
-template <int RP, bool FAST, bool HOT>
+template <int RP, bool FAST, bool HOT, bool TAME>
 __device__ __forceinline__ void bwalk_strip(const DevArgs &a, const Lds &s, int xs, int lane, int row0, int out_row0,
                                             int out_row1, int out_x0, int out_x1) {
     BWalk W;
@@ -100,19 +116,25 @@ __device__ __forceinline__ void bwalk_strip(const DevArgs &a, const Lds &s, int 
         for (int i = 0; i < (int)(sizeof(BWalk) / 4) - BPF; i++) f[i] = 0.0f;
     }
     const int x0 = xs + 2 * lane;
-    const bool in0 = x0 >= 0 && x0 < (int)a.width, in1 = x0 + 1 >= 0 && x0 + 1 < (int)a.width;
+    BStripCols C;
+    C.in0 = x0 >= 0 && x0 < (int)a.width;
+    C.in1 = x0 + 1 >= 0 && x0 + 1 < (int)a.width;
+    C.st0 = x0 >= out_x0 && x0 < out_x1;
+    C.st1 = x0 + 1 >= out_x0 && x0 + 1 < out_x1;
+    // the output row R - 1 of row parity PY has parity PY ^ 1
+    const uint32_t cpar = (uint32_t)(xs - (int)a.crop_x) & 1u;
+#pragma unroll
+    for (int py = 0; py < 2; py++)
+        C.pair_even[py] = ((cpar + (uint32_t)(py ^ 1 ^ (int)(a.crop_y & 1u)) * a.dst_w) & 1u) == 0;
     const __amdgpu_buffer_rsrc_t rs = make_rsrc(a.src, src_bytes(a));
     const __amdgpu_buffer_rsrc_t rd = make_rsrc(a.dst, a.dst_w * a.dev_h * 4u);
 #pragma unroll
-    for (int i = 0; i < BPF; i++) W.pf[i] = fetch_pair<HOT>(a, rs, x0, row0 - 2 + i);
+    for (int i = 0; i < BPF; i++) W.pf[i] = fetch_pair<HOT>(a, rs, x0, min(row0 - 2 + i, out_row1));
     for (int R = row0 - 2; R < out_row1 + 1; R += BPF) {
-        bwalk_row<RP, 0, 0, FAST, HOT>(a, s, W, R, xs, lane, out_row0, out_row1, out_x0, out_x1, in0, in1, rs, rd);
-        bwalk_row<RP, 1, 1, FAST, HOT>(a, s, W, R + 1, xs, lane, out_row0, out_row1, out_x0, out_x1, in0, in1, rs,
-                                       rd);
-        bwalk_row<RP, 0, 2, FAST, HOT>(a, s, W, R + 2, xs, lane, out_row0, out_row1, out_x0, out_x1, in0, in1, rs,
-                                       rd);
-        bwalk_row<RP, 1, 3, FAST, HOT>(a, s, W, R + 3, xs, lane, out_row0, out_row1, out_x0, out_x1, in0, in1, rs,
-                                       rd);
+        bwalk_row<RP, 0, 0, FAST, HOT, TAME>(a, s, W, R, xs, lane, out_row0, out_row1, C, rs, rd);
+        bwalk_row<RP, 1, 1, FAST, HOT, TAME>(a, s, W, R + 1, xs, lane, out_row0, out_row1, C, rs, rd);
+        bwalk_row<RP, 0, 2, FAST, HOT, TAME>(a, s, W, R + 2, xs, lane, out_row0, out_row1, C, rs, rd);
+        bwalk_row<RP, 1, 3, FAST, HOT, TAME>(a, s, W, R + 3, xs, lane, out_row0, out_row1, C, rs, rd);
     }
 }
 
@@ -145,7 +167,7 @@ __host__ inline BWalkPlan make_bwalk_plan(const DevArgs &a, int rows_per_strip) 
 }
 
 // Strip `item` of plan P, developed by the calling wave.
-template <int RP, bool FAST>
+template <int RP, bool FAST, bool TAME>
 __device__ __forceinline__ void bwalk_item(const DevArgs &a, const BWalkPlan &P, const Lds &s, int item) {
     const int lane = threadIdx.x & 63;
     const int sx = item % P.strips, sy = item / P.strips;
@@ -157,9 +179,9 @@ __device__ __forceinline__ void bwalk_item(const DevArgs &a, const BWalkPlan &P,
     const bool hot = a.simple_cal && xs >= 0 && xs + 128 <= (int)a.width && ((uintptr_t)a.src & 3u) == 0 &&
                      (a.stride & 1u) == 0;
     if (hot)
-        bwalk_strip<RP, FAST, true>(a, s, xs, lane, row0, out_row0, out_row1, out_x0, out_x1);
+        bwalk_strip<RP, FAST, true, TAME>(a, s, xs, lane, row0, out_row0, out_row1, out_x0, out_x1);
     else
-        bwalk_strip<RP, FAST, false>(a, s, xs, lane, row0, out_row0, out_row1, out_x0, out_x1);
+        bwalk_strip<RP, FAST, false, TAME>(a, s, xs, lane, row0, out_row0, out_row1, out_x0, out_x1);
 }
 
 }  // namespace frh

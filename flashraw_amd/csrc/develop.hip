@@ -236,33 +236,41 @@ __global__ __launch_bounds__(256) void k_develop_ahd(DevArgs a, uint32_t tiles_x
 // ------------------------------------------------------------------------------------------
 // Bilinear row walker launcher (the walker is in bil_walk.h).
 
-template <int RP, bool FAST>
+template <int RP, bool FAST, bool TAME>
 __global__ __launch_bounds__(BWALK_WAVES * 64) void k_develop_bilinear_walk(DevArgs a, BWalkPlan P,
                                                                             uint32_t walk_blocks) {
     __shared__ Lds s;
     load_lds(a, s);
     __syncthreads();
-    if (blockIdx.x >= walk_blocks) {
-        const uint32_t i = (blockIdx.x - walk_blocks) * blockDim.x + threadIdx.x;
+    // the walker blocks come first and are renumbered among themselves; border blocks follow
+    const uint32_t blk = blockIdx.x < walk_blocks ? xcd_block(blockIdx.x, walk_blocks) : blockIdx.x;
+    if (blk >= walk_blocks) {
+        const uint32_t i = (blk - walk_blocks) * blockDim.x + threadIdx.x;
         if (i < P.border.count) border_pixel<false>(a, s, P.border, i);
         return;
     }
     // the wave index is the same for every lane; readfirstlane lets the compiler keep all the strip
     // bookkeeping in scalar registers
-    const int item = (int)uniform(blockIdx.x * BWALK_WAVES + (threadIdx.x >> 6));
+    const int item = (int)uniform(blk * BWALK_WAVES + (threadIdx.x >> 6));
     if (item >= P.strips * P.chunks) return;
-    bwalk_item<RP, FAST>(a, P, s, item);
+    bwalk_item<RP, FAST, TAME>(a, P, s, item);
 }
 
 template <int RP>
 static void launch_bil_rp(const DevArgs &a, const BWalkPlan &P, uint32_t walk_blocks, uint32_t grid,
                           hipStream_t stream) {
-    if (fast_rgba8(a))
-        hipLaunchKernelGGL((k_develop_bilinear_walk<RP, true>), dim3(grid), dim3(BWALK_WAVES * 64), 0, stream, a,
-                           P, walk_blocks);
+    // the tame frame's finish takes the matrix pre-scaled by 4096 (every product is 0 or normal:
+    // the bound of ahd_tame covers the bilinear means too)
+    const bool fast = fast_rgba8(a), tame = a.ahd_tame != 0;
+    if (fast && tame)
+        hipLaunchKernelGGL((k_develop_bilinear_walk<RP, true, true>), dim3(grid), dim3(BWALK_WAVES * 64), 0, stream,
+                           a, P, walk_blocks);
+    else if (fast)
+        hipLaunchKernelGGL((k_develop_bilinear_walk<RP, true, false>), dim3(grid), dim3(BWALK_WAVES * 64), 0,
+                           stream, a, P, walk_blocks);
     else
-        hipLaunchKernelGGL((k_develop_bilinear_walk<RP, false>), dim3(grid), dim3(BWALK_WAVES * 64), 0, stream, a,
-                           P, walk_blocks);
+        hipLaunchKernelGGL((k_develop_bilinear_walk<RP, false, false>), dim3(grid), dim3(BWALK_WAVES * 64), 0,
+                           stream, a, P, walk_blocks);
 }
 
 static hipError_t launch_develop_bilinear_walk(const DevArgs &a, hipStream_t stream, int rows_per_strip) {

@@ -10,9 +10,11 @@ __global__ __launch_bounds__(WALK_WAVES * 64) __attribute__((amdgpu_waves_per_eu
     __shared__ Lds s;
     __shared__ float wt[73 * 73];
     load_lds(a, s);
-    if (blockIdx.x >= walk_blocks) {
+    // the walker blocks come first and are renumbered among themselves; border blocks follow
+    const uint32_t blk = blockIdx.x < walk_blocks ? xcd_block(blockIdx.x, walk_blocks) : blockIdx.x;
+    if (blk >= walk_blocks) {
         __syncthreads();
-        const uint32_t i = (blockIdx.x - walk_blocks) * blockDim.x + threadIdx.x;
+        const uint32_t i = (blk - walk_blocks) * blockDim.x + threadIdx.x;
         if (i < P.border.count) border_pixel<false>(a, s, P.border, i);
         return;
     }
@@ -20,7 +22,7 @@ __global__ __launch_bounds__(WALK_WAVES * 64) __attribute__((amdgpu_waves_per_eu
     __syncthreads();
     // the wave index is the same for every lane; readfirstlane lets the compiler keep all the strip
     // bookkeeping in scalar registers
-    const int item = (int)uniform(blockIdx.x * WALK_WAVES + (threadIdx.x >> 6));
+    const int item = (int)uniform(blk * WALK_WAVES + (threadIdx.x >> 6));
     if (item >= P.strips * P.chunks) return;
     walk_item<RP, FAST, TAME>(a, P, s, wt, item);
 }

@@ -212,6 +212,16 @@ __device__ __forceinline__ uint32_t rgba8_word(const Lds &s, float r, float g, f
     return lut8(s, lut8_index(r)) | (lut8(s, lut8_index(g)) << 8) | (lut8(s, lut8_index(b)) << 16) | 0xff000000u;
 }
 
+// The workgroups of one dispatch go to the device's 8 XCDs round-robin (workgroup b to XCD
+// b mod 8), and each XCD has its own L2. xcd_block renumbers the n workgroups so that every XCD
+// runs one contiguous range of logical blocks: neighbouring strips, which share cache lines at
+// their edges and halo rows, then meet in one L2.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t n) {
+    constexpr uint32_t X = 8;
+    const uint32_t q = n / X, r = n % X, x = b % X, slot = b / X;
+    return x < r ? x * (q + 1) + slot : r * (q + 1) + (x - r) * q + slot;
+}
+
 // True when a launch stores RGBA8 only, unrotated, without levels, with bounded parameters: the
 // kernels' inline fast path.
 __host__ __device__ __forceinline__ bool fast_rgba8(const DevArgs &a) {
