@@ -31,7 +31,7 @@ struct BStripCols {
     bool pair_even[2];
 };
 
-template <int RP, int PY, int SLOT, bool FAST, bool HOT, bool TAME>
+template <int RP, int PY, int SLOT, bool FAST, bool HOT, bool TAME, bool OUT = true>
 __device__ __forceinline__ void bwalk_row(const DevArgs &a, const Lds &s, BWalk &W, int R, int xs, int lane,
                                           int out_row0, int out_row1, const BStripCols &C,
                                           __amdgpu_buffer_rsrc_t rs, __amdgpu_buffer_rsrc_t rd) {
@@ -49,10 +49,11 @@ __device__ __forceinline__ void bwalk_row(const DevArgs &a, const Lds &s, BWalk 
     norm_pair<PY, HOT>(a, s, raw, x0, R, R >= 0 && R < (int)a.height, C.in0, C.in1, W.n[2][0], W.n[2][1]);
     W.nl[2] = from_left(W.n[2][1]);
     W.nr[2] = from_right(W.n[2][0]);
+    if (!OUT) return;
     const int ro = R - 1;
-    // rows outside the strip's output (the two it starts above, those a trip walks past its end)
-    // skip the finish: a uniform branch
-    if (ro < out_row0 || ro >= out_row1 || (!FAST && (lane < 1 || lane > 62))) return;
+    // FAST: every row of the main loop has its output row inside the strip but the first row of a
+    // frame's first strip (the frame border), whose stores are dropped like those of the halo lanes
+    if (!FAST && (ro < out_row0 || ro >= out_row1 || lane < 1 || lane > 62)) return;
     constexpr int PO = PY ^ 1;
     float o[2][3];
 #pragma unroll
@@ -88,8 +89,9 @@ __device__ __forceinline__ void bwalk_row(const DevArgs &a, const Lds &s, BWalk 
     if (FAST) {
         // every lane finishes its pixels; stores of columns outside the strip's output address past
         // the destination resource, where the hardware drops them (no divergent branch)
+        const bool rv = ro >= out_row0 && ro < out_row1;
         const uint32_t off = ((uint32_t)(ro - (int)a.crop_y) * a.dst_w + (uint32_t)(x0 - (int)a.crop_x)) * 4u;
-        const bool s0 = C.st0, s1 = C.st1, pe = C.pair_even[PY];
+        const bool s0 = rv && C.st0, s1 = rv && C.st1, pe = C.pair_even[PY];
         constexpr uint32_t PAST = 0xfffffff0u;
         typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
         u32x2 v;
@@ -105,8 +107,11 @@ __device__ __forceinline__ void bwalk_row(const DevArgs &a, const Lds &s, BWalk 
     }
 }
 
-// One wave's strip: rows from row0 - 2 (even) until the last output row out_row1 - 1 = R - 1,
-// four rows per trip so every prefetch slot is static.
+// One wave's strip: mosaic rows row0 - 2 (even) .. out_row1; output row R - 1 follows row R.
+// FAST (no divergent branch in a row): three prologue rows fill the window (row0 gives output row
+// row0 - 1, outside the strip), then whole trips of four rows, each straight-line code the compiler
+// schedules as one block (the LDS table reads of one row overlap the next row's arithmetic), then
+// at most three rows more. Every row has a static prefetch slot (R - row0 + 2) mod 4 and parity.
 template <int RP, bool FAST, bool HOT, bool TAME>
 __device__ __forceinline__ void bwalk_strip(const DevArgs &a, const Lds &s, int xs, int lane, int row0, int out_row0,
                                             int out_row1, int out_x0, int out_x1) {
@@ -130,12 +135,32 @@ __device__ __forceinline__ void bwalk_strip(const DevArgs &a, const Lds &s, int 
     const __amdgpu_buffer_rsrc_t rd = make_rsrc(a.dst, a.dst_w * a.dev_h * 4u);
 #pragma unroll
     for (int i = 0; i < BPF; i++) W.pf[i] = fetch_pair<HOT>(a, rs, x0, min(row0 - 2 + i, out_row1));
-    for (int R = row0 - 2; R < out_row1 + 1; R += BPF) {
-        bwalk_row<RP, 0, 0, FAST, HOT, TAME>(a, s, W, R, xs, lane, out_row0, out_row1, C, rs, rd);
-        bwalk_row<RP, 1, 1, FAST, HOT, TAME>(a, s, W, R + 1, xs, lane, out_row0, out_row1, C, rs, rd);
-        bwalk_row<RP, 0, 2, FAST, HOT, TAME>(a, s, W, R + 2, xs, lane, out_row0, out_row1, C, rs, rd);
-        bwalk_row<RP, 1, 3, FAST, HOT, TAME>(a, s, W, R + 3, xs, lane, out_row0, out_row1, C, rs, rd);
+    if (!FAST) {
+        // the general path (lanes and rows leave each row early, a divergent branch): whole trips
+        // of four rows from row0 - 2, the rows past out_row1 only shifting the window
+        for (int R = row0 - 2; R < out_row1 + 1; R += BPF) {
+            bwalk_row<RP, 0, 0, FAST, HOT, TAME>(a, s, W, R, xs, lane, out_row0, out_row1, C, rs, rd);
+            bwalk_row<RP, 1, 1, FAST, HOT, TAME>(a, s, W, R + 1, xs, lane, out_row0, out_row1, C, rs, rd);
+            bwalk_row<RP, 0, 2, FAST, HOT, TAME>(a, s, W, R + 2, xs, lane, out_row0, out_row1, C, rs, rd);
+            bwalk_row<RP, 1, 3, FAST, HOT, TAME>(a, s, W, R + 3, xs, lane, out_row0, out_row1, C, rs, rd);
+        }
+        return;
     }
+    bwalk_row<RP, 0, 0, FAST, HOT, TAME, false>(a, s, W, row0 - 2, xs, lane, out_row0, out_row1, C, rs, rd);
+    bwalk_row<RP, 1, 1, FAST, HOT, TAME, false>(a, s, W, row0 - 1, xs, lane, out_row0, out_row1, C, rs, rd);
+    bwalk_row<RP, 0, 2, FAST, HOT, TAME, false>(a, s, W, row0, xs, lane, out_row0, out_row1, C, rs, rd);
+    int R = row0 + 1;
+    for (; R + 3 <= out_row1; R += 4) {
+        bwalk_row<RP, 1, 3, FAST, HOT, TAME>(a, s, W, R, xs, lane, out_row0, out_row1, C, rs, rd);
+        bwalk_row<RP, 0, 0, FAST, HOT, TAME>(a, s, W, R + 1, xs, lane, out_row0, out_row1, C, rs, rd);
+        bwalk_row<RP, 1, 1, FAST, HOT, TAME>(a, s, W, R + 2, xs, lane, out_row0, out_row1, C, rs, rd);
+        bwalk_row<RP, 0, 2, FAST, HOT, TAME>(a, s, W, R + 3, xs, lane, out_row0, out_row1, C, rs, rd);
+    }
+    if (R <= out_row1) bwalk_row<RP, 1, 3, FAST, HOT, TAME>(a, s, W, R, xs, lane, out_row0, out_row1, C, rs, rd);
+    if (R + 1 <= out_row1)
+        bwalk_row<RP, 0, 0, FAST, HOT, TAME>(a, s, W, R + 1, xs, lane, out_row0, out_row1, C, rs, rd);
+    if (R + 2 <= out_row1)
+        bwalk_row<RP, 1, 1, FAST, HOT, TAME>(a, s, W, R + 2, xs, lane, out_row0, out_row1, C, rs, rd);
 }
 
 struct BWalkPlan {

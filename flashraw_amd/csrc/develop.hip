@@ -237,8 +237,9 @@ __global__ __launch_bounds__(256) void k_develop_ahd(DevArgs a, uint32_t tiles_x
 // Bilinear row walker launcher (the walker is in bil_walk.h).
 
 template <int RP, bool FAST, bool TAME>
-__global__ __launch_bounds__(BWALK_WAVES * 64) void k_develop_bilinear_walk(DevArgs a, BWalkPlan P,
+__global__ __launch_bounds__(BWALK_WAVES * 64) void k_develop_bilinear_walk(DevArgs args, BWalkPlan P,
                                                                             uint32_t walk_blocks) {
+    const DevArgs &a = kernel_args(args);
     __shared__ Lds s;
     load_lds(a, s);
     __syncthreads();

@@ -6,7 +6,8 @@
 namespace frh {
 
 template <int RP, bool FAST, bool TAME>
-__global__ __launch_bounds__(WALK_WAVES * 64) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_develop_ahd_walk(DevArgs a, WalkPlan P, uint32_t walk_blocks) {
+__global__ __launch_bounds__(WALK_WAVES * 64) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_develop_ahd_walk(DevArgs args, WalkPlan P, uint32_t walk_blocks) {
+    const DevArgs &a = kernel_args(args);
     __shared__ Lds s;
     __shared__ float wt[73 * 73];
     load_lds(a, s);
