@@ -12,6 +12,7 @@
 // kernels write only each frame's pixels), alone or in any mix.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -109,21 +110,31 @@ struct frh_server {
         if (e) free_events.push_back(e);
         e = nullptr;
     }
-    // A device buffer of at least `bytes` from the cache or hipMalloc (sizes rounded to 32 MiB so
-    // buffers of similar jobs are interchangeable); idle buffers are freed when the cache would
-    // push the total past the budget.
-    void *get(uint64_t bytes, uint64_t *cap) {
-        const uint64_t g = 32ull << 20;
-        const uint64_t want = (bytes + g - 1) / g * g;
+    // A device buffer of at least `bytes` from the cache or hipMalloc. Sizes are rounded up to a
+    // size class (a multiple of an eighth of the power of two at or below the size, at least
+    // 64 KiB: at most 12.5 % more) so buffers of similar jobs are interchangeable; a cached buffer
+    // is reused when its capacity exceeds the rounded size by at most `room` (the caller's slack
+    // under the budget, reduced by what the reuse takes). Every buffer the server holds, in use or
+    // idle, counts against the budget: before hipMalloc, idle buffers are freed until in-use
+    // (`used`, plus `taken` for the job being admitted), idle and new bytes fit.
+    static uint64_t rounded(uint64_t bytes) {
+        uint64_t p = 1;
+        while (p <= bytes / 2) p *= 2;
+        const uint64_t g = std::max<uint64_t>(p / 8, 64ull << 10);
+        return (bytes + g - 1) / g * g;
+    }
+    void *get(uint64_t bytes, uint64_t *cap, uint64_t &room, uint64_t taken) {
+        const uint64_t want = rounded(bytes);
         auto it = cache.lower_bound(want);
-        if (it != cache.end() && it->first <= 2 * want) {
+        if (it != cache.end() && it->first - want <= room) {
             void *p = it->second;
             *cap = it->first;
+            room -= it->first - want;
             cached -= it->first;
             cache.erase(it);
             return p;
         }
-        while (!cache.empty() && used + cached + want > cfg.device_budget_bytes + (cfg.device_budget_bytes >> 2)) {
+        while (!cache.empty() && used + taken + cached + want > cfg.device_budget_bytes) {
             auto last = std::prev(cache.end());
             (void)hipFree(last->second);
             cached -= last->first;
@@ -162,21 +173,23 @@ struct frh_server {
 // One pass over the pipeline; returns whether anything moved.
 bool frh_server::step() {
     bool moved = false;
-    // 1. admission, in arrival order, within the device budget
+    // 1. admission, in arrival order, within the device budget (counted in allocated capacities)
     while (!pending.empty()) {
         auto j = pending.front();
-        if (j->dev_bytes > cfg.device_budget_bytes) {
+        const uint64_t need = rounded(j->in_bytes) * (j->denoise ? 2 : 1) + rounded(j->out_bytes);
+        if (need > cfg.device_budget_bytes) {
             pending.pop_front();
             fail(FRH_ERROR_LIMIT_EXCEEDED, "job larger than the server's device budget");
             finish(j, FRH_ERROR_LIMIT_EXCEEDED);
             moved = true;
             continue;
         }
-        if (used + j->dev_bytes > cfg.device_budget_bytes) break;
+        if (used + need > cfg.device_budget_bytes) break;
         pending.pop_front();
-        j->d_in = get(j->in_bytes, &j->cap_in);
-        j->d_dn = j->denoise ? get(j->in_bytes, &j->cap_dn) : nullptr;
-        j->d_out = get(j->out_bytes, &j->cap_out);
+        uint64_t room = cfg.device_budget_bytes - used - need;
+        j->d_in = get(j->in_bytes, &j->cap_in, room, 0);
+        j->d_dn = j->denoise ? get(j->in_bytes, &j->cap_dn, room, j->cap_in) : nullptr;
+        j->d_out = get(j->out_bytes, &j->cap_out, room, j->cap_in + j->cap_dn);
         j->ev_up = event();
         j->ev_down = event();
         if (!j->d_in || !j->d_out || (j->denoise && !j->d_dn) || !j->ev_up || !j->ev_down) {
@@ -184,6 +197,7 @@ bool frh_server::step() {
             moved = true;
             continue;
         }
+        j->dev_bytes = j->cap_in + j->cap_dn + j->cap_out;
         used += j->dev_bytes;
         {
             std::lock_guard<std::mutex> g(mu);
@@ -201,6 +215,8 @@ bool frh_server::step() {
                                               hipMemcpyHostToDevice, us);
         if (e == hipSuccess) e = hipEventRecord(j->ev_up, us);
         if (e != hipSuccess) {
+            // a copy may be queued into d_in: wait for the stream before the buffer goes back
+            (void)hipStreamSynchronize(us);
             used -= j->dev_bytes;
             hip_fail(e, "upload");
             finish(j, FRH_ERROR_GPU_UNAVAILABLE);
@@ -234,6 +250,7 @@ bool frh_server::step() {
                 d.height = j->h;
                 d.stride = j->w;
                 if (frh_denoise_cfa(ctx, &d, j->d_in, j->d_dn, nullptr, comp) != FRH_OK) {
+                    (void)hipStreamSynchronize(comp);  // work it queued may still use the buffers
                     used -= j->dev_bytes;
                     finish(j, FRH_ERROR_INTERNAL);
                     continue;
@@ -273,6 +290,9 @@ bool frh_server::step() {
             if (rc == FRH_OK) e = batch_run(f.batch, comp);
             if (rc == FRH_OK && e == hipSuccess) e = hipEventRecord(f.ev_done, comp);
             if (rc != FRH_OK || e != hipSuccess) {
+                // denoise passes (and part of the batch) may be queued on comp and use the jobs'
+                // buffers: wait for it before they go back
+                (void)hipStreamSynchronize(comp);
                 for (auto &j : f.jobs) {
                     used -= j->dev_bytes;
                     finish(j, rc != FRH_OK ? rc : FRH_ERROR_GPU_UNAVAILABLE);
@@ -290,6 +310,10 @@ bool frh_server::step() {
                         de = hipMemcpyAsync(j->host_out, j->d_out, j->out_bytes, hipMemcpyDeviceToHost, ds);
                     if (de == hipSuccess) de = hipEventRecord(j->ev_down, ds);
                     if (de != hipSuccess) {
+                        // the job finishes at once (step 5): its buffers go back only after the
+                        // batch, which writes d_out, and any queued copy are done
+                        (void)hipEventSynchronize(f.ev_done);
+                        (void)hipStreamSynchronize(ds);
                         hip_fail(de, "download");
                         j->status = FRH_ERROR_GPU_UNAVAILABLE;
                     }
@@ -427,7 +451,6 @@ extern "C" int32_t frh_server_submit(frh_server *s, const frh_frame *frame, cons
             j->denoise = true;
             j->dn = *denoise;
         }
-        j->dev_bytes = j->in_bytes * (j->denoise ? 2 : 1) + j->out_bytes;
         j->host_out = host_out;
         j->t_submit = Clock::now();
         {

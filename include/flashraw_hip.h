@@ -424,7 +424,10 @@ int32_t frh_nr_model_fit(frh_context *ctx, const float *stats, uint32_t count, u
 typedef struct frh_server frh_server;
 typedef struct frh_server_config {
     uint32_t size;
-    uint64_t device_budget_bytes;  /* admission bound on resident job buffers (0: 64 GiB) */
+    uint64_t device_budget_bytes;  /* bound on every device buffer the server holds (0: 64 GiB): the
+                                    * buffers of admitted jobs, counted at their allocated sizes (size
+                                    * classes at most 12.5 % above the request), and the idle ones kept
+                                    * for reuse */
     uint32_t max_batch_frames;     /* frames per batched launch (0: 64) */
     uint32_t max_inflight_batches; /* batches queued on the device at once (0: 2) */
 } frh_server_config;
@@ -440,7 +443,7 @@ typedef struct frh_job_result {
 typedef struct frh_server_stats {
     uint32_t size;
     uint64_t jobs_submitted, jobs_done, jobs_failed, batches, batched_frames;
-    uint64_t peak_device_bytes, device_budget_bytes;
+    uint64_t peak_device_bytes, device_budget_bytes; /* peak: allocated bytes of admitted jobs */
 } frh_server_stats;
 int32_t frh_server_create(frh_context *ctx, const frh_server_config *cfg, frh_server **out);
 /* Queue a job; `denoise` NULL for none (otherwise its width/height/stride are the frame's). */
