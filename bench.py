@@ -49,6 +49,8 @@ def _parse():
                    help="denoise mode: MFMA operand precision, or the scalar f32 variant")
     p.add_argument("--seg-rows", type=int, default=0, help="denoise mode: 8-row tile rows per wave (0: default)")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    p.add_argument("--waves-per-block", type=int, default=0,
+                   help="with --strip-rows: the walkers' waves per block (1, 2, 4, 8, or 16 for bilinear; 0: 4)")
     p.add_argument("--strip-rows", type=int, default=0,
                    help="use this strip height instead of autotuning (profiled runs, where the autotuner's "
                         "own launches would mix into the per-kernel statistics)")
@@ -726,6 +728,8 @@ def main():
 
     if args.strip_rows:
         os.environ["FLASHRAW_AHD_ROWS" if args.demosaic == "ahd" else "FLASHRAW_BILINEAR_ROWS"] = str(args.strip_rows)
+        if args.waves_per_block:
+            os.environ["FLASHRAW_AHD_WPB" if args.demosaic == "ahd" else "FLASHRAW_BILINEAR_WPB"] = str(args.waves_per_block)
     ctx = F.context()
     W, H = 6048, 4024
     mos = synth.frame_24mp()
@@ -836,7 +840,8 @@ def main():
                                + (f" (trained Bayer tables, beta {args.beta:g})" if args.demosaic == "edge" else "")
                                + ", resident in HBM",
                    "frame_mp": round(mp, 3), "demosaic": args.demosaic, "output": "RGBA8",
-                   "max_abs_diff_vs_oracle_lsb": max_diff, "strip_rows": rows, "strip_rows_autotuned": not args.strip_rows},
+                   "max_abs_diff_vs_oracle_lsb": max_diff, "strip_rows": rows,
+                   "waves_per_block": plan.tile_shape[1], "strip_rows_autotuned": not args.strip_rows},
         "roofline": {"bound": _bound(valu, achieved / HBM_PEAK_GBS), "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": _traffic(tag), "valu": valu,

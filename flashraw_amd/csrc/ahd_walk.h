@@ -23,7 +23,7 @@
 namespace frh {
 
 constexpr int WALK_COLS = 116;  // output columns per strip (lanes 3..60)
-constexpr int WALK_WAVES = 4;   // waves per block
+constexpr int WALK_WAVES = 4;   // waves per block of the batch kernels
 
 // Lightness step and squared chroma distance between two pixels of one candidate
 // (ahd.rs:172-180): |L_q - L_p| and (a_q - a_p)^2 + (b_q - b_p)^2 (the same from either end).

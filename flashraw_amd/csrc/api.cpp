@@ -331,11 +331,12 @@ static std::tuple<int, uint32_t, uint32_t, int> tune_key(bool ahd, const DevArgs
     return std::make_tuple(ahd ? 1 : 0, a.dev_w, a.dev_h, (a.out_mode == OUT_RGBA8 && a.out_mode2 == OUT_NONE) ? 1 : 0);
 }
 
-// The autotuned strip height for this geometry, or `fallback`.
-static uint32_t tuned_rows(const frh_context *ctx, bool ahd, const DevArgs &a, uint32_t fallback) {
+// The autotuned tile shape (strip height, waves per block) for this geometry, or (fallback, 0).
+static std::pair<uint32_t, uint32_t> tuned_shape(const frh_context *ctx, bool ahd, const DevArgs &a,
+                                                 uint32_t fallback) {
     std::lock_guard<std::mutex> g(const_cast<frh_context *>(ctx)->mu);
     auto it = ctx->tuned.find(tune_key(ahd, a));
-    return it == ctx->tuned.end() ? fallback : it->second;
+    return it == ctx->tuned.end() ? std::make_pair(fallback, 0u) : it->second;
 }
 
 // Everything a develop launch needs, validated (run.rs:439-646).
@@ -841,8 +842,10 @@ int32_t frh::plan_develop(const frh_context *ctx, const frh_frame *f, const frh_
     {
         const char *impl = getenv("FLASHRAW_BILINEAR_IMPL");
         a.bil_impl = (impl && strcmp(impl, "tiled") == 0) ? 1u : 0u;
-        const char *rows = getenv("FLASHRAW_BILINEAR_ROWS");
-        a.bil_rows = rows ? (uint32_t)atoi(rows) : tuned_rows(ctx, false, a, ctx->bil_rows);
+        const char *rows = getenv("FLASHRAW_BILINEAR_ROWS"), *wpb = getenv("FLASHRAW_BILINEAR_WPB");
+        const auto t = tuned_shape(ctx, false, a, ctx->bil_rows);
+        a.bil_rows = rows ? (uint32_t)atoi(rows) : t.first;
+        a.bil_wpb = wpb ? (uint32_t)atoi(wpb) : t.second;
     }
     // The display outputs' tone and output curve (run.rs:626-675)
     if ((rc = output_tables(mctx, spec->tone, spec->curve, a))) return rc;
@@ -878,8 +881,10 @@ int32_t frh::plan_develop(const frh_context *ctx, const frh_frame *f, const frh_
             // LDS-tiled AHD kernel instead of the row walker, and the walker's rows per strip.
             const char *impl = getenv("FLASHRAW_AHD_IMPL");
             a.ahd_impl = (impl && strcmp(impl, "tiled") == 0) ? 1u : 0u;
-            const char *rows = getenv("FLASHRAW_AHD_ROWS");
-            a.ahd_rows = rows ? (uint32_t)atoi(rows) : tuned_rows(ctx, true, a, ctx->ahd_rows);
+            const char *rows = getenv("FLASHRAW_AHD_ROWS"), *wpb = getenv("FLASHRAW_AHD_WPB");
+            const auto t = tuned_shape(ctx, true, a, ctx->ahd_rows);
+            a.ahd_rows = rows ? (uint32_t)atoi(rows) : t.first;
+            a.ahd_wpb = wpb ? (uint32_t)atoi(wpb) : t.second;
             p.ahd = true;
         } else {
             // the edge-aware filters (edgeaware/mod.rs:57-160): P * P canonical positions
@@ -1146,23 +1151,36 @@ extern "C" int32_t frh_plan_autotune(frh_plan *plan, const void *src, void *dst,
         frh_context *ctx = plan->ctx;
         hipStream_t st = (hipStream_t)stream;
         const bool ahd = plan->p.ahd;
-        std::vector<uint32_t> cands;
-        if (ahd)
-            for (uint32_t r = 16; r <= 128; r += 4) cands.push_back(r);
-        else
-            for (uint32_t r = 8; r <= 64; r += 4) cands.push_back(r);
+        // the tile shape: strip height (rows one wave develops) x waves per block; the LDS-tiled
+        // alternates have no shape to tune
+        if (ahd ? plan->p.a.ahd_impl != 0 : plan->p.a.bil_impl != 0) {
+            if (best) *best = 0;
+            return FRH_OK;
+        }
+        std::vector<std::pair<uint32_t, uint32_t>> cands;
+        for (uint32_t w = 1; w <= (ahd ? AHD_MAX_WAVES : BIL_MAX_WAVES); w *= 2) {
+            if (ahd)
+                for (uint32_t r = 12; r <= 132; r += 6) cands.push_back({r, w});
+            else
+                for (uint32_t r = 8; r <= 64; r += 4) cands.push_back({r, w});
+        }
         hipEvent_t e0, e1;
         if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
             return fail(FRH_ERROR_GPU_UNAVAILABLE, "hipEventCreate failed");
         float best_ms = 1e30f;
         uint32_t best_rows = ahd ? plan->p.a.ahd_rows : plan->p.a.bil_rows;
+        uint32_t best_wpb = ahd ? plan->p.a.ahd_wpb : plan->p.a.bil_wpb;
         int32_t rc = FRH_OK;
-        for (uint32_t r : cands) {
+        for (const auto &c : cands) {
+            const uint32_t r = c.first;
             Plan p = plan->p;
-            if (ahd)
+            if (ahd) {
                 p.a.ahd_rows = r;
-            else
+                p.a.ahd_wpb = c.second;
+            } else {
                 p.a.bil_rows = r;
+                p.a.bil_wpb = c.second;
+            }
             if (p.in.kind == FRH_INPUT_F32)
                 p.a.srcf = static_cast<const float *>(src) + plan->src_off;
             else
@@ -1183,24 +1201,38 @@ extern "C" int32_t frh_plan_autotune(frh_plan *plan, const void *src, void *dst,
             if (ms < best_ms) {
                 best_ms = ms;
                 best_rows = r;
+                best_wpb = c.second;
             }
         }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         if (rc != FRH_OK) return rc;
-        if (ahd)
+        if (ahd) {
             plan->p.a.ahd_rows = best_rows;
-        else
+            plan->p.a.ahd_wpb = best_wpb;
+        } else {
             plan->p.a.bil_rows = best_rows;
+            plan->p.a.bil_wpb = best_wpb;
+        }
         {
             std::lock_guard<std::mutex> g(ctx->mu);
-            ctx->tuned[tune_key(ahd, plan->p.a)] = best_rows;
+            ctx->tuned[tune_key(ahd, plan->p.a)] = std::make_pair(best_rows, best_wpb);
         }
         if (best) *best = best_rows;
         return FRH_OK;
     } catch (...) {
         return fail(FRH_ERROR_INTERNAL, "unexpected exception");
     }
+}
+
+extern "C" int32_t frh_plan_tile_shape(const frh_plan *plan, uint32_t *rows, uint32_t *waves_per_block) {
+    if (!plan) return fail(FRH_ERROR_INVALID_ARGUMENT, "NULL plan");
+    const DevArgs &a = plan->p.a;
+    const bool walker = !a.generic && !a.edge_aware && (plan->p.ahd ? a.ahd_impl == 0 : a.bil_impl == 0);
+    if (rows) *rows = walker ? (plan->p.ahd ? a.ahd_rows : a.bil_rows) : 0u;
+    if (waves_per_block)
+        *waves_per_block = walker ? (plan->p.ahd ? walk_wpb(a.ahd_wpb, AHD_MAX_WAVES) : walk_wpb(a.bil_wpb, BIL_MAX_WAVES)) : 0u;
+    return FRH_OK;
 }
 
 extern "C" uint64_t frh_plan_output_bytes(const frh_plan *plan, int32_t which) {

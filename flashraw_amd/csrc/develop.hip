@@ -237,7 +237,7 @@ __global__ __launch_bounds__(256) void k_develop_ahd(DevArgs a, uint32_t tiles_x
 // Bilinear row walker launcher (the walker is in bil_walk.h).
 
 template <int RP, bool FAST, bool TAME>
-__global__ __launch_bounds__(BWALK_WAVES * 64) void k_develop_bilinear_walk(DevArgs args, BWalkPlan P,
+__global__ __launch_bounds__(BIL_MAX_WAVES * 64) void k_develop_bilinear_walk(DevArgs args, BWalkPlan P,
                                                                             uint32_t walk_blocks) {
     const DevArgs &a = kernel_args(args);
     __shared__ Lds s;
@@ -252,25 +252,25 @@ __global__ __launch_bounds__(BWALK_WAVES * 64) void k_develop_bilinear_walk(DevA
     }
     // the wave index is the same for every lane; readfirstlane lets the compiler keep all the strip
     // bookkeeping in scalar registers
-    const int item = (int)uniform(blk * BWALK_WAVES + (threadIdx.x >> 6));
+    const int item = (int)uniform(blk * (blockDim.x >> 6) + (threadIdx.x >> 6));
     if (item >= P.strips * P.chunks) return;
     bwalk_item<RP, FAST, TAME>(a, P, s, item);
 }
 
 template <int RP>
-static void launch_bil_rp(const DevArgs &a, const BWalkPlan &P, uint32_t walk_blocks, uint32_t grid,
+static void launch_bil_rp(const DevArgs &a, const BWalkPlan &P, uint32_t walk_blocks, uint32_t grid, uint32_t block,
                           hipStream_t stream) {
     // the tame frame's finish takes the matrix pre-scaled by 4096 (every product is 0 or normal:
     // the bound of ahd_tame covers the bilinear means too)
     const bool fast = fast_rgba8(a), tame = a.ahd_tame != 0;
     if (fast && tame)
-        hipLaunchKernelGGL((k_develop_bilinear_walk<RP, true, true>), dim3(grid), dim3(BWALK_WAVES * 64), 0, stream,
+        hipLaunchKernelGGL((k_develop_bilinear_walk<RP, true, true>), dim3(grid), dim3(block), 0, stream,
                            a, P, walk_blocks);
     else if (fast)
-        hipLaunchKernelGGL((k_develop_bilinear_walk<RP, true, false>), dim3(grid), dim3(BWALK_WAVES * 64), 0,
+        hipLaunchKernelGGL((k_develop_bilinear_walk<RP, true, false>), dim3(grid), dim3(block), 0,
                            stream, a, P, walk_blocks);
     else
-        hipLaunchKernelGGL((k_develop_bilinear_walk<RP, false, false>), dim3(grid), dim3(BWALK_WAVES * 64), 0,
+        hipLaunchKernelGGL((k_develop_bilinear_walk<RP, false, false>), dim3(grid), dim3(block), 0,
                            stream, a, P, walk_blocks);
 }
 
@@ -278,15 +278,16 @@ static hipError_t launch_develop_bilinear_walk(const DevArgs &a, hipStream_t str
     const int rp = red_position(a);
     if (rp < 0) return hipErrorInvalidValue;
     const BWalkPlan P = make_bwalk_plan(a, rows_per_strip);
-    const uint32_t walk_blocks = (uint32_t)((P.strips * P.chunks + BWALK_WAVES - 1) / BWALK_WAVES);
-    const uint32_t border_blocks = (P.border.count + BWALK_WAVES * 64 - 1) / (BWALK_WAVES * 64);
+    const uint32_t wpb = walk_wpb(a.bil_wpb, BIL_MAX_WAVES), block = wpb * 64;
+    const uint32_t walk_blocks = (uint32_t)((P.strips * P.chunks + wpb - 1) / wpb);
+    const uint32_t border_blocks = (P.border.count + block - 1) / block;
     const uint32_t grid = walk_blocks + border_blocks;
     if (grid == 0) return hipSuccess;
     switch (rp) {
-    case 0: launch_bil_rp<0>(a, P, walk_blocks, grid, stream); break;
-    case 1: launch_bil_rp<1>(a, P, walk_blocks, grid, stream); break;
-    case 2: launch_bil_rp<2>(a, P, walk_blocks, grid, stream); break;
-    default: launch_bil_rp<3>(a, P, walk_blocks, grid, stream); break;
+    case 0: launch_bil_rp<0>(a, P, walk_blocks, grid, block, stream); break;
+    case 1: launch_bil_rp<1>(a, P, walk_blocks, grid, block, stream); break;
+    case 2: launch_bil_rp<2>(a, P, walk_blocks, grid, block, stream); break;
+    default: launch_bil_rp<3>(a, P, walk_blocks, grid, block, stream); break;
     }
     return hipGetLastError();
 }

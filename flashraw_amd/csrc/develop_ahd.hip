@@ -6,7 +6,7 @@
 namespace frh {
 
 template <int RP, bool FAST, bool TAME>
-__global__ __launch_bounds__(WALK_WAVES * 64) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_develop_ahd_walk(DevArgs args, WalkPlan P, uint32_t walk_blocks) {
+__global__ __launch_bounds__(AHD_MAX_WAVES * 64) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_develop_ahd_walk(DevArgs args, WalkPlan P, uint32_t walk_blocks) {
     const DevArgs &a = kernel_args(args);
     __shared__ Lds s;
     __shared__ float wt[73 * 73];
@@ -23,26 +23,26 @@ __global__ __launch_bounds__(WALK_WAVES * 64) __attribute__((amdgpu_waves_per_eu
     __syncthreads();
     // the wave index is the same for every lane; readfirstlane lets the compiler keep all the strip
     // bookkeeping in scalar registers
-    const int item = (int)uniform(blk * WALK_WAVES + (threadIdx.x >> 6));
+    const int item = (int)uniform(blk * (blockDim.x >> 6) + (threadIdx.x >> 6));
     if (item >= P.strips * P.chunks) return;
     walk_item<RP, FAST, TAME>(a, P, s, wt, item);
 }
 
 template <int RP>
-static void launch_ahd_rp(const DevArgs &a, const WalkPlan &P, uint32_t walk_blocks, uint32_t grid,
+static void launch_ahd_rp(const DevArgs &a, const WalkPlan &P, uint32_t walk_blocks, uint32_t grid, uint32_t block,
                           hipStream_t stream) {
     const bool fast = fast_rgba8(a), tame = a.ahd_tame != 0;
     if (fast && tame)
-        hipLaunchKernelGGL((k_develop_ahd_walk<RP, true, true>), dim3(grid), dim3(WALK_WAVES * 64), 0, stream, a, P,
+        hipLaunchKernelGGL((k_develop_ahd_walk<RP, true, true>), dim3(grid), dim3(block), 0, stream, a, P,
                            walk_blocks);
     else if (fast)
-        hipLaunchKernelGGL((k_develop_ahd_walk<RP, true, false>), dim3(grid), dim3(WALK_WAVES * 64), 0, stream, a,
+        hipLaunchKernelGGL((k_develop_ahd_walk<RP, true, false>), dim3(grid), dim3(block), 0, stream, a,
                            P, walk_blocks);
     else if (tame)
-        hipLaunchKernelGGL((k_develop_ahd_walk<RP, false, true>), dim3(grid), dim3(WALK_WAVES * 64), 0, stream, a,
+        hipLaunchKernelGGL((k_develop_ahd_walk<RP, false, true>), dim3(grid), dim3(block), 0, stream, a,
                            P, walk_blocks);
     else
-        hipLaunchKernelGGL((k_develop_ahd_walk<RP, false, false>), dim3(grid), dim3(WALK_WAVES * 64), 0, stream, a,
+        hipLaunchKernelGGL((k_develop_ahd_walk<RP, false, false>), dim3(grid), dim3(block), 0, stream, a,
                            P, walk_blocks);
 }
 
@@ -76,15 +76,16 @@ hipError_t launch_develop_ahd_walk(const DevArgs &a, hipStream_t stream, int row
     const int rp = red_position(a);
     if (rp < 0) return hipErrorInvalidValue;
     const WalkPlan P = make_walk_plan(a, rows_per_strip);
-    const uint32_t walk_blocks = (uint32_t)((P.strips * P.chunks + WALK_WAVES - 1) / WALK_WAVES);
-    const uint32_t border_blocks = (P.border.count + WALK_WAVES * 64 - 1) / (WALK_WAVES * 64);
+    const uint32_t wpb = walk_wpb(a.ahd_wpb, AHD_MAX_WAVES), block = wpb * 64;
+    const uint32_t walk_blocks = (uint32_t)((P.strips * P.chunks + wpb - 1) / wpb);
+    const uint32_t border_blocks = (P.border.count + block - 1) / block;
     const uint32_t grid = walk_blocks + border_blocks;
     if (grid == 0) return hipSuccess;
     switch (rp) {
-    case 0: launch_ahd_rp<0>(a, P, walk_blocks, grid, stream); break;
-    case 1: launch_ahd_rp<1>(a, P, walk_blocks, grid, stream); break;
-    case 2: launch_ahd_rp<2>(a, P, walk_blocks, grid, stream); break;
-    default: launch_ahd_rp<3>(a, P, walk_blocks, grid, stream); break;
+    case 0: launch_ahd_rp<0>(a, P, walk_blocks, grid, block, stream); break;
+    case 1: launch_ahd_rp<1>(a, P, walk_blocks, grid, block, stream); break;
+    case 2: launch_ahd_rp<2>(a, P, walk_blocks, grid, block, stream); break;
+    default: launch_ahd_rp<3>(a, P, walk_blocks, grid, block, stream); break;
     }
     return hipGetLastError();
 }

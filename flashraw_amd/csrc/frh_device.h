@@ -96,8 +96,10 @@ struct DevArgs {
     uint32_t ahd_halo;       // 6: pixels nearer the border than this are bilinear (ahd.rs:33)
     uint32_t ahd_impl;       // 0: register row walker, 1: LDS-tiled (both give the same bytes)
     uint32_t ahd_rows;       // rows per walker strip (even)
+    uint32_t ahd_wpb;        // waves per block of the AHD walker (1, 2, 4 or 8; 0: 4)
     uint32_t bil_impl;       // bilinear: 0 register row walker, 1 LDS-tiled (the same bytes)
     uint32_t bil_rows;       // rows per bilinear walker strip (even)
+    uint32_t bil_wpb;        // waves per block of the bilinear walker (1, 2, 4 or 8; 0: 4)
     // edge-aware demosaic: tap lists, canonical position of frame sample (0, 0), smoothing weight
     // (the border width is ahd_halo)
     uint32_t edge_aware;
@@ -136,6 +138,14 @@ struct DevArgs {
     const uint8_t *curve8;
     uint32_t curve8_n;
 };
+
+// Waves per block of a single-frame walker launch, a tile-shape parameter the autotuner chooses with
+// the strip height: a power of two up to `max` (8 for AHD, whose registers allow 3 waves per SIMD
+// and so no 16-wave block; 16 for bilinear), anything else 4.
+constexpr int AHD_MAX_WAVES = 8, BIL_MAX_WAVES = 16;
+__host__ __device__ inline uint32_t walk_wpb(uint32_t w, uint32_t max) {
+    return (w >= 1 && w <= max && (w & (w - 1)) == 0) ? w : 4u;
+}
 
 // Window of a raster for the input kernels (unpack, linearise, normalise): packed payload bytes
 // (kind 2) or u16 / f32 samples (kinds 0 / 1), the window's first sample (x0 counts samples,

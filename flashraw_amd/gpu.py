@@ -236,6 +236,8 @@ def lib():
             L.frh_plan_launch.restype = i32
             L.frh_plan_autotune.argtypes = [vp, vp, vp, vp, vp, i32, ctypes.POINTER(u32)]
             L.frh_plan_autotune.restype = i32
+            L.frh_plan_tile_shape.argtypes = [vp, ctypes.POINTER(u32), ctypes.POINTER(u32)]
+            L.frh_plan_tile_shape.restype = i32
             L.frh_plan_output_bytes.argtypes = [vp, i32]
             L.frh_plan_output_bytes.restype = u64
             L.frh_plan_destroy.argtypes = [vp]
@@ -1356,13 +1358,21 @@ class Plan:
             check(rc)
 
     def autotune(self, src, dst, dst2=None, stream=None, iters: int = 3) -> int:
-        """Time the candidate strip heights on these resident buffers and keep the fastest
-        (frh_plan_autotune); returns it."""
+        """Time the candidate tile shapes (strip height x waves per block) on these resident buffers
+        and keep the fastest (frh_plan_autotune); returns its strip height (tile_shape has both)."""
         best = ctypes.c_uint32()
         check(lib().frh_plan_autotune(self._h, _device_ptr(src), _device_ptr(dst),
                                       None if dst2 is None else _device_ptr(dst2), self.ctx.stream_handle(stream),
                                       iters, ctypes.byref(best)))
         return best.value
+
+    @property
+    def tile_shape(self) -> tuple:
+        """(strip height, waves per block) of the plan's row-walker launch, (0, 0) for the kernels
+        without one (frh_plan_tile_shape)."""
+        rows, wpb = ctypes.c_uint32(), ctypes.c_uint32()
+        check(lib().frh_plan_tile_shape(self._h, ctypes.byref(rows), ctypes.byref(wpb)))
+        return rows.value, wpb.value
 
     KERNELS = ("bilinear_walk", "bilinear_tiled", "ahd_walk", "ahd_tiled", "edge_tame", "edge_general", "generic")
 

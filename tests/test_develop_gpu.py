@@ -264,7 +264,8 @@ def test_autotuned_plan(ctx, demosaic):
     plan = ctx.prepare(F.GpuFrame.from_u16(d_in, w, h), info, spec)
     stream = torch.cuda.Stream()
     rows = plan.autotune(d_in, dst, stream=stream, iters=2)
-    assert (16 <= rows <= 128) if demosaic == "ahd" else (8 <= rows <= 64)
+    assert (12 <= rows <= 132) if demosaic == "ahd" else (8 <= rows <= 64)
+    assert plan.tile_shape[0] == rows and plan.tile_shape[1] in (1, 2, 4, 8, 16)
     with torch.cuda.stream(stream):  # the clear and the launch on one stream: ordered
         dst.zero_()
     plan.launch(d_in, dst, stream=stream)
@@ -275,6 +276,19 @@ def test_autotuned_plan(ctx, demosaic):
     assert bad.size == 0, (rows, len(bad), bad[:4].tolist())
     again = ctx.run(F.GpuFrame.from_u16(mos, w, h), info, F.GpuOp.Develop(spec)).image.download()
     assert np.array_equal(again, want)
+
+
+@pytest.mark.parametrize("demosaic,wpb", [("bilinear", 1), ("bilinear", 2), ("bilinear", 8), ("bilinear", 16),
+                                          ("ahd", 1), ("ahd", 2), ("ahd", 8)])
+def test_waves_per_block_keep_bytes(ctx, monkeypatch, demosaic, wpb):
+    """The walkers' waves per block (a tile-shape parameter the autotuner chooses) change timing
+    only: every block size gives the oracle's bytes, at two strip heights."""
+    mos = synth.mosaic(301, 203, "BGGR", seed=31, edges=30, discs=8, stripes=12)
+    monkeypatch.setenv("FLASHRAW_AHD_WPB" if demosaic == "ahd" else "FLASHRAW_BILINEAR_WPB", str(wpb))
+    for rows in (12, 36):
+        monkeypatch.setenv("FLASHRAW_AHD_ROWS" if demosaic == "ahd" else "FLASHRAW_BILINEAR_ROWS", str(rows))
+        for out in (F.DevelopOutput.Rgba8, F.DevelopOutput.LinearRgb16):
+            _compare(ctx, mos, "BGGR", demosaic=demosaic, output=out, origin=(1, 1), what=f"wpb {wpb} rows {rows}")
 
 
 @pytest.mark.parametrize("cfa", ["RGGB", "BGGR", "GRBG", "GBRG"])
