@@ -188,7 +188,7 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void *base, uint32_
 // The offset of a store that must not happen: past any resource (records stay below 2^31).
 constexpr uint32_t NO_STORE = 0x80000000u;
 
-template <typename T>
+template <typename T, bool F32OUT>
 __global__ __launch_bounds__(256) void k_dn_kron(DnkArgs A) {
     const int lane = threadIdx.x & 63;
     const int wave = (int)__builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
@@ -231,28 +231,19 @@ __global__ __launch_bounds__(256) void k_dn_kron(DnkArgs A) {
     // this lane's block columns: class (*, 0) at xb0, class (*, 1) at xb0 + 4
     const int xb0 = x0 + 8 * (n - 1);
     const bool ok0 = xb0 >= 0 && xb0 <= plx, ok1 = xb0 + 4 >= 0 && xb0 + 4 <= plx;
-    const bool border_x = x0 == 0 || x0 + 128 > plx;  // some block of the strip is outside
     const int ca = min(max(xb0, 0), pw - 4), cb = min(max(xb0 + 4, 0), pw - 4);
-    // byte offsets of this lane's two 16-byte loads from the start of mosaic row 2 yb
-    const uint32_t la = (uint32_t)(((size_t)(2 * g + dy) * stride + 2 * ca) * 2);
-    const uint32_t lb = (uint32_t)(((size_t)(2 * g + dy) * stride + 2 * cb) * 2);
 
     // The raw samples of plane rows yb + g (rows clamped to the plane): two 16-byte loads per lane.
     struct Raw {
         u4 a, b;
     };
+    // (one path for every row, clamped into the plane: no branch splits the step loop)
     auto issue = [&](int yb) {
         Raw r;
-        if (yb >= 0 && yb + 3 < ph) {
-            const char *base = reinterpret_cast<const char *>(src + (size_t)(2 * yb) * stride);
-            r.a = *reinterpret_cast<const u4 *>(base + la);
-            r.b = *reinterpret_cast<const u4 *>(base + lb);
-        } else {
-            const int rr = min(max(yb + g, 0), ph - 1);
-            const uint16_t *p = src + (size_t)(2 * rr + dy) * stride;
-            r.a = *reinterpret_cast<const u4 *>(p + 2 * ca);
-            r.b = *reinterpret_cast<const u4 *>(p + 2 * cb);
-        }
+        const int rr = min(max(yb + g, 0), ph - 1);
+        const uint16_t *p = src + (size_t)(2 * rr + dy) * stride;
+        r.a = *reinterpret_cast<const u4 *>(p + 2 * ca);
+        r.b = *reinterpret_cast<const u4 *>(p + 2 * cb);
         return r;
     };
     // Both planes' class-(*, 0) samples of a row.
@@ -325,6 +316,70 @@ __global__ __launch_bounds__(256) void k_dn_kron(DnkArgs A) {
         }
     }
 
+    // Stores. Pixel (row, col) of output register j is (y0 + 4 (j & 1) + g, xb0 + 4 (j >> 1) + r),
+    // both planes interleaved; where fewer than four blocks cover a pixel (the plane's border rows
+    // and columns) its sum is scaled by the cover factor 2 per short direction. The u16 sample is
+    // round(sum * f + offset + black), one fma onto the per-row offset; the f32 output (F32OUT)
+    // is sum * f + offset. Stores of pixels a lane does not own address past the resource.
+    float cf[2][4];  // column cover factors of this lane's two column groups
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int col = xb0 + 4 * h + r;
+            cf[h][r] = (col < 4 || col > plx + 3) ? 2.0f : 1.0f;
+        }
+    const float bh0 = black[0] + 0.5f, bh1 = black[1] + 0.5f;
+    auto store_j = [&](int j, const f4 O[2][4], const float ot[2], const float om[2], int y0) {
+        {
+            const int row = y0 + 4 * (j & 1) + g;
+            const int col = xb0 + 4 * (j >> 1);
+            // (bitwise, not short-circuit: a select, not a branch)
+            const bool owned = (n != 0) & !((n == 15) & (j >= 2)) & (row < ph) & (col < pw);
+            const uint32_t o = (uint32_t)((2 * row + dy) * stride + 2 * col);  // sample index
+            const float o0 = (j & 1) ? om[0] : ot[0], o1 = (j & 1) ? om[1] : ot[1];
+            const float c0 = o0 + bh0, c1 = o1 + bh1;
+            const float fy = (row < 4 || row > ply + 3) ? 2.0f : 1.0f;
+            uint32_t q[4];
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const float f = fy * cf[j >> 1][r];
+                const float u0 = __builtin_amdgcn_fmed3f(fmaf(O[0][j][r], f, c0), 0.0f, 65535.0f);
+                const float u1 = __builtin_amdgcn_fmed3f(fmaf(O[1][j][r], f, c1), 0.0f, 65535.0f);
+                q[r] = (uint32_t)u0 | ((uint32_t)u1 << 16);
+            }
+            __builtin_amdgcn_raw_buffer_store_b128(words(q), rdst, owned ? o * 2u : NO_STORE, 0, 0);
+            if (F32OUT) {
+                float v0[4], v1[4];
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const float f = fy * cf[j >> 1][r];
+                    v0[r] = fmaf(O[0][j][r], f, o0);
+                    v1[r] = fmaf(O[1][j][r], f, o1);
+                }
+                const uint32_t of = owned ? o * 4u : NO_STORE;
+                const uint32_t a0[4] = {__builtin_bit_cast(uint32_t, v0[0]), __builtin_bit_cast(uint32_t, v1[0]),
+                                        __builtin_bit_cast(uint32_t, v0[1]), __builtin_bit_cast(uint32_t, v1[1])};
+                const uint32_t a1[4] = {__builtin_bit_cast(uint32_t, v0[2]), __builtin_bit_cast(uint32_t, v1[2]),
+                                        __builtin_bit_cast(uint32_t, v0[3]), __builtin_bit_cast(uint32_t, v1[3])};
+                __builtin_amdgcn_raw_buffer_store_b128(words(a0), rdf, of, 0, 0);
+                __builtin_amdgcn_raw_buffer_store_b128(words(a1), rdf, owned ? of + 16u : NO_STORE, 0, 0);
+            }
+        }
+    };
+    auto store_step = [&](const f4 O[2][4], const float ot[2], const float om[2], int y0) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) store_j(j, O, ot, om, y0);
+    };
+
+    // The previous step's outputs and offsets: stored while this step's products run (the loop
+    // body has no branch, so the compiler interleaves the stores' vector work with the MFMAs).
+    f4 P[2][4];
+#pragma unroll
+    for (int p = 0; p < 2; p++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) P[p][j] = z;
+    float pt[2] = {0.f, 0.f}, pm[2] = {0.f, 0.f};
     // rows y0 + 4 + g and y0 + 8 + g of the next step, loaded one step ahead
     Raw nB = issue(y_first + 4), nC = issue(y_first + 8);
     for (int s = 0; s < nsteps; s++) {
@@ -358,15 +413,35 @@ __global__ __launch_bounds__(256) void k_dn_kron(DnkArgs A) {
             // The overlap-add, accumulated in the inverse products: rows 0..3 (j = 0, 2) of this
             // block row take the previous step's lower blocks (the carry), rows 4..7 (j = 1, 3)
             // this step's; class-(*, 1) blocks add in register j ^ 2, from column n - 1 for j < 2.
+            // Between the first plane's product chains go the previous step's stores, one output
+            // register each (independent vector work under the MFMAs; the first step has none:
+            // its stores address past the resource).
+            const int ys = s > 0 ? y0 - 8 : ph;
             const f4 o0 = W.inv(0, Kt0, C[p][0]);
             const f4 s0 = W.inv(2, Kt1, z);
+            if (p == 0) {
+                store_j(0, P, pt, pm, ys);
+                __builtin_amdgcn_sched_barrier(0);  // keep the stores with these products
+            }
             const f4 o1 = W.inv(0, Km0, W.inv(1, Kt0, z));
             const f4 s1 = W.inv(2, Km1, W.inv(3, Kt1, z));
+            if (p == 0) {
+                store_j(1, P, pt, pm, ys);
+                __builtin_amdgcn_sched_barrier(0);  // keep the stores with these products
+            }
             out[p][2] = W.inv(0, Kt1, W.inv(2, Kt0, C[p][1]));
             out[p][3] = W.inv(0, Km1, W.inv(2, Km0, W.inv(1, Kt1, W.inv(3, Kt0, z))));
+            if (p == 0) {
+                store_j(2, P, pt, pm, ys);
+                __builtin_amdgcn_sched_barrier(0);  // keep the stores with these products
+            }
             const f4 c0 = W.inv(1, Km0, z);
             const f4 s2 = W.inv(3, Km1, z);
             C[p][1] = W.inv(1, Km1, W.inv(3, Km0, z));
+            if (p == 0) {
+                store_j(3, P, pt, pm, ys);
+                __builtin_amdgcn_sched_barrier(0);  // keep the stores with these products
+            }
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 out[p][0][r] = o0[r] + shr1(s0[r]);
@@ -378,58 +453,15 @@ __global__ __launch_bounds__(256) void k_dn_kron(DnkArgs A) {
             wA[p] = wC[p];
         }
         c_ok = mid_ok;
-        // store: pixel (y0 + 4 (j & 1) + g, xb0 + 4 (j >> 1) + r), both planes interleaved; cover
-        // factors where fewer than four blocks cover a pixel (plane borders). Every lane issues its
-        // stores (buffer stores; lanes that own no pixel address past the end), so the loop carries
-        // no branches around memory operations.
-        const bool edge_store = border_x || y0 < 4 || y0 + 8 > ply + 4;
-        const float bh0 = black[0] + 0.5f, bh1 = black[1] + 0.5f;
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int row = y0 + 4 * (j & 1) + g;
-            const int col = xb0 + 4 * (j >> 1);
-            const bool owned = !(n == 0 || (n == 15 && j >= 2) || row >= ph || col >= pw);
-            const uint32_t o = (uint32_t)((2 * row + dy) * stride + 2 * col);  // sample index
-            const float o0 = (j & 1) ? offf[0] : off_top[0], o1 = (j & 1) ? offf[1] : off_top[1];
-            uint32_t q[4];
-            if (!edge_store && !dst_f) {
-                // interior, u16 output only: sample = round(out + offset + black)
-                const float c0 = o0 + bh0, c1 = o1 + bh1;
+        for (int p = 0; p < 2; p++) {
 #pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    const float u0 = fminf(fmaxf(out[0][j][r] + c0, 0.0f), 65535.0f);
-                    const float u1 = fminf(fmaxf(out[1][j][r] + c1, 0.0f), 65535.0f);
-                    q[r] = (uint32_t)u0 | ((uint32_t)u1 << 16);
-                }
-                __builtin_amdgcn_raw_buffer_store_b128(words(q), rdst, owned ? o * 2u : NO_STORE, 0, 0);
-                continue;
-            }
-            float v0[4], v1[4];
-            const float fy = edge_store && (row < 4 || row > ply + 3) ? 2.0f : 1.0f;
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const float f = fy * (edge_store && (col + r < 4 || col + r > plx + 3) ? 2.0f : 1.0f);
-                v0[r] = out[0][j][r] * f + o0;
-                v1[r] = out[1][j][r] * f + o1;
-            }
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const float u0 = fminf(fmaxf(v0[r] + bh0, 0.0f), 65535.0f);
-                const float u1 = fminf(fmaxf(v1[r] + bh1, 0.0f), 65535.0f);
-                q[r] = (uint32_t)u0 | ((uint32_t)u1 << 16);
-            }
-            __builtin_amdgcn_raw_buffer_store_b128(words(q), rdst, owned ? o * 2u : NO_STORE, 0, 0);
-            if (dst_f) {
-                const uint32_t of = owned ? o * 4u : NO_STORE;
-                const uint32_t a0[4] = {__builtin_bit_cast(uint32_t, v0[0]), __builtin_bit_cast(uint32_t, v1[0]),
-                                        __builtin_bit_cast(uint32_t, v0[1]), __builtin_bit_cast(uint32_t, v1[1])};
-                const uint32_t a1[4] = {__builtin_bit_cast(uint32_t, v0[2]), __builtin_bit_cast(uint32_t, v1[2]),
-                                        __builtin_bit_cast(uint32_t, v0[3]), __builtin_bit_cast(uint32_t, v1[3])};
-                __builtin_amdgcn_raw_buffer_store_b128(words(a0), rdf, of, 0, 0);
-                __builtin_amdgcn_raw_buffer_store_b128(words(a1), rdf, owned ? of + 16u : NO_STORE, 0, 0);
-            }
+            for (int j = 0; j < 4; j++) P[p][j] = out[p][j];
+            pt[p] = off_top[p];
+            pm[p] = offf[p];
         }
     }
+    if (nsteps > 0) store_step(P, pt, pm, y_first + 8 * (nsteps - 1));
 }
 
 // f32 -> f16 / bf16 bits, round to nearest even (host).
@@ -494,10 +526,15 @@ int dn_kron_seg_steps(int strips, int nsteps, int num_cus) {
 hipError_t launch_dn_kron(const DnkArgs &a, bool bf16, hipStream_t st) {
     const uint32_t waves = 2u * (uint32_t)a.strips * (uint32_t)a.segs;
     const uint32_t grid = (waves + 3) / 4;
-    if (bf16)
-        hipLaunchKernelGGL(k_dn_kron<__bf16>, dim3(grid), dim3(256), 0, st, a);
+    const bool f32out = a.dst_f != nullptr;
+    if (bf16 && f32out)
+        hipLaunchKernelGGL((k_dn_kron<__bf16, true>), dim3(grid), dim3(256), 0, st, a);
+    else if (bf16)
+        hipLaunchKernelGGL((k_dn_kron<__bf16, false>), dim3(grid), dim3(256), 0, st, a);
+    else if (f32out)
+        hipLaunchKernelGGL((k_dn_kron<_Float16, true>), dim3(grid), dim3(256), 0, st, a);
     else
-        hipLaunchKernelGGL(k_dn_kron<_Float16>, dim3(grid), dim3(256), 0, st, a);
+        hipLaunchKernelGGL((k_dn_kron<_Float16, false>), dim3(grid), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 
