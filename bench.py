@@ -51,6 +51,8 @@ def _parse():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--waves-per-block", type=int, default=0,
                    help="with --strip-rows: the walkers' waves per block (1, 2, 4, 8, or 16 for bilinear; 0: 4)")
+    p.add_argument("--cols-per-lane", type=int, default=0,
+                   help="with --strip-rows: the bilinear walker's frame columns per lane (2 or 4; 0: 2)")
     p.add_argument("--strip-rows", type=int, default=0,
                    help="use this strip height instead of autotuning (profiled runs, where the autotuner's "
                         "own launches would mix into the per-kernel statistics)")
@@ -730,6 +732,8 @@ def main():
         os.environ["FLASHRAW_AHD_ROWS" if args.demosaic == "ahd" else "FLASHRAW_BILINEAR_ROWS"] = str(args.strip_rows)
         if args.waves_per_block:
             os.environ["FLASHRAW_AHD_WPB" if args.demosaic == "ahd" else "FLASHRAW_BILINEAR_WPB"] = str(args.waves_per_block)
+        if args.cols_per_lane and args.demosaic == "bilinear":
+            os.environ["FLASHRAW_BILINEAR_CPL"] = str(args.cols_per_lane)
     ctx = F.context()
     W, H = 6048, 4024
     mos = synth.frame_24mp()
@@ -841,7 +845,8 @@ def main():
                                + ", resident in HBM",
                    "frame_mp": round(mp, 3), "demosaic": args.demosaic, "output": "RGBA8",
                    "max_abs_diff_vs_oracle_lsb": max_diff, "strip_rows": rows,
-                   "waves_per_block": plan.tile_shape[1], "strip_rows_autotuned": not args.strip_rows},
+                   "waves_per_block": plan.tile_shape[1], "cols_per_lane": plan.tile_shape[2],
+                   "strip_rows_autotuned": not args.strip_rows},
         "roofline": {"bound": _bound(valu, achieved / HBM_PEAK_GBS), "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": _traffic(tag), "valu": valu,

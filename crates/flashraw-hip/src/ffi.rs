@@ -277,7 +277,7 @@ unsafe extern "C" {
     pub fn frh_plan_create(ctx: *mut frh_context, frame: *const frh_frame, info: *const frh_frame_info, spec: *const frh_develop_spec, out: *mut *mut frh_plan) -> i32;
     pub fn frh_plan_launch(plan: *const frh_plan, src: *const c_void, dst: *mut c_void, dst2: *mut c_void, stream: *mut c_void) -> i32;
     pub fn frh_plan_autotune(plan: *mut frh_plan, src: *const c_void, dst: *mut c_void, dst2: *mut c_void, stream: *mut c_void, iters: i32, best: *mut u32) -> i32;
-    pub fn frh_plan_tile_shape(plan: *const frh_plan, rows: *mut u32, waves_per_block: *mut u32) -> i32;
+    pub fn frh_plan_tile_shape(plan: *const frh_plan, rows: *mut u32, waves_per_block: *mut u32, cols_per_lane: *mut u32) -> i32;
     pub fn frh_plan_output_bytes(plan: *const frh_plan, which: i32) -> u64;
     pub fn frh_plan_kernel(plan: *const frh_plan) -> i32;
     pub fn frh_plan_destroy(plan: *mut frh_plan);

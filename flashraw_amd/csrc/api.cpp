@@ -22,6 +22,7 @@
 #include "frh_device.h"
 #include "frh_launch.h"
 #include "frh_internal.h"
+#include "bil_walk4.h"
 
 using namespace frh;
 
@@ -331,12 +332,12 @@ static std::tuple<int, uint32_t, uint32_t, int> tune_key(bool ahd, const DevArgs
     return std::make_tuple(ahd ? 1 : 0, a.dev_w, a.dev_h, (a.out_mode == OUT_RGBA8 && a.out_mode2 == OUT_NONE) ? 1 : 0);
 }
 
-// The autotuned tile shape (strip height, waves per block) for this geometry, or (fallback, 0).
-static std::pair<uint32_t, uint32_t> tuned_shape(const frh_context *ctx, bool ahd, const DevArgs &a,
-                                                 uint32_t fallback) {
+// The autotuned tile shape (strip height, waves per block, columns per lane) for this geometry, or
+// (fallback, 0, 0): the defaults.
+static frh_context::TileShape tuned_shape(const frh_context *ctx, bool ahd, const DevArgs &a, uint32_t fallback) {
     std::lock_guard<std::mutex> g(const_cast<frh_context *>(ctx)->mu);
     auto it = ctx->tuned.find(tune_key(ahd, a));
-    return it == ctx->tuned.end() ? std::make_pair(fallback, 0u) : it->second;
+    return it == ctx->tuned.end() ? frh_context::TileShape{fallback, 0u, 0u} : it->second;
 }
 
 // Everything a develop launch needs, validated (run.rs:439-646).
@@ -842,10 +843,12 @@ int32_t frh::plan_develop(const frh_context *ctx, const frh_frame *f, const frh_
     {
         const char *impl = getenv("FLASHRAW_BILINEAR_IMPL");
         a.bil_impl = (impl && strcmp(impl, "tiled") == 0) ? 1u : 0u;
-        const char *rows = getenv("FLASHRAW_BILINEAR_ROWS"), *wpb = getenv("FLASHRAW_BILINEAR_WPB");
+        const char *rows = getenv("FLASHRAW_BILINEAR_ROWS"), *wpb = getenv("FLASHRAW_BILINEAR_WPB"),
+                   *cpl = getenv("FLASHRAW_BILINEAR_CPL");
         const auto t = tuned_shape(ctx, false, a, ctx->bil_rows);
-        a.bil_rows = rows ? (uint32_t)atoi(rows) : t.first;
-        a.bil_wpb = wpb ? (uint32_t)atoi(wpb) : t.second;
+        a.bil_rows = rows ? (uint32_t)atoi(rows) : t.rows;
+        a.bil_wpb = wpb ? (uint32_t)atoi(wpb) : t.wpb;
+        a.bil_cpl = cpl ? (uint32_t)atoi(cpl) : t.cpl;
     }
     // The display outputs' tone and output curve (run.rs:626-675)
     if ((rc = output_tables(mctx, spec->tone, spec->curve, a))) return rc;
@@ -883,8 +886,8 @@ int32_t frh::plan_develop(const frh_context *ctx, const frh_frame *f, const frh_
             a.ahd_impl = (impl && strcmp(impl, "tiled") == 0) ? 1u : 0u;
             const char *rows = getenv("FLASHRAW_AHD_ROWS"), *wpb = getenv("FLASHRAW_AHD_WPB");
             const auto t = tuned_shape(ctx, true, a, ctx->ahd_rows);
-            a.ahd_rows = rows ? (uint32_t)atoi(rows) : t.first;
-            a.ahd_wpb = wpb ? (uint32_t)atoi(wpb) : t.second;
+            a.ahd_rows = rows ? (uint32_t)atoi(rows) : t.rows;
+            a.ahd_wpb = wpb ? (uint32_t)atoi(wpb) : t.wpb;
             p.ahd = true;
         } else {
             // the edge-aware filters (edgeaware/mod.rs:57-160): P * P canonical positions
@@ -1157,29 +1160,39 @@ extern "C" int32_t frh_plan_autotune(frh_plan *plan, const void *src, void *dst,
             if (best) *best = 0;
             return FRH_OK;
         }
-        std::vector<std::pair<uint32_t, uint32_t>> cands;
-        for (uint32_t w = 1; w <= (ahd ? AHD_MAX_WAVES : BIL_MAX_WAVES); w *= 2) {
-            if (ahd)
-                for (uint32_t r = 12; r <= 132; r += 6) cands.push_back({r, w});
-            else
-                for (uint32_t r = 8; r <= 64; r += 4) cands.push_back({r, w});
+        // the four-column bilinear walker is a candidate where the caller's buffers and the
+        // frame's geometry allow it (walk4_ok)
+        bool four = false;
+        if (!ahd) {
+            DevArgs t = plan->p.a;
+            t.src = static_cast<const uint16_t *>(src) + plan->src_off;
+            four = plan->p.in.kind != FRH_INPUT_F32 && walk4_ok(t);
         }
+        std::vector<frh_context::TileShape> cands;
+        for (uint32_t c = 2; c <= (four ? 4u : 2u); c += 2)
+            for (uint32_t w = 1; w <= (ahd ? AHD_MAX_WAVES : BIL_MAX_WAVES); w *= 2) {
+                if (ahd)
+                    for (uint32_t r = 12; r <= 132; r += 6) cands.push_back({r, w, c});
+                else
+                    for (uint32_t r = 8; r <= 64; r += 4) cands.push_back({r, w, c});
+            }
         hipEvent_t e0, e1;
         if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
             return fail(FRH_ERROR_GPU_UNAVAILABLE, "hipEventCreate failed");
         float best_ms = 1e30f;
-        uint32_t best_rows = ahd ? plan->p.a.ahd_rows : plan->p.a.bil_rows;
-        uint32_t best_wpb = ahd ? plan->p.a.ahd_wpb : plan->p.a.bil_wpb;
+        frh_context::TileShape bst = ahd ? frh_context::TileShape{plan->p.a.ahd_rows, plan->p.a.ahd_wpb, 2u}
+                                         : frh_context::TileShape{plan->p.a.bil_rows, plan->p.a.bil_wpb,
+                                                                  plan->p.a.bil_cpl};
         int32_t rc = FRH_OK;
         for (const auto &c : cands) {
-            const uint32_t r = c.first;
             Plan p = plan->p;
             if (ahd) {
-                p.a.ahd_rows = r;
-                p.a.ahd_wpb = c.second;
+                p.a.ahd_rows = c.rows;
+                p.a.ahd_wpb = c.wpb;
             } else {
-                p.a.bil_rows = r;
-                p.a.bil_wpb = c.second;
+                p.a.bil_rows = c.rows;
+                p.a.bil_wpb = c.wpb;
+                p.a.bil_cpl = c.cpl;
             }
             if (p.in.kind == FRH_INPUT_F32)
                 p.a.srcf = static_cast<const float *>(src) + plan->src_off;
@@ -1200,38 +1213,42 @@ extern "C" int32_t frh_plan_autotune(frh_plan *plan, const void *src, void *dst,
             (void)hipEventElapsedTime(&ms, e0, e1);
             if (ms < best_ms) {
                 best_ms = ms;
-                best_rows = r;
-                best_wpb = c.second;
+                bst = c;
             }
         }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         if (rc != FRH_OK) return rc;
         if (ahd) {
-            plan->p.a.ahd_rows = best_rows;
-            plan->p.a.ahd_wpb = best_wpb;
+            plan->p.a.ahd_rows = bst.rows;
+            plan->p.a.ahd_wpb = bst.wpb;
         } else {
-            plan->p.a.bil_rows = best_rows;
-            plan->p.a.bil_wpb = best_wpb;
+            plan->p.a.bil_rows = bst.rows;
+            plan->p.a.bil_wpb = bst.wpb;
+            plan->p.a.bil_cpl = bst.cpl;
         }
         {
             std::lock_guard<std::mutex> g(ctx->mu);
-            ctx->tuned[tune_key(ahd, plan->p.a)] = std::make_pair(best_rows, best_wpb);
+            ctx->tuned[tune_key(ahd, plan->p.a)] = bst;
         }
-        if (best) *best = best_rows;
+        if (best) *best = bst.rows;
         return FRH_OK;
     } catch (...) {
         return fail(FRH_ERROR_INTERNAL, "unexpected exception");
     }
 }
 
-extern "C" int32_t frh_plan_tile_shape(const frh_plan *plan, uint32_t *rows, uint32_t *waves_per_block) {
+extern "C" int32_t frh_plan_tile_shape(const frh_plan *plan, uint32_t *rows, uint32_t *waves_per_block,
+                                       uint32_t *cols_per_lane) {
     if (!plan) return fail(FRH_ERROR_INVALID_ARGUMENT, "NULL plan");
     const DevArgs &a = plan->p.a;
     const bool walker = !a.generic && !a.edge_aware && (plan->p.ahd ? a.ahd_impl == 0 : a.bil_impl == 0);
     if (rows) *rows = walker ? (plan->p.ahd ? a.ahd_rows : a.bil_rows) : 0u;
     if (waves_per_block)
         *waves_per_block = walker ? (plan->p.ahd ? walk_wpb(a.ahd_wpb, AHD_MAX_WAVES) : walk_wpb(a.bil_wpb, BIL_MAX_WAVES)) : 0u;
+    // (the four-column walker also needs an aligned source, checked per launch: the plan's own
+    // frame pointer stands for it here)
+    if (cols_per_lane) *cols_per_lane = walker ? (!plan->p.ahd && a.bil_cpl == 4 && walk4_ok(a) ? 4u : 2u) : 0u;
     return FRH_OK;
 }
 

@@ -6,6 +6,7 @@
 // Both kernels are persistent: a block loads the sRGB table and the calibration pattern into LDS
 // once, then walks tiles `blockIdx.x, blockIdx.x + gridDim.x, ...` of the developed image.
 #include "bil_walk.h"
+#include "bil_walk4.h"
 #include "develop_common.h"
 #include "frh_launch.h"
 
@@ -274,9 +275,55 @@ static void launch_bil_rp(const DevArgs &a, const BWalkPlan &P, uint32_t walk_bl
                            stream, a, P, walk_blocks);
 }
 
+// The four-column bilinear walker (RGBA8 fast path of aligned frames), blocks renumbered per XCD
+// like the two-column one.
+template <int RP, bool TAME>
+__global__ __launch_bounds__(BIL_MAX_WAVES * 64) void k_develop_bilinear_walk4(DevArgs args, BWalkPlan P,
+                                                                             uint32_t walk_blocks) {
+    const DevArgs &a = kernel_args(args);
+    __shared__ Lds s;
+    load_lds(a, s);
+    __syncthreads();
+    const uint32_t blk = blockIdx.x < walk_blocks ? xcd_block(blockIdx.x, walk_blocks) : blockIdx.x;
+    if (blk >= walk_blocks) {
+        const uint32_t i = (blk - walk_blocks) * blockDim.x + threadIdx.x;
+        if (i < P.border.count) border_pixel<false>(a, s, P.border, i);
+        return;
+    }
+    const int item = (int)uniform(blk * (blockDim.x >> 6) + (threadIdx.x >> 6));
+    if (item >= P.strips * P.chunks) return;
+    bwalk4_item<RP, TAME>(a, P, s, item);
+}
+
+template <int RP>
+static void launch_bil4_rp(const DevArgs &a, const BWalkPlan &P, uint32_t walk_blocks, uint32_t grid,
+                           uint32_t block, hipStream_t stream) {
+    if (a.ahd_tame)
+        hipLaunchKernelGGL((k_develop_bilinear_walk4<RP, true>), dim3(grid), dim3(block), 0, stream, a, P,
+                           walk_blocks);
+    else
+        hipLaunchKernelGGL((k_develop_bilinear_walk4<RP, false>), dim3(grid), dim3(block), 0, stream, a, P,
+                           walk_blocks);
+}
+
 static hipError_t launch_develop_bilinear_walk(const DevArgs &a, hipStream_t stream, int rows_per_strip) {
     const int rp = red_position(a);
     if (rp < 0) return hipErrorInvalidValue;
+    if (a.bil_cpl == 4 && walk4_ok(a)) {
+        const BWalkPlan P = make_bwalk4_plan(a, rows_per_strip);
+        const uint32_t wpb = walk_wpb(a.bil_wpb, BIL_MAX_WAVES), block = wpb * 64;
+        const uint32_t walk_blocks = (uint32_t)((P.strips * P.chunks + wpb - 1) / wpb);
+        const uint32_t border_blocks = (P.border.count + block - 1) / block;
+        const uint32_t grid = walk_blocks + border_blocks;
+        if (grid == 0) return hipSuccess;
+        switch (rp) {
+        case 0: launch_bil4_rp<0>(a, P, walk_blocks, grid, block, stream); break;
+        case 1: launch_bil4_rp<1>(a, P, walk_blocks, grid, block, stream); break;
+        case 2: launch_bil4_rp<2>(a, P, walk_blocks, grid, block, stream); break;
+        default: launch_bil4_rp<3>(a, P, walk_blocks, grid, block, stream); break;
+        }
+        return hipGetLastError();
+    }
     const BWalkPlan P = make_bwalk_plan(a, rows_per_strip);
     const uint32_t wpb = walk_wpb(a.bil_wpb, BIL_MAX_WAVES), block = wpb * 64;
     const uint32_t walk_blocks = (uint32_t)((P.strips * P.chunks + wpb - 1) / wpb);

@@ -99,7 +99,8 @@ struct DevArgs {
     uint32_t ahd_wpb;        // waves per block of the AHD walker (1, 2, 4 or 8; 0: 4)
     uint32_t bil_impl;       // bilinear: 0 register row walker, 1 LDS-tiled (the same bytes)
     uint32_t bil_rows;       // rows per bilinear walker strip (even)
-    uint32_t bil_wpb;        // waves per block of the bilinear walker (1, 2, 4 or 8; 0: 4)
+    uint32_t bil_wpb;        // waves per block of the bilinear walker (1, 2, 4, 8 or 16; 0: 4)
+    uint32_t bil_cpl;        // frame columns per lane of the bilinear walker: 4 (walk4_ok launches) or 2
     // edge-aware demosaic: tap lists, canonical position of frame sample (0, 0), smoothing weight
     // (the border width is ahd_halo)
     uint32_t edge_aware;

@@ -69,7 +69,11 @@ struct frh_context {
     uint32_t ahd_rows = 64;  // rows per AHD walker strip
     uint32_t bil_rows = 32;  // rows per bilinear walker strip
     // strip heights found by frh_plan_autotune, by (AHD, developed width, rows, fast path)
-    std::map<std::tuple<int, uint32_t, uint32_t, int>, std::pair<uint32_t, uint32_t>> tuned;  // (rows, waves per block)
+    // autotuned walker tile shapes per geometry: strip height, waves per block, columns per lane
+    struct TileShape {
+        uint32_t rows, wpb, cpl;
+    };
+    std::map<std::tuple<int, uint32_t, uint32_t, int>, TileShape> tuned;
     // device copies of small per-frame tables (calibration, CFA sites, black-level deltas,
     // linearisation curves, tone and curve tables: kind 0), of edge-aware tap lists (kind 1) and
     // of the Kronecker denoiser's operand tables (kind 2), by (kind, hash of the key bytes, key

@@ -236,7 +236,7 @@ def lib():
             L.frh_plan_launch.restype = i32
             L.frh_plan_autotune.argtypes = [vp, vp, vp, vp, vp, i32, ctypes.POINTER(u32)]
             L.frh_plan_autotune.restype = i32
-            L.frh_plan_tile_shape.argtypes = [vp, ctypes.POINTER(u32), ctypes.POINTER(u32)]
+            L.frh_plan_tile_shape.argtypes = [vp, ctypes.POINTER(u32), ctypes.POINTER(u32), ctypes.POINTER(u32)]
             L.frh_plan_tile_shape.restype = i32
             L.frh_plan_output_bytes.argtypes = [vp, i32]
             L.frh_plan_output_bytes.restype = u64
@@ -1368,11 +1368,11 @@ class Plan:
 
     @property
     def tile_shape(self) -> tuple:
-        """(strip height, waves per block) of the plan's row-walker launch, (0, 0) for the kernels
-        without one (frh_plan_tile_shape)."""
-        rows, wpb = ctypes.c_uint32(), ctypes.c_uint32()
-        check(lib().frh_plan_tile_shape(self._h, ctypes.byref(rows), ctypes.byref(wpb)))
-        return rows.value, wpb.value
+        """(strip height, waves per block, frame columns per lane) of the plan's row-walker launch,
+        (0, 0, 0) for the kernels without one (frh_plan_tile_shape)."""
+        rows, wpb, cpl = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+        check(lib().frh_plan_tile_shape(self._h, ctypes.byref(rows), ctypes.byref(wpb), ctypes.byref(cpl)))
+        return rows.value, wpb.value, cpl.value
 
     KERNELS = ("bilinear_walk", "bilinear_tiled", "ahd_walk", "ahd_tiled", "edge_tame", "edge_general", "generic")
 

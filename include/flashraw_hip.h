@@ -260,18 +260,20 @@ int32_t frh_plan_create(frh_context *ctx, const frh_frame *frame, const frh_fram
 /* Develop device input `src` (the raster the frame describes) into `dst` (and `dst2` for
  * output2) on `stream` (NULL: the null stream), without synchronising. */
 int32_t frh_plan_launch(const frh_plan *plan, const void *src, void *dst, void *dst2, void *stream);
-/* Autotune the plan's tile shape: the strip height (frame rows one wave develops) and the waves per
- * block (1, 2, 4, 8, and 16 for bilinear) of the row-walker kernels. Each candidate pair is timed with HIP events on
- * `stream` over `iters` launches on the given resident buffers; the fastest is kept in the plan and
- * remembered by the context for develops of the same geometry. *best (optional) receives the strip
- * height (frh_plan_tile_shape gives both). No reference counterpart: the reference fixes 16x16
- * workgroups (crates/flashraw-gpu/shaders/develop.wgsl:185). */
+/* Autotune the plan's tile shape: the strip height (frame rows one wave develops), the waves per
+ * block (1, 2, 4, 8, and 16 for bilinear) and, for the bilinear RGBA8 develop of 4-aligned frames,
+ * the frame columns per lane (2 or 4) of the row-walker kernels. Each candidate is timed with HIP
+ * events on `stream` over `iters` launches on the given resident buffers; the fastest is kept in the
+ * plan and remembered by the context for develops of the same geometry. *best (optional) receives
+ * the strip height (frh_plan_tile_shape gives all three). No reference counterpart: the reference
+ * fixes 16x16 workgroups (crates/flashraw-gpu/shaders/develop.wgsl:185). */
 int32_t frh_plan_autotune(frh_plan *plan, const void *src, void *dst, void *dst2, void *stream, int32_t iters,
                           uint32_t *best);
-/* The plan's tile shape: strip height and waves per block of its row-walker launch (both 0 for the
- * kernels without one: edge-aware, generic layouts, the LDS-tiled alternates). No reference
- * counterpart (see frh_plan_autotune). */
-int32_t frh_plan_tile_shape(const frh_plan *plan, uint32_t *rows, uint32_t *waves_per_block);
+/* The plan's tile shape: strip height, waves per block and frame columns per lane of its row-walker
+ * launch (all 0 for the kernels without one: edge-aware, generic layouts, the LDS-tiled
+ * alternates). Any pointer may be NULL. No reference counterpart (see frh_plan_autotune). */
+int32_t frh_plan_tile_shape(const frh_plan *plan, uint32_t *rows, uint32_t *waves_per_block,
+                            uint32_t *cols_per_lane);
 /* Bytes of the plan's first and second outputs. */
 uint64_t frh_plan_output_bytes(const frh_plan *plan, int32_t which);
 /* No reference counterpart (the reference picks one WGSL pipeline per demosaic, run.rs:545-600):

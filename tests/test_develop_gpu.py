@@ -265,7 +265,7 @@ def test_autotuned_plan(ctx, demosaic):
     stream = torch.cuda.Stream()
     rows = plan.autotune(d_in, dst, stream=stream, iters=2)
     assert (12 <= rows <= 132) if demosaic == "ahd" else (8 <= rows <= 64)
-    assert plan.tile_shape[0] == rows and plan.tile_shape[1] in (1, 2, 4, 8, 16)
+    assert plan.tile_shape[0] == rows and plan.tile_shape[1] in (1, 2, 4, 8, 16) and plan.tile_shape[2] in (2, 4)
     with torch.cuda.stream(stream):  # the clear and the launch on one stream: ordered
         dst.zero_()
     plan.launch(d_in, dst, stream=stream)
@@ -289,6 +289,23 @@ def test_waves_per_block_keep_bytes(ctx, monkeypatch, demosaic, wpb):
         monkeypatch.setenv("FLASHRAW_AHD_ROWS" if demosaic == "ahd" else "FLASHRAW_BILINEAR_ROWS", str(rows))
         for out in (F.DevelopOutput.Rgba8, F.DevelopOutput.LinearRgb16):
             _compare(ctx, mos, "BGGR", demosaic=demosaic, output=out, origin=(1, 1), what=f"wpb {wpb} rows {rows}")
+
+
+@pytest.mark.parametrize("cfa", ["RGGB", "BGGR", "GRBG", "GBRG"])
+@pytest.mark.parametrize("origin", [(0, 0), (1, 0), (0, 1), (3, 5)])
+def test_four_column_bilinear_walker(ctx, monkeypatch, cfa, origin):
+    """The bilinear walker with four frame columns per lane (RGBA8 of frames aligned to four
+    samples) gives the oracle's bytes: every phase and origin, frames narrower and wider than a
+    strip, crops that keep the alignment, strip heights and block sizes, tame or not."""
+    monkeypatch.setenv("FLASHRAW_BILINEAR_CPL", "4")
+    for (w, h) in [(60, 37), (300, 41), (516, 77)]:
+        mos = synth.mosaic(w, h, cfa, seed=w + h, edges=20, discs=6, stripes=8)
+        for rows, wpb in [(8, 1), (20, 8), (36, 4)]:
+            monkeypatch.setenv("FLASHRAW_BILINEAR_ROWS", str(rows))
+            monkeypatch.setenv("FLASHRAW_BILINEAR_WPB", str(wpb))
+            _compare(ctx, mos, cfa, origin=origin, what=f"{w}x{h} rows {rows} wpb {wpb}")
+        _compare(ctx, mos, cfa, origin=origin, crop=(4, 3, w - 8, h - 5), what=f"{w}x{h} crop")
+        _compare(ctx, mos, cfa, origin=origin, wb=(2.1, 0.0, 1.6, 1.0), what=f"{w}x{h} untame")
 
 
 @pytest.mark.parametrize("cfa", ["RGGB", "BGGR", "GRBG", "GBRG"])
