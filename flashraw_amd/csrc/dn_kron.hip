@@ -26,10 +26,10 @@
 //
 // Throughput. On MI355X the vector ALU and the MFMA unit of a SIMD overlap little, so a step costs
 // about its vector instructions plus its MFMAs, and a lone wave per SIMD runs nearly as fast as
-// two: the launch is sized to one round of two waves per SIMD (dn_kron_seg_steps), the
-// coefficient masks are folded into the threshold's bit operation, the DC broadcast goes through
-// ds_bpermute, and stores are issued by every lane (buffer stores, dropped past the end) so the
-// step loop has no branches around memory operations.
+// two: the launch is sized to one round of two waves per SIMD (dn_kron_seg_steps), blocks
+// outside the plane drop their coefficients through an infinite threshold, the DC broadcast goes
+// through ds_bpermute, and stores are issued by every lane (buffer stores, dropped past the end)
+// so the step loop has no branches around memory operations.
 //
 // Inputs. Lane (g, n) loads row 4 h + g of its class-(0, 0) block as two 16-byte loads (both
 // planes of the mosaic row parity), converts them to 16-bit operands (sample minus an offset per
@@ -77,11 +77,14 @@ __device__ __forceinline__ uint32_t sub2(uint32_t a, uint32_t b) {
 }
 __device__ __forceinline__ u4 words(const uint32_t t[4]) { return u4{t[0], t[1], t[2], t[3]}; }
 
-// Two 16-bit integers (sample minus offset) as a packed pair of operands.
+// Two 16-bit integers (sample minus offset) as a packed pair of operands. For f16 the high half
+// is converted in place (SDWA: source word 1 into destination word 1, word 0 preserved), so the
+// pair costs two conversions and no packing instruction.
 __device__ __forceinline__ uint32_t int_pair(uint32_t w, _Float16) {
-    const s2 v = __builtin_bit_cast(s2, w);
-    const h2 r = {(_Float16)v.x, (_Float16)v.y};
-    return __builtin_bit_cast(uint32_t, r);
+    const _Float16 lo = (_Float16)__builtin_bit_cast(s2, w).x;
+    uint32_t r = (uint32_t)__builtin_bit_cast(uint16_t, lo);
+    asm("v_cvt_f16_i16_sdwa %0, %1 dst_sel:WORD_1 dst_unused:UNUSED_PRESERVE src0_sel:WORD_1" : "+v"(r) : "v"(w));
+    return r;
 }
 __device__ __forceinline__ uint32_t int_pair(uint32_t w, __bf16) {
     const s2 v = __builtin_bit_cast(s2, w);
@@ -102,19 +105,20 @@ __device__ __forceinline__ float bcast_row0(float v, int addr) {
 }
 
 // The thresholded coefficients of one tile (forward result D[4], this lane's block threshold t2
-// and whether this lane holds the DC coefficient in D[0][0]) as the inverse's two K-halves; `km`
-// is all ones, or zero for a block outside the plane (its coefficients are dropped).
+// and the threshold tdc of its coefficient in D[0][0]: t2, or -1 for a kept DC coefficient) as
+// the inverse's two K-halves. A block outside the plane has t2 = tdc = +inf: every coefficient is
+// dropped by the threshold itself (a select per word after the bit operation would cost 8 more
+// instructions).
 // A pair of coefficients as packed f16, zeroed where c^2 < t (t as a packed pair).
-__device__ __forceinline__ uint32_t keep_pair(float c0, float c1, h2 t, uint32_t km) {
+__device__ __forceinline__ uint32_t keep_pair(float c0, float c1, h2 t) {
     const h2 c = {(_Float16)c0, (_Float16)c1};
     const h2 q = __builtin_elementwise_fma(c, c, -t);
     const uint32_t m = __builtin_bit_cast(uint32_t, __builtin_bit_cast(s2, q) >> (short)15);  // ones where c^2 < t
-    return __builtin_bit_cast(uint32_t, c) & ~m & km;  // one three-input bit operation
+    return __builtin_bit_cast(uint32_t, c) & ~m;
 }
-__device__ __forceinline__ void threshold_pack(const f4 D[4], float t2, bool dc_lane, uint32_t km, u4 K[2], _Float16) {
-    const _Float16 th = (_Float16)t2;
-    const h2 T = {th, th};
-    const h2 Tdc = {dc_lane ? (_Float16)-1.0f : th, th};
+__device__ __forceinline__ void threshold_pack(const f4 D[4], float t2, float tdc, u4 K[2], _Float16) {
+    const h2 Tdc = {(_Float16)tdc, (_Float16)t2};
+    const h2 T = {Tdc.y, Tdc.y};
 #pragma unroll
     for (int h = 0; h < 2; h++) {
         uint32_t t[4];
@@ -122,7 +126,7 @@ __device__ __forceinline__ void threshold_pack(const f4 D[4], float t2, bool dc_
         for (int w = 0; w < 4; w++) {
             const f4 &src = D[2 * h + (w >> 1)];
             const int r = 2 * (w & 1);
-            t[w] = keep_pair(src[r], src[r + 1], (h == 0 && w == 0) ? Tdc : T, km);
+            t[w] = keep_pair(src[r], src[r + 1], (h == 0 && w == 0) ? Tdc : T);
         }
         K[h] = words(t);
     }
@@ -131,7 +135,7 @@ __device__ __forceinline__ uint32_t bf16_pair(float c0, float c1) {
     const bh2 v = {(__bf16)c0, (__bf16)c1};
     return __builtin_bit_cast(uint32_t, v);
 }
-__device__ __forceinline__ void threshold_pack(const f4 D[4], float t2, bool dc_lane, uint32_t km, u4 K[2], __bf16) {
+__device__ __forceinline__ void threshold_pack(const f4 D[4], float t2, float tdc, u4 K[2], __bf16) {
 #pragma unroll
     for (int h = 0; h < 2; h++) {
         uint32_t t[4];
@@ -140,10 +144,10 @@ __device__ __forceinline__ void threshold_pack(const f4 D[4], float t2, bool dc_
             const f4 &src = D[2 * h + (w >> 1)];
             const int r = 2 * (w & 1);
             float c0 = src[r], c1 = src[r + 1];
-            const bool keep0 = c0 * c0 >= t2 || (h == 0 && w == 0 && dc_lane);
+            const bool keep0 = c0 * c0 >= ((h == 0 && w == 0) ? tdc : t2);
             c0 = keep0 ? c0 : 0.0f;
             c1 = c1 * c1 >= t2 ? c1 : 0.0f;
-            t[w] = bf16_pair(c0, c1) & km;
+            t[w] = bf16_pair(c0, c1);
         }
         K[h] = words(t);
     }
@@ -152,23 +156,25 @@ __device__ __forceinline__ void threshold_pack(const f4 D[4], float t2, bool dc_
 template <typename T>
 struct Wave {
     u4 kf[4][2], ki[4][2];  // forward and inverse constants: M tile j, K-half h
-    float t2a, t2b;
+    float t2a2;  // 2 t2a: the DC coefficient's factor in the threshold
     bool dc_lane;
     int bp_addr;  // 4 (l % 16): the DC broadcast's source lane
     T tag;
 
-    // Forward transform of the 16 blocks whose K-halves are q0, q1 and the threshold (block mean
-    // from the DC coefficient plus `off`, the operands' offset above black): the kept
-    // coefficients as the inverse's two K-halves (all zero where km is zero).
-    __device__ __forceinline__ void forward(const u4 &q0, const u4 &q1, float off, uint32_t km, u4 K[2]) const {
+    // Forward transform of the 16 blocks whose K-halves are q0, q1 and the threshold: the kept
+    // coefficients as the inverse's two K-halves. The squared threshold t2a max(mean, 0) + t2b
+    // (block mean = 2 DC + the operands' offset above black) is max(2 t2a DC + b, lo) with
+    // b = t2a off + t2b and lo = t2b, as t2a >= 0; a block outside the plane passes lo = +inf
+    // and dc = +inf (everything dropped), one inside dc = -1 (its DC coefficient kept).
+    __device__ __forceinline__ void forward(const u4 &q0, const u4 &q1, float b, float lo, float dc_t, u4 K[2]) const {
         f4 D[4];
         const f4 z = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int j = 0; j < 4; j++) D[j] = mma(kf[j][1], q1, mma(kf[j][0], q0, z, tag), tag);
         const float dc = bcast_row0(D[0][0], bp_addr);
-        const float mean = fmaf(2.0f, dc, off);  // block mean in DN: the DC coefficient is 8 x mean / 16
-        const float t2 = fmaf(t2a, fmaxf(mean, 0.0f), t2b);
-        threshold_pack(D, t2, dc_lane, km, K, tag);
+        // (block mean in DN: the DC coefficient is 8 x mean / 16)
+        const float t2 = fmaxf(fmaf(t2a2, dc, b), lo);
+        threshold_pack(D, t2, dc_lane ? dc_t : t2, K, tag);
     }
     // Inverse tile j of the coefficients K, accumulated onto acc.
     __device__ __forceinline__ f4 inv(int j, const u4 K[2], f4 acc) const {
@@ -229,8 +235,11 @@ __global__ __launch_bounds__(256) void k_dn_kron(DnkArgs A) {
                 W.ki[j][h] = ti[2 * j + h];
             }
     }
-    W.t2a = A.t2a;
-    W.t2b = A.t2b;
+    W.t2a2 = 2.0f * A.t2a;
+    const float inf = __builtin_inff();
+    // a tile kind's threshold floor and DC threshold, inside the plane or not
+    auto lo_of = [&](bool live) { return live ? A.t2b : inf; };
+    auto dc_of = [&](bool live) { return live ? -1.0f : inf; };
     W.dc_lane = g == 0;
     W.bp_addr = 4 * n;
 
@@ -310,8 +319,9 @@ __global__ __launch_bounds__(256) void k_dn_kron(DnkArgs A) {
             for (int p = 0; p < 2; p++) {
                 const u4 c0 = convert(hw[p], off2[p]), c1 = convert(wA[p], off2[p]);
                 u4 K0[2], K1[2];
-                W.forward(c0, c1, offC[p], ok0 ? ~0u : 0u, K0);
-                W.forward(partner(c0), partner(c1), offC[p], ok1 ? ~0u : 0u, K1);
+                const float b = fmaf(A.t2a, offC[p], A.t2b);
+                W.forward(c0, c1, b, lo_of(ok0), dc_of(ok0), K0);
+                W.forward(partner(c0), partner(c1), b, lo_of(ok1), dc_of(ok1), K1);
                 C[p][0] = W.inv(1, K0, z);
                 const f4 s2 = W.inv(3, K1, z);
 #pragma unroll
@@ -325,17 +335,20 @@ __global__ __launch_bounds__(256) void k_dn_kron(DnkArgs A) {
     // Stores. Pixel (row, col) of output register j is (y0 + 4 (j & 1) + g, xb0 + 4 (j >> 1) + r),
     // both planes interleaved; where fewer than four blocks cover a pixel (the plane's border rows
     // and columns) its sum is scaled by the cover factor 2 per short direction. The u16 sample is
-    // round(sum * f + offset + black), one fma onto the per-row offset; the f32 output (F32OUT)
-    // is sum * f + offset. Stores of pixels a lane does not own address past the resource.
-    float cf[2][4];  // column cover factors of this lane's two column groups
+    // sum * f + offset + black, rounded to nearest and clamped to [0, 65535]: one fma in units of
+    // 1/65535 and the hardware's packed unorm16 conversion for two samples (clamp to [0, 1],
+    // scale, round); the f32 output (F32OUT) is sum * f + offset. Stores of pixels a lane does
+    // not own address past the resource.
+    constexpr float kUnorm = 1.0f / 65535.0f;
+    float cf[2][4], cfu[2][4];  // column cover factors of this lane's two column groups (and / 65535)
 #pragma unroll
     for (int h = 0; h < 2; h++)
 #pragma unroll
         for (int r = 0; r < 4; r++) {
             const int col = xb0 + 4 * h + r;
             cf[h][r] = (col < 4 || col > plx + 3) ? 2.0f : 1.0f;
+            cfu[h][r] = cf[h][r] * kUnorm;
         }
-    const float bh0 = black[0] + 0.5f, bh1 = black[1] + 0.5f;
     auto store_j = [&](int j, const f4 O[2][4], const float ot[2], const float om[2], int y0) {
         {
             const int row = y0 + 4 * (j & 1) + g;
@@ -344,15 +357,14 @@ __global__ __launch_bounds__(256) void k_dn_kron(DnkArgs A) {
             const bool owned = (n != 0) & !((n == 15) & (j >= 2)) & (row < ph) & (col < pw);
             const uint32_t o = (uint32_t)((2 * row + dy) * stride + 2 * col);  // sample index
             const float o0 = (j & 1) ? om[0] : ot[0], o1 = (j & 1) ? om[1] : ot[1];
-            const float c0 = o0 + bh0, c1 = o1 + bh1;
+            const float c0 = (o0 + black[0]) * kUnorm, c1 = (o1 + black[1]) * kUnorm;
             const float fy = (row < 4 || row > ply + 3) ? 2.0f : 1.0f;
             uint32_t q[4];
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const float f = fy * cf[j >> 1][r];
-                const float u0 = __builtin_amdgcn_fmed3f(fmaf(O[0][j][r], f, c0), 0.0f, 65535.0f);
-                const float u1 = __builtin_amdgcn_fmed3f(fmaf(O[1][j][r], f, c1), 0.0f, 65535.0f);
-                q[r] = (uint32_t)u0 | ((uint32_t)u1 << 16);
+                const float f = fy * cfu[j >> 1][r];
+                q[r] = __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pknorm_u16(fmaf(O[0][j][r], f, c0),
+                                                                                   fmaf(O[1][j][r], f, c1)));
             }
             __builtin_amdgcn_raw_buffer_store_b128(words(q), rdst, owned ? o * 2u : NO_STORE, 0, 0);
             if (F32OUT) {
@@ -391,9 +403,11 @@ __global__ __launch_bounds__(256) void k_dn_kron(DnkArgs A) {
     for (int s = 0; s < nsteps; s++) {
         const int y0 = y_first + 8 * s;
         const bool top_ok = y0 <= ply, mid_ok = y0 + 4 <= ply;
-        // coefficient masks: blocks outside the plane (columns at the strip ends, rows at the bottom)
-        const uint32_t kt0 = (ok0 && top_ok) ? ~0u : 0u, kt1 = (ok1 && top_ok) ? ~0u : 0u;
-        const uint32_t km0 = (ok0 && mid_ok) ? ~0u : 0u, km1 = (ok1 && mid_ok) ? ~0u : 0u;
+        // blocks inside the plane (the others, columns at the strip ends and rows at the bottom, keep nothing)
+        const bool kt0 = ok0 && top_ok, kt1 = ok1 && top_ok;
+        const bool km0 = ok0 && mid_ok, km1 = ok1 && mid_ok;
+        const float lo_t0 = lo_of(kt0), lo_t1 = lo_of(kt1), lo_m0 = lo_of(km0), lo_m1 = lo_of(km1);
+        const float dc_t0 = dc_of(kt0), dc_t1 = dc_of(kt1), dc_m0 = dc_of(km0), dc_m1 = dc_of(km1);
         u4 wB[2], wC[2];
         planes(nB, wB);
         planes(nC, wC);
@@ -412,10 +426,11 @@ __global__ __launch_bounds__(256) void k_dn_kron(DnkArgs A) {
             const u4 cA = convert(wA[p], off2[p]), cB = convert(wB[p], off2[p]), cC = convert(wC[p], off2[p]);
             // classes (0, 0), (0, 1) at block row y0 and (1, 0), (1, 1) at y0 + 4
             u4 Kt0[2], Kt1[2], Km0[2], Km1[2];
-            W.forward(cA, cB, offf[p], kt0, Kt0);
-            W.forward(partner(cA), partner(cB), offf[p], kt1, Kt1);
-            W.forward(cB, cC, offf[p], km0, Km0);
-            W.forward(partner(cB), partner(cC), offf[p], km1, Km1);
+            const float b = fmaf(A.t2a, offf[p], A.t2b);
+            W.forward(cA, cB, b, lo_t0, dc_t0, Kt0);
+            W.forward(partner(cA), partner(cB), b, lo_t1, dc_t1, Kt1);
+            W.forward(cB, cC, b, lo_m0, dc_m0, Km0);
+            W.forward(partner(cB), partner(cC), b, lo_m1, dc_m1, Km1);
             // The overlap-add, accumulated in the inverse products: rows 0..3 (j = 0, 2) of this
             // block row take the previous step's lower blocks (the carry), rows 4..7 (j = 1, 3)
             // this step's; class-(*, 1) blocks add in register j ^ 2, from column n - 1 for j < 2.
