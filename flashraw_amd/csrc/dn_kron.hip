@@ -338,27 +338,36 @@ __global__ __launch_bounds__(256) void k_dn_kron(DnkArgs A) {
     // sum * f + offset + black, rounded to nearest and clamped to [0, 65535]: one fma in units of
     // 1/65535 and the hardware's packed unorm16 conversion for two samples (clamp to [0, 1],
     // scale, round); the f32 output (F32OUT) is sum * f + offset. Stores of pixels a lane does
-    // not own address past the resource.
+    // not own address past the resource. A store's byte offset is the saturating sum of this
+    // lane's part (its row g in the group, its column group) and a part uniform over the wave (the
+    // group's first row: rows come in aligned groups of four, as do the plane's height and its
+    // border rows); NO_STORE in either part takes the sum past the resource.
     constexpr float kUnorm = 1.0f / 65535.0f;
     float cf[2][4], cfu[2][4];  // column cover factors of this lane's two column groups (and / 65535)
+    uint32_t lane16[2], lane32[2];  // this lane's part of the u16 and f32 byte offsets
 #pragma unroll
-    for (int h = 0; h < 2; h++)
+    for (int h = 0; h < 2; h++) {
 #pragma unroll
         for (int r = 0; r < 4; r++) {
             const int col = xb0 + 4 * h + r;
             cf[h][r] = (col < 4 || col > plx + 3) ? 2.0f : 1.0f;
             cfu[h][r] = cf[h][r] * kUnorm;
         }
+        const int col = xb0 + 4 * h;
+        // (bitwise, not short-circuit: a select, not a branch)
+        const bool owned = (n != 0) & !((n == 15) & (h == 1)) & (col < pw);
+        const uint32_t o = (uint32_t)((2 * g + dy) * (int)stride + 2 * col);  // sample index past the group's row
+        lane16[h] = owned ? o * 2u : NO_STORE;
+        lane32[h] = owned ? o * 4u : NO_STORE;
+    }
     auto store_j = [&](int j, const f4 O[2][4], const float ot[2], const float om[2], int y0) {
         {
-            const int row = y0 + 4 * (j & 1) + g;
-            const int col = xb0 + 4 * (j >> 1);
-            // (bitwise, not short-circuit: a select, not a branch)
-            const bool owned = (n != 0) & !((n == 15) & (j >= 2)) & (row < ph) & (col < pw);
-            const uint32_t o = (uint32_t)((2 * row + dy) * stride + 2 * col);  // sample index
+            const int rowg = y0 + 4 * (j & 1);  // the group's first row (wave-uniform)
+            const uint32_t rs = (uint32_t)(2 * rowg) * stride;  // its sample index
+            const uint32_t o16 = __builtin_elementwise_add_sat(lane16[j >> 1], rowg < ph ? rs * 2u : NO_STORE);
             const float o0 = (j & 1) ? om[0] : ot[0], o1 = (j & 1) ? om[1] : ot[1];
             const float c0 = (o0 + black[0]) * kUnorm, c1 = (o1 + black[1]) * kUnorm;
-            const float fy = (row < 4 || row > ply + 3) ? 2.0f : 1.0f;
+            const float fy = (rowg < 4 || rowg >= ph - 4) ? 2.0f : 1.0f;
             uint32_t q[4];
 #pragma unroll
             for (int r = 0; r < 4; r++) {
@@ -366,7 +375,7 @@ __global__ __launch_bounds__(256) void k_dn_kron(DnkArgs A) {
                 q[r] = __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pknorm_u16(fmaf(O[0][j][r], f, c0),
                                                                                    fmaf(O[1][j][r], f, c1)));
             }
-            __builtin_amdgcn_raw_buffer_store_b128(words(q), rdst, owned ? o * 2u : NO_STORE, 0, 0);
+            __builtin_amdgcn_raw_buffer_store_b128(words(q), rdst, o16, 0, 0);
             if (F32OUT) {
                 float v0[4], v1[4];
 #pragma unroll
@@ -375,13 +384,13 @@ __global__ __launch_bounds__(256) void k_dn_kron(DnkArgs A) {
                     v0[r] = fmaf(O[0][j][r], f, o0);
                     v1[r] = fmaf(O[1][j][r], f, o1);
                 }
-                const uint32_t of = owned ? o * 4u : NO_STORE;
+                const uint32_t of = __builtin_elementwise_add_sat(lane32[j >> 1], rowg < ph ? rs * 4u : NO_STORE);
                 const uint32_t a0[4] = {__builtin_bit_cast(uint32_t, v0[0]), __builtin_bit_cast(uint32_t, v1[0]),
                                         __builtin_bit_cast(uint32_t, v0[1]), __builtin_bit_cast(uint32_t, v1[1])};
                 const uint32_t a1[4] = {__builtin_bit_cast(uint32_t, v0[2]), __builtin_bit_cast(uint32_t, v1[2]),
                                         __builtin_bit_cast(uint32_t, v0[3]), __builtin_bit_cast(uint32_t, v1[3])};
                 __builtin_amdgcn_raw_buffer_store_b128(words(a0), rdf, of, 0, 0);
-                __builtin_amdgcn_raw_buffer_store_b128(words(a1), rdf, owned ? of + 16u : NO_STORE, 0, 0);
+                __builtin_amdgcn_raw_buffer_store_b128(words(a1), rdf, __builtin_elementwise_add_sat(of, 16u), 0, 0);
             }
         }
     };
