@@ -38,8 +38,9 @@
 // offset term is the mean offset of the blocks that cover it.
 //
 // Threshold. A coefficient c is kept when c^2 >= t^2 with t^2 = thr^2 (sig_a max(mean, 0) +
-// sig_b) in the transform's scaled units (the DC coefficient always); for f16 the test runs on the
-// packed 16-bit coefficients (one fused multiply-add and a sign mask per pair), for bf16 in f32.
+// sig_b) in the transform's scaled units (the DC coefficient always); the test runs on the
+// coefficients rounded to f16, two at a time (one fused multiply-add and a sign mask per pair),
+// for bf16 operands too.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -109,12 +110,17 @@ __device__ __forceinline__ float bcast_row0(float v, int addr) {
 // the inverse's two K-halves. A block outside the plane has t2 = tdc = +inf: every coefficient is
 // dropped by the threshold itself (a select per word after the bit operation would cost 8 more
 // instructions).
-// A pair of coefficients as packed f16, zeroed where c^2 < t (t as a packed pair).
+// Ones in the halves of a pair of coefficients (rounded to f16) where c^2 < t (t as a packed
+// pair): one fused multiply-add and a sign mask. Coefficients beyond f16's range round to
+// infinity and are kept, as their squares exceed any threshold.
+__device__ __forceinline__ uint32_t drop_mask(h2 c, h2 t) {
+    const h2 q = __builtin_elementwise_fma(c, c, -t);
+    return __builtin_bit_cast(uint32_t, __builtin_bit_cast(s2, q) >> (short)15);
+}
+// A pair of coefficients as packed f16, zeroed where c^2 < t.
 __device__ __forceinline__ uint32_t keep_pair(float c0, float c1, h2 t) {
     const h2 c = {(_Float16)c0, (_Float16)c1};
-    const h2 q = __builtin_elementwise_fma(c, c, -t);
-    const uint32_t m = __builtin_bit_cast(uint32_t, __builtin_bit_cast(s2, q) >> (short)15);  // ones where c^2 < t
-    return __builtin_bit_cast(uint32_t, c) & ~m;
+    return __builtin_bit_cast(uint32_t, c) & ~drop_mask(c, t);
 }
 __device__ __forceinline__ void threshold_pack(const f4 D[4], float t2, float tdc, u4 K[2], _Float16) {
     const h2 Tdc = {(_Float16)tdc, (_Float16)t2};
@@ -135,7 +141,11 @@ __device__ __forceinline__ uint32_t bf16_pair(float c0, float c1) {
     const bh2 v = {(__bf16)c0, (__bf16)c1};
     return __builtin_bit_cast(uint32_t, v);
 }
+// bf16 operands: the same test on the coefficients rounded to f16 (finer than bf16), applied to
+// their bf16 pair.
 __device__ __forceinline__ void threshold_pack(const f4 D[4], float t2, float tdc, u4 K[2], __bf16) {
+    const h2 Tdc = {(_Float16)tdc, (_Float16)t2};
+    const h2 T = {Tdc.y, Tdc.y};
 #pragma unroll
     for (int h = 0; h < 2; h++) {
         uint32_t t[4];
@@ -143,11 +153,8 @@ __device__ __forceinline__ void threshold_pack(const f4 D[4], float t2, float td
         for (int w = 0; w < 4; w++) {
             const f4 &src = D[2 * h + (w >> 1)];
             const int r = 2 * (w & 1);
-            float c0 = src[r], c1 = src[r + 1];
-            const bool keep0 = c0 * c0 >= ((h == 0 && w == 0) ? tdc : t2);
-            c0 = keep0 ? c0 : 0.0f;
-            c1 = c1 * c1 >= t2 ? c1 : 0.0f;
-            t[w] = bf16_pair(c0, c1);
+            const h2 c = {(_Float16)src[r], (_Float16)src[r + 1]};
+            t[w] = bf16_pair(src[r], src[r + 1]) & ~drop_mask(c, (h == 0 && w == 0) ? Tdc : T);
         }
         K[h] = words(t);
     }
