@@ -33,9 +33,11 @@
 //
 // Inputs. Lane (g, n) loads row 4 h + g of its class-(0, 0) block as two 16-byte loads (both
 // planes of the mosaic row parity), converts them to 16-bit operands (sample minus an offset per
-// step, taken from one sample of the step's rows, so the operands carry local contrast), and builds
-// the class-(0, 1) operand from its own right half and column n + 1's left half (DPP). An output's
-// offset term is the mean offset of the blocks that cover it.
+// step, taken from one sample of the step's rows, so the operands carry local contrast; the
+// saturating 16-bit difference is exact while samples differ by at most 32767, which holds for any
+// mosaic of 15 bits or fewer), and builds the class-(0, 1) operand from its own right half and
+// column n + 1's left half (DPP). An output's offset term is the mean offset of the blocks that
+// cover it.
 //
 // Threshold. A coefficient c is kept when c^2 >= t^2 with t^2 = thr^2 (sig_a max(mean, 0) +
 // sig_b) in the transform's scaled units (the DC coefficient always); the test runs on the

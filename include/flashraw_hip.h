@@ -367,7 +367,9 @@ typedef struct frh_dn_spec {
 /* Denoise the device u16 mosaic `src` into `dst` (u16 mosaic, black added back, rounded and
  * clamped) and/or `dst_f32` (f32 mosaic in DN above black); either may be NULL, not both. The
  * output must not overlap the input. Asynchronous on `stream` (NULL: the null stream) for the
- * MFMA precisions. */
+ * MFMA precisions. The f16 and bf16 precisions form their operands as a sample minus one sample
+ * of the same 8-row step as 16-bit integers: exact for any mosaic of 15 bits or fewer, while
+ * larger differences saturate at 32767; split f16 and f32 take any u16 samples. */
 int32_t frh_denoise_cfa(frh_context *ctx, const frh_dn_spec *spec, const void *src, void *dst, float *dst_f32,
                         void *stream);
 
