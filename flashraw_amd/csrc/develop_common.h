@@ -316,14 +316,18 @@ __device__ __forceinline__ float norm_raw(const DevArgs &a, const Lds &s, float 
 
 // Normalised samples of a fetched pair at row y (parity PY); 0 outside the frame (`in0`, `in1`:
 // the lane's columns are inside, `row_in`: the row is). HOT: every column of the wave is inside
-// the frame and the calibration is the scalar one.
+// the frame and the calibration is the scalar one; a row outside the frame is then normalised
+// from its clamped fetch like any other. No stored pixel depends on such a row (the walkers store
+// only pixels whose whole neighbourhood, 1 row for bilinear and 5 for AHD, lies in the frame;
+// frame-edge pixels come from the border blocks), and without the wave-uniform select, which
+// the compiler makes a branch, a walker's trip stays one block of straight-line code.
 template <int PY, bool HOT>
 __device__ __forceinline__ void norm_pair(const DevArgs &a, const Lds &s, uint32_t raw, int x0, int y, bool row_in,
                                           bool in0, bool in1, float &n0, float &n1) {
     const float v0 = (float)(raw & 0xffffu), v1 = (float)(raw >> 16);
     if (HOT) {
-        n0 = row_in ? (v0 - a.qb[PY * 2 + 0]) * a.qs[PY * 2 + 0] : 0.0f;
-        n1 = row_in ? (v1 - a.qb[PY * 2 + 1]) * a.qs[PY * 2 + 1] : 0.0f;
+        n0 = (v0 - a.qb[PY * 2 + 0]) * a.qs[PY * 2 + 0];
+        n1 = (v1 - a.qb[PY * 2 + 1]) * a.qs[PY * 2 + 1];
         return;
     }
     if (a.simple_cal) {
