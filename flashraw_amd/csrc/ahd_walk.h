@@ -67,10 +67,12 @@ struct StripCols {
 };
 
 // One walker row at trip position T: load mosaic row R (parity T & 1) and advance every stage.
-template <int RP, int T, bool FAST, bool HOT, bool TAME>
+// DUAL (with FAST and TAME): the RGBA8 output goes to `rd` and the linear RGB16 output to `rl`.
+template <int RP, int T, bool FAST, bool HOT, bool TAME, bool DUAL = false>
 __device__ __forceinline__ void walk_row(const DevArgs &a, const Lds &s, const float *wt, Walk &W, int R, int xs,
                                          int lane, int out_row0, int out_row1, const StripCols &C,
-                                         __amdgpu_buffer_rsrc_t rs, __amdgpu_buffer_rsrc_t rd) {
+                                         __amdgpu_buffer_rsrc_t rs, __amdgpu_buffer_rsrc_t rd,
+                                         __amdgpu_buffer_rsrc_t rl) {
     constexpr int PY = T & 1;
     // ring slots
     constexpr int M0 = T % 6, M1 = (T + 5) % 6, M2 = (T + 4) % 6, M3 = (T + 3) % 6, M4 = (T + 2) % 6,
@@ -286,14 +288,21 @@ __device__ __forceinline__ void walk_row(const DevArgs &a, const Lds &s, const f
         // of rows outside it (pipeline fill, the rows past the strip's end), address past the
         // destination resource, where the hardware drops them: no branch splits the trip.
         uint32_t o[2];
+        float c[2][3];  // DUAL: 4096 clamp(output, 0, 1) per channel
 #pragma unroll
         for (int k = 0; k < 2; k++) {
             float r, g, b;
             blend(k, r, g, b);
-            o[k] = TAME ? finish_rgba8_scaled(a, s, r, g, b) : finish_rgba8_fast(a, s, r, g, b, true);
+            if (DUAL)
+                o[k] = finish_rgba8_scaled_c(a, s, r, g, b, c[k]);
+            else
+                o[k] = TAME ? finish_rgba8_scaled(a, s, r, g, b) : finish_rgba8_fast(a, s, r, g, b, true);
         }
         const bool rv = ro >= out_row0 && ro < out_row1;
-        const uint32_t off = ((uint32_t)(ro - (int)a.crop_y) * a.dst_w + (uint32_t)(x0 - (int)a.crop_x)) * 4u;
+        // destination pixel index of the lane's first pixel, modulo 2^32 (-1 for a lane whose second
+        // pixel starts a row: both outputs' offsets are multiples of it, so pixel 1 still lands at 0)
+        const uint32_t pix = (uint32_t)(ro - (int)a.crop_y) * a.dst_w + (uint32_t)(x0 - (int)a.crop_x);
+        const uint32_t off = pix * 4u;
         const bool s0 = rv && C.st0, s1 = rv && C.st1, pe = C.pair_even[PY];
         constexpr uint32_t PAST = 0xfffffff0u;
         typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
@@ -303,6 +312,32 @@ __device__ __forceinline__ void walk_row(const DevArgs &a, const Lds &s, const f
         __builtin_amdgcn_raw_buffer_store_b64(v, rd, (pe && s0 && s1) ? (int)off : (int)PAST, 0, 0);
         __builtin_amdgcn_raw_buffer_store_b32(o[0], rd, (s0 && !(pe && s1)) ? (int)off : (int)PAST, 0, 0);
         __builtin_amdgcn_raw_buffer_store_b32(o[1], rd, (s1 && !(pe && s0)) ? (int)(off + 4u) : (int)PAST, 0, 0);
+        if (DUAL) {
+            // linear RGB16, 6 bytes per pixel at the same pixel index (the outputs are unrotated and
+            // dst_w = dev_w): a pair at an even pixel is one 12-byte store, anything else goes out
+            // one 16-bit sample at a time
+            uint32_t q[2][3];
+#pragma unroll
+            for (int k = 0; k < 2; k++)
+#pragma unroll
+                for (int ch = 0; ch < 3; ch++) q[k][ch] = lin16_of_4k(c[k][ch]);
+            const uint32_t off6 = pix * 6u;
+            const bool full = pe && s0 && s1;
+            typedef unsigned int u32x3 __attribute__((ext_vector_type(3)));
+            u32x3 t;
+            t.x = q[0][0] | (q[0][1] << 16);
+            t.y = q[0][2] | (q[1][0] << 16);
+            t.z = q[1][1] | (q[1][2] << 16);
+            __builtin_amdgcn_raw_buffer_store_b96(t, rl, full ? (int)off6 : (int)PAST, 0, 0);
+#pragma unroll
+            for (int k = 0; k < 2; k++)
+#pragma unroll
+                for (int ch = 0; ch < 3; ch++)
+                    __builtin_amdgcn_raw_buffer_store_b16((unsigned short)q[k][ch], rl,
+                                                          (!full && (k ? s1 : s0)) ? (int)(off6 + 6u * k + 2u * ch)
+                                                                                   : (int)PAST,
+                                                          0, 0);
+        }
     } else if (ro >= out_row0 && ro < out_row1 && lane >= 3 && lane <= 60) {
         float o[2][3];
 #pragma unroll
@@ -321,7 +356,7 @@ __device__ __forceinline__ void walk_row(const DevArgs &a, const Lds &s, const f
 // out_row1 - 1 = R - 5, in whole trips of six rows (prefetch slot = row parity), so a trip is
 // straight-line code the compiler schedules as one block; the rows walked past the strip's end
 // read clamped rows and store nothing.
-template <int RP, bool FAST, bool HOT, bool TAME>
+template <int RP, bool FAST, bool HOT, bool TAME, bool DUAL = false>
 __device__ __forceinline__ void walk_strip(const DevArgs &a, const Lds &s, const float *wt, int xs, int lane,
                                            int row0, int out_row0, int out_row1, int out_x0, int out_x1) {
     Walk W;
@@ -344,13 +379,18 @@ __device__ __forceinline__ void walk_strip(const DevArgs &a, const Lds &s, const
     for (int py = 0; py < 2; py++)
         C.pair_even[py] = ((cpar + (uint32_t)(py ^ 1 ^ (int)(a.crop_y & 1u)) * a.dst_w) & 1u) == 0;
     const __amdgpu_buffer_rsrc_t rs = make_rsrc(a.src, src_bytes(a));
-    const __amdgpu_buffer_rsrc_t rd = make_rsrc(a.dst, a.dst_w * a.dev_h * 4u);
+    // DUAL: the RGBA8 and linear RGB16 destinations, in whichever order the outputs name them
+    void *const d8 = (DUAL && a.out_mode != OUT_RGBA8) ? a.dst2 : a.dst;
+    void *const d16 = (DUAL && a.out_mode == OUT_RGBA8) ? a.dst2 : a.dst;
+    const __amdgpu_buffer_rsrc_t rd = make_rsrc(d8, a.dst_w * a.dev_h * 4u);
+    const __amdgpu_buffer_rsrc_t rl = make_rsrc(DUAL ? d16 : nullptr, DUAL ? a.dev_w * a.dev_h * 6u : 0u);
 #pragma unroll
     for (int i = 0; i < 2; i++) W.pf[i] = fetch_pair<HOT>(a, rs, x0, row0 - 6 + i);
     const int trips = (out_row1 + 5 - (row0 - 6) + 5) / 6;
     int R = row0 - 6;
     for (int t = 0; t < trips; t++, R += 6) {
-#define WALK_ROW(t) walk_row<RP, t, FAST, HOT, TAME>(a, s, wt, W, R + t, xs, lane, out_row0, out_row1, C, rs, rd);
+#define WALK_ROW(t) \
+    walk_row<RP, t, FAST, HOT, TAME, DUAL>(a, s, wt, W, R + t, xs, lane, out_row0, out_row1, C, rs, rd, rl);
         WALK_ROW(0)
         WALK_ROW(1)
         WALK_ROW(2)
@@ -392,7 +432,7 @@ __host__ inline WalkPlan make_walk_plan(const DevArgs &a, int rows_per_strip) {
 }
 
 // Strip `item` (0 .. strips * chunks - 1) of plan P, developed by the calling wave.
-template <int RP, bool FAST, bool TAME>
+template <int RP, bool FAST, bool TAME, bool DUAL = false>
 __device__ __forceinline__ void walk_item(const DevArgs &a, const WalkPlan &P, const Lds &s, const float *wt,
                                           int item) {
     const int lane = threadIdx.x & 63;
@@ -405,9 +445,9 @@ __device__ __forceinline__ void walk_item(const DevArgs &a, const WalkPlan &P, c
     const bool hot = a.simple_cal && xs >= 0 && xs + 128 <= (int)a.width && ((uintptr_t)a.src & 3u) == 0 &&
                      (a.stride & 1u) == 0;
     if (hot)
-        walk_strip<RP, FAST, true, TAME>(a, s, wt, xs, lane, row0, out_row0, out_row1, out_x0, out_x1);
+        walk_strip<RP, FAST, true, TAME, DUAL>(a, s, wt, xs, lane, row0, out_row0, out_row1, out_x0, out_x1);
     else
-        walk_strip<RP, FAST, false, TAME>(a, s, wt, xs, lane, row0, out_row0, out_row1, out_x0, out_x1);
+        walk_strip<RP, FAST, false, TAME, DUAL>(a, s, wt, xs, lane, row0, out_row0, out_row1, out_x0, out_x1);
 }
 
 }  // namespace frh

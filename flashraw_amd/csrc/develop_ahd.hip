@@ -1,38 +1,21 @@
 // AHD develop launcher: the register row walker of ahd_walk.h, one strip per wave, plus the
 // bilinear border pixels in extra blocks of the same launch.
-#include "ahd_walk.h"
+#include "ahd_kernel.h"
 #include "frh_launch.h"
 
 namespace frh {
 
-template <int RP, bool FAST, bool TAME>
-__global__ __launch_bounds__(AHD_MAX_WAVES * 64) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_develop_ahd_walk(DevArgs args, WalkPlan P, uint32_t walk_blocks) {
-    const DevArgs &a = kernel_args(args);
-    __shared__ Lds s;
-    __shared__ float wt[73 * 73];
-    load_lds(a, s);
-    // the walker blocks come first and are renumbered among themselves; border blocks follow
-    const uint32_t blk = blockIdx.x < walk_blocks ? xcd_block(blockIdx.x, walk_blocks) : blockIdx.x;
-    if (blk >= walk_blocks) {
-        __syncthreads();
-        const uint32_t i = (blk - walk_blocks) * blockDim.x + threadIdx.x;
-        if (i < P.border.count) border_pixel<false>(a, s, P.border, i);
-        return;
-    }
-    for (int i = threadIdx.x; i < 73 * 73; i += blockDim.x) wt[i] = a.wtab[i];
-    __syncthreads();
-    // the wave index is the same for every lane; readfirstlane lets the compiler keep all the strip
-    // bookkeeping in scalar registers
-    const int item = (int)uniform(blk * (blockDim.x >> 6) + (threadIdx.x >> 6));
-    if (item >= P.strips * P.chunks) return;
-    walk_item<RP, FAST, TAME>(a, P, s, wt, item);
-}
+// The two-output (RGBA8 + linear RGB16) form is built in develop_ahd_dual.hip.
+void launch_ahd_dual(int rp, const DevArgs &a, const WalkPlan &P, uint32_t walk_blocks, uint32_t grid, uint32_t block,
+                     hipStream_t stream);
 
 template <int RP>
 static void launch_ahd_rp(const DevArgs &a, const WalkPlan &P, uint32_t walk_blocks, uint32_t grid, uint32_t block,
                           hipStream_t stream) {
     const bool fast = fast_rgba8(a), tame = a.ahd_tame != 0;
-    if (fast && tame)
+    if (tame && fast_rgba8_lin16(a))
+        launch_ahd_dual(RP, a, P, walk_blocks, grid, block, stream);
+    else if (fast && tame)
         hipLaunchKernelGGL((k_develop_ahd_walk<RP, true, true>), dim3(grid), dim3(block), 0, stream, a, P,
                            walk_blocks);
     else if (fast)

@@ -229,6 +229,15 @@ __host__ __device__ __forceinline__ bool fast_rgba8(const DevArgs &a) {
            !a.tone && !a.curve8 && a.unit_k;
 }
 
+// True when a launch stores RGBA8 and linear RGB16 (in either order) under the same conditions:
+// the AHD walker's two-output fast path, for tame frames.
+__host__ __device__ __forceinline__ bool fast_rgba8_lin16(const DevArgs &a) {
+    const bool pair = (a.out_mode == OUT_RGBA8 && a.out_mode2 == OUT_LINEAR16) ||
+                      (a.out_mode == OUT_LINEAR16 && a.out_mode2 == OUT_RGBA8);
+    return pair && a.orientation == 1 && a.dst_w == a.dev_w && a.bounded && !a.has_levels && !a.tone && !a.curve8 &&
+           a.unit_k;
+}
+
 __device__ __forceinline__ float med01(float x) { return __builtin_amdgcn_fmed3f(x, 0.0f, 1.0f); }
 
 // finish + RGBA8 encode of the fast path (finite values, no levels): the same operations as
@@ -266,6 +275,25 @@ __device__ __forceinline__ uint32_t finish_rgba8_scaled(const DevArgs &a, const 
                    i2 = (uint32_t)__builtin_amdgcn_fmed3f(o2, 0.0f, 4096.0f);
     return lut8(s, i0) | (lut8(s, i1) << 8) | (lut8(s, i2) << 16) | 0xff000000u;
 }
+
+// finish_rgba8_scaled that also returns the three clamped outputs c = 4096 clamp(o, 0, 1) (exact),
+// for the linear 16-bit code of the same pixel: linear16(o) = trunc(clamp(o, 0, 1) * 65535 + 0.5)
+// (process.rs:1971-1973) is trunc(c * (65535 / 4096) + 0.5), since 65535 / 4096 is a float and
+// c * (65535 / 4096) is the same real number as clamp(o, 0, 1) * 65535, rounded once.
+__device__ __forceinline__ uint32_t finish_rgba8_scaled_c(const DevArgs &a, const Lds &s, float r, float g, float b,
+                                                          float c[3]) {
+    r = med01(r);
+    g = med01(g);
+    b = med01(b);
+    const float o0 = a.m4k[0] * r + a.m4k[1] * g + a.m4k[2] * b;
+    const float o1 = a.m4k[3] * r + a.m4k[4] * g + a.m4k[5] * b;
+    const float o2 = a.m4k[6] * r + a.m4k[7] * g + a.m4k[8] * b;
+    c[0] = __builtin_amdgcn_fmed3f(o0, 0.0f, 4096.0f);
+    c[1] = __builtin_amdgcn_fmed3f(o1, 0.0f, 4096.0f);
+    c[2] = __builtin_amdgcn_fmed3f(o2, 0.0f, 4096.0f);
+    return lut8(s, (uint32_t)c[0]) | (lut8(s, (uint32_t)c[1]) << 8) | (lut8(s, (uint32_t)c[2]) << 16) | 0xff000000u;
+}
+__device__ __forceinline__ uint32_t lin16_of_4k(float c) { return (uint32_t)(c * (65535.0f / 4096.0f) + 0.5f); }
 
 // Raw samples of frame columns x0, x0 + 1 of row y as one word (x0 in the low half). Addresses
 // are clamped into the frame, so the load never faults and needs no branch; the caller masks

@@ -118,6 +118,19 @@ def test_levels_and_dual_output(ctx, demosaic):
              output2=F.DevelopOutput.Rgba8, what="levels + dual")
 
 
+@pytest.mark.parametrize("cfa", ["RGGB", "BGGR", "GRBG", "GBRG"])
+def test_ahd_rgba8_and_linear16_in_one_pass(ctx, cfa):
+    """RGBA8 and linear RGB16 from one AHD pass of a tame frame (the walker's two-output form):
+    both output orders, an odd width and height, and crops whose pixel pairs start at odd
+    destination pixels (16-bit stores instead of one 12-byte store), against the oracle."""
+    mos = synth.mosaic(151, 97, cfa, seed=17, stripes=20)
+    for crop in [None, (13, 9, 101, 77), (3, 2, 140, 90), (8, 7, 127, 81)]:
+        for first, second in [(F.DevelopOutput.LinearRgb16, F.DevelopOutput.Rgba8),
+                              (F.DevelopOutput.Rgba8, F.DevelopOutput.LinearRgb16)]:
+            _compare(ctx, mos, cfa, demosaic="ahd", output=first, output2=second, crop=crop,
+                     what=f"{cfa} crop {crop} {first.name}+{second.name}")
+
+
 @pytest.mark.parametrize("case", [
     # AHD-tame frames (the walker's square root skips its tiny/infinite-input test, clamps are
     # medians, the RGBA8 finish takes the matrix scaled by 4096)
