@@ -150,6 +150,31 @@ struct BatchDev {
     uint32_t *counters;
 };
 
+// Side streams and events of a batch: its lists run on up to four streams at once, so that one
+// persistent kernel's last waves overlap the next kernel's first ones (each kernel alone leaves
+// the GPU part idle while its last strips finish). Created on the batch's first launch.
+struct ListFork {
+    hipStream_t side[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t fork = nullptr, join[3] = {nullptr, nullptr, nullptr};
+    bool ready = false;
+    hipError_t create() {
+        hipError_t e = hipSuccess;
+        for (int k = 0; k < 3 && e == hipSuccess; k++) e = hipStreamCreateWithFlags(&side[k], hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&fork, hipEventDisableTiming);
+        for (int k = 0; k < 3 && e == hipSuccess; k++) e = hipEventCreateWithFlags(&join[k], hipEventDisableTiming);
+        ready = e == hipSuccess;
+        return e;
+    }
+    ~ListFork() {
+        for (int k = 0; k < 3; k++) {
+            if (side[k]) (void)hipStreamSynchronize(side[k]);
+            if (side[k]) (void)hipStreamDestroy(side[k]);
+            if (join[k]) (void)hipEventDestroy(join[k]);
+        }
+        if (fork) (void)hipEventDestroy(fork);
+    }
+};
+
 struct frh_batch {
     frh_context *ctx = nullptr;
     std::vector<Plan> plans;
@@ -163,6 +188,7 @@ struct frh_batch {
     Lists lists;
     hipGraphExec_t graph = nullptr;
     hipGraph_t graph_def = nullptr;
+    ListFork fork;
     uint32_t capacity = 0;        // frames the device and staging buffers hold
     uint8_t *h_stage = nullptr;   // pinned staging of descriptors and lists
     // recorded after each frh_batch_launch: rebuilding the lists waits for it, so a launch still
@@ -328,22 +354,43 @@ static void launch_list_rp(const frh_context *ctx, const Lists &Ls, const BatchD
         launch_list<RP, false, false>(ctx, Ls, D, l, st);
 }
 
-// The launch sequence: counter reset, then each non-empty list (AHD lists first).
-static hipError_t enqueue_lists(const frh_context *ctx, const Lists &Ls, const BatchDev &D, hipStream_t st) {
+// The launch sequence: counter reset, then each non-empty list (AHD lists first). With `fk`, the
+// lists go round-robin to `st` and the three side streams, forked after the reset and joined
+// back into `st` (a capture of `st` records them as parallel graph nodes); without, all on `st`.
+static hipError_t enqueue_lists(const frh_context *ctx, const Lists &Ls, const BatchDev &D, hipStream_t st,
+                                const ListFork *fk = nullptr) {
     hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, st, D.counters, (uint32_t)NLIST);
+    hipError_t e = hipSuccess;
+    if (fk && (e = hipEventRecord(fk->fork, st)) != hipSuccess) return e;
+    int used = 0;
     for (int l = 0; l < NLIST; l++) {
         if (!Ls.total[l]) continue;
+        const int k = fk ? used % 4 : 0;
+        hipStream_t s = k ? fk->side[k - 1] : st;
+        if (k && used < 4 && (e = hipStreamWaitEvent(s, fk->fork, 0)) != hipSuccess) return e;
         switch (l & 3) {
-        case 0: launch_list_rp<0>(ctx, Ls, D, l, st); break;
-        case 1: launch_list_rp<1>(ctx, Ls, D, l, st); break;
-        case 2: launch_list_rp<2>(ctx, Ls, D, l, st); break;
-        default: launch_list_rp<3>(ctx, Ls, D, l, st); break;
+        case 0: launch_list_rp<0>(ctx, Ls, D, l, s); break;
+        case 1: launch_list_rp<1>(ctx, Ls, D, l, s); break;
+        case 2: launch_list_rp<2>(ctx, Ls, D, l, s); break;
+        default: launch_list_rp<3>(ctx, Ls, D, l, s); break;
         }
+        used++;
     }
+    if (fk)
+        for (int k = 1; k < 4 && k < used; k++) {
+            if ((e = hipEventRecord(fk->join[k - 1], fk->side[k - 1])) != hipSuccess) return e;
+            if ((e = hipStreamWaitEvent(st, fk->join[k - 1], 0)) != hipSuccess) return e;
+        }
     return hipGetLastError();
 }
 
-static hipError_t batch_enqueue(frh_batch *b, hipStream_t st) { return enqueue_lists(b->ctx, b->lists, b->dev(), st); }
+static hipError_t batch_enqueue(frh_batch *b, hipStream_t st) {
+    if (!b->fork.ready) {
+        const hipError_t e = b->fork.create();
+        if (e != hipSuccess) return e;
+    }
+    return enqueue_lists(b->ctx, b->lists, b->dev(), st, &b->fork);
+}
 
 extern "C" int32_t frh_batch_create(frh_context *ctx, uint32_t n, const frh_frame *frames,
                                     const frh_frame_info *infos, const frh_develop_spec *specs, void *const *dsts,
@@ -529,7 +576,9 @@ int32_t frh::batch_assign(frh_batch *b, std::vector<Plan> &&plans, hipStream_t s
     return batch_build(b, st, false);
 }
 
-hipError_t frh::batch_run(frh_batch *b, hipStream_t st) { return batch_enqueue(b, st); }
+// (The serving loop runs its batches on its one compute stream: its copy streams already share
+// the process's hardware queues.)
+hipError_t frh::batch_run(frh_batch *b, hipStream_t st) { return enqueue_lists(b->ctx, b->lists, b->dev(), st); }
 
 // ---- the batched develop that allocates nothing ----
 //
