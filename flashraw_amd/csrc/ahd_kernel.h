@@ -7,7 +7,7 @@
 
 namespace frh {
 
-template <int RP, bool FAST, bool TAME, bool DUAL = false>
+template <int RP, bool FAST, bool TAME, int OUT = 0>
 __global__ __launch_bounds__(AHD_MAX_WAVES * 64) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_develop_ahd_walk(DevArgs args, WalkPlan P, uint32_t walk_blocks) {
     const DevArgs &a = kernel_args(args);
     __shared__ Lds s;
@@ -27,7 +27,7 @@ __global__ __launch_bounds__(AHD_MAX_WAVES * 64) __attribute__((amdgpu_waves_per
     // bookkeeping in scalar registers
     const int item = (int)uniform(blk * (blockDim.x >> 6) + (threadIdx.x >> 6));
     if (item >= P.strips * P.chunks) return;
-    walk_item<RP, FAST, TAME, DUAL>(a, P, s, wt, item);
+    walk_item<RP, FAST, TAME, OUT>(a, P, s, wt, item);
 }
 
 }  // namespace frh

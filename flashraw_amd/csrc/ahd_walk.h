@@ -64,11 +64,15 @@ struct StripCols {
     bool in0, in1, st0, st1;
     bool pair_even[2];
     float inf;  // +inf (opaque_inf)
+    // rotated RGBA8 (OUT 2): destination pixel index of the lane's first pixel on the crop's first
+    // row, and its steps per developed column and row (modulo 2^32)
+    uint32_t rot_lane, rot_cx, rot_cy;
 };
 
 // One walker row at trip position T: load mosaic row R (parity T & 1) and advance every stage.
-// DUAL (with FAST and TAME): the RGBA8 output goes to `rd` and the linear RGB16 output to `rl`.
-template <int RP, int T, bool FAST, bool HOT, bool TAME, bool DUAL = false>
+// OUT (with FAST and TAME): 0 RGBA8 unrotated; 1 RGBA8 to `rd` and linear RGB16 to `rl`; 2 RGBA8
+// under EXIF orientation 2..8.
+template <int RP, int T, bool FAST, bool HOT, bool TAME, int OUT = 0>
 __device__ __forceinline__ void walk_row(const DevArgs &a, const Lds &s, const float *wt, Walk &W, int R, int xs,
                                          int lane, int out_row0, int out_row1, const StripCols &C,
                                          __amdgpu_buffer_rsrc_t rs, __amdgpu_buffer_rsrc_t rd,
@@ -288,12 +292,12 @@ __device__ __forceinline__ void walk_row(const DevArgs &a, const Lds &s, const f
         // of rows outside it (pipeline fill, the rows past the strip's end), address past the
         // destination resource, where the hardware drops them: no branch splits the trip.
         uint32_t o[2];
-        float c[2][3];  // DUAL: 4096 clamp(output, 0, 1) per channel
+        float c[2][3];  // OUT 1: 4096 clamp(output, 0, 1) per channel
 #pragma unroll
         for (int k = 0; k < 2; k++) {
             float r, g, b;
             blend(k, r, g, b);
-            if (DUAL)
+            if (OUT == 1)
                 o[k] = finish_rgba8_scaled_c(a, s, r, g, b, c[k]);
             else
                 o[k] = TAME ? finish_rgba8_scaled(a, s, r, g, b) : finish_rgba8_fast(a, s, r, g, b, true);
@@ -306,13 +310,23 @@ __device__ __forceinline__ void walk_row(const DevArgs &a, const Lds &s, const f
         const bool s0 = rv && C.st0, s1 = rv && C.st1, pe = C.pair_even[PY];
         constexpr uint32_t PAST = 0xfffffff0u;
         typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
-        u32x2 v;
-        v.x = o[0];
-        v.y = o[1];
-        __builtin_amdgcn_raw_buffer_store_b64(v, rd, (pe && s0 && s1) ? (int)off : (int)PAST, 0, 0);
-        __builtin_amdgcn_raw_buffer_store_b32(o[0], rd, (s0 && !(pe && s1)) ? (int)off : (int)PAST, 0, 0);
-        __builtin_amdgcn_raw_buffer_store_b32(o[1], rd, (s1 && !(pe && s0)) ? (int)(off + 4u) : (int)PAST, 0, 0);
-        if (DUAL) {
+        if (OUT == 2) {
+            // rotated: the destination index is linear in the developed position (C0 + Cx dx + Cy dy).
+            // One 4-byte store per pixel: pairing pixels into 8-byte stores here (adjacent pixels of
+            // a row, or of two rows for the transposed orientations) costs 30-40 spilled registers.
+            const uint32_t i0 = C.rot_lane + C.rot_cy * (uint32_t)(ro - (int)a.crop_y), i1 = i0 + C.rot_cx;
+            __builtin_amdgcn_raw_buffer_store_b32(o[0], rd, s0 ? (int)(i0 * 4u) : (int)PAST, 0, 0);
+            __builtin_amdgcn_raw_buffer_store_b32(o[1], rd, s1 ? (int)(i1 * 4u) : (int)PAST, 0, 0);
+        } else {
+            u32x2 v;
+            v.x = o[0];
+            v.y = o[1];
+            __builtin_amdgcn_raw_buffer_store_b64(v, rd, (pe && s0 && s1) ? (int)off : (int)PAST, 0, 0);
+            __builtin_amdgcn_raw_buffer_store_b32(o[0], rd, (s0 && !(pe && s1)) ? (int)off : (int)PAST, 0, 0);
+            __builtin_amdgcn_raw_buffer_store_b32(o[1], rd, (s1 && !(pe && s0)) ? (int)(off + 4u) : (int)PAST, 0,
+                                                  0);
+        }
+        if (OUT == 1) {
             // linear RGB16, 6 bytes per pixel at the same pixel index (the outputs are unrotated and
             // dst_w = dev_w): a pair at an even pixel is one 12-byte store, anything else goes out
             // one 16-bit sample at a time
@@ -356,7 +370,7 @@ __device__ __forceinline__ void walk_row(const DevArgs &a, const Lds &s, const f
 // out_row1 - 1 = R - 5, in whole trips of six rows (prefetch slot = row parity), so a trip is
 // straight-line code the compiler schedules as one block; the rows walked past the strip's end
 // read clamped rows and store nothing.
-template <int RP, bool FAST, bool HOT, bool TAME, bool DUAL = false>
+template <int RP, bool FAST, bool HOT, bool TAME, int OUT = 0>
 __device__ __forceinline__ void walk_strip(const DevArgs &a, const Lds &s, const float *wt, int xs, int lane,
                                            int row0, int out_row0, int out_row1, int out_x0, int out_x1) {
     Walk W;
@@ -379,18 +393,41 @@ __device__ __forceinline__ void walk_strip(const DevArgs &a, const Lds &s, const
     for (int py = 0; py < 2; py++)
         C.pair_even[py] = ((cpar + (uint32_t)(py ^ 1 ^ (int)(a.crop_y & 1u)) * a.dst_w) & 1u) == 0;
     const __amdgpu_buffer_rsrc_t rs = make_rsrc(a.src, src_bytes(a));
-    // DUAL: the RGBA8 and linear RGB16 destinations, in whichever order the outputs name them
-    void *const d8 = (DUAL && a.out_mode != OUT_RGBA8) ? a.dst2 : a.dst;
-    void *const d16 = (DUAL && a.out_mode == OUT_RGBA8) ? a.dst2 : a.dst;
-    const __amdgpu_buffer_rsrc_t rd = make_rsrc(d8, a.dst_w * a.dev_h * 4u);
-    const __amdgpu_buffer_rsrc_t rl = make_rsrc(DUAL ? d16 : nullptr, DUAL ? a.dev_w * a.dev_h * 6u : 0u);
+    // OUT 1: the RGBA8 and linear RGB16 destinations, in whichever order the outputs name them
+    void *const d8 = (OUT == 1 && a.out_mode != OUT_RGBA8) ? a.dst2 : a.dst;
+    void *const d16 = (OUT == 1 && a.out_mode == OUT_RGBA8) ? a.dst2 : a.dst;
+    const __amdgpu_buffer_rsrc_t rd = make_rsrc(d8, a.dev_w * a.dev_h * 4u);
+    const __amdgpu_buffer_rsrc_t rl = make_rsrc(OUT == 1 ? d16 : nullptr, OUT == 1 ? a.dev_w * a.dev_h * 6u : 0u);
+    C.rot_lane = C.rot_cx = C.rot_cy = 0u;
+    if (OUT == 2) {
+        // orient() (process.rs:2035-2044) as px = X0 + Xx dx + Xy dy, py = Y0 + Yx dx + Yy dy; the
+        // destination index py dst_w + px is then C0 + Cx dx + Cy dy
+        const int w = (int)a.dev_w, h = (int)a.dev_h;
+        int X0 = 0, Xx = 1, Xy = 0, Y0 = 0, Yx = 0, Yy = 1;
+        switch (a.orientation) {
+        case 2: X0 = w - 1; Xx = -1; break;
+        case 3: X0 = w - 1; Xx = -1; Y0 = h - 1; Yy = -1; break;
+        case 4: Y0 = h - 1; Yy = -1; break;
+        case 5: Xx = 0; Xy = 1; Yx = 1; Yy = 0; break;
+        case 6: X0 = h - 1; Xx = 0; Xy = -1; Yx = 1; Yy = 0; break;
+        case 7: X0 = h - 1; Xx = 0; Xy = -1; Y0 = w - 1; Yx = -1; Yy = 0; break;
+        case 8: Xx = 0; Xy = 1; Y0 = w - 1; Yx = -1; Yy = 0; break;
+        default: break;
+        }
+        const uint32_t dw = a.dst_w;
+        const uint32_t c0 = (uint32_t)Y0 * dw + (uint32_t)X0, cx = (uint32_t)Yx * dw + (uint32_t)Xx,
+                       cy = (uint32_t)Yy * dw + (uint32_t)Xy;
+        C.rot_lane = c0 + cx * (uint32_t)(x0 - (int)a.crop_x);
+        C.rot_cx = cx;
+        C.rot_cy = cy;
+    }
 #pragma unroll
     for (int i = 0; i < 2; i++) W.pf[i] = fetch_pair<HOT>(a, rs, x0, row0 - 6 + i);
     const int trips = (out_row1 + 5 - (row0 - 6) + 5) / 6;
     int R = row0 - 6;
     for (int t = 0; t < trips; t++, R += 6) {
 #define WALK_ROW(t) \
-    walk_row<RP, t, FAST, HOT, TAME, DUAL>(a, s, wt, W, R + t, xs, lane, out_row0, out_row1, C, rs, rd, rl);
+    walk_row<RP, t, FAST, HOT, TAME, OUT>(a, s, wt, W, R + t, xs, lane, out_row0, out_row1, C, rs, rd, rl);
         WALK_ROW(0)
         WALK_ROW(1)
         WALK_ROW(2)
@@ -432,7 +469,7 @@ __host__ inline WalkPlan make_walk_plan(const DevArgs &a, int rows_per_strip) {
 }
 
 // Strip `item` (0 .. strips * chunks - 1) of plan P, developed by the calling wave.
-template <int RP, bool FAST, bool TAME, bool DUAL = false>
+template <int RP, bool FAST, bool TAME, int OUT = 0>
 __device__ __forceinline__ void walk_item(const DevArgs &a, const WalkPlan &P, const Lds &s, const float *wt,
                                           int item) {
     const int lane = threadIdx.x & 63;
@@ -445,9 +482,9 @@ __device__ __forceinline__ void walk_item(const DevArgs &a, const WalkPlan &P, c
     const bool hot = a.simple_cal && xs >= 0 && xs + 128 <= (int)a.width && ((uintptr_t)a.src & 3u) == 0 &&
                      (a.stride & 1u) == 0;
     if (hot)
-        walk_strip<RP, FAST, true, TAME, DUAL>(a, s, wt, xs, lane, row0, out_row0, out_row1, out_x0, out_x1);
+        walk_strip<RP, FAST, true, TAME, OUT>(a, s, wt, xs, lane, row0, out_row0, out_row1, out_x0, out_x1);
     else
-        walk_strip<RP, FAST, false, TAME, DUAL>(a, s, wt, xs, lane, row0, out_row0, out_row1, out_x0, out_x1);
+        walk_strip<RP, FAST, false, TAME, OUT>(a, s, wt, xs, lane, row0, out_row0, out_row1, out_x0, out_x1);
 }
 
 }  // namespace frh

@@ -10,19 +10,19 @@ void launch_ahd_dual(int rp, const DevArgs &a, const WalkPlan &P, uint32_t walk_
                      hipStream_t stream) {
     switch (rp) {
     case 0:
-        hipLaunchKernelGGL((k_develop_ahd_walk<0, true, true, true>), dim3(grid), dim3(block), 0, stream, a, P,
+        hipLaunchKernelGGL((k_develop_ahd_walk<0, true, true, 1>), dim3(grid), dim3(block), 0, stream, a, P,
                            walk_blocks);
         break;
     case 1:
-        hipLaunchKernelGGL((k_develop_ahd_walk<1, true, true, true>), dim3(grid), dim3(block), 0, stream, a, P,
+        hipLaunchKernelGGL((k_develop_ahd_walk<1, true, true, 1>), dim3(grid), dim3(block), 0, stream, a, P,
                            walk_blocks);
         break;
     case 2:
-        hipLaunchKernelGGL((k_develop_ahd_walk<2, true, true, true>), dim3(grid), dim3(block), 0, stream, a, P,
+        hipLaunchKernelGGL((k_develop_ahd_walk<2, true, true, 1>), dim3(grid), dim3(block), 0, stream, a, P,
                            walk_blocks);
         break;
     default:
-        hipLaunchKernelGGL((k_develop_ahd_walk<3, true, true, true>), dim3(grid), dim3(block), 0, stream, a, P,
+        hipLaunchKernelGGL((k_develop_ahd_walk<3, true, true, 1>), dim3(grid), dim3(block), 0, stream, a, P,
                            walk_blocks);
         break;
     }

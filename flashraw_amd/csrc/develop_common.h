@@ -229,6 +229,13 @@ __host__ __device__ __forceinline__ bool fast_rgba8(const DevArgs &a) {
            !a.tone && !a.curve8 && a.unit_k;
 }
 
+// True when a launch stores rotated RGBA8 (EXIF orientation 2..8) only, under the same conditions:
+// the AHD walker's rotated fast path, for tame frames.
+__host__ __device__ __forceinline__ bool fast_rgba8_rot(const DevArgs &a) {
+    return a.out_mode == OUT_RGBA8 && a.out_mode2 == OUT_NONE && a.orientation >= 2 && a.orientation <= 8 &&
+           a.bounded && !a.has_levels && !a.tone && !a.curve8 && a.unit_k;
+}
+
 // True when a launch stores RGBA8 and linear RGB16 (in either order) under the same conditions:
 // the AHD walker's two-output fast path, for tame frames.
 __host__ __device__ __forceinline__ bool fast_rgba8_lin16(const DevArgs &a) {
