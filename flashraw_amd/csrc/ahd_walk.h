@@ -400,23 +400,9 @@ __device__ __forceinline__ void walk_strip(const DevArgs &a, const Lds &s, const
     const __amdgpu_buffer_rsrc_t rl = make_rsrc(OUT == 1 ? d16 : nullptr, OUT == 1 ? a.dev_w * a.dev_h * 6u : 0u);
     C.rot_lane = C.rot_cx = C.rot_cy = 0u;
     if (OUT == 2) {
-        // orient() (process.rs:2035-2044) as px = X0 + Xx dx + Xy dy, py = Y0 + Yx dx + Yy dy; the
-        // destination index py dst_w + px is then C0 + Cx dx + Cy dy
-        const int w = (int)a.dev_w, h = (int)a.dev_h;
-        int X0 = 0, Xx = 1, Xy = 0, Y0 = 0, Yx = 0, Yy = 1;
-        switch (a.orientation) {
-        case 2: X0 = w - 1; Xx = -1; break;
-        case 3: X0 = w - 1; Xx = -1; Y0 = h - 1; Yy = -1; break;
-        case 4: Y0 = h - 1; Yy = -1; break;
-        case 5: Xx = 0; Xy = 1; Yx = 1; Yy = 0; break;
-        case 6: X0 = h - 1; Xx = 0; Xy = -1; Yx = 1; Yy = 0; break;
-        case 7: X0 = h - 1; Xx = 0; Xy = -1; Y0 = w - 1; Yx = -1; Yy = 0; break;
-        case 8: Xx = 0; Xy = 1; Y0 = w - 1; Yx = -1; Yy = 0; break;
-        default: break;
-        }
-        const uint32_t dw = a.dst_w;
-        const uint32_t c0 = (uint32_t)Y0 * dw + (uint32_t)X0, cx = (uint32_t)Yx * dw + (uint32_t)Xx,
-                       cy = (uint32_t)Yy * dw + (uint32_t)Xy;
+        // destination index c0 + cx dx + cy dy (orient, process.rs:2035-2044)
+        uint32_t c0, cx, cy;
+        orient_linear(a, c0, cx, cy);
         C.rot_lane = c0 + cx * (uint32_t)(x0 - (int)a.crop_x);
         C.rot_cx = cx;
         C.rot_cy = cy;

@@ -20,6 +20,7 @@ struct BWalk {
     float n[3][2];           // normalised rows R-2, R-1, R; [k] even / odd column
     float nl[3];             // odd column of lane - 1
     float nr[3];             // even column of lane + 1
+    uint32_t hold[2];        // ROT 2: the previous output row's two pixels
     uint32_t pf[BPF];        // raw pairs fetched ahead, slot (row - start) % BPF
 };
 
@@ -29,9 +30,16 @@ struct BWalk {
 struct BStripCols {
     bool in0, in1, st0, st1;
     bool pair_even[2];
+    // ROT: destination pixel index of the lane's first pixel on the crop's first row, and its steps
+    // per developed column and row (modulo 2^32); ROT 2: rot_lo is 1 when the previous row's pixel
+    // of a column has the lower destination index (cy = +1)
+    uint32_t rot_lane, rot_cx, rot_cy, rot_lo;
 };
 
-template <int RP, int PY, int SLOT, bool FAST, bool HOT, bool TAME, bool OUT = true>
+// ROT (with FAST and TAME): RGBA8 under EXIF orientation 2..4 (ROT 1: rows stay rows, one 4-byte
+// store per pixel) or 5..8 (ROT 2: a column's pixels on consecutive rows are adjacent, and go out
+// in pairs).
+template <int RP, int PY, int SLOT, bool FAST, bool HOT, bool TAME, bool OUT = true, int ROT = 0>
 __device__ __forceinline__ void bwalk_row(const DevArgs &a, const Lds &s, BWalk &W, int R, int xs, int lane,
                                           int out_row0, int out_row1, const BStripCols &C,
                                           __amdgpu_buffer_rsrc_t rs, __amdgpu_buffer_rsrc_t rd) {
@@ -97,9 +105,36 @@ __device__ __forceinline__ void bwalk_row(const DevArgs &a, const Lds &s, BWalk 
         u32x2 v;
         v.x = __float_as_uint(o[0][0]);
         v.y = __float_as_uint(o[1][0]);
-        __builtin_amdgcn_raw_buffer_store_b64(v, rd, (pe && s0 && s1) ? (int)off : (int)PAST, 0, 0);
-        __builtin_amdgcn_raw_buffer_store_b32(v.x, rd, (s0 && !(pe && s1)) ? (int)off : (int)PAST, 0, 0);
-        __builtin_amdgcn_raw_buffer_store_b32(v.y, rd, (s1 && !(pe && s0)) ? (int)(off + 4u) : (int)PAST, 0, 0);
+        if (ROT == 1) {
+            const uint32_t i0 = C.rot_lane + C.rot_cy * (uint32_t)(ro - (int)a.crop_y), i1 = i0 + C.rot_cx;
+            __builtin_amdgcn_raw_buffer_store_b32(v.x, rd, s0 ? (int)(i0 * 4u) : (int)PAST, 0, 0);
+            __builtin_amdgcn_raw_buffer_store_b32(v.y, rd, s1 ? (int)(i1 * 4u) : (int)PAST, 0, 0);
+        } else if (ROT == 2) {
+            // A row whose pixel closes an aligned pair with the previous row's (lower index even)
+            // stores both as 8 bytes when the previous row is the strip's; a row that opens one leaves
+            // its pixel to the next row unless that row is past the strip's output; any other owned
+            // pixel goes out alone. The rows before the strip's first output row store nothing.
+            const uint32_t i0 = C.rot_lane + C.rot_cy * (uint32_t)(ro - (int)a.crop_y), i1 = i0 + C.rot_cx;
+            const bool rp = ro - 1 >= out_row0, rn = ro + 1 < out_row1, sw = C.rot_lo != 0u;
+#pragma unroll
+            for (int k = 0; k < 2; k++) {
+                const uint32_t ic = k ? i1 : i0, lo = ic - C.rot_lo, ok = k ? v.y : v.x;
+                const bool own = k ? s1 : s0, closer = !(lo & 1u);
+                u32x2 q;
+                q.x = sw ? W.hold[k] : ok;
+                q.y = sw ? ok : W.hold[k];
+                __builtin_amdgcn_raw_buffer_store_b64(q, rd, (own && closer && rp) ? (int)(lo * 4u) : (int)PAST, 0,
+                                                      0);
+                __builtin_amdgcn_raw_buffer_store_b32(ok, rd, (own && (closer ? !rp : !rn)) ? (int)(ic * 4u) : (int)PAST,
+                                                      0, 0);
+                W.hold[k] = ok;
+            }
+        } else {
+            __builtin_amdgcn_raw_buffer_store_b64(v, rd, (pe && s0 && s1) ? (int)off : (int)PAST, 0, 0);
+            __builtin_amdgcn_raw_buffer_store_b32(v.x, rd, (s0 && !(pe && s1)) ? (int)off : (int)PAST, 0, 0);
+            __builtin_amdgcn_raw_buffer_store_b32(v.y, rd, (s1 && !(pe && s0)) ? (int)(off + 4u) : (int)PAST, 0,
+                                                  0);
+        }
     } else {
         const int dx = x0 - (int)a.crop_x, dy = ro - (int)a.crop_y;
         if (C.st0) store_both<false>(a, s, (uint32_t)dx, (uint32_t)dy, o[0][0], o[0][1], o[0][2]);
@@ -112,7 +147,7 @@ __device__ __forceinline__ void bwalk_row(const DevArgs &a, const Lds &s, BWalk 
 // row0 - 1, outside the strip), then whole trips of four rows, each straight-line code the compiler
 // schedules as one block (the LDS table reads of one row overlap the next row's arithmetic), then
 // at most three rows more. Every row has a static prefetch slot (R - row0 + 2) mod 4 and parity.
-template <int RP, bool FAST, bool HOT, bool TAME>
+template <int RP, bool FAST, bool HOT, bool TAME, int ROT = 0>
 __device__ __forceinline__ void bwalk_strip(const DevArgs &a, const Lds &s, int xs, int lane, int row0, int out_row0,
                                             int out_row1, int out_x0, int out_x1) {
     BWalk W;
@@ -132,35 +167,43 @@ __device__ __forceinline__ void bwalk_strip(const DevArgs &a, const Lds &s, int 
     for (int py = 0; py < 2; py++)
         C.pair_even[py] = ((cpar + (uint32_t)(py ^ 1 ^ (int)(a.crop_y & 1u)) * a.dst_w) & 1u) == 0;
     const __amdgpu_buffer_rsrc_t rs = make_rsrc(a.src, src_bytes(a));
-    const __amdgpu_buffer_rsrc_t rd = make_rsrc(a.dst, a.dst_w * a.dev_h * 4u);
+    const __amdgpu_buffer_rsrc_t rd = make_rsrc(a.dst, a.dev_w * a.dev_h * 4u);
+    C.rot_lane = C.rot_cx = C.rot_cy = C.rot_lo = 0u;
+    if (ROT) {
+        // destination index c0 + cx dx + cy dy (orient, process.rs:2035-2044)
+        uint32_t c0;
+        orient_linear(a, c0, C.rot_cx, C.rot_cy);
+        C.rot_lane = c0 + C.rot_cx * (uint32_t)(x0 - (int)a.crop_x);
+        C.rot_lo = C.rot_cy == 1u ? 1u : 0u;
+    }
 #pragma unroll
     for (int i = 0; i < BPF; i++) W.pf[i] = fetch_pair<HOT>(a, rs, x0, min(row0 - 2 + i, out_row1));
     if (!FAST) {
         // the general path (lanes and rows leave each row early, a divergent branch): whole trips
         // of four rows from row0 - 2, the rows past out_row1 only shifting the window
         for (int R = row0 - 2; R < out_row1 + 1; R += BPF) {
-            bwalk_row<RP, 0, 0, FAST, HOT, TAME>(a, s, W, R, xs, lane, out_row0, out_row1, C, rs, rd);
-            bwalk_row<RP, 1, 1, FAST, HOT, TAME>(a, s, W, R + 1, xs, lane, out_row0, out_row1, C, rs, rd);
-            bwalk_row<RP, 0, 2, FAST, HOT, TAME>(a, s, W, R + 2, xs, lane, out_row0, out_row1, C, rs, rd);
-            bwalk_row<RP, 1, 3, FAST, HOT, TAME>(a, s, W, R + 3, xs, lane, out_row0, out_row1, C, rs, rd);
+            bwalk_row<RP, 0, 0, FAST, HOT, TAME, true, ROT>(a, s, W, R, xs, lane, out_row0, out_row1, C, rs, rd);
+            bwalk_row<RP, 1, 1, FAST, HOT, TAME, true, ROT>(a, s, W, R + 1, xs, lane, out_row0, out_row1, C, rs, rd);
+            bwalk_row<RP, 0, 2, FAST, HOT, TAME, true, ROT>(a, s, W, R + 2, xs, lane, out_row0, out_row1, C, rs, rd);
+            bwalk_row<RP, 1, 3, FAST, HOT, TAME, true, ROT>(a, s, W, R + 3, xs, lane, out_row0, out_row1, C, rs, rd);
         }
         return;
     }
-    bwalk_row<RP, 0, 0, FAST, HOT, TAME, false>(a, s, W, row0 - 2, xs, lane, out_row0, out_row1, C, rs, rd);
-    bwalk_row<RP, 1, 1, FAST, HOT, TAME, false>(a, s, W, row0 - 1, xs, lane, out_row0, out_row1, C, rs, rd);
-    bwalk_row<RP, 0, 2, FAST, HOT, TAME, false>(a, s, W, row0, xs, lane, out_row0, out_row1, C, rs, rd);
+    bwalk_row<RP, 0, 0, FAST, HOT, TAME, false, ROT>(a, s, W, row0 - 2, xs, lane, out_row0, out_row1, C, rs, rd);
+    bwalk_row<RP, 1, 1, FAST, HOT, TAME, false, ROT>(a, s, W, row0 - 1, xs, lane, out_row0, out_row1, C, rs, rd);
+    bwalk_row<RP, 0, 2, FAST, HOT, TAME, false, ROT>(a, s, W, row0, xs, lane, out_row0, out_row1, C, rs, rd);
     int R = row0 + 1;
     for (; R + 3 <= out_row1; R += 4) {
-        bwalk_row<RP, 1, 3, FAST, HOT, TAME>(a, s, W, R, xs, lane, out_row0, out_row1, C, rs, rd);
-        bwalk_row<RP, 0, 0, FAST, HOT, TAME>(a, s, W, R + 1, xs, lane, out_row0, out_row1, C, rs, rd);
-        bwalk_row<RP, 1, 1, FAST, HOT, TAME>(a, s, W, R + 2, xs, lane, out_row0, out_row1, C, rs, rd);
-        bwalk_row<RP, 0, 2, FAST, HOT, TAME>(a, s, W, R + 3, xs, lane, out_row0, out_row1, C, rs, rd);
+        bwalk_row<RP, 1, 3, FAST, HOT, TAME, true, ROT>(a, s, W, R, xs, lane, out_row0, out_row1, C, rs, rd);
+        bwalk_row<RP, 0, 0, FAST, HOT, TAME, true, ROT>(a, s, W, R + 1, xs, lane, out_row0, out_row1, C, rs, rd);
+        bwalk_row<RP, 1, 1, FAST, HOT, TAME, true, ROT>(a, s, W, R + 2, xs, lane, out_row0, out_row1, C, rs, rd);
+        bwalk_row<RP, 0, 2, FAST, HOT, TAME, true, ROT>(a, s, W, R + 3, xs, lane, out_row0, out_row1, C, rs, rd);
     }
-    if (R <= out_row1) bwalk_row<RP, 1, 3, FAST, HOT, TAME>(a, s, W, R, xs, lane, out_row0, out_row1, C, rs, rd);
+    if (R <= out_row1) bwalk_row<RP, 1, 3, FAST, HOT, TAME, true, ROT>(a, s, W, R, xs, lane, out_row0, out_row1, C, rs, rd);
     if (R + 1 <= out_row1)
-        bwalk_row<RP, 0, 0, FAST, HOT, TAME>(a, s, W, R + 1, xs, lane, out_row0, out_row1, C, rs, rd);
+        bwalk_row<RP, 0, 0, FAST, HOT, TAME, true, ROT>(a, s, W, R + 1, xs, lane, out_row0, out_row1, C, rs, rd);
     if (R + 2 <= out_row1)
-        bwalk_row<RP, 1, 1, FAST, HOT, TAME>(a, s, W, R + 2, xs, lane, out_row0, out_row1, C, rs, rd);
+        bwalk_row<RP, 1, 1, FAST, HOT, TAME, true, ROT>(a, s, W, R + 2, xs, lane, out_row0, out_row1, C, rs, rd);
 }
 
 struct BWalkPlan {
@@ -192,7 +235,7 @@ __host__ inline BWalkPlan make_bwalk_plan(const DevArgs &a, int rows_per_strip) 
 }
 
 // Strip `item` of plan P, developed by the calling wave.
-template <int RP, bool FAST, bool TAME>
+template <int RP, bool FAST, bool TAME, int ROT = 0>
 __device__ __forceinline__ void bwalk_item(const DevArgs &a, const BWalkPlan &P, const Lds &s, int item) {
     const int lane = threadIdx.x & 63;
     const int sx = item % P.strips, sy = item / P.strips;
@@ -204,9 +247,9 @@ __device__ __forceinline__ void bwalk_item(const DevArgs &a, const BWalkPlan &P,
     const bool hot = a.simple_cal && xs >= 0 && xs + 128 <= (int)a.width && ((uintptr_t)a.src & 3u) == 0 &&
                      (a.stride & 1u) == 0;
     if (hot)
-        bwalk_strip<RP, FAST, true, TAME>(a, s, xs, lane, row0, out_row0, out_row1, out_x0, out_x1);
+        bwalk_strip<RP, FAST, true, TAME, ROT>(a, s, xs, lane, row0, out_row0, out_row1, out_x0, out_x1);
     else
-        bwalk_strip<RP, FAST, false, TAME>(a, s, xs, lane, row0, out_row0, out_row1, out_x0, out_x1);
+        bwalk_strip<RP, FAST, false, TAME, ROT>(a, s, xs, lane, row0, out_row0, out_row1, out_x0, out_x1);
 }
 
 }  // namespace frh

@@ -237,7 +237,7 @@ __global__ __launch_bounds__(256) void k_develop_ahd(DevArgs a, uint32_t tiles_x
 // ------------------------------------------------------------------------------------------
 // Bilinear row walker launcher (the walker is in bil_walk.h).
 
-template <int RP, bool FAST, bool TAME>
+template <int RP, bool FAST, bool TAME, int ROT = 0>
 __global__ __launch_bounds__(BIL_MAX_WAVES * 64) void k_develop_bilinear_walk(DevArgs args, BWalkPlan P,
                                                                             uint32_t walk_blocks) {
     const DevArgs &a = kernel_args(args);
@@ -255,7 +255,7 @@ __global__ __launch_bounds__(BIL_MAX_WAVES * 64) void k_develop_bilinear_walk(De
     // bookkeeping in scalar registers
     const int item = (int)uniform(blk * (blockDim.x >> 6) + (threadIdx.x >> 6));
     if (item >= P.strips * P.chunks) return;
-    bwalk_item<RP, FAST, TAME>(a, P, s, item);
+    bwalk_item<RP, FAST, TAME, ROT>(a, P, s, item);
 }
 
 template <int RP>
@@ -266,6 +266,12 @@ static void launch_bil_rp(const DevArgs &a, const BWalkPlan &P, uint32_t walk_bl
     const bool fast = fast_rgba8(a), tame = a.ahd_tame != 0;
     if (fast && tame)
         hipLaunchKernelGGL((k_develop_bilinear_walk<RP, true, true>), dim3(grid), dim3(block), 0, stream,
+                           a, P, walk_blocks);
+    else if (tame && fast_rgba8_rot(a) && a.orientation <= 4)
+        hipLaunchKernelGGL((k_develop_bilinear_walk<RP, true, true, 1>), dim3(grid), dim3(block), 0, stream,
+                           a, P, walk_blocks);
+    else if (tame && fast_rgba8_rot(a))
+        hipLaunchKernelGGL((k_develop_bilinear_walk<RP, true, true, 2>), dim3(grid), dim3(block), 0, stream,
                            a, P, walk_blocks);
     else if (fast)
         hipLaunchKernelGGL((k_develop_bilinear_walk<RP, true, false>), dim3(grid), dim3(block), 0,

@@ -162,6 +162,27 @@ __device__ __forceinline__ void orient(int o, uint32_t dx, uint32_t dy, uint32_t
     }
 }
 
+// orient() as a linear map: px = X0 + Xx dx + Xy dy and py = Y0 + Yx dx + Yy dy, so the destination
+// index py * dst_w + px of developed pixel (dx, dy) is c0 + cx dx + cy dy (modulo 2^32).
+__device__ __forceinline__ void orient_linear(const DevArgs &a, uint32_t &c0, uint32_t &cx, uint32_t &cy) {
+    const int w = (int)a.dev_w, h = (int)a.dev_h;
+    int X0 = 0, Xx = 1, Xy = 0, Y0 = 0, Yx = 0, Yy = 1;
+    switch (a.orientation) {
+    case 2: X0 = w - 1; Xx = -1; break;
+    case 3: X0 = w - 1; Xx = -1; Y0 = h - 1; Yy = -1; break;
+    case 4: Y0 = h - 1; Yy = -1; break;
+    case 5: Xx = 0; Xy = 1; Yx = 1; Yy = 0; break;
+    case 6: X0 = h - 1; Xx = 0; Xy = -1; Yx = 1; Yy = 0; break;
+    case 7: X0 = h - 1; Xx = 0; Xy = -1; Y0 = w - 1; Yx = -1; Yy = 0; break;
+    case 8: Xx = 0; Xy = 1; Y0 = w - 1; Yx = -1; Yy = 0; break;
+    default: break;
+    }
+    const uint32_t dw = a.dst_w;
+    c0 = (uint32_t)Y0 * dw + (uint32_t)X0;
+    cx = (uint32_t)Yx * dw + (uint32_t)Xx;
+    cy = (uint32_t)Yy * dw + (uint32_t)Xy;
+}
+
 // Encode and store linear output pixel (r, g, b) of developed pixel (dx, dy) (any encoding and
 // orientation).
 template <bool FULL = true>

@@ -131,14 +131,16 @@ def test_ahd_rgba8_and_linear16_in_one_pass(ctx, cfa):
                      what=f"{cfa} crop {crop} {first.name}+{second.name}")
 
 
+@pytest.mark.parametrize("demosaic", ["bilinear", "ahd"])
 @pytest.mark.parametrize("orientation", range(2, 9))
-def test_ahd_rotated_rgba8_crops(ctx, orientation):
-    """Rotated RGBA8 from the AHD walker's rotated form (tame frames): odd widths and heights,
-    crops that start at odd columns and rows, all four CFA phases, against the oracle."""
+def test_rotated_rgba8_crops(ctx, demosaic, orientation):
+    """Rotated RGBA8 from the walkers' rotated forms (tame frames): odd widths and heights, crops
+    that start at odd columns and rows, and two CFA phases (each crop parity then reaches all
+    four), against the oracle."""
     for cfa, seed in [("RGGB", 21), ("GBRG", 22)]:
         mos = synth.mosaic(131, 87, cfa, seed=seed, stripes=15, edges=10)
         for crop in [None, (13, 9, 101, 71), (3, 2, 120, 80), (8, 7, 117, 79)]:
-            _compare(ctx, mos, cfa, demosaic="ahd", output=F.DevelopOutput.Rgba8, orientation=orientation,
+            _compare(ctx, mos, cfa, demosaic=demosaic, output=F.DevelopOutput.Rgba8, orientation=orientation,
                      crop=crop, what=f"{cfa} crop {crop} orientation {orientation}")
 
 
