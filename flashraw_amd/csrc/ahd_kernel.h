@@ -7,8 +7,11 @@
 
 namespace frh {
 
+// Three waves per SIMD (168 VGPRs); the transposed RGBA8 form (OUT 3) holds a row of pixels more
+// and runs at two (256 VGPRs), where it does not spill.
 template <int RP, bool FAST, bool TAME, int OUT = 0>
-__global__ __launch_bounds__(AHD_MAX_WAVES * 64) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_develop_ahd_walk(DevArgs args, WalkPlan P, uint32_t walk_blocks) {
+__global__ __launch_bounds__(AHD_MAX_WAVES * 64)
+__attribute__((amdgpu_waves_per_eu(OUT == 3 ? 2 : 3, OUT == 3 ? 2 : 3))) void k_develop_ahd_walk(DevArgs args, WalkPlan P, uint32_t walk_blocks) {
     const DevArgs &a = kernel_args(args);
     __shared__ Lds s;
     __shared__ float wt[73 * 73];

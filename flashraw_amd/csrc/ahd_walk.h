@@ -55,6 +55,7 @@ struct Walk {
     uint32_t hom[2][2];     // [slot][k] homogeneity counts H + (V << 16): R-6 in T mod 2, R-5 in the other
     float u[3][2][2][2];    // [slot][cand][k][2] interpolated channels: R-3 in T mod 3, R-4 T+2, R-5 T+1
     uint32_t pf[2];         // raw pairs fetched two rows ahead, slot = row parity (consumed in place)
+    uint32_t hold[2];       // OUT 3: the previous output row's two pixels
 };
 
 // Per-lane constants of a strip: its columns x0, x0 + 1 are inside the frame (in0, in1) and are
@@ -64,14 +65,16 @@ struct StripCols {
     bool in0, in1, st0, st1;
     bool pair_even[2];
     float inf;  // +inf (opaque_inf)
-    // rotated RGBA8 (OUT 2): destination pixel index of the lane's first pixel on the crop's first
-    // row, and its steps per developed column and row (modulo 2^32)
-    uint32_t rot_lane, rot_cx, rot_cy;
+    // rotated RGBA8 (OUT 2, 3): destination pixel index of the lane's first pixel on the crop's
+    // first row, and its steps per developed column and row (modulo 2^32); OUT 3: rot_lo is 1 when
+    // the previous row's pixel of a column has the lower destination index (cy = +1)
+    uint32_t rot_lane, rot_cx, rot_cy, rot_lo;
 };
 
 // One walker row at trip position T: load mosaic row R (parity T & 1) and advance every stage.
 // OUT (with FAST and TAME): 0 RGBA8 unrotated; 1 RGBA8 to `rd` and linear RGB16 to `rl`; 2 RGBA8
-// under EXIF orientation 2..8.
+// under EXIF orientation 2..4 (rows stay rows); 3 RGBA8 under orientation 5..8 (rows become
+// columns, pixels of consecutive rows stored in pairs).
 template <int RP, int T, bool FAST, bool HOT, bool TAME, int OUT = 0>
 __device__ __forceinline__ void walk_row(const DevArgs &a, const Lds &s, const float *wt, Walk &W, int R, int xs,
                                          int lane, int out_row0, int out_row1, const StripCols &C,
@@ -317,6 +320,26 @@ __device__ __forceinline__ void walk_row(const DevArgs &a, const Lds &s, const f
             const uint32_t i0 = C.rot_lane + C.rot_cy * (uint32_t)(ro - (int)a.crop_y), i1 = i0 + C.rot_cx;
             __builtin_amdgcn_raw_buffer_store_b32(o[0], rd, s0 ? (int)(i0 * 4u) : (int)PAST, 0, 0);
             __builtin_amdgcn_raw_buffer_store_b32(o[1], rd, s1 ? (int)(i1 * 4u) : (int)PAST, 0, 0);
+        } else if (OUT == 3) {
+            // A row whose pixel closes an aligned pair with the previous row's (lower index even)
+            // stores both as 8 bytes when the previous row is the strip's; a row that opens one leaves
+            // its pixel to the next row unless that row is past the strip's output; any other owned
+            // pixel goes out alone.
+            const uint32_t i0 = C.rot_lane + C.rot_cy * (uint32_t)(ro - (int)a.crop_y), i1 = i0 + C.rot_cx;
+            const bool rp = ro - 1 >= out_row0, rn = ro + 1 < out_row1, sw = C.rot_lo != 0u;
+#pragma unroll
+            for (int k = 0; k < 2; k++) {
+                const uint32_t ic = k ? i1 : i0, lo = ic - C.rot_lo;
+                const bool own = k ? s1 : s0, closer = !(lo & 1u);
+                u32x2 q;
+                q.x = sw ? W.hold[k] : o[k];
+                q.y = sw ? o[k] : W.hold[k];
+                __builtin_amdgcn_raw_buffer_store_b64(q, rd, (own && closer && rp) ? (int)(lo * 4u) : (int)PAST, 0,
+                                                      0);
+                __builtin_amdgcn_raw_buffer_store_b32(o[k], rd,
+                                                      (own && (closer ? !rp : !rn)) ? (int)(ic * 4u) : (int)PAST, 0, 0);
+                W.hold[k] = o[k];
+            }
         } else {
             u32x2 v;
             v.x = o[0];
@@ -398,12 +421,14 @@ __device__ __forceinline__ void walk_strip(const DevArgs &a, const Lds &s, const
     void *const d16 = (OUT == 1 && a.out_mode == OUT_RGBA8) ? a.dst2 : a.dst;
     const __amdgpu_buffer_rsrc_t rd = make_rsrc(d8, a.dev_w * a.dev_h * 4u);
     const __amdgpu_buffer_rsrc_t rl = make_rsrc(OUT == 1 ? d16 : nullptr, OUT == 1 ? a.dev_w * a.dev_h * 6u : 0u);
-    C.rot_lane = C.rot_cx = C.rot_cy = 0u;
-    if (OUT == 2) {
+    C.rot_lane = C.rot_cx = C.rot_cy = C.rot_lo = 0u;
+    W.hold[0] = W.hold[1] = 0u;
+    if (OUT == 2 || OUT == 3) {
         // destination index c0 + cx dx + cy dy (orient, process.rs:2035-2044)
         uint32_t c0, cx, cy;
         orient_linear(a, c0, cx, cy);
         C.rot_lane = c0 + cx * (uint32_t)(x0 - (int)a.crop_x);
+        C.rot_lo = cy == 1u ? 1u : 0u;
         C.rot_cx = cx;
         C.rot_cy = cy;
     }

@@ -15,8 +15,11 @@ static void launch_ahd_rp(const DevArgs &a, const WalkPlan &P, uint32_t walk_blo
     const bool fast = fast_rgba8(a), tame = a.ahd_tame != 0;
     if (tame && fast_rgba8_lin16(a))
         launch_ahd_dual(RP, a, P, walk_blocks, grid, block, stream);
-    else if (tame && fast_rgba8_rot(a))
+    else if (tame && fast_rgba8_rot(a) && a.orientation <= 4)
         hipLaunchKernelGGL((k_develop_ahd_walk<RP, true, true, 2>), dim3(grid), dim3(block), 0, stream, a, P,
+                           walk_blocks);
+    else if (tame && fast_rgba8_rot(a))
+        hipLaunchKernelGGL((k_develop_ahd_walk<RP, true, true, 3>), dim3(grid), dim3(block), 0, stream, a, P,
                            walk_blocks);
     else if (fast && tame)
         hipLaunchKernelGGL((k_develop_ahd_walk<RP, true, true>), dim3(grid), dim3(block), 0, stream, a, P,
