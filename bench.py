@@ -35,6 +35,8 @@ def _parse():
     p.add_argument("--steps", type=int, default=200)
     p.add_argument("--warmup", type=int, default=20)
     p.add_argument("--demosaic", default="ahd", choices=["ahd", "bilinear", "edge"])
+    p.add_argument("--orientation", type=int, default=1, choices=range(1, 9),
+                   help="develop mode: EXIF orientation of the RGBA8 output")
     p.add_argument("--beta", type=float, default=0.0,
                    help="edge demosaic: weight of the gated smoothing (0 for clean frames)")
     p.add_argument("--mode", default="develop", choices=["develop", "large", "denoise", "nr", "batch", "serve", "copy", "unpack", "link"])
@@ -748,9 +750,9 @@ def main():
     else:
         edge = None
     spec = F.DevelopSpec(F.Rect.full(W, H), wb=synth.WB, cam_to_rgb=synth.MATRIX, output=F.DevelopOutput.Rgba8,
-                         edge=edge)
+                         edge=edge, orientation=F.Orientation(args.orientation))
     d_in = torch.from_numpy(mos.view(np.int16)).to("cuda")
-    dst = torch.empty((H, W, 4), dtype=torch.uint8, device="cuda")
+    dst = torch.empty((W, H, 4) if args.orientation >= 5 else (H, W, 4), dtype=torch.uint8, device="cuda")
     frame = F.GpuFrame.from_u16(d_in, W, H)
     plan = ctx.prepare(frame, info, spec)
     stream = torch.cuda.Stream()  # the launch stream; the HIP events below are recorded on it
@@ -809,6 +811,7 @@ def main():
             t = time.perf_counter()
             want = O.develop(mos, "RGGB", black=(1, 1, 1, [512.0]), white=(16383,), wb=synth.WB,
                              matrix=synth.MATRIX, demosaic=args.demosaic, encoding="rgba8", threads=threads,
+                             orientation=args.orientation,
                              edge=None if edge is None or edge.ahd else
                              dict(g0=tables[0], dir=tables[1], cx=edge.cx, cy=edge.cy, beta=args.beta))
             if i:
@@ -817,7 +820,7 @@ def main():
         max_diff = int(np.abs(got.astype(np.int32) - want.astype(np.int32)).max())
         t = time.perf_counter()
         O.develop(mos, "RGGB", black=(1, 1, 1, [512.0]), white=(16383,), wb=synth.WB, matrix=synth.MATRIX,
-                  demosaic=args.demosaic, encoding="rgba8", threads=1,
+                  demosaic=args.demosaic, encoding="rgba8", threads=1, orientation=args.orientation,
                   edge=None if edge is None or edge.ahd else
                   dict(g0=tables[0], dir=tables[1], cx=edge.cx, cy=edge.cy, beta=args.beta))
         t_one = time.perf_counter() - t
@@ -825,7 +828,8 @@ def main():
                "sample": f"the same 6048x4024 frame, median of 5 runs after 1 warm-up, "
                          f"{args.demosaic} RGBA8, C restatement of the reference CPU path",
                "one_thread": {"value": round(mp / t_one, 2), "cores": 1, "sample": "the same frame, one run"}}
-    tag = f"develop_{args.demosaic}_6048x4024_rgba8" + (f"_beta{args.beta:g}" if args.beta else "")
+    tag = (f"develop_{args.demosaic}_6048x4024_rgba8" + (f"_beta{args.beta:g}" if args.beta else "")
+           + (f"_orientation{args.orientation}" if args.orientation != 1 else ""))
     valu = _valu(tag, k_avg)
     line = {
         "metric": "MP/s RAW->sRGB develop (24 MP AHD, % HBM roofline)",
@@ -842,7 +846,9 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"6048x4024 RGGB 14-bit u16 -> RGBA8, {args.demosaic} demosaic"
                                + (f" (trained Bayer tables, beta {args.beta:g})" if args.demosaic == "edge" else "")
+                               + (f", orientation {args.orientation}" if args.orientation != 1 else "")
                                + ", resident in HBM",
+                   "orientation": args.orientation,
                    "frame_mp": round(mp, 3), "demosaic": args.demosaic, "output": "RGBA8",
                    "max_abs_diff_vs_oracle_lsb": max_diff, "strip_rows": rows,
                    "waves_per_block": plan.tile_shape[1], "cols_per_lane": plan.tile_shape[2],
